@@ -1,0 +1,318 @@
+/*
+ * rvx.h - C ABI of the MI355X (gfx950) receive chain.
+ *
+ * This is the drop-in boundary for the GPU half of Revenant's receive chain:
+ * what core/gpu (Vulkan context, buffers, pipelines, run_kernel) plus the
+ * core/dsp operator contract (POD param structs + twin semantics) give the
+ * engine today. Every launch takes the same POD parameter block as the
+ * reference kernel's push constants, byte for byte, and a small grid struct
+ * for what the reference makes specialization constants.
+ *
+ * Conventions (after core/plugin/engine_plugin_abi.h:22-60, 129-140):
+ *   - int32_t return: RVX_OK (0) or a negative RVX_E* code; the message is in
+ *     rvx_last_error(), thread-local, valid until the next call on the thread.
+ *   - Device pointers are plain pointers into device memory (from
+ *     rvx_buf_alloc, hipMalloc or a torch CUDA tensor); every launch also takes
+ *     the element count of each span so the library refuses what would be an
+ *     out-of-bounds access, exactly where the reference twin refuses.
+ *   - Launches are asynchronous on the context's stream. Host pointers are
+ *     read during the call only.
+ *   - Nothing throws across this line.
+ *
+ * Complex samples are interleaved float pairs (Complex32, core/dsp/types.h:45).
+ */
+#ifndef RVX_H
+#define RVX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RVX_OK 0
+#define RVX_EINVAL (-1)   /* parameters refused (the twin's validate() rules) */
+#define RVX_EDEVICE (-2)  /* a HIP call failed */
+#define RVX_ENOMEM (-3)   /* allocation failed */
+
+#define RVX_ABI_VERSION 1
+
+/* ------------------------------------------------------------------------ */
+/* Parameter blocks                                                          */
+/* ------------------------------------------------------------------------ */
+
+/* core/dsp/pfb.h:32-74 GridParams */
+typedef struct rvx_grid {
+    uint32_t channels;        /* M, power of two */
+    uint32_t taps_per_branch; /* L + 1 */
+    uint32_t decimation;      /* D, divides M */
+} rvx_grid;
+
+/* core/dsp/convert.h:116-141 ConvertParams */
+typedef struct rvx_convert_params {
+    uint32_t ring_mask;
+    uint32_t dst_offset;
+    uint32_t src_offset;
+    uint32_t count;
+} rvx_convert_params;
+
+/* core/dsp/pfb_branch_reference.h:337-360 PfbBranchParams */
+typedef struct rvx_pfb_branch_params {
+    uint32_t ring_mask;
+    uint32_t base_offset;
+    uint32_t block_count;
+} rvx_pfb_branch_params;
+
+/* core/dsp/pfb_fft_reference.h:42-82 PfbFftParams */
+typedef struct rvx_pfb_fft_params {
+    uint32_t channels;
+    uint32_t decimation;
+    uint32_t stages;
+    uint32_t block_base;
+    uint32_t block_count;
+    uint32_t out_ring_blocks;
+    uint32_t out_ring_mask;
+} rvx_pfb_fft_params;
+
+/* core/dsp/spectrum_reference.h:55-79 SpectrumParams */
+typedef struct rvx_spectrum_params {
+    uint32_t channels;
+    uint32_t transform;
+    uint32_t stages;
+    uint32_t chan_blocks;
+    uint32_t chan_mask;
+    uint32_t in_offset;
+} rvx_spectrum_params;
+
+/* core/dsp/spectrum_levels_reference.h SpectrumLevelsParams */
+typedef struct rvx_spectrum_levels_params {
+    uint32_t bins;
+    uint32_t low_permille;
+    uint32_t high_permille;
+} rvx_spectrum_levels_params;
+
+/* The exponentially averaged power spectrum. No reference counterpart: the
+ * reference averages on the host, in double, from dB frames
+ * (core/detect/detector.cpp:621-627). Operation order, per channel k:
+ *   partial: for each frame f, s = 0; for each of its frame_blocks channel
+ *            samples v in ascending order: x = v.re*v.re + v.im*v.im;
+ *            s = s + (x - s) * alpha
+ *   combine: P = P * frame_decay + s_f for ascending f; row f = P.
+ * frame_decay is (1 - alpha)^frame_blocks, rounded once to float. */
+typedef struct rvx_ema_params {
+    uint32_t channels;
+    uint32_t chan_blocks;
+    uint32_t chan_mask;
+    uint32_t in_offset;
+    uint32_t frame_blocks;
+    uint32_t frames;
+    float alpha;
+    float frame_decay;
+} rvx_ema_params;
+
+/* core/dsp/vrx_reference.h:178-195 VrxFineConfig */
+typedef struct rvx_vrx_fine_config {
+    uint32_t taps;
+    uint32_t phases;
+    uint32_t nco_log2;
+} rvx_vrx_fine_config;
+
+/* core/dsp/vrx_reference.h:232-271 VrxFineParams (fifteen words) */
+typedef struct rvx_vrx_fine_params {
+    uint32_t chan_base;
+    uint32_t chan_mask;
+    uint32_t in_offset;
+    uint32_t out_offset;
+    uint32_t out_mask;
+    uint32_t count;
+    uint32_t step_whole;
+    uint32_t step_rem;
+    uint32_t out_rate;
+    uint32_t frac0;
+    uint32_t nco_phase_high;
+    uint32_t nco_phase_low;
+    uint32_t nco_delta_high;
+    uint32_t nco_delta_low;
+    float inv_out_rate;
+} rvx_vrx_fine_params;
+
+/* core/dsp/vrx_reference.h:506-544 VrxDemodConfig */
+typedef struct rvx_vrx_demod_config {
+    uint32_t mode;
+    uint32_t decimation;
+    uint32_t audio_taps;
+    uint32_t dc_taps;
+    uint32_t channels;
+    uint32_t pilot_taps;
+} rvx_vrx_demod_config;
+
+/* core/dsp/vrx_reference.h:569-584 VrxDemodParams */
+typedef struct rvx_vrx_demod_params {
+    uint32_t in_mask;
+    uint32_t in_offset;
+    uint32_t count;
+    float gain;
+} rvx_vrx_demod_params;
+
+/* Demodulator mode ids, core/dsp/vrx_reference.h:300-331 */
+#define RVX_DEMOD_RAW 0
+#define RVX_DEMOD_AM 1
+#define RVX_DEMOD_NFM 2
+#define RVX_DEMOD_WFM 3
+#define RVX_DEMOD_USB 4
+#define RVX_DEMOD_LSB 5
+#define RVX_DEMOD_DSB 6
+#define RVX_DEMOD_CW 7
+
+/* Native sample formats, core/dsp/convert.h */
+#define RVX_FORMAT_CU8 0
+#define RVX_FORMAT_CS8 1
+#define RVX_FORMAT_CS16 2
+#define RVX_FORMAT_CS24 3
+
+/* ------------------------------------------------------------------------ */
+/* Context, stream, buffers (core/gpu/context.h:64-143, buffer.h:33-73)      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct rvx_ctx rvx_ctx;
+
+/* core/gpu/context.cpp:274 Context::create. Creates a context on HIP device
+ * `device` with its own stream (default priority). */
+int32_t rvx_ctx_create(int32_t device, rvx_ctx** out);
+void rvx_ctx_destroy(rvx_ctx* ctx);
+/* Replace the stream launches go to (e.g. torch's current stream). NULL
+ * restores the context's own stream. */
+int32_t rvx_ctx_set_stream(rvx_ctx* ctx, void* hip_stream);
+void* rvx_ctx_stream(rvx_ctx* ctx);
+/* Wait for everything launched on the context's stream. */
+int32_t rvx_stream_sync(rvx_ctx* ctx);
+const char* rvx_last_error(void);
+int32_t rvx_abi_version(void);
+
+#define RVX_MEM_DEVICE 0   /* MemoryKind::DeviceLocal */
+#define RVX_MEM_UPLOAD 1   /* MemoryKind::Upload: pinned host, device-readable */
+#define RVX_MEM_READBACK 2 /* MemoryKind::Readback: pinned host */
+
+/* core/gpu/buffer.cpp:24 Buffer::create */
+int32_t rvx_buf_alloc(rvx_ctx* ctx, uint64_t bytes, int32_t kind, void** out);
+int32_t rvx_buf_free(rvx_ctx* ctx, void* ptr);
+/* Copies on the context's stream, synchronous with respect to the host. */
+int32_t rvx_buf_write(rvx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+int32_t rvx_buf_read(rvx_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* core/gpu/kernel.h:126-137 CommandRunner::clear: zero fill by a kernel (not
+ * a memset node, so it is safe inside a captured graph). */
+int32_t rvx_buf_clear(rvx_ctx* ctx, void* dst, uint64_t bytes);
+
+/* Timing on the context's stream (hipEvents): begin, end, then elapsed ms. */
+int32_t rvx_timer_begin(rvx_ctx* ctx);
+int32_t rvx_timer_end(rvx_ctx* ctx, float* elapsed_ms);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side design (core/dsp/pfb_design.cpp, spectrum_reference.cpp,        */
+/* vrx_reference.cpp). FP64, rounded once to float.                          */
+/* ------------------------------------------------------------------------ */
+
+/* core/dsp/pfb_design.cpp:381-510 design_prototype. attenuation_db selects
+ * Kaiser beta by kaiser_beta (:115); if beta_override >= 0 it is used as the
+ * Kaiser shape parameter directly. out receives channels*taps_per_branch
+ * floats. stopband_db / group_delay may be NULL. */
+int32_t rvx_design_prototype(const rvx_grid* grid, double attenuation_db, double beta_override,
+                             float* out, uint64_t out_len, double* stopband_db,
+                             uint64_t* group_delay);
+/* core/dsp/pfb_design.cpp:515-591 build_twiddles: W_M^j, M cf32. */
+int32_t rvx_build_twiddles(uint32_t channels, float* out, uint64_t out_len);
+/* core/dsp/spectrum_reference.cpp:195-285 build_spectrum_window: N window
+ * taps then N/2 channel-shape correction gains. */
+int32_t rvx_build_spectrum_window(uint32_t size, uint32_t taps_per_branch,
+                                  double attenuation_db, float* out, uint64_t out_len);
+/* core/dsp/vrx_reference.cpp:364 build_nco_table: 2^log2_size cf32. */
+int32_t rvx_build_nco_table(uint32_t log2_size, float* out, uint64_t out_len);
+/* core/dsp/vrx_reference.cpp:308 turn_fixed64 and :333 nco_delta. */
+uint64_t rvx_turn_fixed64(int64_t numerator, int64_t denominator);
+int32_t rvx_nco_delta(int64_t hz_numerator, int64_t hz_denominator, int64_t rate,
+                      uint64_t* out);
+
+/* ------------------------------------------------------------------------ */
+/* Operators. Each replaces one reference compute shader + its dispatch.     */
+/* ------------------------------------------------------------------------ */
+
+/* core/shaders/convert_{cu8,cs8,cs16,cs24}_cf32.comp; twin
+ * core/dsp/convert.cpp:153,199,235,271. format = RVX_FORMAT_*. packed is the
+ * staging buffer as the device sees it, packed_bytes its size. */
+int32_t rvx_convert(rvx_ctx* ctx, int32_t format, const void* packed, uint64_t packed_bytes,
+                    const rvx_convert_params* params, void* ring, uint64_t ring_samples);
+
+/* core/shaders/cmul.comp; twin core/dsp/complex_ops.cpp:33. The device
+ * denormal probe and plumbing check. */
+int32_t rvx_cmul(rvx_ctx* ctx, const void* lhs, const void* rhs, void* out, uint64_t count);
+
+/* core/shaders/pfb_branch.comp; twin core/dsp/pfb_branch_reference.cpp:22.
+ * taps: M*(L+1) floats. iq_ring: ring_samples cf32. out: block_count*M cf32,
+ * block-major, slot (M - r) & (M - 1). */
+int32_t rvx_pfb_branch(rvx_ctx* ctx, const rvx_grid* grid, const float* taps, uint64_t n_taps,
+                       const void* iq_ring, uint64_t ring_samples,
+                       const rvx_pfb_branch_params* params, void* out, uint64_t out_samples);
+
+/* core/shaders/pfb_fft.comp; twin core/dsp/pfb_fft_reference.cpp:143.
+ * branch: block_count*M cf32; twiddles: M cf32; channel ring: M *
+ * out_ring_blocks cf32, channel-major. */
+int32_t rvx_pfb_fft(rvx_ctx* ctx, const rvx_pfb_fft_params* params, const void* branch,
+                    uint64_t branch_samples, const void* twiddles, uint64_t n_twiddles,
+                    void* channel_ring, uint64_t ring_samples);
+
+/* Twin core/dsp/pfb_fft_reference.cpp:234 reference_fft_radix2: batch
+ * transforms of `size` points, no correction. */
+int32_t rvx_fft_radix2(rvx_ctx* ctx, const void* twiddles, uint64_t n_twiddles, const void* in,
+                       void* out, uint32_t size, uint32_t batch);
+
+/* core/shaders/spectrum.comp; twin core/dsp/spectrum_reference.cpp:286.
+ * window: N + N/2 floats from rvx_build_spectrum_window. frame: M*N/2 dB. */
+int32_t rvx_spectrum(rvx_ctx* ctx, const rvx_spectrum_params* params, const void* channel_ring,
+                     uint64_t ring_samples, const void* twiddles, uint64_t n_twiddles,
+                     const float* window, uint64_t n_window, float* frame, uint64_t n_frame);
+
+/* core/shaders/spectrum_levels.comp; twin
+ * core/dsp/spectrum_levels_reference.cpp:38. levels: 2 floats. */
+int32_t rvx_spectrum_levels(rvx_ctx* ctx, const rvx_spectrum_levels_params* params,
+                            const float* frame, uint64_t n_frame, float* levels);
+
+/* The averaged power spectrum (see rvx_ema_params). partial: frames*channels
+ * floats. state: channels floats, read and updated. rows: frames*channels,
+ * or NULL. */
+int32_t rvx_ema_partial(rvx_ctx* ctx, const rvx_ema_params* params, const void* channel_ring,
+                        uint64_t ring_samples, float* partial, uint64_t n_partial);
+int32_t rvx_ema_combine(rvx_ctx* ctx, const rvx_ema_params* params, const float* partial,
+                        uint64_t n_partial, float* state, float* rows, uint64_t n_rows);
+
+/* The fused plain chain: convert -> pfb_branch -> pfb_fft (-> ema_partial) in
+ * one launch, bit-identical to running the four operators in sequence
+ * (core/engine/graph.cpp:5120-5196 records them as separate dispatches).
+ * input: native codes from absolute sample `input_origin` (input_samples of
+ * them); block m (absolute, first_block <= m < first_block + block_count) is
+ * taken at base m*D and needs samples m*D - M*(L+1) + 1 .. m*D. Writes the
+ * channel ring (channel-major, slot m & (out_ring_blocks - 1)) and, when
+ * frame_blocks != 0, the EMA partials of block_count / frame_blocks frames. */
+typedef struct rvx_chain_desc {
+    int32_t format;           /* RVX_FORMAT_CU8 or RVX_FORMAT_CS16 */
+    uint32_t channels;
+    uint32_t taps_per_branch;
+    uint32_t decimation;
+    uint64_t input_origin;
+    uint64_t input_samples;
+    uint64_t first_block;
+    uint32_t block_count;
+    uint32_t out_ring_blocks;
+    uint32_t frame_blocks;
+    float alpha;
+} rvx_chain_desc;
+
+int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const void* input,
+                  const float* taps, uint64_t n_taps, const void* twiddles, uint64_t n_twiddles,
+                  void* channel_ring, uint64_t ring_samples, float* partial, uint64_t n_partial);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RVX_H */

@@ -1,0 +1,177 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, the smoke check and the
+benchmark's CPU-baseline leg. It is the checker, never the thing measured or
+shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
+
+u32, i32, u64, f32, vp = C.c_uint32, C.c_int32, C.c_uint64, C.c_float, C.c_void_p
+
+
+class OChain(C.Structure):
+    _fields_ = [("format", i32), ("channels", u32), ("taps_per_branch", u32),
+                ("decimation", u32), ("first_block", u64), ("block_count", u32),
+                ("out_ring_blocks", u32), ("frame_blocks", u32), ("alpha", f32),
+                ("frame_decay", f32)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+        _lib = C.CDLL(str(LIB_PATH))
+        for name in ("oracle_convert", "oracle_cmul", "oracle_pfb_branch", "oracle_pfb_fft",
+                     "oracle_fft_radix2", "oracle_spectrum", "oracle_spectrum_levels",
+                     "oracle_det", "oracle_ema_partial", "oracle_ema_combine", "oracle_chain",
+                     "fp64_channelize", "fp64_dft"):
+            getattr(_lib, name).restype = i32
+    return _lib
+
+
+def _a(x: np.ndarray):
+    return x.ctypes.data_as(vp)
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise ValueError(f"oracle refused {what} (rc={rc})")
+
+
+def f32c(x: np.ndarray) -> np.ndarray:
+    """Complex samples as contiguous float32 pairs."""
+    if np.iscomplexobj(x):
+        x = np.ascontiguousarray(x.astype(np.complex64)).view(np.float32)
+    return np.ascontiguousarray(x, dtype=np.float32)
+
+
+def convert(fmt: int, packed: np.ndarray, params, ring: np.ndarray) -> None:
+    """ring (float32 pairs) updated in place. params: ctypes ConvertParams."""
+    _check(lib().oracle_convert(i32(fmt), _a(packed), u64(packed.nbytes), C.byref(params),
+                                _a(ring), u64(ring.size // 2)), "convert")
+
+
+def cmul(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    a, b = f32c(a), f32c(b)
+    out = np.empty_like(a)
+    _check(lib().oracle_cmul(_a(a), _a(b), _a(out), u64(a.size // 2)), "cmul")
+    return out
+
+
+def pfb_branch(grid, taps: np.ndarray, ring: np.ndarray, params) -> np.ndarray:
+    taps = np.ascontiguousarray(taps, dtype=np.float32)
+    ring = f32c(ring)
+    out = np.zeros(2 * params.block_count * grid.channels, dtype=np.float32)
+    _check(lib().oracle_pfb_branch(C.byref(grid), _a(taps), u64(taps.size), _a(ring),
+                                   u64(ring.size // 2), C.byref(params), _a(out),
+                                   u64(out.size // 2)), "pfb_branch")
+    return out
+
+
+def pfb_fft(params, branch: np.ndarray, twiddles: np.ndarray, ring: np.ndarray) -> None:
+    branch, tw = f32c(branch), f32c(twiddles)
+    _check(lib().oracle_pfb_fft(C.byref(params), _a(branch), u64(branch.size // 2), _a(tw),
+                                u64(tw.size // 2), _a(ring), u64(ring.size // 2)), "pfb_fft")
+
+
+def fft_radix2(twiddles: np.ndarray, x: np.ndarray, size: int, batch: int) -> np.ndarray:
+    tw, x = f32c(twiddles), f32c(x)
+    out = np.zeros(2 * size * batch, dtype=np.float32)
+    _check(lib().oracle_fft_radix2(_a(tw), u64(tw.size // 2), _a(x), _a(out), u32(size),
+                                   u32(batch)), "fft_radix2")
+    return out
+
+
+def spectrum(params, ring: np.ndarray, twiddles: np.ndarray, window: np.ndarray) -> np.ndarray:
+    ring, tw = f32c(ring), f32c(twiddles)
+    window = np.ascontiguousarray(window, dtype=np.float32)
+    frame = np.zeros(params.channels * (params.transform // 2), dtype=np.float32)
+    _check(lib().oracle_spectrum(C.byref(params), _a(ring), u64(ring.size // 2), _a(tw),
+                                 u64(tw.size // 2), _a(window), u64(window.size), _a(frame),
+                                 u64(frame.size)), "spectrum")
+    return frame
+
+
+def spectrum_levels(params, frame: np.ndarray) -> np.ndarray:
+    frame = np.ascontiguousarray(frame, dtype=np.float32)
+    out = np.zeros(2, dtype=np.float32)
+    _check(lib().oracle_spectrum_levels(C.byref(params), _a(frame), u64(frame.size), _a(out)),
+           "spectrum_levels")
+    return out
+
+
+DET = {"log2": 0, "recip": 1, "rsqrt": 2, "sqrt": 3, "atan2": 4, "atan_unit": 5}
+
+
+def det(which: str, a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = a if b is None else np.ascontiguousarray(b, dtype=np.float32)
+    out = np.zeros_like(a)
+    _check(lib().oracle_det(i32(DET[which]), _a(a), _a(b), _a(out), u64(a.size)), which)
+    return out
+
+
+def ema_partial(params, ring: np.ndarray) -> np.ndarray:
+    ring = f32c(ring)
+    out = np.zeros(params.frames * params.channels, dtype=np.float32)
+    _check(lib().oracle_ema_partial(C.byref(params), _a(ring), u64(ring.size // 2), _a(out),
+                                    u64(out.size)), "ema_partial")
+    return out
+
+
+def ema_combine(params, partial: np.ndarray, state: np.ndarray) -> np.ndarray:
+    partial = np.ascontiguousarray(partial, dtype=np.float32)
+    rows = np.zeros(params.frames * params.channels, dtype=np.float32)
+    _check(lib().oracle_ema_combine(C.byref(params), _a(partial), u64(partial.size),
+                                    _a(state), _a(rows), u64(rows.size)), "ema_combine")
+    return rows
+
+
+def chain(desc: OChain, packed: np.ndarray, taps: np.ndarray, twiddles: np.ndarray,
+          threads: int = 1) -> tuple[np.ndarray, np.ndarray | None]:
+    """The plain chain over absolute blocks first_block .. + block_count.
+    Returns (channel ring float32 pairs M x out_ring_blocks, EMA partials)."""
+    taps = np.ascontiguousarray(taps, dtype=np.float32)
+    tw = f32c(twiddles)
+    M = desc.channels
+    ring = np.zeros(2 * M * desc.out_ring_blocks, dtype=np.float32)
+    partial = None
+    np_ = 0
+    if desc.frame_blocks:
+        partial = np.zeros((desc.block_count // desc.frame_blocks) * M, dtype=np.float32)
+        np_ = partial.size
+    _check(lib().oracle_chain(C.byref(desc), _a(packed), u64(packed.nbytes), _a(taps),
+                              u64(taps.size), _a(tw), u64(tw.size // 2), _a(ring),
+                              u64(ring.size // 2),
+                              _a(partial) if partial is not None else None, u64(np_),
+                              i32(threads)), "chain")
+    return ring, partial
+
+
+def fp64_channelize(x: np.ndarray, h: np.ndarray, M: int, D: int, m0: int,
+                    blocks: int) -> np.ndarray:
+    """Direct FP64 channelizer definition; returns complex128 [blocks, M]."""
+    xd = np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).view(np.float64)
+    hd = np.ascontiguousarray(h, dtype=np.float64)
+    out = np.zeros(2 * blocks * M, dtype=np.float64)
+    _check(lib().fp64_channelize(_a(xd), u64(xd.size // 2), _a(hd), u32(hd.size), u32(M),
+                                 u32(D), u64(m0), u32(blocks), _a(out)), "fp64_channelize")
+    return out.view(np.complex128).reshape(blocks, M)
+
+
+def fp64_dft(x: np.ndarray) -> np.ndarray:
+    xd = np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).view(np.float64)
+    out = np.zeros_like(xd)
+    _check(lib().fp64_dft(_a(xd), u32(xd.size // 2), _a(out)), "fp64_dft")
+    return out.view(np.complex128)

@@ -1,0 +1,682 @@
+// CPU oracle for the receive chain: a restatement of Revenant's CPU twins.
+//
+// TEST INFRASTRUCTURE. Only the test suite, the smoke check and the CPU
+// baseline leg of the benchmark load this library, and only to check or time
+// the GPU path. Nothing in the product calls it.
+//
+// Every function here fixes the same floating-point operation order as the
+// reference twin it cites: no contraction (-ffp-contract=off),
+// denormals flushed on input and output (MXCSR FTZ|DAZ, 0x8040, exactly the
+// policy of core/dsp/denormal_mode.h and lw-radio-dsp primitives.cpp:319-348),
+// and 32-bit index arithmetic that wraps where the kernels' does.
+//
+// Pinning (what checks each function against something other than itself) is
+// listed beside each function. The tests named there live in tests/.
+
+// Contraction is switched off by the build (-ffp-contract=off in oracle/Makefile;
+// g++ ignores the standard FP_CONTRACT pragma).
+
+#include <xmmintrin.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+namespace {
+
+struct FlushGuard {
+    unsigned saved;
+    FlushGuard() : saved(_mm_getcsr()) { _mm_setcsr(saved | 0x8040U); }
+    ~FlushGuard() { _mm_setcsr(saved); }
+};
+
+inline float bits_to_float(uint32_t b) {
+    float f;
+    std::memcpy(&f, &b, 4);
+    return f;
+}
+inline uint32_t float_to_bits(float f) {
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return b;
+}
+inline bool pow2(uint64_t v) { return v != 0 && (v & (v - 1)) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------------------
+// Parameter blocks, byte-identical to the reference's POD push-constant
+// structs (sizes static_asserted there and here).
+// ---------------------------------------------------------------------------
+
+struct OConvertParams {  // core/dsp/convert.h:116-141
+    uint32_t ring_mask, dst_offset, src_offset, count;
+};
+struct OGrid {  // core/dsp/pfb.h:32-74
+    uint32_t channels, taps_per_branch, decimation;
+};
+struct OBranchParams {  // core/dsp/pfb_branch_reference.h:337-360
+    uint32_t ring_mask, base_offset, block_count;
+};
+struct OFftParams {  // core/dsp/pfb_fft_reference.h:42-82
+    uint32_t channels, decimation, stages, block_base, block_count, out_ring_blocks,
+        out_ring_mask;
+};
+struct OSpectrumParams {  // core/dsp/spectrum_reference.h:55-79
+    uint32_t channels, transform, stages, chan_blocks, chan_mask, in_offset;
+};
+struct OLevelsParams {  // core/dsp/spectrum_levels_reference.h
+    uint32_t bins, low_permille, high_permille;
+};
+// The exponentially averaged power spectrum. The reference has no GPU
+// operator for it (its average is the host detector's, detector.cpp:621-627),
+// so this block and its operation order are this project's definition.
+struct OEmaParams {
+    uint32_t channels, chan_blocks, chan_mask, in_offset, frame_blocks, frames;
+    float alpha, frame_decay;
+};
+struct OFineConfig {  // core/dsp/vrx_reference.h:178-195
+    uint32_t taps, phases, nco_log2;
+};
+struct OFineParams {  // core/dsp/vrx_reference.h:232-271
+    uint32_t chan_base, chan_mask, in_offset, out_offset, out_mask, count, step_whole, step_rem,
+        out_rate, frac0, nco_phase_high, nco_phase_low, nco_delta_high, nco_delta_low;
+    float inv_out_rate;
+};
+struct ODemodConfig {  // core/dsp/vrx_reference.h:506-544
+    uint32_t mode, decimation, audio_taps, dc_taps, channels, pilot_taps;
+};
+struct ODemodParams {  // core/dsp/vrx_reference.h:569-584
+    uint32_t in_mask, in_offset, count;
+    float gain;
+};
+
+static_assert(sizeof(OConvertParams) == 16);
+static_assert(sizeof(OBranchParams) == 12);
+static_assert(sizeof(OFineParams) == 60);
+static_assert(sizeof(ODemodParams) == 16);
+
+// ---------------------------------------------------------------------------
+// Format conversion. core/dsp/convert.cpp:126-290 (cu8_to_float .. cs24) and
+// convert_*_cf32.comp. Words are assembled little-endian and fields extracted
+// as the kernels do.
+// Pinned by: every cu8/cs8 code and the cs16/cs24 ranges against the exact
+// closed forms (tests/test_oracle_convert.py), the constant words
+// 0x42FF0000 / 0x3C008081 and the +-1 endpoints (convert.h:82-105).
+// ---------------------------------------------------------------------------
+
+static uint32_t load_word(const uint8_t* p, uint64_t word) {
+    const uint8_t* b = p + word * 4;
+    return uint32_t(b[0]) | (uint32_t(b[1]) << 8) | (uint32_t(b[2]) << 16) | (uint32_t(b[3]) << 24);
+}
+static int32_t sext(uint32_t word, int offset, int bits) {
+    const int spare = 32 - offset - bits;
+    return int32_t(word << spare) >> (32 - bits);
+}
+
+int oracle_convert(int format, const uint8_t* packed, uint64_t packed_bytes,
+                   const OConvertParams* p, float* ring, uint64_t ring_samples) {
+    static const uint64_t bytes_per_sample[4] = {2, 2, 4, 6};
+    if (format < 0 || format > 3) return -1;
+    const uint64_t capacity = uint64_t(p->ring_mask) + 1;
+    if ((capacity & p->ring_mask) != 0 || ring_samples < capacity || p->count > capacity) return -1;
+    const uint64_t last = uint64_t(p->src_offset) + p->count;
+    if (last > 0xFFFFFFFFull) return -1;
+    const uint64_t words = (last * bytes_per_sample[format] + 3) / 4;
+    if (packed_bytes < words * 4) return -1;
+    if (p->count == 0) return 0;
+    FlushGuard guard;
+    const float kCu8Bias = bits_to_float(0x42FF0000u);   // 127.5
+    const float kCu8Scale = bits_to_float(0x3C008081u);  // nearest float to 2/255
+    const float kCs8Scale = bits_to_float(0x3C000000u);  // 2^-7
+    const float kCs16Scale = bits_to_float(0x38000000u); // 2^-15
+    const float kCs24Scale = bits_to_float(0x34000000u); // 2^-23
+    for (uint32_t index = 0; index < p->count; ++index) {
+        const uint32_t s = p->src_offset + index;
+        float iv = 0.0f, qv = 0.0f;
+        if (format == 0 || format == 1) {
+            const uint32_t w = load_word(packed, s >> 1);
+            const int shift = int((s & 1u) << 4);
+            if (format == 0) {
+                const uint32_t ic = (w >> shift) & 0xFFu;
+                const uint32_t qc = (w >> (shift + 8)) & 0xFFu;
+                iv = (float(ic) - kCu8Bias) * kCu8Scale;
+                qv = (float(qc) - kCu8Bias) * kCu8Scale;
+            } else {
+                iv = float(sext(w, shift, 8)) * kCs8Scale;
+                qv = float(sext(w, shift + 8, 8)) * kCs8Scale;
+            }
+        } else if (format == 2) {
+            const uint32_t w = load_word(packed, s);
+            iv = float(sext(w, 0, 16)) * kCs16Scale;
+            qv = float(sext(w, 16, 16)) * kCs16Scale;
+        } else {
+            const uint32_t wi = s + (s >> 1);
+            const uint32_t lo = load_word(packed, wi);
+            const uint32_t hi = load_word(packed, uint64_t(wi) + 1);
+            const uint32_t shift = (s & 1u) << 4;
+            const uint32_t window = (lo >> shift) | ((hi << 16) << (16 - shift));
+            const uint32_t qraw = (window >> 24) | ((hi >> shift) << 8);
+            iv = float(sext(window, 0, 24)) * kCs24Scale;
+            qv = float(sext(qraw, 0, 24)) * kCs24Scale;
+        }
+        const uint64_t slot = (p->dst_offset + index) & p->ring_mask;
+        ring[2 * slot] = iv;
+        ring[2 * slot + 1] = qv;
+    }
+    return 0;
+}
+
+// Complex multiply, core/dsp/complex_ops.cpp:33-60 (cmul.comp). Used as the
+// device denormal probe and the plumbing check.
+int oracle_cmul(const float* a, const float* b, float* out, uint64_t n) {
+    FlushGuard guard;
+    for (uint64_t i = 0; i < n; ++i) {
+        const float ar = a[2 * i], ai = a[2 * i + 1], br = b[2 * i], bi = b[2 * i + 1];
+        const float re = ar * br - ai * bi;
+        const float im = ar * bi + ai * br;
+        out[2 * i] = re;
+        out[2 * i + 1] = im;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Grid validation, core/dsp/pfb_design.cpp:301-347, with the 2048-channel
+// ceiling lifted: that ceiling was Vulkan's 16 KiB shared-memory floor, and
+// the butterfly graph is the same at any power of two.
+// ---------------------------------------------------------------------------
+
+static bool grid_ok(const OGrid* g) {
+    if (g->channels < 2 || !pow2(g->channels) || g->channels > 65536) return false;
+    if (g->taps_per_branch == 0 || g->decimation == 0) return false;
+    if (g->channels % g->decimation != 0) return false;
+    if (uint64_t(g->channels) * g->taps_per_branch > (1u << 20)) return false;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Branch filter, core/dsp/pfb_branch_reference.cpp:22-147 (pfb_branch.comp).
+// u_r[m] = sum over ascending q of h[qM+r] * x[(base - qM - r) & mask], each
+// step acc = acc + s*c, written to slot (M - r) & (M - 1).
+// Pinned by: the FP64 direct channelizer definition through the FFT
+// (tests/test_oracle_pfb.py), the tone-in-its-channel test, and a brute-force
+// single-branch sum on tiny grids.
+// ---------------------------------------------------------------------------
+
+int oracle_pfb_branch(const OGrid* g, const float* taps, uint64_t ntaps, const float* ring,
+                      uint64_t ring_samples, const OBranchParams* p, float* out,
+                      uint64_t out_samples) {
+    if (!grid_ok(g)) return -1;
+    const uint32_t M = g->channels, T = g->taps_per_branch, D = g->decimation;
+    if (ntaps != uint64_t(M) * T) return -1;
+    if (out_samples != uint64_t(p->block_count) * M) return -1;
+    if (p->block_count == 0) return 0;
+    const uint64_t capacity = uint64_t(p->ring_mask) + 1;
+    if ((capacity & p->ring_mask) != 0 || ring_samples < capacity) return -1;
+    if (uint64_t(M) * T + uint64_t(p->block_count - 1) * D > capacity) return -1;
+    FlushGuard guard;
+    for (uint32_t block = 0; block < p->block_count; ++block) {
+        const uint32_t base = p->base_offset + block * D;
+        for (uint32_t r = 0; r < M; ++r) {
+            float acc_re = 0.0f, acc_im = 0.0f;
+            for (uint32_t q = 0; q < T; ++q) {
+                const uint32_t n = q * M + r;
+                const uint64_t idx = (base - n) & p->ring_mask;
+                const float c = taps[n];
+                acc_re = acc_re + ring[2 * idx] * c;
+                acc_im = acc_im + ring[2 * idx + 1] * c;
+            }
+            const uint32_t slot = (M - r) & (M - 1);
+            const uint64_t o = uint64_t(block) * M + slot;
+            out[2 * o] = acc_re;
+            out[2 * o + 1] = acc_im;
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Radix-2 DIT transform, core/dsp/pfb_fft_reference.cpp:54-104 (pfb_fft.comp
+// stage loop). Bit-reversed load, log2 M stages, the six-statement butterfly.
+// Pinned by: an FP64 brute-force DFT on sizes 2..4096 (tests), Parseval, and a
+// unit impulse giving an exact all-ones spectrum.
+// ---------------------------------------------------------------------------
+
+static uint32_t reverse_bits(uint32_t v, uint32_t bits) {
+    uint32_t r = 0;
+    for (uint32_t b = 0; b < bits; ++b) r |= ((v >> b) & 1u) << (bits - 1 - b);
+    return r;
+}
+
+static void transform(const float* in, const float* tw, float* s, uint32_t n, uint32_t stages) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t src = reverse_bits(i, stages);
+        s[2 * i] = in[2 * src];
+        s[2 * i + 1] = in[2 * src + 1];
+    }
+    for (uint32_t stage = 0; stage < stages; ++stage) {
+        const uint32_t half = 1u << stage;
+        const uint32_t step = n >> (stage + 1);
+        for (uint32_t b = 0; b < n / 2; ++b) {
+            const uint32_t j = b & (half - 1);
+            const uint32_t ia = ((b >> stage) << (stage + 1)) + j;
+            const uint32_t ib = ia + half;
+            const float ar = s[2 * ia], ai = s[2 * ia + 1];
+            const float br = s[2 * ib], bi = s[2 * ib + 1];
+            const float wr = tw[2 * (j * step)], wi = tw[2 * (j * step) + 1];
+            const float vr = br * wr - bi * wi;
+            const float vi = br * wi + bi * wr;
+            const float sr = ar + vr, si = ai + vi;
+            const float dr = ar - vr, di = ai - vi;
+            s[2 * ia] = sr;
+            s[2 * ia + 1] = si;
+            s[2 * ib] = dr;
+            s[2 * ib + 1] = di;
+        }
+    }
+}
+
+static uint32_t log2u(uint32_t v) {
+    uint32_t s = 0;
+    while ((1u << s) < v) ++s;
+    return s;
+}
+
+int oracle_fft_radix2(const float* tw, uint64_t ntw, const float* in, float* out, uint32_t size,
+                      uint32_t batch) {
+    if (size < 2 || !pow2(size) || ntw != size) return -1;
+    FlushGuard guard;
+    const uint32_t stages = log2u(size);
+    for (uint32_t b = 0; b < batch; ++b)
+        transform(in + 2 * uint64_t(b) * size, tw, out + 2 * uint64_t(b) * size, size, stages);
+    return 0;
+}
+
+// core/dsp/pfb_fft_reference.cpp:106-141 (validate)
+static bool fft_params_ok(const OFftParams* p) {
+    if (p->channels < 2 || !pow2(p->channels)) return false;
+    if (p->stages >= 32 || (1u << p->stages) != p->channels) return false;
+    if (p->decimation == 0 || p->decimation > p->channels || p->channels % p->decimation) return false;
+    if (!pow2(p->out_ring_blocks) || p->out_ring_mask != p->out_ring_blocks - 1) return false;
+    if (p->block_count > p->out_ring_blocks) return false;
+    return true;
+}
+
+// core/dsp/pfb_fft_reference.cpp:143-232: transform, then the output phase
+// correction x W[(k * dm) & (M - 1)] with dm = ((m & (M-1)) * D) & (M-1)
+// whenever D != M, written channel-major at k * out_ring_blocks + (m & mask).
+int oracle_pfb_fft(const OFftParams* p, const float* branch, uint64_t nbranch, const float* tw,
+                   uint64_t ntw, float* ring, uint64_t ring_samples) {
+    if (!fft_params_ok(p)) return -1;
+    const uint32_t M = p->channels;
+    if (ntw != M || nbranch < uint64_t(p->block_count) * M) return -1;
+    if (ring_samples < uint64_t(M) * p->out_ring_blocks) return -1;
+    FlushGuard guard;
+    std::vector<float> s(2 * size_t(M));
+    for (uint32_t block = 0; block < p->block_count; ++block) {
+        transform(branch + 2 * uint64_t(block) * M, tw, s.data(), M, p->stages);
+        const uint32_t m_abs = p->block_base + block;
+        const uint32_t slot = m_abs & p->out_ring_mask;
+        const uint32_t dm = ((m_abs & (M - 1)) * p->decimation) & (M - 1);
+        for (uint32_t k = 0; k < M; ++k) {
+            float vr = s[2 * k], vi = s[2 * k + 1];
+            if (p->decimation != M) {
+                const uint32_t c = (k * dm) & (M - 1);
+                const float cr = tw[2 * c], ci = tw[2 * c + 1];
+                const float pr = vr * cr - vi * ci;
+                const float pi = vr * ci + vi * cr;
+                vr = pr;
+                vi = pi;
+            }
+            const uint64_t o = uint64_t(k) * p->out_ring_blocks + slot;
+            ring[2 * o] = vr;
+            ring[2 * o + 1] = vi;
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic transcendentals, core/dsp/vrx_reference.cpp:227-306 and
+// spectrum_reference.cpp:123 (det_log2). Fixed Newton iterations and fixed
+// polynomials, no library calls, so any IEEE float implementation agrees.
+// Pinned by: libm log2/atan2/sqrt/1/x within stated ulp bounds on dense
+// sweeps (tests/test_oracle_det.py).
+// ---------------------------------------------------------------------------
+
+static float det_recip(float v) {
+    const float seed = bits_to_float(0x7EF311C3u - float_to_bits(v));
+    const float r1 = seed * (2.0f - v * seed);
+    const float r2 = r1 * (2.0f - v * r1);
+    const float r3 = r2 * (2.0f - v * r2);
+    const float r4 = r3 * (2.0f - v * r3);
+    return r4;
+}
+static float det_rsqrt(float v) {
+    const float seed = bits_to_float(0x5F375A86u - (float_to_bits(v) >> 1));
+    const float h = 0.5f * v;
+    const float r1 = seed * (1.5f - h * (seed * seed));
+    const float r2 = r1 * (1.5f - h * (r1 * r1));
+    const float r3 = r2 * (1.5f - h * (r2 * r2));
+    const float r4 = r3 * (1.5f - h * (r3 * r3));
+    return r4;
+}
+static float det_sqrt(float v) {
+    if (!(v > 0.0f)) return 0.0f;
+    return v * det_rsqrt(v);
+}
+static float det_atan_unit(float t) {
+    static const float c[10] = {
+        float(0.9999999998),  float(-0.3333331929), float(0.1999923872), float(-0.1427217912),
+        float(0.1099615976),  float(-0.0853800324), float(0.0603537785), float(-0.0336503503),
+        float(0.0122847765),  float(-0.0021090222)};
+    const float u = t * t;
+    float p = c[9];
+    for (int i = 8; i >= 0; --i) p = p * u + c[i];
+    return t * p;
+}
+static float det_atan2(float y, float x) {
+    const float kHalfPi = float(1.57079632679489661923);
+    const float kPi = float(3.14159265358979323846);
+    const float ay = std::fabs(y), ax = std::fabs(x);
+    if (ay == 0.0f && ax == 0.0f) return 0.0f;
+    const float num = (ax < ay) ? ax : ay;
+    const float den = (ay < ax) ? ax : ay;
+    const float ratio = num * det_recip(den);
+    float angle = det_atan_unit(ratio);
+    if (ay > ax) angle = kHalfPi - angle;
+    if (x < 0.0f) angle = kPi - angle;
+    return (y < 0.0f) ? -angle : angle;
+}
+static float det_log2(float v) {
+    const float kSqrt2 = 1.41421356237309505f;
+    const uint32_t bits = float_to_bits(v);
+    int e = int((bits >> 23) & 0xFFu) - 127;
+    float m = bits_to_float((bits & 0x007FFFFFu) | 0x3F800000u);
+    if (m > kSqrt2) {
+        m = m * 0.5f;
+        e = e + 1;
+    }
+    const float t = (m - 1.0f) * det_recip(m + 1.0f);
+    const float u = t * t;
+    float p = 0.32059889797532520f;
+    p = p * u + 0.41219858311113240f;
+    p = p * u + 0.57707801635558536f;
+    p = p * u + 0.96179669392597560f;
+    p = p * u + 2.88539008177792681f;
+    const float fraction = t * p;
+    return float(e) + fraction;
+}
+
+int oracle_det(int which, const float* a, const float* b, float* out, uint64_t n) {
+    FlushGuard guard;
+    for (uint64_t i = 0; i < n; ++i) {
+        switch (which) {
+            case 0: out[i] = det_log2(a[i]); break;
+            case 1: out[i] = det_recip(a[i]); break;
+            case 2: out[i] = det_rsqrt(a[i]); break;
+            case 3: out[i] = det_sqrt(a[i]); break;
+            case 4: out[i] = det_atan2(a[i], b[i]); break;
+            case 5: out[i] = det_atan_unit(a[i]); break;
+            default: return -1;
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Full-span spectrum, core/dsp/spectrum_reference.cpp:286-394 (spectrum.comp).
+// Per channel: window x N samples from in_offset, N-point DIT, keep the central
+// half (bin + N - N/4) & (N-1), p = re^2 + im^2, x correction, floor 1e-20,
+// det_log2 * 3.0103 dB, written at slot (k + M/2) & (M-1).
+// Pinned by: a tone landing in its bin, the unit coherent gain of the window
+// and the det_log2 checks (tests/test_oracle_spectrum.py).
+// ---------------------------------------------------------------------------
+
+int oracle_spectrum(const OSpectrumParams* p, const float* ring, uint64_t ring_samples,
+                    const float* tw, uint64_t ntw, const float* window, uint64_t nwindow,
+                    float* frame, uint64_t nframe) {
+    const uint32_t N = p->transform, M = p->channels;
+    if (M < 2 || !pow2(M) || N < 4 || !pow2(N) || N > 65536) return -1;
+    if (p->stages >= 32 || (1u << p->stages) != N) return -1;
+    if (!pow2(p->chan_blocks) || p->chan_mask != p->chan_blocks - 1 || N > p->chan_blocks) return -1;
+    if (ntw != N || nwindow != uint64_t(N) + N / 2) return -1;
+    if (ring_samples < uint64_t(M) * p->chan_blocks) return -1;
+    const uint32_t half = N / 2;
+    if (nframe < uint64_t(M) * half) return -1;
+    FlushGuard guard;
+    const float kDbPerOctave = 3.01029995663981195f;
+    std::vector<float> win(2 * size_t(N)), out(2 * size_t(N));
+    for (uint32_t ch = 0; ch < M; ++ch) {
+        const uint64_t origin = uint64_t(ch) * p->chan_blocks;
+        for (uint32_t n = 0; n < N; ++n) {
+            const uint64_t slot = (p->in_offset + n) & p->chan_mask;
+            const float g = window[n];
+            const float re = ring[2 * (origin + slot)] * g;
+            const float im = ring[2 * (origin + slot) + 1] * g;
+            win[2 * n] = re;
+            win[2 * n + 1] = im;
+        }
+        transform(win.data(), tw, out.data(), N, p->stages);
+        const uint64_t out_origin = uint64_t((ch + M / 2) & (M - 1)) * half;
+        for (uint32_t bin = 0; bin < half; ++bin) {
+            const uint32_t src = (bin + N - N / 4) & (N - 1);
+            const float vr = out[2 * src], vi = out[2 * src + 1];
+            const float power = vr * vr + vi * vi;
+            const float flat = power * window[N + bin];
+            const float floored = (flat < 1.0e-20f) ? 1.0e-20f : flat;
+            frame[out_origin + bin] = det_log2(floored) * kDbPerOctave;
+        }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Spectrum levels, core/dsp/spectrum_levels_reference.cpp:38-120: a 1024
+// bucket histogram over [-200, 40) dB and the buckets holding two permille
+// ranks. Pinned by: bucket boundaries and an exact sort of the frame agreeing
+// to within half a bucket (tests/test_oracle_spectrum.py).
+// ---------------------------------------------------------------------------
+
+static uint32_t level_bucket(float db) {
+    float t = (db - (-200.0f)) * 4.26666666666666666f;
+    t = (t < 0.0f) ? 0.0f : t;
+    t = (t > 1023.0f) ? 1023.0f : t;
+    return uint32_t(t);
+}
+static float level_value(uint32_t b) { return -200.0f + (float(b) + 0.5f) * 0.234375f; }
+static uint32_t rank_of(uint32_t count, uint32_t permille) {
+    return (count / 1000u) * permille + ((count % 1000u) * permille) / 1000u;
+}
+
+int oracle_spectrum_levels(const OLevelsParams* p, const float* frame, uint64_t nframe,
+                           float* levels) {
+    if (p->bins == 0 || p->low_permille > 1000 || p->high_permille > 1000 ||
+        p->low_permille >= p->high_permille || nframe < p->bins)
+        return -1;
+    FlushGuard guard;
+    std::vector<uint32_t> hist(1024, 0);
+    for (uint32_t i = 0; i < p->bins; ++i) ++hist[level_bucket(frame[i])];
+    const uint32_t lo_rank = std::min(rank_of(p->bins, p->low_permille), p->bins - 1);
+    const uint32_t hi_rank = std::min(rank_of(p->bins, p->high_permille), p->bins - 1);
+    uint32_t lo = 0, hi = 0, cum = 0;
+    bool lo_found = false, hi_found = false;
+    for (uint32_t b = 0; b < 1024; ++b) {
+        cum += hist[b];
+        if (!lo_found && cum > lo_rank) { lo = b; lo_found = true; }
+        if (!hi_found && cum > hi_rank) { hi = b; hi_found = true; }
+    }
+    levels[0] = level_value(lo);
+    levels[1] = level_value(hi);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Exponentially averaged power spectrum (this project's operator; the
+// reference's average lives in the host detector, detector.cpp:621-627).
+//
+// Partial: for each frame f of frame_blocks consecutive channel samples and
+// each channel k, s starts at 0 and for ascending blocks
+//     x = re*re + im*im;  s = s + (x - s) * alpha
+// Combine: P = P * frame_decay + s for ascending frames, frame_decay being
+// (1 - alpha)^frame_blocks rounded once to float. P after frame f is frame
+// f's row of the averaged spectrum.
+// Pinned by: the FP64 exponential average of the same powers (relative error
+// bound), a constant input converging to its value, and alpha = 1 reducing to
+// the last power (tests/test_oracle_ema.py).
+// ---------------------------------------------------------------------------
+
+int oracle_ema_partial(const OEmaParams* p, const float* ring, uint64_t ring_samples,
+                       float* partial, uint64_t npartial) {
+    if (p->channels == 0 || !pow2(p->chan_blocks) || p->chan_mask != p->chan_blocks - 1) return -1;
+    if (p->frame_blocks == 0 || uint64_t(p->frame_blocks) * p->frames > p->chan_blocks) return -1;
+    if (ring_samples < uint64_t(p->channels) * p->chan_blocks) return -1;
+    if (npartial < uint64_t(p->frames) * p->channels) return -1;
+    FlushGuard guard;
+    for (uint32_t f = 0; f < p->frames; ++f) {
+        for (uint32_t k = 0; k < p->channels; ++k) {
+            float s = 0.0f;
+            for (uint32_t i = 0; i < p->frame_blocks; ++i) {
+                const uint64_t slot = (p->in_offset + f * p->frame_blocks + i) & p->chan_mask;
+                const uint64_t o = uint64_t(k) * p->chan_blocks + slot;
+                const float re = ring[2 * o], im = ring[2 * o + 1];
+                const float x = re * re + im * im;
+                s = s + (x - s) * p->alpha;
+            }
+            partial[uint64_t(f) * p->channels + k] = s;
+        }
+    }
+    return 0;
+}
+
+int oracle_ema_combine(const OEmaParams* p, const float* partial, uint64_t npartial, float* state,
+                       float* rows, uint64_t nrows) {
+    if (npartial < uint64_t(p->frames) * p->channels) return -1;
+    if (rows != nullptr && nrows < uint64_t(p->frames) * p->channels) return -1;
+    FlushGuard guard;
+    for (uint32_t k = 0; k < p->channels; ++k) {
+        float P = state[k];
+        for (uint32_t f = 0; f < p->frames; ++f) {
+            P = P * p->frame_decay + partial[uint64_t(f) * p->channels + k];
+            if (rows != nullptr) rows[uint64_t(f) * p->channels + k] = P;
+        }
+        state[k] = P;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// The plain chain as a composition of the twins above: convert the input
+// codes into a ring, branch filter, transform + correction into a channel
+// ring, exponentially averaged power per frame. Input index 0 is absolute
+// sample 0; block m (absolute) is taken at base m*D and needs samples
+// m*D - N + 1 .. m*D. This is the CPU baseline of the benchmark and the
+// referee of the fused GPU chain.
+// ---------------------------------------------------------------------------
+
+struct OChain {
+    int format;                // 0 cu8, 2 cs16
+    uint32_t channels, taps_per_branch, decimation;
+    uint64_t first_block;      // absolute block index of block 0
+    uint32_t block_count;
+    uint32_t out_ring_blocks;  // channel ring capacity in blocks (power of two)
+    uint32_t frame_blocks;     // EMA frame; 0 skips the average
+    float alpha, frame_decay;
+};
+
+static int chain_range(const OChain* c, const uint8_t* input, uint64_t input_bytes,
+                       const float* taps, const float* tw, uint64_t b0, uint32_t nb,
+                       float* ring_out, float* partial) {
+    const uint32_t M = c->channels, D = c->decimation, N = M * c->taps_per_branch;
+    const uint64_t bps = (c->format == 0) ? 2 : 4;
+    OGrid g{M, c->taps_per_branch, D};
+    // Work in sub-batches whose window fits a local ring.
+    const uint32_t sub = std::max<uint32_t>(1, std::min<uint32_t>(nb, 64));
+    uint64_t cap = 1;
+    while (cap < uint64_t(N) + uint64_t(sub - 1) * D) cap <<= 1;
+    std::vector<float> ring(2 * cap), branch(2 * size_t(sub) * M);
+    for (uint32_t done = 0; done < nb; done += sub) {
+        const uint32_t n = std::min(sub, nb - done);
+        const uint64_t m0 = c->first_block + b0 + done;
+        const uint64_t first_sample = m0 * D - (N - 1);
+        const uint64_t count = uint64_t(N) + uint64_t(n - 1) * D;
+        if ((first_sample + count) * bps > input_bytes) return -1;
+        OConvertParams cp{uint32_t(cap - 1), uint32_t(first_sample), 0, uint32_t(count)};
+        const uint64_t whole = (count * bps + 3) & ~uint64_t(3);
+        const uint8_t* src = input + first_sample * bps;
+        std::vector<uint8_t> padded;
+        if (first_sample * bps + whole > input_bytes) {
+            // The conversion loads whole words; give the odd tail its padding.
+            padded.assign(whole, 0);
+            std::memcpy(padded.data(), src, count * bps);
+            src = padded.data();
+        }
+        if (oracle_convert(c->format, src, whole, &cp, ring.data(), cap)) return -1;
+        OBranchParams bp{uint32_t(cap - 1), uint32_t(m0 * D), n};
+        if (oracle_pfb_branch(&g, taps, uint64_t(N), ring.data(), cap, &bp, branch.data(),
+                              uint64_t(n) * M))
+            return -1;
+        OFftParams fp{M, D, log2u(M), uint32_t(m0), n, c->out_ring_blocks, c->out_ring_blocks - 1};
+        if (oracle_pfb_fft(&fp, branch.data(), uint64_t(n) * M, tw, M, ring_out,
+                           uint64_t(M) * c->out_ring_blocks))
+            return -1;
+    }
+    if (c->frame_blocks != 0 && partial != nullptr) {
+        const uint32_t frames = nb / c->frame_blocks;
+        OEmaParams ep{M, c->out_ring_blocks, c->out_ring_blocks - 1,
+                      uint32_t(c->first_block + b0), c->frame_blocks, frames, c->alpha,
+                      c->frame_decay};
+        if (frames && oracle_ema_partial(&ep, ring_out, uint64_t(M) * c->out_ring_blocks,
+                                         partial + (b0 / c->frame_blocks) * M,
+                                         uint64_t(frames) * M))
+            return -1;
+    }
+    return 0;
+}
+
+// input: packed codes from absolute sample 0. ring_out: M x out_ring_blocks
+// cf32. partial: (block_count / frame_blocks) x M floats, or null.
+int oracle_chain(const OChain* c, const uint8_t* input, uint64_t input_bytes, const float* taps,
+                 uint64_t ntaps, const float* tw, uint64_t ntw, float* ring_out,
+                 uint64_t ring_samples, float* partial, uint64_t npartial, int threads) {
+    const uint32_t M = c->channels;
+    OGrid g{M, c->taps_per_branch, c->decimation};
+    if (!grid_ok(&g) || (c->format != 0 && c->format != 2)) return -1;
+    if (ntaps != uint64_t(M) * c->taps_per_branch || ntw != M) return -1;
+    if (!pow2(c->out_ring_blocks) || c->block_count > c->out_ring_blocks) return -1;
+    if (ring_samples < uint64_t(M) * c->out_ring_blocks) return -1;
+    const uint64_t N = uint64_t(M) * c->taps_per_branch;
+    if (c->first_block * c->decimation < N - 1) return -1;
+    if (c->frame_blocks != 0) {
+        if (c->block_count % c->frame_blocks) return -1;
+        if (partial && npartial < uint64_t(c->block_count / c->frame_blocks) * M) return -1;
+    }
+    if (threads < 1) threads = 1;
+    // Split on frame boundaries so every frame's average is one thread's.
+    const uint32_t unit = c->frame_blocks ? c->frame_blocks : 1;
+    const uint32_t units = c->block_count / unit;
+    const uint32_t per = (units + threads - 1) / threads;
+    std::vector<int> rc(threads, 0);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) {
+        const uint64_t u0 = uint64_t(t) * per;
+        if (u0 >= units) break;
+        const uint32_t nu = uint32_t(std::min<uint64_t>(per, units - u0));
+        pool.emplace_back([&, t, u0, nu] {
+            rc[t] = chain_range(c, input, input_bytes, taps, tw, u0 * unit, nu * unit, ring_out,
+                                partial);
+        });
+    }
+    for (auto& th : pool) th.join();
+    for (int r : rc)
+        if (r) return r;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,151 @@
+"""ctypes binding of ``librvx_hip.so`` (the C ABI declared in include/rvx.h).
+
+The library is required: importing the operators without it raises, and on a
+machine with a GPU nothing falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "librvx_hip.so"
+
+u32, i32, u64, i64, f32, f64, vp = (C.c_uint32, C.c_int32, C.c_uint64, C.c_int64, C.c_float,
+                                    C.c_double, C.c_void_p)
+fptr = C.POINTER(C.c_float)
+
+
+class RvxError(RuntimeError):
+    """A refused parameter block or a failed device call, with the library's message."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code
+
+
+class Grid(C.Structure):
+    _fields_ = [("channels", u32), ("taps_per_branch", u32), ("decimation", u32)]
+
+
+class ConvertParams(C.Structure):
+    _fields_ = [("ring_mask", u32), ("dst_offset", u32), ("src_offset", u32), ("count", u32)]
+
+
+class PfbBranchParams(C.Structure):
+    _fields_ = [("ring_mask", u32), ("base_offset", u32), ("block_count", u32)]
+
+
+class PfbFftParams(C.Structure):
+    _fields_ = [("channels", u32), ("decimation", u32), ("stages", u32), ("block_base", u32),
+                ("block_count", u32), ("out_ring_blocks", u32), ("out_ring_mask", u32)]
+
+
+class SpectrumParams(C.Structure):
+    _fields_ = [("channels", u32), ("transform", u32), ("stages", u32), ("chan_blocks", u32),
+                ("chan_mask", u32), ("in_offset", u32)]
+
+
+class SpectrumLevelsParams(C.Structure):
+    _fields_ = [("bins", u32), ("low_permille", u32), ("high_permille", u32)]
+
+
+class EmaParams(C.Structure):
+    _fields_ = [("channels", u32), ("chan_blocks", u32), ("chan_mask", u32), ("in_offset", u32),
+                ("frame_blocks", u32), ("frames", u32), ("alpha", f32), ("frame_decay", f32)]
+
+
+class VrxFineConfig(C.Structure):
+    _fields_ = [("taps", u32), ("phases", u32), ("nco_log2", u32)]
+
+
+class VrxFineParams(C.Structure):
+    _fields_ = [(n, u32) for n in ("chan_base", "chan_mask", "in_offset", "out_offset",
+                                   "out_mask", "count", "step_whole", "step_rem", "out_rate",
+                                   "frac0", "nco_phase_high", "nco_phase_low", "nco_delta_high",
+                                   "nco_delta_low")] + [("inv_out_rate", f32)]
+
+
+class VrxDemodConfig(C.Structure):
+    _fields_ = [(n, u32) for n in ("mode", "decimation", "audio_taps", "dc_taps", "channels",
+                                   "pilot_taps")]
+
+
+class VrxDemodParams(C.Structure):
+    _fields_ = [("in_mask", u32), ("in_offset", u32), ("count", u32), ("gain", f32)]
+
+
+class ChainDesc(C.Structure):
+    _fields_ = [("format", i32), ("channels", u32), ("taps_per_branch", u32),
+                ("decimation", u32), ("input_origin", u64), ("input_samples", u64),
+                ("first_block", u64), ("block_count", u32), ("out_ring_blocks", u32),
+                ("frame_blocks", u32), ("alpha", f32)]
+
+
+assert C.sizeof(VrxFineParams) == 60 and C.sizeof(VrxDemodParams) == 16
+assert C.sizeof(ConvertParams) == 16 and C.sizeof(PfbBranchParams) == 12
+
+_SIGNATURES = {
+    "rvx_abi_version": (i32, []),
+    "rvx_last_error": (C.c_char_p, []),
+    "rvx_ctx_create": (i32, [i32, C.POINTER(vp)]),
+    "rvx_ctx_destroy": (None, [vp]),
+    "rvx_ctx_set_stream": (i32, [vp, vp]),
+    "rvx_ctx_stream": (vp, [vp]),
+    "rvx_stream_sync": (i32, [vp]),
+    "rvx_buf_alloc": (i32, [vp, u64, i32, C.POINTER(vp)]),
+    "rvx_buf_free": (i32, [vp, vp]),
+    "rvx_buf_write": (i32, [vp, vp, vp, u64]),
+    "rvx_buf_read": (i32, [vp, vp, vp, u64]),
+    "rvx_buf_clear": (i32, [vp, vp, u64]),
+    "rvx_timer_begin": (i32, [vp]),
+    "rvx_timer_end": (i32, [vp, fptr]),
+    "rvx_design_prototype": (i32, [C.POINTER(Grid), f64, f64, fptr, u64, C.POINTER(f64),
+                                   C.POINTER(u64)]),
+    "rvx_build_twiddles": (i32, [u32, fptr, u64]),
+    "rvx_build_spectrum_window": (i32, [u32, u32, f64, fptr, u64]),
+    "rvx_build_nco_table": (i32, [u32, fptr, u64]),
+    "rvx_turn_fixed64": (u64, [i64, i64]),
+    "rvx_nco_delta": (i32, [i64, i64, i64, C.POINTER(u64)]),
+    "rvx_convert": (i32, [vp, i32, vp, u64, C.POINTER(ConvertParams), vp, u64]),
+    "rvx_cmul": (i32, [vp, vp, vp, vp, u64]),
+    "rvx_pfb_branch": (i32, [vp, C.POINTER(Grid), vp, u64, vp, u64, C.POINTER(PfbBranchParams),
+                             vp, u64]),
+    "rvx_pfb_fft": (i32, [vp, C.POINTER(PfbFftParams), vp, u64, vp, u64, vp, u64]),
+    "rvx_fft_radix2": (i32, [vp, vp, u64, vp, vp, u32, u32]),
+    "rvx_spectrum": (i32, [vp, C.POINTER(SpectrumParams), vp, u64, vp, u64, vp, u64, vp, u64]),
+    "rvx_spectrum_levels": (i32, [vp, C.POINTER(SpectrumLevelsParams), vp, u64, vp]),
+    "rvx_ema_partial": (i32, [vp, C.POINTER(EmaParams), vp, u64, vp, u64]),
+    "rvx_ema_combine": (i32, [vp, C.POINTER(EmaParams), vp, u64, vp, vp, u64]),
+    "rvx_chain": (i32, [vp, C.POINTER(ChainDesc), vp, vp, u64, vp, u64, vp, u64, vp, u64]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with "
+                              "`python -m revenant_amd.build` (there is no CPU fallback)")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    """Names the C ABI promises (every function declared in include/rvx.h)."""
+    return list(_SIGNATURES)
+
+
+def check(code: int) -> None:
+    if code != 0:
+        msg = lib().rvx_last_error()
+        raise RvxError(code, msg.decode() if msg else "")

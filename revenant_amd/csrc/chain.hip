@@ -1,0 +1,335 @@
+// The plain receive chain fused into one launch: native IQ codes -> branch
+// filter -> M-point transform -> output phase correction -> channel ring, with
+// the averaged power spectrum accumulated on chip.
+//
+// Bit-identical to convert (convert.cpp:153,235) -> pfb_branch
+// (pfb_branch_reference.cpp:22) -> pfb_fft (pfb_fft_reference.cpp:143) ->
+// rvx_ema_partial run one after another, which is how the reference records
+// them (core/engine/graph.cpp:5120-5196): every value is computed by the same
+// operations in the same order. What changes is where values live:
+//   - the converted samples are never written: each tap converts its code on
+//     the fly, exactly as convert would have;
+//   - the branch outputs never leave the chip: each thread's branch sums are
+//     the inputs of its first group of butterflies, so they go straight from
+//     the tap loop into the first FFT pass in registers;
+//   - the power average's state stays in registers across the frame.
+//
+// Work split. A workgroup owns a chunk of consecutive blocks (one averaging
+// frame) and walks it in order, so the frame's average accumulates in the
+// order its definition fixes. Thread t of a transform owns V branches; their
+// V*TAPS prototype taps are loaded once and kept in registers for the whole
+// chunk. Chunks are numbered so that the workgroups one XCD runs at the same
+// time own neighbouring stretches of input and share its L2.
+
+#pragma clang fp contract(off)
+
+#include <algorithm>
+
+#include "common.h"
+#include "fft.h"
+
+namespace {
+
+using rvx::Fft;
+
+struct ChainArgs {
+    const void* input;
+    const float* taps;
+    const float2* tw;
+    float2* ring;
+    float* partial;
+    uint64_t first_block;
+    uint64_t input_origin;
+    uint32_t decimation;
+    uint32_t taps_per_branch;
+    uint32_t block_count;
+    uint32_t ring_mask;
+    uint32_t ring_blocks;
+    uint32_t chunk;   // blocks per workgroup (the averaging frame when averaging)
+    uint32_t chunks;  // workgroups
+    uint32_t average; // nonzero: write the frame partials
+    float alpha;
+};
+
+template <int LOGM>
+struct ChainShape {
+    static constexpr int LOGV = LOGM < 2 ? LOGM : 2;
+    using F = Fft<LOGM, LOGV>;
+    static constexpr int THREADS = F::T >= 256 ? F::T : 256;
+    static constexpr int BPW = THREADS / F::T;  // blocks in flight per workgroup
+};
+
+template <int FORMAT>
+__device__ __forceinline__ float2 load_sample(const void* input, uint32_t index) {
+    if constexpr (FORMAT == RVX_FORMAT_CS16) {
+        const uint32_t w = static_cast<const uint32_t*>(input)[index];
+        const float scale = __uint_as_float(0x38000000u);
+        const float i = float(int32_t(w << 16) >> 16) * scale;
+        const float q = float(int32_t(w) >> 16) * scale;
+        return make_float2(i, q);
+    } else {
+        const uint32_t w = static_cast<const uint16_t*>(input)[index];
+        const float bias = __uint_as_float(0x42FF0000u);
+        const float scale = __uint_as_float(0x3C008081u);
+        const float i = (float(w & 0xFFu) - bias) * scale;
+        const float q = (float(w >> 8) - bias) * scale;
+        return make_float2(i, q);
+    }
+}
+
+// TAPS > 0: the tap count is a compile-time constant and taps live in
+// registers; TAPS == 0: any tap count, taps read from memory each block.
+template <int LOGM, int FORMAT, int TAPS>
+__global__ void __launch_bounds__(ChainShape<LOGM>::THREADS)
+chain_kernel(ChainArgs a) {
+    using S = ChainShape<LOGM>;
+    using F = typename S::F;
+    constexpr uint32_t M = F::N;
+    constexpr int V = F::V;
+    constexpr int T = F::T;
+    constexpr int LOGT = F::LOGT;
+    constexpr int LAST = F::PASSES - 1;
+    constexpr int NT = TAPS > 0 ? TAPS : 1;
+
+    __shared__ float2 lds[S::BPW * M];
+    __shared__ float pow_lds[S::BPW > 1 ? S::BPW * M : 1];
+
+    // XCD-aware chunk numbering: the hardware deals workgroups to the 8 XCDs
+    // round robin, so workgroups w, w+8, w+16, ... share an XCD. Give each
+    // XCD a contiguous run of chunks.
+    const uint32_t w = blockIdx.x;
+    const uint32_t per_xcd = (a.chunks + 7) / 8;
+    const uint32_t xcd = w % 8, seq = w / 8;
+    uint32_t chunk = xcd * per_xcd + seq;
+    // When chunks is not a multiple of 8 the mapping above can overshoot;
+    // fall back to the identity numbering, which is always a bijection.
+    if ((a.chunks % 8) != 0) chunk = w;
+    if (chunk >= a.chunks) return;  // whole workgroup leaves together
+
+    const uint32_t sub = threadIdx.x / T;
+    const uint32_t t = threadIdx.x % T;
+    float2* my = lds + sub * M;
+
+    // The V branches this thread filters: slots t + T*i (i = 0..V-1), which
+    // are exactly the inputs of the first-pass butterfly group that starts at
+    // position bitrev(t) * V (slot s -> position bitrev_M(s)).
+    uint32_t branch[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) branch[i] = (M - (t + uint32_t(T) * i)) & (M - 1);
+
+    float c[V][NT];
+    if constexpr (TAPS > 0) {
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+#pragma unroll
+            for (int q = 0; q < TAPS; ++q) c[i][q] = a.taps[q * M + branch[i]];
+    }
+
+    float ema[V];
+#pragma unroll
+    for (int r = 0; r < V; ++r) ema[r] = 0.0f;
+
+    const uint32_t first = chunk * a.chunk;
+    for (uint32_t b0 = 0; b0 < a.chunk; b0 += S::BPW) {
+        const uint32_t rel = first + b0 + sub;           // block within the launch
+        const bool active = (b0 + sub < a.chunk) && (rel < a.block_count);
+        const uint64_t m = a.first_block + rel;           // absolute block index
+        const uint32_t base = uint32_t(m * a.decimation - a.input_origin);
+
+        // The twiddle loads stay inside the block loop (served by L1): hoisted
+        // out of it they would take registers the prototype taps need.
+        const float2* tw = a.tw;
+        asm volatile("" : "+s"(tw));
+
+        float2 v[V];
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+                float re = 0.0f, im = 0.0f;
+                const uint32_t start = base - branch[i];
+                if constexpr (TAPS > 0) {
+#pragma unroll
+                    for (int q = 0; q < TAPS; ++q) {
+                        const float2 x = load_sample<FORMAT>(a.input, start - uint32_t(q) * M);
+                        re = re + x.x * c[i][q];
+                        im = im + x.y * c[i][q];
+                    }
+                } else {
+                    for (uint32_t q = 0; q < a.taps_per_branch; ++q) {
+                        const float2 x = load_sample<FORMAT>(a.input, start - q * M);
+                        const float tap = a.taps[q * M + branch[i]];
+                        re = re + x.x * tap;
+                        im = im + x.y * tap;
+                    }
+                }
+                // Register index of slot t + T*i within the group is bitrev(i).
+                v[rvx::bitrev<F::LOG_V>(i)] = make_float2(re, im);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < V; ++i) v[i] = make_float2(0.0f, 0.0f);
+        }
+
+        F::template run_pass<0>(v, tw, 0);
+        if constexpr (F::PASSES > 1) {
+            // Everyone has finished reading the previous block's last pass.
+            __syncthreads();
+            const uint32_t g0 = rvx::bitrev<LOGT>(t) * V;
+#pragma unroll
+            for (int r = 0; r < V; ++r) my[F::phys(g0 + r)] = v[r];
+            __syncthreads();
+            F::template load<1>(my, v, t);
+            F::template run_pass<1>(v, tw, t);
+            F::template rest<2>(my, v, tw, t);
+        }
+
+        const uint32_t m_lo = uint32_t(m);
+        const uint32_t slot = m_lo & a.ring_mask;
+        const uint32_t dm = ((m_lo & (M - 1)) * a.decimation) & (M - 1);
+        float x[V];
+#pragma unroll
+        for (int r = 0; r < V; ++r) {
+            const uint32_t k = F::template pos<LAST>(t, r);
+            float2 value = v[r];
+            if (a.decimation != M) value = rvx::cmul_exact(value, tw[(k * dm) & (M - 1)]);
+            if (active) a.ring[uint64_t(k) * a.ring_blocks + slot] = value;
+            x[r] = value.x * value.x + value.y * value.y;
+        }
+        if constexpr (S::BPW == 1) {
+#pragma unroll
+            for (int r = 0; r < V; ++r) ema[r] = ema[r] + (x[r] - ema[r]) * a.alpha;
+        } else {
+            // Blocks in flight finish together; fold their powers in block
+            // order through LDS.
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < V; ++r) pow_lds[sub * M + F::template pos<LAST>(t, r)] = x[r];
+            __syncthreads();
+            if (sub == 0) {
+                for (uint32_t bb = 0; bb < S::BPW; ++bb) {
+                    if (b0 + bb >= a.chunk || first + b0 + bb >= a.block_count) break;
+#pragma unroll
+                    for (int r = 0; r < V; ++r) {
+                        const float xr = pow_lds[bb * M + F::template pos<LAST>(t, r)];
+                        ema[r] = ema[r] + (xr - ema[r]) * a.alpha;
+                    }
+                }
+            }
+        }
+    }
+    if (a.average && sub == 0) {
+#pragma unroll
+        for (int r = 0; r < V; ++r)
+            a.partial[uint64_t(chunk) * M + F::template pos<LAST>(t, r)] = ema[r];
+    }
+}
+
+template <int LOGM, int FORMAT>
+void launch_chain_m(const ChainArgs& a, uint32_t taps, hipStream_t stream) {
+    using S = ChainShape<LOGM>;
+    const dim3 grid(a.chunks), block(S::THREADS);
+    if (taps == 16)
+        chain_kernel<LOGM, FORMAT, 16><<<grid, block, 0, stream>>>(a);
+    else if (taps == 17)
+        chain_kernel<LOGM, FORMAT, 17><<<grid, block, 0, stream>>>(a);
+    else
+        chain_kernel<LOGM, FORMAT, 0><<<grid, block, 0, stream>>>(a);
+}
+
+template <int FORMAT>
+int32_t launch_chain(uint32_t logm, const ChainArgs& a, uint32_t taps, hipStream_t stream) {
+    switch (logm) {
+        case 1: launch_chain_m<1, FORMAT>(a, taps, stream); break;
+        case 2: launch_chain_m<2, FORMAT>(a, taps, stream); break;
+        case 3: launch_chain_m<3, FORMAT>(a, taps, stream); break;
+        case 4: launch_chain_m<4, FORMAT>(a, taps, stream); break;
+        case 5: launch_chain_m<5, FORMAT>(a, taps, stream); break;
+        case 6: launch_chain_m<6, FORMAT>(a, taps, stream); break;
+        case 7: launch_chain_m<7, FORMAT>(a, taps, stream); break;
+        case 8: launch_chain_m<8, FORMAT>(a, taps, stream); break;
+        case 9: launch_chain_m<9, FORMAT>(a, taps, stream); break;
+        case 10: launch_chain_m<10, FORMAT>(a, taps, stream); break;
+        case 11: launch_chain_m<11, FORMAT>(a, taps, stream); break;
+        case 12: launch_chain_m<12, FORMAT>(a, taps, stream); break;
+        default: return rvx::fail(RVX_EINVAL, "rvx_chain: 2^%u channels is not supported", logm);
+    }
+    return RVX_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const void* input,
+                             const float* taps, uint64_t n_taps, const void* twiddles,
+                             uint64_t n_twiddles, void* channel_ring, uint64_t ring_samples,
+                             float* partial, uint64_t n_partial) {
+    static const char* who = "rvx_chain";
+    RVX_ENTER(ctx, who);
+    if (desc == nullptr) return rvx::fail(RVX_EINVAL, "%s: null descriptor", who);
+    const rvx_chain_desc d = *desc;
+    const uint32_t M = d.channels, T = d.taps_per_branch, D = d.decimation;
+    if (d.format != RVX_FORMAT_CU8 && d.format != RVX_FORMAT_CS16)
+        return rvx::fail(RVX_EINVAL, "%s: format %d (the fused chain reads cu8 or cs16)", who,
+                         d.format);
+    if (M < 2 || !rvx::pow2(M) || M > 4096 || T == 0 || D == 0 || M % D != 0 ||
+        uint64_t(M) * T > (1u << 20))
+        return rvx::fail(RVX_EINVAL, "%s: invalid grid M=%u taps=%u D=%u (M <= 4096)", who, M, T,
+                         D);
+    if (n_taps != uint64_t(M) * T || n_twiddles != M)
+        return rvx::fail(RVX_EINVAL, "%s: wants %llu taps and %u twiddles", who,
+                         (unsigned long long)(uint64_t(M) * T), M);
+    if (d.block_count == 0) return RVX_OK;
+    if (!rvx::pow2(d.out_ring_blocks) || d.block_count > d.out_ring_blocks)
+        return rvx::fail(RVX_EINVAL,
+                         "%s: %u blocks into a %u-block ring (power of two, >= blocks)", who,
+                         d.block_count, d.out_ring_blocks);
+    if (ring_samples < uint64_t(M) * d.out_ring_blocks)
+        return rvx::fail(RVX_EINVAL, "%s: channel ring holds %llu samples, needs %llu", who,
+                         (unsigned long long)ring_samples,
+                         (unsigned long long)(uint64_t(M) * d.out_ring_blocks));
+    const uint64_t N = uint64_t(M) * T;
+    if (d.input_samples >= (uint64_t(1) << 31))
+        return rvx::fail(RVX_EINVAL, "%s: %llu input samples exceed 2^31", who,
+                         (unsigned long long)d.input_samples);
+    const uint64_t first_base = d.first_block * D;
+    const uint64_t last_base = (d.first_block + d.block_count - 1) * D;
+    if (first_base < d.input_origin + (N - 1))
+        return rvx::fail(RVX_EINVAL, "%s: block %llu needs samples before the input starts", who,
+                         (unsigned long long)d.first_block);
+    if (last_base >= d.input_origin + d.input_samples)
+        return rvx::fail(RVX_EINVAL, "%s: the last block needs samples past the input's end", who);
+    uint32_t chunk = 16;
+    if (d.frame_blocks != 0) {
+        if (d.block_count % d.frame_blocks)
+            return rvx::fail(RVX_EINVAL, "%s: frame of %u blocks does not divide %u blocks", who,
+                             d.frame_blocks, d.block_count);
+        if (n_partial < uint64_t(d.block_count / d.frame_blocks) * M || partial == nullptr)
+            return rvx::fail(RVX_EINVAL, "%s: partial output too small", who);
+        chunk = d.frame_blocks;
+    }
+    if (!input || !taps || !twiddles || !channel_ring)
+        return rvx::fail(RVX_EINVAL, "%s: null buffer", who);
+
+    ChainArgs a{};
+    a.input = input;
+    a.taps = taps;
+    a.tw = static_cast<const float2*>(twiddles);
+    a.ring = static_cast<float2*>(channel_ring);
+    a.partial = partial;
+    a.first_block = d.first_block;
+    a.input_origin = d.input_origin;
+    a.decimation = D;
+    a.taps_per_branch = T;
+    a.block_count = d.block_count;
+    a.ring_mask = d.out_ring_blocks - 1;
+    a.ring_blocks = d.out_ring_blocks;
+    a.chunk = chunk;
+    a.chunks = (d.block_count + chunk - 1) / chunk;
+    a.average = d.frame_blocks != 0;
+    a.alpha = d.alpha;
+    const uint32_t logm = rvx::log2u(M);
+    const int32_t rc = d.format == RVX_FORMAT_CS16
+                           ? launch_chain<RVX_FORMAT_CS16>(logm, a, T, ctx->stream)
+                           : launch_chain<RVX_FORMAT_CU8>(logm, a, T, ctx->stream);
+    if (rc) return rc;
+    return rvx::check_launch(who);
+}

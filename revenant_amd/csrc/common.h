@@ -1,0 +1,52 @@
+// Internal plumbing shared by the library's translation units: the context,
+// the thread-local error, and the HIP-call check.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+
+#include "../../include/rvx.h"
+
+struct rvx_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t t0 = nullptr;
+    hipEvent_t t1 = nullptr;
+};
+
+namespace rvx {
+
+void set_error(const std::string& message);
+int32_t fail(int32_t code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+inline bool pow2(uint64_t v) { return v != 0 && (v & (v - 1)) == 0; }
+inline uint32_t log2u(uint64_t v) {
+    uint32_t s = 0;
+    while ((uint64_t(1) << s) < v) ++s;
+    return s;
+}
+
+// Makes the context's device current and clears any stale launch error.
+int32_t enter(rvx_ctx* ctx, const char* who);
+// Reports a launch failure as RVX_EDEVICE.
+int32_t check_launch(const char* who);
+
+}  // namespace rvx
+
+#define RVX_TRY_HIP(call, who)                                                      \
+    do {                                                                            \
+        hipError_t rvx_err_ = (call);                                               \
+        if (rvx_err_ != hipSuccess)                                                 \
+            return rvx::fail(RVX_EDEVICE, "%s: %s failed: %s", who, #call,          \
+                             hipGetErrorString(rvx_err_));                          \
+    } while (0)
+
+#define RVX_ENTER(ctx, who)                                    \
+    do {                                                       \
+        int32_t rvx_rc_ = rvx::enter((ctx), (who));            \
+        if (rvx_rc_ != RVX_OK) return rvx_rc_;                 \
+    } while (0)

@@ -1,0 +1,154 @@
+// Radix-2 decimation-in-time transform, register-blocked for CDNA4.
+//
+// The butterfly graph is exactly the one core/dsp/pfb_fft_reference.cpp:54-104
+// fixes (pfb_fft.comp:291-332): bit-reversed load, log2 N stages, stage s
+// pairing (ia, ia + 2^s) with twiddle W[j * (N >> (s + 1))], j = ia mod 2^s,
+// and the six-statement butterfly
+//     vr = b.x*w.x - b.y*w.y;  vi = b.x*w.y + b.y*w.x;
+//     a' = a + v;              b' = a - v.
+// Each butterfly reads only its own two values, so its result depends on
+// (N, stage, index) and not on the order butterflies are visited. That is what
+// lets a thread hold V = 2^LOGV values and run LOGV consecutive stages in
+// registers (one "pass") between LDS exchanges: every butterfly is the
+// reference's, with the reference's twiddle, computed the same way.
+//
+// Pass p covers stages [p*LOGV, p*LOGV + LS) with LS = LOGV except possibly the
+// last. A pass partitions the N positions into groups of G = 2^LS:
+//     group g: j = g mod 2^S0, base = (g >> S0) << (S0 + LS),
+//     element i at base + j + (i << S0)
+// and a thread owns V/G whole groups. Stage S0 + l pairs elements i and
+// i + 2^l (bit l of i clear) with twiddle index
+//     (j + ((i mod 2^l) << S0)) * (N >> (S0 + l + 1)).
+//
+// LDS holds the N values between passes under a bank swizzle that XORs the low
+// LOGV bits of a position with higher bits; it is a bijection inside each
+// aligned group of V, so it moves no value anywhere it is not read back from.
+#pragma once
+
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rvx {
+
+__device__ __forceinline__ void butterfly(float2& a, float2& b, const float2 w) {
+    const float vr = b.x * w.x - b.y * w.y;
+    const float vi = b.x * w.y + b.y * w.x;
+    const float sr = a.x + vr;
+    const float si = a.y + vi;
+    const float dr = a.x - vr;
+    const float di = a.y - vi;
+    a = make_float2(sr, si);
+    b = make_float2(dr, di);
+}
+
+__device__ __forceinline__ float2 cmul_exact(const float2 v, const float2 c) {
+    const float pr = v.x * c.x - v.y * c.y;
+    const float pi = v.x * c.y + v.y * c.x;
+    return make_float2(pr, pi);
+}
+
+template <int BITS>
+__host__ __device__ constexpr uint32_t bitrev(uint32_t v) {
+    uint32_t r = 0;
+    for (int b = 0; b < BITS; ++b) r |= ((v >> b) & 1u) << (BITS - 1 - b);
+    return r;
+}
+
+template <int LOGN, int LOGV>
+struct Fft {
+    static_assert(LOGV >= 1 && LOGV <= LOGN, "need 2 <= V <= N");
+    static constexpr int N = 1 << LOGN;
+    static constexpr int LOG_V = LOGV;
+    static constexpr int V = 1 << LOGV;
+    static constexpr int LOGT = LOGN - LOGV;
+    static constexpr int T = 1 << LOGT;  // threads per transform
+    static constexpr int PASSES = (LOGN + LOGV - 1) / LOGV;
+
+    static constexpr int stages_in(int p) {
+        return (p + 1) * LOGV <= LOGN ? LOGV : LOGN - p * LOGV;
+    }
+
+    // LDS position swizzle (a bijection within each aligned group of V).
+    __device__ __forceinline__ static uint32_t phys(uint32_t p) {
+        return p ^ ((p >> (LOGV + 2)) & (V - 1));
+    }
+
+    // Position of register r (= u * G + i) of thread t in pass P.
+    template <int P>
+    __device__ __forceinline__ static uint32_t pos(uint32_t t, int r) {
+        constexpr int S0 = P * LOGV;
+        constexpr int LS = stages_in(P);
+        constexpr int G = 1 << LS;
+        const uint32_t u = uint32_t(r) / G, i = uint32_t(r) % G;
+        const uint32_t g = t * (V / G) + u;
+        const uint32_t j = g & ((1u << S0) - 1u);
+        const uint32_t base = (g >> S0) << (S0 + LS);
+        return base + j + (i << S0);
+    }
+
+    // The butterflies of pass P on the thread's registers.
+    template <int P>
+    __device__ __forceinline__ static void run_pass(float2 (&v)[V], const float2* __restrict__ tw,
+                                                    uint32_t t) {
+        constexpr int S0 = P * LOGV;
+        constexpr int LS = stages_in(P);
+        constexpr int G = 1 << LS;
+#pragma unroll
+        for (int u = 0; u < V / G; ++u) {
+            const uint32_t g = t * (V / G) + u;
+            const uint32_t j = g & ((1u << S0) - 1u);
+#pragma unroll
+            for (int l = 0; l < LS; ++l) {
+                const int s = S0 + l;
+#pragma unroll
+                for (int i = 0; i < G; ++i) {
+                    if (i & (1 << l)) continue;
+                    const uint32_t widx = (j + (uint32_t(i & ((1 << l) - 1)) << S0)) *
+                                          uint32_t(N >> (s + 1));
+                    butterfly(v[u * G + i], v[u * G + i + (1 << l)], tw[widx]);
+                }
+            }
+        }
+    }
+
+    template <int P>
+    __device__ __forceinline__ static void store(float2* lds, const float2 (&v)[V], uint32_t t) {
+#pragma unroll
+        for (int r = 0; r < V; ++r) lds[phys(pos<P>(t, r))] = v[r];
+    }
+    template <int P>
+    __device__ __forceinline__ static void load(const float2* lds, float2 (&v)[V], uint32_t t) {
+#pragma unroll
+        for (int r = 0; r < V; ++r) v[r] = lds[phys(pos<P>(t, r))];
+    }
+
+    // Passes 1 .. PASSES-1, entering with pass 0 already done in registers and
+    // leaving the last pass's results in registers (natural order positions
+    // given by pos<PASSES-1>). Ends with the LDS free for reuse only after a
+    // barrier the caller places.
+    template <int P = 1>
+    __device__ __forceinline__ static void rest(float2* lds, float2 (&v)[V],
+                                                const float2* __restrict__ tw, uint32_t t) {
+        if constexpr (P < PASSES) {
+            store<P - 1>(lds, v, t);
+            __syncthreads();
+            load<P>(lds, v, t);
+            run_pass<P>(v, tw, t);
+            rest<P + 1>(lds, v, tw, t);
+        }
+    }
+
+    // Whole transform of data already in LDS at bit-reversed positions
+    // (lds[phys(i)] = x[bitrev(i)]); results left in registers.
+    __device__ __forceinline__ static void from_lds(float2* lds, float2 (&v)[V],
+                                                    const float2* __restrict__ tw, uint32_t t) {
+        load<0>(lds, v, t);
+        run_pass<0>(v, tw, t);
+        rest<1>(lds, v, tw, t);
+    }
+};
+
+}  // namespace rvx

@@ -1,0 +1,152 @@
+"""Device-side operators over torch CUDA tensors, through the C ABI.
+
+``Device`` owns one ``rvx_ctx`` bound to torch's current stream, so the
+library's launches are ordered with torch's own copies and allocations.
+Complex sample spans are float32 tensors with a trailing dimension of 2 (or
+complex64 viewed as such); every size the library checks comes from the
+tensors themselves.
+
+There is no fallback: without ``librvx_hip.so`` or without a GPU these raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from .dsp import GridParams
+
+
+def _p(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("device operators take CUDA tensors")
+    if not t.is_contiguous():
+        raise ValueError("device operators take contiguous tensors")
+    return t.data_ptr()
+
+
+def _csamples(t: torch.Tensor) -> int:
+    """Number of complex samples in a float32 (..., 2) / complex64 tensor."""
+    if t.dtype == torch.complex64:
+        return t.numel()
+    if t.dtype != torch.float32:
+        raise ValueError(f"complex span must be float32 pairs or complex64, got {t.dtype}")
+    return t.numel() // 2
+
+
+def _n(t: torch.Tensor) -> int:
+    return t.numel()
+
+
+class Device:
+    """One context (core/gpu/context.h Context) on a HIP device."""
+
+    def __init__(self, index: int = 0):
+        self.lib = _lib.lib()
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU: the receive chain runs only on the device")
+        self.index = index
+        ctx = C.c_void_p()
+        _lib.check(self.lib.rvx_ctx_create(index, C.byref(ctx)))
+        self.ctx = ctx
+        self.bind_stream()
+
+    def bind_stream(self, stream: torch.cuda.Stream | None = None) -> None:
+        s = stream if stream is not None else torch.cuda.current_stream(self.index)
+        _lib.check(self.lib.rvx_ctx_set_stream(self.ctx, C.c_void_p(s.cuda_stream)))
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.rvx_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self) -> None:
+        _lib.check(self.lib.rvx_stream_sync(self.ctx))
+
+    # -- operators ------------------------------------------------------
+    def convert(self, fmt: int, packed: torch.Tensor, params: _lib.ConvertParams,
+                ring: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_convert(self.ctx, fmt, _p(packed),
+                                        packed.numel() * packed.element_size(),
+                                        C.byref(params), _p(ring), _csamples(ring)))
+
+    def cmul(self, a: torch.Tensor, b: torch.Tensor, out: torch.Tensor) -> None:
+        n = _csamples(a)
+        if _csamples(b) != n or _csamples(out) != n:
+            raise ValueError("cmul spans differ in length")
+        _lib.check(self.lib.rvx_cmul(self.ctx, _p(a), _p(b), _p(out), n))
+
+    def pfb_branch(self, grid: GridParams, taps: torch.Tensor, ring: torch.Tensor,
+                   params: _lib.PfbBranchParams, out: torch.Tensor) -> None:
+        g = grid.c()
+        _lib.check(self.lib.rvx_pfb_branch(self.ctx, C.byref(g), _p(taps), _n(taps), _p(ring),
+                                           _csamples(ring), C.byref(params), _p(out),
+                                           _csamples(out)))
+
+    def pfb_fft(self, params: _lib.PfbFftParams, branch: torch.Tensor, twiddles: torch.Tensor,
+                ring: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_pfb_fft(self.ctx, C.byref(params), _p(branch),
+                                        _csamples(branch), _p(twiddles), _csamples(twiddles),
+                                        _p(ring), _csamples(ring)))
+
+    def fft_radix2(self, twiddles: torch.Tensor, x: torch.Tensor, out: torch.Tensor, size: int,
+                   batch: int) -> None:
+        if _csamples(x) < size * batch or _csamples(out) < size * batch:
+            raise ValueError("fft_radix2 spans shorter than size * batch")
+        _lib.check(self.lib.rvx_fft_radix2(self.ctx, _p(twiddles), _csamples(twiddles), _p(x),
+                                           _p(out), size, batch))
+
+    def spectrum(self, params: _lib.SpectrumParams, ring: torch.Tensor, twiddles: torch.Tensor,
+                 window: torch.Tensor, frame: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_spectrum(self.ctx, C.byref(params), _p(ring), _csamples(ring),
+                                         _p(twiddles), _csamples(twiddles), _p(window),
+                                         _n(window), _p(frame), _n(frame)))
+
+    def spectrum_levels(self, params: _lib.SpectrumLevelsParams, frame: torch.Tensor,
+                        levels: torch.Tensor) -> None:
+        if _n(levels) < 2:
+            raise ValueError("levels holds two floats")
+        _lib.check(self.lib.rvx_spectrum_levels(self.ctx, C.byref(params), _p(frame),
+                                                _n(frame), _p(levels)))
+
+    def ema_partial(self, params: _lib.EmaParams, ring: torch.Tensor,
+                    partial: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_ema_partial(self.ctx, C.byref(params), _p(ring),
+                                            _csamples(ring), _p(partial), _n(partial)))
+
+    def ema_combine(self, params: _lib.EmaParams, partial: torch.Tensor, state: torch.Tensor,
+                    rows: torch.Tensor | None = None) -> None:
+        if _n(state) < params.channels:
+            raise ValueError("state holds one float per channel")
+        _lib.check(self.lib.rvx_ema_combine(self.ctx, C.byref(params), _p(partial),
+                                            _n(partial), _p(state), _p(rows),
+                                            0 if rows is None else _n(rows)))
+
+    def chain(self, desc: _lib.ChainDesc, packed: torch.Tensor, taps: torch.Tensor,
+              twiddles: torch.Tensor, ring: torch.Tensor,
+              partial: torch.Tensor | None = None) -> None:
+        bps = 2 if desc.format == 0 else 4
+        if packed.numel() * packed.element_size() < desc.input_samples * bps:
+            raise ValueError("input tensor smaller than input_samples")
+        _lib.check(self.lib.rvx_chain(self.ctx, C.byref(desc), _p(packed), _p(taps), _n(taps),
+                                      _p(twiddles), _csamples(twiddles), _p(ring),
+                                      _csamples(ring), _p(partial),
+                                      0 if partial is None else _n(partial)))
+
+    # -- timing ---------------------------------------------------------
+    def timer_begin(self) -> None:
+        _lib.check(self.lib.rvx_timer_begin(self.ctx))
+
+    def timer_end(self) -> float:
+        ms = C.c_float()
+        _lib.check(self.lib.rvx_timer_end(self.ctx, C.byref(ms)))
+        return ms.value

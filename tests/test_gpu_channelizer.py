@@ -1,0 +1,188 @@
+"""GPU vs oracle, bit for bit: convert, branch filter, transform, fused chain.
+
+Mirrors tests/reference/test_convert.cpp, test_pfb.cpp (227-513) and
+test_cmul.cpp of the reference: seeded uniform input, adversarial values,
+several grids, and the device denormal probe (gpu_fixture.h:117-136).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from helpers import (SEED_BRANCH, SEED_CHAIN, SEED_FFT, adversarial, assert_bit_exact, from_dev,
+                     rng, to_dev, uniform_complex)
+from oracle import oracle
+from revenant_amd import _lib
+from revenant_amd.dsp import (FORMAT_CS16, FORMAT_CU8, GridParams, build_twiddles,
+                              design_prototype, ema_frame_decay)
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def test_device_flushes_denormals(dev):
+    """cmul with a product of 1e-40 must give exact zero (gpu_fixture.h:117-136)."""
+    a = np.array([1e-20, 1e-20, 3e-20, -2e-20], dtype=np.float32)
+    b = np.array([1e-20, 0.0, 1e-20, 1e-20], dtype=np.float32)
+    out = torch.empty(4, dtype=torch.float32, device="cuda")
+    dev.cmul(to_dev(a), to_dev(b), out)
+    got = from_dev(out)
+    assert np.all(got == 0.0), got
+    assert_bit_exact(got, oracle.cmul(a, b), what="cmul denormal probe")
+
+
+def test_cmul_matches_twin(dev):
+    a = adversarial(0x434D554C00000001, 4096)
+    b = uniform_complex(0x434D554C00000002, 4096, 3.0)
+    out = torch.empty(a.size, dtype=torch.float32, device="cuda")
+    dev.cmul(to_dev(a), to_dev(b), out)
+    assert_bit_exact(from_dev(out), oracle.cmul(a, b), what="cmul")
+
+
+@pytest.mark.parametrize("fmt,bps", [(0, 2), (1, 2), (2, 4), (3, 6)])
+@pytest.mark.parametrize("src_offset,count", [(0, 1000), (3, 4093), (17, 1)])
+def test_convert_matches_twin(dev, fmt, bps, src_offset, count):
+    g = rng(0x434F4E5600000000 + fmt)
+    nbytes = ((src_offset + count) * bps + 3) // 4 * 4
+    packed = g.integers(0, 256, size=nbytes, dtype=np.uint8)
+    cap = 4096
+    p = _lib.ConvertParams(cap - 1, 0xFFFFFF00, src_offset, count)  # wraps the ring
+    ring_cpu = np.full(2 * cap, 7.0, dtype=np.float32)
+    oracle.convert(fmt, packed, p, ring_cpu)
+    ring = to_dev(np.full(2 * cap, 7.0, dtype=np.float32))
+    dev.convert(fmt, to_dev(packed), p, ring)
+    assert_bit_exact(from_dev(ring), ring_cpu, what=f"convert fmt {fmt}")
+
+
+def test_convert_every_cu8_code(dev):
+    codes = np.arange(256, dtype=np.uint8)
+    packed = np.stack([codes, codes[::-1]], axis=1).ravel()
+    p = _lib.ConvertParams(255, 0, 0, 256)
+    ring = torch.zeros(512, dtype=torch.float32, device="cuda")
+    dev.convert(FORMAT_CU8, to_dev(packed), p, ring)
+    cpu = np.zeros(512, dtype=np.float32)
+    oracle.convert(FORMAT_CU8, packed, p, cpu)
+    assert_bit_exact(from_dev(ring), cpu, what="cu8 codes")
+
+
+def test_convert_refuses_short_source(dev):
+    p = _lib.ConvertParams(1023, 0, 0, 3)
+    ring = torch.zeros(2048, dtype=torch.float32, device="cuda")
+    with pytest.raises(_lib.RvxError):
+        dev.convert(FORMAT_CU8, to_dev(np.zeros(6, dtype=np.uint8)), p, ring)
+
+
+def _ring_for(grid: GridParams, blocks: int, seed: int, adv: bool):
+    need = grid.prototype_length + (blocks - 1) * grid.decimation
+    cap = 1 << (need - 1).bit_length()
+    data = adversarial(seed, cap) if adv else uniform_complex(seed, cap)
+    return cap, data
+
+
+GRIDS = [GridParams(8, 4, 8), GridParams(8, 17, 4), GridParams(64, 17, 32),
+         GridParams(64, 16, 64), GridParams(256, 16, 128), GridParams(4096, 16, 2048),
+         GridParams(2048, 17, 1024)]
+
+
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: f"M{g.channels}L{g.taps_per_branch}D{g.decimation}")
+@pytest.mark.parametrize("adv", [False, True], ids=["uniform", "adversarial"])
+def test_branch_matches_twin(dev, grid, adv):
+    blocks = 24
+    cap, ring = _ring_for(grid, blocks, SEED_BRANCH, adv)
+    taps = design_prototype(grid).taps
+    p = _lib.PfbBranchParams(cap - 1, 0xFFFFFFF0 if grid.channels < 1024 else 12345, blocks)
+    cpu = oracle.pfb_branch(grid.c(), taps, ring, p)
+    out = torch.empty(2 * blocks * grid.channels, dtype=torch.float32, device="cuda")
+    dev.pfb_branch(grid, to_dev(taps), to_dev(ring), p, out)
+    assert_bit_exact(from_dev(out), cpu, SEED_BRANCH, f"branch {grid}")
+
+
+@pytest.mark.parametrize("M,D", [(2, 2), (4, 2), (8, 8), (16, 8), (64, 32), (64, 64),
+                                 (512, 256), (4096, 2048), (4096, 4096), (16384, 8192)])
+@pytest.mark.parametrize("adv", [False, True], ids=["uniform", "adversarial"])
+def test_fft_matches_twin(dev, M, D, adv):
+    blocks = 13
+    tw = build_twiddles(M)
+    branch = adversarial(SEED_FFT, blocks * M) if adv else uniform_complex(SEED_FFT, blocks * M)
+    ring_blocks = 16
+    p = _lib.PfbFftParams(M, D, M.bit_length() - 1, 0xFFFFFFFB, blocks, ring_blocks,
+                          ring_blocks - 1)
+    cpu = np.full(2 * M * ring_blocks, 5.0, dtype=np.float32)
+    oracle.pfb_fft(p, branch, tw, cpu)
+    out = to_dev(np.full(2 * M * ring_blocks, 5.0, dtype=np.float32))
+    dev.pfb_fft(p, to_dev(branch), to_dev(tw.view(np.float32)), out)
+    assert_bit_exact(from_dev(out), cpu, SEED_FFT, f"pfb_fft M={M} D={D}")
+
+
+@pytest.mark.parametrize("size", [2, 4, 8, 16, 32, 64, 128, 1024, 2048, 4096, 8192, 16384])
+def test_fft_radix2_matches_twin(dev, size):
+    batch = 5
+    tw = build_twiddles(size)
+    x = uniform_complex(SEED_FFT + size, batch * size)
+    cpu = oracle.fft_radix2(tw, x, size, batch)
+    out = torch.empty(2 * batch * size, dtype=torch.float32, device="cuda")
+    dev.fft_radix2(to_dev(tw.view(np.float32)), to_dev(x), out, size, batch)
+    assert_bit_exact(from_dev(out), cpu, SEED_FFT, f"fft_radix2 {size}")
+
+
+def _codes(fmt: int, n: int, seed: int) -> np.ndarray:
+    g = rng(seed)
+    if fmt == FORMAT_CU8:
+        return g.integers(0, 256, size=2 * n, dtype=np.uint8)
+    return g.integers(-32768, 32768, size=2 * n, dtype=np.int16).view(np.uint8)
+
+
+CHAINS = [
+    # (format, grid, blocks, frame) -- the C1 grid, the C2 grid, and odd shapes
+    (FORMAT_CU8, GridParams(64, 16, 64), 256, 16),
+    (FORMAT_CU8, GridParams(64, 16, 32), 96, 24),
+    (FORMAT_CS16, GridParams(4096, 16, 2048), 64, 16),
+    (FORMAT_CS16, GridParams(4096, 17, 2048), 32, 8),
+    (FORMAT_CS16, GridParams(1024, 16, 512), 48, 0),
+    (FORMAT_CS16, GridParams(8, 5, 4), 40, 10),
+    (FORMAT_CU8, GridParams(256, 12, 256), 32, 4),
+]
+
+
+@pytest.mark.parametrize("fmt,grid,blocks,frame", CHAINS,
+                         ids=lambda v: str(v) if not isinstance(v, GridParams) else
+                         f"M{v.channels}L{v.taps_per_branch}D{v.decimation}")
+def test_chain_matches_composed_twins(dev, fmt, grid, blocks, frame):
+    """The fused launch equals convert -> branch -> fft -> ema_partial."""
+    M, D, N = grid.channels, grid.decimation, grid.prototype_length
+    first_block = (N + D - 1) // D + 3
+    samples = (first_block + blocks) * D + 7
+    packed = _codes(fmt, samples, SEED_CHAIN + M)
+    taps = design_prototype(grid, beta=8.0).taps
+    tw = build_twiddles(M)
+    ring_blocks = 1 << (blocks - 1).bit_length()
+    alpha = np.float32(1.0 / 16.0)
+    od = oracle.OChain(fmt, M, grid.taps_per_branch, D, first_block, blocks, ring_blocks, frame,
+                       alpha, ema_frame_decay(alpha, frame) if frame else 0.0)
+    ring_cpu, partial_cpu = oracle.chain(od, packed, taps, tw, threads=4)
+    desc = _lib.ChainDesc(fmt, M, grid.taps_per_branch, D, 0, samples, first_block, blocks,
+                          ring_blocks, frame, alpha)
+    ring = torch.full((2 * M * ring_blocks,), 3.0, dtype=torch.float32, device="cuda")
+    partial = (torch.zeros((blocks // frame) * M, dtype=torch.float32, device="cuda")
+               if frame else None)
+    dev.chain(desc, to_dev(packed), to_dev(taps), to_dev(tw.view(np.float32)), ring, partial)
+    got = from_dev(ring)
+    # Slots the launch does not write keep the fill in both (oracle starts at 0).
+    written = np.zeros((M, ring_blocks), dtype=bool)
+    for b in range(blocks):
+        written[:, (first_block + b) & (ring_blocks - 1)] = True
+    mask = np.repeat(written.ravel(), 2)
+    assert_bit_exact(got[mask], ring_cpu[mask], SEED_CHAIN, f"chain ring {grid}")
+    if frame:
+        assert_bit_exact(from_dev(partial), partial_cpu, SEED_CHAIN, f"chain ema {grid}")
+
+
+def test_chain_refuses_missing_history(dev):
+    grid = GridParams(64, 16, 64)
+    desc = _lib.ChainDesc(FORMAT_CU8, 64, 16, 64, 0, 10000, 2, 8, 8, 0, 0.0625)
+    ring = torch.zeros(2 * 64 * 8, dtype=torch.float32, device="cuda")
+    taps = to_dev(design_prototype(grid).taps)
+    tw = to_dev(build_twiddles(64).view(np.float32))
+    with pytest.raises(_lib.RvxError):
+        dev.chain(desc, to_dev(np.zeros(20000, dtype=np.uint8)), taps, tw, ring)
