@@ -234,6 +234,71 @@ int32_t rvx_nco_delta(int64_t hz_numerator, int64_t hz_denominator, int64_t rate
                       uint64_t* out);
 
 /* ------------------------------------------------------------------------ */
+/* Receiver planning (core/engine/vrx_place.cpp, core/dsp/vrx_reference.cpp) */
+/* ------------------------------------------------------------------------ */
+
+/* core/engine/vrx.h VrxParams, the part planning reads. */
+typedef struct rvx_vrx_request {
+    int64_t center_hz;     /* relative to the source's tuned centre */
+    int64_t bandwidth_hz;  /* 0: the mode's default passband */
+    int64_t passband_low;  /* both 0: derived from bandwidth_hz */
+    int64_t passband_high;
+    uint32_t mode;         /* RVX_DEMOD_* */
+    uint32_t reserved;
+    int64_t audio_rate;    /* 0: 48000 */
+    int64_t cw_pitch;
+} rvx_vrx_request;
+
+/* core/engine/vrx.h:746-783 VrxPlacement */
+typedef struct rvx_vrx_placement {
+    uint32_t channel;
+    uint32_t bandwidth_clamped;
+    int64_t centre_num, centre_den;     /* channel centre, reduced rational Hz */
+    int64_t residual_num, residual_den; /* receiver centre minus channel centre */
+    int64_t channel_rate;
+    int64_t granted_low, granted_high;
+} rvx_vrx_placement;
+
+/* core/dsp/vrx_reference.h:1334-1476 VrxPlan, the part the kernels use. */
+typedef struct rvx_vrx_plan {
+    rvx_vrx_placement placement;
+    uint32_t mode;
+    uint32_t audio_decimation_taps;
+    int64_t channel_rate, audio_rate, demod_rate, output_rate;
+    int64_t passband_low, passband_high, bandwidth, deviation;
+    int64_t filter_num, filter_den, mix_num, mix_den;
+    rvx_vrx_fine_config fine;
+    uint32_t reserved;
+    uint64_t fine_nco_delta;
+    rvx_vrx_demod_config demod;
+    float demod_gain;
+    double audio_pass_hz, audio_stop_hz;
+} rvx_vrx_plan;
+
+/* core/engine/vrx_place.cpp:216 place: nearest channel round(center*M/rate),
+ * exact rational residual, refusal when the channel cannot carry the band. */
+int32_t rvx_place(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* request,
+                  rvx_vrx_placement* out);
+/* core/dsp/vrx_reference.cpp:2568 plan_vrx (AM, NFM, WFM mono, USB, LSB,
+ * DSB, CW, raw; no de-emphasis, stereo or carrier loop). */
+int32_t rvx_plan_vrx(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* request,
+                     const rvx_vrx_placement* placement, rvx_vrx_plan* out);
+/* The plan's tables: fine taps (2*(phases*taps+1) floats, design_fine_taps
+ * :1808) and demod weights (audio_taps + dc_taps floats, design_audio_taps
+ * :1921 and design_dc_weights :1983). Either pointer may be NULL. */
+int32_t rvx_plan_vrx_tables(const rvx_vrx_plan* plan, float* fine_taps, uint64_t n_fine_floats,
+                            float* weights, uint64_t n_weights);
+/* core/dsp/vrx_reference.cpp:2855 fine_block and :2873 demod_block: one
+ * dispatch's params from the absolute index of its first output. */
+int32_t rvx_fine_block(const rvx_vrx_plan* plan, uint32_t channel_base, uint32_t channel_mask,
+                       uint32_t fine_mask, uint64_t first_output, uint32_t count,
+                       rvx_vrx_fine_params* out, uint64_t* first_input);
+int32_t rvx_demod_block(const rvx_vrx_plan* plan, uint32_t fine_mask, uint64_t first_audio,
+                        uint32_t count, rvx_vrx_demod_params* out);
+/* core/dsp/vrx_reference.h:555 demod_fine_history. */
+uint32_t rvx_demod_history(const rvx_vrx_demod_config* config);
+
+/* ------------------------------------------------------------------------ */
 /* Operators. Each replaces one reference compute shader + its dispatch.     */
 /* ------------------------------------------------------------------------ */
 
@@ -284,6 +349,23 @@ int32_t rvx_ema_partial(rvx_ctx* ctx, const rvx_ema_params* params, const void* 
                         uint64_t ring_samples, float* partial, uint64_t n_partial);
 int32_t rvx_ema_combine(rvx_ctx* ctx, const rvx_ema_params* params, const float* partial,
                         uint64_t n_partial, float* state, float* rows, uint64_t n_rows);
+
+/* core/shaders/vrx_fine.comp; twin core/dsp/vrx_reference.cpp:489. The
+ * stateless rational resampler + complex FIR + 0.64 NCO rotation. channel
+ * ring: reads chan_base + ((in_offset + n - k) & chan_mask). taps: phases *
+ * taps + 1 cf32. nco: 2^nco_log2 cf32. fine ring: out_mask + 1 cf32. */
+int32_t rvx_vrx_fine(rvx_ctx* ctx, const rvx_vrx_fine_config* config,
+                     const rvx_vrx_fine_params* params, const void* channel_ring,
+                     uint64_t ring_samples, const void* taps, uint64_t n_taps, const void* nco,
+                     uint64_t n_nco, void* fine_ring, uint64_t fine_samples);
+
+/* core/shaders/vrx_demod.comp; twin core/dsp/vrx_reference.cpp:714. weights:
+ * audio_taps + dc_taps (+ 2*pilot_taps + 2*audio_taps for FM stereo) floats.
+ * audio: count * channels floats. */
+int32_t rvx_vrx_demod(rvx_ctx* ctx, const rvx_vrx_demod_config* config,
+                      const rvx_vrx_demod_params* params, const void* fine_ring,
+                      uint64_t fine_samples, const float* weights, uint64_t n_weights,
+                      float* audio, uint64_t n_audio);
 
 /* The fused plain chain: convert -> pfb_branch -> pfb_fft (-> ema_partial) in
  * one launch, bit-identical to running the four operators in sequence

@@ -572,6 +572,203 @@ int oracle_ema_combine(const OEmaParams* p, const float* partial, uint64_t npart
 }
 
 // ---------------------------------------------------------------------------
+// Receiver fine stage, core/dsp/vrx_reference.cpp:429-620 (vrx_fine.comp).
+// Stateless rational resampler with a T-tap complex polyphase FIR whose taps
+// are lerped between adjacent phases, then a rotation by the NCO table entry
+// the 0.64 phase's top nco_log2 bits select. Every index is 32-bit and wraps.
+// Pinned by: blocking invariance (any split of a run gives the same bits),
+// a unit tone at the centre coming out at unit level, and the FP64 resampler
+// definition (tests/test_oracle_vrx.py).
+// ---------------------------------------------------------------------------
+
+static int fine_valid(const OFineConfig* c, const OFineParams* p) {
+    if (c->taps < 2 || c->taps > 256 || c->phases < 1 || c->phases > 4096 || c->nco_log2 > 24)
+        return -1;
+    if (p->out_rate == 0 || p->frac0 >= p->out_rate) return -1;
+    const uint64_t cap = uint64_t(p->chan_mask) + 1, ocap = uint64_t(p->out_mask) + 1;
+    if ((cap & p->chan_mask) != 0 || (ocap & p->out_mask) != 0) return -1;
+    if (uint64_t(p->frac0) + uint64_t(p->count) * p->step_rem > 0xFFFFFFFFull) return -1;
+    if (uint64_t(p->out_rate - 1) * c->phases > 0xFFFFFFFFull) return -1;
+    return 0;
+}
+
+int oracle_vrx_fine(const OFineConfig* c, const OFineParams* p, const float* chan,
+                    uint64_t chan_samples, const float* taps, uint64_t ntaps, const float* nco,
+                    uint64_t nnco, float* out, uint64_t out_samples) {
+    if (fine_valid(c, p)) return -1;
+    if (ntaps != uint64_t(c->phases) * c->taps + 1) return -1;
+    if (nnco != (uint64_t(1) << c->nco_log2)) return -1;
+    if (chan_samples < uint64_t(p->chan_base) + uint64_t(p->chan_mask) + 1) return -1;
+    if (out_samples < uint64_t(p->out_mask) + 1 || p->count > uint64_t(p->out_mask) + 1) return -1;
+    FlushGuard guard;
+    for (uint32_t i = 0; i < p->count; ++i) {
+        const uint32_t acc_idx = p->frac0 + i * p->step_rem;
+        const uint32_t carry = acc_idx / p->out_rate;
+        const uint32_t frac = acc_idx - carry * p->out_rate;
+        const uint32_t n_rel = i * p->step_whole + carry;
+        const uint32_t scaled = frac * c->phases;
+        const uint32_t phase = scaled / p->out_rate;
+        const uint32_t blend_num = scaled - phase * p->out_rate;
+        const float blend = float(blend_num) * p->inv_out_rate;
+        const float comp = 1.0f - blend;
+        const uint32_t base = p->in_offset + n_rel;
+        float ar = 0.0f, ai = 0.0f;
+        for (uint32_t k = 0; k < c->taps; ++k) {
+            const uint64_t slot = phase + uint64_t(k) * c->phases;
+            const float g0r = taps[2 * slot], g0i = taps[2 * slot + 1];
+            const float g1r = taps[2 * slot + 2], g1i = taps[2 * slot + 3];
+            const float tr = g0r * comp + g1r * blend;
+            const float ti = g0i * comp + g1i * blend;
+            const uint64_t idx = uint64_t(p->chan_base) + ((base - k) & p->chan_mask);
+            const float sr = chan[2 * idx], si = chan[2 * idx + 1];
+            ar = ar + (sr * tr - si * ti);
+            ai = ai + (sr * ti + si * tr);
+        }
+        const uint64_t prod = uint64_t(i) * p->nco_delta_low;
+        const uint32_t ph_lo = p->nco_phase_low + uint32_t(prod);
+        const uint32_t carry_bit = ph_lo < p->nco_phase_low ? 1u : 0u;
+        const uint32_t ph_hi =
+            p->nco_phase_high + i * p->nco_delta_high + uint32_t(prod >> 32) + carry_bit;
+        const uint32_t ni = c->nco_log2 == 0 ? 0u : (ph_hi >> (32 - c->nco_log2));
+        const float rr = nco[2 * ni], ri = nco[2 * ni + 1];
+        const float orr = ar * rr - ai * ri;
+        const float oi = ar * ri + ai * rr;
+        const uint64_t o = (p->out_offset + i) & p->out_mask;
+        out[2 * o] = orr;
+        out[2 * o + 1] = oi;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Receiver demodulator, core/dsp/vrx_reference.cpp:621-905 (vrx_demod.comp):
+// detect() per mode, then the decimating audio FIR sum_t detect(idx - t) * w[t]
+// in ascending t, times the gain. FM stereo (channels 2 with pilot taps)
+// follows the reference's pilot / rotation / matrix order.
+// Pinned by: tone-modulated carriers recovered at the stated level for AM,
+// NFM/WFM, USB/LSB/CW, and detect() composed from the det_* functions pinned
+// against libm (tests/test_oracle_vrx.py).
+// ---------------------------------------------------------------------------
+
+static bool complex_tap(uint32_t mode) { return mode == 0 || (mode >= 8 && mode <= 11); }
+
+static int demod_valid(const ODemodConfig* c, const ODemodParams* p) {
+    if (c->mode > 12) return -1;
+    const bool cx = complex_tap(c->mode);
+    if (c->decimation == 0 || c->audio_taps == 0 || c->audio_taps > 4096) return -1;
+    if (c->dc_taps == 0 || c->dc_taps > 4096) return -1;
+    if (cx && (c->decimation != 1 || c->audio_taps != 1)) return -1;
+    if (c->channels != 1 && c->channels != 2) return -1;
+    if (cx && c->channels != 2) return -1;
+    if (c->pilot_taps > 2048) return -1;
+    if (c->pilot_taps != 0 && (c->mode != 3 || c->channels != 2)) return -1;
+    if (c->channels == 2 && !cx && c->pilot_taps == 0) return -1;
+    const uint64_t cap = uint64_t(p->in_mask) + 1;
+    if ((cap & p->in_mask) != 0) return -1;
+    const uint32_t det = (c->mode == 1 || c->mode == 12) ? c->dc_taps - 1
+                         : (c->mode == 2 || c->mode == 3) ? 1u : 0u;
+    const uint32_t audio_reach = c->audio_taps - 1;
+    const uint32_t pilot_reach = c->pilot_taps == 0 ? 0u : c->audio_taps / 2 + c->pilot_taps - 1;
+    const uint64_t reach = uint64_t(std::max(audio_reach, pilot_reach)) + det;
+    const uint64_t span = p->count == 0 ? 0 : uint64_t(p->count - 1) * c->decimation + reach + 1;
+    if (span > cap) return -1;
+    return 0;
+}
+
+int oracle_vrx_demod(const ODemodConfig* c, const ODemodParams* p, const float* fine,
+                     uint64_t fine_samples, const float* w, uint64_t nw, float* audio,
+                     uint64_t naudio) {
+    if (demod_valid(c, p)) return -1;
+    const uint64_t pilot_base = uint64_t(c->audio_taps) + c->dc_taps;
+    const uint64_t rot_base = pilot_base + 2ull * c->pilot_taps;
+    const uint64_t want = c->pilot_taps == 0 ? pilot_base : rot_base + 2ull * c->audio_taps;
+    if (nw != want) return -1;
+    if (fine_samples < uint64_t(p->in_mask) + 1) return -1;
+    if (naudio != uint64_t(p->count) * c->channels) return -1;
+    if (p->count == 0) return 0;
+    FlushGuard guard;
+    auto re = [&](uint32_t i) { return fine[2 * uint64_t(i & p->in_mask)]; };
+    auto im = [&](uint32_t i) { return fine[2 * uint64_t(i & p->in_mask) + 1]; };
+    auto mag = [&](uint32_t i) {
+        const float a = re(i), b = im(i);
+        const float sq = a * a + b * b;
+        return det_sqrt(sq);
+    };
+    auto detect = [&](uint32_t i) -> float {
+        if (c->mode == 1) {
+            float dc = 0.0f;
+            for (uint32_t k = 0; k < c->dc_taps; ++k) {
+                const float term = mag(i - k) * w[c->audio_taps + k];
+                dc = dc + term;
+            }
+            return mag(i) - dc;
+        }
+        if (c->mode == 12) {
+            float dc = 0.0f;
+            for (uint32_t k = 0; k < c->dc_taps; ++k) {
+                const float term = re(i - k) * w[c->audio_taps + k];
+                dc = dc + term;
+            }
+            return re(i) - dc;
+        }
+        if (c->mode == 2 || c->mode == 3) {
+            const float zr = re(i), zi = im(i), pr = re(i - 1), pi = im(i - 1);
+            const float prod_r = zr * pr + zi * pi;
+            const float prod_i = zi * pr - zr * pi;
+            return det_atan2(prod_i, prod_r);
+        }
+        return re(i);
+    };
+    const float kPilotLock = 2.0e-5f;
+    for (uint32_t i = 0; i < p->count; ++i) {
+        const uint32_t idx = p->in_offset + i * c->decimation;
+        if (complex_tap(c->mode)) {
+            audio[2 * uint64_t(i)] = re(idx) * p->gain;
+            audio[2 * uint64_t(i) + 1] = im(idx) * p->gain;
+            continue;
+        }
+        if (c->channels == 2) {
+            const uint32_t at = idx - c->audio_taps / 2;
+            float pr = 0.0f, pi = 0.0f;
+            for (uint32_t k = 0; k < c->pilot_taps; ++k) {
+                const float comp = detect(at - k);
+                const float tr = comp * w[pilot_base + 2 * k];
+                const float ti = comp * w[pilot_base + 2 * k + 1];
+                pr = pr + tr;
+                pi = pi + ti;
+            }
+            const float sr = pr * p->gain, si = pi * p->gain;
+            const float rsq = sr * sr, isq = si * si;
+            const float power = rsq + isq;
+            const float inv = det_recip(std::max(power, kPilotLock));
+            const float cross = sr * si;
+            const float cos2 = (isq - rsq) * inv;
+            const float sin2 = (-2.0f * cross) * inv;
+            const float locked = power >= kPilotLock ? 1.0f : 0.0f;
+            float sum = 0.0f, diff = 0.0f;
+            for (uint32_t t = 0; t < c->audio_taps; ++t) {
+                const float weighted = detect(idx - t) * w[t];
+                sum = sum + weighted;
+                const float ref = sin2 * w[rot_base + 2 * t] - cos2 * w[rot_base + 2 * t + 1];
+                const float term = weighted * ref;
+                diff = diff + term;
+            }
+            const float gated = diff * locked;
+            audio[2 * uint64_t(i)] = (sum + gated) * p->gain;
+            audio[2 * uint64_t(i) + 1] = (sum - gated) * p->gain;
+            continue;
+        }
+        float acc = 0.0f;
+        for (uint32_t t = 0; t < c->audio_taps; ++t) {
+            const float term = detect(idx - t) * w[t];
+            acc = acc + term;
+        }
+        audio[i] = acc * p->gain;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // The plain chain as a composition of the twins above: convert the input
 // codes into a ring, branch filter, transform + correction into a channel
 // ring, exponentially averaged power per frame. Input index 0 is absolute

@@ -81,6 +81,28 @@ class ChainDesc(C.Structure):
                 ("frame_blocks", u32), ("alpha", f32)]
 
 
+class VrxRequest(C.Structure):
+    _fields_ = [("center_hz", i64), ("bandwidth_hz", i64), ("passband_low", i64),
+                ("passband_high", i64), ("mode", u32), ("reserved", u32), ("audio_rate", i64),
+                ("cw_pitch", i64)]
+
+
+class VrxPlacement(C.Structure):
+    _fields_ = [("channel", u32), ("bandwidth_clamped", u32), ("centre_num", i64),
+                ("centre_den", i64), ("residual_num", i64), ("residual_den", i64),
+                ("channel_rate", i64), ("granted_low", i64), ("granted_high", i64)]
+
+
+class VrxPlan(C.Structure):
+    _fields_ = [("placement", VrxPlacement), ("mode", u32), ("audio_decimation_taps", u32),
+                ("channel_rate", i64), ("audio_rate", i64), ("demod_rate", i64),
+                ("output_rate", i64), ("passband_low", i64), ("passband_high", i64),
+                ("bandwidth", i64), ("deviation", i64), ("filter_num", i64), ("filter_den", i64),
+                ("mix_num", i64), ("mix_den", i64), ("fine", VrxFineConfig), ("reserved", u32),
+                ("fine_nco_delta", u64), ("demod", VrxDemodConfig), ("demod_gain", f32),
+                ("audio_pass_hz", f64), ("audio_stop_hz", f64)]
+
+
 assert C.sizeof(VrxFineParams) == 60 and C.sizeof(VrxDemodParams) == 16
 assert C.sizeof(ConvertParams) == 16 and C.sizeof(PfbBranchParams) == 12
 
@@ -116,6 +138,18 @@ _SIGNATURES = {
     "rvx_spectrum_levels": (i32, [vp, C.POINTER(SpectrumLevelsParams), vp, u64, vp]),
     "rvx_ema_partial": (i32, [vp, C.POINTER(EmaParams), vp, u64, vp, u64]),
     "rvx_ema_combine": (i32, [vp, C.POINTER(EmaParams), vp, u64, vp, vp, u64]),
+    "rvx_vrx_fine": (i32, [vp, C.POINTER(VrxFineConfig), C.POINTER(VrxFineParams), vp, u64, vp,
+                           u64, vp, u64, vp, u64]),
+    "rvx_vrx_demod": (i32, [vp, C.POINTER(VrxDemodConfig), C.POINTER(VrxDemodParams), vp, u64,
+                            vp, u64, vp, u64]),
+    "rvx_place": (i32, [C.POINTER(Grid), i64, C.POINTER(VrxRequest), C.POINTER(VrxPlacement)]),
+    "rvx_plan_vrx": (i32, [C.POINTER(Grid), i64, C.POINTER(VrxRequest), C.POINTER(VrxPlacement),
+                           C.POINTER(VrxPlan)]),
+    "rvx_plan_vrx_tables": (i32, [C.POINTER(VrxPlan), fptr, u64, fptr, u64]),
+    "rvx_fine_block": (i32, [C.POINTER(VrxPlan), u32, u32, u32, u64, u32,
+                             C.POINTER(VrxFineParams), C.POINTER(u64)]),
+    "rvx_demod_block": (i32, [C.POINTER(VrxPlan), u32, u64, u32, C.POINTER(VrxDemodParams)]),
+    "rvx_demod_history": (u32, [C.POINTER(VrxDemodConfig)]),
     "rvx_chain": (i32, [vp, C.POINTER(ChainDesc), vp, vp, u64, vp, u64, vp, u64, vp, u64]),
 }
 

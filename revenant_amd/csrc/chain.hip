@@ -40,6 +40,7 @@ struct ChainArgs {
     float* partial;
     uint64_t first_block;
     uint64_t input_origin;
+    uint64_t input_samples;
     uint32_t decimation;
     uint32_t taps_per_branch;
     uint32_t block_count;
@@ -51,38 +52,80 @@ struct ChainArgs {
     float alpha;
 };
 
-template <int LOGM>
+template <int LOGM, int LOGV_ = 2>
 struct ChainShape {
-    static constexpr int LOGV = LOGM < 2 ? LOGM : 2;
+    static constexpr int LOGV = LOGM < LOGV_ ? LOGM : LOGV_;
     using F = Fft<LOGM, LOGV>;
     static constexpr int THREADS = F::T >= 256 ? F::T : 256;
     static constexpr int BPW = THREADS / F::T;  // blocks in flight per workgroup
 };
 
+// The native code of one sample (cs16: one word; cu8: two bytes).
 template <int FORMAT>
-__device__ __forceinline__ float2 load_sample(const void* input, uint32_t index) {
+__device__ __forceinline__ uint32_t load_code(const void* input, uint32_t index) {
+    if constexpr (FORMAT == RVX_FORMAT_CS16)
+        return static_cast<const uint32_t*>(input)[index];
+    else
+        return static_cast<const uint16_t*>(input)[index];
+}
+
+// convert_cs16_cf32.comp / convert_cu8_cf32.comp on one code.
+template <int FORMAT>
+__device__ __forceinline__ float2 convert_code(uint32_t w) {
     if constexpr (FORMAT == RVX_FORMAT_CS16) {
-        const uint32_t w = static_cast<const uint32_t*>(input)[index];
         const float scale = __uint_as_float(0x38000000u);
         const float i = float(int32_t(w << 16) >> 16) * scale;
         const float q = float(int32_t(w) >> 16) * scale;
         return make_float2(i, q);
     } else {
-        const uint32_t w = static_cast<const uint16_t*>(input)[index];
         const float bias = __uint_as_float(0x42FF0000u);
         const float scale = __uint_as_float(0x3C008081u);
         const float i = (float(w & 0xFFu) - bias) * scale;
-        const float q = (float(w >> 8) - bias) * scale;
+        const float q = (float((w >> 8) & 0xFFu) - bias) * scale;
         return make_float2(i, q);
     }
 }
 
-// TAPS > 0: the tap count is a compile-time constant and taps live in
-// registers; TAPS == 0: any tap count, taps read from memory each block.
-template <int LOGM, int FORMAT, int TAPS>
-__global__ void __launch_bounds__(ChainShape<LOGM>::THREADS)
+// Buffer loads: a uniform base and record count in a resource, a 32-bit
+// per-lane offset and a uniform offset per tap. The tap's stride rides in
+// the uniform offset, so a thread needs one offset register per branch
+// instead of one address per (branch, tap), and reads past the resource
+// return zero instead of faulting.
+constexpr int kBufferDword3 = 0x00020000;  // raw dword access on gfx9-class parts
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint64_t bytes) {
+    const uint32_t records = bytes > 0xFFFFFFFFull ? 0xFFFFFFFFu : uint32_t(bytes);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, records, kBufferDword3);
+}
+
+template <int FORMAT>
+__device__ __forceinline__ uint32_t buffer_code(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                uint32_t soff) {
+    if constexpr (FORMAT == RVX_FORMAT_CS16)
+        return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+    else
+        return __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+}
+
+template <int FORMAT>
+__device__ __forceinline__ float2 load_sample(const void* input, uint32_t index) {
+    return convert_code<FORMAT>(load_code<FORMAT>(input, index));
+}
+
+// Taps whose sample loads are issued ahead of their use. The tap loop and
+// the transform between blocks run while PREFETCH rows of codes are in
+// flight, so an L2 round trip is paid once per PREFETCH taps rather than
+// once per tap.
+constexpr int kPrefetch = 4;
+
+// TAPS > 0: the tap count is a compile-time constant; REGTAPS then keeps the
+// thread's V*TAPS taps in registers for the whole chunk, otherwise they are
+// read (L1/L2-resident) alongside the samples, through the same prefetch.
+// TAPS == 0: any tap count, read from memory tap by tap.
+template <int LOGM, int LOGV_, int FORMAT, int TAPS, bool REGTAPS>
+__global__ void __launch_bounds__((ChainShape<LOGM, LOGV_>::THREADS))
 chain_kernel(ChainArgs a) {
-    using S = ChainShape<LOGM>;
+    using S = ChainShape<LOGM, LOGV_>;
     using F = typename S::F;
     constexpr uint32_t M = F::N;
     constexpr int V = F::V;
@@ -117,8 +160,8 @@ chain_kernel(ChainArgs a) {
 #pragma unroll
     for (int i = 0; i < V; ++i) branch[i] = (M - (t + uint32_t(T) * i)) & (M - 1);
 
-    float c[V][NT];
-    if constexpr (TAPS > 0) {
+    float c[V][REGTAPS ? NT : 1];
+    if constexpr (TAPS > 0 && REGTAPS) {
 #pragma unroll
         for (int i = 0; i < V; ++i)
 #pragma unroll
@@ -130,11 +173,52 @@ chain_kernel(ChainArgs a) {
     for (int r = 0; r < V; ++r) ema[r] = 0.0f;
 
     const uint32_t first = chunk * a.chunk;
+    constexpr int PF = (TAPS > 0 && TAPS < kPrefetch) ? TAPS : kPrefetch;
+    auto block_base = [&](uint32_t b0) {
+        const uint64_t m = a.first_block + first + b0 + sub;
+        return uint32_t(m * a.decimation - a.input_origin);
+    };
+    auto block_active = [&](uint32_t b0) {
+        return (b0 + sub < a.chunk) && (first + b0 + sub < a.block_count);
+    };
+    // Codes of the first PF taps of the next block, loaded while the current
+    // block's transform runs.
+    uint32_t raw[PF][V];
+    float ctap[REGTAPS ? 1 : PF][V];
+    // The chunk's input window starts at the oldest sample its first block
+    // reads; a sample's byte offset from there splits into a per-lane part
+    // (block base minus branch, minus the oldest tap) and a per-tap uniform
+    // part ((TAPS-1-q) rows of M samples).
+    constexpr uint32_t BPS = FORMAT == RVX_FORMAT_CS16 ? 4u : 2u;
+    constexpr uint32_t NTAPS = TAPS > 0 ? TAPS : 1;
+    const uint32_t window0 =
+        uint32_t((a.first_block + first) * a.decimation - a.input_origin) - (NTAPS * M - 1);
+    const __amdgpu_buffer_rsrc_t in_rsrc = make_rsrc(
+        static_cast<const uint8_t*>(a.input) + uint64_t(window0) * BPS,
+        (a.input_samples - window0) * BPS);
+    const __amdgpu_buffer_rsrc_t tap_rsrc = make_rsrc(a.taps, uint64_t(NTAPS) * M * 4);
+    auto issue = [&](int slot, int q, uint32_t base_) {
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const uint32_t voff = (base_ - window0 - branch[i] - (NTAPS - 1) * M) * BPS;
+            raw[slot][i] = buffer_code<FORMAT>(in_rsrc, voff, uint32_t(NTAPS - 1 - q) * M * BPS);
+            if constexpr (!REGTAPS)
+                ctap[REGTAPS ? 0 : slot][i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                    tap_rsrc, branch[i] * 4u, uint32_t(q) * M * 4u, 0));
+        }
+    };
+    if constexpr (TAPS > 0) {
+        if (block_active(0)) {
+#pragma unroll
+            for (int q = 0; q < PF; ++q) issue(q, q, block_base(0));
+        }
+    }
+
     for (uint32_t b0 = 0; b0 < a.chunk; b0 += S::BPW) {
         const uint32_t rel = first + b0 + sub;           // block within the launch
-        const bool active = (b0 + sub < a.chunk) && (rel < a.block_count);
+        const bool active = block_active(b0);
         const uint64_t m = a.first_block + rel;           // absolute block index
-        const uint32_t base = uint32_t(m * a.decimation - a.input_origin);
+        const uint32_t base = block_base(b0);
 
         // The twiddle loads stay inside the block loop (served by L1): hoisted
         // out of it they would take registers the prototype taps need.
@@ -143,31 +227,61 @@ chain_kernel(ChainArgs a) {
 
         float2 v[V];
         if (active) {
+            if constexpr (TAPS > 0) {
+                float re[V], im[V];
 #pragma unroll
-            for (int i = 0; i < V; ++i) {
-                float re = 0.0f, im = 0.0f;
-                const uint32_t start = base - branch[i];
-                if constexpr (TAPS > 0) {
+                for (int i = 0; i < V; ++i) re[i] = im[i] = 0.0f;
+                // Ascending q for every branch (the twin's order); the V
+                // branches interleave, which no branch's sum can see.
 #pragma unroll
-                    for (int q = 0; q < TAPS; ++q) {
-                        const float2 x = load_sample<FORMAT>(a.input, start - uint32_t(q) * M);
-                        re = re + x.x * c[i][q];
-                        im = im + x.y * c[i][q];
+                for (int q = 0; q < TAPS; ++q) {
+                    uint32_t cur[V];
+                    float tap[V];
+#pragma unroll
+                    for (int i = 0; i < V; ++i) {
+                        cur[i] = raw[q % PF][i];
+                        if constexpr (REGTAPS)
+                            tap[i] = c[i][REGTAPS ? q : 0];
+                        else
+                            tap[i] = ctap[REGTAPS ? 0 : q % PF][i];
                     }
-                } else {
+                    if (q + PF < TAPS) issue(q % PF, q + PF, base);
+#pragma unroll
+                    for (int i = 0; i < V; ++i) {
+                        const float2 x = convert_code<FORMAT>(cur[i]);
+                        re[i] = re[i] + x.x * tap[i];
+                        im[i] = im[i] + x.y * tap[i];
+                    }
+                }
+                // Register index of slot t + T*i within the group is bitrev(i).
+#pragma unroll
+                for (int i = 0; i < V; ++i)
+                    v[rvx::bitrev<F::LOG_V>(i)] = make_float2(re[i], im[i]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < V; ++i) {
+                    float re = 0.0f, im = 0.0f;
+                    const uint32_t start = base - branch[i];
                     for (uint32_t q = 0; q < a.taps_per_branch; ++q) {
                         const float2 x = load_sample<FORMAT>(a.input, start - q * M);
                         const float tap = a.taps[q * M + branch[i]];
                         re = re + x.x * tap;
                         im = im + x.y * tap;
                     }
+                    v[rvx::bitrev<F::LOG_V>(i)] = make_float2(re, im);
                 }
-                // Register index of slot t + T*i within the group is bitrev(i).
-                v[rvx::bitrev<F::LOG_V>(i)] = make_float2(re, im);
             }
         } else {
 #pragma unroll
             for (int i = 0; i < V; ++i) v[i] = make_float2(0.0f, 0.0f);
+        }
+        if constexpr (TAPS > 0) {
+            const uint32_t nb = b0 + S::BPW;
+            if (nb < a.chunk && block_active(nb)) {
+                const uint32_t next = block_base(nb);
+#pragma unroll
+                for (int q = 0; q < PF; ++q) issue(q, q, next);
+            }
         }
 
         F::template run_pass<0>(v, tw, 0);
@@ -226,14 +340,17 @@ chain_kernel(ChainArgs a) {
 
 template <int LOGM, int FORMAT>
 void launch_chain_m(const ChainArgs& a, uint32_t taps, hipStream_t stream) {
-    using S = ChainShape<LOGM>;
+    // The 4096-channel grid keeps its taps in memory: in registers they take
+    // half the register file and starve the prefetch.
+    constexpr bool kRegTaps = LOGM < 12;
+    using S = ChainShape<LOGM, 2>;
     const dim3 grid(a.chunks), block(S::THREADS);
     if (taps == 16)
-        chain_kernel<LOGM, FORMAT, 16><<<grid, block, 0, stream>>>(a);
+        chain_kernel<LOGM, 2, FORMAT, 16, kRegTaps><<<grid, block, 0, stream>>>(a);
     else if (taps == 17)
-        chain_kernel<LOGM, FORMAT, 17><<<grid, block, 0, stream>>>(a);
+        chain_kernel<LOGM, 2, FORMAT, 17, kRegTaps><<<grid, block, 0, stream>>>(a);
     else
-        chain_kernel<LOGM, FORMAT, 0><<<grid, block, 0, stream>>>(a);
+        chain_kernel<LOGM, 2, FORMAT, 0, false><<<grid, block, 0, stream>>>(a);
 }
 
 template <int FORMAT>
@@ -317,6 +434,7 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
     a.partial = partial;
     a.first_block = d.first_block;
     a.input_origin = d.input_origin;
+    a.input_samples = d.input_samples;
     a.decimation = D;
     a.taps_per_branch = T;
     a.block_count = d.block_count;

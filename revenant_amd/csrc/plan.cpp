@@ -1,0 +1,445 @@
+// Receiver planning on the host: where a receiver sits on the grid and what
+// its fine stage and demodulator are. FP64 design rounded once to float;
+// every rate and phase is an exact integer or rational.
+//
+//   rvx_place           core/engine/vrx_place.cpp:216-340 (nearest channel by
+//                       round(centre*M/rate), exact rational residual)
+//   rvx_plan_vrx        core/dsp/vrx_reference.cpp:2243-2662 (design_vrx +
+//                       plan_vrx) for the receivers the chain runs: AM, NFM,
+//                       WFM mono, USB, LSB, DSB, CW and raw. Not planned here:
+//                       de-emphasis, FM stereo, the SAM/DSB carrier loop and
+//                       the digital-voice taps.
+//   rvx_fine_block      vrx_reference.cpp:2786-2871 (resample_block)
+//   rvx_demod_block     vrx_reference.cpp:2873-2909
+// Tap tables: design_fine_taps (:1808), design_audio_taps (:1921),
+// design_dc_weights (:1983).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <numeric>
+#include <vector>
+
+#include "common.h"
+#include "plan.h"
+
+namespace {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr double kTwoPi = 2.0 * kPi;
+constexpr int64_t kDefaultAudioRate = 48000;
+constexpr double kFineAttenuationDb = 80.0;
+constexpr double kAudioAttenuationDb = 80.0;
+constexpr int64_t kAmDcCornerHz = 200;
+constexpr double kFmAudioCeilingHz = 15000.0;
+constexpr double kFmPilotHz = 19000.0;
+constexpr int64_t kCompositeAudioRateHz = 114000;
+
+int64_t iabs(int64_t v) { return v < 0 ? -v : v; }
+
+void reduce(int64_t& num, int64_t& den) {
+    if (den < 0) {
+        num = -num;
+        den = -den;
+    }
+    const int64_t g = std::gcd(num, den);
+    if (g > 1) {
+        num /= g;
+        den /= g;
+    }
+}
+
+int64_t divide_nearest(int64_t num, int64_t den) {
+    const int64_t q = num / den, r = num % den;
+    if (2 * iabs(r) < iabs(den)) return q;
+    return r < 0 ? q - 1 : q + 1;
+}
+
+double i0(double x) {
+    const double q = 0.25 * x * x;
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 256; ++k) {
+        term *= q / (double(k) * double(k));
+        sum += term;
+        if (term < 1e-18 * sum) break;
+    }
+    return sum;
+}
+
+double kaiser_shape(double a) {
+    if (a > 50.0) return 0.1102 * (a - 8.7);
+    if (a >= 21.0) return 0.5842 * std::pow(a - 21.0, 0.4) + 0.07886 * (a - 21.0);
+    return 0.0;
+}
+
+// Kaiser's order estimate for a transition given as a fraction of the rate.
+uint32_t kaiser_taps(double attenuation_db, double transition_fraction) {
+    if (!(transition_fraction > 0.0)) return 0;
+    const double order = (attenuation_db - 8.0) / (2.285 * kTwoPi * transition_fraction);
+    if (!(order > 0.0)) return 2;
+    return uint32_t(std::ceil(order)) + 1u;
+}
+
+double sinc_pi(double z) {
+    if (z == 0.0) return 1.0;
+    const double a = kPi * z;
+    return std::sin(a) / a;
+}
+
+double kaiser_window(size_t i, size_t len, double beta, double norm) {
+    if (len < 2) return 1.0;
+    const double x = 2.0 * double(i) / double(len - 1) - 1.0;
+    return i0(beta * std::sqrt(std::max(0.0, 1.0 - x * x))) / norm;
+}
+
+struct Band {
+    int64_t low = 0, high = 0;
+    int64_t width() const { return high - low; }
+    int64_t reach() const { return std::max(-low, high); }
+};
+
+Band default_band(uint32_t mode) {
+    switch (mode) {
+        case RVX_DEMOD_RAW: return {-6000, 6000};
+        case RVX_DEMOD_AM: return {-5000, 5000};
+        case RVX_DEMOD_NFM: return {-8000, 8000};
+        case RVX_DEMOD_WFM: return {-100000, 100000};
+        case RVX_DEMOD_USB: return {300, 2700};
+        case RVX_DEMOD_LSB: return {-2700, -300};
+        case RVX_DEMOD_DSB: return {-3000, 3000};
+        case RVX_DEMOD_CW: return {-600, 600};
+        default: return {};
+    }
+}
+
+int32_t resolve_band(const rvx_vrx_request& r, Band& out, const char* who) {
+    if (r.passband_low != 0 || r.passband_high != 0) {
+        if (r.passband_low >= r.passband_high)
+            return rvx::fail(RVX_EINVAL, "%s: passband %lld..%lld Hz is empty or inverted", who,
+                             (long long)r.passband_low, (long long)r.passband_high);
+        out = {r.passband_low, r.passband_high};
+        return RVX_OK;
+    }
+    if (r.bandwidth_hz < 0) return rvx::fail(RVX_EINVAL, "%s: negative bandwidth", who);
+    if (r.bandwidth_hz == 0) {
+        out = default_band(r.mode);
+        if (out.width() <= 0) return rvx::fail(RVX_EINVAL, "%s: mode %u has no default", who, r.mode);
+        return RVX_OK;
+    }
+    const int64_t half = r.bandwidth_hz / 2;
+    if (r.mode == RVX_DEMOD_USB)
+        out = {0, r.bandwidth_hz};
+    else if (r.mode == RVX_DEMOD_LSB)
+        out = {-r.bandwidth_hz, 0};
+    else
+        out = {-half, half};
+    return RVX_OK;
+}
+
+int64_t widest(const rvx_vrx_placement& p) {
+    if (p.channel_rate <= 0 || p.residual_den <= 0) return 0;
+    const int64_t scaled = p.channel_rate * p.residual_den - 2 * iabs(p.residual_num);
+    return scaled <= 0 ? 0 : scaled / p.residual_den;
+}
+
+Band clamp_band(const rvx_vrx_placement& p, Band b) {
+    const int64_t w = widest(p);
+    if (w <= 0) return {};
+    const int64_t lim = w / 2;
+    return {std::clamp(b.low, -lim, lim), std::clamp(b.high, -lim, lim)};
+}
+
+Band mix_frame(Band b, uint32_t mode, int64_t pitch) {
+    if (mode != RVX_DEMOD_CW) return b;
+    return {b.low + pitch, b.high + pitch};
+}
+
+int64_t minimum_rate(uint32_t mode, Band b) {
+    const int64_t width = b.width();
+    if (width <= 0) return 0;
+    const int64_t floor_rate = std::max((3 * width + 1) / 2, 2 * b.reach());
+    if (mode == RVX_DEMOD_AM) return std::max(floor_rate, 2 * width);
+    return floor_rate;
+}
+
+bool complex_output(uint32_t mode) { return mode == RVX_DEMOD_RAW; }
+
+}  // namespace
+
+extern "C" {
+
+int32_t rvx_place(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* request,
+                  rvx_vrx_placement* out) {
+    static const char* who = "rvx_place";
+    if (!grid || !request || !out) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    const rvx_grid g = *grid;
+    if (g.channels < 2 || !rvx::pow2(g.channels) || g.decimation == 0 ||
+        g.channels % g.decimation != 0)
+        return rvx::fail(RVX_EINVAL, "%s: invalid grid", who);
+    if (rate <= 0) return rvx::fail(RVX_EINVAL, "%s: rate %lld", who, (long long)rate);
+    if (rate % int64_t(g.decimation) != 0)
+        return rvx::fail(RVX_EINVAL, "%s: rate %lld not divisible by decimation %u", who,
+                         (long long)rate, g.decimation);
+    const int64_t nyq = rate / 2;
+    if (request->center_hz > nyq || request->center_hz < -nyq)
+        return rvx::fail(RVX_EINVAL, "%s: centre %lld Hz outside +-%lld", who,
+                         (long long)request->center_hz, (long long)nyq);
+    Band req;
+    if (int32_t rc = resolve_band(*request, req, who)) return rc;
+    const int64_t M = g.channels;
+    int64_t idx = divide_nearest(request->center_hz * M, rate) % M;
+    if (idx < 0) idx += M;
+    rvx_vrx_placement p{};
+    p.channel = uint32_t(idx);
+    // Channel k's centre k*rate/M folded into [-rate/2, rate/2), reduced.
+    const int64_t signed_k = idx < M / 2 ? idx : idx - M;
+    p.centre_num = signed_k * rate;
+    p.centre_den = M;
+    reduce(p.centre_num, p.centre_den);
+    p.channel_rate = rate / int64_t(g.decimation);
+    p.residual_num = request->center_hz * p.centre_den - p.centre_num;
+    p.residual_den = p.centre_den;
+    const int64_t span = rate * p.residual_den;
+    while (2 * p.residual_num > span) p.residual_num -= span;
+    while (2 * p.residual_num <= -span) p.residual_num += span;
+    reduce(p.residual_num, p.residual_den);
+    const Band granted = clamp_band(p, req);
+    if (granted.width() <= 0)
+        return rvx::fail(RVX_EINVAL,
+                         "%s: one %lld S/s channel cannot carry a %lld..%lld Hz passband here",
+                         who, (long long)p.channel_rate, (long long)req.low, (long long)req.high);
+    p.granted_low = granted.low;
+    p.granted_high = granted.high;
+    p.bandwidth_clamped = (granted.low != req.low || granted.high != req.high) ? 1u : 0u;
+    *out = p;
+    return RVX_OK;
+}
+
+int32_t rvx_plan_vrx(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* request,
+                     const rvx_vrx_placement* placement, rvx_vrx_plan* out) {
+    static const char* who = "rvx_plan_vrx";
+    if (!grid || !request || !placement || !out)
+        return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    const rvx_vrx_request r = *request;
+    const rvx_vrx_placement pl = *placement;
+    if (rate <= 0 || pl.channel_rate <= 0 || pl.residual_den <= 0)
+        return rvx::fail(RVX_EINVAL, "%s: placement carries no rates", who);
+    if (r.mode > RVX_DEMOD_CW)
+        return rvx::fail(RVX_EINVAL, "%s: mode %u is not planned by this build", who, r.mode);
+    rvx_vrx_plan plan{};
+    plan.placement = pl;
+    plan.mode = r.mode;
+    plan.channel_rate = pl.channel_rate;
+    plan.audio_rate = r.audio_rate > 0 ? r.audio_rate : kDefaultAudioRate;
+    Band req;
+    if (int32_t rc = resolve_band(r, req, who)) return rc;
+    const Band band = clamp_band(pl, req);
+    if (band.width() <= 0) return rvx::fail(RVX_EINVAL, "%s: the channel carries nothing", who);
+    plan.passband_low = band.low;
+    plan.passband_high = band.high;
+    plan.bandwidth = band.width();
+    const int64_t pitch = std::max<int64_t>(0, r.cw_pitch);
+    const Band mixed = mix_frame(band, r.mode, pitch);
+    const int64_t required = minimum_rate(r.mode, mixed);
+    const int64_t step = plan.audio_rate;
+    plan.demod_rate = std::max<int64_t>(1, (required + step - 1) / step) * step;
+    const uint32_t decimation = uint32_t(plan.demod_rate / plan.audio_rate);
+
+    const double stop_edge = 0.5 * double(std::min(plan.demod_rate, plan.channel_rate));
+    const double limit =
+        2.0 * std::min(stop_edge - double(mixed.high), stop_edge + double(mixed.low));
+    const double transition = std::min(limit, 0.5 * double(plan.bandwidth));
+    if (!(transition > 0.0))
+        return rvx::fail(RVX_EINVAL, "%s: no transition band left at %lld S/s", who,
+                         (long long)plan.demod_rate);
+    plan.fine.phases = 256;
+    plan.fine.nco_log2 = 16;
+    plan.fine.taps = std::clamp(
+        kaiser_taps(kFineAttenuationDb, transition / double(plan.channel_rate)), 8u, 256u);
+
+    plan.filter_num = 2 * pl.residual_num + pl.residual_den * (band.low + band.high);
+    plan.filter_den = 2 * pl.residual_den;
+    plan.mix_num = pl.residual_num;
+    plan.mix_den = pl.residual_den;
+    if (r.mode == RVX_DEMOD_CW) plan.mix_num = pl.residual_num - pl.residual_den * pitch;
+    reduce(plan.filter_num, plan.filter_den);
+    reduce(plan.mix_num, plan.mix_den);
+    if (int32_t rc = rvx_nco_delta(plan.mix_num, plan.mix_den, plan.demod_rate,
+                                   &plan.fine_nco_delta))
+        return rc;
+
+    const bool cx = complex_output(r.mode);
+    plan.demod.mode = r.mode;
+    plan.demod.decimation = cx ? 1u : decimation;
+    plan.output_rate = cx ? plan.demod_rate : plan.audio_rate;
+    const bool programme = r.mode == RVX_DEMOD_WFM && plan.audio_rate < kCompositeAudioRateHz;
+    const double nyq = 0.5 * double(plan.audio_rate);
+    plan.audio_pass_hz = programme ? std::min(kFmAudioCeilingHz, 0.8 * nyq) : 0.8 * nyq;
+    plan.audio_stop_hz = programme ? std::min(kFmPilotHz, nyq) : nyq;
+    if (plan.demod.decimation == 1u) {
+        plan.audio_decimation_taps = 1;
+        plan.audio_pass_hz = plan.audio_stop_hz = 0.0;
+    } else {
+        const double frac = (plan.audio_stop_hz - plan.audio_pass_hz) / double(plan.demod_rate);
+        uint32_t taps = std::clamp(kaiser_taps(kAudioAttenuationDb, frac), 3u, 1024u);
+        if ((taps % 2u) == 0u) ++taps;
+        plan.audio_decimation_taps = std::min(taps, 1024u);
+    }
+    plan.demod.audio_taps = plan.audio_decimation_taps;
+    plan.demod.channels = cx ? 2u : 1u;
+    plan.demod.pilot_taps = 0;
+    plan.demod.dc_taps =
+        (r.mode == RVX_DEMOD_AM)
+            ? uint32_t(std::clamp<int64_t>(plan.demod_rate / kAmDcCornerHz, 16, 4096))
+            : 1u;
+    if (r.mode == RVX_DEMOD_NFM)
+        plan.deviation = std::max<int64_t>(1, plan.bandwidth / 5);
+    else if (r.mode == RVX_DEMOD_WFM)
+        plan.deviation = std::max<int64_t>(1, (3 * plan.bandwidth) / 8);
+    plan.demod_gain = 1.0f;
+    if ((r.mode == RVX_DEMOD_NFM || r.mode == RVX_DEMOD_WFM) && plan.deviation > 0)
+        plan.demod_gain = float(double(plan.demod_rate) / (kTwoPi * double(plan.deviation)));
+    *out = plan;
+    return RVX_OK;
+}
+
+int32_t rvx_plan_vrx_tables(const rvx_vrx_plan* plan, float* fine_taps, uint64_t n_fine_floats,
+                            float* weights, uint64_t n_weights) {
+    static const char* who = "rvx_plan_vrx_tables";
+    if (!plan) return rvx::fail(RVX_EINVAL, "%s: null plan", who);
+    const rvx_vrx_plan& p = *plan;
+    const size_t phases = p.fine.phases, taps = p.fine.taps, length = phases * taps;
+    if (fine_taps != nullptr) {
+        if (n_fine_floats != 2 * (length + 1))
+            return rvx::fail(RVX_EINVAL, "%s: fine table wants %zu floats", who, 2 * (length + 1));
+        const int64_t half_width = p.bandwidth / 2;
+        if (half_width <= 0 || 2 * half_width >= p.channel_rate || p.filter_den == 0)
+            return rvx::fail(RVX_EINVAL, "%s: half width %lld at %lld S/s", who,
+                             (long long)half_width, (long long)p.channel_rate);
+        // Kaiser-windowed lowpass of cutoff half_width over P*T taps, each
+        // phase normalised to unit DC gain, then modulated to the passband's
+        // centre filter_num/filter_den Hz with the angle reduced in integers.
+        const double centre = (double(length) - 1.0) / 2.0;
+        const double cutoff = double(half_width) / double(p.channel_rate);
+        const double beta = kaiser_shape(kFineAttenuationDb), norm = i0(beta);
+        std::vector<double> proto(length);
+        for (size_t i = 0; i < length; ++i) {
+            const double pos = (double(i) - centre) / double(phases);
+            proto[i] = 2.0 * cutoff * sinc_pi(2.0 * cutoff * pos) *
+                       kaiser_window(i, length, beta, norm);
+        }
+        for (size_t ph = 0; ph < phases; ++ph) {
+            double sum = 0.0;
+            for (size_t k = 0; k < taps; ++k) sum += proto[ph + k * phases];
+            if (std::abs(sum) < 1e-12) continue;
+            for (size_t k = 0; k < taps; ++k) proto[ph + k * phases] /= sum;
+        }
+        const int64_t period = 2 * iabs(p.filter_den) * int64_t(phases) * p.channel_rate;
+        const int64_t num = p.filter_den < 0 ? -p.filter_num : p.filter_num;
+        for (size_t i = 0; i < length; ++i) {
+            const int64_t twice = 2 * int64_t(i) - (int64_t(length) - 1);
+            int64_t turns = (num * twice) % period;
+            if (turns < 0) turns += period;
+            const double ang = kTwoPi * double(turns) / double(period);
+            fine_taps[2 * i] = float(proto[i] * std::cos(ang));
+            fine_taps[2 * i + 1] = float(proto[i] * std::sin(ang));
+        }
+        fine_taps[2 * length] = 0.0f;
+        fine_taps[2 * length + 1] = 0.0f;
+    }
+    if (weights != nullptr) {
+        const uint32_t at = p.demod.audio_taps, dc = p.demod.dc_taps;
+        if (n_weights != uint64_t(at) + dc)
+            return rvx::fail(RVX_EINVAL, "%s: weights want %u floats", who, at + dc);
+        // Audio decimation filter: Kaiser lowpass, unit DC gain.
+        if (at == 1) {
+            weights[0] = 1.0f;
+        } else {
+            const double cut = (p.audio_pass_hz + p.audio_stop_hz) * 0.5 / double(p.demod_rate);
+            const double centre = (double(at) - 1.0) / 2.0;
+            const double beta = kaiser_shape(kAudioAttenuationDb), norm = i0(beta);
+            std::vector<double> d(at);
+            double sum = 0.0;
+            for (uint32_t i = 0; i < at; ++i) {
+                d[i] = 2.0 * cut * sinc_pi(2.0 * cut * (double(i) - centre)) *
+                       kaiser_window(i, at, beta, norm);
+                sum += d[i];
+            }
+            const double scale = std::abs(sum) < 1e-12 ? 1.0 : 1.0 / sum;
+            for (uint32_t i = 0; i < at; ++i) weights[i] = float(d[i] * scale);
+        }
+        // DC window: raised cosine over dc taps, unit sum.
+        if (dc <= 1) {
+            weights[at] = 1.0f;
+        } else {
+            std::vector<double> wdc(dc);
+            double sum = 0.0;
+            for (uint32_t i = 0; i < dc; ++i) {
+                wdc[i] = 0.5 - 0.5 * std::cos(kTwoPi * double(i + 1) / double(dc + 1));
+                sum += wdc[i];
+            }
+            const double scale = sum < 1e-12 ? 1.0 : 1.0 / sum;
+            for (uint32_t i = 0; i < dc; ++i) weights[at + i] = float(wdc[i] * scale);
+        }
+    }
+    return RVX_OK;
+}
+
+int32_t rvx_fine_block(const rvx_vrx_plan* plan, uint32_t channel_base, uint32_t channel_mask,
+                       uint32_t fine_mask, uint64_t first_output, uint32_t count,
+                       rvx_vrx_fine_params* out, uint64_t* first_input) {
+    static const char* who = "rvx_fine_block";
+    if (!plan || !out) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    if (plan->channel_rate <= 0 || plan->demod_rate <= 0)
+        return rvx::fail(RVX_EINVAL, "%s: the plan carries no rates", who);
+    if (count == 0) return rvx::fail(RVX_EINVAL, "%s: zero outputs", who);
+    const uint64_t fc = uint64_t(plan->channel_rate), fd = uint64_t(plan->demod_rate);
+    const uint64_t sw = fc / fd, sr = fc % fd;
+    // n0 = floor(j0*Fc/Fd), frac0 = (j0*Fc) mod Fd without forming j0*Fc.
+    const uint64_t whole = first_output / fd, part = first_output % fd;
+    const uint64_t cp = part * sr;
+    const uint64_t n0 = first_output * sw + whole * sr + cp / fd;
+    rvx_vrx_fine_params p{};
+    p.chan_base = channel_base;
+    p.chan_mask = channel_mask;
+    p.in_offset = uint32_t(n0 & channel_mask);
+    p.out_offset = uint32_t(first_output & fine_mask);
+    p.out_mask = fine_mask;
+    p.count = count;
+    p.step_whole = uint32_t(sw);
+    p.step_rem = uint32_t(sr);
+    p.out_rate = uint32_t(fd);
+    p.frac0 = uint32_t(cp % fd);
+    p.inv_out_rate = 1.0f / float(fd);
+    const uint64_t phase = plan->fine_nco_delta * first_output;  // wraps mod 2^64
+    p.nco_phase_high = uint32_t(phase >> 32);
+    p.nco_phase_low = uint32_t(phase);
+    p.nco_delta_high = uint32_t(plan->fine_nco_delta >> 32);
+    p.nco_delta_low = uint32_t(plan->fine_nco_delta);
+    if (int32_t rc = rvx::vrx_fine_valid(plan->fine, p, who)) return rc;
+    *out = p;
+    if (first_input) *first_input = n0;
+    return RVX_OK;
+}
+
+int32_t rvx_demod_block(const rvx_vrx_plan* plan, uint32_t fine_mask, uint64_t first_audio,
+                        uint32_t count, rvx_vrx_demod_params* out) {
+    static const char* who = "rvx_demod_block";
+    if (!plan || !out) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    if (count == 0) return rvx::fail(RVX_EINVAL, "%s: zero outputs", who);
+    const uint64_t first_fine = first_audio * plan->demod.decimation;
+    rvx_vrx_demod_params p{};
+    p.in_mask = fine_mask;
+    p.in_offset = uint32_t(first_fine & fine_mask);
+    p.count = count;
+    p.gain = plan->demod_gain;
+    if (int32_t rc = rvx::vrx_demod_valid(plan->demod, p, who)) return rc;
+    *out = p;
+    return RVX_OK;
+}
+
+uint32_t rvx_demod_history(const rvx_vrx_demod_config* config) {
+    return config ? rvx::demod_fine_history(*config) : 0u;
+}
+
+}  // extern "C"

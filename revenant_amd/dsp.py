@@ -105,3 +105,86 @@ def nco_delta(hz_numerator: int, hz_denominator: int, rate: int) -> int:
 def ema_frame_decay(alpha: float, frame_blocks: int) -> np.float32:
     """(1 - alpha)^frame_blocks rounded once to float: the frame recurrence's factor."""
     return np.float32((1.0 - float(np.float32(alpha))) ** frame_blocks)
+
+
+# ---------------------------------------------------------------------------
+# Receivers (core/engine/vrx_place.cpp, core/dsp/vrx_reference.cpp)
+# ---------------------------------------------------------------------------
+
+@dataclass
+class VrxRequest:
+    """core/engine/vrx.h VrxParams: what a receiver asks for."""
+
+    center_hz: int
+    mode: int
+    bandwidth_hz: int = 0
+    audio_rate: int = 48000
+    cw_pitch: int = 0
+    passband: tuple[int, int] | None = None
+
+    def c(self) -> _lib.VrxRequest:
+        lo, hi = self.passband if self.passband else (0, 0)
+        return _lib.VrxRequest(self.center_hz, self.bandwidth_hz, lo, hi, self.mode, 0,
+                               self.audio_rate, self.cw_pitch)
+
+
+@dataclass
+class VrxPlan:
+    """A planned receiver: the C plan block and its float tables."""
+
+    plan: _lib.VrxPlan
+    fine_taps: np.ndarray  # complex64, phases*taps + 1
+    weights: np.ndarray    # float32, audio_taps + dc_taps
+
+    @property
+    def fine(self) -> _lib.VrxFineConfig:
+        return self.plan.fine
+
+    @property
+    def demod(self) -> _lib.VrxDemodConfig:
+        return self.plan.demod
+
+    def fine_block(self, channel_base: int, channel_mask: int, fine_mask: int,
+                   first_output: int, count: int) -> tuple[_lib.VrxFineParams, int]:
+        out = _lib.VrxFineParams()
+        first_input = C.c_uint64()
+        _lib.check(_lib.lib().rvx_fine_block(C.byref(self.plan), channel_base, channel_mask,
+                                             fine_mask, first_output, count, C.byref(out),
+                                             C.byref(first_input)))
+        return out, first_input.value
+
+    def demod_block(self, fine_mask: int, first_audio: int, count: int) -> _lib.VrxDemodParams:
+        out = _lib.VrxDemodParams()
+        _lib.check(_lib.lib().rvx_demod_block(C.byref(self.plan), fine_mask, first_audio, count,
+                                              C.byref(out)))
+        return out
+
+    @property
+    def history(self) -> int:
+        """Fine samples below an output's index the demodulator reads."""
+        return int(_lib.lib().rvx_demod_history(C.byref(self.plan.demod)))
+
+
+def place(grid: GridParams, rate: int, request: VrxRequest) -> _lib.VrxPlacement:
+    """core/engine/vrx_place.cpp:216."""
+    out = _lib.VrxPlacement()
+    g, r = grid.c(), request.c()
+    _lib.check(_lib.lib().rvx_place(C.byref(g), rate, C.byref(r), C.byref(out)))
+    return out
+
+
+def plan_vrx(grid: GridParams, rate: int, request: VrxRequest,
+             placement: _lib.VrxPlacement | None = None) -> VrxPlan:
+    """core/dsp/vrx_reference.cpp:2568 (the modes this build plans)."""
+    if placement is None:
+        placement = place(grid, rate, request)
+    out = _lib.VrxPlan()
+    g, r = grid.c(), request.c()
+    _lib.check(_lib.lib().rvx_plan_vrx(C.byref(g), rate, C.byref(r), C.byref(placement),
+                                       C.byref(out)))
+    n = out.fine.phases * out.fine.taps + 1
+    taps = np.zeros(2 * n, dtype=np.float32)
+    w = np.zeros(out.demod.audio_taps + out.demod.dc_taps, dtype=np.float32)
+    _lib.check(_lib.lib().rvx_plan_vrx_tables(C.byref(out), _fptr(taps), taps.size, _fptr(w),
+                                              w.size))
+    return VrxPlan(out, taps.view(np.complex64), w)
