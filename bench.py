@@ -5,7 +5,7 @@ Workload (the wideband configuration): 2^27 complex int16 IQ samples at
 122.88 MS/s holding 400 random carriers (CW/AM/NFM/WFM, -80..-10 dBFS) over
 AWGN; a 4096-channel, 2x-oversampled polyphase channelizer (16 taps per
 branch, Kaiser beta 8 prototype, 4096-point radix-2 transform); |X|^2 averaged
-with alpha = 1/16 over 16-block frames. One step = one fused launch over the
+with alpha = 1/16 over 4-block frames. One step = one fused launch over the
 whole resident input (convert -> branch filter -> transform -> correction ->
 channel ring + averaged spectrum) plus the frame recurrence.
 
@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 RATE = 122_880_000
 SAMPLES = 1 << 27
 M, TAPS, D = 4096, 16, 2048
-FRAME = 16
+FRAME = 4
 ALPHA = 1.0 / 16.0
 SEED = 0x5743454E45000002
 
@@ -177,7 +177,7 @@ def main() -> int:
         "config": {
             "workload": "plain chain: 2^27 cs16 @ 122.88 MS/s, 400 carriers + AWGN -> "
                         "4096-ch PFB (D=2048, 16 taps/branch, Kaiser beta 8) -> channel ring "
-                        "+ |X|^2 EMA alpha=1/16 (16-block frames); VRX stage not in this line",
+                        "+ |X|^2 EMA alpha=1/16 (4-block frames); VRX stage not in this line",
             "input_samples": int(in_samples),
             "blocks": int(blocks),
             "channels": M,
@@ -191,7 +191,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic_from_profiles(),
-            "kernel": "chain_kernel<12, cs16, 16>",
+            "kernel": "chain_kernel<M=4096, V=8, cs16, 16 taps>",
             "kernel_ms": round(chain_ms, 4),
             "algorithmic_bytes": int(algo_bytes),
         },

@@ -181,9 +181,11 @@ typedef struct rvx_ctx rvx_ctx;
  * `device` with its own stream (default priority). */
 int32_t rvx_ctx_create(int32_t device, rvx_ctx** out);
 void rvx_ctx_destroy(rvx_ctx* ctx);
-/* Replace the stream launches go to (e.g. torch's current stream). NULL
- * restores the context's own stream. */
+/* Launch on `hip_stream` from now on (e.g. torch's current stream); NULL is
+ * the device's null stream. */
 int32_t rvx_ctx_set_stream(rvx_ctx* ctx, void* hip_stream);
+/* Go back to the context's own stream. */
+int32_t rvx_ctx_use_own_stream(rvx_ctx* ctx);
 void* rvx_ctx_stream(rvx_ctx* ctx);
 /* Wait for everything launched on the context's stream. */
 int32_t rvx_stream_sync(rvx_ctx* ctx);

@@ -35,7 +35,7 @@ def lib() -> C.CDLL:
         for name in ("oracle_convert", "oracle_cmul", "oracle_pfb_branch", "oracle_pfb_fft",
                      "oracle_fft_radix2", "oracle_spectrum", "oracle_spectrum_levels",
                      "oracle_det", "oracle_ema_partial", "oracle_ema_combine", "oracle_chain",
-                     "fp64_channelize", "fp64_dft"):
+                     "fp64_channelize", "fp64_dft", "oracle_vrx_fine", "oracle_vrx_demod"):
             getattr(_lib, name).restype = i32
     return _lib
 
@@ -175,3 +175,23 @@ def fp64_dft(x: np.ndarray) -> np.ndarray:
     out = np.zeros_like(xd)
     _check(lib().fp64_dft(_a(xd), u32(xd.size // 2), _a(out)), "fp64_dft")
     return out.view(np.complex128)
+
+
+def vrx_fine(config, params, channel_ring: np.ndarray, taps: np.ndarray, nco: np.ndarray,
+             out_samples: int) -> np.ndarray:
+    """reference_vrx_fine; returns the fine ring (float32 pairs, out_samples)."""
+    ch, tp, nc = f32c(channel_ring), f32c(taps), f32c(nco)
+    out = np.zeros(2 * out_samples, dtype=np.float32)
+    _check(lib().oracle_vrx_fine(C.byref(config), C.byref(params), _a(ch), u64(ch.size // 2),
+                                 _a(tp), u64(tp.size // 2), _a(nc), u64(nc.size // 2), _a(out),
+                                 u64(out_samples)), "vrx_fine")
+    return out
+
+
+def vrx_demod(config, params, fine_ring: np.ndarray, weights: np.ndarray) -> np.ndarray:
+    fr = f32c(fine_ring)
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    out = np.zeros(params.count * config.channels, dtype=np.float32)
+    _check(lib().oracle_vrx_demod(C.byref(config), C.byref(params), _a(fr), u64(fr.size // 2),
+                                  _a(w), u64(w.size), _a(out), u64(out.size)), "vrx_demod")
+    return out

@@ -112,6 +112,7 @@ _SIGNATURES = {
     "rvx_ctx_create": (i32, [i32, C.POINTER(vp)]),
     "rvx_ctx_destroy": (None, [vp]),
     "rvx_ctx_set_stream": (i32, [vp, vp]),
+    "rvx_ctx_use_own_stream": (i32, [vp]),
     "rvx_ctx_stream": (vp, [vp]),
     "rvx_stream_sync": (i32, [vp]),
     "rvx_buf_alloc": (i32, [vp, u64, i32, C.POINTER(vp)]),

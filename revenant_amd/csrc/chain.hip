@@ -15,11 +15,12 @@
 //   - the power average's state stays in registers across the frame.
 //
 // Work split. A workgroup owns a chunk of consecutive blocks (one averaging
-// frame) and walks it in order, so the frame's average accumulates in the
-// order its definition fixes. Thread t of a transform owns V branches; their
-// V*TAPS prototype taps are loaded once and kept in registers for the whole
-// chunk. Chunks are numbered so that the workgroups one XCD runs at the same
-// time own neighbouring stretches of input and share its L2.
+// frame, or a single block) and walks it in order, so a frame's average
+// accumulates in the order its definition fixes. Thread t of a transform owns
+// V branches. Chunks are numbered so that the workgroups one XCD runs at the
+// same time own neighbouring blocks: they share the input window in L2, and
+// their channel-ring writes, 8 bytes per channel per block, land in the same
+// 128-byte lines while those are still in L2.
 
 #pragma clang fp contract(off)
 
@@ -140,13 +141,11 @@ chain_kernel(ChainArgs a) {
     // XCD-aware chunk numbering: the hardware deals workgroups to the 8 XCDs
     // round robin, so workgroups w, w+8, w+16, ... share an XCD. Give each
     // XCD a contiguous run of chunks.
+    // The grid is 8 * ceil(chunks / 8); the tail of the last XCD's run is
+    // empty and those workgroups leave at once.
     const uint32_t w = blockIdx.x;
-    const uint32_t per_xcd = (a.chunks + 7) / 8;
-    const uint32_t xcd = w % 8, seq = w / 8;
-    uint32_t chunk = xcd * per_xcd + seq;
-    // When chunks is not a multiple of 8 the mapping above can overshoot;
-    // fall back to the identity numbering, which is always a bijection.
-    if ((a.chunks % 8) != 0) chunk = w;
+    const uint32_t per_xcd = gridDim.x / 8;
+    const uint32_t chunk = (w % 8) * per_xcd + w / 8;
     if (chunk >= a.chunks) return;  // whole workgroup leaves together
 
     const uint32_t sub = threadIdx.x / T;
@@ -338,19 +337,22 @@ chain_kernel(ChainArgs a) {
     }
 }
 
+// Eight values per thread: radix-8 register passes (four for 4096 points).
+constexpr int kChainLogV = 3;
+
 template <int LOGM, int FORMAT>
 void launch_chain_m(const ChainArgs& a, uint32_t taps, hipStream_t stream) {
     // The 4096-channel grid keeps its taps in memory: in registers they take
     // half the register file and starve the prefetch.
     constexpr bool kRegTaps = LOGM < 12;
-    using S = ChainShape<LOGM, 2>;
-    const dim3 grid(a.chunks), block(S::THREADS);
+    using S = ChainShape<LOGM, kChainLogV>;
+    const dim3 grid(8 * ((a.chunks + 7) / 8)), block(S::THREADS);
     if (taps == 16)
-        chain_kernel<LOGM, 2, FORMAT, 16, kRegTaps><<<grid, block, 0, stream>>>(a);
+        chain_kernel<LOGM, kChainLogV, FORMAT, 16, kRegTaps><<<grid, block, 0, stream>>>(a);
     else if (taps == 17)
-        chain_kernel<LOGM, 2, FORMAT, 17, kRegTaps><<<grid, block, 0, stream>>>(a);
+        chain_kernel<LOGM, kChainLogV, FORMAT, 17, kRegTaps><<<grid, block, 0, stream>>>(a);
     else
-        chain_kernel<LOGM, 2, FORMAT, 0, false><<<grid, block, 0, stream>>>(a);
+        chain_kernel<LOGM, kChainLogV, FORMAT, 0, false><<<grid, block, 0, stream>>>(a);
 }
 
 template <int FORMAT>
@@ -414,7 +416,11 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
                          (unsigned long long)d.first_block);
     if (last_base >= d.input_origin + d.input_samples)
         return rvx::fail(RVX_EINVAL, "%s: the last block needs samples past the input's end", who);
-    uint32_t chunk = 16;
+    // One block per workgroup unless averaging: the XCD numbering then hands
+    // an XCD's concurrent workgroups consecutive blocks, so the channel
+    // ring's 128-byte lines (16 blocks of one channel) fill while they are
+    // still in that XCD's L2.
+    uint32_t chunk = 1;
     if (d.frame_blocks != 0) {
         if (d.block_count % d.frame_blocks)
             return rvx::fail(RVX_EINVAL, "%s: frame of %u blocks does not divide %u blocks", who,

@@ -100,7 +100,13 @@ void rvx_ctx_destroy(rvx_ctx* ctx) {
 
 int32_t rvx_ctx_set_stream(rvx_ctx* ctx, void* hip_stream) {
     if (ctx == nullptr) return rvx::fail(RVX_EINVAL, "rvx_ctx_set_stream: null context");
-    ctx->stream = hip_stream ? static_cast<hipStream_t>(hip_stream) : ctx->own;
+    ctx->stream = static_cast<hipStream_t>(hip_stream);
+    return RVX_OK;
+}
+
+int32_t rvx_ctx_use_own_stream(rvx_ctx* ctx) {
+    if (ctx == nullptr) return rvx::fail(RVX_EINVAL, "rvx_ctx_use_own_stream: null context");
+    ctx->stream = ctx->own;
     return RVX_OK;
 }
 

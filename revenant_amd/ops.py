@@ -131,6 +131,20 @@ class Device:
                                             _n(partial), _p(state), _p(rows),
                                             0 if rows is None else _n(rows)))
 
+    def vrx_fine(self, config: _lib.VrxFineConfig, params: _lib.VrxFineParams,
+                 channel_ring: torch.Tensor, taps: torch.Tensor, nco: torch.Tensor,
+                 fine_ring: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_vrx_fine(self.ctx, C.byref(config), C.byref(params),
+                                         _p(channel_ring), _csamples(channel_ring), _p(taps),
+                                         _csamples(taps), _p(nco), _csamples(nco),
+                                         _p(fine_ring), _csamples(fine_ring)))
+
+    def vrx_demod(self, config: _lib.VrxDemodConfig, params: _lib.VrxDemodParams,
+                  fine_ring: torch.Tensor, weights: torch.Tensor, audio: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_vrx_demod(self.ctx, C.byref(config), C.byref(params),
+                                          _p(fine_ring), _csamples(fine_ring), _p(weights),
+                                          _n(weights), _p(audio), _n(audio)))
+
     def chain(self, desc: _lib.ChainDesc, packed: torch.Tensor, taps: torch.Tensor,
               twiddles: torch.Tensor, ring: torch.Tensor,
               partial: torch.Tensor | None = None) -> None:
