@@ -98,17 +98,23 @@ typedef struct rvx_spectrum_levels_params {
  *   partial: for each frame f, s = 0; for each of its frame_blocks channel
  *            samples v in ascending order: x = v.re*v.re + v.im*v.im;
  *            s = s + (x - s) * alpha
- *   combine: P = P * frame_decay + s_f for ascending f; row f = P.
- * frame_decay is (1 - alpha)^frame_blocks, rounded once to float. */
+ *   row:     for each row r of row_frames consecutive frames, S = 0; for its
+ *            frames in ascending order: S = S * frame_decay + s_f
+ *   combine: P = P * row_decay + S_r for ascending r; row r = P.
+ * frame_decay = (1 - alpha)^frame_blocks and row_decay = (1 - alpha)^
+ * (frame_blocks * row_frames), each rounded once to float. The row level makes
+ * the only serial part one step per row instead of one per frame. */
 typedef struct rvx_ema_params {
     uint32_t channels;
     uint32_t chan_blocks;
     uint32_t chan_mask;
     uint32_t in_offset;
     uint32_t frame_blocks;
-    uint32_t frames;
+    uint32_t frames;       /* a multiple of row_frames */
     float alpha;
     float frame_decay;
+    uint32_t row_frames;   /* >= 1 */
+    float row_decay;
 } rvx_ema_params;
 
 /* core/dsp/vrx_reference.h:178-195 VrxFineConfig */
@@ -345,8 +351,8 @@ int32_t rvx_spectrum_levels(rvx_ctx* ctx, const rvx_spectrum_levels_params* para
                             const float* frame, uint64_t n_frame, float* levels);
 
 /* The averaged power spectrum (see rvx_ema_params). partial: frames*channels
- * floats. state: channels floats, read and updated. rows: frames*channels,
- * or NULL. */
+ * floats. state: channels floats, read and updated. rows: (frames /
+ * row_frames) * channels floats, required. */
 int32_t rvx_ema_partial(rvx_ctx* ctx, const rvx_ema_params* params, const void* channel_ring,
                         uint64_t ring_samples, float* partial, uint64_t n_partial);
 int32_t rvx_ema_combine(rvx_ctx* ctx, const rvx_ema_params* params, const float* partial,

@@ -132,24 +132,28 @@ def ema_partial(params, ring: np.ndarray) -> np.ndarray:
 
 def ema_combine(params, partial: np.ndarray, state: np.ndarray) -> np.ndarray:
     partial = np.ascontiguousarray(partial, dtype=np.float32)
-    rows = np.zeros(params.frames * params.channels, dtype=np.float32)
+    rows = np.zeros((params.frames // max(1, params.row_frames)) * params.channels,
+                    dtype=np.float32)
     _check(lib().oracle_ema_combine(C.byref(params), _a(partial), u64(partial.size),
                                     _a(state), _a(rows), u64(rows.size)), "ema_combine")
     return rows
 
 
 def chain(desc: OChain, packed: np.ndarray, taps: np.ndarray, twiddles: np.ndarray,
-          threads: int = 1) -> tuple[np.ndarray, np.ndarray | None]:
+          threads: int = 1, ring: np.ndarray | None = None,
+          partial: np.ndarray | None = None) -> tuple[np.ndarray, np.ndarray | None]:
     """The plain chain over absolute blocks first_block .. + block_count.
-    Returns (channel ring float32 pairs M x out_ring_blocks, EMA partials)."""
+    Returns (channel ring float32 pairs M x out_ring_blocks, EMA partials);
+    `ring` / `partial` may be passed in preallocated."""
     taps = np.ascontiguousarray(taps, dtype=np.float32)
     tw = f32c(twiddles)
     M = desc.channels
-    ring = np.zeros(2 * M * desc.out_ring_blocks, dtype=np.float32)
-    partial = None
+    if ring is None:
+        ring = np.zeros(2 * M * desc.out_ring_blocks, dtype=np.float32)
     np_ = 0
     if desc.frame_blocks:
-        partial = np.zeros((desc.block_count // desc.frame_blocks) * M, dtype=np.float32)
+        if partial is None:
+            partial = np.zeros((desc.block_count // desc.frame_blocks) * M, dtype=np.float32)
         np_ = partial.size
     _check(lib().oracle_chain(C.byref(desc), _a(packed), u64(packed.nbytes), _a(taps),
                               u64(taps.size), _a(tw), u64(tw.size // 2), _a(ring),

@@ -79,6 +79,8 @@ struct OLevelsParams {  // core/dsp/spectrum_levels_reference.h
 struct OEmaParams {
     uint32_t channels, chan_blocks, chan_mask, in_offset, frame_blocks, frames;
     float alpha, frame_decay;
+    uint32_t row_frames;
+    float row_decay;
 };
 struct OFineConfig {  // core/dsp/vrx_reference.h:178-195
     uint32_t taps, phases, nco_log2;
@@ -524,9 +526,10 @@ int oracle_spectrum_levels(const OLevelsParams* p, const float* frame, uint64_t 
 // Partial: for each frame f of frame_blocks consecutive channel samples and
 // each channel k, s starts at 0 and for ascending blocks
 //     x = re*re + im*im;  s = s + (x - s) * alpha
-// Combine: P = P * frame_decay + s for ascending frames, frame_decay being
-// (1 - alpha)^frame_blocks rounded once to float. P after frame f is frame
-// f's row of the averaged spectrum.
+// Row: S = 0 and S = S * frame_decay + s over a row's row_frames frames in
+// ascending order. Combine: P = P * row_decay + S over ascending rows; P after
+// row r is row r of the averaged spectrum. frame_decay = (1-alpha)^F and
+// row_decay = (1-alpha)^(F*G), each rounded once to float.
 // Pinned by: the FP64 exponential average of the same powers (relative error
 // bound), a constant input converging to its value, and alpha = 1 reducing to
 // the last power (tests/test_oracle_ema.py).
@@ -557,14 +560,20 @@ int oracle_ema_partial(const OEmaParams* p, const float* ring, uint64_t ring_sam
 
 int oracle_ema_combine(const OEmaParams* p, const float* partial, uint64_t npartial, float* state,
                        float* rows, uint64_t nrows) {
+    const uint32_t G = p->row_frames;
+    if (G == 0 || p->frames % G != 0) return -1;
+    const uint32_t R = p->frames / G;
     if (npartial < uint64_t(p->frames) * p->channels) return -1;
-    if (rows != nullptr && nrows < uint64_t(p->frames) * p->channels) return -1;
+    if (rows == nullptr || nrows < uint64_t(R) * p->channels) return -1;
     FlushGuard guard;
     for (uint32_t k = 0; k < p->channels; ++k) {
         float P = state[k];
-        for (uint32_t f = 0; f < p->frames; ++f) {
-            P = P * p->frame_decay + partial[uint64_t(f) * p->channels + k];
-            if (rows != nullptr) rows[uint64_t(f) * p->channels + k] = P;
+        for (uint32_t r = 0; r < R; ++r) {
+            float S = 0.0f;
+            for (uint32_t g = 0; g < G; ++g)
+                S = S * p->frame_decay + partial[uint64_t(r * G + g) * p->channels + k];
+            P = P * p->row_decay + S;
+            rows[uint64_t(r) * p->channels + k] = P;
         }
         state[k] = P;
     }
@@ -828,7 +837,7 @@ static int chain_range(const OChain* c, const uint8_t* input, uint64_t input_byt
         const uint32_t frames = nb / c->frame_blocks;
         OEmaParams ep{M, c->out_ring_blocks, c->out_ring_blocks - 1,
                       uint32_t(c->first_block + b0), c->frame_blocks, frames, c->alpha,
-                      c->frame_decay};
+                      c->frame_decay, 1, c->frame_decay};
         if (frames && oracle_ema_partial(&ep, ring_out, uint64_t(M) * c->out_ring_blocks,
                                          partial + (b0 / c->frame_blocks) * M,
                                          uint64_t(frames) * M))

@@ -51,7 +51,8 @@ class SpectrumLevelsParams(C.Structure):
 
 class EmaParams(C.Structure):
     _fields_ = [("channels", u32), ("chan_blocks", u32), ("chan_mask", u32), ("in_offset", u32),
-                ("frame_blocks", u32), ("frames", u32), ("alpha", f32), ("frame_decay", f32)]
+                ("frame_blocks", u32), ("frames", u32), ("alpha", f32), ("frame_decay", f32),
+                ("row_frames", u32), ("row_decay", f32)]
 
 
 class VrxFineConfig(C.Structure):

@@ -334,14 +334,40 @@ __global__ void ema_partial_kernel(const float2* __restrict__ ring, rvx_ema_para
     partial[uint64_t(f) * p.channels + k] = s;
 }
 
-__global__ void ema_combine_kernel(const float* __restrict__ partial, rvx_ema_params p,
-                                   float* __restrict__ state, float* __restrict__ rows) {
+// Row partials, in place into the rows buffer: S_r from zero over its frames.
+__global__ void ema_rows_kernel(const float* __restrict__ partial, rvx_ema_params p,
+                                float* __restrict__ rows, uint32_t nrows) {
+    const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= uint64_t(nrows) * p.channels) return;
+    const uint32_t r = uint32_t(id / p.channels), k = uint32_t(id % p.channels);
+    float S = 0.0f;
+    for (uint32_t g = 0; g < p.row_frames; ++g)
+        S = S * p.frame_decay + partial[uint64_t(r * p.row_frames + g) * p.channels + k];
+    rows[id] = S;
+}
+
+// The serial part, one step per row: P = P * row_decay + S_r. Row values are
+// fetched kBatch at a time so the recurrence does not wait on each load.
+__global__ void ema_combine_kernel(rvx_ema_params p, float* __restrict__ state,
+                                   float* __restrict__ rows, uint32_t nrows) {
+    constexpr uint32_t kBatch = 16;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= p.channels) return;
     float P = state[k];
-    for (uint32_t f = 0; f < p.frames; ++f) {
-        P = P * p.frame_decay + partial[uint64_t(f) * p.channels + k];
-        if (rows) rows[uint64_t(f) * p.channels + k] = P;
+    uint32_t r = 0;
+    for (; r + kBatch <= nrows; r += kBatch) {
+        float S[kBatch];
+#pragma unroll
+        for (uint32_t i = 0; i < kBatch; ++i) S[i] = rows[uint64_t(r + i) * p.channels + k];
+#pragma unroll
+        for (uint32_t i = 0; i < kBatch; ++i) {
+            P = P * p.row_decay + S[i];
+            rows[uint64_t(r + i) * p.channels + k] = P;
+        }
+    }
+    for (; r < nrows; ++r) {
+        P = P * p.row_decay + rows[uint64_t(r) * p.channels + k];
+        rows[uint64_t(r) * p.channels + k] = P;
     }
     state[k] = P;
 }
@@ -594,12 +620,20 @@ int32_t rvx_ema_combine(rvx_ctx* ctx, const rvx_ema_params* params, const float*
     if (params == nullptr) return rvx::fail(RVX_EINVAL, "%s: null params", who);
     const rvx_ema_params p = *params;
     if (p.channels == 0) return rvx::fail(RVX_EINVAL, "%s: no channels", who);
-    if (n_partial < uint64_t(p.frames) * p.channels ||
-        (rows != nullptr && n_rows < uint64_t(p.frames) * p.channels))
+    if (p.row_frames == 0 || p.frames % p.row_frames != 0)
+        return rvx::fail(RVX_EINVAL, "%s: %u frames are not whole rows of %u", who, p.frames,
+                         p.row_frames);
+    const uint32_t nrows = p.frames / p.row_frames;
+    if (n_partial < uint64_t(p.frames) * p.channels || rows == nullptr ||
+        n_rows < uint64_t(nrows) * p.channels)
         return rvx::fail(RVX_EINVAL, "%s: buffer sizes do not match the params", who);
     if (!partial || !state) return rvx::fail(RVX_EINVAL, "%s: null buffer", who);
-    ema_combine_kernel<<<dim3((p.channels + 255) / 256), dim3(256), 0, ctx->stream>>>(
-        partial, p, state, rows);
+    if (nrows == 0) return RVX_OK;
+    const uint64_t n = uint64_t(nrows) * p.channels;
+    ema_rows_kernel<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, ctx->stream>>>(partial, p,
+                                                                                    rows, nrows);
+    ema_combine_kernel<<<dim3((p.channels + 63) / 64), dim3(64), 0, ctx->stream>>>(p, state, rows,
+                                                                                  nrows);
     return rvx::check_launch(who);
 }
 

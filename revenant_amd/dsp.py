@@ -107,6 +107,15 @@ def ema_frame_decay(alpha: float, frame_blocks: int) -> np.float32:
     return np.float32((1.0 - float(np.float32(alpha))) ** frame_blocks)
 
 
+def ema_params(channels: int, chan_blocks: int, in_offset: int, frame_blocks: int, frames: int,
+               alpha: float, row_frames: int = 1) -> "_lib.EmaParams":
+    """The averaged-spectrum parameter block with both decay factors filled in."""
+    a = np.float32(alpha)
+    return _lib.EmaParams(channels, chan_blocks, chan_blocks - 1, in_offset, frame_blocks,
+                          frames, a, ema_frame_decay(a, frame_blocks), row_frames,
+                          ema_frame_decay(a, frame_blocks * row_frames))
+
+
 # ---------------------------------------------------------------------------
 # Receivers (core/engine/vrx_place.cpp, core/dsp/vrx_reference.cpp)
 # ---------------------------------------------------------------------------

@@ -9,7 +9,7 @@ from helpers import SEED_SPECTRUM, adversarial, assert_bit_exact, from_dev, rng,
     uniform_complex
 from oracle import oracle
 from revenant_amd import _lib
-from revenant_amd.dsp import build_spectrum_window, build_twiddles, ema_frame_decay
+from revenant_amd.dsp import build_spectrum_window, build_twiddles, ema_params
 
 pytestmark = pytest.mark.gpu
 
@@ -41,14 +41,13 @@ def test_levels_match_twin(dev, bins):
     assert_bit_exact(from_dev(out), cpu, what=f"levels {bins}")
 
 
-@pytest.mark.parametrize("M,frame_blocks,frames", [(64, 16, 32), (4096, 16, 8), (5, 3, 7)])
-def test_ema_matches_twin(dev, M, frame_blocks, frames):
+@pytest.mark.parametrize("M,frame_blocks,frames,row_frames",
+                         [(64, 16, 32, 1), (4096, 4, 64, 4), (5, 3, 7, 7), (33, 2, 60, 3)])
+def test_ema_matches_twin(dev, M, frame_blocks, frames, row_frames):
     blocks = 1 << (frame_blocks * frames - 1).bit_length()
     ring = adversarial(SEED_SPECTRUM + M, M * blocks)
     ring[np.abs(ring) > 1e18] = 0.5  # keep |x|^2 finite so the recurrence is exercised
-    alpha = np.float32(1.0 / 16.0)
-    p = _lib.EmaParams(M, blocks, blocks - 1, 3, frame_blocks, frames, alpha,
-                       ema_frame_decay(alpha, frame_blocks))
+    p = ema_params(M, blocks, 3, frame_blocks, frames, 1.0 / 16.0, row_frames)
     part_cpu = oracle.ema_partial(p, ring)
     state_cpu = np.linspace(0.0, 1.0, M, dtype=np.float32)
     rows_cpu = oracle.ema_combine(p, part_cpu, state_cpu)
@@ -56,7 +55,7 @@ def test_ema_matches_twin(dev, M, frame_blocks, frames):
     dev.ema_partial(p, to_dev(ring), part)
     assert_bit_exact(from_dev(part), part_cpu, what="ema partial")
     state = to_dev(np.linspace(0.0, 1.0, M, dtype=np.float32))
-    rows = torch.zeros(frames * M, dtype=torch.float32, device="cuda")
+    rows = torch.zeros((frames // row_frames) * M, dtype=torch.float32, device="cuda")
     dev.ema_combine(p, part, state, rows)
     assert_bit_exact(from_dev(rows), rows_cpu, what="ema rows")
     assert_bit_exact(from_dev(state), state_cpu, what="ema state")

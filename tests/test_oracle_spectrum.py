@@ -9,7 +9,7 @@ import pytest
 
 from oracle import oracle
 from revenant_amd import _lib
-from revenant_amd.dsp import build_spectrum_window, build_twiddles, ema_frame_decay
+from revenant_amd.dsp import build_spectrum_window, build_twiddles, ema_params
 
 
 def test_window_unit_coherent_gain():
@@ -60,36 +60,38 @@ def test_levels_bucket_boundaries():
     assert lv[1] == np.float32(-200.0 + 1023.5 * 0.234375)
 
 
-def test_ema_against_fp64_definition():
-    """Frame partial + recurrence equals the FP64 exponential average of the
-    same powers to float rounding."""
-    M, F, frames = 16, 16, 40
+@pytest.mark.parametrize("F,G", [(16, 1), (4, 4), (2, 8), (1, 16)])
+def test_ema_against_fp64_definition(F, G):
+    """Frame partial + row partial + recurrence equals the FP64 exponential
+    average of the same powers to float rounding, for any frame and row size;
+    a mistaken decay factor or a dropped term fails by orders of magnitude."""
+    M, frames = 16, 640 // F
     blocks = 1024
     g = np.random.default_rng(3)
     ring = (g.standard_normal((M, blocks)) + 1j * g.standard_normal((M, blocks))).astype(
         np.complex64)
-    alpha = np.float32(1 / 16)
-    p = _lib.EmaParams(M, blocks, blocks - 1, 0, F, frames, alpha, ema_frame_decay(alpha, F))
+    p = ema_params(M, blocks, 0, F, frames, 1 / 16, G)
     part = oracle.ema_partial(p, ring.view(np.float32).ravel())
     state = np.zeros(M, np.float32)
-    rows = oracle.ema_combine(p, part, state).reshape(frames, M)
+    rows = oracle.ema_combine(p, part, state).reshape(frames // G, M)
     x = np.abs(ring.astype(np.complex128)) ** 2
     P = np.zeros(M)
-    for f in range(frames):
-        for i in range(F):
-            P = P + (x[:, f * F + i] - P) / 16.0
-        assert np.max(np.abs(rows[f] - P) / P) < 2e-6
+    for r in range(frames // G):
+        for i in range(F * G):
+            P = P + (x[:, r * F * G + i] - P) / 16.0
+        assert np.max(np.abs(rows[r] - P) / P) < 3e-6
+    assert np.array_equal(state, rows[-1])
 
 
 def test_ema_alpha_one_and_constant_input():
     M, F = 4, 8
     ring = np.full((M, 64), 2.0 + 0j, np.complex64)
     ring[:, 7] = 3.0
-    p = _lib.EmaParams(M, 64, 63, 0, F, 1, np.float32(1.0), np.float32(0.0))
+    p = ema_params(M, 64, 0, F, 1, 1.0)
     assert np.all(oracle.ema_partial(p, ring.view(np.float32).ravel()) == 9.0)
     ring[:, 7] = 2.0
     alpha = np.float32(1 / 16)
-    p = _lib.EmaParams(M, 64, 63, 0, F, 8, alpha, ema_frame_decay(alpha, F))
+    p = ema_params(M, 64, 0, F, 8, alpha, 2)
     part = oracle.ema_partial(p, ring.view(np.float32).ravel())
-    rows = oracle.ema_combine(p, part, np.zeros(M, np.float32)).reshape(8, M)
+    rows = oracle.ema_combine(p, part, np.zeros(M, np.float32)).reshape(4, M)
     assert abs(rows[-1, 0] - 4.0 * (1 - (15 / 16) ** 64)) < 1e-5
