@@ -80,9 +80,9 @@ def main() -> int:
     print(f"[bench] rendering {SAMPLES} cs16 samples (400 carriers + AWGN) on the GPU",
           file=sys.stderr, flush=True)
     scene = synth.wideband_scene(RATE, 400, SEED)
-    x = synth.render(scene, RATE, 0, SAMPLES, noise_dbfs=-90.0, seed=SEED, device="cuda")
-    packed = synth.quantize_cs16(x)
-    del x
+    packed = torch.empty(4 * SAMPLES, dtype=torch.uint8, device="cuda")
+    synth.render_device(dev, scene, RATE, 0, SAMPLES, noise_dbfs=-60.0, seed=SEED,
+                        fmt=FORMAT_CS16, out=packed)
     torch.cuda.synchronize()
 
     N = grid.prototype_length

@@ -375,6 +375,41 @@ int32_t rvx_vrx_demod(rvx_ctx* ctx, const rvx_vrx_demod_config* config,
                       uint64_t fine_samples, const float* weights, uint64_t n_weights,
                       float* audio, uint64_t n_audio);
 
+/* Synthetic IQ source (the inputs of tests and benchmarks; after
+ * core/dsp/synth, whose scenes are pure functions of the absolute index).
+ * Each sample n = start + i is the FP64 sum over carriers of
+ *   env * exp(j 2 pi (f n / rate + fm(n)))   with phases reduced exactly
+ * plus complex Gaussian noise (Box-Muller over a splitmix64 hash of seed and
+ * n), rounded to float and quantised: cs16 (x 32768, saturated), cu8
+ * (127.5 + 127.5 x, saturated) or format -1 for cf32. */
+#define RVX_CARRIER_CW 0
+#define RVX_CARRIER_AM 1
+#define RVX_CARRIER_FM 2
+
+typedef struct rvx_carrier {
+    int32_t kind;        /* RVX_CARRIER_* */
+    int32_t reserved;
+    int64_t freq_hz;     /* offset from the tuned centre */
+    int64_t tone_hz;     /* modulating tone */
+    double amplitude;    /* linear, full scale 1 */
+    double index;        /* FM: peak phase deviation in radians (deviation / tone) */
+    double depth;        /* AM: modulation depth */
+} rvx_carrier;
+
+typedef struct rvx_synth_params {
+    int64_t rate;
+    uint64_t start;
+    uint64_t count;
+    uint64_t seed;
+    double noise_sigma;  /* per component */
+    int32_t format;      /* RVX_FORMAT_CS16, RVX_FORMAT_CU8 or -1 (cf32) */
+    int32_t reserved;
+} rvx_synth_params;
+
+/* Renders the scene into `out` (count samples) and waits for it. */
+int32_t rvx_synth(rvx_ctx* ctx, const rvx_carrier* carriers, uint32_t ncarriers,
+                  const rvx_synth_params* params, void* out, uint64_t out_bytes);
+
 /* The fused plain chain: convert -> pfb_branch -> pfb_fft (-> ema_partial) in
  * one launch, bit-identical to running the four operators in sequence
  * (core/engine/graph.cpp:5120-5196 records them as separate dispatches).

@@ -75,6 +75,16 @@ class VrxDemodParams(C.Structure):
     _fields_ = [("in_mask", u32), ("in_offset", u32), ("count", u32), ("gain", f32)]
 
 
+class Carrier(C.Structure):
+    _fields_ = [("kind", i32), ("reserved", i32), ("freq_hz", i64), ("tone_hz", i64),
+                ("amplitude", f64), ("index", f64), ("depth", f64)]
+
+
+class SynthParams(C.Structure):
+    _fields_ = [("rate", i64), ("start", u64), ("count", u64), ("seed", u64),
+                ("noise_sigma", f64), ("format", i32), ("reserved", i32)]
+
+
 class ChainDesc(C.Structure):
     _fields_ = [("format", i32), ("channels", u32), ("taps_per_branch", u32),
                 ("decimation", u32), ("input_origin", u64), ("input_samples", u64),
@@ -152,6 +162,7 @@ _SIGNATURES = {
                              C.POINTER(VrxFineParams), C.POINTER(u64)]),
     "rvx_demod_block": (i32, [C.POINTER(VrxPlan), u32, u64, u32, C.POINTER(VrxDemodParams)]),
     "rvx_demod_history": (u32, [C.POINTER(VrxDemodConfig)]),
+    "rvx_synth": (i32, [vp, C.POINTER(Carrier), u32, C.POINTER(SynthParams), vp, u64]),
     "rvx_chain": (i32, [vp, C.POINTER(ChainDesc), vp, vp, u64, vp, u64, vp, u64, vp, u64]),
 }
 

@@ -95,3 +95,77 @@ def quantize_cu8(x: torch.Tensor) -> torch.Tensor:
     """complex -> offset-binary uint8 pairs (code = 127.5 + 127.5 * x)."""
     iq = torch.view_as_real(x).reshape(-1) * 127.5 + 127.5
     return torch.clamp(torch.round(iq), 0, 255).to(torch.uint8)
+
+
+# ---------------------------------------------------------------------------
+# Device render (rvx_synth): the same scene, one thread per sample.
+# ---------------------------------------------------------------------------
+
+_KIND = {"cw": 0, "am": 1, "nfm": 2, "wfm": 2}
+
+
+def carrier_table(carriers: list[Carrier]):
+    """The scene as the C ABI's rvx_carrier array."""
+    from . import _lib
+
+    arr = (_lib.Carrier * max(1, len(carriers)))()
+    for i, c in enumerate(carriers):
+        index = (c.deviation_hz / c.tone_hz) if c.kind in ("nfm", "wfm") else 0.0
+        arr[i] = _lib.Carrier(_KIND[c.kind], 0, int(c.freq_hz), int(c.tone_hz),
+                              10.0 ** (c.level_dbfs / 20.0), index, float(c.depth))
+    return arr
+
+
+def render_device(dev, carriers: list[Carrier], rate: int, start: int, count: int,
+                  noise_dbfs: float | None, seed: int, fmt: int, out: torch.Tensor) -> None:
+    """Render samples start .. start+count into `out` (a CUDA tensor): native
+    cs16 / cu8 codes, or cf32 with fmt = -1."""
+    import ctypes as C
+
+    from . import _lib
+
+    table = carrier_table(carriers)
+    sigma = 0.0 if noise_dbfs is None else 10.0 ** (noise_dbfs / 20.0) / math.sqrt(2.0)
+    p = _lib.SynthParams(rate, start, count, seed & (2 ** 64 - 1), sigma, fmt, 0)
+    _lib.check(dev.lib.rvx_synth(dev.ctx, table, len(carriers), C.byref(p), out.data_ptr(),
+                                 out.numel() * out.element_size()))
+
+
+def render_numpy(carriers: list[Carrier], rate: int, start: int, count: int,
+                 noise_dbfs: float | None, seed: int) -> np.ndarray:
+    """FP64 CPU evaluation of exactly what rvx_synth computes, before rounding
+    to float (for spot checks of the device render)."""
+    n = np.arange(start, start + count, dtype=np.uint64)
+    acc = np.zeros(count, dtype=np.complex128)
+
+    def turns(f):
+        fr = np.uint64(int(f) % rate)
+        return ((fr * (n % np.uint64(rate))) % np.uint64(rate)).astype(np.float64) / rate
+
+    for c in carriers:
+        amp = 10.0 ** (c.level_dbfs / 20.0)
+        ph = turns(c.freq_hz)
+        env = amp
+        if c.kind in ("nfm", "wfm"):
+            ph = ph + (c.deviation_hz / c.tone_hz) * np.sin(2 * np.pi * turns(c.tone_hz)) / (
+                2 * np.pi)
+        elif c.kind == "am":
+            env = amp * (1.0 + c.depth * np.cos(2 * np.pi * turns(c.tone_hz))) / (1.0 + c.depth)
+        acc += env * np.exp(2j * np.pi * ph)
+    if noise_dbfs is not None:
+        sigma = 10.0 ** (noise_dbfs / 20.0) / math.sqrt(2.0)
+
+        def mix(x):
+            x = x + np.uint64(0x9E3779B97F4A7C15)
+            x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            return x ^ (x >> np.uint64(31))
+
+        with np.errstate(over="ignore"):
+            h = mix(np.uint64(seed & (2 ** 64 - 1)) ^ mix(n))
+            h2 = mix(h)
+        u1 = ((h >> np.uint64(11)) + np.uint64(1)).astype(np.float64) / 9007199254740992.0
+        u2 = ((h2 >> np.uint64(11)) + np.uint64(1)).astype(np.float64) / 9007199254740992.0
+        r = np.sqrt(-2.0 * np.log(u1))
+        acc += sigma * r * np.exp(2j * np.pi * u2)
+    return acc
