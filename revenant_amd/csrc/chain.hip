@@ -18,9 +18,10 @@
 // frame, or a single block) and walks it in order, so a frame's average
 // accumulates in the order its definition fixes. Thread t of a transform owns
 // V branches. Chunks are numbered so that the workgroups one XCD runs at the
-// same time own neighbouring blocks: they share the input window in L2, and
-// their channel-ring writes, 8 bytes per channel per block, land in the same
-// 128-byte lines while those are still in L2.
+// same time own neighbouring blocks: they share the input window in L2. A
+// workgroup stages four blocks of outputs in LDS and writes each channel's
+// four samples as one 32-byte run (the channel ring is channel-major, so a
+// block's outputs alone would be 4096 scattered 8-byte writes).
 
 #pragma clang fp contract(off)
 
@@ -119,6 +120,10 @@ __device__ __forceinline__ float2 load_sample(const void* input, uint32_t index)
 // once per tap.
 constexpr int kPrefetch = 4;
 
+// Blocks of channel outputs staged in LDS before they are written: four
+// blocks make a 32-byte run per channel, a whole memory sector.
+constexpr int kStage = 4;
+
 // TAPS > 0: the tap count is a compile-time constant; REGTAPS then keeps the
 // thread's V*TAPS taps in registers for the whole chunk, otherwise they are
 // read (L1/L2-resident) alongside the samples, through the same prefetch.
@@ -135,8 +140,17 @@ chain_kernel(ChainArgs a) {
     constexpr int LAST = F::PASSES - 1;
     constexpr int NT = TAPS > 0 ? TAPS : 1;
 
-    __shared__ float2 lds[S::BPW * M];
-    __shared__ float pow_lds[S::BPW > 1 ? S::BPW * M : 1];
+    // One transform's LDS per block in flight; with one block in flight, up
+    // to kStage blocks of channel outputs are staged so that each channel's
+    // outputs leave as one 32-byte run; with several, their powers are folded
+    // in block order through LDS.
+    constexpr int STAGE = S::BPW == 1 ? kStage : 1;
+    constexpr uint32_t kLdsFloat2 =
+        S::BPW * M + (STAGE > 1 ? STAGE * M : 0) + (S::BPW > 1 ? (S::BPW * M + 1) / 2 : 0);
+    __shared__ float2 smem[kLdsFloat2];
+    float2* lds = smem;
+    float2* stage = smem + S::BPW * M;
+    float* pow_lds = reinterpret_cast<float*>(smem + S::BPW * M);
 
     // XCD-aware chunk numbering: the hardware deals workgroups to the 8 XCDs
     // round robin, so workgroups w, w+8, w+16, ... share an XCD. Give each
@@ -305,8 +319,38 @@ chain_kernel(ChainArgs a) {
             const uint32_t k = F::template pos<LAST>(t, r);
             float2 value = v[r];
             if (a.decimation != M) value = rvx::cmul_exact(value, tw[(k * dm) & (M - 1)]);
-            if (active) a.ring[uint64_t(k) * a.ring_blocks + slot] = value;
+            if constexpr (STAGE > 1) {
+                stage[(b0 % STAGE) * M + k] = value;
+            } else {
+                if (active) a.ring[uint64_t(k) * a.ring_blocks + slot] = value;
+            }
             x[r] = value.x * value.x + value.y * value.y;
+        }
+        if constexpr (STAGE > 1) {
+            // Flush when the stage is full or the chunk ends. Every thread
+            // flushes exactly the channels it staged, so no barrier is needed.
+            const uint32_t last = min(a.chunk, a.block_count - first) - 1;
+            if (b0 % STAGE == STAGE - 1 || b0 == last) {
+                const uint32_t n = b0 % STAGE + 1;
+                const uint32_t m0 = m_lo - (n - 1);
+                const uint32_t s0 = m0 & a.ring_mask;
+                const bool whole = n == STAGE && (s0 % STAGE) == 0 && STAGE == 4;
+#pragma unroll
+                for (int r = 0; r < V; ++r) {
+                    const uint32_t k = F::template pos<LAST>(t, r);
+                    float2* row = a.ring + uint64_t(k) * a.ring_blocks;
+                    if (whole) {
+                        const float2 v0 = stage[0 * M + k], v1 = stage[1 * M + k];
+                        const float2 v2 = stage[2 * M + k], v3 = stage[3 * M + k];
+                        float4* dst = reinterpret_cast<float4*>(row + s0);
+                        dst[0] = make_float4(v0.x, v0.y, v1.x, v1.y);
+                        dst[1] = make_float4(v2.x, v2.y, v3.x, v3.y);
+                    } else {
+                        for (uint32_t j = 0; j < n; ++j)
+                            row[(m0 + j) & a.ring_mask] = stage[j * M + k];
+                    }
+                }
+            }
         }
         if constexpr (S::BPW == 1) {
 #pragma unroll
@@ -416,11 +460,12 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
                          (unsigned long long)d.first_block);
     if (last_base >= d.input_origin + d.input_samples)
         return rvx::fail(RVX_EINVAL, "%s: the last block needs samples past the input's end", who);
-    // One block per workgroup unless averaging: the XCD numbering then hands
-    // an XCD's concurrent workgroups consecutive blocks, so the channel
-    // ring's 128-byte lines (16 blocks of one channel) fill while they are
-    // still in that XCD's L2.
-    uint32_t chunk = 1;
+    // Four blocks per workgroup unless averaging (then one frame): the XCD
+    // numbering hands an XCD's concurrent workgroups consecutive chunks, and
+    // each writes a 32-byte run per channel, so the channel ring's 128-byte
+    // lines (16 blocks of one channel) fill from a few workgroups while they
+    // are still in that XCD's L2.
+    uint32_t chunk = 4;
     if (d.frame_blocks != 0) {
         if (d.block_count % d.frame_blocks)
             return rvx::fail(RVX_EINVAL, "%s: frame of %u blocks does not divide %u blocks", who,
