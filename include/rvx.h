@@ -375,6 +375,59 @@ int32_t rvx_vrx_demod(rvx_ctx* ctx, const rvx_vrx_demod_config* config,
                       uint64_t fine_samples, const float* weights, uint64_t n_weights,
                       float* audio, uint64_t n_audio);
 
+/* Batched receivers: many VRXs of any mix of modes and rates in one
+ * persistent launch, each computing fine stage -> detector -> audio filter on
+ * chip per tile of audio outputs, bit-identical to rvx_vrx_fine followed by
+ * rvx_vrx_demod. No counterpart in the reference, which records two or three
+ * dispatches per receiver (core/engine/vrx_stage.cpp:1306,1346,1399).
+ *
+ * One descriptor per receiver and step. `fine` holds the params of the fine
+ * block the step needs: fine outputs F0 .. F0 + (count-1)*R + hist, where F0
+ * = first_audio * R - hist and hist = rvx_demod_history(&demod); its
+ * out_offset/out_mask are unused (fine samples stay on chip). `demod.count`
+ * audio outputs are written to `audio` (count * channels floats); its
+ * in_offset/in_mask are unused. `tile` is the audio outputs one work item
+ * computes (chosen by the tuner). */
+typedef struct rvx_vrx_desc {
+    rvx_vrx_fine_config fine_config;
+    rvx_vrx_demod_config demod_config;
+    rvx_vrx_fine_params fine;
+    rvx_vrx_demod_params demod;
+    uint32_t hist;
+    uint32_t tile;
+    uint32_t reserved[2];
+    const void* channel_ring;  /* device, cf32; fine.chan_base/chan_mask index it */
+    const void* taps;          /* device, fine_config.phases*taps + 1 cf32 */
+    const float* weights;      /* device, demod weights */
+    float* audio;              /* device, demod.count * channels */
+} rvx_vrx_desc;
+
+/* Work list: (receiver, tile) pairs in the order the persistent workgroups
+ * take them; the host orders them largest-first. */
+typedef struct rvx_vrx_work {
+    uint32_t vrx;
+    uint32_t tile;
+} rvx_vrx_work;
+
+/* descs / work / counter are device memory (descs and work written by the
+ * caller, e.g. with rvx_buf_write; the counter is one uint32 the launch
+ * zeroes itself, so a captured graph replays correctly). nco: 2^16 cf32. */
+int32_t rvx_vrx_batch(rvx_ctx* ctx, const rvx_vrx_desc* descs, uint32_t n_desc,
+                      const rvx_vrx_work* work, uint32_t n_work, uint32_t* counter,
+                      const void* nco, uint64_t n_nco, uint32_t workgroups);
+
+/* Host-side check of one descriptor (sizes are the caller's to guarantee for
+ * device pointers; this checks the parameter rules of rvx_vrx_fine/demod and
+ * that a tile fits the LDS budget). */
+int32_t rvx_vrx_desc_check(const rvx_vrx_desc* desc);
+
+/* Graph capture of the context's own stream (HIP graphs): begin, end, launch. */
+typedef struct rvx_graph rvx_graph;
+int32_t rvx_graph_begin(rvx_ctx* ctx);
+int32_t rvx_graph_end(rvx_ctx* ctx, rvx_graph** out);
+int32_t rvx_graph_launch(rvx_ctx* ctx, rvx_graph* graph);
+void rvx_graph_destroy(rvx_graph* graph);
+
 /* Synthetic IQ source (the inputs of tests and benchmarks; after
  * core/dsp/synth, whose scenes are pure functions of the absolute index).
  * Each sample n = start + i is the FP64 sum over carriers of

@@ -114,6 +114,18 @@ class VrxPlan(C.Structure):
                 ("audio_pass_hz", f64), ("audio_stop_hz", f64)]
 
 
+class VrxDesc(C.Structure):
+    _fields_ = [("fine_config", VrxFineConfig), ("demod_config", VrxDemodConfig),
+                ("fine", VrxFineParams), ("demod", VrxDemodParams), ("hist", u32), ("tile", u32),
+                ("reserved", u32 * 2), ("channel_ring", vp), ("taps", vp), ("weights", vp),
+                ("audio", vp)]
+
+
+class VrxWork(C.Structure):
+    _fields_ = [("vrx", u32), ("tile", u32)]
+
+
+assert C.sizeof(VrxDesc) == 160 and C.sizeof(VrxWork) == 8
 assert C.sizeof(VrxFineParams) == 60 and C.sizeof(VrxDemodParams) == 16
 assert C.sizeof(ConvertParams) == 16 and C.sizeof(PfbBranchParams) == 12
 
@@ -164,6 +176,12 @@ _SIGNATURES = {
     "rvx_demod_history": (u32, [C.POINTER(VrxDemodConfig)]),
     "rvx_synth": (i32, [vp, C.POINTER(Carrier), u32, C.POINTER(SynthParams), vp, u64]),
     "rvx_chain": (i32, [vp, C.POINTER(ChainDesc), vp, vp, u64, vp, u64, vp, u64, vp, u64]),
+    "rvx_vrx_batch": (i32, [vp, vp, u32, vp, u32, vp, vp, u64, u32]),
+    "rvx_vrx_desc_check": (i32, [C.POINTER(VrxDesc)]),
+    "rvx_graph_begin": (i32, [vp]),
+    "rvx_graph_end": (i32, [vp, C.POINTER(vp)]),
+    "rvx_graph_launch": (i32, [vp, vp]),
+    "rvx_graph_destroy": (None, [vp]),
 }
 
 _lib = None

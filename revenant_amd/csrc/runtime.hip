@@ -206,3 +206,58 @@ int32_t rvx_timer_end(rvx_ctx* ctx, float* elapsed_ms) {
 }
 
 }  // extern "C"
+
+// HIP graphs over the context's own stream: a step's launches are captured
+// once and replayed with one call. Capture needs a stream that is not the
+// null stream, so begin switches nothing: it fails unless the context runs on
+// its own stream or a caller's stream.
+struct rvx_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int32_t rvx_graph_begin(rvx_ctx* ctx) {
+    static const char* who = "rvx_graph_begin";
+    RVX_ENTER(ctx, who);
+    if (ctx->stream == nullptr)
+        return rvx::fail(RVX_EINVAL, "%s: the null stream cannot be captured", who);
+    RVX_TRY_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), who);
+    return RVX_OK;
+}
+
+int32_t rvx_graph_end(rvx_ctx* ctx, rvx_graph** out) {
+    static const char* who = "rvx_graph_end";
+    RVX_ENTER(ctx, who);
+    if (out == nullptr) return rvx::fail(RVX_EINVAL, "%s: null output", who);
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    RVX_TRY_HIP(hipStreamEndCapture(ctx->stream, &g), who);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t err = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (err != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return rvx::fail(RVX_EDEVICE, "%s: hipGraphInstantiate failed: %s", who,
+                         hipGetErrorString(err));
+    }
+    *out = new rvx_graph{g, exec};
+    return RVX_OK;
+}
+
+int32_t rvx_graph_launch(rvx_ctx* ctx, rvx_graph* graph) {
+    static const char* who = "rvx_graph_launch";
+    RVX_ENTER(ctx, who);
+    if (graph == nullptr) return rvx::fail(RVX_EINVAL, "%s: null graph", who);
+    RVX_TRY_HIP(hipGraphLaunch(graph->exec, ctx->stream), who);
+    return RVX_OK;
+}
+
+void rvx_graph_destroy(rvx_graph* graph) {
+    if (graph == nullptr) return;
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    delete graph;
+}
+
+}  // extern "C"

@@ -19,90 +19,20 @@
 
 #include "common.h"
 #include "plan.h"
+#include "vrx_math.h"
 
 namespace {
 
-__device__ __forceinline__ float det_recip(float v) {
-    const float seed = __uint_as_float(0x7EF311C3u - __float_as_uint(v));
-    const float r1 = seed * (2.0f - v * seed);
-    const float r2 = r1 * (2.0f - v * r1);
-    const float r3 = r2 * (2.0f - v * r2);
-    const float r4 = r3 * (2.0f - v * r3);
-    return r4;
-}
-__device__ __forceinline__ float det_rsqrt(float v) {
-    const float seed = __uint_as_float(0x5F375A86u - (__float_as_uint(v) >> 1));
-    const float h = 0.5f * v;
-    const float r1 = seed * (1.5f - h * (seed * seed));
-    const float r2 = r1 * (1.5f - h * (r1 * r1));
-    const float r3 = r2 * (1.5f - h * (r2 * r2));
-    const float r4 = r3 * (1.5f - h * (r3 * r3));
-    return r4;
-}
-__device__ __forceinline__ float det_sqrt(float v) {
-    if (!(v > 0.0f)) return 0.0f;
-    return v * det_rsqrt(v);
-}
-__device__ __forceinline__ float det_atan2(float y, float x) {
-    const float ay = fabsf(y), ax = fabsf(x);
-    if (ay == 0.0f && ax == 0.0f) return 0.0f;
-    const float num = (ax < ay) ? ax : ay;
-    const float den = (ay < ax) ? ax : ay;
-    const float t = num * det_recip(den);
-    const float u = t * t;
-    float p = float(-0.0021090222);
-    p = p * u + float(0.0122847765);
-    p = p * u + float(-0.0336503503);
-    p = p * u + float(0.0603537785);
-    p = p * u + float(-0.0853800324);
-    p = p * u + float(0.1099615976);
-    p = p * u + float(-0.1427217912);
-    p = p * u + float(0.1999923872);
-    p = p * u + float(-0.3333331929);
-    p = p * u + float(0.9999999998);
-    float angle = t * p;
-    if (ay > ax) angle = float(1.57079632679489661923) - angle;
-    if (x < 0.0f) angle = float(3.14159265358979323846) - angle;
-    return (y < 0.0f) ? -angle : angle;
-}
+using rvx::det_atan2;
+using rvx::det_recip;
+using rvx::det_sqrt;
 
 __global__ void vrx_fine_kernel(rvx_vrx_fine_config c, rvx_vrx_fine_params p,
                                 const float2* __restrict__ chan, const float2* __restrict__ taps,
                                 const float2* __restrict__ nco, float2* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.count) return;
-    const uint32_t acc_idx = p.frac0 + i * p.step_rem;
-    const uint32_t carry = acc_idx / p.out_rate;
-    const uint32_t frac = acc_idx - carry * p.out_rate;
-    const uint32_t n_rel = i * p.step_whole + carry;
-    const uint32_t scaled = frac * c.phases;
-    const uint32_t phase = scaled / p.out_rate;
-    const uint32_t blend_num = scaled - phase * p.out_rate;
-    const float blend = float(blend_num) * p.inv_out_rate;
-    const float comp = 1.0f - blend;
-    const uint32_t base = p.in_offset + n_rel;
-    const float2* ch = chan + p.chan_base;
-    float ar = 0.0f, ai = 0.0f;
-    for (uint32_t k = 0; k < c.taps; ++k) {
-        const uint32_t slot = phase + k * c.phases;
-        const float2 g0 = taps[slot];
-        const float2 g1 = taps[slot + 1];
-        const float tr = g0.x * comp + g1.x * blend;
-        const float ti = g0.y * comp + g1.y * blend;
-        const float2 s = ch[(base - k) & p.chan_mask];
-        ar = ar + (s.x * tr - s.y * ti);
-        ai = ai + (s.x * ti + s.y * tr);
-    }
-    const uint32_t prod_lo = i * p.nco_delta_low;
-    const uint32_t prod_hi = __umulhi(i, p.nco_delta_low);
-    const uint32_t ph_lo = p.nco_phase_low + prod_lo;
-    const uint32_t carry_bit = ph_lo < p.nco_phase_low ? 1u : 0u;
-    const uint32_t ph_hi = p.nco_phase_high + i * p.nco_delta_high + prod_hi + carry_bit;
-    const uint32_t ni = c.nco_log2 == 0 ? 0u : (ph_hi >> (32 - c.nco_log2));
-    const float2 r = nco[ni];
-    const float orr = ar * r.x - ai * r.y;
-    const float oi = ar * r.y + ai * r.x;
-    out[(p.out_offset + i) & p.out_mask] = make_float2(orr, oi);
+    out[(p.out_offset + i) & p.out_mask] = rvx::fine_output(c, p, chan + p.chan_base, taps, nco, i);
 }
 
 bool complex_tap(uint32_t mode) { return mode == 0 || (mode >= 8 && mode <= 11); }

@@ -1,0 +1,238 @@
+// Batched receivers: every VRX of a step in one persistent launch.
+//
+// The reference records two or three dispatches per receiver per block, each
+// reading and writing device memory (core/engine/vrx_stage.cpp:1306-1399).
+// Here a persistent workgroup takes (receiver, audio tile) items from a
+// queue and, for one tile, computes on chip:
+//   1. the fine samples the tile's audio filter reaches, with the receiver's
+//      fine-block params (fine_output, exactly vrx_fine.comp's arithmetic);
+//   2. detect() once per fine sample (AM magnitudes first), as vrx.hip's
+//      tiled demodulator does;
+//   3. the decimating audio filter, mono, FM stereo or complex tap.
+// Tiles overlap only by the filter's history, recomputed: every value is a
+// function of its index, so the bits equal rvx_vrx_fine + rvx_vrx_demod.
+// Receivers of any mode and rate mix freely in one launch; the host orders
+// the work largest-first and the queue balances the rest.
+
+#pragma clang fp contract(off)
+
+#include <algorithm>
+
+#include "common.h"
+#include "plan.h"
+#include "vrx_math.h"
+
+namespace {
+
+constexpr uint32_t kBatchThreads = 256;
+constexpr uint32_t kBatchLdsBytes = 64 * 1024;  // fine (8 B) + magnitude + detector per sample
+constexpr uint32_t kMaxSpan = kBatchLdsBytes / 16;
+static_assert(sizeof(rvx_vrx_desc) == 160, "descriptor layout is part of the ABI");
+
+__host__ __device__ bool complex_tap(uint32_t mode) { return mode == 0 || (mode >= 8 && mode <= 11); }
+
+__host__ __device__ uint32_t detector_history(const rvx_vrx_demod_config& c) {
+    return (c.mode == RVX_DEMOD_AM || c.mode == 12) ? c.dc_taps - 1
+           : (c.mode == RVX_DEMOD_NFM || c.mode == RVX_DEMOD_WFM) ? 1u
+                                                                  : 0u;
+}
+
+__global__ void zero_word(uint32_t* p) { *p = 0u; }
+
+__global__ void __launch_bounds__(kBatchThreads)
+vrx_batch_kernel(const rvx_vrx_desc* __restrict__ descs, uint32_t n_desc,
+                 const rvx_vrx_work* __restrict__ work, uint32_t n_work, uint32_t* __restrict__ counter,
+                 const float2* __restrict__ nco, uint32_t nco_log2_max) {
+    __shared__ float4 smem4[kBatchLdsBytes / 16];
+    __shared__ uint32_t item_sh;
+    float2* fine = reinterpret_cast<float2*>(smem4);
+    float* mag = reinterpret_cast<float*>(fine + kMaxSpan);
+    float* det = mag + kMaxSpan;
+    const uint32_t tid = threadIdx.x;
+
+    for (;;) {
+        if (tid == 0) item_sh = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t item = __builtin_amdgcn_readfirstlane(item_sh);
+        if (item >= n_work) break;  // uniform: every thread read the same item
+        const rvx_vrx_work w = work[item];
+        if (w.vrx >= n_desc) {  // a malformed work list computes nothing
+            __syncthreads();
+            continue;
+        }
+        const rvx_vrx_desc& d = descs[w.vrx];
+        const rvx_vrx_fine_config fc = d.fine_config;
+        const rvx_vrx_fine_params fp = d.fine;
+        const rvx_vrx_demod_config dc = d.demod_config;
+        const rvx_vrx_demod_params dp = d.demod;
+        const uint32_t R = dc.decimation;
+        const uint32_t a0 = w.tile * d.tile;
+        // Descriptors pass rvx_vrx_desc_check on the host; these guards keep an
+        // unchecked one inside the NCO table and the LDS tile.
+        const uint32_t n = a0 < dp.count ? min(d.tile, dp.count - a0) : 0u;
+        const uint64_t want = n ? uint64_t(n - 1) * R + d.hist + 1 : 0u;
+        const uint32_t pilot_reach = dc.pilot_taps ? dc.audio_taps / 2 + dc.pilot_taps - 1 : 0u;
+        const uint32_t filter_reach = max(dc.audio_taps - 1, pilot_reach);
+        const bool valid = fc.nco_log2 <= nco_log2_max && want <= kMaxSpan &&
+                           d.hist >= detector_history(dc) + filter_reach;
+        const uint32_t span = valid ? uint32_t(want) : 0u;
+        const uint32_t dh = detector_history(dc);
+        const float2* ch = static_cast<const float2*>(d.channel_ring) + fp.chan_base;
+        const float2* taps = static_cast<const float2*>(d.taps);
+        const float* wts = d.weights;
+        float* audio = d.audio;
+
+        // 1. fine samples a0*R .. a0*R + span - 1 of the receiver's fine block
+        for (uint32_t s = tid; s < span; s += kBatchThreads)
+            fine[s] = rvx::fine_output(fc, fp, ch, taps, nco, a0 * R + s);
+        __syncthreads();
+
+        if (complex_tap(dc.mode)) {
+            for (uint32_t i = tid; span && i < n; i += kBatchThreads) {
+                const float2 z = fine[i * R + d.hist];
+                audio[2 * uint64_t(a0 + i)] = z.x * dp.gain;
+                audio[2 * uint64_t(a0 + i) + 1] = z.y * dp.gain;
+            }
+            __syncthreads();
+            continue;
+        }
+
+        // 2. detector: det[s] is detect() at fine index s + dh of the tile
+        const uint32_t det_span = span ? span - dh : 0u;
+        if (dc.mode == RVX_DEMOD_AM || dc.mode == 12) {
+            for (uint32_t s = tid; s < span; s += kBatchThreads) {
+                const float2 z = fine[s];
+                if (dc.mode == RVX_DEMOD_AM) {
+                    const float sq = z.x * z.x + z.y * z.y;
+                    mag[s] = rvx::det_sqrt(sq);
+                } else {
+                    mag[s] = z.x;
+                }
+            }
+            __syncthreads();
+            const float* dcw = wts + dc.audio_taps;
+            for (uint32_t s = tid; s < det_span; s += kBatchThreads) {
+                float acc = 0.0f;
+                for (uint32_t k = 0; k < dc.dc_taps; ++k) {
+                    const float term = mag[s + dh - k] * dcw[k];
+                    acc = acc + term;
+                }
+                det[s] = mag[s + dh] - acc;
+            }
+        } else if (dc.mode == RVX_DEMOD_NFM || dc.mode == RVX_DEMOD_WFM) {
+            for (uint32_t s = tid; s < det_span; s += kBatchThreads) {
+                const float2 z = fine[s + dh];
+                const float2 q = fine[s + dh - 1];
+                const float pr = z.x * q.x + z.y * q.y;
+                const float pi = z.y * q.x - z.x * q.y;
+                det[s] = rvx::det_atan2(pi, pr);
+            }
+        } else {
+            for (uint32_t s = tid; s < det_span; s += kBatchThreads) det[s] = fine[s].x;
+        }
+        __syncthreads();
+
+        // 3. audio filter; output i sits at detector index i*R + (hist - dh)
+        const uint32_t reach = d.hist - dh;
+        const uint32_t pilot_base = dc.audio_taps + dc.dc_taps;
+        const uint32_t rot_base = pilot_base + 2 * dc.pilot_taps;
+        for (uint32_t i = tid; span && i < n; i += kBatchThreads) {
+            const uint32_t rel = i * R + reach;
+            const uint32_t o = a0 + i;
+            if (dc.channels == 2) {
+                const uint32_t at = rel - dc.audio_taps / 2;
+                float pr = 0.0f, pi = 0.0f;
+                for (uint32_t k = 0; k < dc.pilot_taps; ++k) {
+                    const float comp = det[at - k];
+                    const float tr = comp * wts[pilot_base + 2 * k];
+                    const float ti = comp * wts[pilot_base + 2 * k + 1];
+                    pr = pr + tr;
+                    pi = pi + ti;
+                }
+                const float sr = pr * dp.gain, si = pi * dp.gain;
+                const float rsq = sr * sr, isq = si * si;
+                const float power = rsq + isq;
+                const float inv = rvx::det_recip(fmaxf(power, 2.0e-5f));
+                const float cross = sr * si;
+                const float cos2 = (isq - rsq) * inv;
+                const float sin2 = (-2.0f * cross) * inv;
+                const float locked = power >= 2.0e-5f ? 1.0f : 0.0f;
+                float sum = 0.0f, diff = 0.0f;
+                for (uint32_t t = 0; t < dc.audio_taps; ++t) {
+                    const float weighted = det[rel - t] * wts[t];
+                    sum = sum + weighted;
+                    const float ref = sin2 * wts[rot_base + 2 * t] - cos2 * wts[rot_base + 2 * t + 1];
+                    const float term = weighted * ref;
+                    diff = diff + term;
+                }
+                const float gated = diff * locked;
+                audio[2 * uint64_t(o)] = (sum + gated) * dp.gain;
+                audio[2 * uint64_t(o) + 1] = (sum - gated) * dp.gain;
+            } else {
+                float acc = 0.0f;
+                for (uint32_t t = 0; t < dc.audio_taps; ++t) {
+                    const float term = det[rel - t] * wts[t];
+                    acc = acc + term;
+                }
+                audio[o] = acc * dp.gain;
+            }
+        }
+        __syncthreads();  // the tile's LDS is free and item_sh may be rewritten
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rvx_vrx_desc_check(const rvx_vrx_desc* desc) {
+    static const char* who = "rvx_vrx_desc_check";
+    if (desc == nullptr) return rvx::fail(RVX_EINVAL, "%s: null descriptor", who);
+    const rvx_vrx_desc& d = *desc;
+    if (int32_t rc = rvx::vrx_fine_valid(d.fine_config, d.fine, who)) return rc;
+    rvx_vrx_demod_params dp = d.demod;
+    const uint32_t hist = rvx::demod_fine_history(d.demod_config);
+    if (d.hist != hist)
+        return rvx::fail(RVX_EINVAL, "%s: hist %u, the demodulator reaches %u", who, d.hist, hist);
+    // The fine block must hold every fine sample the audio outputs reach.
+    const uint64_t need =
+        d.demod.count == 0 ? 0 : uint64_t(d.demod.count - 1) * d.demod_config.decimation + hist + 1;
+    if (d.fine.count < need)
+        return rvx::fail(RVX_EINVAL, "%s: fine block of %u outputs, the audio needs %llu", who,
+                         d.fine.count, (unsigned long long)need);
+    dp.in_mask = 0xFFFFFFFFu;  // the fine block is on chip: no ring to alias
+    if (int32_t rc = rvx::vrx_demod_valid(d.demod_config, dp, who)) return rc;
+    if (d.tile == 0) return rvx::fail(RVX_EINVAL, "%s: zero tile", who);
+    const uint64_t span = uint64_t(d.tile - 1) * d.demod_config.decimation + hist + 1;
+    if (span > kMaxSpan)
+        return rvx::fail(RVX_EINVAL, "%s: a tile of %u outputs needs %llu fine samples; at most %u fit",
+                         who, d.tile, (unsigned long long)span, kMaxSpan);
+    if (!d.channel_ring || !d.taps || !d.weights || !d.audio)
+        return rvx::fail(RVX_EINVAL, "%s: null buffer", who);
+    return RVX_OK;
+}
+
+int32_t rvx_vrx_batch(rvx_ctx* ctx, const rvx_vrx_desc* descs, uint32_t n_desc,
+                      const rvx_vrx_work* work, uint32_t n_work, uint32_t* counter,
+                      const void* nco, uint64_t n_nco, uint32_t workgroups) {
+    static const char* who = "rvx_vrx_batch";
+    RVX_ENTER(ctx, who);
+    if (n_work == 0) return RVX_OK;
+    if (!descs || !work || !counter || !nco || n_desc == 0)
+        return rvx::fail(RVX_EINVAL, "%s: null buffer", who);
+    if (!rvx::pow2(n_nco) || n_nco > (1u << 24))
+        return rvx::fail(RVX_EINVAL, "%s: NCO table of %llu entries", who, (unsigned long long)n_nco);
+    if (workgroups == 0) {
+        int cus = 0;
+        RVX_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
+                    who);
+        workgroups = uint32_t(std::max(1, cus)) * 2;
+    }
+    workgroups = std::min(workgroups, n_work);
+    zero_word<<<dim3(1), dim3(1), 0, ctx->stream>>>(counter);
+    vrx_batch_kernel<<<dim3(workgroups), dim3(kBatchThreads), 0, ctx->stream>>>(
+        descs, n_desc, work, n_work, counter, static_cast<const float2*>(nco), rvx::log2u(n_nco));
+    return rvx::check_launch(who);
+}
+
+}  // extern "C"

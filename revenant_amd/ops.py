@@ -156,6 +156,16 @@ class Device:
                                       _csamples(ring), _p(partial),
                                       0 if partial is None else _n(partial)))
 
+    # -- graphs ---------------------------------------------------------
+    def graph_begin(self) -> None:
+        """Start capturing this context's stream (not the null stream)."""
+        _lib.check(self.lib.rvx_graph_begin(self.ctx))
+
+    def graph_end(self) -> "Graph":
+        g = C.c_void_p()
+        _lib.check(self.lib.rvx_graph_end(self.ctx, C.byref(g)))
+        return Graph(self, g)
+
     # -- timing ---------------------------------------------------------
     def timer_begin(self) -> None:
         _lib.check(self.lib.rvx_timer_begin(self.ctx))
@@ -164,3 +174,25 @@ class Device:
         ms = C.c_float()
         _lib.check(self.lib.rvx_timer_end(self.ctx, C.byref(ms)))
         return ms.value
+
+
+class Graph:
+    """A captured, instantiated sequence of launches (HIP graph)."""
+
+    def __init__(self, dev: Device, handle: C.c_void_p):
+        self.dev = dev
+        self.handle = handle
+
+    def launch(self) -> None:
+        _lib.check(self.dev.lib.rvx_graph_launch(self.dev.ctx, self.handle))
+
+    def close(self) -> None:
+        if self.handle:
+            self.dev.lib.rvx_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

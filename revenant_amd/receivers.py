@@ -1,0 +1,213 @@
+"""Batched receivers: descriptors, the work list, tile tuning and the launch.
+
+The reference records two or three dispatches per receiver and block
+(`DemodStage::record`, core/engine/vrx_stage.cpp:1306-1399): vrx_fine into a
+fine ring, then vrx_demod from it. Here every receiver of a step runs in one
+`rvx_vrx_batch` launch. A persistent grid takes (receiver, audio tile) items
+from a queue; each item computes its fine samples, detector values and audio
+on chip, so the fine ring never exists in HBM. The result is bit-identical to
+`rvx_vrx_fine` followed by `rvx_vrx_demod` for the same plan and indices.
+
+A receiver's block of audio outputs a0 .. a0+count-1 needs fine outputs
+F0 .. F0 + (count-1)*R + hist with F0 = a0*R - hist
+(`demod_fine_history`, core/dsp/vrx_reference.h:555); the descriptor carries
+the fine-block params of exactly that range (`resample_block`,
+vrx_reference.cpp:2786-2871) and the demod-block params of the audio
+(vrx_reference.cpp:2873-2909).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .dsp import VrxPlan
+from .ops import Device, _p
+
+MAX_SPAN = 4096           # fine samples one tile holds on chip (64 KiB of LDS)
+_UNBOUNDED = 0xFFFFFFFF   # ring masks of the on-chip fine block: no wrap
+
+
+def max_tile(plan: VrxPlan) -> int:
+    """Largest audio tile whose fine span fits the on-chip budget."""
+    r, hist = plan.demod.decimation, plan.history
+    if hist + 1 > MAX_SPAN:
+        raise ValueError(f"a receiver whose filters reach {hist} fine samples does not fit a tile")
+    return (MAX_SPAN - hist - 1) // r + 1
+
+
+def tile_cost(plan: VrxPlan, n: int) -> int:
+    """Multiply count of one tile of n audio outputs (for ordering the work)."""
+    r, hist = plan.demod.decimation, plan.history
+    span = (n - 1) * r + hist + 1
+    d = plan.demod
+    cost = span * plan.fine.taps * 4
+    if d.mode in (1, 12):
+        cost += span * d.dc_taps
+    cost += n * d.audio_taps * (3 if d.channels == 2 and d.pilot_taps else 1)
+    return cost
+
+
+@dataclass
+class Receiver:
+    """One receiver of the batch: its plan and where its channel sits."""
+
+    plan: VrxPlan
+    channel_base: int    # cf32 index of the channel's first slot in the channel ring
+    channel_mask: int    # channel length - 1 (a power of two minus one)
+    ring: torch.Tensor   # the channel ring it reads (cf32 as float32 pairs)
+    taps: torch.Tensor = field(default=None)      # device copies of the plan's tables
+    weights: torch.Tensor = field(default=None)
+
+    def upload(self, device: torch.device) -> None:
+        if self.taps is None:
+            self.taps = torch.from_numpy(self.plan.fine_taps.view(np.float32).copy()).to(device)
+        if self.weights is None:
+            self.weights = torch.from_numpy(self.plan.weights.copy()).to(device)
+
+    def max_desc_audio(self) -> int:
+        """Most audio outputs one descriptor carries: the fine block's 32-bit
+        resampler recurrence frac0 + count*step_rem must not overflow
+        (vrx_reference.cpp:429-486); longer blocks split into several."""
+        plan = self.plan
+        fc, fd = plan.plan.channel_rate, plan.plan.demod_rate
+        rem = fc % fd
+        if rem == 0:
+            fine = 0xFFFFFFFF
+        else:
+            fine = (0xFFFFFFFF - (fd - 1)) // rem
+        fine = min(fine, 0xFFFFFFFF)
+        r, hist = plan.demod.decimation, plan.history
+        return max(1, (fine - hist - 1) // r + 1)
+
+    def first_input(self, first_audio: int) -> int:
+        """Lowest channel sample index the audio block from first_audio reads."""
+        f0 = first_audio * self.plan.demod.decimation - self.plan.history
+        _, n0 = self.plan.fine_block(self.channel_base, self.channel_mask, _UNBOUNDED, f0, 1)
+        return n0 - (self.plan.fine.taps - 1)
+
+    def descriptor(self, first_audio: int, count: int, audio: torch.Tensor,
+                   tile: int) -> _lib.VrxDesc:
+        plan = self.plan
+        r, hist = plan.demod.decimation, plan.history
+        if first_audio * r < hist:
+            raise ValueError("the first audio output reaches below fine sample 0")
+        d = _lib.VrxDesc()
+        d.fine_config = plan.fine
+        d.demod_config = plan.demod
+        fine_count = (count - 1) * r + hist + 1
+        d.fine, _ = plan.fine_block(self.channel_base, self.channel_mask, _UNBOUNDED,
+                                    first_audio * r - hist, fine_count)
+        d.demod = plan.demod_block(_UNBOUNDED, first_audio, count)
+        d.hist = hist
+        d.tile = tile
+        d.channel_ring = _p(self.ring)
+        d.taps = _p(self.taps)
+        d.weights = _p(self.weights)
+        d.audio = _p(audio)
+        if audio.numel() < count * plan.demod.channels:
+            raise ValueError("audio tensor smaller than count * channels")
+        _lib.check(_lib.lib().rvx_vrx_desc_check(C.byref(d)))
+        return d
+
+
+class ReceiverBatch:
+    """Every receiver of a step in one launch.
+
+    `prepare` writes the descriptors and the largest-first work list to the
+    device (the host's per-step record); `launch` is the one kernel launch
+    (plus the queue reset), safe to capture in a HIP graph.
+    """
+
+    def __init__(self, dev: Device, receivers: list[Receiver], nco: torch.Tensor,
+                 workgroups: int = 0):
+        self.dev = dev
+        self.receivers = receivers
+        self.nco = nco
+        self.workgroups = workgroups
+        device = nco.device
+        for r in receivers:
+            r.upload(device)
+        self.counter = torch.zeros(1, dtype=torch.int32, device=device)
+        self.tiles = [max_tile(r.plan) for r in receivers]
+        self._descs = None
+        self._work = None
+        self.n_desc = 0
+        self.n_work = 0
+
+    def prepare(self, first_audio: list[int], counts: list[int], audio: list[torch.Tensor],
+                tiles: list[int] | None = None) -> None:
+        if tiles is not None:
+            self.tiles = [min(t, max_tile(r.plan)) for t, r in zip(tiles, self.receivers)]
+        descs: list[_lib.VrxDesc] = []
+        items: list[tuple[int, int, int]] = []
+        for r, a0, n, out, t in zip(self.receivers, first_audio, counts, audio, self.tiles):
+            t = max(1, min(t, n))
+            step = r.max_desc_audio()
+            step -= step % t  # whole tiles per descriptor
+            step = max(step, t)
+            ch = r.plan.demod.channels
+            for lo in range(0, n, step):
+                m = min(step, n - lo)
+                part = out[lo * ch:(lo + m) * ch]
+                descs.append(r.descriptor(a0 + lo, m, part, t))
+                v = len(descs) - 1
+                for j in range((m + t - 1) // t):
+                    items.append((-tile_cost(r.plan, min(t, m - j * t)), v, j))
+        items.sort()
+        work = np.array([(v, j) for _, v, j in items], dtype=np.uint32).reshape(-1, 2)
+        device = self.nco.device
+        table = (_lib.VrxDesc * len(descs))(*descs)
+        self._descs = torch.frombuffer(bytearray(bytes(table)), dtype=torch.uint8).to(device)
+        self._work = torch.from_numpy(work.ravel().copy()).to(device)
+        self.n_desc = len(descs)
+        self.n_work = len(items)
+
+    def launch(self) -> None:
+        if self._descs is None:
+            raise RuntimeError("prepare() the batch first")
+        _lib.check(self.dev.lib.rvx_vrx_batch(
+            self.dev.ctx, _p(self._descs), self.n_desc, _p(self._work), self.n_work,
+            _p(self.counter), _p(self.nco), self.nco.numel() // 2, self.workgroups))
+
+    def tune(self, first_audio: list[int], counts: list[int], audio: list[torch.Tensor],
+             candidates=(32, 64, 128, 256, 512, 1024, 2048), reps: int = 3) -> list[int]:
+        """Per mode class, the audio tile that minimises the whole batch's time.
+
+        Coordinate descent over receiver classes (mode, decimation, filter
+        lengths): each class's tile is swept with the others held, keeping the
+        fastest. Tiles change only the work split, never the output bits.
+        """
+        classes: dict[tuple, list[int]] = {}
+        for v, r in enumerate(self.receivers):
+            d, f = r.plan.demod, r.plan.fine
+            classes.setdefault((d.mode, d.decimation, d.audio_taps, d.dc_taps, f.taps), []).append(v)
+        tiles = list(self.tiles)
+
+        def timed(ts: list[int]) -> float:
+            self.prepare(first_audio, counts, audio, ts)
+            self.launch()
+            best = float("inf")
+            for _ in range(reps):
+                self.dev.timer_begin()
+                self.launch()
+                best = min(best, self.dev.timer_end())
+            return best
+
+        best_ms = timed(tiles)
+        for members in classes.values():
+            cap = max_tile(self.receivers[members[0]].plan)
+            for t in candidates:
+                if t > cap or t == tiles[members[0]]:
+                    continue
+                trial = list(tiles)
+                for v in members:
+                    trial[v] = t
+                ms = timed(trial)
+                if ms < best_ms:
+                    best_ms, tiles = ms, trial
+        self.prepare(first_audio, counts, audio, tiles)
+        return tiles
