@@ -1,28 +1,39 @@
 #!/usr/bin/env python3
-"""Throughput of the plain receive chain on one MI355X.
+"""Throughput of the receive chain with 256 receivers on one MI355X.
 
-Workload (the wideband configuration): 2^27 complex int16 IQ samples at
-122.88 MS/s holding 400 random carriers (CW/AM/NFM/WFM, -80..-10 dBFS) over
-AWGN; a 4096-channel, 2x-oversampled polyphase channelizer (16 taps per
-branch, Kaiser beta 8 prototype, 4096-point radix-2 transform); |X|^2 averaged
-with alpha = 1/16 over 4-block frames. One step = one fused launch over the
-whole resident input (convert -> branch filter -> transform -> correction ->
-channel ring + averaged spectrum) plus the frame recurrence.
+Workload (the batching configuration, on the wideband stream): 2^27 complex
+int16 IQ samples at 122.88 MS/s holding 400 random carriers (CW/AM/NFM/WFM,
+-80..-10 dBFS) over AWGN, resident in HBM. One step, captured once as a HIP
+graph and replayed:
+  1. the 4096-channel, 2x-oversampled polyphase channelizer (16 taps per
+     branch, Kaiser beta 8, 4096-point radix-2 transform) fused with the
+     conversion and the |X|^2 average (alpha = 1/16, 4-block frames): one
+     launch writing the full channel ring and the frame partials;
+  2. the averaged spectrum's frame recurrence;
+  3. the 512-channel grid the 200 kHz WFM receivers need (a 200 kHz receiver
+     does not fit a 60 kHz channel, core/engine/vrx_place.cpp:187-197),
+     storing only the channels those receivers read;
+  4. 256 receivers in one persistent launch: 64 NFM 12.5 kHz, 64 AM 10 kHz,
+     32 WFM 200 kHz, 64 USB 2.7 kHz, 32 CW 500 Hz at seeded uniform offsets,
+     each resampled to 48 kHz and demodulated to FP32 audio over the whole
+     step; tiles picked by the autotuner before capture.
+`value` is input MS/s through all four. `--workload plain` times steps 1-2
+alone (the plain chain).
 
 Prints one JSON line. The timed region brackets exactly K steps with device
-synchronisation; the kernel's own average duration is measured with HIP
-events on the stream it is launched on. The CPU baseline times the oracle
-chain (a C++ restatement of the reference's CPU twins) on a bounded sample of
-the same workload on this host.
+synchronisation; each kernel's average duration is measured with HIP events
+on the stream it is launched on. The CPU baseline times the oracle (a C++
+restatement of the reference's CPU twins) over a bounded sample of the same
+workload on this host.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 import numpy as np
@@ -36,10 +47,16 @@ HBM_PEAK_GBS = 8000.0
 RATE = 122_880_000
 SAMPLES = 1 << 27
 M, TAPS, D = 4096, 16, 2048
+WM, WD = 512, 256  # the wide receivers' grid
 FRAME = 4        # blocks per averaging frame (one workgroup's chunk)
 ROW_FRAMES = 4   # frames per output row of the averaged spectrum (16 blocks)
 ALPHA = 1.0 / 16.0
 SEED = 0x5743454E45000002
+RX_SEED = 0x5652584241544348
+AUDIO_RATE = 48000
+# (mode name, count, bandwidth Hz) of the 256-receiver mix
+MIX = [("nfm", 64, 12500), ("am", 64, 10000), ("wfm", 32, 200_000), ("usb", 64, 2700),
+       ("cw", 32, 500)]
 
 
 def parse():
@@ -47,123 +64,232 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["vrx", "plain"], default="vrx")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default tiles, no autotuning")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 1 GiB device to device three times: the "
                          "known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
 
 
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+class Grid:
+    """One channelizer over the resident input: descriptor, tables, ring."""
+
+    def __init__(self, torch, channels, decimation, frame, rows=None):
+        from revenant_amd import _lib
+        from revenant_amd.dsp import (FORMAT_CS16, GridParams, build_twiddles,
+                                      design_prototype)
+
+        self.params = GridParams(channels, TAPS, decimation)
+        self.proto = design_prototype(self.params, beta=8.0)
+        self.taps = torch.from_numpy(self.proto.taps).cuda()
+        self.tw_np = build_twiddles(channels)
+        self.tw = torch.from_numpy(self.tw_np.view(np.float32)).cuda()
+        N = self.params.prototype_length
+        self.first_block = (N - 1 + decimation - 1) // decimation  # first with full history
+        blocks = (SAMPLES - 1) // decimation - self.first_block + 1
+        if frame:
+            blocks -= blocks % (frame * ROW_FRAMES)
+        self.blocks = blocks
+        self.ring_blocks = 1 << (blocks - 1).bit_length()
+        self.unique_in = (blocks - 1) * decimation + N
+        self.rows = rows
+        n_rows = channels if rows is None else len(rows)
+        self.ring = torch.empty(2 * n_rows * self.ring_blocks, dtype=torch.float32,
+                                device="cuda")
+        self.desc = _lib.ChainDesc(FORMAT_CS16, channels, TAPS, decimation, 0, SAMPLES,
+                                   self.first_block, blocks, self.ring_blocks, frame,
+                                   np.float32(ALPHA))
+        self.row_table = None
+        if rows is not None:
+            table = np.full(channels, 0xFFFFFFFF, dtype=np.uint32)
+            for r, k in enumerate(rows):
+                table[k] = r
+            self.row_table = torch.from_numpy(table.view(np.int32)).cuda()
+            self.desc.ring_rows = len(rows)
+            self.desc.channel_rows = self.row_table.data_ptr()
+        self.frames = blocks // frame if frame else 0
+        # algorithmic bytes per launch: the unique input window read once, each
+        # stored channel sample written once, the frame partials written once
+        self.algo_bytes = 4 * self.unique_in + 8 * n_rows * blocks + 4 * channels * self.frames
+
+
+def build_receivers(main: Grid):
+    """The 256-receiver mix, placed and planned; returns (receivers, ranges)."""
+    from revenant_amd.dsp import (DEMOD_AM, DEMOD_CW, DEMOD_NFM, DEMOD_USB, DEMOD_WFM,
+                                  VrxRequest, place, plan_vrx)
+
+    modes = {"nfm": DEMOD_NFM, "am": DEMOD_AM, "wfm": DEMOD_WFM, "usb": DEMOD_USB,
+             "cw": DEMOD_CW}
+    g = np.random.default_rng(RX_SEED)
+    reqs = []
+    for name, count, bw in MIX:
+        for _ in range(count):
+            f = int(g.integers(-int(0.45 * RATE), int(0.45 * RATE)))
+            reqs.append((name, VrxRequest(f, modes[name], bw, AUDIO_RATE,
+                                          cw_pitch=700 if name == "cw" else 0)))
+    from revenant_amd.dsp import GridParams
+    wide = GridParams(WM, TAPS, WD)
+    plans = []
+    for name, req in reqs:
+        grid = wide if name == "wfm" else main.params
+        plans.append((name, plan_vrx(grid, RATE, req, place(grid, RATE, req))))
+    wide_set = sorted({p.plan.placement.channel for n, p in plans if n == "wfm"})
+    return plans, wide_set
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the chain shards by time block with a halo; see DESIGN.md)", file=sys.stderr)
+              "(the chain shards by time block with a halo, the receivers by receiver)",
+              file=sys.stderr)
         return 2
 
     import torch
 
-    from revenant_amd import _lib, synth
-    from revenant_amd.dsp import (FORMAT_CS16, GridParams, build_twiddles, design_prototype,
-                                  ema_frame_decay, ema_params)
+    from revenant_amd import synth
+    from revenant_amd.dsp import FORMAT_CS16, build_nco_table, ema_frame_decay, ema_params
     from revenant_amd.ops import Device
+    from revenant_amd.receivers import Receiver, ReceiverBatch
 
     dev = Device(0)
-    grid = GridParams(M, TAPS, D)
-    proto = design_prototype(grid, beta=8.0)
-    taps = torch.from_numpy(proto.taps).cuda()
-    tw_np = build_twiddles(M)
-    tw = torch.from_numpy(tw_np.view(np.float32)).cuda()
+    side = torch.cuda.Stream()  # default priority; graphs capture a non-null stream
+    dev.bind_stream(side)
+    vrx = args.workload == "vrx"
 
-    print(f"[bench] rendering {SAMPLES} cs16 samples (400 carriers + AWGN) on the GPU",
-          file=sys.stderr, flush=True)
-    scene = synth.wideband_scene(RATE, 400, SEED)
-    packed = torch.empty(4 * SAMPLES, dtype=torch.uint8, device="cuda")
-    synth.render_device(dev, scene, RATE, 0, SAMPLES, noise_dbfs=-60.0, seed=SEED,
-                        fmt=FORMAT_CS16, out=packed)
-    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        log(f"rendering {SAMPLES} cs16 samples (400 carriers + AWGN) on the GPU")
+        scene = synth.wideband_scene(RATE, 400, SEED)
+        packed = torch.empty(4 * SAMPLES, dtype=torch.uint8, device="cuda")
+        synth.render_device(dev, scene, RATE, 0, SAMPLES, noise_dbfs=-60.0, seed=SEED,
+                            fmt=FORMAT_CS16, out=packed)
 
-    N = grid.prototype_length
-    first_block = (N - 1 + D - 1) // D               # first block with full history
-    blocks = (SAMPLES - 1) // D - first_block + 1
-    blocks -= blocks % (FRAME * ROW_FRAMES)
-    ring_blocks = 1 << (blocks - 1).bit_length()
-    frames = blocks // FRAME
-    alpha = np.float32(ALPHA)
-    decay = ema_frame_decay(alpha, FRAME)
-    ring = torch.empty(2 * M * ring_blocks, dtype=torch.float32, device="cuda")
-    partial = torch.empty(frames * M, dtype=torch.float32, device="cuda")
-    state = torch.zeros(M, dtype=torch.float32, device="cuda")
-    rows = torch.empty((frames // ROW_FRAMES) * M, dtype=torch.float32, device="cuda")
-    desc = _lib.ChainDesc(FORMAT_CS16, M, TAPS, D, 0, SAMPLES, first_block, blocks, ring_blocks,
-                          FRAME, alpha)
-    ema = ema_params(M, ring_blocks, first_block, FRAME, frames, alpha, ROW_FRAMES)
+        main_grid = Grid(torch, M, D, FRAME)
+        alpha = np.float32(ALPHA)
+        frames = main_grid.frames
+        partial = torch.empty(frames * M, dtype=torch.float32, device="cuda")
+        state = torch.zeros(M, dtype=torch.float32, device="cuda")
+        rows_out = torch.empty((frames // ROW_FRAMES) * M, dtype=torch.float32, device="cuda")
+        ema = ema_params(M, main_grid.ring_blocks, main_grid.first_block, FRAME, frames, alpha,
+                         ROW_FRAMES)
 
-    def step():
-        dev.chain(desc, packed, taps, tw, ring, partial)
-        dev.ema_combine(ema, partial, state, rows)
+        launches = {}
+        launches["chain_m4096"] = lambda: dev.chain(main_grid.desc, packed, main_grid.taps,
+                                                    main_grid.tw, main_grid.ring, partial)
+        launches["ema_combine"] = lambda: dev.ema_combine(ema, partial, state, rows_out)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
+        batch = wide_grid = None
+        audio_total = 0
+        if vrx:
+            plans, wide_set = build_receivers(main_grid)
+            wide_grid = Grid(torch, WM, WD, 0, rows=wide_set)
+            nco = torch.from_numpy(build_nco_table(16).view(np.float32)).cuda()
+            receivers, starts, counts = [], [], []
+            for name, p in plans:
+                if name == "wfm":
+                    g, row = wide_grid, wide_set.index(p.plan.placement.channel)
+                else:
+                    g, row = main_grid, p.plan.placement.channel
+                rx = Receiver(p, row * g.ring_blocks, g.ring_blocks - 1, g.ring)
+                a0, n = rx.audio_range(g.first_block, g.first_block + g.blocks - 1)
+                receivers.append(rx)
+                starts.append(a0)
+                counts.append(n)
+            sizes = [n * r.plan.demod.channels for n, r in zip(counts, receivers)]
+            audio_buf = torch.empty(sum(sizes), dtype=torch.float32, device="cuda")
+            offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+            audio = [audio_buf[offs[i]:offs[i + 1]] for i in range(len(receivers))]
+            audio_total = int(sum(counts))
+            batch = ReceiverBatch(dev, receivers, nco)
+            launches["chain_m512_wfm"] = lambda: dev.chain(wide_grid.desc, packed, wide_grid.taps,
+                                                           wide_grid.tw, wide_grid.ring)
+            # the receivers read channel rings that the chains fill
+            launches["chain_m512_wfm"]()
+            launches["chain_m4096"]()
+            batch.prepare(starts, counts, audio)
+            if not args.no_tune:
+                log("tuning receiver tiles")
+                tiles = batch.tune(starts, counts, audio)
+            launches["vrx_batch"] = batch.launch
+        side.synchronize()
 
-    # Kernel-only timing of the dominant launch (HIP events on its stream).
-    dev.timer_begin()
-    for _ in range(args.steps):
-        dev.chain(desc, packed, taps, tw, ring, partial)
-    chain_ms = dev.timer_end() / args.steps
+        # One step, captured once.
+        log("capturing the step as a HIP graph")
+        order = ["chain_m4096", "ema_combine"] + (["chain_m512_wfm", "vrx_batch"] if vrx else [])
+        for name in order:  # eager warm launch of every kernel before capture
+            launches[name]()
+        side.synchronize()
+        dev.graph_begin()
+        for name in order:
+            launches[name]()
+        graph = dev.graph_end()
 
-    # Whole-step timing, bracketed by device synchronisation.
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+        for _ in range(args.warmup):
+            graph.launch()
+        side.synchronize()
 
-    in_samples = blocks * D
+        # Kernel-only timing of each launch (HIP events on the launch stream).
+        kernel_ms = {}
+        for name in order:
+            dev.timer_begin()
+            for _ in range(args.steps):
+                launches[name]()
+            kernel_ms[name] = dev.timer_end() / args.steps
+
+        # Whole-step timing, bracketed by device synchronisation.
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            graph.launch()
+        torch.cuda.synchronize()
+        ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+        graph.close()
+
+    in_samples = main_grid.blocks * D
     value = in_samples / (ms_per_step * 1e-3) / 1e6
-    # Algorithmic bytes per launch: each input sample read once (4 B cs16,
-    # the unique window incl. the filter's history), the channel ring written
-    # once (8 B per channel sample), the frame partials written once.
-    unique_in = (blocks - 1) * D + N
-    algo_bytes = 4 * unique_in + 8 * M * blocks + 4 * M * frames
-    achieved = algo_bytes / (chain_ms * 1e-3) / 1e9
 
-    # Parity spot check of this very run: two frames against the oracle. The
-    # oracle sees the input from sample start*D on, i.e. block indices shifted
-    # by `start`; start is a multiple of the frame (and of M/D), so the phase
-    # correction and the frame alignment are unchanged by the shift.
-    parity = "skipped"
-    try:
-        from oracle import oracle
-        check = 2 * FRAME
-        start = blocks // 2 - (blocks // 2) % FRAME
-        hi = (first_block + start + check - 1) * D + 1
-        sub = packed[start * D * 4:hi * 4].cpu().numpy()
-        cap = 1 << (check - 1).bit_length()
-        od = oracle.OChain(FORMAT_CS16, M, TAPS, D, first_block, check, cap, FRAME, alpha, decay)
-        ring_cpu, part_cpu = oracle.chain(od, sub, proto.taps, tw_np, threads=4)
-        ring_cpu = ring_cpu.reshape(M, cap, 2)
-        g_ring = ring.view(M, ring_blocks, 2).cpu().numpy()
-        ok = True
-        for b in range(check):
-            m = first_block + start + b
-            col = ring_cpu[:, (first_block + b) & (cap - 1)]
-            ok &= np.array_equal(g_ring[:, m & (ring_blocks - 1)].view(np.uint32),
-                                 col.view(np.uint32))
-        gp = partial.view(frames, M).cpu().numpy()[start // FRAME:start // FRAME + 2]
-        ok &= np.array_equal(gp.view(np.uint32), part_cpu.reshape(2, M).view(np.uint32))
-        parity = f"bit-exact vs oracle on {check} blocks" if ok else "MISMATCH vs oracle"
-    except Exception as exc:  # the oracle is optional on the box
-        parity = f"skipped ({type(exc).__name__}: {exc})"
+    algo = {"chain_m4096": main_grid.algo_bytes,
+            "ema_combine": 4 * M * frames + 4 * M * (frames // ROW_FRAMES)}
+    if vrx:
+        algo["chain_m512_wfm"] = wide_grid.algo_bytes
+        # each receiver reads its channel span once and writes its audio once
+        rx_bytes = 0
+        for rx, n in zip(batch.receivers, counts):
+            p = rx.plan
+            fine = (n - 1) * p.demod.decimation + p.history + 1
+            chan = fine * p.plan.channel_rate // p.plan.demod_rate + p.fine.taps
+            rx_bytes += 8 * chan + 4 * n * p.demod.channels
+        algo["vrx_batch"] = rx_bytes
+    stages = {k: {"ms": round(kernel_ms[k], 4),
+                  "GB/s": round(algo[k] / (kernel_ms[k] * 1e-3) / 1e9, 1)} for k in order}
+    dominant = max(order, key=lambda k: kernel_ms[k])
+    achieved = algo[dominant] / (kernel_ms[dominant] * 1e-3) / 1e9
+
+    parity = spot_check(torch, packed, main_grid, partial, frames, alpha,
+                        batch if vrx else None, starts if vrx else None,
+                        counts if vrx else None, audio if vrx else None)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(packed, proto.taps, tw_np, first_block, args.cpu_seconds)
+        cpu = cpu_baseline(packed, main_grid, wide_grid, batch, args.cpu_seconds)
 
+    workload = ("PFB + 256 VRX: 2^27 cs16 @ 122.88 MS/s (400 carriers + AWGN) -> 4096-ch PFB "
+                "(D=2048, 16 taps/branch, Kaiser beta 8) -> channel ring + |X|^2 EMA "
+                "alpha=1/16; 512-ch PFB (D=256) for the WFM receivers; 256 VRX "
+                "(64 NFM 12.5k, 64 AM 10k, 32 WFM 200k, 64 USB 2.7k, 32 CW 500) -> 48 kHz "
+                "FP32 audio, one persistent launch; the step is one HIP graph") if vrx else (
+                "plain chain: 2^27 cs16 @ 122.88 MS/s, 400 carriers + AWGN -> 4096-ch PFB "
+                "(D=2048, 16 taps/branch, Kaiser beta 8) -> channel ring + |X|^2 EMA "
+                "alpha=1/16 (4-block frames)")
+    traffic = traffic_from_profiles(dominant)
     line = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -178,14 +304,13 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "plain chain: 2^27 cs16 @ 122.88 MS/s, 400 carriers + AWGN -> "
-                        "4096-ch PFB (D=2048, 16 taps/branch, Kaiser beta 8) -> channel ring "
-                        "+ |X|^2 EMA alpha=1/16 (4-block frames); VRX stage not in this line",
+            "workload": workload,
             "input_samples": int(in_samples),
-            "blocks": int(blocks),
             "channels": M,
             "decimation": D,
             "taps_per_branch": TAPS,
+            "receivers": len(batch.receivers) if vrx else 0,
+            "audio_samples": audio_total,
         },
         "roofline": {
             "bound": "hbm",
@@ -193,11 +318,12 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(),
-            "kernel": "chain_kernel<M=4096, V=8, cs16, 16 taps>",
-            "kernel_ms": round(chain_ms, 4),
-            "algorithmic_bytes": int(algo_bytes),
+            "traffic": traffic,
+            "kernel": dominant,
+            "kernel_ms": round(kernel_ms[dominant], 4),
+            "algorithmic_bytes": int(algo[dominant]),
         },
+        "stages": stages,
         "cpu_baseline": cpu,
         "parity": parity,
     }
@@ -208,13 +334,74 @@ def main() -> int:
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
+    dev.bind_stream()
     return 0
 
 
-def traffic_from_profiles():
+def spot_check(torch, packed, g: Grid, partial, frames, alpha, batch, starts, counts, audio):
+    """Parity of this very run against the oracle: two frames of the channel
+    ring and averaged spectrum, and the first audio of one receiver of each
+    kind recomputed from the GPU's own channel rings."""
+    try:
+        from oracle import oracle
+        from revenant_amd.dsp import FORMAT_CS16, ema_frame_decay
+    except Exception as exc:  # the oracle is optional on the box
+        return f"skipped ({type(exc).__name__}: {exc})"
+    # The oracle sees the input from sample start*D on, i.e. block indices
+    # shifted by `start`; start is a multiple of the frame (and of M/D), so the
+    # phase correction and the frame alignment are unchanged by the shift.
+    check = 2 * FRAME
+    start = g.blocks // 2 - (g.blocks // 2) % FRAME
+    hi = (g.first_block + start + check - 1) * D + 1
+    sub = packed[start * D * 4:hi * 4].cpu().numpy()
+    cap = 1 << (check - 1).bit_length()
+    od = oracle.OChain(FORMAT_CS16, M, TAPS, D, g.first_block, check, cap, FRAME, alpha,
+                       ema_frame_decay(alpha, FRAME))
+    ring_cpu, part_cpu = oracle.chain(od, sub, g.proto.taps, g.tw_np, threads=4)
+    ring_cpu = ring_cpu.reshape(M, cap, 2)
+    g_ring = g.ring.view(M, g.ring_blocks, 2)
+    ok = True
+    for b in range(check):
+        m = g.first_block + start + b
+        col = ring_cpu[:, (g.first_block + b) & (cap - 1)]
+        ok &= np.array_equal(g_ring[:, m & (g.ring_blocks - 1)].cpu().numpy().view(np.uint32),
+                             col.view(np.uint32))
+    gp = partial.view(frames, M)[start // FRAME:start // FRAME + 2].cpu().numpy()
+    ok &= np.array_equal(gp.view(np.uint32), part_cpu.reshape(2, M).view(np.uint32))
+    what = f"chain bit-exact vs oracle on {check} blocks" if ok else "chain MISMATCH vs oracle"
+    if batch is None:
+        return what
+    from revenant_amd.dsp import build_nco_table
+    nco = build_nco_table(16)
+    seen, rx_ok, n_rx = set(), True, 0
+    for v, rx in enumerate(batch.receivers):
+        kind = (rx.plan.demod.mode, rx.plan.demod.decimation)
+        if kind in seen:
+            continue
+        seen.add(kind)
+        n = min(600, counts[v])
+        p = rx.plan
+        rows = rx.ring.numel() // 2 // (rx.channel_mask + 1)
+        chan = rx.ring.view(rows, rx.channel_mask + 1, 2)[rx.channel_base // (rx.channel_mask + 1)]
+        chan = chan.cpu().numpy()
+        r, hist = p.demod.decimation, p.history
+        fine_count = (n - 1) * r + hist + 1
+        out_cap = 1 << (fine_count - 1).bit_length()
+        fp, _ = p.fine_block(0, rx.channel_mask, out_cap - 1, starts[v] * r - hist, fine_count)
+        fine = oracle.vrx_fine(p.fine, fp, chan.ravel(), p.fine_taps, nco, out_cap)
+        dp = p.demod_block(out_cap - 1, starts[v], n)
+        ref = oracle.vrx_demod(p.demod, dp, fine, p.weights)
+        got = audio[v][:n * p.demod.channels].cpu().numpy()
+        rx_ok &= np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+        n_rx += 1
+    return what + (f"; audio bit-exact vs oracle for {n_rx} receiver kinds" if rx_ok else
+                   "; audio MISMATCH vs oracle")
+
+
+def traffic_from_profiles(kernel: str):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
     calibrated against a copy kernel), as committed under profiles/."""
-    p = ROOT / "profiles" / "chain_c2_traffic.json"
+    p = ROOT / "profiles" / f"{kernel}_traffic.json"
     if p.exists():
         try:
             return json.loads(p.read_text()).get("bytes_per_launch")
@@ -223,37 +410,88 @@ def traffic_from_profiles():
     return None
 
 
-def cpu_baseline(packed, taps, tw, first_block, seconds):
-    """The oracle chain (convert -> branch -> transform -> correction -> frame
-    average, the reference twins' operation order) on this host's cores, over
-    a bounded sample of the same workload, repeated to about `seconds`."""
+def cpu_baseline(packed, main: Grid, wide: Grid | None, batch, seconds):
+    """The oracle over a bounded sample of the same workload on this host's
+    cores: the 4096-channel chain (convert -> branch -> transform ->
+    correction -> frame average), and with receivers the 512-channel chain and
+    every receiver's fine stage + demodulator over the sample's channel
+    samples, repeated to about `seconds`."""
     try:
         from oracle import oracle
     except Exception as exc:
         return {"value": None, "unit": "MS/s", "cores": 0, "kind": "port",
                 "sample": f"oracle unavailable: {exc}"}
-    from revenant_amd.dsp import FORMAT_CS16, ema_frame_decay
+    from revenant_amd.dsp import FORMAT_CS16, build_nco_table, ema_frame_decay
 
     threads = max(1, min(16, len(os.sched_getaffinity(0))))
     alpha = np.float32(ALPHA)
     decay = ema_frame_decay(alpha, FRAME)
-    blocks = 8192 - 8192 % (FRAME * threads)
-    hi = (first_block + blocks) * D + 1
+    blocks = 2048 - 2048 % (FRAME * threads)
+    hi = (main.first_block + blocks) * D + 1
     sub = packed[: hi * 4].cpu().numpy()
     cap = 1 << (blocks - 1).bit_length()
-    od = oracle.OChain(FORMAT_CS16, M, TAPS, D, first_block, blocks, cap, FRAME, alpha, decay)
+    od = oracle.OChain(FORMAT_CS16, M, TAPS, D, main.first_block, blocks, cap, FRAME, alpha,
+                       decay)
     ring = np.zeros(2 * M * cap, dtype=np.float32)
     partial = np.zeros((blocks // FRAME) * M, dtype=np.float32)
+    wide_od = wide_ring = None
+    if wide is not None:
+        wblocks = (hi - 1) // WD - wide.first_block + 1
+        wblocks -= wblocks % threads
+        wcap = 1 << (wblocks - 1).bit_length()
+        wide_od = oracle.OChain(FORMAT_CS16, WM, TAPS, WD, wide.first_block, wblocks, wcap, 0,
+                                alpha, 0.0)
+        wide_ring = np.zeros(2 * WM * wcap, dtype=np.float32)
+    nco = build_nco_table(16)
+
+    def receivers():
+        # every receiver over the channel samples of this sample
+        jobs = []
+        for rx in batch.receivers:
+            p = rx.plan
+            if rx.ring is main.ring:
+                first, last, rows, rcap, src = main.first_block, main.first_block + blocks - 1, \
+                    M, cap, ring
+            else:
+                first, last, rows, rcap, src = wide.first_block, \
+                    wide.first_block + wide_od.block_count - 1, WM, wide_od.out_ring_blocks, \
+                    wide_ring
+            a0, n = rx.audio_range(first, last)
+            if n == 0:
+                continue
+            k = p.plan.placement.channel
+            jobs.append((p, src.reshape(rows, rcap, 2)[k], rcap, a0, n))
+
+        def one(job):
+            p, chan, rcap, a0, n = job
+            r, hist = p.demod.decimation, p.history
+            fine_count = (n - 1) * r + hist + 1
+            out_cap = 1 << (fine_count - 1).bit_length()
+            fp, _ = p.fine_block(0, rcap - 1, out_cap - 1, a0 * r - hist, fine_count)
+            fine = oracle.vrx_fine(p.fine, fp, np.ascontiguousarray(chan).ravel(),
+                                   p.fine_taps, nco, out_cap)
+            dp = p.demod_block(out_cap - 1, a0, n)
+            return oracle.vrx_demod(p.demod, dp, fine, p.weights).size
+
+        with ThreadPoolExecutor(threads) as pool:
+            return sum(pool.map(one, jobs))
+
     passes, total = 0, 0.0
     while total < seconds or passes == 0:
         t0 = time.perf_counter()
-        oracle.chain(od, sub, taps, tw, threads=threads, ring=ring, partial=partial)
+        oracle.chain(od, sub, main.proto.taps, main.tw_np, threads=threads, ring=ring,
+                     partial=partial)
+        if wide is not None:
+            oracle.chain(wide_od, sub, wide.proto.taps, wide.tw_np, threads=threads,
+                         ring=wide_ring)
+            receivers()
         total += time.perf_counter() - t0
         passes += 1
     rate = passes * blocks * D / total / 1e6
+    what = "4096-ch chain" + (", 512-ch chain and 256 receivers" if wide is not None else "")
     return {"value": round(rate, 2), "unit": "MS/s", "cores": threads, "kind": "port",
-            "sample": f"{passes} passes over {blocks} blocks ({blocks * D} input samples) of "
-                      f"the same workload, {total:.1f} s on {threads} threads"}
+            "sample": f"{passes} passes over {blocks * D} input samples ({blocks} blocks) of "
+                      f"the same workload ({what}), {total:.1f} s on {threads} threads"}
 
 
 if __name__ == "__main__":

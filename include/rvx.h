@@ -470,7 +470,12 @@ int32_t rvx_synth(rvx_ctx* ctx, const rvx_carrier* carriers, uint32_t ncarriers,
  * them); block m (absolute, first_block <= m < first_block + block_count) is
  * taken at base m*D and needs samples m*D - M*(L+1) + 1 .. m*D. Writes the
  * channel ring (channel-major, slot m & (out_ring_blocks - 1)) and, when
- * frame_blocks != 0, the EMA partials of block_count / frame_blocks frames. */
+ * frame_blocks != 0, the EMA partials of block_count / frame_blocks frames.
+ * channel_rows (device, M uint32, optional): channel k is stored in ring row
+ * channel_rows[k] of a ring of ring_rows rows, and not stored when that is
+ * >= ring_rows; null stores every channel in row k (ring_rows is then
+ * ignored). A coarse grid that exists only to feed a few wide receivers keeps
+ * just their channels; the averaged spectrum still covers every channel. */
 typedef struct rvx_chain_desc {
     int32_t format;           /* RVX_FORMAT_CU8 or RVX_FORMAT_CS16 */
     uint32_t channels;
@@ -483,6 +488,9 @@ typedef struct rvx_chain_desc {
     uint32_t out_ring_blocks;
     uint32_t frame_blocks;
     float alpha;
+    uint32_t ring_rows;
+    uint32_t reserved;
+    const uint32_t* channel_rows;
 } rvx_chain_desc;
 
 int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const void* input,

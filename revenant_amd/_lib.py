@@ -89,7 +89,8 @@ class ChainDesc(C.Structure):
     _fields_ = [("format", i32), ("channels", u32), ("taps_per_branch", u32),
                 ("decimation", u32), ("input_origin", u64), ("input_samples", u64),
                 ("first_block", u64), ("block_count", u32), ("out_ring_blocks", u32),
-                ("frame_blocks", u32), ("alpha", f32)]
+                ("frame_blocks", u32), ("alpha", f32), ("ring_rows", u32), ("reserved", u32),
+                ("channel_rows", vp)]
 
 
 class VrxRequest(C.Structure):

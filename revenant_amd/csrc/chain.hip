@@ -52,7 +52,15 @@ struct ChainArgs {
     uint32_t chunks;  // workgroups
     uint32_t average; // nonzero: write the frame partials
     float alpha;
+    const uint32_t* rows;  // ring row of channel k (null: row k); rows >= ring_rows are not stored
+    uint32_t ring_rows;
 };
+
+// Ring row of channel k, or ring_rows when the channel is not stored.
+__device__ __forceinline__ uint32_t ring_row(const ChainArgs& a, uint32_t k) {
+    const uint32_t row = a.rows ? a.rows[k] : k;
+    return row < a.ring_rows ? row : a.ring_rows;
+}
 
 template <int LOGM, int LOGV_ = 2>
 struct ChainShape {
@@ -322,7 +330,8 @@ chain_kernel(ChainArgs a) {
             if constexpr (STAGE > 1) {
                 stage[(b0 % STAGE) * M + k] = value;
             } else {
-                if (active) a.ring[uint64_t(k) * a.ring_blocks + slot] = value;
+                const uint32_t row = ring_row(a, k);
+                if (active && row < a.ring_rows) a.ring[uint64_t(row) * a.ring_blocks + slot] = value;
             }
             x[r] = value.x * value.x + value.y * value.y;
         }
@@ -338,7 +347,9 @@ chain_kernel(ChainArgs a) {
 #pragma unroll
                 for (int r = 0; r < V; ++r) {
                     const uint32_t k = F::template pos<LAST>(t, r);
-                    float2* row = a.ring + uint64_t(k) * a.ring_blocks;
+                    const uint32_t ring_r = ring_row(a, k);
+                    if (ring_r >= a.ring_rows) continue;
+                    float2* row = a.ring + uint64_t(ring_r) * a.ring_blocks;
                     if (whole) {
                         const float2 v0 = stage[0 * M + k], v1 = stage[1 * M + k];
                         const float2 v2 = stage[2 * M + k], v3 = stage[3 * M + k];
@@ -445,10 +456,13 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
         return rvx::fail(RVX_EINVAL,
                          "%s: %u blocks into a %u-block ring (power of two, >= blocks)", who,
                          d.block_count, d.out_ring_blocks);
-    if (ring_samples < uint64_t(M) * d.out_ring_blocks)
+    const uint32_t rows = d.channel_rows ? d.ring_rows : M;
+    if (d.channel_rows && (d.ring_rows == 0 || d.ring_rows > M))
+        return rvx::fail(RVX_EINVAL, "%s: %u ring rows for %u channels", who, d.ring_rows, M);
+    if (ring_samples < uint64_t(rows) * d.out_ring_blocks)
         return rvx::fail(RVX_EINVAL, "%s: channel ring holds %llu samples, needs %llu", who,
                          (unsigned long long)ring_samples,
-                         (unsigned long long)(uint64_t(M) * d.out_ring_blocks));
+                         (unsigned long long)(uint64_t(rows) * d.out_ring_blocks));
     const uint64_t N = uint64_t(M) * T;
     if (d.input_samples >= (uint64_t(1) << 31))
         return rvx::fail(RVX_EINVAL, "%s: %llu input samples exceed 2^31", who,
@@ -495,6 +509,8 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
     a.chunks = (d.block_count + chunk - 1) / chunk;
     a.average = d.frame_blocks != 0;
     a.alpha = d.alpha;
+    a.rows = d.channel_rows;
+    a.ring_rows = rows;
     const uint32_t logm = rvx::log2u(M);
     const int32_t rc = d.format == RVX_FORMAT_CS16
                            ? launch_chain<RVX_FORMAT_CS16>(logm, a, T, ctx->stream)

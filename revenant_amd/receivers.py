@@ -83,6 +83,23 @@ class Receiver:
         r, hist = plan.demod.decimation, plan.history
         return max(1, (fine - hist - 1) // r + 1)
 
+    def audio_range(self, first_sample: int, last_sample: int) -> tuple[int, int]:
+        """(first_audio, count): the audio outputs whose every input lies in
+        channel samples first_sample .. last_sample (absolute indices).
+        Fine output j reads samples floor(j*Fc/Fd) - (taps-1) .. floor(j*Fc/Fd)
+        (vrx_reference.cpp:489-620); audio a reads fine a*R - hist .. a*R."""
+        plan = self.plan
+        fc, fd = plan.plan.channel_rate, plan.plan.demod_rate
+        r, hist, taps = plan.demod.decimation, plan.history, plan.fine.taps
+        # lowest fine j with floor(j*Fc/Fd) >= first_sample + taps - 1
+        lo_n = first_sample + taps - 1
+        j_lo = -(-lo_n * fd // fc)
+        # highest fine j with floor(j*Fc/Fd) <= last_sample
+        j_hi = ((last_sample + 1) * fd - 1) // fc
+        a_lo = -(-(j_lo + hist) // r)
+        a_hi = j_hi // r
+        return a_lo, max(0, a_hi - a_lo + 1)
+
     def first_input(self, first_audio: int) -> int:
         """Lowest channel sample index the audio block from first_audio reads."""
         f0 = first_audio * self.plan.demod.decimation - self.plan.history
@@ -143,8 +160,10 @@ class ReceiverBatch:
         if tiles is not None:
             self.tiles = [min(t, max_tile(r.plan)) for t, r in zip(tiles, self.receivers)]
         descs: list[_lib.VrxDesc] = []
-        items: list[tuple[int, int, int]] = []
+        vs, js, costs = [], [], []
         for r, a0, n, out, t in zip(self.receivers, first_audio, counts, audio, self.tiles):
+            if n == 0:
+                continue
             t = max(1, min(t, n))
             step = r.max_desc_audio()
             step -= step % t  # whole tiles per descriptor
@@ -154,11 +173,19 @@ class ReceiverBatch:
                 m = min(step, n - lo)
                 part = out[lo * ch:(lo + m) * ch]
                 descs.append(r.descriptor(a0 + lo, m, part, t))
-                v = len(descs) - 1
-                for j in range((m + t - 1) // t):
-                    items.append((-tile_cost(r.plan, min(t, m - j * t)), v, j))
-        items.sort()
-        work = np.array([(v, j) for _, v, j in items], dtype=np.uint32).reshape(-1, 2)
+                tiles_here = (m + t - 1) // t
+                j = np.arange(tiles_here, dtype=np.int64)
+                vs.append(np.full(tiles_here, len(descs) - 1, dtype=np.int64))
+                js.append(j)
+                cost = np.full(tiles_here, tile_cost(r.plan, t), dtype=np.int64)
+                cost[-1] = tile_cost(r.plan, m - (tiles_here - 1) * t)
+                costs.append(cost)
+        if not descs:
+            raise ValueError("no receiver has audio to compute")
+        v_all, j_all, c_all = np.concatenate(vs), np.concatenate(js), np.concatenate(costs)
+        order = np.lexsort((j_all, v_all, -c_all))  # largest first, then by receiver, tile
+        work = np.stack([v_all[order], j_all[order]], axis=1).astype(np.uint32)
+        items = order
         device = self.nco.device
         table = (_lib.VrxDesc * len(descs))(*descs)
         self._descs = torch.frombuffer(bytearray(bytes(table)), dtype=torch.uint8).to(device)

@@ -186,3 +186,41 @@ def test_chain_refuses_missing_history(dev):
     tw = to_dev(build_twiddles(64).view(np.float32))
     with pytest.raises(_lib.RvxError):
         dev.chain(desc, to_dev(np.zeros(20000, dtype=np.uint8)), taps, tw, ring)
+
+
+@pytest.mark.parametrize("grid,blocks", [(GridParams(512, 16, 256), 64),
+                                         (GridParams(4096, 16, 2048), 24)],
+                         ids=["M512", "M4096"])
+def test_chain_stores_selected_channels(dev, grid, blocks):
+    """With a channel-row table the launch stores only the chosen channels,
+    each in its row of a compact ring, bit-identical to the full ring's."""
+    M, D, N = grid.channels, grid.decimation, grid.prototype_length
+    first_block = (N + D - 1) // D + 5
+    samples = (first_block + blocks) * D + 3
+    packed = _codes(FORMAT_CS16, samples, SEED_CHAIN + 7 * M)
+    taps = design_prototype(grid, beta=8.0).taps
+    tw = build_twiddles(M)
+    ring_blocks = 1 << (blocks - 1).bit_length()
+    alpha = np.float32(1.0 / 16.0)
+    od = oracle.OChain(FORMAT_CS16, M, 16, D, first_block, blocks, ring_blocks, 4, alpha,
+                       ema_frame_decay(alpha, 4))
+    ring_cpu, partial_cpu = oracle.chain(od, packed, taps, tw, threads=4)
+    chosen = [M - 1, 3, 0, M // 2 + 1, 77]
+    rows = np.full(M, 0xFFFFFFFF, dtype=np.uint32)
+    rows[9] = len(chosen) + 4   # beyond ring_rows: not stored either
+    for r, k in enumerate(chosen):
+        rows[k] = r
+    desc = _lib.ChainDesc(FORMAT_CS16, M, 16, D, 0, samples, first_block, blocks, ring_blocks,
+                          4, alpha)
+    rows_dev = to_dev(rows.view(np.int32))
+    desc.ring_rows = len(chosen)
+    desc.channel_rows = rows_dev.data_ptr()
+    ring = torch.full((2 * len(chosen) * ring_blocks,), 3.0, dtype=torch.float32, device="cuda")
+    partial = torch.zeros((blocks // 4) * M, dtype=torch.float32, device="cuda")
+    dev.chain(desc, to_dev(packed), to_dev(taps), to_dev(tw.view(np.float32)), ring, partial)
+    got = from_dev(ring).reshape(len(chosen), ring_blocks, 2)
+    full = ring_cpu.reshape(M, ring_blocks, 2)
+    slots = [(first_block + b) & (ring_blocks - 1) for b in range(blocks)]
+    for r, k in enumerate(chosen):
+        assert_bit_exact(got[r, slots], full[k, slots], SEED_CHAIN, f"row {r} = channel {k}")
+    assert_bit_exact(from_dev(partial), partial_cpu, SEED_CHAIN, "spectrum over every channel")
