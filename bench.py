@@ -216,8 +216,15 @@ def main() -> int:
             launches["chain_m4096"]()
             batch.prepare(starts, counts, audio)
             if not args.no_tune:
-                log("tuning receiver tiles")
+                dev.timer_begin()
+                batch.launch()
+                before = dev.timer_end()
                 tiles = batch.tune(starts, counts, audio)
+                dev.timer_begin()
+                batch.launch()
+                after = dev.timer_end()
+                chosen = sorted({(p.plan.demod.mode, t) for (n, p), t in zip(plans, tiles)})
+                log(f"tuned receiver tiles (mode, tile) {chosen}: {before:.3f} -> {after:.3f} ms")
             launches["vrx_batch"] = batch.launch
         side.synchronize()
 

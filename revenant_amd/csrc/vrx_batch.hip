@@ -25,8 +25,8 @@
 namespace {
 
 constexpr uint32_t kBatchThreads = 256;
-constexpr uint32_t kBatchLdsBytes = 64 * 1024;  // fine (8 B) + magnitude + detector per sample
-constexpr uint32_t kMaxSpan = kBatchLdsBytes / 16;
+constexpr uint32_t kMaxSpan = 2048;     // fine samples per tile: fine (8 B) + magnitude + detector
+constexpr uint32_t kTapEntries = 2048;  // interpolated taps of every phase class (8 B each)
 static_assert(sizeof(rvx_vrx_desc) == 160, "descriptor layout is part of the ABI");
 
 __host__ __device__ bool complex_tap(uint32_t mode) { return mode == 0 || (mode >= 8 && mode <= 11); }
@@ -43,11 +43,11 @@ __global__ void __launch_bounds__(kBatchThreads)
 vrx_batch_kernel(const rvx_vrx_desc* __restrict__ descs, uint32_t n_desc,
                  const rvx_vrx_work* __restrict__ work, uint32_t n_work, uint32_t* __restrict__ counter,
                  const float2* __restrict__ nco, uint32_t nco_log2_max) {
-    __shared__ float4 smem4[kBatchLdsBytes / 16];
+    __shared__ float2 tap_lds[kTapEntries];
+    __shared__ float2 fine[kMaxSpan];
+    __shared__ float mag[kMaxSpan];
+    __shared__ float det[kMaxSpan];
     __shared__ uint32_t item_sh;
-    float2* fine = reinterpret_cast<float2*>(smem4);
-    float* mag = reinterpret_cast<float*>(fine + kMaxSpan);
-    float* det = mag + kMaxSpan;
     const uint32_t tid = threadIdx.x;
 
     for (;;) {
@@ -82,9 +82,24 @@ vrx_batch_kernel(const rvx_vrx_desc* __restrict__ descs, uint32_t n_desc,
         const float* wts = d.weights;
         float* audio = d.audio;
 
-        // 1. fine samples a0*R .. a0*R + span - 1 of the receiver's fine block
-        for (uint32_t s = tid; s < span; s += kBatchThreads)
-            fine[s] = rvx::fine_output(fc, fp, ch, taps, nco, a0 * R + s);
+        // 1. fine samples a0*R .. a0*R + span - 1 of the receiver's fine block.
+        // Fine indices one phase period apart share their interpolated taps,
+        // so when the period's taps fit they are formed once per tile.
+        const uint32_t period = rvx::phase_period(fp);
+        if (uint64_t(period) * fc.taps <= kTapEntries) {
+            const uint32_t entries = period * fc.taps;
+            for (uint32_t e = tid; span && e < entries; e += kBatchThreads)
+                rvx::fine_taps_row(fc, fp, taps, e / fc.taps, e % fc.taps,
+                                   tap_lds + (e / fc.taps) * fc.taps);
+            __syncthreads();
+            for (uint32_t s = tid; s < span; s += kBatchThreads) {
+                const uint32_t i = a0 * R + s;
+                fine[s] = rvx::fine_output_row(fc, fp, ch, tap_lds + (i % period) * fc.taps, nco, i);
+            }
+        } else {
+            for (uint32_t s = tid; s < span; s += kBatchThreads)
+                fine[s] = rvx::fine_output(fc, fp, ch, taps, nco, a0 * R + s);
+        }
         __syncthreads();
 
         if (complex_tap(dc.mode)) {

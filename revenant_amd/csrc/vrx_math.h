@@ -104,4 +104,79 @@ __device__ __forceinline__ float2 fine_output(const rvx_vrx_fine_config& c,
     return make_float2(orr, oi);
 }
 
+// Resampler phase period: fine indices i and i + period share (phase, blend),
+// since period * step_rem is a multiple of out_rate.
+__host__ __device__ __forceinline__ uint32_t phase_period(const rvx_vrx_fine_params& p) {
+    uint32_t a = p.step_rem, b = p.out_rate;
+    while (a != 0) {
+        const uint32_t t = b % a;
+        b = a;
+        a = t;
+    }
+    return p.out_rate / b;  // b = gcd(step_rem, out_rate); step_rem = 0 gives 1
+}
+
+// The interpolated taps of phase class cls (fine indices i = cls mod period):
+// exactly the (tr, ti) fine_output forms for such an i, written to row[0..taps).
+__device__ __forceinline__ void fine_taps_row(const rvx_vrx_fine_config& c,
+                                              const rvx_vrx_fine_params& p,
+                                              const float2* __restrict__ taps, uint32_t cls,
+                                              uint32_t k, float2* row) {
+    const uint32_t frac = uint32_t((uint64_t(p.frac0) + uint64_t(cls) * p.step_rem) % p.out_rate);
+    const uint32_t scaled = frac * c.phases;
+    const uint32_t phase = scaled / p.out_rate;
+    const uint32_t blend_num = scaled - phase * p.out_rate;
+    const float blend = float(blend_num) * p.inv_out_rate;
+    const float comp = 1.0f - blend;
+    const uint32_t slot = phase + k * c.phases;
+    const float2 g0 = taps[slot];
+    const float2 g1 = taps[slot + 1];
+    const float tr = g0.x * comp + g1.x * blend;
+    const float ti = g0.y * comp + g1.y * blend;
+    row[k] = make_float2(tr, ti);
+}
+
+// fine_output with the interpolated taps of i's phase class already in `row`.
+__device__ __forceinline__ float2 fine_output_row(const rvx_vrx_fine_config& c,
+                                                  const rvx_vrx_fine_params& p,
+                                                  const float2* __restrict__ ch,
+                                                  const float2* row,
+                                                  const float2* __restrict__ nco, uint32_t i) {
+    const uint32_t acc_idx = p.frac0 + i * p.step_rem;
+    const uint32_t carry = acc_idx / p.out_rate;
+    const uint32_t n_rel = i * p.step_whole + carry;
+    const uint32_t base = p.in_offset + n_rel;
+    float ar = 0.0f, ai = 0.0f;
+    uint32_t k = 0;
+    for (; k + 4 <= c.taps; k += 4) {
+        float2 s[4], t[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            s[u] = ch[(base - (k + u)) & p.chan_mask];
+            t[u] = row[k + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            ar = ar + (s[u].x * t[u].x - s[u].y * t[u].y);
+            ai = ai + (s[u].x * t[u].y + s[u].y * t[u].x);
+        }
+    }
+    for (; k < c.taps; ++k) {
+        const float2 s = ch[(base - k) & p.chan_mask];
+        const float2 t = row[k];
+        ar = ar + (s.x * t.x - s.y * t.y);
+        ai = ai + (s.x * t.y + s.y * t.x);
+    }
+    const uint32_t prod_lo = i * p.nco_delta_low;
+    const uint32_t prod_hi = __umulhi(i, p.nco_delta_low);
+    const uint32_t ph_lo = p.nco_phase_low + prod_lo;
+    const uint32_t carry_bit = ph_lo < p.nco_phase_low ? 1u : 0u;
+    const uint32_t ph_hi = p.nco_phase_high + i * p.nco_delta_high + prod_hi + carry_bit;
+    const uint32_t ni = c.nco_log2 == 0 ? 0u : (ph_hi >> (32 - c.nco_log2));
+    const float2 r = nco[ni];
+    const float orr = ar * r.x - ai * r.y;
+    const float oi = ar * r.y + ai * r.x;
+    return make_float2(orr, oi);
+}
+
 }  // namespace rvx

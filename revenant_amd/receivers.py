@@ -27,7 +27,7 @@ from . import _lib
 from .dsp import VrxPlan
 from .ops import Device, _p
 
-MAX_SPAN = 4096           # fine samples one tile holds on chip (64 KiB of LDS)
+MAX_SPAN = 2048           # fine samples one tile holds on chip
 _UNBOUNDED = 0xFFFFFFFF   # ring masks of the on-chip fine block: no wrap
 
 

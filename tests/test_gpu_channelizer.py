@@ -140,6 +140,7 @@ CHAINS = [
     (FORMAT_CS16, GridParams(4096, 16, 2048), 64, 16),
     (FORMAT_CS16, GridParams(4096, 17, 2048), 32, 8),
     (FORMAT_CS16, GridParams(1024, 16, 512), 48, 0),
+    (FORMAT_CS16, GridParams(512, 16, 256), 64, 4),
     (FORMAT_CS16, GridParams(8, 5, 4), 40, 10),
     (FORMAT_CU8, GridParams(256, 12, 256), 32, 4),
 ]
