@@ -421,6 +421,35 @@ int32_t rvx_vrx_batch(rvx_ctx* ctx, const rvx_vrx_desc* descs, uint32_t n_desc,
  * that a tile fits the LDS budget). */
 int32_t rvx_vrx_desc_check(const rvx_vrx_desc* desc);
 
+/* Fine samples one batched work item holds on chip: a receiver's tile of n
+ * audio outputs needs (n - 1) * R + hist + 1 <= RVX_VRX_TILE_SPAN. */
+#define RVX_VRX_TILE_SPAN 2048
+
+/* One receiver's share of a batched launch: audio outputs first_audio ..
+ * first_audio + count - 1, written to `audio` (count * channels floats). */
+typedef struct rvx_vrx_span {
+    const rvx_vrx_plan* plan;  /* host */
+    uint32_t channel_base;     /* cf32 index of the receiver's channel in channel_ring */
+    uint32_t channel_mask;     /* channel length - 1 */
+    const void* channel_ring;  /* device */
+    const void* taps;          /* device, the plan's fine taps (phases*taps + 1 cf32) */
+    const float* weights;      /* device, the plan's demod weights */
+    float* audio;              /* device */
+    uint64_t first_audio;
+    uint32_t count;
+    uint32_t tile;             /* audio outputs per work item; 0: the largest that fits */
+} rvx_vrx_span;
+
+/* Host: the descriptors and the work list of one rvx_vrx_batch launch over
+ * `spans` (the per-step record of DemodStage::record,
+ * core/engine/vrx_stage.cpp:1119-1480, for a whole receiver set). A span is
+ * split into several descriptors where the fine block's 32-bit resampler
+ * recurrence would overflow; work items are ordered by estimated cost, largest
+ * first. With null descs/work only the counts are returned. */
+int32_t rvx_vrx_batch_plan(const rvx_vrx_span* spans, uint32_t n_spans, rvx_vrx_desc* descs,
+                           uint32_t max_descs, uint32_t* n_descs, rvx_vrx_work* work,
+                           uint32_t max_work, uint32_t* n_work);
+
 /* Graph capture of the context's own stream (HIP graphs): begin, end, launch. */
 typedef struct rvx_graph rvx_graph;
 int32_t rvx_graph_begin(rvx_ctx* ctx);
@@ -496,6 +525,65 @@ typedef struct rvx_chain_desc {
 int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const void* input,
                   const float* taps, uint64_t n_taps, const void* twiddles, uint64_t n_twiddles,
                   void* channel_ring, uint64_t ring_samples, float* partial, uint64_t n_partial);
+
+/* ---- Streaming engine ------------------------------------------------------
+ * The per-block orchestration of Graph::on_block (core/engine/graph.cpp:
+ * 4770-5480) and the receivers' stream cursors of DemodStage
+ * (core/engine/vrx_stage.cpp:1119-1480), over the fused chain and the batched
+ * receivers. Samples arrive in pushes of any size; each push channelizes every
+ * block whose history is complete (whole spectrum rows when the averaged
+ * spectrum is on), then runs every receiver over the audio its channel
+ * samples now allow. Every output is a pure function of its absolute index,
+ * so results do not depend on how the stream is cut into pushes. */
+typedef struct rvx_engine rvx_engine;
+
+typedef struct rvx_engine_config {
+    int32_t format;            /* RVX_FORMAT_CU8 or RVX_FORMAT_CS16 */
+    uint32_t channels;         /* M, a power of two <= 4096 */
+    uint32_t taps_per_branch;
+    uint32_t decimation;       /* D, dividing M */
+    uint32_t max_push;         /* largest push, samples */
+    uint32_t ring_blocks;      /* channel ring depth, a power of two */
+    uint32_t frame_blocks;     /* averaged spectrum frame; 0: no averaged spectrum */
+    uint32_t row_frames;       /* frames per spectrum row */
+    float alpha;
+    uint32_t reserved;
+    uint64_t start_index;      /* absolute index of the first sample pushed */
+} rvx_engine_config;
+
+typedef struct rvx_engine_stats {
+    uint64_t samples_in;       /* samples pushed */
+    uint64_t next_block;       /* first block not yet channelized */
+    uint64_t blocks;           /* blocks channelized */
+    uint64_t spectrum_rows;    /* averaged-spectrum rows folded into the state */
+    uint64_t audio_lost;       /* receiver audio outputs skipped because their
+                                * channel samples had left the ring */
+} rvx_engine_stats;
+
+/* prototype: M * taps_per_branch floats (host), e.g. from rvx_design_prototype. */
+int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const float* prototype,
+                          uint64_t n_prototype, rvx_engine** out);
+void rvx_engine_destroy(rvx_engine* engine);
+/* A receiver planned on this engine's grid (rvx_plan_vrx); tables are host
+ * arrays (fine taps as cf32 pairs). audio_capacity: audio outputs its ring
+ * holds until read. tile: 0 = the largest that fits. */
+int32_t rvx_engine_add_vrx(rvx_engine* engine, const rvx_vrx_plan* plan, const float* fine_taps,
+                           uint64_t n_fine_floats, const float* weights, uint64_t n_weights,
+                           uint32_t audio_capacity, uint32_t tile, uint32_t* id);
+int32_t rvx_engine_remove_vrx(rvx_engine* engine, uint32_t id);
+/* codes: `samples` native samples, host memory (on_device = 0) or device. */
+int32_t rvx_engine_push(rvx_engine* engine, const void* codes, uint64_t samples,
+                        int32_t on_device);
+/* Copies the receiver's unread audio (oldest first, at most max_values floats)
+ * to host memory and waits for it; *first_audio is the index of the first
+ * output copied, *values the floats copied. */
+int32_t rvx_engine_read_audio(rvx_engine* engine, uint32_t id, float* out, uint64_t max_values,
+                              uint64_t* first_audio, uint64_t* values);
+/* The averaged spectrum state (M floats, host) after the last row; waits. */
+int32_t rvx_engine_spectrum(rvx_engine* engine, float* out, uint64_t n);
+/* The channel ring (device, channel-major, M x ring_blocks cf32). */
+const void* rvx_engine_channel_ring(rvx_engine* engine);
+int32_t rvx_engine_stats_get(rvx_engine* engine, rvx_engine_stats* out);
 
 #ifdef __cplusplus
 }

@@ -126,7 +126,27 @@ class VrxWork(C.Structure):
     _fields_ = [("vrx", u32), ("tile", u32)]
 
 
-assert C.sizeof(VrxDesc) == 160 and C.sizeof(VrxWork) == 8
+class VrxSpan(C.Structure):
+    _fields_ = [("plan", C.POINTER(VrxPlan)), ("channel_base", u32), ("channel_mask", u32),
+                ("channel_ring", vp), ("taps", vp), ("weights", vp), ("audio", vp),
+                ("first_audio", u64), ("count", u32), ("tile", u32)]
+
+
+class EngineConfig(C.Structure):
+    _fields_ = [("format", i32), ("channels", u32), ("taps_per_branch", u32),
+                ("decimation", u32), ("max_push", u32), ("ring_blocks", u32),
+                ("frame_blocks", u32), ("row_frames", u32), ("alpha", f32), ("reserved", u32),
+                ("start_index", u64)]
+
+
+class EngineStats(C.Structure):
+    _fields_ = [(n, u64) for n in ("samples_in", "next_block", "blocks", "spectrum_rows",
+                                   "audio_lost")]
+
+
+VRX_TILE_SPAN = 2048  # RVX_VRX_TILE_SPAN
+
+assert C.sizeof(VrxDesc) == 160 and C.sizeof(VrxWork) == 8 and C.sizeof(VrxSpan) == 64
 assert C.sizeof(VrxFineParams) == 60 and C.sizeof(VrxDemodParams) == 16
 assert C.sizeof(ConvertParams) == 16 and C.sizeof(PfbBranchParams) == 12
 
@@ -179,6 +199,18 @@ _SIGNATURES = {
     "rvx_chain": (i32, [vp, C.POINTER(ChainDesc), vp, vp, u64, vp, u64, vp, u64, vp, u64]),
     "rvx_vrx_batch": (i32, [vp, vp, u32, vp, u32, vp, vp, u64, u32]),
     "rvx_vrx_desc_check": (i32, [C.POINTER(VrxDesc)]),
+    "rvx_vrx_batch_plan": (i32, [C.POINTER(VrxSpan), u32, C.POINTER(VrxDesc), u32,
+                                 C.POINTER(u32), C.POINTER(VrxWork), u32, C.POINTER(u32)]),
+    "rvx_engine_create": (i32, [vp, C.POINTER(EngineConfig), fptr, u64, C.POINTER(vp)]),
+    "rvx_engine_destroy": (None, [vp]),
+    "rvx_engine_add_vrx": (i32, [vp, C.POINTER(VrxPlan), fptr, u64, fptr, u64, u32, u32,
+                                 C.POINTER(u32)]),
+    "rvx_engine_remove_vrx": (i32, [vp, u32]),
+    "rvx_engine_push": (i32, [vp, vp, u64, i32]),
+    "rvx_engine_read_audio": (i32, [vp, u32, fptr, u64, C.POINTER(u64), C.POINTER(u64)]),
+    "rvx_engine_spectrum": (i32, [vp, fptr, u64]),
+    "rvx_engine_channel_ring": (vp, [vp]),
+    "rvx_engine_stats_get": (i32, [vp, C.POINTER(EngineStats)]),
     "rvx_graph_begin": (i32, [vp]),
     "rvx_graph_end": (i32, [vp, C.POINTER(vp)]),
     "rvx_graph_launch": (i32, [vp, vp]),

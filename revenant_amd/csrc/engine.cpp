@@ -1,0 +1,645 @@
+// The streaming engine and the batched receivers' record step.
+//
+// rvx_vrx_batch_plan: DemodStage::record (core/engine/vrx_stage.cpp:1119-1480)
+// for a whole receiver set: per receiver the fine-block and demod-block params
+// (rvx_fine_block / rvx_demod_block, vrx_reference.cpp:2786-2909) of the audio
+// it can now produce, cut into tiles and ordered largest first.
+//
+// rvx_engine: the per-block orchestration of Graph::on_block
+// (core/engine/graph.cpp:4770-5480) over the fused chain:
+//   - samples land in a device window that keeps the filter history
+//     (M*L - 1 samples) of the next block, so a push of any size continues the
+//     stream exactly (the reference keeps a sample ring; the chain here reads
+//     native codes, so the window holds codes);
+//   - the first block is the first whose history lies in the stream
+//     (graph.cpp:5036-5050), block_count is what the window completes, in
+//     whole spectrum rows when the averaged spectrum is on;
+//   - each receiver keeps its next audio index (the reference's next_audio_
+//     cursor) and produces every output whose channel samples are in the ring;
+//     outputs whose samples have left the ring are counted as lost.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <tuple>
+#include <vector>
+
+#include "common.h"
+#include "plan.h"
+
+namespace {
+
+constexpr uint32_t kUnbounded = 0xFFFFFFFFu;
+
+uint32_t max_tile(const rvx_vrx_plan& p) {
+    const uint32_t hist = rvx::demod_fine_history(p.demod);
+    if (hist + 1 > RVX_VRX_TILE_SPAN || p.demod.decimation == 0) return 0;
+    return (RVX_VRX_TILE_SPAN - hist - 1) / p.demod.decimation + 1;
+}
+
+// Multiply count of one tile of n audio outputs, for ordering the work.
+uint64_t tile_cost(const rvx_vrx_plan& p, uint32_t n) {
+    const auto& d = p.demod;
+    const uint64_t span = uint64_t(n - 1) * d.decimation + rvx::demod_fine_history(d) + 1;
+    uint64_t cost = span * p.fine.taps * 4;
+    if (d.mode == RVX_DEMOD_AM || d.mode == 12) cost += span * d.dc_taps;
+    cost += uint64_t(n) * d.audio_taps * (d.channels == 2 && d.pilot_taps ? 3 : 1);
+    return cost;
+}
+
+// Most audio outputs one descriptor carries: the fine block's recurrence
+// frac0 + count * step_rem must stay below 2^32 (vrx_reference.cpp:429-486).
+uint64_t max_desc_audio(const rvx_vrx_plan& p) {
+    const uint64_t fc = uint64_t(p.channel_rate), fd = uint64_t(p.demod_rate);
+    const uint64_t rem = fc % fd;
+    const uint64_t fine = rem == 0 ? 0xFFFFFFFFull : (0xFFFFFFFFull - (fd - 1)) / rem;
+    const uint64_t hist = rvx::demod_fine_history(p.demod);
+    if (fine < hist + 1) return 1;
+    return (fine - hist - 1) / p.demod.decimation + 1;
+}
+
+// (first_audio, count) of the audio whose every input lies in channel samples
+// first .. last: fine output j reads floor(j*Fc/Fd) - (taps-1) .. floor(j*Fc/Fd)
+// (vrx_reference.cpp:489-620), audio a reads fine a*R - hist .. a*R.
+void audio_range(const rvx_vrx_plan& p, uint64_t first, uint64_t last, uint64_t* a0,
+                 uint64_t* n) {
+    using u128 = unsigned __int128;
+    const u128 fc = uint64_t(p.channel_rate), fd = uint64_t(p.demod_rate);
+    const uint64_t R = p.demod.decimation, hist = rvx::demod_fine_history(p.demod);
+    const u128 lo_n = u128(first) + p.fine.taps - 1;
+    const uint64_t j_lo = uint64_t((lo_n * fd + fc - 1) / fc);
+    const uint64_t j_hi = uint64_t(((u128(last) + 1) * fd - 1) / fc);
+    const uint64_t a_lo = (j_lo + hist + R - 1) / R;
+    const uint64_t a_hi = j_hi / R;
+    *a0 = a_lo;
+    *n = a_hi >= a_lo ? a_hi - a_lo + 1 : 0;
+}
+
+}  // namespace
+
+extern "C" int32_t rvx_vrx_batch_plan(const rvx_vrx_span* spans, uint32_t n_spans,
+                                      rvx_vrx_desc* descs, uint32_t max_descs,
+                                      uint32_t* n_descs, rvx_vrx_work* work, uint32_t max_work,
+                                      uint32_t* n_work) {
+    static const char* who = "rvx_vrx_batch_plan";
+    if (!n_descs || !n_work || (n_spans && !spans))
+        return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    struct Item {
+        uint64_t cost;
+        uint32_t v, j;
+    };
+    std::vector<Item> items;
+    uint32_t nd = 0;
+    for (uint32_t s = 0; s < n_spans; ++s) {
+        const rvx_vrx_span& sp = spans[s];
+        if (!sp.plan) return rvx::fail(RVX_EINVAL, "%s: span %u has no plan", who, s);
+        if (sp.count == 0) continue;
+        const rvx_vrx_plan& p = *sp.plan;
+        const uint32_t cap = max_tile(p);
+        if (cap == 0) return rvx::fail(RVX_EINVAL, "%s: span %u's filters do not fit a tile", who, s);
+        uint32_t tile = sp.tile ? std::min(sp.tile, cap) : cap;
+        tile = std::min(tile, sp.count);
+        uint64_t step = max_desc_audio(p);
+        step -= step % tile;
+        step = std::max<uint64_t>(step, tile);
+        const uint32_t R = p.demod.decimation, hist = rvx::demod_fine_history(p.demod);
+        if (sp.first_audio * R < hist)
+            return rvx::fail(RVX_EINVAL, "%s: span %u reaches below fine sample 0", who, s);
+        for (uint64_t lo = 0; lo < sp.count; lo += step) {
+            const uint32_t m = uint32_t(std::min<uint64_t>(step, sp.count - lo));
+            if (descs) {
+                if (nd >= max_descs) return rvx::fail(RVX_EINVAL, "%s: descriptor array full", who);
+                rvx_vrx_desc d{};
+                d.fine_config = p.fine;
+                d.demod_config = p.demod;
+                const uint64_t a0 = sp.first_audio + lo;
+                if (int32_t rc = rvx_fine_block(&p, sp.channel_base, sp.channel_mask, kUnbounded,
+                                                a0 * R - hist, (m - 1) * R + hist + 1, &d.fine,
+                                                nullptr))
+                    return rc;
+                if (int32_t rc = rvx_demod_block(&p, kUnbounded, a0, m, &d.demod)) return rc;
+                d.hist = hist;
+                d.tile = tile;
+                d.channel_ring = sp.channel_ring;
+                d.taps = sp.taps;
+                d.weights = sp.weights;
+                d.audio = sp.audio + lo * p.demod.channels;
+                if (int32_t rc = rvx_vrx_desc_check(&d)) return rc;
+                descs[nd] = d;
+            }
+            const uint32_t tiles = (m + tile - 1) / tile;
+            for (uint32_t j = 0; j < tiles; ++j)
+                items.push_back({tile_cost(p, std::min(tile, m - j * tile)), nd, j});
+            ++nd;
+        }
+    }
+    *n_descs = nd;
+    *n_work = uint32_t(items.size());
+    if (items.size() > 0xFFFFFFFFull) return rvx::fail(RVX_EINVAL, "%s: too many tiles", who);
+    if (!work) return RVX_OK;
+    if (items.size() > max_work) return rvx::fail(RVX_EINVAL, "%s: work array full", who);
+    std::sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+        return std::make_tuple(b.cost, a.v, a.j) < std::make_tuple(a.cost, b.v, b.j);
+    });
+    for (size_t i = 0; i < items.size(); ++i) work[i] = {items[i].v, items[i].j};
+    return RVX_OK;
+}
+
+// ---------------------------------------------------------------------------
+
+namespace {
+
+struct EngineRx {
+    uint32_t id = 0;
+    rvx_vrx_plan plan{};
+    void* taps = nullptr;
+    float* weights = nullptr;
+    float* audio = nullptr;
+    uint64_t cap = 0;  // audio outputs the ring holds
+    uint32_t tile = 0;
+    bool started = false;
+    uint64_t next_audio = 0;
+    uint64_t read_audio = 0;
+};
+
+template <class T>
+int32_t device_alloc(T** p, uint64_t bytes, const char* who) {
+    void* q = nullptr;
+    const hipError_t err = hipMalloc(&q, bytes);
+    if (err != hipSuccess)
+        return rvx::fail(RVX_ENOMEM, "%s: hipMalloc of %llu bytes failed: %s", who,
+                         (unsigned long long)bytes, hipGetErrorString(err));
+    *p = static_cast<T*>(q);
+    return RVX_OK;
+}
+
+}  // namespace
+
+struct rvx_engine {
+    rvx_ctx* ctx = nullptr;
+    rvx_engine_config cfg{};
+    uint64_t N = 0;  // prototype length M*L
+    uint32_t bps = 0;
+    float* taps = nullptr;
+    void* tw = nullptr;
+    void* nco = nullptr;
+    uint32_t nco_entries = 0;
+    uint8_t* win = nullptr;
+    uint8_t* bounce = nullptr;
+    uint8_t* stage_h = nullptr;  // pinned copy of a host push
+    hipEvent_t pushed = nullptr;
+    uint64_t win_cap = 0, win_origin = 0, win_fill = 0;
+    void* ring = nullptr;
+    float* partial = nullptr;
+    float* rows = nullptr;
+    float* state = nullptr;
+    uint32_t max_blocks = 0;
+    uint64_t next_block = 0, first_block = 0;
+    bool any = false;
+    std::vector<EngineRx> rx;
+    uint32_t next_id = 1;
+    rvx_vrx_desc* descs_h = nullptr;
+    rvx_vrx_work* work_h = nullptr;
+    void* descs_d = nullptr;
+    void* work_d = nullptr;
+    uint32_t descs_cap = 0, work_cap = 0;
+    uint32_t* counter = nullptr;
+    hipEvent_t staged = nullptr;
+    rvx_engine_stats stats{};
+
+    ~rvx_engine() {
+        if (ctx) (void)hipStreamSynchronize(ctx->stream);
+        for (auto& r : rx) {
+            (void)hipFree(r.taps);
+            (void)hipFree(r.weights);
+            (void)hipFree(r.audio);
+        }
+        for (void* p : {(void*)taps, tw, nco, (void*)win, (void*)bounce, ring, (void*)partial,
+                        (void*)rows, (void*)state, descs_d, work_d, (void*)counter})
+            if (p) (void)hipFree(p);
+        if (stage_h) (void)hipHostFree(stage_h);
+        if (pushed) (void)hipEventDestroy(pushed);
+        if (descs_h) (void)hipHostFree(descs_h);
+        if (work_h) (void)hipHostFree(work_h);
+        if (staged) (void)hipEventDestroy(staged);
+    }
+};
+
+namespace {
+
+int32_t engine_receivers(rvx_engine& e, const char* who) {
+    const uint64_t newest = e.next_block - 1;
+    const uint64_t ring = e.cfg.ring_blocks;
+    const uint64_t oldest = std::max(e.first_block, newest + 1 >= ring ? newest + 1 - ring : 0);
+    std::vector<rvx_vrx_span> spans;
+    for (auto& r : e.rx) {
+        uint64_t a_lo = 0, n = 0;
+        audio_range(r.plan, oldest, newest, &a_lo, &n);
+        if (n == 0) continue;
+        const uint64_t a_hi = a_lo + n - 1;
+        if (!r.started) {
+            r.started = true;
+            r.next_audio = r.read_audio = a_lo;
+        }
+        if (r.next_audio < a_lo) {
+            e.stats.audio_lost += a_lo - r.next_audio;
+            r.next_audio = a_lo;
+        }
+        if (r.next_audio > a_hi) continue;
+        uint64_t count = a_hi - r.next_audio + 1;
+        if (count > r.cap) {  // only the last cap outputs fit the audio ring
+            e.stats.audio_lost += count - r.cap;
+            r.next_audio += count - r.cap;
+            count = r.cap;
+        }
+        const uint32_t ch = r.plan.demod.channels;
+        uint64_t a = r.next_audio, left = count;
+        while (left) {  // one span per contiguous stretch of the audio ring
+            const uint64_t pos = a % r.cap;
+            const uint64_t seg = std::min<uint64_t>({left, r.cap - pos, 0x7FFFFFFFull});
+            rvx_vrx_span s{};
+            s.plan = &r.plan;
+            s.channel_base = r.plan.placement.channel * e.cfg.ring_blocks;
+            s.channel_mask = e.cfg.ring_blocks - 1;
+            s.channel_ring = e.ring;
+            s.taps = r.taps;
+            s.weights = r.weights;
+            s.audio = r.audio + pos * ch;
+            s.first_audio = a;
+            s.count = uint32_t(seg);
+            s.tile = r.tile;
+            spans.push_back(s);
+            a += seg;
+            left -= seg;
+        }
+        r.next_audio += count;
+        if (r.next_audio - r.read_audio > r.cap) r.read_audio = r.next_audio - r.cap;
+    }
+    if (spans.empty()) return RVX_OK;
+    uint32_t nd = 0, nw = 0;
+    if (int32_t rc = rvx_vrx_batch_plan(spans.data(), uint32_t(spans.size()), nullptr, 0, &nd,
+                                        nullptr, 0, &nw))
+        return rc;
+    // The staging buffers are rewritten only after the previous upload left them.
+    RVX_TRY_HIP(hipEventSynchronize(e.staged), who);
+    if (nd > e.descs_cap || nw > e.work_cap) {
+        RVX_TRY_HIP(hipStreamSynchronize(e.ctx->stream), who);
+        if (nd > e.descs_cap) {
+            (void)hipHostFree(e.descs_h);
+            (void)hipFree(e.descs_d);
+            e.descs_h = nullptr;
+            e.descs_d = nullptr;
+            e.descs_cap = std::max(nd, 2 * e.descs_cap);
+            RVX_TRY_HIP(hipHostMalloc((void**)&e.descs_h, e.descs_cap * sizeof(rvx_vrx_desc),
+                                      hipHostMallocDefault),
+                        who);
+            if (int32_t rc = device_alloc(&e.descs_d, e.descs_cap * sizeof(rvx_vrx_desc), who))
+                return rc;
+        }
+        if (nw > e.work_cap) {
+            (void)hipHostFree(e.work_h);
+            (void)hipFree(e.work_d);
+            e.work_h = nullptr;
+            e.work_d = nullptr;
+            e.work_cap = std::max(nw, 2 * e.work_cap);
+            RVX_TRY_HIP(hipHostMalloc((void**)&e.work_h, e.work_cap * sizeof(rvx_vrx_work),
+                                      hipHostMallocDefault),
+                        who);
+            if (int32_t rc = device_alloc(&e.work_d, e.work_cap * sizeof(rvx_vrx_work), who))
+                return rc;
+        }
+    }
+    if (int32_t rc = rvx_vrx_batch_plan(spans.data(), uint32_t(spans.size()), e.descs_h,
+                                        e.descs_cap, &nd, e.work_h, e.work_cap, &nw))
+        return rc;
+    RVX_TRY_HIP(hipMemcpyAsync(e.descs_d, e.descs_h, nd * sizeof(rvx_vrx_desc),
+                               hipMemcpyHostToDevice, e.ctx->stream),
+                who);
+    RVX_TRY_HIP(hipMemcpyAsync(e.work_d, e.work_h, nw * sizeof(rvx_vrx_work),
+                               hipMemcpyHostToDevice, e.ctx->stream),
+                who);
+    RVX_TRY_HIP(hipEventRecord(e.staged, e.ctx->stream), who);
+    return rvx_vrx_batch(e.ctx, static_cast<const rvx_vrx_desc*>(e.descs_d), nd,
+                         static_cast<const rvx_vrx_work*>(e.work_d), nw, e.counter, e.nco,
+                         e.nco_entries, 0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const float* prototype,
+                          uint64_t n_prototype, rvx_engine** out) {
+    static const char* who = "rvx_engine_create";
+    RVX_ENTER(ctx, who);
+    if (!config || !prototype || !out) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    *out = nullptr;
+    const rvx_engine_config c = *config;
+    const uint32_t M = c.channels, L = c.taps_per_branch, D = c.decimation;
+    if (c.format != RVX_FORMAT_CU8 && c.format != RVX_FORMAT_CS16)
+        return rvx::fail(RVX_EINVAL, "%s: format %d (the chain reads cu8 or cs16)", who, c.format);
+    if (M < 2 || !rvx::pow2(M) || M > 4096 || L == 0 || D == 0 || M % D != 0)
+        return rvx::fail(RVX_EINVAL, "%s: invalid grid M=%u L=%u D=%u", who, M, L, D);
+    if (n_prototype != uint64_t(M) * L)
+        return rvx::fail(RVX_EINVAL, "%s: wants %llu prototype taps", who,
+                         (unsigned long long)(uint64_t(M) * L));
+    if (c.max_push == 0 || !rvx::pow2(c.ring_blocks))
+        return rvx::fail(RVX_EINVAL, "%s: max_push must be > 0 and ring_blocks a power of two",
+                         who);
+    const uint32_t unit = c.frame_blocks ? c.frame_blocks * std::max(1u, c.row_frames) : 1;
+    if (c.frame_blocks && c.row_frames == 0)
+        return rvx::fail(RVX_EINVAL, "%s: row_frames must be >= 1", who);
+    auto e = new rvx_engine();
+    e->ctx = ctx;
+    e->cfg = c;
+    e->cfg.row_frames = std::max(1u, c.row_frames);
+    e->N = uint64_t(M) * L;
+    e->bps = c.format == RVX_FORMAT_CS16 ? 4 : 2;
+    // Blocks one push can complete: its own, one straddling the previous end,
+    // and up to a spectrum row's worth waiting from before.
+    const uint64_t blocks = uint64_t(c.max_push) / D + 2 + unit;
+    e->max_blocks = uint32_t(blocks + (unit - blocks % unit) % unit);
+    if (e->max_blocks > c.ring_blocks) {
+        delete e;
+        return rvx::fail(RVX_EINVAL, "%s: a push can complete %u blocks; the ring holds %u", who,
+                         blocks > 0xFFFFFFFF ? 0xFFFFFFFFu : uint32_t(blocks), c.ring_blocks);
+    }
+    e->win_cap = e->N - 1 + uint64_t(e->max_blocks + 1) * D + c.max_push;
+    if (e->win_cap >= (uint64_t(1) << 31)) {
+        delete e;
+        return rvx::fail(RVX_EINVAL, "%s: a window of %llu samples exceeds 2^31", who,
+                         (unsigned long long)e->win_cap);
+    }
+    e->win_origin = c.start_index;
+    e->next_block = (c.start_index + e->N - 1 + D - 1) / D;
+    std::vector<float> tw(2 * size_t(M)), nco(2 * size_t(1) << 16);
+    int32_t rc = rvx_build_twiddles(M, tw.data(), tw.size());
+    if (!rc) rc = rvx_build_nco_table(16, nco.data(), nco.size());
+    e->nco_entries = 1u << 16;
+    const uint64_t frames = c.frame_blocks ? e->max_blocks / c.frame_blocks : 0;
+    if (!rc) rc = device_alloc(&e->taps, n_prototype * sizeof(float), who);
+    if (!rc) rc = device_alloc(&e->tw, tw.size() * sizeof(float), who);
+    if (!rc) rc = device_alloc(&e->nco, nco.size() * sizeof(float), who);
+    if (!rc) rc = device_alloc(&e->win, e->win_cap * e->bps, who);
+    if (!rc) rc = device_alloc(&e->bounce, (e->N + uint64_t(e->max_blocks + 1) * D) * e->bps, who);
+    if (!rc) rc = device_alloc(&e->ring, uint64_t(M) * c.ring_blocks * 8, who);
+    if (!rc) rc = device_alloc(&e->counter, sizeof(uint32_t), who);
+    if (!rc && frames) {
+        rc = device_alloc(&e->partial, frames * M * sizeof(float), who);
+        if (!rc) rc = device_alloc(&e->rows, (frames / e->cfg.row_frames) * M * sizeof(float), who);
+        if (!rc) rc = device_alloc(&e->state, uint64_t(M) * sizeof(float), who);
+    }
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    const hipStream_t s = ctx->stream;
+    hipError_t err = hipMemcpyAsync(e->taps, prototype, n_prototype * sizeof(float),
+                                    hipMemcpyHostToDevice, s);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(e->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice, s);
+    if (err == hipSuccess)
+        err = hipMemcpyAsync(e->nco, nco.data(), nco.size() * sizeof(float),
+                             hipMemcpyHostToDevice, s);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&e->staged, hipEventDisableTiming);
+    if (err == hipSuccess) err = hipEventRecord(e->staged, s);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&e->pushed, hipEventDisableTiming);
+    if (err == hipSuccess) err = hipEventRecord(e->pushed, s);
+    if (err == hipSuccess)
+        err = hipHostMalloc((void**)&e->stage_h, uint64_t(c.max_push) * e->bps,
+                            hipHostMallocDefault);
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
+    if (err != hipSuccess) {
+        delete e;
+        return rvx::fail(RVX_EDEVICE, "%s: upload failed: %s", who, hipGetErrorString(err));
+    }
+    if (e->state) {
+        if ((rc = rvx_buf_clear(ctx, e->state, uint64_t(M) * sizeof(float)))) {
+            delete e;
+            return rc;
+        }
+    }
+    *out = e;
+    return RVX_OK;
+}
+
+void rvx_engine_destroy(rvx_engine* engine) { delete engine; }
+
+int32_t rvx_engine_add_vrx(rvx_engine* e, const rvx_vrx_plan* plan, const float* fine_taps,
+                           uint64_t n_fine_floats, const float* weights, uint64_t n_weights,
+                           uint32_t audio_capacity, uint32_t tile, uint32_t* id) {
+    static const char* who = "rvx_engine_add_vrx";
+    if (!e) return rvx::fail(RVX_EINVAL, "%s: null engine", who);
+    RVX_ENTER(e->ctx, who);
+    if (!plan || !fine_taps || !weights || !id)
+        return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    const rvx_vrx_plan& p = *plan;
+    if (p.placement.channel >= e->cfg.channels)
+        return rvx::fail(RVX_EINVAL, "%s: channel %u on a %u-channel grid", who,
+                         p.placement.channel, e->cfg.channels);
+    if (n_fine_floats != 2 * (uint64_t(p.fine.phases) * p.fine.taps + 1))
+        return rvx::fail(RVX_EINVAL, "%s: wants %llu fine-tap floats", who,
+                         (unsigned long long)(2 * (uint64_t(p.fine.phases) * p.fine.taps + 1)));
+    const uint64_t pilot_base = uint64_t(p.demod.audio_taps) + p.demod.dc_taps;
+    const uint64_t want_w = p.demod.pilot_taps == 0
+                                ? pilot_base
+                                : pilot_base + 2ull * p.demod.pilot_taps + 2ull * p.demod.audio_taps;
+    if (n_weights != want_w)
+        return rvx::fail(RVX_EINVAL, "%s: wants %llu weights", who, (unsigned long long)want_w);
+    if (max_tile(p) == 0) return rvx::fail(RVX_EINVAL, "%s: the filters do not fit a tile", who);
+    if (p.fine.nco_log2 > 16)
+        return rvx::fail(RVX_EINVAL, "%s: NCO table of 2^%u entries (the engine's holds 2^16)",
+                         who, p.fine.nco_log2);
+    if (audio_capacity == 0) return rvx::fail(RVX_EINVAL, "%s: zero audio capacity", who);
+    EngineRx r;
+    r.plan = p;
+    r.cap = audio_capacity;
+    r.tile = tile;
+    int32_t rc = device_alloc(&r.taps, n_fine_floats * sizeof(float), who);
+    if (!rc) rc = device_alloc(&r.weights, n_weights * sizeof(float), who);
+    if (!rc) rc = device_alloc(&r.audio, uint64_t(audio_capacity) * p.demod.channels * sizeof(float), who);
+    hipError_t err = hipSuccess;
+    if (!rc) {
+        err = hipMemcpy(r.taps, fine_taps, n_fine_floats * sizeof(float), hipMemcpyHostToDevice);
+        if (err == hipSuccess)
+            err = hipMemcpy(r.weights, weights, n_weights * sizeof(float), hipMemcpyHostToDevice);
+        if (err != hipSuccess) rc = rvx::fail(RVX_EDEVICE, "%s: %s", who, hipGetErrorString(err));
+    }
+    if (rc) {
+        (void)hipFree(r.taps);
+        (void)hipFree(r.weights);
+        (void)hipFree(r.audio);
+        return rc;
+    }
+    r.id = e->next_id++;
+    *id = r.id;
+    e->rx.push_back(r);
+    return RVX_OK;
+}
+
+int32_t rvx_engine_remove_vrx(rvx_engine* e, uint32_t id) {
+    static const char* who = "rvx_engine_remove_vrx";
+    if (!e) return rvx::fail(RVX_EINVAL, "%s: null engine", who);
+    RVX_ENTER(e->ctx, who);
+    auto it = std::find_if(e->rx.begin(), e->rx.end(), [&](const EngineRx& r) { return r.id == id; });
+    if (it == e->rx.end()) return rvx::fail(RVX_EINVAL, "%s: no receiver %u", who, id);
+    RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
+    (void)hipFree(it->taps);
+    (void)hipFree(it->weights);
+    (void)hipFree(it->audio);
+    e->rx.erase(it);
+    return RVX_OK;
+}
+
+int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int32_t on_device) {
+    static const char* who = "rvx_engine_push";
+    if (!e) return rvx::fail(RVX_EINVAL, "%s: null engine", who);
+    RVX_ENTER(e->ctx, who);
+    if (samples == 0) return RVX_OK;
+    if (!codes) return rvx::fail(RVX_EINVAL, "%s: null samples", who);
+    if (samples > e->cfg.max_push)
+        return rvx::fail(RVX_EINVAL, "%s: %llu samples, the engine takes at most %u per push", who,
+                         (unsigned long long)samples, e->cfg.max_push);
+    const hipStream_t s = e->ctx->stream;
+    const uint32_t D = e->cfg.decimation, M = e->cfg.channels;
+    const uint64_t end = e->win_origin + e->win_fill;
+    if (e->win_fill + samples > e->win_cap) {
+        // Keep the history of the next block; move it to the window's front
+        // through the bounce buffer (source and destination may overlap).
+        const uint64_t need_from = e->next_block * D - (e->N - 1);
+        const uint64_t keep_from = std::min(std::max(need_from, e->win_origin), end);
+        const uint64_t kept = end - keep_from;
+        if (kept) {
+            const uint64_t off = (keep_from - e->win_origin) * e->bps;
+            RVX_TRY_HIP(hipMemcpyAsync(e->bounce, e->win + off, kept * e->bps,
+                                       hipMemcpyDeviceToDevice, s),
+                        who);
+            RVX_TRY_HIP(hipMemcpyAsync(e->win, e->bounce, kept * e->bps, hipMemcpyDeviceToDevice, s),
+                        who);
+        }
+        e->win_origin = keep_from;
+        e->win_fill = kept;
+    }
+    if (on_device) {
+        RVX_TRY_HIP(hipMemcpyAsync(e->win + e->win_fill * e->bps, codes, samples * e->bps,
+                                   hipMemcpyDeviceToDevice, s),
+                    who);
+    } else {
+        // Host samples go through pinned staging, so the caller's buffer is
+        // free on return and the upload is asynchronous.
+        RVX_TRY_HIP(hipEventSynchronize(e->pushed), who);
+        std::memcpy(e->stage_h, codes, samples * e->bps);
+        RVX_TRY_HIP(hipMemcpyAsync(e->win + e->win_fill * e->bps, e->stage_h, samples * e->bps,
+                                   hipMemcpyHostToDevice, s),
+                    who);
+        RVX_TRY_HIP(hipEventRecord(e->pushed, s), who);
+    }
+    e->win_fill += samples;
+    e->stats.samples_in += samples;
+
+    const uint64_t stop = e->win_origin + e->win_fill;  // one past the last sample
+    if (stop == 0 || (stop - 1) / D < e->next_block) return RVX_OK;
+    uint64_t count = (stop - 1) / D - e->next_block + 1;
+    const uint32_t unit = e->cfg.frame_blocks ? e->cfg.frame_blocks * e->cfg.row_frames : 1;
+    count -= count % unit;
+    count = std::min<uint64_t>(count, e->max_blocks);
+    if (count == 0) return RVX_OK;
+
+    rvx_chain_desc d{};
+    d.format = e->cfg.format;
+    d.channels = M;
+    d.taps_per_branch = e->cfg.taps_per_branch;
+    d.decimation = D;
+    d.input_origin = e->win_origin;
+    d.input_samples = e->win_fill;
+    d.first_block = e->next_block;
+    d.block_count = uint32_t(count);
+    d.out_ring_blocks = e->cfg.ring_blocks;
+    d.frame_blocks = e->cfg.frame_blocks;
+    d.alpha = e->cfg.alpha;
+    const uint64_t frames = e->cfg.frame_blocks ? count / e->cfg.frame_blocks : 0;
+    if (int32_t rc = rvx_chain(e->ctx, &d, e->win, e->taps, e->N, e->tw, M, e->ring,
+                               uint64_t(M) * e->cfg.ring_blocks, e->partial, frames * M))
+        return rc;
+    if (frames) {
+        rvx_ema_params p{};
+        p.channels = M;
+        p.chan_blocks = e->cfg.ring_blocks;
+        p.chan_mask = e->cfg.ring_blocks - 1;
+        p.in_offset = 0;
+        p.frame_blocks = e->cfg.frame_blocks;
+        p.frames = uint32_t(frames);
+        p.alpha = e->cfg.alpha;
+        p.frame_decay = float(std::pow(1.0 - double(e->cfg.alpha), double(e->cfg.frame_blocks)));
+        p.row_frames = e->cfg.row_frames;
+        p.row_decay = float(std::pow(1.0 - double(e->cfg.alpha),
+                                     double(e->cfg.frame_blocks) * e->cfg.row_frames));
+        const uint64_t nrows = frames / e->cfg.row_frames;
+        if (int32_t rc = rvx_ema_combine(e->ctx, &p, e->partial, frames * M, e->state, e->rows,
+                                         nrows * M))
+            return rc;
+        e->stats.spectrum_rows += nrows;
+    }
+    if (!e->any) {
+        e->any = true;
+        e->first_block = e->next_block;
+    }
+    e->next_block += count;
+    e->stats.blocks += count;
+    e->stats.next_block = e->next_block;
+    return engine_receivers(*e, who);
+}
+
+int32_t rvx_engine_read_audio(rvx_engine* e, uint32_t id, float* out, uint64_t max_values,
+                              uint64_t* first_audio, uint64_t* values) {
+    static const char* who = "rvx_engine_read_audio";
+    if (!e) return rvx::fail(RVX_EINVAL, "%s: null engine", who);
+    RVX_ENTER(e->ctx, who);
+    if (!first_audio || !values || (max_values && !out))
+        return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    auto it = std::find_if(e->rx.begin(), e->rx.end(), [&](const EngineRx& r) { return r.id == id; });
+    if (it == e->rx.end()) return rvx::fail(RVX_EINVAL, "%s: no receiver %u", who, id);
+    EngineRx& r = *it;
+    RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
+    const uint32_t ch = r.plan.demod.channels;
+    const uint64_t n = std::min<uint64_t>(r.next_audio - r.read_audio, max_values / ch);
+    *first_audio = r.read_audio;
+    uint64_t done = 0;
+    while (done < n) {
+        const uint64_t pos = (r.read_audio + done) % r.cap;
+        const uint64_t seg = std::min(n - done, r.cap - pos);
+        RVX_TRY_HIP(hipMemcpy(out + done * ch, r.audio + pos * ch, seg * ch * sizeof(float),
+                              hipMemcpyDeviceToHost),
+                    who);
+        done += seg;
+    }
+    r.read_audio += n;
+    *values = n * ch;
+    return RVX_OK;
+}
+
+int32_t rvx_engine_spectrum(rvx_engine* e, float* out, uint64_t n) {
+    static const char* who = "rvx_engine_spectrum";
+    if (!e) return rvx::fail(RVX_EINVAL, "%s: null engine", who);
+    RVX_ENTER(e->ctx, who);
+    if (!e->state) return rvx::fail(RVX_EINVAL, "%s: the averaged spectrum is off", who);
+    if (!out || n < e->cfg.channels)
+        return rvx::fail(RVX_EINVAL, "%s: wants %u floats", who, e->cfg.channels);
+    RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
+    RVX_TRY_HIP(hipMemcpy(out, e->state, e->cfg.channels * sizeof(float), hipMemcpyDeviceToHost),
+                who);
+    return RVX_OK;
+}
+
+const void* rvx_engine_channel_ring(rvx_engine* e) { return e ? e->ring : nullptr; }
+
+int32_t rvx_engine_stats_get(rvx_engine* e, rvx_engine_stats* out) {
+    if (!e || !out) return rvx::fail(RVX_EINVAL, "rvx_engine_stats_get: null argument");
+    *out = e->stats;
+    out->next_block = e->next_block;
+    return RVX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+"""The streaming engine (rvx_engine_*): pushes of native samples in, channel
+ring, averaged spectrum and per-receiver audio out.
+
+Graph::on_block (core/engine/graph.cpp:4770-5480) and the receivers' stream
+cursors (core/engine/vrx_stage.cpp:1119-1480), run natively in the library;
+this is the Python face of it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .dsp import GridParams, VrxPlan
+from .ops import Device
+
+
+class Engine:
+    def __init__(self, dev: Device, grid: GridParams, prototype: np.ndarray, fmt: int,
+                 max_push: int, ring_blocks: int, frame_blocks: int = 0, row_frames: int = 1,
+                 alpha: float = 1.0 / 16.0, start_index: int = 0):
+        self.dev = dev
+        self.lib = _lib.lib()
+        self.grid = grid
+        self.fmt = fmt
+        cfg = _lib.EngineConfig(fmt, grid.channels, grid.taps_per_branch, grid.decimation,
+                                max_push, ring_blocks, frame_blocks, row_frames,
+                                np.float32(alpha), 0, start_index)
+        taps = np.ascontiguousarray(prototype, dtype=np.float32)
+        h = C.c_void_p()
+        _lib.check(self.lib.rvx_engine_create(dev.ctx, C.byref(cfg), taps.ctypes.data_as(
+            _lib.fptr), taps.size, C.byref(h)))
+        self.handle = h
+        self.channels = {}
+
+    def add(self, plan: VrxPlan, audio_capacity: int, tile: int = 0) -> int:
+        taps = np.ascontiguousarray(plan.fine_taps.view(np.float32))
+        w = np.ascontiguousarray(plan.weights, dtype=np.float32)
+        rid = _lib.u32()
+        _lib.check(self.lib.rvx_engine_add_vrx(self.handle, C.byref(plan.plan),
+                                               taps.ctypes.data_as(_lib.fptr), taps.size,
+                                               w.ctypes.data_as(_lib.fptr), w.size,
+                                               audio_capacity, tile, C.byref(rid)))
+        self.channels[rid.value] = plan.demod.channels
+        return rid.value
+
+    def remove(self, rid: int) -> None:
+        _lib.check(self.lib.rvx_engine_remove_vrx(self.handle, rid))
+        self.channels.pop(rid, None)
+
+    def push(self, codes) -> None:
+        """Native codes: a numpy array (host) or a CUDA tensor (device)."""
+        bps = 4 if self.fmt == 2 else 2
+        if isinstance(codes, np.ndarray):
+            a = np.ascontiguousarray(codes)
+            _lib.check(self.lib.rvx_engine_push(self.handle, a.ctypes.data, a.nbytes // bps, 0))
+        else:
+            if not codes.is_cuda or not codes.is_contiguous():
+                raise ValueError("device pushes take contiguous CUDA tensors")
+            n = codes.numel() * codes.element_size() // bps
+            _lib.check(self.lib.rvx_engine_push(self.handle, codes.data_ptr(), n, 1))
+
+    def read_audio(self, rid: int, max_values: int = 1 << 24) -> tuple[int, np.ndarray]:
+        out = np.empty(max_values, dtype=np.float32)
+        first, n = _lib.u64(), _lib.u64()
+        _lib.check(self.lib.rvx_engine_read_audio(self.handle, rid, out.ctypes.data_as(_lib.fptr),
+                                                  max_values, C.byref(first), C.byref(n)))
+        return first.value, out[:n.value].copy()
+
+    def spectrum(self) -> np.ndarray:
+        out = np.empty(self.grid.channels, dtype=np.float32)
+        _lib.check(self.lib.rvx_engine_spectrum(self.handle, out.ctypes.data_as(_lib.fptr),
+                                                out.size))
+        return out
+
+    def channel_ring(self) -> int:
+        return self.lib.rvx_engine_channel_ring(self.handle)
+
+    def stats(self) -> _lib.EngineStats:
+        s = _lib.EngineStats()
+        _lib.check(self.lib.rvx_engine_stats_get(self.handle, C.byref(s)))
+        return s
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.rvx_engine_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -27,8 +27,8 @@ from . import _lib
 from .dsp import VrxPlan
 from .ops import Device, _p
 
-MAX_SPAN = 2048           # fine samples one tile holds on chip
-_UNBOUNDED = 0xFFFFFFFF   # ring masks of the on-chip fine block: no wrap
+MAX_SPAN = _lib.VRX_TILE_SPAN  # fine samples one tile holds on chip
+_UNBOUNDED = 0xFFFFFFFF         # ring masks of the on-chip fine block: no wrap
 
 
 def max_tile(plan: VrxPlan) -> int:
@@ -37,18 +37,6 @@ def max_tile(plan: VrxPlan) -> int:
     if hist + 1 > MAX_SPAN:
         raise ValueError(f"a receiver whose filters reach {hist} fine samples does not fit a tile")
     return (MAX_SPAN - hist - 1) // r + 1
-
-
-def tile_cost(plan: VrxPlan, n: int) -> int:
-    """Multiply count of one tile of n audio outputs (for ordering the work)."""
-    r, hist = plan.demod.decimation, plan.history
-    span = (n - 1) * r + hist + 1
-    d = plan.demod
-    cost = span * plan.fine.taps * 4
-    if d.mode in (1, 12):
-        cost += span * d.dc_taps
-    cost += n * d.audio_taps * (3 if d.channels == 2 and d.pilot_taps else 1)
-    return cost
 
 
 @dataclass
@@ -67,21 +55,6 @@ class Receiver:
             self.taps = torch.from_numpy(self.plan.fine_taps.view(np.float32).copy()).to(device)
         if self.weights is None:
             self.weights = torch.from_numpy(self.plan.weights.copy()).to(device)
-
-    def max_desc_audio(self) -> int:
-        """Most audio outputs one descriptor carries: the fine block's 32-bit
-        resampler recurrence frac0 + count*step_rem must not overflow
-        (vrx_reference.cpp:429-486); longer blocks split into several."""
-        plan = self.plan
-        fc, fd = plan.plan.channel_rate, plan.plan.demod_rate
-        rem = fc % fd
-        if rem == 0:
-            fine = 0xFFFFFFFF
-        else:
-            fine = (0xFFFFFFFF - (fd - 1)) // rem
-        fine = min(fine, 0xFFFFFFFF)
-        r, hist = plan.demod.decimation, plan.history
-        return max(1, (fine - hist - 1) // r + 1)
 
     def audio_range(self, first_sample: int, last_sample: int) -> tuple[int, int]:
         """(first_audio, count): the audio outputs whose every input lies in
@@ -106,37 +79,14 @@ class Receiver:
         _, n0 = self.plan.fine_block(self.channel_base, self.channel_mask, _UNBOUNDED, f0, 1)
         return n0 - (self.plan.fine.taps - 1)
 
-    def descriptor(self, first_audio: int, count: int, audio: torch.Tensor,
-                   tile: int) -> _lib.VrxDesc:
-        plan = self.plan
-        r, hist = plan.demod.decimation, plan.history
-        if first_audio * r < hist:
-            raise ValueError("the first audio output reaches below fine sample 0")
-        d = _lib.VrxDesc()
-        d.fine_config = plan.fine
-        d.demod_config = plan.demod
-        fine_count = (count - 1) * r + hist + 1
-        d.fine, _ = plan.fine_block(self.channel_base, self.channel_mask, _UNBOUNDED,
-                                    first_audio * r - hist, fine_count)
-        d.demod = plan.demod_block(_UNBOUNDED, first_audio, count)
-        d.hist = hist
-        d.tile = tile
-        d.channel_ring = _p(self.ring)
-        d.taps = _p(self.taps)
-        d.weights = _p(self.weights)
-        d.audio = _p(audio)
-        if audio.numel() < count * plan.demod.channels:
-            raise ValueError("audio tensor smaller than count * channels")
-        _lib.check(_lib.lib().rvx_vrx_desc_check(C.byref(d)))
-        return d
-
 
 class ReceiverBatch:
     """Every receiver of a step in one launch.
 
-    `prepare` writes the descriptors and the largest-first work list to the
-    device (the host's per-step record); `launch` is the one kernel launch
-    (plus the queue reset), safe to capture in a HIP graph.
+    `prepare` records the step (rvx_vrx_batch_plan: descriptors split where
+    the 32-bit resampler recurrence requires, work ordered largest first) and
+    writes it to the device; `launch` is the one kernel launch (plus the queue
+    reset), safe to capture in a HIP graph.
     """
 
     def __init__(self, dev: Device, receivers: list[Receiver], nco: torch.Tensor,
@@ -159,39 +109,28 @@ class ReceiverBatch:
                 tiles: list[int] | None = None) -> None:
         if tiles is not None:
             self.tiles = [min(t, max_tile(r.plan)) for t, r in zip(tiles, self.receivers)]
-        descs: list[_lib.VrxDesc] = []
-        vs, js, costs = [], [], []
-        for r, a0, n, out, t in zip(self.receivers, first_audio, counts, audio, self.tiles):
-            if n == 0:
-                continue
-            t = max(1, min(t, n))
-            step = r.max_desc_audio()
-            step -= step % t  # whole tiles per descriptor
-            step = max(step, t)
-            ch = r.plan.demod.channels
-            for lo in range(0, n, step):
-                m = min(step, n - lo)
-                part = out[lo * ch:(lo + m) * ch]
-                descs.append(r.descriptor(a0 + lo, m, part, t))
-                tiles_here = (m + t - 1) // t
-                j = np.arange(tiles_here, dtype=np.int64)
-                vs.append(np.full(tiles_here, len(descs) - 1, dtype=np.int64))
-                js.append(j)
-                cost = np.full(tiles_here, tile_cost(r.plan, t), dtype=np.int64)
-                cost[-1] = tile_cost(r.plan, m - (tiles_here - 1) * t)
-                costs.append(cost)
-        if not descs:
+        spans = (_lib.VrxSpan * len(self.receivers))()
+        for i, (r, a0, n, out, t) in enumerate(zip(self.receivers, first_audio, counts, audio,
+                                                  self.tiles)):
+            if out.numel() < n * r.plan.demod.channels:
+                raise ValueError("audio tensor smaller than count * channels")
+            spans[i] = _lib.VrxSpan(C.pointer(r.plan.plan), r.channel_base, r.channel_mask,
+                                    _p(r.ring), _p(r.taps), _p(r.weights), _p(out), a0, n, t)
+        lib = _lib.lib()
+        nd, nw = _lib.u32(), _lib.u32()
+        _lib.check(lib.rvx_vrx_batch_plan(spans, len(spans), None, 0, C.byref(nd), None, 0,
+                                          C.byref(nw)))
+        if nd.value == 0:
             raise ValueError("no receiver has audio to compute")
-        v_all, j_all, c_all = np.concatenate(vs), np.concatenate(js), np.concatenate(costs)
-        order = np.lexsort((j_all, v_all, -c_all))  # largest first, then by receiver, tile
-        work = np.stack([v_all[order], j_all[order]], axis=1).astype(np.uint32)
-        items = order
+        descs = (_lib.VrxDesc * nd.value)()
+        work = (_lib.VrxWork * nw.value)()
+        _lib.check(lib.rvx_vrx_batch_plan(spans, len(spans), descs, nd.value, C.byref(nd), work,
+                                          nw.value, C.byref(nw)))
         device = self.nco.device
-        table = (_lib.VrxDesc * len(descs))(*descs)
-        self._descs = torch.frombuffer(bytearray(bytes(table)), dtype=torch.uint8).to(device)
-        self._work = torch.from_numpy(work.ravel().copy()).to(device)
-        self.n_desc = len(descs)
-        self.n_work = len(items)
+        self._descs = torch.frombuffer(bytearray(bytes(descs)), dtype=torch.uint8).to(device)
+        self._work = torch.frombuffer(bytearray(bytes(work)), dtype=torch.uint8).to(device)
+        self.n_desc = nd.value
+        self.n_work = nw.value
 
     def launch(self) -> None:
         if self._descs is None:

@@ -10,7 +10,7 @@ import pytest
 
 from revenant_amd import _lib
 from revenant_amd.dsp import DEMOD_AM, DEMOD_NFM, DEMOD_WFM, GridParams, VrxRequest, plan_vrx
-from revenant_amd.receivers import MAX_SPAN, max_tile, tile_cost
+from revenant_amd.receivers import MAX_SPAN, max_tile
 
 RATE = 122_880_000
 GRID = GridParams(4096, 16, 2048)
@@ -62,8 +62,57 @@ def test_desc_check_accepts_planned_and_refuses_broken(grid, req):
     assert _check(zero_tile) != 0
 
 
-def test_tile_cost_orders_by_work():
+def _span(plan, first_audio, count, tile=0, audio=FAKE):
+    return _lib.VrxSpan(C.pointer(plan.plan), 0, (1 << 16) - 1, FAKE, FAKE, FAKE, audio,
+                        first_audio, count, tile)
+
+
+def _plan_batch(spans):
+    lib = _lib.lib()
+    arr = (_lib.VrxSpan * len(spans))(*spans)
+    nd, nw = _lib.u32(), _lib.u32()
+    assert lib.rvx_vrx_batch_plan(arr, len(spans), None, 0, C.byref(nd), None, 0,
+                                  C.byref(nw)) == 0
+    descs, work = (_lib.VrxDesc * nd.value)(), (_lib.VrxWork * nw.value)()
+    assert lib.rvx_vrx_batch_plan(arr, len(spans), descs, nd.value, C.byref(nd), work, nw.value,
+                                  C.byref(nw)) == 0
+    return list(descs), [(w.vrx, w.tile) for w in work]
+
+
+def test_batch_plan_splits_tiles_and_orders_largest_first():
+    """rvx_vrx_batch_plan: every audio output of every span lands in exactly
+    one (descriptor, tile) item; a span past the 32-bit resampler recurrence
+    is split into descriptors that continue each other; the costliest tiles
+    come first."""
     nfm = plan_vrx(GRID, RATE, VrxRequest(1_000_000, DEMOD_NFM, 12500))
     wfm = plan_vrx(WIDE, RATE, VrxRequest(20_000_000, DEMOD_WFM, 200_000))
-    assert tile_cost(nfm, 256) < tile_cost(nfm, 512)
-    assert tile_cost(wfm, 256) > tile_cost(nfm, 256)
+    spans = [_span(nfm, 5_000_000, 3000, tile=256),
+             _span(wfm, 7_000_000, 20000, tile=100, audio=FAKE * 2)]
+    descs, work = _plan_batch(spans)
+    assert len(descs) > 2  # the WFM span needs several fine blocks
+    covered = {}
+    for v, d in enumerate(descs):
+        assert _lib.lib().rvx_vrx_desc_check(C.byref(d)) == 0
+        a0 = (d.fine_config.taps, d.demod.count)
+        covered.setdefault(d.fine_config.taps, 0)
+        covered[d.fine_config.taps] += d.demod.count
+    assert covered == {nfm.fine.taps: 3000, wfm.fine.taps: 20000}
+    # WFM descriptors continue each other: audio pointers step by count floats
+    w = [d for d in descs if d.fine_config.taps == wfm.fine.taps]
+    for a, b in zip(w, w[1:]):
+        assert b.audio - a.audio == 4 * a.demod.count
+    # every (descriptor, tile) exactly once
+    want = {(v, j) for v, d in enumerate(descs) for j in range(-(-d.demod.count // d.tile))}
+    assert sorted(work) == sorted(want)
+    # WFM tiles (423-tap audio filter at R = 7) cost more than NFM ones: first
+    first_kind = descs[work[0][0]].fine_config.taps
+    last_kind = descs[work[-1][0]].fine_config.taps
+    assert first_kind == wfm.fine.taps and last_kind == nfm.fine.taps
+
+
+def test_batch_plan_refuses_reach_below_zero():
+    am = plan_vrx(GRID, RATE, VrxRequest(-7_345_678, DEMOD_AM, 10000))
+    arr = (_lib.VrxSpan * 1)(_span(am, 3, 100))  # a0 * R < hist
+    nd, nw = _lib.u32(), _lib.u32()
+    assert _lib.lib().rvx_vrx_batch_plan(arr, 1, None, 0, C.byref(nd), None, 0,
+                                         C.byref(nw)) != 0

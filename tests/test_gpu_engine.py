@@ -1,0 +1,180 @@
+"""GPU: the streaming engine (Graph::on_block + the receivers' stream cursors).
+
+Blocking invariance, after tests/reference/test_vrx.cpp:1241 (a receiver's
+output does not depend on how the stream is cut into blocks): the same stream
+pushed in different cuttings gives bit-identical audio, averaged spectrum and
+block counts. And the engine's outputs equal the oracle's composition over the
+whole stream: the chain twin over every block, the averaged-spectrum twin over
+every row, and each receiver's fine + demod twins over the audio the engine
+produced.
+"""
+import numpy as np
+import pytest
+
+from helpers import assert_bit_exact, rng
+from oracle import oracle
+from revenant_amd.dsp import (DEMOD_AM, DEMOD_CW, DEMOD_NFM, DEMOD_USB, FORMAT_CS16, GridParams,
+                              VrxRequest, build_nco_table, build_twiddles, design_prototype,
+                              ema_frame_decay, ema_params, plan_vrx)
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+RATE = 2_400_000
+GRID = GridParams(64, 16, 32)   # 2x oversampled, 75 kHz channels
+SAMPLES = 1 << 20
+FRAME, ROWS = 4, 2
+ALPHA = np.float32(1.0 / 16.0)
+REQUESTS = [VrxRequest(200_000, DEMOD_NFM, 12500), VrxRequest(-333_333, DEMOD_AM, 10000),
+            VrxRequest(510_000, DEMOD_USB, 2700), VrxRequest(-1_000_500, DEMOD_CW, 500,
+                                                                 cw_pitch=700)]
+
+
+@pytest.fixture(scope="module")
+def stream():
+    g = rng(0x454E47494E450001)
+    codes = g.integers(-9000, 9000, size=2 * SAMPLES, dtype=np.int16)
+    # a few strong tones so the receivers carry signal
+    t = np.arange(SAMPLES)
+    for f, a in ((200_000, 6000), (-333_333, 4000), (510_400, 3000), (-1_000_500, 5000)):
+        z = a * np.exp(2j * np.pi * f * t / RATE)
+        codes[0::2] = np.clip(codes[0::2] + z.real, -32768, 32767).astype(np.int16)
+        codes[1::2] = np.clip(codes[1::2] + z.imag, -32768, 32767).astype(np.int16)
+    proto = design_prototype(GRID, beta=8.0).taps
+    plans = [plan_vrx(GRID, RATE, r) for r in REQUESTS]
+    return codes.view(np.uint32), proto, plans
+
+
+def _run(dev, stream, cuts, on_device=False, start=0):
+    from revenant_amd.engine import Engine
+
+    codes, proto, plans = stream
+    eng = Engine(dev, GRID, proto, FORMAT_CS16, max_push=65536, ring_blocks=4096,
+                 frame_blocks=FRAME, row_frames=ROWS, alpha=float(ALPHA), start_index=start)
+    ids = [eng.add(p, audio_capacity=1 << 16) for p in plans]
+    pos = 0
+    src = torch.from_numpy(codes.view(np.int32)).cuda() if on_device else codes
+    for n in cuts:
+        eng.push(src[pos:pos + n])
+        pos += n
+    assert pos == codes.size
+    audio = [eng.read_audio(i) for i in ids]
+    spec = eng.spectrum()
+    st = eng.stats()
+    out = (audio, spec, (st.blocks, st.next_block, st.spectrum_rows, st.audio_lost))
+    eng.close()
+    return out
+
+
+def _cuts(seed, total, lo, hi):
+    g = rng(seed)
+    out, left = [], total
+    while left:
+        n = int(min(left, g.integers(lo, hi + 1)))
+        out.append(n)
+        left -= n
+    return out
+
+
+def test_engine_blocking_invariance(dev, stream):
+    even = _run(dev, stream, [65536] * (SAMPLES // 65536))
+    ragged = _run(dev, stream, _cuts(7, SAMPLES, 1, 9000))
+    tiny_then_big = _run(dev, stream, [1, 31, 32, 33, 1000] + _cuts(8, SAMPLES - 1097, 20000,
+                                                                     65536), on_device=True)
+    for other, name in ((ragged, "ragged"), (tiny_then_big, "tiny then big, device pushes")):
+        assert other[2][:3] == even[2][:3], name
+        assert_bit_exact(other[1], even[1], what=f"spectrum, {name}")
+        for v, ((a0, x), (b0, y)) in enumerate(zip(even[0], other[0])):
+            assert a0 == b0, f"receiver {v} first audio, {name}"
+            assert x.size > 10000
+            # a cut can leave the last few outputs for a later push; compare the common part
+            n = min(x.size, y.size)
+            assert n > x.size - 64
+            assert_bit_exact(y[:n], x[:n], what=f"receiver {v} audio, {name}")
+
+
+@pytest.fixture(scope="module")
+def oracle_run(stream):
+    """The chain twin over every block of the stream (from sample 0) and the
+    averaged-spectrum twin over every row."""
+    codes, proto, plans = stream
+    M, D, N = GRID.channels, GRID.decimation, GRID.prototype_length
+    first = (N - 1 + D - 1) // D
+    blocks = (SAMPLES - 1) // D - first + 1
+    blocks -= blocks % (FRAME * ROWS)
+    cap = 1 << (blocks - 1).bit_length()
+    od = oracle.OChain(FORMAT_CS16, M, GRID.taps_per_branch, D, first, blocks, cap, FRAME,
+                       ALPHA, ema_frame_decay(ALPHA, FRAME))
+    ring, partial = oracle.chain(od, codes.view(np.uint8), proto, build_twiddles(M), threads=4)
+    state = np.zeros(M, dtype=np.float32)
+    oracle.ema_combine(ema_params(M, cap, first, FRAME, blocks // FRAME, ALPHA, ROWS), partial,
+                       state)
+    return first, blocks, cap, ring.reshape(M, cap, 2), state
+
+
+def _oracle_audio(p, ring, cap, a0, n):
+    """Receiver p's audio a0 .. a0+n-1 from the twin ring; channel sample m
+    sits in slot m & (cap-1), also for absolute indices past the stream's
+    start when the start is a multiple of M (the output phase correction
+    depends on m*D mod M only)."""
+    r, hist = p.demod.decimation, p.history
+    fine_count = (n - 1) * r + hist + 1
+    out_cap = 1 << (fine_count - 1).bit_length()
+    chan = np.ascontiguousarray(ring[p.plan.placement.channel]).ravel()
+    fp, _ = p.fine_block(0, cap - 1, out_cap - 1, a0 * r - hist, fine_count)
+    fine = oracle.vrx_fine(p.fine, fp, chan, p.fine_taps, build_nco_table(16), out_cap)
+    dp = p.demod_block(out_cap - 1, a0, n)
+    return oracle.vrx_demod(p.demod, dp, fine, p.weights)
+
+
+def test_engine_matches_oracle(dev, stream, oracle_run):
+    codes, proto, plans = stream
+    first, blocks_all, cap, ring, state = oracle_run
+    (audio, spec, (blocks, next_block, rows, lost)) = _run(dev, stream, _cuts(9, SAMPLES, 5000,
+                                                                              60000))
+    assert lost == 0
+    assert blocks == blocks_all and next_block == first + blocks
+    assert rows == blocks // (FRAME * ROWS)
+    assert_bit_exact(spec, state, what="averaged spectrum after the stream")
+    for v, (p, (a0, got)) in enumerate(zip(plans, audio)):
+        ref = _oracle_audio(p, ring, cap, a0, got.size // p.demod.channels)
+        assert_bit_exact(got, ref, what=f"engine receiver {v} vs oracle")
+        assert np.abs(got).max() > 0
+
+
+def test_engine_late_start_and_receiver_removal(dev, stream, oracle_run):
+    """A stream starting past 2^32 (a multiple of M, so the channel values
+    equal the twin's at shifted block indices) gives the twin's spectrum and
+    audio; a receiver removed midway leaves the others unchanged."""
+    from revenant_amd.engine import Engine
+
+    codes, proto, plans = stream
+    first, blocks_all, cap, ring, state = oracle_run
+    start = 1 << 33
+    eng = Engine(dev, GRID, proto, FORMAT_CS16, max_push=65536, ring_blocks=4096,
+                 frame_blocks=FRAME, row_frames=ROWS, alpha=float(ALPHA), start_index=start)
+    ids = [eng.add(p, audio_capacity=1 << 16) for p in plans]
+    half = SAMPLES // 2
+    for pos in range(0, half, 50000):
+        eng.push(codes[pos:min(pos + 50000, half)])
+    eng.remove(ids[1])
+    for pos in range(half, SAMPLES, 50000):
+        eng.push(codes[pos:min(pos + 50000, SAMPLES)])
+    st = eng.stats()
+    assert st.blocks == blocks_all and st.audio_lost == 0
+    assert_bit_exact(eng.spectrum(), state, what="late-start spectrum")
+    for i, p in ((ids[0], plans[0]), (ids[2], plans[2]), (ids[3], plans[3])):
+        a0, got = eng.read_audio(i)
+        assert a0 * p.demod.decimation * p.plan.channel_rate // p.plan.demod_rate > start // 32
+        ref = _oracle_audio(p, ring, cap, a0, got.size // p.demod.channels)
+        assert_bit_exact(got, ref, what=f"late start receiver {i}")
+    eng.close()
+
+
+def test_engine_refuses_small_ring(dev, stream):
+    from revenant_amd import _lib
+    from revenant_amd.engine import Engine
+
+    codes, proto, plans = stream
+    with pytest.raises(_lib.RvxError):
+        Engine(dev, GRID, proto, FORMAT_CS16, max_push=65536, ring_blocks=1024)
