@@ -526,6 +526,49 @@ int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const void* input,
                   const float* taps, uint64_t n_taps, const void* twiddles, uint64_t n_twiddles,
                   void* channel_ring, uint64_t ring_samples, float* partial, uint64_t n_partial);
 
+/* ---- Plain detector ----------------------------------------------------------
+ * The decision of core/detect/detector.cpp on an averaged linear spectrum:
+ * estimate_noise_floor (:794-876), then find_candidates (:879-1120) through
+ * the accepted, grown bands. Noise floor, prefix sums, the summed-bin ladder
+ * and the peak search run on the GPU; the strongest-first greedy growth runs
+ * on the host over the peaks. DetectorConfig fields (detector.h:579-909). */
+typedef struct rvx_detect_config {
+    uint32_t bins;
+    uint32_t noise_knots;         /* 32 */
+    uint32_t noise_window_knots;  /* 8 */
+    uint32_t max_sum_bins;        /* 0: bit_floor(bins / 8) */
+    uint32_t max_peaks;           /* 0: bins */
+    uint32_t split_gap_bins;      /* 8 */
+    double bin_width_hz;
+    double detection_threshold_db;  /* 6 */
+    double noise_percentile;        /* 0.10 */
+    double noise_percentile_high;   /* 0.40 */
+    double noise_excision_sigma;    /* 4 */
+    double edge_floor_fraction;     /* 0 */
+    double edge_floor_sigma;        /* 3 */
+} rvx_detect_config;
+
+typedef struct rvx_detect_band {
+    uint32_t start;      /* first bin */
+    uint32_t width;      /* bins */
+    double deflection;   /* of the seed window that won */
+} rvx_detect_band;
+
+typedef struct rvx_detector rvx_detector;
+
+int32_t rvx_detector_create(rvx_ctx* ctx, const rvx_detect_config* config, rvx_detector** out);
+void rvx_detector_destroy(rvx_detector* detector);
+/* average: device, bins doubles. allow_prefix (device, bins + 1 int32,
+ * optional): the running count of bins a proposal allows; a ladder window
+ * is searched only where it holds an allowed bin (null: everywhere). Waits for
+ * the result. *n_peaks: peaks the ladder found before the budget. */
+int32_t rvx_detect(rvx_detector* detector, const double* average, const int32_t* allow_prefix,
+                   rvx_detect_band* bands, uint32_t max_bands, uint32_t* n_bands,
+                   uint32_t* n_peaks);
+/* The last decision's per-bin floor and spread (device, bins doubles each). */
+const double* rvx_detector_floor(rvx_detector* detector);
+const double* rvx_detector_sigma(rvx_detector* detector);
+
 /* ---- Streaming engine ------------------------------------------------------
  * The per-block orchestration of Graph::on_block (core/engine/graph.cpp:
  * 4770-5480) and the receivers' stream cursors of DemodStage

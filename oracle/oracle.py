@@ -23,6 +23,30 @@ class OChain(C.Structure):
                 ("frame_decay", f32)]
 
 
+class ODetectConfig(C.Structure):
+    _fields_ = [(n, u32) for n in ("bins", "noise_knots", "noise_window_knots", "max_sum_bins",
+                                   "max_peaks", "split_gap_bins")] + \
+               [(n, C.c_double) for n in ("bin_width_hz", "detection_threshold_db",
+                                          "noise_percentile", "noise_percentile_high",
+                                          "noise_excision_sigma", "edge_floor_fraction",
+                                          "edge_floor_sigma")]
+
+
+class OBand(C.Structure):
+    _fields_ = [("start", u32), ("width", u32), ("deflection", C.c_double)]
+
+
+def detect_config(bins: int, bin_width_hz: float, threshold_db: float = 6.0, **kw) -> ODetectConfig:
+    """DetectorConfig's defaults (core/detect/detector.h:579-909)."""
+    c = ODetectConfig(bins, kw.get("noise_knots", 32), kw.get("noise_window_knots", 8),
+                      kw.get("max_sum_bins", 0), kw.get("max_peaks", 0),
+                      kw.get("split_gap_bins", 8), bin_width_hz, threshold_db,
+                      kw.get("noise_percentile", 0.10), kw.get("noise_percentile_high", 0.40),
+                      kw.get("noise_excision_sigma", 4.0), kw.get("edge_floor_fraction", 0.0),
+                      kw.get("edge_floor_sigma", 3.0))
+    return c
+
+
 _lib = None
 
 
@@ -199,3 +223,17 @@ def vrx_demod(config, params, fine_ring: np.ndarray, weights: np.ndarray) -> np.
     _check(lib().oracle_vrx_demod(C.byref(config), C.byref(params), _a(fr), u64(fr.size // 2),
                                   _a(w), u64(w.size), _a(out), u64(out.size)), "vrx_demod")
     return out
+
+
+def detect(config, average: np.ndarray):
+    """The plain detector's decision on an averaged linear spectrum: returns
+    (bands as (start, width, deflection) tuples, floor, sigma, peaks found)."""
+    avg = np.ascontiguousarray(average, dtype=np.float64)
+    fl = np.zeros(avg.size)
+    sg = np.zeros(avg.size)
+    cap = avg.size
+    bands = (OBand * cap)()
+    nb, npk = u32(), u32()
+    _check(lib().oracle_detect(C.byref(config), _a(avg), _a(fl), _a(sg), bands, u32(cap),
+                               C.byref(nb), C.byref(npk)), "detect")
+    return [(b.start, b.width, b.deflection) for b in bands[:nb.value]], fl, sg, npk.value

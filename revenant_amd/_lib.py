@@ -144,6 +144,18 @@ class EngineStats(C.Structure):
                                    "audio_lost")]
 
 
+class DetectConfig(C.Structure):
+    _fields_ = [(n, u32) for n in ("bins", "noise_knots", "noise_window_knots", "max_sum_bins",
+                                   "max_peaks", "split_gap_bins")] + \
+               [(n, f64) for n in ("bin_width_hz", "detection_threshold_db", "noise_percentile",
+                                   "noise_percentile_high", "noise_excision_sigma",
+                                   "edge_floor_fraction", "edge_floor_sigma")]
+
+
+class DetectBand(C.Structure):
+    _fields_ = [("start", u32), ("width", u32), ("deflection", f64)]
+
+
 VRX_TILE_SPAN = 2048  # RVX_VRX_TILE_SPAN
 
 assert C.sizeof(VrxDesc) == 160 and C.sizeof(VrxWork) == 8 and C.sizeof(VrxSpan) == 64
@@ -211,6 +223,12 @@ _SIGNATURES = {
     "rvx_engine_spectrum": (i32, [vp, fptr, u64]),
     "rvx_engine_channel_ring": (vp, [vp]),
     "rvx_engine_stats_get": (i32, [vp, C.POINTER(EngineStats)]),
+    "rvx_detector_create": (i32, [vp, C.POINTER(DetectConfig), C.POINTER(vp)]),
+    "rvx_detector_destroy": (None, [vp]),
+    "rvx_detect": (i32, [vp, vp, vp, C.POINTER(DetectBand), u32, C.POINTER(u32),
+                         C.POINTER(u32)]),
+    "rvx_detector_floor": (vp, [vp]),
+    "rvx_detector_sigma": (vp, [vp]),
     "rvx_graph_begin": (i32, [vp]),
     "rvx_graph_end": (i32, [vp, C.POINTER(vp)]),
     "rvx_graph_launch": (i32, [vp, vp]),

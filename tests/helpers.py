@@ -50,6 +50,18 @@ def assert_bit_exact(gpu, cpu, seed: int | None = None, what: str = "") -> None:
     NaN has a vendor-chosen sign and payload (x86 gives 0xFFC00000, gfx950
     0x7FC00000), and the reference's harness compares NaNs the same way
     (tests/reference/reference_diff.h:71-75). Signed zeros still differ."""
+    ga, ca = np.asarray(gpu), np.asarray(cpu)
+    if ga.dtype == np.float64 or ca.dtype == np.float64:
+        assert ga.dtype == ca.dtype == np.float64, (ga.dtype, ca.dtype)
+        g64, c64 = ga.view(np.uint64).ravel(), ca.view(np.uint64).ravel()
+        assert g64.size == c64.size, f"{what}: sizes differ {g64.size} vs {c64.size}"
+        nan = np.isnan(ga.ravel()) & np.isnan(ca.ravel())
+        bad = np.nonzero((g64 != c64) & ~nan)[0]
+        if bad.size:
+            i = int(bad[0])
+            raise AssertionError(f"{what}: {bad.size} of {g64.size} doubles differ; first at "
+                                 f"{i}: gpu {ga.ravel()[i]!r} vs twin {ca.ravel()[i]!r}")
+        return
     g, c = bits(gpu), bits(cpu)
     assert g.size == c.size, f"{what}: sizes differ {g.size} vs {c.size}"
     nan_g = (g & 0x7F800000) == 0x7F800000
