@@ -569,6 +569,32 @@ int32_t rvx_detect(rvx_detector* detector, const double* average, const int32_t*
 const double* rvx_detector_floor(rvx_detector* detector);
 const double* rvx_detector_sigma(rvx_detector* detector);
 
+/* ---- Learned proposals ------------------------------------------------------
+ * A five-layer 3x3 convolutional network (1 -> 32 -> 32 -> 32 -> 32 -> 1,
+ * ReLU) over log-power spectrogram tiles, per-pixel logits of "a signal the
+ * plain detector would report is here". bf16 matrix-core convolutions with
+ * fp32 accumulation; trained on the GPU with Adam. No counterpart in the
+ * reference: it proposes, the plain detector (rvx_detect with an allow mask)
+ * confirms. Tiles are device bf16 [batch][height][width], row-major. */
+typedef struct rvx_net rvx_net;
+int32_t rvx_net_create(rvx_ctx* ctx, uint32_t height, uint32_t width, uint32_t max_batch,
+                       uint64_t seed, rvx_net** out);
+void rvx_net_destroy(rvx_net* net);
+uint32_t rvx_net_param_count(const rvx_net* net);
+/* Master fp32 parameters, per layer W[cout][3][3][cin] then b[cout]; write
+ * nonzero copies host -> device. */
+int32_t rvx_net_params(rvx_net* net, float* host, uint64_t count, int32_t write);
+/* The last train step's gradients, same layout (host copy; waits). */
+int32_t rvx_net_grads(rvx_net* net, float* host, uint64_t count);
+/* logits: device fp32 [batch][height][width]. */
+int32_t rvx_net_forward(rvx_net* net, const uint16_t* tiles, uint32_t batch, float* logits);
+/* labels: device fp32 0/1. lr 0 computes gradients without updating.
+ * *loss (optional): the mean pos_weight-weighted binary cross-entropy; waits. */
+int32_t rvx_net_train_step(rvx_net* net, const uint16_t* tiles, const float* labels,
+                           uint32_t batch, float lr, float pos_weight, float* loss);
+/* fp32 -> bf16, round to nearest even (device arrays). */
+int32_t rvx_to_bf16(rvx_ctx* ctx, const float* in, uint64_t count, uint16_t* out);
+
 /* ---- Streaming engine ------------------------------------------------------
  * The per-block orchestration of Graph::on_block (core/engine/graph.cpp:
  * 4770-5480) and the receivers' stream cursors of DemodStage

@@ -229,6 +229,14 @@ _SIGNATURES = {
                          C.POINTER(u32)]),
     "rvx_detector_floor": (vp, [vp]),
     "rvx_detector_sigma": (vp, [vp]),
+    "rvx_net_create": (i32, [vp, u32, u32, u32, u64, C.POINTER(vp)]),
+    "rvx_net_destroy": (None, [vp]),
+    "rvx_net_param_count": (u32, [vp]),
+    "rvx_net_params": (i32, [vp, fptr, u64, i32]),
+    "rvx_net_grads": (i32, [vp, fptr, u64]),
+    "rvx_net_forward": (i32, [vp, vp, u32, vp]),
+    "rvx_net_train_step": (i32, [vp, vp, vp, u32, f32, f32, C.POINTER(f32)]),
+    "rvx_to_bf16": (i32, [vp, vp, u64, vp]),
     "rvx_graph_begin": (i32, [vp]),
     "rvx_graph_end": (i32, [vp, C.POINTER(vp)]),
     "rvx_graph_launch": (i32, [vp, vp]),
