@@ -69,6 +69,8 @@ def parse():
                     help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default tiles, no autotuning")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-detector leg (scenes, training, recall and speed)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 1 GiB device to device three times: the "
                          "known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE")
@@ -288,6 +290,17 @@ def main() -> int:
     if not args.no_cpu:
         cpu = cpu_baseline(packed, main_grid, wide_grid, batch, args.cpu_seconds)
 
+    learned = None
+    if not args.no_learned:
+        log("learned detector: scenes, training on the GPU, held-out evaluation")
+        from revenant_amd import proposals
+        with torch.cuda.stream(side):
+            learned = proposals.run(dev)
+            if not args.no_cpu:
+                learned.update(plain_twin_per_block(dev, proposals))
+                learned["speedup_vs_cpu_twin"] = round(
+                    learned["ms_plain_cpu_twin_per_block"] / learned["ms_learned_per_block"], 1)
+
     workload = ("PFB + 256 VRX: 2^27 cs16 @ 122.88 MS/s (400 carriers + AWGN) -> 4096-ch PFB "
                 "(D=2048, 16 taps/branch, Kaiser beta 8) -> channel ring + |X|^2 EMA "
                 "alpha=1/16; 512-ch PFB (D=256) for the WFM receivers; 256 VRX "
@@ -331,6 +344,7 @@ def main() -> int:
             "algorithmic_bytes": int(algo[dominant]),
         },
         "stages": stages,
+        "learned_detector": learned,
         "cpu_baseline": cpu,
         "parity": parity,
     }
@@ -403,6 +417,23 @@ def spot_check(torch, packed, g: Grid, partial, frames, alpha, batch, starts, co
         n_rx += 1
     return what + (f"; audio bit-exact vs oracle for {n_rx} receiver kinds" if rx_ok else
                    "; audio MISMATCH vs oracle")
+
+
+def plain_twin_per_block(dev, proposals) -> dict:
+    """The plain detector's CPU twin (the oracle, as the reference runs it on
+    the host) deciding every row of one 256-row block: ms per block."""
+    try:
+        from oracle import oracle
+    except Exception as exc:
+        return {"plain_cpu_twin": f"oracle unavailable: {exc}"}
+    scene = proposals.build(dev, [99_999])[0]
+    rows = scene.noisy.double().cpu().numpy()
+    cfg = oracle.detect_config(proposals.M, proposals.BIN_HZ)
+    t0 = time.perf_counter()
+    for r in range(rows.shape[0]):
+        oracle.detect(cfg, rows[r])
+    ms = (time.perf_counter() - t0) * 1e3
+    return {"ms_plain_cpu_twin_per_block": round(ms, 2), "plain_cpu_twin_cores": 1}
 
 
 def traffic_from_profiles(kernel: str):

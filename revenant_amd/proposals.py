@@ -154,21 +154,21 @@ def confirm(det: Detector, s: SceneData, prop: torch.Tensor, dilate: int = 2) ->
     prefix = torch.zeros(M + 1, dtype=torch.int32, device="cuda")
     prefix[1:] = torch.cumsum(allow.int(), 0).int()
     bands, _ = det.decide(gated.double().contiguous(), prefix)
-    ok = torch.zeros(M, dtype=torch.bool, device="cuda")
+    ok = np.zeros(M, dtype=bool)
     for b0, w, _ in bands:
         ok[b0:b0 + w] = True
-    return prop & ok
+    return prop & torch.from_numpy(ok).to(prop.device)
 
 
 def plain_rows(det: Detector, s: SceneData) -> torch.Tensor:
     """One plain decision per noisy row."""
-    out = torch.zeros(ROWS, M, dtype=torch.bool, device="cuda")
+    out = np.zeros((ROWS, M), dtype=bool)
     rows = s.noisy.double()
     for r in range(ROWS):
         bands, _ = det.decide(rows[r].contiguous())
         for b0, w, _ in bands:
             out[r, b0:b0 + w] = True
-    return out
+    return torch.from_numpy(out).cuda()
 
 
 def events(labels: np.ndarray):
@@ -274,14 +274,14 @@ def build(dev: Device, seeds) -> list[SceneData]:
     return out
 
 
-THRESHOLDS = (1.0, 0.0, -1.0, -1.5, -2.0, -2.5)
+THRESHOLDS = (1.0, 0.0, -0.5, -1.0, -1.25, -1.5, -1.75, -2.0, -2.25, -2.5)
 
 
-def run(dev: Device, n_train: int = 64, n_val: int = 4, n_test: int = 6, steps: int = 4000,
+def run(dev: Device, n_train: int = 96, n_val: int = 8, n_test: int = 12, steps: int = 6000,
         seed: int = 1) -> dict:
     """Build scenes, train on the GPU, pick the proposal threshold on the
-    validation scenes (the most precise one whose recall reaches 98% of the
-    plain method's there), and report the held-out scenes."""
+    validation scenes (the most precise one whose recall there reaches the
+    plain method's), and report the held-out scenes."""
     t0 = time.perf_counter()
     train_s = build(dev, range(10_000 * seed, 10_000 * seed + n_train))
     val = build(dev, range(20_000 * seed, 20_000 * seed + n_val))
@@ -300,7 +300,7 @@ def run(dev: Device, n_train: int = 64, n_val: int = 4, n_test: int = 6, steps: 
     for thr in THRESHOLDS:  # most precise first
         masks = [confirm(det, s, propose(net, s, thr)).cpu().numpy() for s in val]
         rl = sum(recall(m, l, n) * n for m, (l, n) in zip(masks, ev)) / nev
-        if rl >= 0.98 * rp:
+        if rl >= rp:
             pick = thr
             break
     det.close()
