@@ -20,9 +20,9 @@
 // i + 2^l (bit l of i clear) with twiddle index
 //     (j + ((i mod 2^l) << S0)) * (N >> (S0 + l + 1)).
 //
-// LDS holds the N values between passes under a bank swizzle that XORs the low
-// LOGV bits of a position with higher bits; it is a bijection inside each
-// aligned group of V, so it moves no value anywhere it is not read back from.
+// LDS holds the N values between passes under a bank swizzle that XORs low
+// position bits with higher ones; it is a bijection on [0, N), so it moves no
+// value anywhere it is not read back from.
 #pragma once
 
 #pragma clang fp contract(off)
@@ -71,9 +71,17 @@ struct Fft {
         return (p + 1) * LOGV <= LOGN ? LOGV : LOGN - p * LOGV;
     }
 
-    // LDS position swizzle (a bijection within each aligned group of V).
+    // LDS position swizzle: XOR of higher position bits into the low bits, a
+    // bijection on [0, N). For radix-8 passes over 2048+ points five bits from
+    // position bit LOGV+4 up spread a wave's 64 accesses over the banks in
+    // every exchange (a bank model of the exchange patterns puts it at about
+    // half the conflicted cycles of the V-bit form, which the smaller shapes
+    // keep).
     __device__ __forceinline__ static uint32_t phys(uint32_t p) {
-        return p ^ ((p >> (LOGV + 2)) & (V - 1));
+        if constexpr (LOGV == 3 && LOGN >= 11)
+            return p ^ ((p >> (LOGV + 4)) & 31u);
+        else
+            return p ^ ((p >> (LOGV + 2)) & (V - 1));
     }
 
     // Position of register r (= u * G + i) of thread t in pass P.
