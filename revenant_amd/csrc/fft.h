@@ -80,6 +80,8 @@ struct Fft {
     __device__ __forceinline__ static uint32_t phys(uint32_t p) {
         if constexpr (LOGV == 3 && LOGN >= 11)
             return p ^ ((p >> (LOGV + 4)) & 31u);
+        else if constexpr (LOGV == 2 && LOGN == 9)  // radix-4 over 512 points
+            return p ^ ((p >> 4) & 15u) ^ (((p >> 5) & 3u) << 3);
         else
             return p ^ ((p >> (LOGV + 2)) & (V - 1));
     }

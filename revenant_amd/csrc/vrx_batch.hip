@@ -25,8 +25,10 @@
 namespace {
 
 constexpr uint32_t kBatchThreads = 256;
-constexpr uint32_t kMaxSpan = 2048;     // fine samples per tile: fine (8 B) + magnitude + detector
-constexpr uint32_t kTapEntries = 2048;  // interpolated taps of every phase class (8 B each)
+constexpr uint32_t kMaxSpan = 2048;     // fine samples per tile: fine (8 B) + detector (4 B)
+constexpr uint32_t kTapEntries = 2040;  // interpolated taps of every phase class (8 B each)
+constexpr uint32_t kPerThread = kMaxSpan / kBatchThreads;
+// 40,900 bytes of LDS: four workgroups fit a CU's 160 KiB.
 static_assert(sizeof(rvx_vrx_desc) == 160, "descriptor layout is part of the ABI");
 
 __host__ __device__ bool complex_tap(uint32_t mode) { return mode == 0 || (mode >= 8 && mode <= 11); }
@@ -45,9 +47,9 @@ vrx_batch_kernel(const rvx_vrx_desc* __restrict__ descs, uint32_t n_desc,
                  const float2* __restrict__ nco, uint32_t nco_log2_max) {
     __shared__ float2 tap_lds[kTapEntries];
     __shared__ float2 fine[kMaxSpan];
-    __shared__ float mag[kMaxSpan];
     __shared__ float det[kMaxSpan];
     __shared__ uint32_t item_sh;
+    float* mag = reinterpret_cast<float*>(fine);  // AM magnitudes replace the fine samples
     const uint32_t tid = threadIdx.x;
 
     for (;;) {
@@ -85,16 +87,20 @@ vrx_batch_kernel(const rvx_vrx_desc* __restrict__ descs, uint32_t n_desc,
         // 1. fine samples a0*R .. a0*R + span - 1 of the receiver's fine block.
         // Fine indices one phase period apart share their interpolated taps,
         // so when the period's taps fit they are formed once per tile.
+        // Rows sit taps + 1 entries apart: lanes of different phase classes
+        // read the same tap index, and an odd row stride keeps them on
+        // different banks.
         const uint32_t period = rvx::phase_period(fp);
-        if (uint64_t(period) * fc.taps <= kTapEntries) {
+        const uint32_t stride = fc.taps + 1;
+        if (uint64_t(period) * stride <= kTapEntries) {
             const uint32_t entries = period * fc.taps;
             for (uint32_t e = tid; span && e < entries; e += kBatchThreads)
                 rvx::fine_taps_row(fc, fp, taps, e / fc.taps, e % fc.taps,
-                                   tap_lds + (e / fc.taps) * fc.taps);
+                                   tap_lds + (e / fc.taps) * stride);
             __syncthreads();
             for (uint32_t s = tid; s < span; s += kBatchThreads) {
                 const uint32_t i = a0 * R + s;
-                fine[s] = rvx::fine_output_row(fc, fp, ch, tap_lds + (i % period) * fc.taps, nco, i);
+                fine[s] = rvx::fine_output_row(fc, fp, ch, tap_lds + (i % period) * stride, nco, i);
             }
         } else {
             for (uint32_t s = tid; s < span; s += kBatchThreads)
@@ -115,14 +121,27 @@ vrx_batch_kernel(const rvx_vrx_desc* __restrict__ descs, uint32_t n_desc,
         // 2. detector: det[s] is detect() at fine index s + dh of the tile
         const uint32_t det_span = span ? span - dh : 0u;
         if (dc.mode == RVX_DEMOD_AM || dc.mode == 12) {
-            for (uint32_t s = tid; s < span; s += kBatchThreads) {
-                const float2 z = fine[s];
-                if (dc.mode == RVX_DEMOD_AM) {
-                    const float sq = z.x * z.x + z.y * z.y;
-                    mag[s] = rvx::det_sqrt(sq);
-                } else {
-                    mag[s] = z.x;
+            // every fine sample is read before any magnitude overwrites it
+            float m[kPerThread];
+#pragma unroll
+            for (uint32_t q = 0; q < kPerThread; ++q) {
+                const uint32_t s = tid + q * kBatchThreads;
+                m[q] = 0.0f;
+                if (s < span) {
+                    const float2 z = fine[s];
+                    if (dc.mode == RVX_DEMOD_AM) {
+                        const float sq = z.x * z.x + z.y * z.y;
+                        m[q] = rvx::det_sqrt(sq);
+                    } else {
+                        m[q] = z.x;
+                    }
                 }
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t q = 0; q < kPerThread; ++q) {
+                const uint32_t s = tid + q * kBatchThreads;
+                if (s < span) mag[s] = m[q];
             }
             __syncthreads();
             const float* dcw = wts + dc.audio_taps;
@@ -241,7 +260,7 @@ int32_t rvx_vrx_batch(rvx_ctx* ctx, const rvx_vrx_desc* descs, uint32_t n_desc,
         int cus = 0;
         RVX_TRY_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
                     who);
-        workgroups = uint32_t(std::max(1, cus)) * 2;
+        workgroups = uint32_t(std::max(1, cus)) * 4;  // four workgroups fit a CU
     }
     workgroups = std::min(workgroups, n_work);
     zero_word<<<dim3(1), dim3(1), 0, ctx->stream>>>(counter);
