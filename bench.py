@@ -253,6 +253,21 @@ def main() -> int:
                 launches[name]()
             kernel_ms[name] = dev.timer_end() / args.steps
 
+        # The matrix-core forms of the same 4096-channel launch (not in the
+        # step: their outputs are not bit-exact), timed the same way.
+        from revenant_amd import _lib as L
+        mfma_ms = {}
+        for label, prec in (("bf16x3", L.PREC_BF16X3), ("fp16", L.PREC_FP16)):
+            def run(prec=prec):
+                dev.chain_mfma(main_grid.desc, prec, packed, main_grid.taps, main_grid.tw,
+                               main_grid.ring, partial)
+            run()
+            dev.timer_begin()
+            for _ in range(args.steps):
+                run()
+            mfma_ms[label] = dev.timer_end() / args.steps
+        launches["chain_m4096"]()  # leave the exact ring behind for the parity check
+
         # Whole-step timing, bracketed by device synchronisation.
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -344,6 +359,11 @@ def main() -> int:
             "algorithmic_bytes": int(algo[dominant]),
         },
         "stages": stages,
+        "mixed_precision": {
+            "chain_m4096_ms": {"fp32_exact": round(kernel_ms["chain_m4096"], 4),
+                               **{k: round(v, 4) for k, v in mfma_ms.items()}},
+            "precision_vs_fp64": precision_from_profiles(),
+        },
         "learned_detector": learned,
         "cpu_baseline": cpu,
         "parity": parity,
@@ -434,6 +454,18 @@ def plain_twin_per_block(dev, proposals) -> dict:
         oracle.detect(cfg, rows[r])
     ms = (time.perf_counter() - t0) * 1e3
     return {"ms_plain_cpu_twin_per_block": round(ms, 2), "plain_cpu_twin_cores": 1}
+
+
+def precision_from_profiles():
+    """Error against FP64 of each form (a 100 dB dynamic-range multitone), as
+    measured by tests/test_gpu_mfma.py and committed under profiles/."""
+    p = ROOT / "profiles" / "mfma_precision.json"
+    if p.exists():
+        try:
+            return json.loads(p.read_text()).get("results")
+        except Exception:
+            return None
+    return None
 
 
 def traffic_from_profiles(kernel: str):

@@ -654,6 +654,23 @@ int32_t rvx_engine_spectrum(rvx_engine* engine, float* out, uint64_t n);
 const void* rvx_engine_channel_ring(rvx_engine* engine);
 int32_t rvx_engine_stats_get(rvx_engine* engine, rvx_engine_stats* out);
 
+/* The same 4096-channel chain with its transform as two 64 x 64 complex
+ * DFT-GEMMs on the matrix cores (configuration 4 of the precision study;
+ * no counterpart in the reference). The branch filter stays fp32; the DFT
+ * operands are split bf16 (hi + lo, three products) or fp16, fp32 accumulation.
+ * Same descriptor, buffers and outputs as rvx_chain (cs16, M = 4096, no
+ * channel_rows); not bit-exact: its error is measured against FP64. */
+#define RVX_PREC_BF16X3 1
+#define RVX_PREC_FP16 2
+int32_t rvx_chain_mfma(rvx_ctx* ctx, const rvx_chain_desc* desc, int32_t precision,
+                       const void* input, const float* taps, uint64_t n_taps,
+                       const void* twiddles, uint64_t n_twiddles, void* channel_ring,
+                       uint64_t ring_samples, float* partial, uint64_t n_partial);
+
+/* Display spectrum in FP8 e4m3: out[i] = e4m3(10 log10(max(power[i], 1e-20)) / 4),
+ * round to nearest even, saturating (dB / 4 spans -50 .. 112 in e4m3's +-448). */
+int32_t rvx_spectrum_fp8(rvx_ctx* ctx, const float* power, uint64_t n, uint8_t* out);
+
 #ifdef __cplusplus
 }
 #endif

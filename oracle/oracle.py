@@ -237,3 +237,40 @@ def detect(config, average: np.ndarray):
     _check(lib().oracle_detect(C.byref(config), _a(avg), _a(fl), _a(sg), bands, u32(cap),
                                C.byref(nb), C.byref(npk)), "detect")
     return [(b.start, b.width, b.deflection) for b in bands[:nb.value]], fl, sg, npk.value
+
+
+def fp64_chain(codes: np.ndarray, taps: np.ndarray, M: int, D: int, first_block: int,
+               blocks: int) -> np.ndarray:
+    """The channelizer in float64 by branch sums, numpy's FFT and the output
+    correction: complex128 [blocks, M], channel k of block m equal to
+    sum_n h[n] x[mD - n] e^{-j 2 pi k (mD - n) / M} (pfb_branch.comp:9-19).
+    codes: cs16 words (uint32) from sample 0."""
+    c = np.ascontiguousarray(codes).view(np.int16).astype(np.float64) * 2.0 ** -15
+    x = c[0::2] + 1j * c[1::2]
+    L = taps.size // M
+    h = taps.astype(np.float64).reshape(L, M)  # h[q][r] = taps[q*M + r]
+    r = np.arange(M)
+    slot = (M - r) % M
+    k = np.arange(M)
+    out = np.empty((blocks, M), dtype=np.complex128)
+    for b in range(blocks):
+        m = first_block + b
+        idx = m * D - np.arange(L)[:, None] * M - r[None, :]
+        u = (h * x[idx]).sum(axis=0)
+        y = np.empty(M, dtype=np.complex128)
+        y[slot] = u
+        X = np.fft.fft(y)
+        dm = (m * D) % M
+        out[b] = X * np.exp(-2j * np.pi * ((k * dm) % M) / M)
+    return out
+
+
+def e4m3_decode(b: np.ndarray) -> np.ndarray:
+    """OCP e4m3 bytes -> float64 (0x7F / 0xFF: NaN)."""
+    b = np.asarray(b, dtype=np.uint8).astype(np.int64)
+    s = np.where(b & 0x80, -1.0, 1.0)
+    e = (b >> 3) & 15
+    m = b & 7
+    v = np.where(e == 0, m / 8.0 * 2.0 ** -6, (1 + m / 8.0) * 2.0 ** (e - 7.0))
+    v = np.where((b & 0x7F) == 0x7F, np.nan, v)
+    return s * v

@@ -157,6 +157,7 @@ class DetectBand(C.Structure):
 
 
 VRX_TILE_SPAN = 2048  # RVX_VRX_TILE_SPAN
+PREC_BF16X3, PREC_FP16 = 1, 2  # RVX_PREC_*
 
 assert C.sizeof(VrxDesc) == 160 and C.sizeof(VrxWork) == 8 and C.sizeof(VrxSpan) == 64
 assert C.sizeof(VrxFineParams) == 60 and C.sizeof(VrxDemodParams) == 16
@@ -237,6 +238,9 @@ _SIGNATURES = {
     "rvx_net_forward": (i32, [vp, vp, u32, vp]),
     "rvx_net_train_step": (i32, [vp, vp, vp, u32, f32, f32, C.POINTER(f32)]),
     "rvx_to_bf16": (i32, [vp, vp, u64, vp]),
+    "rvx_chain_mfma": (i32, [vp, C.POINTER(ChainDesc), i32, vp, vp, u64, vp, u64, vp, u64, vp,
+                             u64]),
+    "rvx_spectrum_fp8": (i32, [vp, vp, u64, vp]),
     "rvx_graph_begin": (i32, [vp]),
     "rvx_graph_end": (i32, [vp, C.POINTER(vp)]),
     "rvx_graph_launch": (i32, [vp, vp]),

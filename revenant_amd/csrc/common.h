@@ -16,6 +16,7 @@ struct rvx_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t t0 = nullptr;
     hipEvent_t t1 = nullptr;
+    void* dft[2] = {nullptr, nullptr};  // matrix-core DFT operand tables, made on first use
 };
 
 namespace rvx {

@@ -95,6 +95,8 @@ void rvx_ctx_destroy(rvx_ctx* ctx) {
     }
     if (ctx->t0) (void)hipEventDestroy(ctx->t0);
     if (ctx->t1) (void)hipEventDestroy(ctx->t1);
+    for (void* p : ctx->dft)
+        if (p) (void)hipFree(p);
     delete ctx;
 }
 

@@ -156,6 +156,23 @@ class Device:
                                       _csamples(ring), _p(partial),
                                       0 if partial is None else _n(partial)))
 
+    def chain_mfma(self, desc: _lib.ChainDesc, precision: int, packed: torch.Tensor,
+                   taps: torch.Tensor, twiddles: torch.Tensor, ring: torch.Tensor,
+                   partial: torch.Tensor | None = None) -> None:
+        """The chain with its transform on the matrix cores (not bit-exact)."""
+        if packed.numel() * packed.element_size() < desc.input_samples * 4:
+            raise ValueError("input tensor smaller than input_samples")
+        _lib.check(self.lib.rvx_chain_mfma(self.ctx, C.byref(desc), precision, _p(packed),
+                                           _p(taps), _n(taps), _p(twiddles),
+                                           _csamples(twiddles), _p(ring), _csamples(ring),
+                                           _p(partial), 0 if partial is None else _n(partial)))
+
+    def spectrum_fp8(self, power: torch.Tensor, out: torch.Tensor) -> None:
+        """e4m3 of 10 log10(power) / 4 (display spectrum / waterfall rows)."""
+        if out.numel() < power.numel() or out.element_size() != 1:
+            raise ValueError("out must hold one byte per power value")
+        _lib.check(self.lib.rvx_spectrum_fp8(self.ctx, _p(power), power.numel(), _p(out)))
+
     # -- graphs ---------------------------------------------------------
     def graph_begin(self) -> None:
         """Start capturing this context's stream (not the null stream)."""
