@@ -119,7 +119,7 @@ __device__ __forceinline__ void gemm(const uint16_t* ahi, const uint16_t* alo, c
         const int m = (wave + 4 * h) * 16 + row;  // A row of this lane
         const uint16_t* ah = ahi + m * 128;
         const uint16_t* al = alo + m * 128;
-#pragma unroll
+#pragma unroll 2
         for (int ks = 0; ks < 8; ++ks) {
 #pragma unroll
             for (int tn = 0; tn < 4; ++tn) {
@@ -133,7 +133,7 @@ __device__ __forceinline__ void gemm(const uint16_t* ahi, const uint16_t* alo, c
     }
 }
 
-template <int PREC>
+template <int PREC, int TAPS>
 __global__ void __launch_bounds__(kThreads) chain_mfma_kernel(MfmaArgs a) {
     constexpr int NA = PREC == 0 ? 2 : 1;
     __shared__ __align__(16) uint16_t amat[NA][128 * 128];
@@ -165,25 +165,60 @@ __global__ void __launch_bounds__(kThreads) chain_mfma_kernel(MfmaArgs a) {
         const uint64_t m = a.first_block + first + b0;
         const uint32_t base = uint32_t(m * D - a.input_origin);
         __syncthreads();  // the previous block's B reads are done
-        // FIR: branches r = tid + 256 j
-        for (int j = 0; j < 16; ++j) {
-            const uint32_t r = uint32_t(tid) + 256u * j;
-            float ur = 0.f, ui = 0.f;
-            for (uint32_t q = 0; q < L; ++q) {
-                const uint32_t idx = base - q * M - r;
-                const uint32_t code = static_cast<const uint32_t*>(a.input)[idx];
-                const float xr = float(int32_t(code << 16) >> 16) * __uint_as_float(0x38000000u);
-                const float xi = float(int32_t(code) >> 16) * __uint_as_float(0x38000000u);
-                const float h = a.taps[q * M + r];
-                const float pr = xr * h, pi = xi * h;
-                ur = ur + pr;
-                ui = ui + pi;
-            }
+        // FIR: branches r = tid + 256 j. With a compile-time tap count every
+        // load of two branches is issued before their sums start.
+        const uint32_t* in = static_cast<const uint32_t*>(a.input);
+        auto emit = [&](uint32_t r, float ur, float ui) {
             const uint32_t s = (M - r) & (M - 1);
             const int n1 = int(s >> 6), n2 = int(s & 63);
             // B1[k = n1 | 64 + n1][n = n2], stored transposed
             put<PREC>(bhi, blo, n2 * kKS + n1, ur, in_scale);
             put<PREC>(bhi, blo, n2 * kKS + 64 + n1, ui, in_scale);
+        };
+        if constexpr (TAPS > 0) {
+            for (int j = 0; j < 16; j += 2) {
+                uint32_t code[2][TAPS];
+                float h[2][TAPS];
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj) {
+                    const uint32_t r = uint32_t(tid) + 256u * (j + jj);
+#pragma unroll
+                    for (int q = 0; q < TAPS; ++q) {
+                        code[jj][q] = in[base - uint32_t(q) * M - r];
+                        h[jj][q] = a.taps[uint32_t(q) * M + r];
+                    }
+                }
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj) {
+                    float ur = 0.f, ui = 0.f;
+#pragma unroll
+                    for (int q = 0; q < TAPS; ++q) {
+                        const float xr = float(int32_t(code[jj][q] << 16) >> 16);
+                        const float xi = float(int32_t(code[jj][q]) >> 16);
+                        const float pr = xr * h[jj][q], pi = xi * h[jj][q];
+                        ur = ur + pr;
+                        ui = ui + pi;
+                    }
+                    // 2^-15 once per branch (exact: a power of two)
+                    emit(uint32_t(tid) + 256u * (j + jj), ur * __uint_as_float(0x38000000u),
+                         ui * __uint_as_float(0x38000000u));
+                }
+            }
+        } else {
+            for (int j = 0; j < 16; ++j) {
+                const uint32_t r = uint32_t(tid) + 256u * j;
+                float ur = 0.f, ui = 0.f;
+                for (uint32_t q = 0; q < L; ++q) {
+                    const uint32_t code = in[base - q * M - r];
+                    const float xr = float(int32_t(code << 16) >> 16);
+                    const float xi = float(int32_t(code) >> 16);
+                    const float h = a.taps[q * M + r];
+                    const float pr = xr * h, pi = xi * h;
+                    ur = ur + pr;
+                    ui = ui + pi;
+                }
+                emit(r, ur * __uint_as_float(0x38000000u), ui * __uint_as_float(0x38000000u));
+            }
         }
         __syncthreads();
         float4_t acc[2][4];
@@ -344,10 +379,15 @@ extern "C" int32_t rvx_chain_mfma(rvx_ctx* ctx, const rvx_chain_desc* desc, int3
     a.average = d.frame_blocks != 0;
     a.alpha = d.alpha;
     const dim3 grid(8 * ((a.chunks + 7) / 8)), block(kThreads);
-    if (prec == 0)
-        chain_mfma_kernel<0><<<grid, block, 0, ctx->stream>>>(a);
+    const bool t16 = d.taps_per_branch == 16;
+    if (prec == 0 && t16)
+        chain_mfma_kernel<0, 16><<<grid, block, 0, ctx->stream>>>(a);
+    else if (prec == 0)
+        chain_mfma_kernel<0, 0><<<grid, block, 0, ctx->stream>>>(a);
+    else if (t16)
+        chain_mfma_kernel<1, 16><<<grid, block, 0, ctx->stream>>>(a);
     else
-        chain_mfma_kernel<1><<<grid, block, 0, ctx->stream>>>(a);
+        chain_mfma_kernel<1, 0><<<grid, block, 0, ctx->stream>>>(a);
     return rvx::check_launch(who);
 }
 
