@@ -268,6 +268,23 @@ def main() -> int:
             mfma_ms[label] = dev.timer_end() / args.steps
         launches["chain_m4096"]()  # leave the exact ring behind for the parity check
 
+        # The receivers recorded the reference's way (fine + demod launches per
+        # receiver segment through a fine ring), captured as one graph so the
+        # host's launch rate does not count; same audio bits.
+        unbatched_ms = None
+        if vrx:
+            fine_ring = torch.empty(2 << 17, dtype=torch.float32, device="cuda")
+            dev.graph_begin()
+            batch.unbatched(fine_ring)
+            ug = dev.graph_end()
+            ug.launch()
+            dev.timer_begin()
+            for _ in range(args.steps):
+                ug.launch()
+            unbatched_ms = dev.timer_end() / args.steps
+            ug.close()
+            batch.launch()  # the batch's own audio for the parity check
+
         # Whole-step timing, bracketed by device synchronisation.
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -359,6 +376,12 @@ def main() -> int:
             "algorithmic_bytes": int(algo[dominant]),
         },
         "stages": stages,
+        "batched_receivers": None if not vrx else {
+            "descriptors": batch.n_desc, "work_items": batch.n_work,
+            "batch_ms": round(kernel_ms["vrx_batch"], 4),
+            "per_receiver_launches_ms": round(unbatched_ms, 4),
+            "launches_replaced": 2 * batch.n_desc,
+            "speedup": round(unbatched_ms / kernel_ms["vrx_batch"], 2)},
         "mixed_precision": {
             "chain_m4096_ms": {"fp32_exact": round(kernel_ms["chain_m4096"], 4),
                                **{k: round(v, 4) for k, v in mfma_ms.items()}},

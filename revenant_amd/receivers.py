@@ -25,7 +25,7 @@ import torch
 
 from . import _lib
 from .dsp import VrxPlan
-from .ops import Device, _p
+from .ops import Device, _csamples, _p
 
 MAX_SPAN = _lib.VRX_TILE_SPAN  # fine samples one tile holds on chip
 _UNBOUNDED = 0xFFFFFFFF         # ring masks of the on-chip fine block: no wrap
@@ -127,6 +127,7 @@ class ReceiverBatch:
         _lib.check(lib.rvx_vrx_batch_plan(spans, len(spans), descs, nd.value, C.byref(nd), work,
                                           nw.value, C.byref(nw)))
         device = self.nco.device
+        self.host_descs = descs
         self._descs = torch.frombuffer(bytearray(bytes(descs)), dtype=torch.uint8).to(device)
         self._work = torch.frombuffer(bytearray(bytes(work)), dtype=torch.uint8).to(device)
         self.n_desc = nd.value
@@ -138,6 +139,32 @@ class ReceiverBatch:
         _lib.check(self.dev.lib.rvx_vrx_batch(
             self.dev.ctx, _p(self._descs), self.n_desc, _p(self._work), self.n_work,
             _p(self.counter), _p(self.nco), self.nco.numel() // 2, self.workgroups))
+
+    def unbatched(self, fine_ring: torch.Tensor) -> None:
+        """The same work as launch(), recorded the way the reference records it
+        (core/engine/vrx_stage.cpp:1306-1399): per receiver segment, rvx_vrx_fine
+        into a fine ring in HBM, then rvx_vrx_demod from it. Same audio bits;
+        two launches per segment instead of one for everything."""
+        lib = self.dev.lib
+        cap = _csamples(fine_ring)
+        if cap & (cap - 1):
+            raise ValueError("the fine ring holds a power of two of samples")
+        owner = {r.taps.data_ptr(): r for r in self.receivers}
+        for d in self.host_descs:
+            r = owner[d.taps]
+            fp = _lib.VrxFineParams.from_buffer_copy(d.fine)
+            if fp.count > cap:
+                raise ValueError(f"a segment of {fp.count} fine samples exceeds the ring")
+            fp.out_offset, fp.out_mask = 0, cap - 1
+            dp = _lib.VrxDemodParams.from_buffer_copy(d.demod)
+            dp.in_offset, dp.in_mask = d.hist, cap - 1
+            _lib.check(lib.rvx_vrx_fine(self.dev.ctx, C.byref(d.fine_config), C.byref(fp),
+                                        d.channel_ring, _csamples(r.ring), d.taps,
+                                        _csamples(r.taps), _p(self.nco), self.nco.numel() // 2,
+                                        _p(fine_ring), cap))
+            _lib.check(lib.rvx_vrx_demod(self.dev.ctx, C.byref(d.demod_config), C.byref(dp),
+                                         _p(fine_ring), cap, d.weights, r.weights.numel(),
+                                         d.audio, d.demod.count * d.demod_config.channels))
 
     def tune(self, first_audio: list[int], counts: list[int], audio: list[torch.Tensor],
              candidates=(32, 64, 128, 256, 512, 1024, 2048), reps: int = 3) -> list[int]:
