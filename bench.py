@@ -87,11 +87,14 @@ class Grid:
     def __init__(self, torch, channels, decimation, frame, rows=None):
         from revenant_amd import _lib
         from revenant_amd.dsp import (FORMAT_CS16, GridParams, build_twiddles,
-                                      design_prototype)
+                                      design_prototype, prescale_taps)
 
         self.params = GridParams(channels, TAPS, decimation)
         self.proto = design_prototype(self.params, beta=8.0)
         self.taps = torch.from_numpy(self.proto.taps).cuda()
+        # the chain reads cs16 against taps carrying the conversion's 2^-15
+        # (bit-identical products; rvx_prescale_taps checks exactness)
+        self.taps_scaled = torch.from_numpy(prescale_taps(self.proto.taps)).cuda()
         self.tw_np = build_twiddles(channels)
         self.tw = torch.from_numpy(self.tw_np.view(np.float32)).cuda()
         N = self.params.prototype_length
@@ -109,6 +112,7 @@ class Grid:
         self.desc = _lib.ChainDesc(FORMAT_CS16, channels, TAPS, decimation, 0, SAMPLES,
                                    self.first_block, blocks, self.ring_blocks, frame,
                                    np.float32(ALPHA))
+        self.desc.taps_scaled = 1
         self.row_table = None
         if rows is not None:
             table = np.full(channels, 0xFFFFFFFF, dtype=np.uint32)
@@ -184,8 +188,9 @@ def main() -> int:
                          ROW_FRAMES)
 
         launches = {}
-        launches["chain_m4096"] = lambda: dev.chain(main_grid.desc, packed, main_grid.taps,
-                                                    main_grid.tw, main_grid.ring, partial)
+        launches["chain_m4096"] = lambda: dev.chain(main_grid.desc, packed,
+                                                    main_grid.taps_scaled, main_grid.tw,
+                                                    main_grid.ring, partial)
         launches["ema_combine"] = lambda: dev.ema_combine(ema, partial, state, rows_out)
 
         batch = wide_grid = None
@@ -211,8 +216,9 @@ def main() -> int:
             audio = [audio_buf[offs[i]:offs[i + 1]] for i in range(len(receivers))]
             audio_total = int(sum(counts))
             batch = ReceiverBatch(dev, receivers, nco)
-            launches["chain_m512_wfm"] = lambda: dev.chain(wide_grid.desc, packed, wide_grid.taps,
-                                                           wide_grid.tw, wide_grid.ring)
+            launches["chain_m512_wfm"] = lambda: dev.chain(wide_grid.desc, packed,
+                                                           wide_grid.taps_scaled, wide_grid.tw,
+                                                           wide_grid.ring)
             # the receivers read channel rings that the chains fill
             launches["chain_m512_wfm"]()
             launches["chain_m4096"]()
@@ -258,8 +264,11 @@ def main() -> int:
         from revenant_amd import _lib as L
         mfma_ms = {}
         for label, prec in (("bf16x3", L.PREC_BF16X3), ("fp16", L.PREC_FP16)):
-            def run(prec=prec):
-                dev.chain_mfma(main_grid.desc, prec, packed, main_grid.taps, main_grid.tw,
+            mdesc = _lib_copy(main_grid.desc)
+            mdesc.taps_scaled = 0
+
+            def run(prec=prec, mdesc=mdesc):
+                dev.chain_mfma(mdesc, prec, packed, main_grid.taps, main_grid.tw,
                                main_grid.ring, partial)
             run()
             dev.timer_begin()
@@ -477,6 +486,10 @@ def plain_twin_per_block(dev, proposals) -> dict:
         oracle.detect(cfg, rows[r])
     ms = (time.perf_counter() - t0) * 1e3
     return {"ms_plain_cpu_twin_per_block": round(ms, 2), "plain_cpu_twin_cores": 1}
+
+
+def _lib_copy(desc):
+    return type(desc).from_buffer_copy(desc)
 
 
 def precision_from_profiles():

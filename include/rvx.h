@@ -518,9 +518,15 @@ typedef struct rvx_chain_desc {
     uint32_t frame_blocks;
     float alpha;
     uint32_t ring_rows;
-    uint32_t reserved;
+    uint32_t taps_scaled;     /* 1: cs16 input and taps from rvx_prescale_taps */
     const uint32_t* channel_rows;
 } rvx_chain_desc;
+
+/* taps * 2^-15 exactly (host arrays), for a cs16 chain with taps_scaled = 1:
+ * the conversion's scale moves into the taps and the products are unchanged
+ * bit for bit. Refuses a tap whose scaled value would not be a normal float
+ * (0 < |h| < 2^-111) or is not finite. */
+int32_t rvx_prescale_taps(const float* taps, uint64_t n, float* out);
 
 int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const void* input,
                   const float* taps, uint64_t n_taps, const void* twiddles, uint64_t n_twiddles,

@@ -89,7 +89,7 @@ class ChainDesc(C.Structure):
     _fields_ = [("format", i32), ("channels", u32), ("taps_per_branch", u32),
                 ("decimation", u32), ("input_origin", u64), ("input_samples", u64),
                 ("first_block", u64), ("block_count", u32), ("out_ring_blocks", u32),
-                ("frame_blocks", u32), ("alpha", f32), ("ring_rows", u32), ("reserved", u32),
+                ("frame_blocks", u32), ("alpha", f32), ("ring_rows", u32), ("taps_scaled", u32),
                 ("channel_rows", vp)]
 
 
@@ -241,6 +241,7 @@ _SIGNATURES = {
     "rvx_chain_mfma": (i32, [vp, C.POINTER(ChainDesc), i32, vp, vp, u64, vp, u64, vp, u64, vp,
                              u64]),
     "rvx_spectrum_fp8": (i32, [vp, vp, u64, vp]),
+    "rvx_prescale_taps": (i32, [fptr, u64, fptr]),
     "rvx_graph_begin": (i32, [vp]),
     "rvx_graph_end": (i32, [vp, C.POINTER(vp)]),
     "rvx_graph_launch": (i32, [vp, vp]),

@@ -70,10 +70,19 @@ struct ChainShape {
     static constexpr int BPW = THREADS / F::T;  // blocks in flight per workgroup
 };
 
+// cs16 codes read against taps that already carry the 2^-15 of the
+// conversion (rvx_prescale_taps): a product code * (h * 2^-15) is the same
+// real number as (code * 2^-15) * h, so it rounds to the same float, and the
+// conversion's multiply leaves the tap loop.
+constexpr int kCs16Scaled = 0x102;
+__host__ __device__ constexpr bool is_cs16(int f) {
+    return f == RVX_FORMAT_CS16 || f == kCs16Scaled;
+}
+
 // The native code of one sample (cs16: one word; cu8: two bytes).
 template <int FORMAT>
 __device__ __forceinline__ uint32_t load_code(const void* input, uint32_t index) {
-    if constexpr (FORMAT == RVX_FORMAT_CS16)
+    if constexpr (is_cs16(FORMAT))
         return static_cast<const uint32_t*>(input)[index];
     else
         return static_cast<const uint16_t*>(input)[index];
@@ -82,7 +91,9 @@ __device__ __forceinline__ uint32_t load_code(const void* input, uint32_t index)
 // convert_cs16_cf32.comp / convert_cu8_cf32.comp on one code.
 template <int FORMAT>
 __device__ __forceinline__ float2 convert_code(uint32_t w) {
-    if constexpr (FORMAT == RVX_FORMAT_CS16) {
+    if constexpr (FORMAT == kCs16Scaled) {
+        return make_float2(float(int32_t(w << 16) >> 16), float(int32_t(w) >> 16));
+    } else if constexpr (FORMAT == RVX_FORMAT_CS16) {
         const float scale = __uint_as_float(0x38000000u);
         const float i = float(int32_t(w << 16) >> 16) * scale;
         const float q = float(int32_t(w) >> 16) * scale;
@@ -111,7 +122,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, ui
 template <int FORMAT>
 __device__ __forceinline__ uint32_t buffer_code(__amdgpu_buffer_rsrc_t r, uint32_t voff,
                                                 uint32_t soff) {
-    if constexpr (FORMAT == RVX_FORMAT_CS16)
+    if constexpr (is_cs16(FORMAT))
         return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
     else
         return __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
@@ -210,7 +221,7 @@ chain_kernel(ChainArgs a) {
     // reads; a sample's byte offset from there splits into a per-lane part
     // (block base minus branch, minus the oldest tap) and a per-tap uniform
     // part ((TAPS-1-q) rows of M samples).
-    constexpr uint32_t BPS = FORMAT == RVX_FORMAT_CS16 ? 4u : 2u;
+    constexpr uint32_t BPS = is_cs16(FORMAT) ? 4u : 2u;
     constexpr uint32_t NTAPS = TAPS > 0 ? TAPS : 1;
     const uint32_t window0 =
         uint32_t((a.first_block + first) * a.decimation - a.input_origin) - (NTAPS * M - 1);
@@ -444,6 +455,8 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
     if (desc == nullptr) return rvx::fail(RVX_EINVAL, "%s: null descriptor", who);
     const rvx_chain_desc d = *desc;
     const uint32_t M = d.channels, T = d.taps_per_branch, D = d.decimation;
+    if (d.taps_scaled > 1 || (d.taps_scaled && d.format != RVX_FORMAT_CS16))
+        return rvx::fail(RVX_EINVAL, "%s: prescaled taps belong to cs16 input", who);
     if (d.format != RVX_FORMAT_CU8 && d.format != RVX_FORMAT_CS16)
         return rvx::fail(RVX_EINVAL, "%s: format %d (the fused chain reads cu8 or cs16)", who,
                          d.format);
@@ -515,9 +528,10 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
     a.rows = d.channel_rows;
     a.ring_rows = rows;
     const uint32_t logm = rvx::log2u(M);
-    const int32_t rc = d.format == RVX_FORMAT_CS16
-                           ? launch_chain<RVX_FORMAT_CS16>(logm, a, T, ctx->stream)
-                           : launch_chain<RVX_FORMAT_CU8>(logm, a, T, ctx->stream);
+    const int32_t rc = d.format == RVX_FORMAT_CU8
+                           ? launch_chain<RVX_FORMAT_CU8>(logm, a, T, ctx->stream)
+                       : d.taps_scaled ? launch_chain<kCs16Scaled>(logm, a, T, ctx->stream)
+                                       : launch_chain<RVX_FORMAT_CS16>(logm, a, T, ctx->stream);
     if (rc) return rc;
     return rvx::check_launch(who);
 }

@@ -329,7 +329,8 @@ extern "C" int32_t rvx_chain_mfma(rvx_ctx* ctx, const rvx_chain_desc* desc, int3
         d.taps_per_branch == 0 || d.taps_per_branch > 64)
         return rvx::fail(RVX_EINVAL, "%s: grid M=%u L=%u D=%u (M = 4096)", who, d.channels,
                          d.taps_per_branch, d.decimation);
-    if (d.channel_rows) return rvx::fail(RVX_EINVAL, "%s: channel rows are not supported", who);
+    if (d.channel_rows || d.taps_scaled)
+        return rvx::fail(RVX_EINVAL, "%s: channel rows and prescaled taps are not supported", who);
     if (n_taps != uint64_t(kM) * d.taps_per_branch || n_twiddles != uint64_t(kM))
         return rvx::fail(RVX_EINVAL, "%s: wants %llu taps and 4096 twiddles", who,
                          (unsigned long long)(uint64_t(kM) * d.taps_per_branch));

@@ -339,3 +339,18 @@ int32_t rvx_nco_delta(int64_t num, int64_t den, int64_t rate, uint64_t* out) {
 }
 
 }  // extern "C"
+
+extern "C" int32_t rvx_prescale_taps(const float* taps, uint64_t n, float* out) {
+    static const char* who = "rvx_prescale_taps";
+    if ((n && (!taps || !out))) return rvx::fail(RVX_EINVAL, "%s: null buffer", who);
+    for (uint64_t i = 0; i < n; ++i) {
+        const float h = taps[i];
+        if (!std::isfinite(h))
+            return rvx::fail(RVX_EINVAL, "%s: tap %llu is not finite", who, (unsigned long long)i);
+        if (h != 0.0f && std::fabs(h) < std::ldexp(1.0f, -111))
+            return rvx::fail(RVX_EINVAL, "%s: tap %llu (%g) would scale below the normal range",
+                             who, (unsigned long long)i, double(h));
+        out[i] = std::ldexp(h, -15);
+    }
+    return RVX_OK;
+}

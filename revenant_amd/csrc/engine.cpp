@@ -186,6 +186,7 @@ struct rvx_engine {
     void* tw = nullptr;
     void* nco = nullptr;
     uint32_t nco_entries = 0;
+    bool taps_scaled = false;
     uint8_t* win = nullptr;
     uint8_t* bounce = nullptr;
     uint8_t* stage_h = nullptr;  // pinned copy of a host push
@@ -396,7 +397,16 @@ int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const f
         return rc;
     }
     const hipStream_t s = ctx->stream;
-    hipError_t err = hipMemcpyAsync(e->taps, prototype, n_prototype * sizeof(float),
+    // cs16: taps carrying the conversion's 2^-15 when that is exact (the
+    // chain's products are then the same bits with one multiply fewer).
+    std::vector<float> scaled(n_prototype);
+    const float* upload = prototype;
+    if (c.format == RVX_FORMAT_CS16 &&
+        rvx_prescale_taps(prototype, n_prototype, scaled.data()) == RVX_OK) {
+        upload = scaled.data();
+        e->taps_scaled = true;
+    }
+    hipError_t err = hipMemcpyAsync(e->taps, upload, n_prototype * sizeof(float),
                                     hipMemcpyHostToDevice, s);
     if (err == hipSuccess)
         err = hipMemcpyAsync(e->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice, s);
@@ -559,6 +569,7 @@ int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int3
     d.out_ring_blocks = e->cfg.ring_blocks;
     d.frame_blocks = e->cfg.frame_blocks;
     d.alpha = e->cfg.alpha;
+    d.taps_scaled = e->taps_scaled ? 1u : 0u;
     const uint64_t frames = e->cfg.frame_blocks ? count / e->cfg.frame_blocks : 0;
     if (int32_t rc = rvx_chain(e->ctx, &d, e->win, e->taps, e->N, e->tw, M, e->ring,
                                uint64_t(M) * e->cfg.ring_blocks, e->partial, frames * M))

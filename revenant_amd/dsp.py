@@ -197,3 +197,12 @@ def plan_vrx(grid: GridParams, rate: int, request: VrxRequest,
     _lib.check(_lib.lib().rvx_plan_vrx_tables(C.byref(out), _fptr(taps), taps.size, _fptr(w),
                                               w.size))
     return VrxPlan(out, taps.view(np.complex64), w)
+
+
+def prescale_taps(taps: np.ndarray) -> np.ndarray:
+    """taps * 2^-15 exactly, for a cs16 chain with taps_scaled = 1
+    (rvx_prescale_taps refuses taps that would leave the normal range)."""
+    t = np.ascontiguousarray(taps, dtype=np.float32)
+    out = np.empty_like(t)
+    _lib.check(_lib.lib().rvx_prescale_taps(_fptr(t), t.size, _fptr(out)))
+    return out

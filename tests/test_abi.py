@@ -6,6 +6,7 @@ import re
 from pathlib import Path
 
 import numpy as np
+import pytest
 
 from revenant_amd import _lib
 
@@ -66,3 +67,17 @@ def test_launch_without_device_reports_an_error():
     rc = lib.rvx_stream_sync(None)
     assert rc == -1
     assert b"null context" in lib.rvx_last_error()
+
+
+
+def test_prescale_taps_exact_and_refusals():
+    import numpy as np
+
+    from revenant_amd.dsp import prescale_taps
+
+    t = np.array([0.0, 1.0, -3.5, 1e-30, 2.0 ** -111, 65504.0], dtype=np.float32)
+    out = prescale_taps(t)
+    assert np.array_equal(out, t * np.float32(2.0 ** -15))
+    for bad in (np.float32(2.0 ** -112), np.float32(np.inf), np.float32(np.nan)):
+        with pytest.raises(_lib.RvxError):
+            prescale_taps(np.array([1.0, bad], dtype=np.float32))

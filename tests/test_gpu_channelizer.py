@@ -225,3 +225,38 @@ def test_chain_stores_selected_channels(dev, grid, blocks):
     for r, k in enumerate(chosen):
         assert_bit_exact(got[r, slots], full[k, slots], SEED_CHAIN, f"row {r} = channel {k}")
     assert_bit_exact(from_dev(partial), partial_cpu, SEED_CHAIN, "spectrum over every channel")
+
+
+
+@pytest.mark.parametrize("grid,blocks", [(GridParams(4096, 16, 2048), 32),
+                                         (GridParams(512, 16, 256), 64),
+                                         (GridParams(1024, 17, 512), 24)],
+                         ids=["M4096", "M512", "M1024L17"])
+def test_chain_prescaled_taps_bit_exact(dev, grid, blocks):
+    """cs16 against taps carrying the conversion's 2^-15: the same bits as the
+    twins with the plain taps (a product of the same real numbers)."""
+    from revenant_amd.dsp import prescale_taps
+
+    M, D, N = grid.channels, grid.decimation, grid.prototype_length
+    first_block = (N + D - 1) // D + 1
+    samples = (first_block + blocks) * D + 5
+    packed = _codes(FORMAT_CS16, samples, SEED_CHAIN + 3 * M)
+    taps = design_prototype(grid, beta=8.0).taps
+    tw = build_twiddles(M)
+    ring_blocks = 1 << (blocks - 1).bit_length()
+    alpha = np.float32(1.0 / 16.0)
+    od = oracle.OChain(FORMAT_CS16, M, grid.taps_per_branch, D, first_block, blocks, ring_blocks,
+                       8, alpha, ema_frame_decay(alpha, 8))
+    ring_cpu, partial_cpu = oracle.chain(od, packed, taps, tw, threads=4)
+    desc = _lib.ChainDesc(FORMAT_CS16, M, grid.taps_per_branch, D, 0, samples, first_block,
+                          blocks, ring_blocks, 8, alpha)
+    desc.taps_scaled = 1
+    ring = torch.full((2 * M * ring_blocks,), 3.0, dtype=torch.float32, device="cuda")
+    partial = torch.zeros((blocks // 8) * M, dtype=torch.float32, device="cuda")
+    dev.chain(desc, to_dev(packed), to_dev(prescale_taps(taps)), to_dev(tw.view(np.float32)),
+              ring, partial)
+    got = from_dev(ring).reshape(M, ring_blocks, 2)
+    slots = [(first_block + b) & (ring_blocks - 1) for b in range(blocks)]
+    assert_bit_exact(got[:, slots], ring_cpu.reshape(M, ring_blocks, 2)[:, slots], SEED_CHAIN,
+                     f"prescaled ring {grid}")
+    assert_bit_exact(from_dev(partial), partial_cpu, SEED_CHAIN, f"prescaled ema {grid}")
