@@ -170,6 +170,34 @@ typedef struct rvx_vrx_demod_params {
 #define RVX_DEMOD_LSB 5
 #define RVX_DEMOD_DSB 6
 #define RVX_DEMOD_CW 7
+#define RVX_DEMOD_SAM 12
+
+/* Carrier loop kinds, core/dsp/vrx_reference.h:674-676: SAM runs the
+ * phase-locked loop, DSB the Costas loop, every other mode none. */
+#define RVX_CARRIER_NONE 0
+#define RVX_CARRIER_PLL 1
+#define RVX_CARRIER_COSTAS 2
+#define RVX_CARRIER_FLAG_RESET 1u /* vrx_reference.h:694 */
+#define RVX_CARRIER_STATE_WORDS 8 /* vrx_reference.h:723-731 */
+
+/* core/dsp/vrx_reference.h:686-690 VrxCarrierConfig */
+typedef struct rvx_vrx_carrier_config {
+    uint32_t power;    /* RVX_CARRIER_PLL or RVX_CARRIER_COSTAS */
+    uint32_t nco_log2; /* 1..24, the NCO table rvx_build_nco_table makes */
+} rvx_vrx_carrier_config;
+
+/* core/dsp/vrx_reference.h:699-718 VrxCarrierParams (fourteen words) */
+typedef struct rvx_vrx_carrier_params {
+    uint32_t in_mask;   /* fine ring capacity - 1 (a power of two minus one) */
+    uint32_t in_offset; /* first fine sample rotated */
+    uint32_t count;     /* samples rotated, <= capacity */
+    uint32_t flags;     /* RVX_CARRIER_FLAG_RESET */
+    float shift;        /* radians per sample added to the integrator first */
+    float k1_acquire, k2_acquire, k1_track, k2_track;
+    float level_alpha, lock_alpha;
+    float clamp_radians; /* integrator bound, (0, pi/2] */
+    uint32_t hold_in, hold_out;
+} rvx_vrx_carrier_params;
 
 /* Native sample formats, core/dsp/convert.h */
 #define RVX_FORMAT_CU8 0
@@ -281,6 +309,9 @@ typedef struct rvx_vrx_plan {
     rvx_vrx_demod_config demod;
     float demod_gain;
     double audio_pass_hz, audio_stop_hz;
+    rvx_vrx_carrier_config carrier;      /* power RVX_CARRIER_NONE: no loop */
+    rvx_vrx_carrier_params carrier_loop; /* design_carrier_loop; per-dispatch words zero */
+    uint32_t reserved2;
 } rvx_vrx_plan;
 
 /* core/engine/vrx_place.cpp:216 place: nearest channel round(center*M/rate),
@@ -288,7 +319,8 @@ typedef struct rvx_vrx_plan {
 int32_t rvx_place(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* request,
                   rvx_vrx_placement* out);
 /* core/dsp/vrx_reference.cpp:2568 plan_vrx (AM, NFM, WFM mono, USB, LSB,
- * DSB, CW, raw; no de-emphasis, stereo or carrier loop). */
+ * DSB, CW, SAM, raw; no de-emphasis or stereo). SAM and DSB plans carry
+ * their carrier loop (:2550-2561). */
 int32_t rvx_plan_vrx(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* request,
                      const rvx_vrx_placement* placement, rvx_vrx_plan* out);
 /* The plan's tables: fine taps (2*(phases*taps+1) floats, design_fine_taps
@@ -305,6 +337,31 @@ int32_t rvx_demod_block(const rvx_vrx_plan* plan, uint32_t fine_mask, uint64_t f
                         uint32_t count, rvx_vrx_demod_params* out);
 /* core/dsp/vrx_reference.h:555 demod_fine_history. */
 uint32_t rvx_demod_history(const rvx_vrx_demod_config* config);
+
+/* Carrier loop design, core/dsp/vrx_reference.cpp:918-1185. */
+/* :918 carrier_power: RVX_CARRIER_* for a demod mode. */
+uint32_t rvx_carrier_power(uint32_t mode);
+/* :948 carrier_loop_gains: K1, K2 of the second-order loop for a noise
+ * bandwidth and damping at a rate, in double (both zero for a non-positive
+ * argument, as the reference returns). */
+int32_t rvx_carrier_loop_gains(double noise_bandwidth_hz, double damping, int64_t rate,
+                               double* k1, double* k2);
+/* :958 design_carrier_loop: the loop for a demodulation rate and a passband
+ * reaching reach_hz from the receiver's centre (per-dispatch words zero). */
+int32_t rvx_design_carrier_loop(int64_t demod_rate, int64_t reach_hz,
+                                rvx_vrx_carrier_params* out);
+/* :1187 carrier_retune: a move further than the granted width restarts the
+ * loop; a move inside it carries the loop by -2*pi*moved/demod_rate. */
+int32_t rvx_carrier_retune(int64_t moved_hz, int64_t granted_width, int64_t demod_rate,
+                           int32_t* reset, double* shift);
+/* :2911 carrier_block: one dispatch's params from the absolute index of its
+ * first fine sample (a reset drops the shift). */
+int32_t rvx_carrier_block(const rvx_vrx_plan* plan, uint32_t fine_mask, uint64_t first_fine,
+                          uint32_t count, int32_t reset, float shift,
+                          rvx_vrx_carrier_params* out);
+/* :986 validate(VrxCarrierConfig, VrxCarrierParams). */
+int32_t rvx_vrx_carrier_check(const rvx_vrx_carrier_config* config,
+                              const rvx_vrx_carrier_params* params);
 
 /* ------------------------------------------------------------------------ */
 /* Operators. Each replaces one reference compute shader + its dispatch.     */
@@ -374,6 +431,35 @@ int32_t rvx_vrx_demod(rvx_ctx* ctx, const rvx_vrx_demod_config* config,
                       const rvx_vrx_demod_params* params, const void* fine_ring,
                       uint64_t fine_samples, const float* weights, uint64_t n_weights,
                       float* audio, uint64_t n_audio);
+
+/* core/shaders/vrx_carrier.comp; twin core/dsp/vrx_reference.cpp:1052. The
+ * serial carrier loop (PLL for SAM, Costas for DSB): rotates params->count
+ * fine samples in place from in_offset onto the recovered carrier and
+ * carries RVX_CARRIER_STATE_WORDS words of state (device) between calls. nco:
+ * 2^nco_log2 cf32. fine ring: >= in_mask + 1 cf32. */
+int32_t rvx_vrx_carrier(rvx_ctx* ctx, const rvx_vrx_carrier_config* config,
+                        const rvx_vrx_carrier_params* params, const void* nco, uint64_t n_nco,
+                        void* fine_ring, uint64_t fine_samples, uint32_t* state);
+
+/* Many receivers' carrier loops in one launch, one lane per receiver, each
+ * bit-identical to rvx_vrx_carrier on its own. No counterpart in the
+ * reference, which dispatches one single-invocation kernel per receiver
+ * (core/engine/vrx_stage.cpp:1346). Items live in device memory; the host
+ * checks each with rvx_vrx_carrier_item_check before uploading it, and the
+ * kernel leaves an item whose sizes disagree untouched. Every item uses the
+ * one NCO table `nco` (n_nco = 2^config.nco_log2 cf32). Items must not share
+ * a fine ring slot or a state block. */
+typedef struct rvx_vrx_carrier_item {
+    rvx_vrx_carrier_config config;
+    rvx_vrx_carrier_params params;
+    void* fine_ring;    /* device, fine_samples cf32 */
+    uint64_t fine_samples;
+    uint32_t* state;    /* device, RVX_CARRIER_STATE_WORDS */
+    uint64_t reserved;
+} rvx_vrx_carrier_item;
+int32_t rvx_vrx_carrier_item_check(const rvx_vrx_carrier_item* item, uint64_t n_nco);
+int32_t rvx_vrx_carrier_batch(rvx_ctx* ctx, const rvx_vrx_carrier_item* items, uint32_t n_items,
+                              const void* nco, uint64_t n_nco);
 
 /* Batched receivers: many VRXs of any mix of modes and rates in one
  * persistent launch, each computing fine stage -> detector -> audio filter on

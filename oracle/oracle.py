@@ -59,7 +59,8 @@ def lib() -> C.CDLL:
         for name in ("oracle_convert", "oracle_cmul", "oracle_pfb_branch", "oracle_pfb_fft",
                      "oracle_fft_radix2", "oracle_spectrum", "oracle_spectrum_levels",
                      "oracle_det", "oracle_ema_partial", "oracle_ema_combine", "oracle_chain",
-                     "fp64_channelize", "fp64_dft", "oracle_vrx_fine", "oracle_vrx_demod"):
+                     "fp64_channelize", "fp64_dft", "oracle_vrx_fine", "oracle_vrx_demod",
+                     "oracle_vrx_carrier"):
             getattr(_lib, name).restype = i32
     return _lib
 
@@ -223,6 +224,20 @@ def vrx_demod(config, params, fine_ring: np.ndarray, weights: np.ndarray) -> np.
     _check(lib().oracle_vrx_demod(C.byref(config), C.byref(params), _a(fr), u64(fr.size // 2),
                                   _a(w), u64(w.size), _a(out), u64(out.size)), "vrx_demod")
     return out
+
+
+def vrx_carrier(config, params, nco: np.ndarray, fine_ring: np.ndarray,
+                state: np.ndarray) -> None:
+    """reference_vrx_carrier, in place on fine_ring (float32 pairs) and state
+    (eight uint32 words)."""
+    nc = f32c(nco)
+    if fine_ring.dtype != np.float32 or not fine_ring.flags.c_contiguous:
+        raise ValueError("fine ring must be contiguous float32 pairs")
+    if state.dtype != np.uint32 or state.size != 8:
+        raise ValueError("state must be eight uint32 words")
+    _check(lib().oracle_vrx_carrier(C.byref(config), C.byref(params), _a(nc), u64(nc.size // 2),
+                                    _a(fine_ring), u64(fine_ring.size // 2), _a(state),
+                                    u64(state.size)), "vrx_carrier")
 
 
 def detect(config, average: np.ndarray):

@@ -885,4 +885,145 @@ int oracle_chain(const OChain* c, const uint8_t* input, uint64_t input_bytes, co
     return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// The carrier loop. core/dsp/vrx_reference.cpp:1052-1185
+// (reference_vrx_carrier) and its validate(), :986-1050: PLL (power 1) for
+// SAM, Costas (power 2) for DSB, one serial pass over the fine ring rotating
+// it in place, eight words of state carried between calls.
+// Pinned by (tests/test_oracle_carrier.py), with the reference's own
+// measurements from tests/reference/test_vrx_carrier.cpp:684-900: both loops
+// lock on an offset carrier within the times the reference states and settle
+// within 1 Hz of it (a sign slip in the detector, the NCO or the advance
+// fails to lock or locks on the mirror), a DSB tone comes back at unit level
+// at sixteen carrier phases (0.995..1.005; a wrong Costas detector or level
+// fails), a drifting carrier is followed to the passband's reach and never
+// past it (the integrator clamp), a restart equals a fresh loop, and the
+// result does not depend on where dispatches fall.
+// ---------------------------------------------------------------------------
+
+struct OCarrierConfig {  // core/dsp/vrx_reference.h:686-690
+    uint32_t power, nco_log2;
+};
+struct OCarrierParams {  // core/dsp/vrx_reference.h:699-714
+    uint32_t in_mask, in_offset, count, flags;
+    float shift, k1_acquire, k2_acquire, k1_track, k2_track, level_alpha, lock_alpha,
+        clamp_radians;
+    uint32_t hold_in, hold_out;
+};
+static_assert(sizeof(OCarrierParams) == 56);
+
+static bool carrier_valid(const OCarrierConfig* c, const OCarrierParams* p) {
+    if (c->power != 1 && c->power != 2) return false;
+    if (c->nco_log2 == 0 || c->nco_log2 > 24) return false;
+    const uint64_t capacity = uint64_t(p->in_mask) + 1;
+    if ((capacity & uint64_t(p->in_mask)) != 0 || p->count > capacity) return false;
+    if ((p->flags & ~1u) != 0) return false;
+    if (!std::isfinite(p->shift)) return false;
+    if (!(p->clamp_radians > 0.0f) || !(double(p->clamp_radians) <= 0.5 * 3.14159265358979323846))
+        return false;
+    for (float k : {p->k1_acquire, p->k1_track, p->k2_acquire, p->k2_track})
+        if (!(k > 0.0f) || !(k < 1.0f)) return false;
+    for (float a : {p->level_alpha, p->lock_alpha})
+        if (!(a > 0.0f) || !(a <= 1.0f)) return false;
+    return p->hold_in != 0 && p->hold_out != 0;
+}
+
+// std::max(v, lo) then std::min(., hi)
+static float clamp_to(float v, float lo, float hi) { return std::min(std::max(v, lo), hi); }
+
+int oracle_vrx_carrier(const OCarrierConfig* c, const OCarrierParams* p, const float* nco_f,
+                       uint64_t n_nco, float* ring_f, uint64_t ring_samples, uint32_t* state,
+                       uint64_t state_words) {
+    if (!carrier_valid(c, p)) return -1;
+    if (n_nco != (uint64_t(1) << c->nco_log2)) return -1;
+    if (ring_samples < uint64_t(p->in_mask) + 1) return -1;
+    if (state_words != 8) return -1;
+    FlushGuard flush;
+    const float kTurnsPerRadian = static_cast<float>(683565275.576431632);
+    const float kLevelFloor = static_cast<float>(1.0e-24);
+    const float kLockRatio = static_cast<float>(2.0);
+    const bool reset = (p->flags & 1u) != 0;
+    const bool pll = c->power == 1;
+    uint32_t phase = reset ? 0u : state[0];
+    float integrator = reset ? 0.0f : bits_to_float(state[1]);
+    float level = reset ? 0.0f : bits_to_float(state[2]);
+    float lock_re = reset ? 0.0f : bits_to_float(state[3]);
+    float lock_im = reset ? 0.0f : bits_to_float(state[4]);
+    uint32_t locked = reset ? 0u : state[5];
+    uint32_t held = reset ? 0u : state[6];
+    uint32_t primed = reset ? 0u : state[7];
+    {
+        const float moved = integrator + p->shift;
+        integrator = clamp_to(moved, -p->clamp_radians, p->clamp_radians);
+    }
+    const uint32_t down = 32u - c->nco_log2;
+    for (uint32_t i = 0; i < p->count; ++i) {
+        const uint64_t slot = (p->in_offset + i) & p->in_mask;
+        const float zr = ring_f[2 * slot], zi = ring_f[2 * slot + 1];
+        const uint64_t t = phase >> down;
+        const float rr = nco_f[2 * t], ri = nco_f[2 * t + 1];
+        const float yr = zr * rr - zi * ri;
+        const float yi = zr * ri + zi * rr;
+        ring_f[2 * slot] = yr;
+        ring_f[2 * slot + 1] = yi;
+        const float rsq = yr * yr, isq = yi * yi;
+        const float power = rsq + isq;
+        const float cross = yr * yi;
+        float wr, wi, magnitude;
+        if (pll) {
+            wr = yr;
+            wi = yi;
+            magnitude = det_sqrt(power);
+        } else {
+            wr = rsq - isq;
+            wi = cross + cross;
+            magnitude = power;
+        }
+        if (primed == 0u) {
+            level = magnitude;
+            primed = 1u;
+        } else {
+            const float step = p->level_alpha * (magnitude - level);
+            level = level + step;
+        }
+        const float sr = p->lock_alpha * (wr - lock_re);
+        const float si = p->lock_alpha * (wi - lock_im);
+        lock_re = lock_re + sr;
+        lock_im = lock_im + si;
+        const bool inside = lock_re > kLockRatio * std::fabs(lock_im);
+        if (locked != 0u) {
+            held = inside ? (held > 0u ? held - 1u : 0u) : held + 1u;
+            if (held >= p->hold_out) {
+                locked = 0u;
+                held = 0u;
+            }
+        } else {
+            held = inside ? held + 1u : 0u;
+            if (held >= p->hold_in) {
+                locked = 1u;
+                held = 0u;
+            }
+        }
+        const float inverse = det_recip(std::max(level, kLevelFloor));
+        const float error = clamp_to((pll ? yi : cross) * inverse, -1.0f, 1.0f);
+        const float k1 = locked ? p->k1_track : p->k1_acquire;
+        const float k2 = locked ? p->k2_track : p->k2_acquire;
+        const float integrated = integrator + k2 * error;
+        integrator = clamp_to(integrated, -p->clamp_radians, p->clamp_radians);
+        const float advance = k1 * error + integrator;
+        const float turns = advance * kTurnsPerRadian;
+        phase = phase + static_cast<uint32_t>(static_cast<int32_t>(turns));
+    }
+    state[0] = phase;
+    state[1] = float_to_bits(integrator);
+    state[2] = float_to_bits(level);
+    state[3] = float_to_bits(lock_re);
+    state[4] = float_to_bits(lock_im);
+    state[5] = locked;
+    state[6] = held;
+    state[7] = primed;
+    return 0;
+}
+
 }  // extern "C"

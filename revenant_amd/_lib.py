@@ -75,6 +75,22 @@ class VrxDemodParams(C.Structure):
     _fields_ = [("in_mask", u32), ("in_offset", u32), ("count", u32), ("gain", f32)]
 
 
+class VrxCarrierConfig(C.Structure):
+    _fields_ = [("power", u32), ("nco_log2", u32)]
+
+
+class VrxCarrierParams(C.Structure):
+    _fields_ = [(n, u32) for n in ("in_mask", "in_offset", "count", "flags")] + \
+        [(n, f32) for n in ("shift", "k1_acquire", "k2_acquire", "k1_track", "k2_track",
+                            "level_alpha", "lock_alpha", "clamp_radians")] + \
+        [("hold_in", u32), ("hold_out", u32)]
+
+
+class VrxCarrierItem(C.Structure):
+    _fields_ = [("config", VrxCarrierConfig), ("params", VrxCarrierParams), ("fine_ring", vp),
+                ("fine_samples", u64), ("state", vp), ("reserved", u64)]
+
+
 class Carrier(C.Structure):
     _fields_ = [("kind", i32), ("reserved", i32), ("freq_hz", i64), ("tone_hz", i64),
                 ("amplitude", f64), ("index", f64), ("depth", f64)]
@@ -112,7 +128,9 @@ class VrxPlan(C.Structure):
                 ("bandwidth", i64), ("deviation", i64), ("filter_num", i64), ("filter_den", i64),
                 ("mix_num", i64), ("mix_den", i64), ("fine", VrxFineConfig), ("reserved", u32),
                 ("fine_nco_delta", u64), ("demod", VrxDemodConfig), ("demod_gain", f32),
-                ("audio_pass_hz", f64), ("audio_stop_hz", f64)]
+                ("audio_pass_hz", f64), ("audio_stop_hz", f64),
+                ("carrier", VrxCarrierConfig), ("carrier_loop", VrxCarrierParams),
+                ("reserved2", u32)]
 
 
 class VrxDesc(C.Structure):
@@ -208,6 +226,17 @@ _SIGNATURES = {
                              C.POINTER(VrxFineParams), C.POINTER(u64)]),
     "rvx_demod_block": (i32, [C.POINTER(VrxPlan), u32, u64, u32, C.POINTER(VrxDemodParams)]),
     "rvx_demod_history": (u32, [C.POINTER(VrxDemodConfig)]),
+    "rvx_carrier_power": (u32, [u32]),
+    "rvx_carrier_loop_gains": (i32, [f64, f64, i64, C.POINTER(f64), C.POINTER(f64)]),
+    "rvx_design_carrier_loop": (i32, [i64, i64, C.POINTER(VrxCarrierParams)]),
+    "rvx_carrier_retune": (i32, [i64, i64, i64, C.POINTER(i32), C.POINTER(f64)]),
+    "rvx_carrier_block": (i32, [C.POINTER(VrxPlan), u32, u64, u32, i32, f32,
+                                C.POINTER(VrxCarrierParams)]),
+    "rvx_vrx_carrier_check": (i32, [C.POINTER(VrxCarrierConfig), C.POINTER(VrxCarrierParams)]),
+    "rvx_vrx_carrier_item_check": (i32, [C.POINTER(VrxCarrierItem), u64]),
+    "rvx_vrx_carrier": (i32, [vp, C.POINTER(VrxCarrierConfig), C.POINTER(VrxCarrierParams), vp,
+                              u64, vp, u64, vp]),
+    "rvx_vrx_carrier_batch": (i32, [vp, vp, u32, vp, u64]),
     "rvx_synth": (i32, [vp, C.POINTER(Carrier), u32, C.POINTER(SynthParams), vp, u64]),
     "rvx_chain": (i32, [vp, C.POINTER(ChainDesc), vp, vp, u64, vp, u64, vp, u64, vp, u64]),
     "rvx_vrx_batch": (i32, [vp, vp, u32, vp, u32, vp, vp, u64, u32]),

@@ -6,11 +6,13 @@
 //                       round(centre*M/rate), exact rational residual)
 //   rvx_plan_vrx        core/dsp/vrx_reference.cpp:2243-2662 (design_vrx +
 //                       plan_vrx) for the receivers the chain runs: AM, NFM,
-//                       WFM mono, USB, LSB, DSB, CW and raw. Not planned here:
-//                       de-emphasis, FM stereo, the SAM/DSB carrier loop and
-//                       the digital-voice taps.
+//                       WFM mono, USB, LSB, DSB, CW, SAM and raw, SAM and DSB
+//                       with their carrier loop. Not planned here:
+//                       de-emphasis, FM stereo and the digital-voice taps.
 //   rvx_fine_block      vrx_reference.cpp:2786-2871 (resample_block)
 //   rvx_demod_block     vrx_reference.cpp:2873-2909
+//   rvx_carrier_*       vrx_reference.cpp:918-1200 and :2911-2933 (the loop's
+//                       design: gains, one-poles, holds, reach, retune, block)
 // Tap tables: design_fine_taps (:1808), design_audio_taps (:1921),
 // design_dc_weights (:1983).
 
@@ -34,6 +36,17 @@ constexpr int64_t kAmDcCornerHz = 200;
 constexpr double kFmAudioCeilingHz = 15000.0;
 constexpr double kFmPilotHz = 19000.0;
 constexpr int64_t kCompositeAudioRateHz = 114000;
+// The carrier loop's design figures (core/dsp/vrx_reference.h:787-820): noise
+// bandwidths acquiring and tracking, damping 1/sqrt(2), the level and lock
+// one-poles' time constants, and the lock test's holds.
+constexpr double kCarrierAcquireHz = 400.0;
+constexpr double kCarrierTrackHz = 30.0;
+constexpr double kCarrierDamping = 0.70710678118654752440;
+constexpr double kCarrierLevelSeconds = 0.005;
+constexpr double kCarrierLockSeconds = 0.020;
+constexpr double kCarrierHoldInSeconds = 0.010;
+constexpr double kCarrierHoldOutSeconds = 0.200;
+constexpr double kCarrierClampCeiling = 0.5 * kPi;
 
 int64_t iabs(int64_t v) { return v < 0 ? -v : v; }
 
@@ -108,6 +121,7 @@ Band default_band(uint32_t mode) {
         case RVX_DEMOD_LSB: return {-2700, -300};
         case RVX_DEMOD_DSB: return {-3000, 3000};
         case RVX_DEMOD_CW: return {-600, 600};
+        case RVX_DEMOD_SAM: return {-5000, 5000};
         default: return {};
     }
 }
@@ -224,7 +238,7 @@ int32_t rvx_plan_vrx(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* 
     const rvx_vrx_placement pl = *placement;
     if (rate <= 0 || pl.channel_rate <= 0 || pl.residual_den <= 0)
         return rvx::fail(RVX_EINVAL, "%s: placement carries no rates", who);
-    if (r.mode > RVX_DEMOD_CW)
+    if (r.mode > RVX_DEMOD_CW && r.mode != RVX_DEMOD_SAM)
         return rvx::fail(RVX_EINVAL, "%s: mode %u is not planned by this build", who, r.mode);
     rvx_vrx_plan plan{};
     plan.placement = pl;
@@ -289,7 +303,7 @@ int32_t rvx_plan_vrx(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* 
     plan.demod.channels = cx ? 2u : 1u;
     plan.demod.pilot_taps = 0;
     plan.demod.dc_taps =
-        (r.mode == RVX_DEMOD_AM)
+        (r.mode == RVX_DEMOD_AM || r.mode == RVX_DEMOD_SAM)
             ? uint32_t(std::clamp<int64_t>(plan.demod_rate / kAmDcCornerHz, 16, 4096))
             : 1u;
     if (r.mode == RVX_DEMOD_NFM)
@@ -299,6 +313,14 @@ int32_t rvx_plan_vrx(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* 
     plan.demod_gain = 1.0f;
     if ((r.mode == RVX_DEMOD_NFM || r.mode == RVX_DEMOD_WFM) && plan.deviation > 0)
         plan.demod_gain = float(double(plan.demod_rate) / (kTwoPi * double(plan.deviation)));
+    // The carrier loop (vrx_reference.cpp:2550-2561): its NCO is the fine
+    // stage's table, and the integrator reaches as far as the passband does.
+    plan.carrier.power = rvx_carrier_power(r.mode);
+    plan.carrier.nco_log2 = plan.fine.nco_log2;
+    if (plan.carrier.power != RVX_CARRIER_NONE) {
+        if (int32_t rc = rvx_design_carrier_loop(plan.demod_rate, band.reach(), &plan.carrier_loop))
+            return rc;
+    }
     *out = plan;
     return RVX_OK;
 }
@@ -440,6 +462,97 @@ int32_t rvx_demod_block(const rvx_vrx_plan* plan, uint32_t fine_mask, uint64_t f
 
 uint32_t rvx_demod_history(const rvx_vrx_demod_config* config) {
     return config ? rvx::demod_fine_history(*config) : 0u;
+}
+
+uint32_t rvx_carrier_power(uint32_t mode) {
+    if (mode == RVX_DEMOD_SAM) return RVX_CARRIER_PLL;
+    if (mode == RVX_DEMOD_DSB) return RVX_CARRIER_COSTAS;
+    return RVX_CARRIER_NONE;
+}
+
+int32_t rvx_carrier_loop_gains(double noise_bandwidth_hz, double damping, int64_t rate,
+                               double* k1, double* k2) {
+    if (!k1 || !k2) return rvx::fail(RVX_EINVAL, "rvx_carrier_loop_gains: null output");
+    *k1 = *k2 = 0.0;
+    if (!(noise_bandwidth_hz > 0.0) || !(damping > 0.0) || rate <= 0) return RVX_OK;
+    // Gardner's second-order loop at sample period T: theta = Bn*T/(zeta +
+    // 1/(4 zeta)), K1 = 4 zeta theta / D, K2 = 4 theta^2 / D with D = 1 + 2
+    // zeta theta + theta^2.
+    const double period = 1.0 / double(rate);
+    const double theta = noise_bandwidth_hz * period / (damping + 1.0 / (4.0 * damping));
+    const double denominator = 1.0 + 2.0 * damping * theta + theta * theta;
+    *k1 = 4.0 * damping * theta / denominator;
+    *k2 = 4.0 * theta * theta / denominator;
+    return RVX_OK;
+}
+
+int32_t rvx_design_carrier_loop(int64_t demod_rate, int64_t reach_hz,
+                                rvx_vrx_carrier_params* out) {
+    static const char* who = "rvx_design_carrier_loop";
+    if (!out) return rvx::fail(RVX_EINVAL, "%s: null output", who);
+    if (demod_rate <= 0)
+        return rvx::fail(RVX_EINVAL, "%s: demodulation rate must be positive, got %lld", who,
+                         (long long)demod_rate);
+    if (reach_hz <= 0)
+        return rvx::fail(RVX_EINVAL, "%s: a passband reaching %lld Hz from its centre holds no "
+                         "carrier", who, (long long)reach_hz);
+    const double fs = double(demod_rate);
+    double a1, a2, t1, t2;
+    rvx_carrier_loop_gains(kCarrierAcquireHz, kCarrierDamping, demod_rate, &a1, &a2);
+    rvx_carrier_loop_gains(kCarrierTrackHz, kCarrierDamping, demod_rate, &t1, &t2);
+    const auto one_pole = [fs](double seconds) {
+        return float(1.0 - std::exp(-1.0 / (seconds * fs)));
+    };
+    const auto samples = [fs](double seconds) {
+        return uint32_t(std::max(1.0, std::round(seconds * fs)));
+    };
+    rvx_vrx_carrier_params p{};
+    p.k1_acquire = float(a1);
+    p.k2_acquire = float(a2);
+    p.k1_track = float(t1);
+    p.k2_track = float(t2);
+    p.level_alpha = one_pole(kCarrierLevelSeconds);
+    p.lock_alpha = one_pole(kCarrierLockSeconds);
+    p.clamp_radians = float(std::min(kTwoPi * double(reach_hz) / fs, kCarrierClampCeiling));
+    p.hold_in = samples(kCarrierHoldInSeconds);
+    p.hold_out = samples(kCarrierHoldOutSeconds);
+    *out = p;
+    return RVX_OK;
+}
+
+int32_t rvx_carrier_retune(int64_t moved_hz, int64_t granted_width, int64_t demod_rate,
+                           int32_t* reset, double* shift) {
+    if (!reset || !shift) return rvx::fail(RVX_EINVAL, "rvx_carrier_retune: null output");
+    *reset = 0;
+    *shift = 0.0;
+    // Strictly further than the granted width either way is another signal.
+    if (moved_hz > granted_width || -moved_hz > granted_width) {
+        *reset = 1;
+        return RVX_OK;
+    }
+    // Inside it the carrier moves the other way in the fine stream by exactly
+    // the move, in radians per sample at the demodulation rate.
+    if (demod_rate > 0) *shift = -kTwoPi * double(moved_hz) / double(demod_rate);
+    return RVX_OK;
+}
+
+int32_t rvx_carrier_block(const rvx_vrx_plan* plan, uint32_t fine_mask, uint64_t first_fine,
+                          uint32_t count, int32_t reset, float shift,
+                          rvx_vrx_carrier_params* out) {
+    static const char* who = "rvx_carrier_block";
+    if (!plan || !out) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    if (plan->carrier.power == RVX_CARRIER_NONE)
+        return rvx::fail(RVX_EINVAL, "%s: a mode %u receiver runs no carrier loop", who,
+                         plan->mode);
+    rvx_vrx_carrier_params p = plan->carrier_loop;
+    p.in_mask = fine_mask;
+    p.in_offset = uint32_t(first_fine & uint64_t(fine_mask));
+    p.count = count;
+    p.flags = reset ? RVX_CARRIER_FLAG_RESET : 0u;
+    p.shift = reset ? 0.0f : shift;  // a restarted loop has no frequency to carry
+    if (int32_t rc = rvx::vrx_carrier_valid(plan->carrier, p, who)) return rc;
+    *out = p;
+    return RVX_OK;
 }
 
 }  // extern "C"

@@ -18,6 +18,13 @@ FORMAT_CU8, FORMAT_CS8, FORMAT_CS16, FORMAT_CS24 = 0, 1, 2, 3
 BYTES_PER_SAMPLE = {FORMAT_CU8: 2, FORMAT_CS8: 2, FORMAT_CS16: 4, FORMAT_CS24: 6}
 
 DEMOD_RAW, DEMOD_AM, DEMOD_NFM, DEMOD_WFM, DEMOD_USB, DEMOD_LSB, DEMOD_DSB, DEMOD_CW = range(8)
+DEMOD_SAM = 12
+CARRIER_NONE, CARRIER_PLL, CARRIER_COSTAS = 0, 1, 2  # core/dsp/vrx_reference.h:674-676
+CARRIER_FLAG_RESET = 1
+CARRIER_STATE_WORDS = 8
+# core/dsp/vrx_reference.h:787-820, the loop's design figures
+CARRIER_ACQUIRE_HZ, CARRIER_TRACK_HZ = 400.0, 30.0
+CARRIER_DAMPING = 0.70710678118654752440
 
 
 @dataclass(frozen=True)
@@ -172,6 +179,51 @@ class VrxPlan:
     def history(self) -> int:
         """Fine samples below an output's index the demodulator reads."""
         return int(_lib.lib().rvx_demod_history(C.byref(self.plan.demod)))
+
+    @property
+    def carrier(self) -> _lib.VrxCarrierConfig:
+        return self.plan.carrier
+
+    def carrier_block(self, fine_mask: int, first_fine: int, count: int, reset: bool = False,
+                      shift: float = 0.0) -> _lib.VrxCarrierParams:
+        """core/dsp/vrx_reference.cpp:2911 carrier_block."""
+        out = _lib.VrxCarrierParams()
+        _lib.check(_lib.lib().rvx_carrier_block(C.byref(self.plan), fine_mask, first_fine, count,
+                                                1 if reset else 0, shift, C.byref(out)))
+        return out
+
+
+def carrier_power(mode: int) -> int:
+    """core/dsp/vrx_reference.cpp:918 carrier_power."""
+    return int(_lib.lib().rvx_carrier_power(mode))
+
+
+def carrier_loop_gains(noise_bandwidth_hz: float, damping: float, rate: int) -> tuple[float, float]:
+    """core/dsp/vrx_reference.cpp:948 carrier_loop_gains: (K1, K2) in double."""
+    k1, k2 = C.c_double(), C.c_double()
+    _lib.check(_lib.lib().rvx_carrier_loop_gains(noise_bandwidth_hz, damping, rate, C.byref(k1),
+                                                 C.byref(k2)))
+    return k1.value, k2.value
+
+
+def design_carrier_loop(demod_rate: int, reach_hz: int) -> _lib.VrxCarrierParams:
+    """core/dsp/vrx_reference.cpp:958 design_carrier_loop."""
+    out = _lib.VrxCarrierParams()
+    _lib.check(_lib.lib().rvx_design_carrier_loop(demod_rate, reach_hz, C.byref(out)))
+    return out
+
+
+def carrier_retune(moved_hz: int, granted_width: int, demod_rate: int) -> tuple[bool, float]:
+    """core/dsp/vrx_reference.cpp:1187 carrier_retune: (reset, shift)."""
+    reset, shift = C.c_int32(), C.c_double()
+    _lib.check(_lib.lib().rvx_carrier_retune(moved_hz, granted_width, demod_rate, C.byref(reset),
+                                             C.byref(shift)))
+    return bool(reset.value), shift.value
+
+
+def carrier_check(config: _lib.VrxCarrierConfig, params: _lib.VrxCarrierParams) -> None:
+    """core/dsp/vrx_reference.cpp:986 validate; raises RvxError with its message."""
+    _lib.check(_lib.lib().rvx_vrx_carrier_check(C.byref(config), C.byref(params)))
 
 
 def place(grid: GridParams, rate: int, request: VrxRequest) -> _lib.VrxPlacement:

@@ -145,6 +145,24 @@ class Device:
                                           _p(fine_ring), _csamples(fine_ring), _p(weights),
                                           _n(weights), _p(audio), _n(audio)))
 
+    def vrx_carrier(self, config: _lib.VrxCarrierConfig, params: _lib.VrxCarrierParams,
+                    nco: torch.Tensor, fine_ring: torch.Tensor, state: torch.Tensor) -> None:
+        """The carrier loop over one receiver's fine ring, in place; state is
+        eight int32/uint32 words on the device."""
+        if state.numel() != 8 or state.element_size() != 4:
+            raise ValueError("carrier state is eight 32-bit words")
+        _lib.check(self.lib.rvx_vrx_carrier(self.ctx, C.byref(config), C.byref(params), _p(nco),
+                                            _csamples(nco), _p(fine_ring), _csamples(fine_ring),
+                                            _p(state)))
+
+    def vrx_carrier_batch(self, items: torch.Tensor, n_items: int, nco: torch.Tensor) -> None:
+        """Many receivers' carrier loops on one NCO table; items is a device
+        copy of rvx_vrx_carrier_item records (each checked on the host)."""
+        if items.numel() * items.element_size() < n_items * C.sizeof(_lib.VrxCarrierItem):
+            raise ValueError("item buffer smaller than n_items records")
+        _lib.check(self.lib.rvx_vrx_carrier_batch(self.ctx, _p(items), n_items, _p(nco),
+                                                  _csamples(nco)))
+
     def chain(self, desc: _lib.ChainDesc, packed: torch.Tensor, taps: torch.Tensor,
               twiddles: torch.Tensor, ring: torch.Tensor,
               partial: torch.Tensor | None = None) -> None:
