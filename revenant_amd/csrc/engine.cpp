@@ -16,7 +16,14 @@
 //     whole spectrum rows when the averaged spectrum is on;
 //   - each receiver keeps its next audio index (the reference's next_audio_
 //     cursor) and produces every output whose channel samples are in the ring;
-//     outputs whose samples have left the ring are counted as lost.
+//     outputs whose samples have left the ring are counted as lost;
+//   - receivers with a carrier loop (SAM, DSB) take the reference's three
+//     stages (core/engine/vrx_stage.cpp:1306,1346,1399): the fine stage into a
+//     per-receiver fine ring, the loops of every such receiver in one launch
+//     (the loop runs over the fine stream in order, so it cannot sit inside a
+//     tile), then the demodulator from the ring. A receiver's loop carries its
+//     state from push to push; a gap in its fine stream (lost audio) restarts
+//     it.
 
 #include <hip/hip_runtime.h>
 
@@ -77,6 +84,14 @@ void audio_range(const rvx_vrx_plan& p, uint64_t first, uint64_t last, uint64_t*
     *n = a_hi >= a_lo ? a_hi - a_lo + 1 : 0;
 }
 
+// The 32-bit resampler recurrence's limit on one fine dispatch
+// (frac0 + i*step_rem below 2^32).
+uint64_t max_fine_outputs(const rvx_vrx_plan& p) {
+    const uint64_t fc = uint64_t(p.channel_rate), fd = uint64_t(p.demod_rate);
+    const uint64_t rem = fc % fd;
+    return rem == 0 ? 0xFFFFFFFFull : (0xFFFFFFFFull - (fd - 1)) / rem + 1;
+}
+
 }  // namespace
 
 extern "C" int32_t rvx_vrx_batch_plan(const rvx_vrx_span* spans, uint32_t n_spans,
@@ -97,6 +112,12 @@ extern "C" int32_t rvx_vrx_batch_plan(const rvx_vrx_span* spans, uint32_t n_span
         if (!sp.plan) return rvx::fail(RVX_EINVAL, "%s: span %u has no plan", who, s);
         if (sp.count == 0) continue;
         const rvx_vrx_plan& p = *sp.plan;
+        if (p.carrier.power != RVX_CARRIER_NONE)
+            return rvx::fail(RVX_EINVAL,
+                             "%s: span %u is a mode %u receiver, whose carrier loop runs over "
+                             "its fine stream in order: it takes rvx_vrx_fine, "
+                             "rvx_vrx_carrier_batch and rvx_vrx_demod, not a batched tile",
+                             who, s, p.mode);
         const uint32_t cap = max_tile(p);
         if (cap == 0) return rvx::fail(RVX_EINVAL, "%s: span %u's filters do not fit a tile", who, s);
         uint32_t tile = sp.tile ? std::min(sp.tile, cap) : cap;
@@ -162,7 +183,23 @@ struct EngineRx {
     bool started = false;
     uint64_t next_audio = 0;
     uint64_t read_audio = 0;
+    uint64_t n_weights = 0;
+    // carrier-loop receivers (SAM, DSB) only
+    bool carrier = false;
+    void* fine = nullptr;        // fine ring, fine_cap cf32
+    uint64_t fine_cap = 0;
+    uint32_t* cstate = nullptr;  // the loop's RVX_CARRIER_STATE_WORDS
+    bool fine_started = false;
+    uint64_t next_fine = 0;      // first fine sample not yet produced and rotated
 };
+
+void free_rx(EngineRx& r) {
+    for (void* q : {r.taps, (void*)r.weights, (void*)r.audio, r.fine, (void*)r.cstate})
+        if (q) (void)hipFree(q);
+    r.taps = r.fine = nullptr;
+    r.weights = r.audio = nullptr;
+    r.cstate = nullptr;
+}
 
 template <class T>
 int32_t device_alloc(T** p, uint64_t bytes, const char* who) {
@@ -208,33 +245,136 @@ struct rvx_engine {
     uint32_t descs_cap = 0, work_cap = 0;
     uint32_t* counter = nullptr;
     hipEvent_t staged = nullptr;
+    rvx_vrx_carrier_item* items_h = nullptr;  // carrier loops of one push
+    void* items_d = nullptr;
+    uint32_t items_cap = 0;
     rvx_engine_stats stats{};
 
     ~rvx_engine() {
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
-        for (auto& r : rx) {
-            (void)hipFree(r.taps);
-            (void)hipFree(r.weights);
-            (void)hipFree(r.audio);
-        }
+        for (auto& r : rx) free_rx(r);
         for (void* p : {(void*)taps, tw, nco, (void*)win, (void*)bounce, ring, (void*)partial,
-                        (void*)rows, (void*)state, descs_d, work_d, (void*)counter})
+                        (void*)rows, (void*)state, descs_d, work_d, (void*)counter, items_d})
             if (p) (void)hipFree(p);
         if (stage_h) (void)hipHostFree(stage_h);
         if (pushed) (void)hipEventDestroy(pushed);
         if (descs_h) (void)hipHostFree(descs_h);
         if (work_h) (void)hipHostFree(work_h);
+        if (items_h) (void)hipHostFree(items_h);
         if (staged) (void)hipEventDestroy(staged);
     }
 };
 
 namespace {
 
+// One carrier-loop receiver's share of a push: audio a0 .. a0+count-1 from
+// fine samples first_fine .. the last, after the loop has rotated them.
+struct CarrierJob {
+    EngineRx* r;
+    uint64_t a0, count;
+};
+
+// The fine stage of one carrier receiver over its new fine samples, in
+// dispatches the resampler recurrence allows, and the loop's item.
+int32_t carrier_fine(rvx_engine& e, EngineRx& r, uint64_t a0, uint64_t count,
+                     rvx_vrx_carrier_item* item, const char* who) {
+    const uint64_t R = r.plan.demod.decimation, hist = rvx::demod_fine_history(r.plan.demod);
+    const uint64_t need = a0 * R - hist;  // audio_range keeps a0*R >= hist
+    const uint64_t last = (a0 + count - 1) * R;
+    bool reset = false;
+    if (!r.fine_started || r.next_fine < need) {  // the first push, or a gap
+        r.fine_started = true;
+        r.next_fine = need;
+        reset = true;
+    }
+    const uint64_t n_new = last + 1 - r.next_fine;
+    const uint32_t mask = uint32_t(r.fine_cap - 1);
+    const uint64_t limit = std::min<uint64_t>(max_fine_outputs(r.plan), 0x7FFFFFFFull);
+    const uint64_t n_taps = uint64_t(r.plan.fine.phases) * r.plan.fine.taps + 1;
+    for (uint64_t at = r.next_fine; at <= last;) {
+        const uint32_t n = uint32_t(std::min<uint64_t>(last + 1 - at, limit));
+        rvx_vrx_fine_params fp{};
+        uint64_t first_input = 0;
+        if (int32_t rc = rvx_fine_block(&r.plan, r.plan.placement.channel * e.cfg.ring_blocks,
+                                        e.cfg.ring_blocks - 1, mask, at, n, &fp, &first_input))
+            return rc;
+        if (int32_t rc = rvx_vrx_fine(e.ctx, &r.plan.fine, &fp, e.ring,
+                                      uint64_t(e.cfg.channels) * e.cfg.ring_blocks, r.taps, n_taps,
+                                      e.nco, e.nco_entries, r.fine, r.fine_cap))
+            return rc;
+        at += n;
+    }
+    rvx_vrx_carrier_item it{};
+    it.config = r.plan.carrier;
+    if (int32_t rc = rvx_carrier_block(&r.plan, mask, r.next_fine, uint32_t(n_new), reset ? 1 : 0,
+                                       0.0f, &it.params))
+        return rc;
+    it.fine_ring = r.fine;
+    it.fine_samples = r.fine_cap;
+    it.state = r.cstate;
+    if (int32_t rc = rvx_vrx_carrier_item_check(&it, e.nco_entries)) return rc;
+    *item = it;
+    r.next_fine = last + 1;
+    (void)who;
+    return RVX_OK;
+}
+
+// The loops of every carrier receiver in one launch, then each receiver's
+// demodulator from its rotated fine ring into its audio ring.
+int32_t carrier_receivers(rvx_engine& e, const std::vector<CarrierJob>& jobs,
+                          const std::vector<rvx_vrx_carrier_item>& items, const char* who) {
+    if (jobs.empty()) return RVX_OK;
+    const uint32_t n = uint32_t(items.size());
+    RVX_TRY_HIP(hipEventSynchronize(e.staged), who);
+    if (n > e.items_cap) {
+        RVX_TRY_HIP(hipStreamSynchronize(e.ctx->stream), who);
+        (void)hipHostFree(e.items_h);
+        (void)hipFree(e.items_d);
+        e.items_h = nullptr;
+        e.items_d = nullptr;
+        e.items_cap = std::max(n, 2 * e.items_cap);
+        RVX_TRY_HIP(hipHostMalloc((void**)&e.items_h, e.items_cap * sizeof(rvx_vrx_carrier_item),
+                                  hipHostMallocDefault),
+                    who);
+        if (int32_t rc = device_alloc(&e.items_d, e.items_cap * sizeof(rvx_vrx_carrier_item), who))
+            return rc;
+    }
+    std::memcpy(e.items_h, items.data(), n * sizeof(rvx_vrx_carrier_item));
+    RVX_TRY_HIP(hipMemcpyAsync(e.items_d, e.items_h, n * sizeof(rvx_vrx_carrier_item),
+                               hipMemcpyHostToDevice, e.ctx->stream),
+                who);
+    RVX_TRY_HIP(hipEventRecord(e.staged, e.ctx->stream), who);
+    if (int32_t rc = rvx_vrx_carrier_batch(e.ctx, static_cast<const rvx_vrx_carrier_item*>(e.items_d),
+                                           n, e.nco, e.nco_entries))
+        return rc;
+    for (const CarrierJob& j : jobs) {
+        EngineRx& r = *j.r;
+        const uint32_t ch = r.plan.demod.channels;
+        uint64_t a = j.a0, left = j.count;
+        while (left) {  // one dispatch per contiguous stretch of the audio ring
+            const uint64_t pos = a % r.cap;
+            const uint64_t seg = std::min<uint64_t>({left, r.cap - pos, 0x7FFFFFFFull});
+            rvx_vrx_demod_params dp{};
+            if (int32_t rc = rvx_demod_block(&r.plan, uint32_t(r.fine_cap - 1), a, uint32_t(seg),
+                                             &dp))
+                return rc;
+            if (int32_t rc = rvx_vrx_demod(e.ctx, &r.plan.demod, &dp, r.fine, r.fine_cap,
+                                           r.weights, r.n_weights, r.audio + pos * ch, seg * ch))
+                return rc;
+            a += seg;
+            left -= seg;
+        }
+    }
+    return RVX_OK;
+}
+
 int32_t engine_receivers(rvx_engine& e, const char* who) {
     const uint64_t newest = e.next_block - 1;
     const uint64_t ring = e.cfg.ring_blocks;
     const uint64_t oldest = std::max(e.first_block, newest + 1 >= ring ? newest + 1 - ring : 0);
     std::vector<rvx_vrx_span> spans;
+    std::vector<CarrierJob> jobs;
+    std::vector<rvx_vrx_carrier_item> items;
     for (auto& r : e.rx) {
         uint64_t a_lo = 0, n = 0;
         audio_range(r.plan, oldest, newest, &a_lo, &n);
@@ -254,6 +394,15 @@ int32_t engine_receivers(rvx_engine& e, const char* who) {
             e.stats.audio_lost += count - r.cap;
             r.next_audio += count - r.cap;
             count = r.cap;
+        }
+        if (r.carrier) {
+            rvx_vrx_carrier_item it{};
+            if (int32_t rc = carrier_fine(e, r, r.next_audio, count, &it, who)) return rc;
+            jobs.push_back({&r, r.next_audio, count});
+            items.push_back(it);
+            r.next_audio += count;
+            if (r.next_audio - r.read_audio > r.cap) r.read_audio = r.next_audio - r.cap;
+            continue;
         }
         const uint32_t ch = r.plan.demod.channels;
         uint64_t a = r.next_audio, left = count;
@@ -278,6 +427,7 @@ int32_t engine_receivers(rvx_engine& e, const char* who) {
         r.next_audio += count;
         if (r.next_audio - r.read_audio > r.cap) r.read_audio = r.next_audio - r.cap;
     }
+    if (int32_t rc = carrier_receivers(e, jobs, items, who)) return rc;
     if (spans.empty()) return RVX_OK;
     uint32_t nd = 0, nw = 0;
     if (int32_t rc = rvx_vrx_batch_plan(spans.data(), uint32_t(spans.size()), nullptr, 0, &nd,
@@ -467,9 +617,35 @@ int32_t rvx_engine_add_vrx(rvx_engine* e, const rvx_vrx_plan* plan, const float*
     r.plan = p;
     r.cap = audio_capacity;
     r.tile = tile;
+    r.n_weights = n_weights;
+    r.carrier = p.carrier.power != RVX_CARRIER_NONE;
+    if (r.carrier) {
+        if ((uint64_t(1) << p.carrier.nco_log2) != e->nco_entries)
+            return rvx::fail(RVX_EINVAL, "%s: carrier NCO table of 2^%u entries (the engine's "
+                             "holds %u)", who, p.carrier.nco_log2, e->nco_entries);
+        // The ring holds every fine sample one push's audio reads: (cap-1)*R
+        // + hist + 1, the history included.
+        const uint64_t span = (uint64_t(audio_capacity) - 1) * p.demod.decimation +
+                              rvx::demod_fine_history(p.demod) + 1;
+        uint64_t cap_f = 2;
+        while (cap_f < span) cap_f <<= 1;
+        if (cap_f > (uint64_t(1) << 31))
+            return rvx::fail(RVX_EINVAL, "%s: a fine ring of %llu samples", who,
+                             (unsigned long long)cap_f);
+        r.fine_cap = cap_f;
+        rvx_vrx_carrier_params probe{};  // the loop's design, checked once here
+        if (int32_t rc = rvx_carrier_block(&p, uint32_t(cap_f - 1), 0, 0, 1, 0.0f, &probe))
+            return rc;
+    }
     int32_t rc = device_alloc(&r.taps, n_fine_floats * sizeof(float), who);
     if (!rc) rc = device_alloc(&r.weights, n_weights * sizeof(float), who);
     if (!rc) rc = device_alloc(&r.audio, uint64_t(audio_capacity) * p.demod.channels * sizeof(float), who);
+    if (!rc && r.carrier) rc = device_alloc(&r.fine, r.fine_cap * 2 * sizeof(float), who);
+    if (!rc && r.carrier) rc = device_alloc(&r.cstate, RVX_CARRIER_STATE_WORDS * sizeof(uint32_t), who);
+    if (!rc && r.carrier) {
+        const hipError_t z = hipMemset(r.cstate, 0, RVX_CARRIER_STATE_WORDS * sizeof(uint32_t));
+        if (z != hipSuccess) rc = rvx::fail(RVX_EDEVICE, "%s: %s", who, hipGetErrorString(z));
+    }
     hipError_t err = hipSuccess;
     if (!rc) {
         err = hipMemcpy(r.taps, fine_taps, n_fine_floats * sizeof(float), hipMemcpyHostToDevice);
@@ -478,9 +654,7 @@ int32_t rvx_engine_add_vrx(rvx_engine* e, const rvx_vrx_plan* plan, const float*
         if (err != hipSuccess) rc = rvx::fail(RVX_EDEVICE, "%s: %s", who, hipGetErrorString(err));
     }
     if (rc) {
-        (void)hipFree(r.taps);
-        (void)hipFree(r.weights);
-        (void)hipFree(r.audio);
+        free_rx(r);
         return rc;
     }
     r.id = e->next_id++;
@@ -496,9 +670,7 @@ int32_t rvx_engine_remove_vrx(rvx_engine* e, uint32_t id) {
     auto it = std::find_if(e->rx.begin(), e->rx.end(), [&](const EngineRx& r) { return r.id == id; });
     if (it == e->rx.end()) return rvx::fail(RVX_EINVAL, "%s: no receiver %u", who, id);
     RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
-    (void)hipFree(it->taps);
-    (void)hipFree(it->weights);
-    (void)hipFree(it->audio);
+    free_rx(*it);
     e->rx.erase(it);
     return RVX_OK;
 }

@@ -116,3 +116,16 @@ def test_batch_plan_refuses_reach_below_zero():
     nd, nw = _lib.u32(), _lib.u32()
     assert _lib.lib().rvx_vrx_batch_plan(arr, 1, None, 0, C.byref(nd), None, 0,
                                          C.byref(nw)) != 0
+
+
+@pytest.mark.parametrize("mode", [12, 6], ids=["sam", "dsb"])
+def test_batch_plan_refuses_carrier_loop_receivers(mode):
+    """A SAM or DSB receiver's loop runs over its fine stream in order, so it
+    cannot be a batched tile: the planner says so instead of dropping the loop."""
+    p = plan_vrx(GRID, RATE, VrxRequest(1_000_000, mode, 6000))
+    assert p.plan.carrier.power != 0
+    arr = (_lib.VrxSpan * 1)(_span(p, 5_000_000, 1000))
+    nd, nw = _lib.u32(), _lib.u32()
+    assert _lib.lib().rvx_vrx_batch_plan(arr, 1, None, 0, C.byref(nd), None, 0,
+                                         C.byref(nw)) != 0
+    assert b"carrier loop" in _lib.lib().rvx_last_error()

@@ -6,14 +6,18 @@ pushed in different cuttings gives bit-identical audio, averaged spectrum and
 block counts. And the engine's outputs equal the oracle's composition over the
 whole stream: the chain twin over every block, the averaged-spectrum twin over
 every row, and each receiver's fine + demod twins over the audio the engine
-produced.
+produced. The SAM and DSB receivers take the fine ring and their carrier loop
+between the two stages, the loop carrying its state from push to push; the
+loop's output does not depend on where dispatches fall, so the twin runs it
+in one dispatch from the engine's first fine sample.
 """
 import numpy as np
 import pytest
 
 from helpers import assert_bit_exact, rng
 from oracle import oracle
-from revenant_amd.dsp import (DEMOD_AM, DEMOD_CW, DEMOD_NFM, DEMOD_USB, FORMAT_CS16, GridParams,
+from revenant_amd.dsp import (CARRIER_NONE, DEMOD_AM, DEMOD_CW, DEMOD_DSB, DEMOD_NFM, DEMOD_SAM,
+                              DEMOD_USB, FORMAT_CS16, GridParams,
                               VrxRequest, build_nco_table, build_twiddles, design_prototype,
                               ema_frame_decay, ema_params, plan_vrx)
 
@@ -27,7 +31,8 @@ FRAME, ROWS = 4, 2
 ALPHA = np.float32(1.0 / 16.0)
 REQUESTS = [VrxRequest(200_000, DEMOD_NFM, 12500), VrxRequest(-333_333, DEMOD_AM, 10000),
             VrxRequest(510_000, DEMOD_USB, 2700), VrxRequest(-1_000_500, DEMOD_CW, 500,
-                                                                 cw_pitch=700)]
+                                                                 cw_pitch=700),
+            VrxRequest(850_000, DEMOD_SAM, 10000), VrxRequest(-650_000, DEMOD_DSB, 6000)]
 
 
 @pytest.fixture(scope="module")
@@ -38,6 +43,13 @@ def stream():
     t = np.arange(SAMPLES)
     for f, a in ((200_000, 6000), (-333_333, 4000), (510_400, 3000), (-1_000_500, 5000)):
         z = a * np.exp(2j * np.pi * f * t / RATE)
+        codes[0::2] = np.clip(codes[0::2] + z.real, -32768, 32767).astype(np.int16)
+        codes[1::2] = np.clip(codes[1::2] + z.imag, -32768, 32767).astype(np.int16)
+    # AM on a carrier 120 Hz off the SAM receiver; DSB of a 700 Hz tone 50 Hz
+    # off the DSB receiver
+    tone = np.cos(2 * np.pi * 700.0 * t / RATE)
+    for f, env in ((850_120, 5000 * (1.0 + 0.6 * tone)), (-649_950, 6000 * tone)):
+        z = env * np.exp(2j * np.pi * f * t / RATE)
         codes[0::2] = np.clip(codes[0::2] + z.real, -32768, 32767).astype(np.int16)
         codes[1::2] = np.clip(codes[1::2] + z.imag, -32768, 32767).astype(np.int16)
     proto = design_prototype(GRID, beta=8.0).taps
@@ -122,7 +134,13 @@ def _oracle_audio(p, ring, cap, a0, n):
     out_cap = 1 << (fine_count - 1).bit_length()
     chan = np.ascontiguousarray(ring[p.plan.placement.channel]).ravel()
     fp, _ = p.fine_block(0, cap - 1, out_cap - 1, a0 * r - hist, fine_count)
-    fine = oracle.vrx_fine(p.fine, fp, chan, p.fine_taps, build_nco_table(16), out_cap)
+    nco = build_nco_table(16)
+    fine = oracle.vrx_fine(p.fine, fp, chan, p.fine_taps, nco, out_cap)
+    if p.plan.carrier.power != CARRIER_NONE:
+        state = np.zeros(8, dtype=np.uint32)
+        cp = p.carrier_block(out_cap - 1, a0 * r - hist, fine_count, reset=True)
+        oracle.vrx_carrier(p.plan.carrier, cp, nco, fine, state)
+        assert state[5] == 1, "the loop locked"
     dp = p.demod_block(out_cap - 1, a0, n)
     return oracle.vrx_demod(p.demod, dp, fine, p.weights)
 
@@ -163,7 +181,7 @@ def test_engine_late_start_and_receiver_removal(dev, stream, oracle_run):
     st = eng.stats()
     assert st.blocks == blocks_all and st.audio_lost == 0
     assert_bit_exact(eng.spectrum(), state, what="late-start spectrum")
-    for i, p in ((ids[0], plans[0]), (ids[2], plans[2]), (ids[3], plans[3])):
+    for i, p in [(ids[k], plans[k]) for k in (0, 2, 3, 4, 5)]:
         a0, got = eng.read_audio(i)
         assert a0 * p.demod.decimation * p.plan.channel_rate // p.plan.demod_rate > start // 32
         ref = _oracle_audio(p, ring, cap, a0, got.size // p.demod.channels)
