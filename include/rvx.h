@@ -606,6 +606,14 @@ typedef struct rvx_chain_desc {
     uint32_t ring_rows;
     uint32_t taps_scaled;     /* 1: cs16 input and taps from rvx_prescale_taps */
     const uint32_t* channel_rows;
+    /* Front-end correction applied to every converted sample before the
+     * branch filter, as rvx_iq_correct on the converted ring would (its ring
+     * fields are ignored): samples with absolute index >= correction_from take
+     * `correction`, earlier ones `correction_before` (NULL: `correction`).
+     * NULL `correction`: none. Not with taps_scaled. */
+    const struct rvx_iq_correct_params* correction;
+    const struct rvx_iq_correct_params* correction_before;
+    uint64_t correction_from;
 } rvx_chain_desc;
 
 /* taps * 2^-15 exactly (host arrays), for a cs16 chain with taps_scaled = 1:
@@ -617,6 +625,92 @@ int32_t rvx_prescale_taps(const float* taps, uint64_t n, float* out);
 int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const void* input,
                   const float* taps, uint64_t n_taps, const void* twiddles, uint64_t n_twiddles,
                   void* channel_ring, uint64_t ring_samples, float* partial, uint64_t n_partial);
+
+/* ---- Front-end correction ---------------------------------------------------
+ * A zero-IF front end's DC offset and I/Q imbalance, removed before the
+ * channelizer (core/dsp/front_end_correction.h). The device sums five moments
+ * per 64-sample chunk of a block; the host averages them into an estimate and
+ * turns it into the correction's four numbers; the device applies
+ * i' = i - dc_i, q' = cross * i' + scale * (q - dc_q) in place. */
+#define RVX_IQ_MOMENTS_CHUNK 64     /* front_end_correction.h:84 */
+#define RVX_IQ_MOMENTS_PER_CHUNK 5  /* :88 (I, Q, II, QQ, IQ) */
+
+/* front_end_correction.h:96-107 IqMomentsParams */
+typedef struct rvx_iq_moments_params {
+    uint32_t ring_mask;
+    uint32_t src_offset;
+    uint32_t count;
+} rvx_iq_moments_params;
+
+/* front_end_correction.h:116-130 IqCorrectParams; the identity is dc 0,
+ * cross 0, scale 1 */
+struct rvx_iq_correct_params {
+    uint32_t ring_mask;
+    uint32_t offset;
+    uint32_t count;
+    float dc_i, dc_q, cross, scale;
+};
+typedef struct rvx_iq_correct_params rvx_iq_correct_params;
+
+/* core/shaders/iq_moments.comp; twin front_end_correction_reference.cpp:46.
+ * sums: ceil(count / 64) * 5 floats (chunk-major). */
+int32_t rvx_iq_moments(rvx_ctx* ctx, const rvx_iq_moments_params* params, const void* ring,
+                       uint64_t ring_samples, float* sums, uint64_t n_sums);
+/* The same chunk sums over native codes (RVX_FORMAT_CU8 / CS16) read from
+ * sample `first` of a linear buffer, each converted as rvx_convert converts
+ * it: the moments of the ring the reference's convert would have written.
+ * No counterpart in the reference, whose moments read the converted ring. */
+int32_t rvx_iq_moments_codes(rvx_ctx* ctx, int32_t format, const void* codes,
+                             uint64_t code_samples, uint64_t first, uint32_t count, float* sums,
+                             uint64_t n_sums);
+/* core/shaders/iq_correct.comp; twin front_end_correction_reference.cpp:90.
+ * in and out may be the same ring. */
+int32_t rvx_iq_correct(rvx_ctx* ctx, const rvx_iq_correct_params* params, const void* in,
+                       uint64_t in_samples, void* out, uint64_t out_samples);
+
+/* front_end_correction.h:151-160 IqMoments */
+typedef struct rvx_iq_moments_total {
+    uint64_t samples;
+    double sum_i, sum_q, sum_ii, sum_qq, sum_iq;
+} rvx_iq_moments_total;
+/* front_end_correction.h:164-189 FrontEndCorrectionConfig */
+typedef struct rvx_front_end_config {
+    double dc_time_constant_s;   /* 0.1 */
+    double iq_time_constant_s;   /* 1.0 */
+    double max_gain_error_db;    /* 6 */
+    double max_phase_error_deg;  /* 30 */
+} rvx_front_end_config;
+/* front_end_correction.h:191-219 FrontEndEstimate */
+typedef struct rvx_front_end_estimate {
+    int32_t measured;
+    int32_t iq_plausible;
+    uint64_t samples_seen;
+    double dc_i, dc_q;
+    double gain, sin_phase;
+} rvx_front_end_estimate;
+typedef struct rvx_front_end rvx_front_end;
+
+/* front_end_correction.cpp:39 total_moments: a block's chunk sums added in
+ * chunk order, in double. */
+int32_t rvx_total_moments(const float* sums, uint64_t n_sums, uint32_t count,
+                          rvx_iq_moments_total* out);
+void rvx_front_end_default_config(rvx_front_end_config* out);
+/* FrontEndCorrector (front_end_correction.h:222-247); config NULL: defaults. */
+int32_t rvx_front_end_create(const rvx_front_end_config* config, rvx_front_end** out);
+void rvx_front_end_destroy(rvx_front_end* f);
+void rvx_front_end_reset(rvx_front_end* f);
+int32_t rvx_front_end_update(rvx_front_end* f, const rvx_iq_moments_total* moments,
+                             int64_t rate);
+int32_t rvx_front_end_estimate_get(const rvx_front_end* f, rvx_front_end_estimate* out);
+/* The correction's numbers (ring fields zero): the identity for a half that
+ * is off, not yet measured, or (I/Q) implausible. */
+int32_t rvx_front_end_correction(const rvx_front_end* f, int32_t dc, int32_t iq,
+                                 rvx_iq_correct_params* out);
+/* FrontEndEstimate's readings: dc_dbfs, gain_error_db, phase_error_deg,
+ * image_rejection_db (front_end_correction.cpp:20-37); any output may be NULL. */
+int32_t rvx_front_end_derived(const rvx_front_end_estimate* e, double* dc_dbfs,
+                              double* gain_error_db, double* phase_error_deg,
+                              double* image_rejection_db);
 
 /* ---- Plain detector ----------------------------------------------------------
  * The decision of core/detect/detector.cpp on an averaged linear spectrum:

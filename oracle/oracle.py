@@ -60,7 +60,7 @@ def lib() -> C.CDLL:
                      "oracle_fft_radix2", "oracle_spectrum", "oracle_spectrum_levels",
                      "oracle_det", "oracle_ema_partial", "oracle_ema_combine", "oracle_chain",
                      "fp64_channelize", "fp64_dft", "oracle_vrx_fine", "oracle_vrx_demod",
-                     "oracle_vrx_carrier"):
+                     "oracle_vrx_carrier", "oracle_iq_moments", "oracle_iq_correct"):
             getattr(_lib, name).restype = i32
     return _lib
 
@@ -238,6 +238,37 @@ def vrx_carrier(config, params, nco: np.ndarray, fine_ring: np.ndarray,
     _check(lib().oracle_vrx_carrier(C.byref(config), C.byref(params), _a(nc), u64(nc.size // 2),
                                     _a(fine_ring), u64(fine_ring.size // 2), _a(state),
                                     u64(state.size)), "vrx_carrier")
+
+
+class OIqMomentsParams(C.Structure):
+    _fields_ = [("ring_mask", u32), ("src_offset", u32), ("count", u32)]
+
+
+class OIqCorrectParams(C.Structure):
+    _fields_ = [("ring_mask", u32), ("offset", u32), ("count", u32), ("dc_i", C.c_float),
+                ("dc_q", C.c_float), ("cross", C.c_float), ("scale", C.c_float)]
+
+
+def iq_moments(ring_mask: int, src_offset: int, count: int, ring: np.ndarray) -> np.ndarray:
+    """reference_iq_moments: (chunks, 5) float32 chunk sums."""
+    r = f32c(ring)
+    chunks = (count + 63) // 64
+    out = np.zeros(chunks * 5, dtype=np.float32)
+    p = OIqMomentsParams(ring_mask, src_offset, count)
+    _check(lib().oracle_iq_moments(C.byref(p), _a(r), u64(r.size // 2), _a(out), u64(out.size)),
+           "iq_moments")
+    return out.reshape(chunks, 5)
+
+
+def iq_correct(ring_mask: int, offset: int, count: int, dc_i: float, dc_q: float, cross: float,
+               scale: float, ring: np.ndarray) -> np.ndarray:
+    """reference_iq_correct on a copy of the ring (float32 pairs)."""
+    r = f32c(ring)
+    out = r.copy()
+    p = OIqCorrectParams(ring_mask, offset, count, dc_i, dc_q, cross, scale)
+    _check(lib().oracle_iq_correct(C.byref(p), _a(r), u64(r.size // 2), _a(out),
+                                   u64(out.size // 2)), "iq_correct")
+    return out
 
 
 def detect(config, average: np.ndarray):

@@ -887,6 +887,75 @@ int oracle_chain(const OChain* c, const uint8_t* input, uint64_t input_bytes, co
 
 
 // ---------------------------------------------------------------------------
+// Front-end correction. core/dsp/front_end_correction_reference.cpp:46-88
+// (reference_iq_moments: five sums per 64-sample chunk, in sample order from
+// zero) and :90-111 (reference_iq_correct: i' = i - dc_i, q' = cross * i' +
+// scale * (q - dc_q) on a ring block).
+// Pinned by (tests/test_oracle_front_end.py): the moments against numpy
+// float64 sums of the same chunks within float rounding and exactly on
+// integer-valued inputs (where every partial sum is exact), the correction
+// undoing the imbalance model I_r = I, Q_r = g (Q cos phi + I sin phi) to
+// 1e-6 (test_iq_correction.cpp:166-199), and the identity parameters leaving
+// every sample's bits alone.
+// ---------------------------------------------------------------------------
+
+struct OIqMomentsParams {  // core/dsp/front_end_correction.h:96-107
+    uint32_t ring_mask, src_offset, count;
+};
+struct OIqCorrectParams {  // core/dsp/front_end_correction.h:116-130
+    uint32_t ring_mask, offset, count;
+    float dc_i, dc_q, cross, scale;
+};
+
+static bool fe_ring_ok(uint64_t ring_samples, uint32_t mask, uint32_t count) {
+    const uint64_t capacity = uint64_t(mask) + 1;
+    return pow2(capacity) && ring_samples >= capacity && count <= capacity;
+}
+
+int oracle_iq_moments(const OIqMomentsParams* p, const float* ring, uint64_t ring_samples,
+                      float* sums, uint64_t n_sums) {
+    if (!fe_ring_ok(ring_samples, p->ring_mask, p->count)) return -1;
+    const uint32_t chunks = (p->count + 63) / 64;
+    if (n_sums < uint64_t(chunks) * 5) return -1;
+    FlushGuard flush;
+    for (uint32_t c = 0; c < chunks; ++c) {
+        const uint32_t first = c * 64, last = std::min(first + 64, p->count);
+        float si = 0.0f, sq = 0.0f, sii = 0.0f, sqq = 0.0f, siq = 0.0f;
+        for (uint32_t k = first; k < last; ++k) {
+            const uint64_t slot = (p->src_offset + k) & p->ring_mask;
+            const float i = ring[2 * slot], q = ring[2 * slot + 1];
+            si = si + i;
+            sq = sq + q;
+            sii = sii + i * i;
+            sqq = sqq + q * q;
+            siq = siq + i * q;
+        }
+        float* o = sums + uint64_t(c) * 5;
+        o[0] = si;
+        o[1] = sq;
+        o[2] = sii;
+        o[3] = sqq;
+        o[4] = siq;
+    }
+    return 0;
+}
+
+int oracle_iq_correct(const OIqCorrectParams* p, const float* in, uint64_t in_samples, float* out,
+                      uint64_t out_samples) {
+    if (!fe_ring_ok(in_samples, p->ring_mask, p->count) || out_samples < in_samples) return -1;
+    FlushGuard flush;
+    for (uint32_t k = 0; k < p->count; ++k) {
+        const uint64_t slot = (p->offset + k) & p->ring_mask;
+        const float i = in[2 * slot] - p->dc_i;
+        const float q_dc = in[2 * slot + 1] - p->dc_q;
+        const float q = p->cross * i + p->scale * q_dc;
+        out[2 * slot] = i;
+        out[2 * slot + 1] = q;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // The carrier loop. core/dsp/vrx_reference.cpp:1052-1185
 // (reference_vrx_carrier) and its validate(), :986-1050: PLL (power 1) for
 // SAM, Costas (power 2) for DSB, one serial pass over the fine ring rotating

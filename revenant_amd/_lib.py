@@ -75,6 +75,30 @@ class VrxDemodParams(C.Structure):
     _fields_ = [("in_mask", u32), ("in_offset", u32), ("count", u32), ("gain", f32)]
 
 
+class IqMomentsParams(C.Structure):
+    _fields_ = [("ring_mask", u32), ("src_offset", u32), ("count", u32)]
+
+
+class IqCorrectParams(C.Structure):
+    _fields_ = [("ring_mask", u32), ("offset", u32), ("count", u32), ("dc_i", f32),
+                ("dc_q", f32), ("cross", f32), ("scale", f32)]
+
+
+class IqMomentsTotal(C.Structure):
+    _fields_ = [("samples", u64), ("sum_i", f64), ("sum_q", f64), ("sum_ii", f64),
+                ("sum_qq", f64), ("sum_iq", f64)]
+
+
+class FrontEndConfig(C.Structure):
+    _fields_ = [("dc_time_constant_s", f64), ("iq_time_constant_s", f64),
+                ("max_gain_error_db", f64), ("max_phase_error_deg", f64)]
+
+
+class FrontEndEstimate(C.Structure):
+    _fields_ = [("measured", i32), ("iq_plausible", i32), ("samples_seen", u64), ("dc_i", f64),
+                ("dc_q", f64), ("gain", f64), ("sin_phase", f64)]
+
+
 class VrxCarrierConfig(C.Structure):
     _fields_ = [("power", u32), ("nco_log2", u32)]
 
@@ -106,7 +130,8 @@ class ChainDesc(C.Structure):
                 ("decimation", u32), ("input_origin", u64), ("input_samples", u64),
                 ("first_block", u64), ("block_count", u32), ("out_ring_blocks", u32),
                 ("frame_blocks", u32), ("alpha", f32), ("ring_rows", u32), ("taps_scaled", u32),
-                ("channel_rows", vp)]
+                ("channel_rows", vp), ("correction", vp), ("correction_before", vp),
+                ("correction_from", u64)]
 
 
 class VrxRequest(C.Structure):
@@ -237,6 +262,19 @@ _SIGNATURES = {
     "rvx_vrx_carrier": (i32, [vp, C.POINTER(VrxCarrierConfig), C.POINTER(VrxCarrierParams), vp,
                               u64, vp, u64, vp]),
     "rvx_vrx_carrier_batch": (i32, [vp, vp, u32, vp, u64]),
+    "rvx_iq_moments": (i32, [vp, C.POINTER(IqMomentsParams), vp, u64, vp, u64]),
+    "rvx_iq_moments_codes": (i32, [vp, i32, vp, u64, u64, u32, vp, u64]),
+    "rvx_iq_correct": (i32, [vp, C.POINTER(IqCorrectParams), vp, u64, vp, u64]),
+    "rvx_total_moments": (i32, [fptr, u64, u32, C.POINTER(IqMomentsTotal)]),
+    "rvx_front_end_default_config": (None, [C.POINTER(FrontEndConfig)]),
+    "rvx_front_end_create": (i32, [C.POINTER(FrontEndConfig), C.POINTER(vp)]),
+    "rvx_front_end_destroy": (None, [vp]),
+    "rvx_front_end_reset": (None, [vp]),
+    "rvx_front_end_update": (i32, [vp, C.POINTER(IqMomentsTotal), i64]),
+    "rvx_front_end_estimate_get": (i32, [vp, C.POINTER(FrontEndEstimate)]),
+    "rvx_front_end_correction": (i32, [vp, i32, i32, C.POINTER(IqCorrectParams)]),
+    "rvx_front_end_derived": (i32, [C.POINTER(FrontEndEstimate), C.POINTER(f64), C.POINTER(f64),
+                                    C.POINTER(f64), C.POINTER(f64)]),
     "rvx_synth": (i32, [vp, C.POINTER(Carrier), u32, C.POINTER(SynthParams), vp, u64]),
     "rvx_chain": (i32, [vp, C.POINTER(ChainDesc), vp, vp, u64, vp, u64, vp, u64, vp, u64]),
     "rvx_vrx_batch": (i32, [vp, vp, u32, vp, u32, vp, vp, u64, u32]),

@@ -26,6 +26,7 @@
 #pragma clang fp contract(off)
 
 #include <algorithm>
+#include <cmath>
 
 #include "common.h"
 #include "fft.h"
@@ -54,6 +55,11 @@ struct ChainArgs {
     float alpha;
     const uint32_t* rows;  // ring row of channel k (null: row k); rows >= ring_rows are not stored
     uint32_t ring_rows;
+    // Front-end correction (corrected formats only): samples at window
+    // offsets below corr_from take corr[0..3], the rest corr[4..7], each
+    // {dc_i, dc_q, cross, scale}.
+    uint32_t corr_from;
+    float corr[8];
 };
 
 // Ring row of channel k, or ring_rows when the channel is not stored.
@@ -75,8 +81,15 @@ struct ChainShape {
 // real number as (code * 2^-15) * h, so it rounds to the same float, and the
 // conversion's multiply leaves the tap loop.
 constexpr int kCs16Scaled = 0x102;
+// Codes converted and then corrected for the front end's DC and I/Q
+// imbalance, exactly as rvx_iq_correct would correct the converted ring.
+constexpr int kCu8Corrected = 0x200;
+constexpr int kCs16Corrected = 0x202;
 __host__ __device__ constexpr bool is_cs16(int f) {
-    return f == RVX_FORMAT_CS16 || f == kCs16Scaled;
+    return f == RVX_FORMAT_CS16 || f == kCs16Scaled || f == kCs16Corrected;
+}
+__host__ __device__ constexpr bool is_corrected(int f) {
+    return f == kCu8Corrected || f == kCs16Corrected;
 }
 
 // The native code of one sample (cs16: one word; cu8: two bytes).
@@ -91,7 +104,11 @@ __device__ __forceinline__ uint32_t load_code(const void* input, uint32_t index)
 // convert_cs16_cf32.comp / convert_cu8_cf32.comp on one code.
 template <int FORMAT>
 __device__ __forceinline__ float2 convert_code(uint32_t w) {
-    if constexpr (FORMAT == kCs16Scaled) {
+    if constexpr (FORMAT == kCs16Corrected) {
+        return convert_code<RVX_FORMAT_CS16>(w);
+    } else if constexpr (FORMAT == kCu8Corrected) {
+        return convert_code<RVX_FORMAT_CU8>(w);
+    } else if constexpr (FORMAT == kCs16Scaled) {
         return make_float2(float(int32_t(w << 16) >> 16), float(int32_t(w) >> 16));
     } else if constexpr (FORMAT == RVX_FORMAT_CS16) {
         const float scale = __uint_as_float(0x38000000u);
@@ -131,6 +148,26 @@ __device__ __forceinline__ uint32_t buffer_code(__amdgpu_buffer_rsrc_t r, uint32
 template <int FORMAT>
 __device__ __forceinline__ float2 load_sample(const void* input, uint32_t index) {
     return convert_code<FORMAT>(load_code<FORMAT>(input, index));
+}
+
+// iq_correct.comp on one converted sample at window offset `index`
+// (front_end_correction_reference.cpp:103-107): i' = i - dc_i, then
+// q' = cross * i' + scale * (q - dc_q).
+template <int FORMAT>
+__device__ __forceinline__ float2 correct_sample(float2 x, uint32_t index, const ChainArgs& a) {
+    if constexpr (!is_corrected(FORMAT)) {
+        return x;
+    } else {
+        const bool before = index < a.corr_from;
+        const float dc_i = before ? a.corr[0] : a.corr[4];
+        const float dc_q = before ? a.corr[1] : a.corr[5];
+        const float cross = before ? a.corr[2] : a.corr[6];
+        const float scale = before ? a.corr[3] : a.corr[7];
+        const float i = x.x - dc_i;
+        const float q_dc = x.y - dc_q;
+        const float q = cross * i + scale * q_dc;
+        return make_float2(i, q);
+    }
 }
 
 // Taps whose sample loads are issued ahead of their use. The tap loop and
@@ -280,7 +317,8 @@ chain_kernel(ChainArgs a) {
                     if (q + PF < TAPS) issue(q % PF, q + PF, base);
 #pragma unroll
                     for (int i = 0; i < V; ++i) {
-                        const float2 x = convert_code<FORMAT>(cur[i]);
+                        const float2 x = correct_sample<FORMAT>(
+                            convert_code<FORMAT>(cur[i]), base - branch[i] - uint32_t(q) * M, a);
                         re[i] = re[i] + x.x * tap[i];
                         im[i] = im[i] + x.y * tap[i];
                     }
@@ -295,7 +333,8 @@ chain_kernel(ChainArgs a) {
                     float re = 0.0f, im = 0.0f;
                     const uint32_t start = base - branch[i];
                     for (uint32_t q = 0; q < a.taps_per_branch; ++q) {
-                        const float2 x = load_sample<FORMAT>(a.input, start - q * M);
+                        const float2 x = correct_sample<FORMAT>(
+                            load_sample<FORMAT>(a.input, start - q * M), start - q * M, a);
                         const float tap = a.taps[q * M + branch[i]];
                         re = re + x.x * tap;
                         im = im + x.y * tap;
@@ -457,6 +496,10 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
     const uint32_t M = d.channels, T = d.taps_per_branch, D = d.decimation;
     if (d.taps_scaled > 1 || (d.taps_scaled && d.format != RVX_FORMAT_CS16))
         return rvx::fail(RVX_EINVAL, "%s: prescaled taps belong to cs16 input", who);
+    if (d.correction && d.taps_scaled)
+        return rvx::fail(RVX_EINVAL,
+                         "%s: a front-end correction needs the converted samples, not prescaled "
+                         "taps", who);
     if (d.format != RVX_FORMAT_CU8 && d.format != RVX_FORMAT_CS16)
         return rvx::fail(RVX_EINVAL, "%s: format %d (the fused chain reads cu8 or cs16)", who,
                          d.format);
@@ -527,11 +570,30 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
     a.alpha = d.alpha;
     a.rows = d.channel_rows;
     a.ring_rows = rows;
+    if (d.correction) {
+        const rvx_iq_correct_params& now = *d.correction;
+        const rvx_iq_correct_params& before = d.correction_before ? *d.correction_before : now;
+        const float vals[8] = {before.dc_i, before.dc_q, before.cross, before.scale,
+                               now.dc_i,    now.dc_q,    now.cross,    now.scale};
+        for (int k = 0; k < 8; ++k) {
+            if (!std::isfinite(vals[k]))
+                return rvx::fail(RVX_EINVAL, "%s: a correction coefficient is not finite", who);
+            a.corr[k] = vals[k];
+        }
+        // window offset of the first sample that takes `correction`
+        const uint64_t from = d.correction_from <= d.input_origin ? 0
+                              : d.correction_from - d.input_origin;
+        a.corr_from = uint32_t(std::min<uint64_t>(from, d.input_samples));
+    }
     const uint32_t logm = rvx::log2u(M);
-    const int32_t rc = d.format == RVX_FORMAT_CU8
-                           ? launch_chain<RVX_FORMAT_CU8>(logm, a, T, ctx->stream)
-                       : d.taps_scaled ? launch_chain<kCs16Scaled>(logm, a, T, ctx->stream)
-                                       : launch_chain<RVX_FORMAT_CS16>(logm, a, T, ctx->stream);
+    int32_t rc;
+    if (d.correction)
+        rc = d.format == RVX_FORMAT_CU8 ? launch_chain<kCu8Corrected>(logm, a, T, ctx->stream)
+                                        : launch_chain<kCs16Corrected>(logm, a, T, ctx->stream);
+    else
+        rc = d.format == RVX_FORMAT_CU8 ? launch_chain<RVX_FORMAT_CU8>(logm, a, T, ctx->stream)
+             : d.taps_scaled            ? launch_chain<kCs16Scaled>(logm, a, T, ctx->stream)
+                                        : launch_chain<RVX_FORMAT_CS16>(logm, a, T, ctx->stream);
     if (rc) return rc;
     return rvx::check_launch(who);
 }

@@ -325,6 +325,8 @@ extern "C" int32_t rvx_chain_mfma(rvx_ctx* ctx, const rvx_chain_desc* desc, int3
         return rvx::fail(RVX_EINVAL, "%s: precision %d", who, precision);
     if (d.format != RVX_FORMAT_CS16)
         return rvx::fail(RVX_EINVAL, "%s: the matrix-core chain reads cs16", who);
+    if (d.correction)
+        return rvx::fail(RVX_EINVAL, "%s: no front-end correction in the matrix-core chain", who);
     if (d.channels != uint32_t(kM) || d.decimation == 0 || kM % d.decimation != 0 ||
         d.taps_per_branch == 0 || d.taps_per_branch > 64)
         return rvx::fail(RVX_EINVAL, "%s: grid M=%u L=%u D=%u (M = 4096)", who, d.channels,

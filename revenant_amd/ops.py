@@ -145,6 +145,28 @@ class Device:
                                           _p(fine_ring), _csamples(fine_ring), _p(weights),
                                           _n(weights), _p(audio), _n(audio)))
 
+    def iq_moments(self, params: _lib.IqMomentsParams, ring: torch.Tensor,
+                   sums: torch.Tensor) -> None:
+        """Five moments per 64-sample chunk of a ring block (iq_moments.comp)."""
+        _lib.check(self.lib.rvx_iq_moments(self.ctx, C.byref(params), _p(ring), _csamples(ring),
+                                           _p(sums), _n(sums)))
+
+    def iq_moments_codes(self, fmt: int, codes: torch.Tensor, first: int, count: int,
+                         sums: torch.Tensor) -> None:
+        """The same sums over native codes, converted as rvx_convert would."""
+        bps = 2 if fmt == 0 else 4
+        n = codes.numel() * codes.element_size() // bps
+        _lib.check(self.lib.rvx_iq_moments_codes(self.ctx, fmt, _p(codes), n, first, count,
+                                                 _p(sums), _n(sums)))
+
+    def iq_correct(self, params: _lib.IqCorrectParams, ring_in: torch.Tensor,
+                   ring_out: torch.Tensor | None = None) -> None:
+        """DC and I/Q correction of a ring block (iq_correct.comp); in place
+        when ring_out is None."""
+        out = ring_in if ring_out is None else ring_out
+        _lib.check(self.lib.rvx_iq_correct(self.ctx, C.byref(params), _p(ring_in),
+                                           _csamples(ring_in), _p(out), _csamples(out)))
+
     def vrx_carrier(self, config: _lib.VrxCarrierConfig, params: _lib.VrxCarrierParams,
                     nco: torch.Tensor, fine_ring: torch.Tensor, state: torch.Tensor) -> None:
         """The carrier loop over one receiver's fine ring, in place; state is
