@@ -802,9 +802,17 @@ typedef struct rvx_engine_config {
     uint32_t frame_blocks;     /* averaged spectrum frame; 0: no averaged spectrum */
     uint32_t row_frames;       /* frames per spectrum row */
     float alpha;
-    uint32_t reserved;
+    uint32_t calibration;      /* RVX_CALIBRATE_DC | RVX_CALIBRATE_IQ: front-end correction */
     uint64_t start_index;      /* absolute index of the first sample pushed */
+    int64_t sample_rate;       /* Hz; the estimator's time constants (calibration only) */
 } rvx_engine_config;
+
+/* Engine::set_calibration's two halves (core/engine/engine.h, the front-end
+ * correction stage): moments of every push are read three pushes later into
+ * the estimator, and each push's samples are corrected, inside the
+ * channelizer, with the estimate held when they arrived. */
+#define RVX_CALIBRATE_DC 1u
+#define RVX_CALIBRATE_IQ 2u
 
 typedef struct rvx_engine_stats {
     uint64_t samples_in;       /* samples pushed */
@@ -814,6 +822,11 @@ typedef struct rvx_engine_stats {
     uint64_t audio_lost;       /* receiver audio outputs skipped because their
                                 * channel samples had left the ring */
 } rvx_engine_stats;
+
+/* The front-end estimate and the correction the newest push's samples took
+ * (the identity before the first estimate or with calibration off). */
+int32_t rvx_engine_front_end(rvx_engine* engine, rvx_front_end_estimate* estimate,
+                             rvx_iq_correct_params* applied);
 
 /* prototype: M * taps_per_branch floats (host), e.g. from rvx_design_prototype. */
 int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const float* prototype,

@@ -178,8 +178,8 @@ class VrxSpan(C.Structure):
 class EngineConfig(C.Structure):
     _fields_ = [("format", i32), ("channels", u32), ("taps_per_branch", u32),
                 ("decimation", u32), ("max_push", u32), ("ring_blocks", u32),
-                ("frame_blocks", u32), ("row_frames", u32), ("alpha", f32), ("reserved", u32),
-                ("start_index", u64)]
+                ("frame_blocks", u32), ("row_frames", u32), ("alpha", f32), ("calibration", u32),
+                ("start_index", u64), ("sample_rate", i64)]
 
 
 class EngineStats(C.Structure):
@@ -291,6 +291,7 @@ _SIGNATURES = {
     "rvx_engine_spectrum": (i32, [vp, fptr, u64]),
     "rvx_engine_channel_ring": (vp, [vp]),
     "rvx_engine_stats_get": (i32, [vp, C.POINTER(EngineStats)]),
+    "rvx_engine_front_end": (i32, [vp, C.POINTER(FrontEndEstimate), C.POINTER(IqCorrectParams)]),
     "rvx_detector_create": (i32, [vp, C.POINTER(DetectConfig), C.POINTER(vp)]),
     "rvx_detector_destroy": (None, [vp]),
     "rvx_detect": (i32, [vp, vp, vp, C.POINTER(DetectBand), u32, C.POINTER(u32),

@@ -23,7 +23,14 @@
 //     (the loop runs over the fine stream in order, so it cannot sit inside a
 //     tile), then the demodulator from the ring. A receiver's loop carries its
 //     state from push to push; a gap in its fine stream (lost audio) restarts
-//     it.
+//     it;
+//   - with calibration on, the front-end correction stage: each push's
+//     moments (read from its codes) reach the estimator three pushes later,
+//     as the reference reads a frame's sums when its slot comes round again,
+//     and each push's samples take the correction the estimate gave when they
+//     arrived. The chain applies it while converting, with one switch point:
+//     a new correction waits until no block still to run reads samples from
+//     before the previous switch.
 
 #include <hip/hip_runtime.h>
 
@@ -248,6 +255,19 @@ struct rvx_engine {
     rvx_vrx_carrier_item* items_h = nullptr;  // carrier loops of one push
     void* items_d = nullptr;
     uint32_t items_cap = 0;
+    // front-end correction (calibration on)
+    static constexpr int kInFlight = 3;
+    rvx_front_end* fe = nullptr;
+    float* sums_d[kInFlight] = {};
+    float* sums_h = nullptr;  // pinned, kInFlight slots of sums_per_slot
+    uint64_t sums_per_slot = 0;
+    uint32_t sums_count[kInFlight] = {};
+    hipEvent_t sums_ready[kInFlight] = {};
+    uint64_t pushes = 0;
+    bool corr_on = false;             // a correction has been applied
+    rvx_iq_correct_params corr_before{0, 0, 0, 0.0f, 0.0f, 0.0f, 1.0f};
+    rvx_iq_correct_params corr_now{0, 0, 0, 0.0f, 0.0f, 0.0f, 1.0f};
+    uint64_t corr_from = 0;           // first sample that takes corr_now
     rvx_engine_stats stats{};
 
     ~rvx_engine() {
@@ -262,6 +282,12 @@ struct rvx_engine {
         if (work_h) (void)hipHostFree(work_h);
         if (items_h) (void)hipHostFree(items_h);
         if (staged) (void)hipEventDestroy(staged);
+        for (int k = 0; k < kInFlight; ++k) {
+            if (sums_d[k]) (void)hipFree(sums_d[k]);
+            if (sums_ready[k]) (void)hipEventDestroy(sums_ready[k]);
+        }
+        if (sums_h) (void)hipHostFree(sums_h);
+        if (fe) rvx_front_end_destroy(fe);
     }
 };
 
@@ -365,6 +391,54 @@ int32_t carrier_receivers(rvx_engine& e, const std::vector<CarrierJob>& jobs,
             left -= seg;
         }
     }
+    return RVX_OK;
+}
+
+// The front-end stage for one push: the moments that have come round feed the
+// estimator; this push's samples get the correction the estimate now gives,
+// unless the previous switch point is still read by a block to come; then
+// this push's moments are summed from its codes and copied out.
+int32_t front_end_push(rvx_engine& e, uint64_t push_first, uint64_t samples, const char* who) {
+    const int slot = int(e.pushes % rvx_engine::kInFlight);
+    if (e.sums_count[slot]) {
+        RVX_TRY_HIP(hipEventSynchronize(e.sums_ready[slot]), who);
+        rvx_iq_moments_total m{};
+        const float* h = e.sums_h + slot * e.sums_per_slot;
+        if (int32_t rc = rvx_total_moments(h, e.sums_per_slot, e.sums_count[slot], &m)) return rc;
+        if (int32_t rc = rvx_front_end_update(e.fe, &m, e.cfg.sample_rate))
+            return rc;
+        e.sums_count[slot] = 0;
+    }
+    rvx_front_end_estimate est{};
+    rvx_front_end_estimate_get(e.fe, &est);
+    if (est.measured) {
+        rvx_iq_correct_params want{};
+        rvx_front_end_correction(e.fe, (e.cfg.calibration & RVX_CALIBRATE_DC) != 0,
+                                 (e.cfg.calibration & RVX_CALIBRATE_IQ) != 0, &want);
+        const bool differs = std::memcmp(&want, &e.corr_now, sizeof(want)) != 0;
+        // the oldest sample any block still to run reads
+        const uint64_t oldest = e.next_block * e.cfg.decimation - (e.N - 1);
+        if (differs && (!e.corr_on || oldest >= e.corr_from)) {
+            e.corr_before = e.corr_now;
+            e.corr_now = want;
+            e.corr_from = push_first;
+            e.corr_on = true;
+        }
+    }
+    // this push's moments, from its codes in the window
+    const uint32_t n = uint32_t(samples);
+    if (int32_t rc = rvx_iq_moments_codes(e.ctx, e.cfg.format, e.win, e.win_fill,
+                                          push_first - e.win_origin, n, e.sums_d[slot],
+                                          e.sums_per_slot))
+        return rc;
+    const uint64_t used = (uint64_t(n) + RVX_IQ_MOMENTS_CHUNK - 1) / RVX_IQ_MOMENTS_CHUNK *
+                          RVX_IQ_MOMENTS_PER_CHUNK;
+    RVX_TRY_HIP(hipMemcpyAsync(e.sums_h + slot * e.sums_per_slot, e.sums_d[slot],
+                               used * sizeof(float), hipMemcpyDeviceToHost, e.ctx->stream),
+                who);
+    RVX_TRY_HIP(hipEventRecord(e.sums_ready[slot], e.ctx->stream), who);
+    e.sums_count[slot] = n;
+    ++e.pushes;
     return RVX_OK;
 }
 
@@ -502,6 +576,10 @@ int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const f
     const uint32_t unit = c.frame_blocks ? c.frame_blocks * std::max(1u, c.row_frames) : 1;
     if (c.frame_blocks && c.row_frames == 0)
         return rvx::fail(RVX_EINVAL, "%s: row_frames must be >= 1", who);
+    if (c.calibration & ~(RVX_CALIBRATE_DC | RVX_CALIBRATE_IQ))
+        return rvx::fail(RVX_EINVAL, "%s: calibration flags 0x%x", who, c.calibration);
+    if (c.calibration && c.sample_rate <= 0)
+        return rvx::fail(RVX_EINVAL, "%s: calibration needs the sample rate", who);
     auto e = new rvx_engine();
     e->ctx = ctx;
     e->cfg = c;
@@ -537,6 +615,20 @@ int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const f
     if (!rc) rc = device_alloc(&e->bounce, (e->N + uint64_t(e->max_blocks + 1) * D) * e->bps, who);
     if (!rc) rc = device_alloc(&e->ring, uint64_t(M) * c.ring_blocks * 8, who);
     if (!rc) rc = device_alloc(&e->counter, sizeof(uint32_t), who);
+    if (!rc && c.calibration) {
+        rc = rvx_front_end_create(nullptr, &e->fe);
+        e->sums_per_slot = (uint64_t(c.max_push) + RVX_IQ_MOMENTS_CHUNK - 1) /
+                           RVX_IQ_MOMENTS_CHUNK * RVX_IQ_MOMENTS_PER_CHUNK;
+        for (int k = 0; !rc && k < rvx_engine::kInFlight; ++k)
+            rc = device_alloc(&e->sums_d[k], e->sums_per_slot * sizeof(float), who);
+        if (!rc && hipHostMalloc((void**)&e->sums_h,
+                                 rvx_engine::kInFlight * e->sums_per_slot * sizeof(float),
+                                 hipHostMallocDefault) != hipSuccess)
+            rc = rvx::fail(RVX_ENOMEM, "%s: pinned moments buffer", who);
+        for (int k = 0; !rc && k < rvx_engine::kInFlight; ++k)
+            if (hipEventCreateWithFlags(&e->sums_ready[k], hipEventDisableTiming) != hipSuccess)
+                rc = rvx::fail(RVX_EDEVICE, "%s: event", who);
+    }
     if (!rc && frames) {
         rc = device_alloc(&e->partial, frames * M * sizeof(float), who);
         if (!rc) rc = device_alloc(&e->rows, (frames / e->cfg.row_frames) * M * sizeof(float), who);
@@ -551,7 +643,8 @@ int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const f
     // chain's products are then the same bits with one multiply fewer).
     std::vector<float> scaled(n_prototype);
     const float* upload = prototype;
-    if (c.format == RVX_FORMAT_CS16 &&
+    // (Not with calibration: the correction needs the converted samples.)
+    if (c.format == RVX_FORMAT_CS16 && !c.calibration &&
         rvx_prescale_taps(prototype, n_prototype, scaled.data()) == RVX_OK) {
         upload = scaled.data();
         e->taps_scaled = true;
@@ -718,8 +811,12 @@ int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int3
                     who);
         RVX_TRY_HIP(hipEventRecord(e->pushed, s), who);
     }
+    const uint64_t push_first = e->win_origin + e->win_fill;  // absolute index
     e->win_fill += samples;
     e->stats.samples_in += samples;
+    if (e->cfg.calibration) {
+        if (int32_t rc = front_end_push(*e, push_first, samples, who)) return rc;
+    }
 
     const uint64_t stop = e->win_origin + e->win_fill;  // one past the last sample
     if (stop == 0 || (stop - 1) / D < e->next_block) return RVX_OK;
@@ -742,6 +839,11 @@ int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int3
     d.frame_blocks = e->cfg.frame_blocks;
     d.alpha = e->cfg.alpha;
     d.taps_scaled = e->taps_scaled ? 1u : 0u;
+    if (e->corr_on) {
+        d.correction = &e->corr_now;
+        d.correction_before = &e->corr_before;
+        d.correction_from = e->corr_from;
+    }
     const uint64_t frames = e->cfg.frame_blocks ? count / e->cfg.frame_blocks : 0;
     if (int32_t rc = rvx_chain(e->ctx, &d, e->win, e->taps, e->N, e->tw, M, e->ring,
                                uint64_t(M) * e->cfg.ring_blocks, e->partial, frames * M))
@@ -817,6 +919,19 @@ int32_t rvx_engine_spectrum(rvx_engine* e, float* out, uint64_t n) {
 }
 
 const void* rvx_engine_channel_ring(rvx_engine* e) { return e ? e->ring : nullptr; }
+
+int32_t rvx_engine_front_end(rvx_engine* e, rvx_front_end_estimate* estimate,
+                             rvx_iq_correct_params* applied) {
+    static const char* who = "rvx_engine_front_end";
+    if (!e) return rvx::fail(RVX_EINVAL, "%s: null engine", who);
+    if (estimate) {
+        *estimate = rvx_front_end_estimate{};
+        estimate->gain = 1.0;
+        if (e->fe) rvx_front_end_estimate_get(e->fe, estimate);
+    }
+    if (applied) *applied = e->corr_now;
+    return RVX_OK;
+}
 
 int32_t rvx_engine_stats_get(rvx_engine* e, rvx_engine_stats* out) {
     if (!e || !out) return rvx::fail(RVX_EINVAL, "rvx_engine_stats_get: null argument");

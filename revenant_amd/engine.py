@@ -16,17 +16,23 @@ from .dsp import GridParams, VrxPlan
 from .ops import Device
 
 
+CALIBRATE_DC, CALIBRATE_IQ = 1, 2  # RVX_CALIBRATE_*
+
+
 class Engine:
     def __init__(self, dev: Device, grid: GridParams, prototype: np.ndarray, fmt: int,
                  max_push: int, ring_blocks: int, frame_blocks: int = 0, row_frames: int = 1,
-                 alpha: float = 1.0 / 16.0, start_index: int = 0):
+                 alpha: float = 1.0 / 16.0, start_index: int = 0, calibration: int = 0,
+                 sample_rate: int = 0):
+        """calibration: CALIBRATE_DC | CALIBRATE_IQ turns on the front-end
+        correction stage (sample_rate in Hz then sets its time constants)."""
         self.dev = dev
         self.lib = _lib.lib()
         self.grid = grid
         self.fmt = fmt
         cfg = _lib.EngineConfig(fmt, grid.channels, grid.taps_per_branch, grid.decimation,
                                 max_push, ring_blocks, frame_blocks, row_frames,
-                                np.float32(alpha), 0, start_index)
+                                np.float32(alpha), calibration, start_index, sample_rate)
         taps = np.ascontiguousarray(prototype, dtype=np.float32)
         h = C.c_void_p()
         _lib.check(self.lib.rvx_engine_create(dev.ctx, C.byref(cfg), taps.ctypes.data_as(
@@ -92,3 +98,9 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+    def front_end(self) -> tuple[_lib.FrontEndEstimate, _lib.IqCorrectParams]:
+        """The front-end estimate and the correction the newest push took."""
+        est, applied = _lib.FrontEndEstimate(), _lib.IqCorrectParams()
+        _lib.check(self.lib.rvx_engine_front_end(self.handle, C.byref(est), C.byref(applied)))
+        return est, applied

@@ -196,3 +196,57 @@ def test_engine_refuses_small_ring(dev, stream):
     codes, proto, plans = stream
     with pytest.raises(_lib.RvxError):
         Engine(dev, GRID, proto, FORMAT_CS16, max_push=65536, ring_blocks=1024)
+
+
+def test_engine_front_end_correction_removes_spike_and_image(dev):
+    """After test_front_end_correction.cpp:262-324 (the engine's stage on the
+    device): a cs16 stream with a -33 dBFS DC offset, a 1.05 gain ratio and a
+    3 degree phase error around a 0.1 tone 300 kHz up. With calibration on,
+    the averaged spectrum's DC channel drops by 30 dB and the image channel
+    (300 kHz down) by 25 dB against the tone, and the applied correction is
+    the model's inverse."""
+    from revenant_amd.engine import CALIBRATE_DC, CALIBRATE_IQ, Engine
+
+    n = 2 * RATE
+    t = np.arange(n, dtype=np.float64)
+    ang = 2 * np.pi * 300_000 * t / RATE
+    g = rng(0x464545494D500001)
+    phi = 3.0 * np.pi / 180.0
+    i_r = 0.1 * np.cos(ang) + 0.02 + 1e-4 * g.standard_normal(n)
+    q_r = 1.05 * (0.1 * np.sin(ang) * np.cos(phi) + 0.1 * np.cos(ang) * np.sin(phi)) + 0.01 + \
+        1e-4 * g.standard_normal(n)
+    codes = np.empty(2 * n, dtype=np.int16)
+    codes[0::2] = np.clip(np.round(i_r * 32768), -32768, 32767)
+    codes[1::2] = np.clip(np.round(q_r * 32768), -32768, 32767)
+    codes = codes.view(np.uint32)
+    proto = design_prototype(GRID, beta=8.0).taps
+
+    def run(cal):
+        eng = Engine(dev, GRID, proto, FORMAT_CS16, max_push=65536, ring_blocks=8192,
+                     frame_blocks=FRAME, row_frames=ROWS, alpha=float(ALPHA), calibration=cal,
+                     sample_rate=RATE)
+        for pos in range(0, n, 65536):
+            eng.push(codes[pos:pos + 65536])
+        spec = eng.spectrum().astype(np.float64)
+        est, applied = eng.front_end()
+        eng.close()
+        return spec, est, applied
+
+    off, est_off, applied_off = run(0)
+    on, est, applied = run(CALIBRATE_DC | CALIBRATE_IQ)
+    M = GRID.channels
+    tone, image = 300_000 * M // RATE, (-300_000 * M // RATE) % M
+    db = lambda x: 10 * np.log10(max(x, 1e-30))
+    assert not est_off.measured and (applied_off.cross, applied_off.scale) == (0.0, 1.0)
+    assert est.measured and est.iq_plausible
+    dc_drop = db(off[0]) - db(on[0])
+    irr_off = db(off[tone]) - db(off[image])
+    irr_on = db(on[tone]) - db(on[image])
+    print(f"front end: DC drop {dc_drop:.1f} dB, image rejection {irr_off:.1f} -> {irr_on:.1f} dB",
+          flush=True)
+    assert abs(irr_off - 28.9) < 0.5, irr_off
+    assert dc_drop >= 30.0, dc_drop
+    assert irr_on - irr_off >= 25.0, (irr_off, irr_on)
+    assert abs(applied.cross - (-np.tan(phi))) < 2e-3
+    assert abs(applied.scale - 1.0 / (1.05 * np.cos(phi))) < 2e-3
+    assert abs(applied.dc_i - 0.02) < 1e-3 and abs(applied.dc_q - 0.01) < 1e-3
