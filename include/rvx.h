@@ -712,6 +712,89 @@ int32_t rvx_front_end_derived(const rvx_front_end_estimate* e, double* dc_dbfs,
                               double* gain_error_db, double* phase_error_deg,
                               double* image_rejection_db);
 
+/* ---- Impulse excision --------------------------------------------------------
+ * core/dsp/impulse_excision.h: wideband impulses cut out of the sample ring
+ * before the channelizer (zeroed in place as -0, their values moved to a
+ * removed ring so cleaned + removed == raw bit for bit). Four passes per
+ * block, planned on the host by the cursor. */
+#define RVX_EXCISE_SEGMENT 0
+#define RVX_EXCISE_REFERENCE 1
+#define RVX_EXCISE_FLAG 2
+#define RVX_EXCISE_APPLY 3
+#define RVX_EXCISE_COUNTERS 4 /* samples excised, events, runs spared, (unused) */
+
+/* impulse_excision.h:155-168 ExcisionConfig */
+typedef struct rvx_excision_config {
+    uint32_t pass;
+    uint32_t seg_log2;   /* segment of 2^seg_log2 samples, 2..12 */
+    uint32_t segments;   /* reference window, 2..64 */
+    uint32_t max_width;  /* longest run excised, 1..4096 */
+    uint32_t hang, lead; /* guard after and before a flag, <= 16 */
+} rvx_excision_config;
+
+/* impulse_excision.h:185-201 ExcisionParams (eight words) */
+typedef struct rvx_excision_params {
+    uint32_t ring_mask;
+    uint32_t first;        /* ring slot of the first sample (or segment's first sample) */
+    uint32_t count;        /* samples (flag, apply) or segments (segment, reference) */
+    uint32_t seg_mask, flag_mask, removed_mask;
+    int32_t history;       /* samples of this run before `first` */
+    float threshold;       /* T as a power ratio */
+} rvx_excision_params;
+
+/* impulse_excision.h:219-223 ExcisionRings */
+typedef struct rvx_excision_rings {
+    uint32_t seg_mask, flag_mask, removed_mask;
+} rvx_excision_rings;
+
+/* impulse_excision.h:228-243 ExcisionPlan: one block's four dispatches */
+typedef struct rvx_excision_plan {
+    rvx_excision_params segment, reference, flag, apply;
+    uint64_t cleaned_end;  /* one past the last finished sample */
+    int32_t restarted;
+    uint32_t reserved;
+} rvx_excision_plan;
+
+/* impulse_excision_reference.cpp:62 design_excision (max width 20 us, guard
+ * 0.1 us (2..16 samples), segments >= a quarter of the width, 32 of them). */
+int32_t rvx_design_excision(int64_t source_rate, double max_width_seconds,
+                            rvx_excision_config* out);
+/* :92 excision_threshold: 10^(dB/10) rounded to float once. */
+float rvx_excision_threshold(double threshold_db);
+/* :96 validate(ExcisionConfig, ExcisionParams) for config->pass. */
+int32_t rvx_excision_check(const rvx_excision_config* config, const rvx_excision_params* params);
+/* :160 size_excision_rings. */
+int32_t rvx_size_excision_rings(const rvx_excision_config* config, uint64_t max_block,
+                                uint32_t frames_in_flight, uint32_t ring_mask,
+                                rvx_excision_rings* out);
+/* :190-247 ExcisionCursor: which samples each pass covers, block by block. */
+typedef struct rvx_excision_cursor rvx_excision_cursor;
+int32_t rvx_excision_cursor_create(const rvx_excision_config* config, uint32_t ring_mask,
+                                   const rvx_excision_rings* rings, float threshold,
+                                   rvx_excision_cursor** out);
+void rvx_excision_cursor_destroy(rvx_excision_cursor* cursor);
+void rvx_excision_cursor_reset(rvx_excision_cursor* cursor);
+int32_t rvx_excision_cursor_plan(rvx_excision_cursor* cursor, uint64_t start, uint32_t count,
+                                 rvx_excision_plan* out);
+/* core/shaders/impulse_excise.comp, the four passes (the config's pass field
+ * is set by each call); twins impulse_excision_reference.cpp:250-387. raw and
+ * cleaned may be the same ring. counters: RVX_EXCISE_COUNTERS words the apply
+ * pass adds to. */
+int32_t rvx_excise_segment(rvx_ctx* ctx, const rvx_excision_config* config,
+                           const rvx_excision_params* params, const void* raw,
+                           uint64_t raw_samples, float* seg_power, uint64_t n_seg);
+int32_t rvx_excise_reference(rvx_ctx* ctx, const rvx_excision_config* config,
+                             const rvx_excision_params* params, const float* seg_power,
+                             float* seg_level, uint64_t n_seg);
+int32_t rvx_excise_flag(rvx_ctx* ctx, const rvx_excision_config* config,
+                        const rvx_excision_params* params, const void* raw, uint64_t raw_samples,
+                        const float* seg_level, uint64_t n_seg, uint32_t* flags, uint64_t n_flags);
+int32_t rvx_excise_apply(rvx_ctx* ctx, const rvx_excision_config* config,
+                         const rvx_excision_params* params, const void* raw, uint64_t raw_samples,
+                         const uint32_t* flags, uint64_t n_flags, void* cleaned,
+                         uint64_t cleaned_samples, void* removed, uint64_t removed_samples,
+                         uint32_t* counters);
+
 /* ---- Plain detector ----------------------------------------------------------
  * The decision of core/detect/detector.cpp on an averaged linear spectrum:
  * estimate_noise_floor (:794-876), then find_candidates (:879-1120) through

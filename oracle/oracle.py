@@ -60,7 +60,9 @@ def lib() -> C.CDLL:
                      "oracle_fft_radix2", "oracle_spectrum", "oracle_spectrum_levels",
                      "oracle_det", "oracle_ema_partial", "oracle_ema_combine", "oracle_chain",
                      "fp64_channelize", "fp64_dft", "oracle_vrx_fine", "oracle_vrx_demod",
-                     "oracle_vrx_carrier", "oracle_iq_moments", "oracle_iq_correct"):
+                     "oracle_vrx_carrier", "oracle_iq_moments", "oracle_iq_correct",
+                     "oracle_excise_segment", "oracle_excise_reference", "oracle_excise_flag",
+                     "oracle_excise_apply"):
             getattr(_lib, name).restype = i32
     return _lib
 
@@ -269,6 +271,35 @@ def iq_correct(ring_mask: int, offset: int, count: int, dc_i: float, dc_q: float
     _check(lib().oracle_iq_correct(C.byref(p), _a(r), u64(r.size // 2), _a(out),
                                    u64(out.size // 2)), "iq_correct")
     return out
+
+
+def excise_segment(config, params, raw: np.ndarray, seg_power: np.ndarray) -> None:
+    """reference_excise_segment, into seg_power (float32, seg ring) in place.
+    The library's rvx_excision_check is the caller's to run first."""
+    _check(lib().oracle_excise_segment(C.byref(config), C.byref(params), _a(f32c(raw)),
+                                       _a(seg_power)), "excise_segment")
+
+
+def excise_reference(config, params, seg_power: np.ndarray, seg_level: np.ndarray) -> None:
+    _check(lib().oracle_excise_reference(C.byref(config), C.byref(params), _a(seg_power),
+                                         _a(seg_level)), "excise_reference")
+
+
+def excise_flag(config, params, raw: np.ndarray, seg_level: np.ndarray,
+                flags: np.ndarray) -> None:
+    _check(lib().oracle_excise_flag(C.byref(config), C.byref(params), _a(f32c(raw)),
+                                    _a(seg_level), _a(flags)), "excise_flag")
+
+
+def excise_apply(config, params, raw: np.ndarray, flags: np.ndarray, cleaned: np.ndarray,
+                 removed: np.ndarray, counters: np.ndarray) -> None:
+    """cleaned may be raw itself (in place, as the engine binds it)."""
+    for a, dt in ((raw, np.float32), (cleaned, np.float32), (removed, np.float32),
+                  (flags, np.uint32), (counters, np.uint32)):
+        if a.dtype != dt or not a.flags.c_contiguous:
+            raise ValueError("excise_apply wants contiguous float32 rings and uint32 words")
+    _check(lib().oracle_excise_apply(C.byref(config), C.byref(params), _a(raw), _a(flags),
+                                     _a(cleaned), _a(removed), _a(counters)), "excise_apply")
 
 
 def detect(config, average: np.ndarray):

@@ -956,6 +956,134 @@ int oracle_iq_correct(const OIqCorrectParams* p, const float* in, uint64_t in_sa
 }
 
 // ---------------------------------------------------------------------------
+// Impulse excision. core/dsp/impulse_excision_reference.cpp:250-387, the four
+// passes: segment powers (sample order from zero), the median reference
+// (insertion sort of the K segments ending one before, zero until a whole
+// window of this run), flags (|x|^2 * S > T * r with r > 0), and apply (a
+// flag marks [f - lead, f + hang]; a maximal marked run of at most max_width
+// is replaced by -0 and moved to the removed ring; counters of excised
+// samples, events and spared runs).
+// Pinned by (tests/test_oracle_excision.py), with the reference's measurements
+// (test_impulse_excision.cpp:344-477): forty impulses each found and removed,
+// a tone kept to 0.1 dB, cleaned + removed == raw to the bit, no sample moved,
+// a strong clean carrier left alone, and bursts longer than the maximum width
+// spared; the median against numpy's on the same windows.
+// ---------------------------------------------------------------------------
+
+struct OExcisionConfig {  // core/dsp/impulse_excision.h:155-168
+    uint32_t pass, seg_log2, segments, max_width, hang, lead;
+};
+struct OExcisionParams {  // :185-201
+    uint32_t ring_mask, first, count, seg_mask, flag_mask, removed_mask;
+    int32_t history;
+    float threshold;
+};
+static_assert(sizeof(OExcisionParams) == 32);
+
+static bool ex_flag_at(const OExcisionParams* p, const uint32_t* flags, int64_t rel) {
+    if (rel + p->history < 0) return false;
+    return flags[(p->first + uint32_t(rel)) & p->flag_mask] != 0u;
+}
+static bool ex_marked(const OExcisionConfig* c, const OExcisionParams* p, const uint32_t* flags,
+                      int64_t rel) {
+    for (uint32_t m = 0; m <= c->hang + c->lead; ++m)
+        if (ex_flag_at(p, flags, rel - int64_t(c->hang) + m)) return true;
+    return false;
+}
+
+int oracle_excise_segment(const OExcisionConfig* c, const OExcisionParams* p, const float* raw,
+                          float* seg_power) {
+    FlushGuard flush;
+    const uint32_t samples = 1u << c->seg_log2;
+    for (uint32_t j = 0; j < p->count; ++j) {
+        const uint32_t base = (p->first + (j << c->seg_log2)) & p->ring_mask;
+        float sum = 0.0f;
+        for (uint32_t m = 0; m < samples; ++m) {
+            const uint64_t k = (base + m) & p->ring_mask;
+            const float q = raw[2 * k] * raw[2 * k] + raw[2 * k + 1] * raw[2 * k + 1];
+            sum = sum + q;
+        }
+        seg_power[(base >> c->seg_log2) & p->seg_mask] = sum;
+    }
+    return 0;
+}
+
+int oracle_excise_reference(const OExcisionConfig* c, const OExcisionParams* p,
+                            const float* seg_power, float* seg_level) {
+    const int64_t segment = int64_t(1) << c->seg_log2;
+    for (uint32_t j = 0; j < p->count; ++j) {
+        const uint32_t base = (p->first + (j << c->seg_log2)) & p->ring_mask;
+        const uint32_t slot = base >> c->seg_log2;
+        const int64_t rel = int64_t(j) * segment;
+        if (rel - (int64_t(c->segments) + 1) * segment + p->history < 0) {
+            seg_level[slot & p->seg_mask] = 0.0f;
+            continue;
+        }
+        float v[64];
+        for (uint32_t m = 0; m < c->segments; ++m)
+            v[m] = seg_power[(slot - 1u - c->segments + m) & p->seg_mask];
+        for (uint32_t a = 1; a < c->segments; ++a) {
+            const float x = v[a];
+            uint32_t b = a;
+            while (b > 0 && v[b - 1] > x) {
+                v[b] = v[b - 1];
+                --b;
+            }
+            v[b] = x;
+        }
+        seg_level[slot & p->seg_mask] = v[c->segments / 2];
+    }
+    return 0;
+}
+
+int oracle_excise_flag(const OExcisionConfig* c, const OExcisionParams* p, const float* raw,
+                       const float* seg_level, uint32_t* flags) {
+    FlushGuard flush;
+    const float samples = float(1u << c->seg_log2);
+    for (uint32_t j = 0; j < p->count; ++j) {
+        const uint32_t slot = (p->first + j) & p->ring_mask;
+        const float pw = raw[2 * uint64_t(slot)] * raw[2 * uint64_t(slot)] +
+                         raw[2 * uint64_t(slot) + 1] * raw[2 * uint64_t(slot) + 1];
+        const float level = seg_level[(slot >> c->seg_log2) & p->seg_mask];
+        const float lhs = pw * samples;
+        const float rhs = p->threshold * level;
+        flags[(p->first + j) & p->flag_mask] = (level > 0.0f && lhs > rhs) ? 1u : 0u;
+    }
+    return 0;
+}
+
+int oracle_excise_apply(const OExcisionConfig* c, const OExcisionParams* p, const float* raw,
+                        const uint32_t* flags, float* cleaned, float* removed,
+                        uint32_t* counters) {
+    for (uint32_t j = 0; j < p->count; ++j) {
+        const int64_t rel = j;
+        const uint64_t slot = (p->first + j) & p->ring_mask;
+        const float xr = raw[2 * slot], xi = raw[2 * slot + 1];
+        bool excise = false;
+        if (ex_marked(c, p, flags, rel)) {
+            uint32_t left = 0, right = 0;
+            for (uint32_t k = 1; k <= c->max_width; ++k) {
+                if (!ex_marked(c, p, flags, rel - int64_t(k))) break;
+                ++left;
+            }
+            for (uint32_t k = 1; k <= c->max_width; ++k) {
+                if (!ex_marked(c, p, flags, rel + int64_t(k))) break;
+                ++right;
+            }
+            excise = left + right + 1u <= c->max_width;
+            if (left == 0u) counters[excise ? 1 : 2] += 1u;
+        }
+        if (excise) counters[0] += 1u;
+        const uint64_t r = (p->first + j) & p->removed_mask;
+        cleaned[2 * slot] = excise ? -0.0f : xr;
+        cleaned[2 * slot + 1] = excise ? -0.0f : xi;
+        removed[2 * r] = excise ? xr : -0.0f;
+        removed[2 * r + 1] = excise ? xi : -0.0f;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // The carrier loop. core/dsp/vrx_reference.cpp:1052-1185
 // (reference_vrx_carrier) and its validate(), :986-1050: PLL (power 1) for
 // SAM, Costas (power 2) for DSB, one serial pass over the fine ring rotating

@@ -99,6 +99,25 @@ class FrontEndEstimate(C.Structure):
                 ("dc_q", f64), ("gain", f64), ("sin_phase", f64)]
 
 
+class ExcisionConfig(C.Structure):
+    _fields_ = [(n, u32) for n in ("pass_", "seg_log2", "segments", "max_width", "hang", "lead")]
+
+
+class ExcisionParams(C.Structure):
+    _fields_ = [(n, u32) for n in ("ring_mask", "first", "count", "seg_mask", "flag_mask",
+                                   "removed_mask")] + [("history", i32), ("threshold", f32)]
+
+
+class ExcisionRings(C.Structure):
+    _fields_ = [("seg_mask", u32), ("flag_mask", u32), ("removed_mask", u32)]
+
+
+class ExcisionPlan(C.Structure):
+    _fields_ = [("segment", ExcisionParams), ("reference", ExcisionParams),
+                ("flag", ExcisionParams), ("apply", ExcisionParams), ("cleaned_end", u64),
+                ("restarted", i32), ("reserved", u32)]
+
+
 class VrxCarrierConfig(C.Structure):
     _fields_ = [("power", u32), ("nco_log2", u32)]
 
@@ -262,6 +281,24 @@ _SIGNATURES = {
     "rvx_vrx_carrier": (i32, [vp, C.POINTER(VrxCarrierConfig), C.POINTER(VrxCarrierParams), vp,
                               u64, vp, u64, vp]),
     "rvx_vrx_carrier_batch": (i32, [vp, vp, u32, vp, u64]),
+    "rvx_design_excision": (i32, [i64, f64, C.POINTER(ExcisionConfig)]),
+    "rvx_excision_threshold": (f32, [f64]),
+    "rvx_excision_check": (i32, [C.POINTER(ExcisionConfig), C.POINTER(ExcisionParams)]),
+    "rvx_size_excision_rings": (i32, [C.POINTER(ExcisionConfig), u64, u32, u32,
+                                      C.POINTER(ExcisionRings)]),
+    "rvx_excision_cursor_create": (i32, [C.POINTER(ExcisionConfig), u32, C.POINTER(ExcisionRings),
+                                         f32, C.POINTER(vp)]),
+    "rvx_excision_cursor_destroy": (None, [vp]),
+    "rvx_excision_cursor_reset": (None, [vp]),
+    "rvx_excision_cursor_plan": (i32, [vp, u64, u32, C.POINTER(ExcisionPlan)]),
+    "rvx_excise_segment": (i32, [vp, C.POINTER(ExcisionConfig), C.POINTER(ExcisionParams), vp, u64,
+                                 vp, u64]),
+    "rvx_excise_reference": (i32, [vp, C.POINTER(ExcisionConfig), C.POINTER(ExcisionParams), vp,
+                                   vp, u64]),
+    "rvx_excise_flag": (i32, [vp, C.POINTER(ExcisionConfig), C.POINTER(ExcisionParams), vp, u64,
+                              vp, u64, vp, u64]),
+    "rvx_excise_apply": (i32, [vp, C.POINTER(ExcisionConfig), C.POINTER(ExcisionParams), vp, u64,
+                               vp, u64, vp, u64, vp, u64, vp]),
     "rvx_iq_moments": (i32, [vp, C.POINTER(IqMomentsParams), vp, u64, vp, u64]),
     "rvx_iq_moments_codes": (i32, [vp, i32, vp, u64, u64, u32, vp, u64]),
     "rvx_iq_correct": (i32, [vp, C.POINTER(IqCorrectParams), vp, u64, vp, u64]),

@@ -167,6 +167,35 @@ class Device:
         _lib.check(self.lib.rvx_iq_correct(self.ctx, C.byref(params), _p(ring_in),
                                            _csamples(ring_in), _p(out), _csamples(out)))
 
+    # -- impulse excision (impulse_excise.comp's four passes) -------------
+    def excise_segment(self, config, params, raw: torch.Tensor, seg_power: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_excise_segment(self.ctx, C.byref(config), C.byref(params),
+                                               _p(raw), _csamples(raw), _p(seg_power),
+                                               _n(seg_power)))
+
+    def excise_reference(self, config, params, seg_power: torch.Tensor,
+                         seg_level: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_excise_reference(self.ctx, C.byref(config), C.byref(params),
+                                                 _p(seg_power), _p(seg_level),
+                                                 min(_n(seg_power), _n(seg_level))))
+
+    def excise_flag(self, config, params, raw: torch.Tensor, seg_level: torch.Tensor,
+                    flags: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_excise_flag(self.ctx, C.byref(config), C.byref(params), _p(raw),
+                                            _csamples(raw), _p(seg_level), _n(seg_level),
+                                            _p(flags), _n(flags)))
+
+    def excise_apply(self, config, params, raw: torch.Tensor, flags: torch.Tensor,
+                     cleaned: torch.Tensor, removed: torch.Tensor,
+                     counters: torch.Tensor) -> None:
+        """cleaned may be raw itself (in place)."""
+        if counters.numel() < 4 or counters.element_size() != 4:
+            raise ValueError("counters are four 32-bit words")
+        _lib.check(self.lib.rvx_excise_apply(self.ctx, C.byref(config), C.byref(params), _p(raw),
+                                             _csamples(raw), _p(flags), _n(flags), _p(cleaned),
+                                             _csamples(cleaned), _p(removed), _csamples(removed),
+                                             _p(counters)))
+
     def vrx_carrier(self, config: _lib.VrxCarrierConfig, params: _lib.VrxCarrierParams,
                     nco: torch.Tensor, fine_ring: torch.Tensor, state: torch.Tensor) -> None:
         """The carrier loop over one receiver's fine ring, in place; state is
