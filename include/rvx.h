@@ -795,6 +795,52 @@ int32_t rvx_excise_apply(rvx_ctx* ctx, const rvx_excision_config* config,
                          uint64_t cleaned_samples, void* removed, uint64_t removed_samples,
                          uint32_t* counters);
 
+/* ---- Comb cancellation (device half) ------------------------------------------
+ * core/dsp/comb_cancel.h: a harmonic comb of spurs projected onto its lines
+ * and subtracted from the sample ring before the channelizer. The lines'
+ * frequencies, phases and amplitudes come from the host's comb tracker. */
+#define RVX_COMB_MAX_LINES 1024   /* comb_cancel.h:109 */
+#define RVX_COMB_TABLE_LOG2 12    /* :117 */
+#define RVX_COMB_MAX_SUB_BLOCKS 16 /* :123 */
+
+/* comb_cancel.h:248-254 CombLine (six words) */
+typedef struct rvx_comb_line {
+    uint32_t phase0;     /* the line's 0.32 phase at the dispatch's first sample */
+    uint32_t step;       /* per sample (rvx_comb_step) */
+    uint32_t pad0, pad1;
+    float amplitude[2];  /* what subtract removes; project ignores it */
+} rvx_comb_line;
+
+/* comb_cancel.h:276-285 CombParams (eight words) */
+typedef struct rvx_comb_params {
+    uint32_t ring_mask;
+    uint32_t first;       /* ring slot of the dispatch's first sample */
+    uint32_t count;       /* a multiple of sub_blocks */
+    uint32_t lines;
+    uint32_t sub_blocks;
+    uint32_t removed_mask;
+    uint32_t pad0, pad1;
+} rvx_comb_params;
+
+/* comb_kernels_reference.cpp:44 comb_step: round(f / rate * 2^32) mod 2^32. */
+uint32_t rvx_comb_step(double frequency_hz, int64_t rate);
+/* :58 comb_table: 2^12 (cos, sin) pairs, 2 * 4096 floats. */
+int32_t rvx_comb_table(float* out, uint64_t n_floats);
+/* :67 validate(CombParams). */
+int32_t rvx_comb_check(const rvx_comb_params* params);
+/* core/shaders/comb_project.comp; twin comb_kernels_reference.cpp:100.
+ * projections: lines * sub_blocks cf32, line-major. lines, table: device. */
+int32_t rvx_comb_project(rvx_ctx* ctx, const rvx_comb_params* params, const void* raw,
+                         uint64_t raw_samples, const rvx_comb_line* lines, uint64_t n_lines,
+                         const void* table, uint64_t n_table, void* projections,
+                         uint64_t n_projections);
+/* core/shaders/comb_subtract.comp; twin :143. raw and cleaned may be the same
+ * ring; removed is indexed from the same slot under removed_mask. */
+int32_t rvx_comb_subtract(rvx_ctx* ctx, const rvx_comb_params* params, const void* raw,
+                          uint64_t raw_samples, const rvx_comb_line* lines, uint64_t n_lines,
+                          const void* table, uint64_t n_table, void* cleaned,
+                          uint64_t cleaned_samples, void* removed, uint64_t removed_samples);
+
 /* ---- Plain detector ----------------------------------------------------------
  * The decision of core/detect/detector.cpp on an averaged linear spectrum:
  * estimate_noise_floor (:794-876), then find_candidates (:879-1120) through

@@ -62,7 +62,7 @@ def lib() -> C.CDLL:
                      "fp64_channelize", "fp64_dft", "oracle_vrx_fine", "oracle_vrx_demod",
                      "oracle_vrx_carrier", "oracle_iq_moments", "oracle_iq_correct",
                      "oracle_excise_segment", "oracle_excise_reference", "oracle_excise_flag",
-                     "oracle_excise_apply"):
+                     "oracle_excise_apply", "oracle_comb_project", "oracle_comb_subtract"):
             getattr(_lib, name).restype = i32
     return _lib
 
@@ -300,6 +300,28 @@ def excise_apply(config, params, raw: np.ndarray, flags: np.ndarray, cleaned: np
             raise ValueError("excise_apply wants contiguous float32 rings and uint32 words")
     _check(lib().oracle_excise_apply(C.byref(config), C.byref(params), _a(raw), _a(flags),
                                      _a(cleaned), _a(removed), _a(counters)), "excise_apply")
+
+
+def comb_project(params, raw: np.ndarray, lines, table: np.ndarray) -> np.ndarray:
+    """reference_comb_project: lines * sub_blocks complex64, line-major.
+    lines: a ctypes array of the library's CombLine records."""
+    r, t = f32c(raw), f32c(table)
+    out = np.zeros(2 * params.lines * params.sub_blocks, dtype=np.float32)
+    _check(lib().oracle_comb_project(C.byref(params), _a(r), C.cast(lines, vp), _a(t), _a(out)),
+           "comb_project")
+    return out.view(np.complex64)
+
+
+def comb_subtract(params, raw: np.ndarray, lines, table: np.ndarray,
+                  removed_samples: int) -> tuple[np.ndarray, np.ndarray]:
+    """reference_comb_subtract on a copy of the ring: (cleaned, removed ring)."""
+    r, t = f32c(raw), f32c(table)
+    cleaned = r.copy()
+    removed = np.zeros(2 * removed_samples, dtype=np.float32)
+    _check(lib().oracle_comb_subtract(C.byref(params), _a(r), C.cast(lines, vp) if lines is not
+                                      None else None, _a(t), _a(cleaned), _a(removed)),
+           "comb_subtract")
+    return cleaned, removed
 
 
 def detect(config, average: np.ndarray):

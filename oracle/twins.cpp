@@ -1084,6 +1084,91 @@ int oracle_excise_apply(const OExcisionConfig* c, const OExcisionParams* p, cons
 }
 
 // ---------------------------------------------------------------------------
+// Comb cancellation, the device half. core/dsp/comb_kernels_reference.cpp:
+// 100-141 (project: per line and sub-block, x * conj(tone) summed in chunks of
+// 64 from zero, the chunk sums from zero in chunk order) and :143-180
+// (subtract: the lines' a * tone summed from -0 in line order, subtracted).
+// tone(n) = table[(phase0 + n * step) >> 20], 32-bit wrapping phase.
+// Pinned by (tests/test_oracle_comb.py): a pure tone at a line's step
+// projects to amplitude times the sub-block length (comb_kernels :237-268);
+// subtracting a comb with its true amplitudes leaves the table's quantisation
+// floor (:270-326); no lines copies the ring and removes -0 (:328-346); a
+// subtract across the ring's end equals the same subtract unwrapped
+// (:348-381); projections against float64 sums to float rounding.
+// ---------------------------------------------------------------------------
+
+struct OCombLine {  // core/dsp/comb_cancel.h:248-254
+    uint32_t phase0, step, pad0, pad1;
+    float amp_re, amp_im;
+};
+struct OCombParams {  // :276-285
+    uint32_t ring_mask, first, count, lines, sub_blocks, removed_mask, pad0, pad1;
+};
+static_assert(sizeof(OCombLine) == 24 && sizeof(OCombParams) == 32);
+
+int oracle_comb_project(const OCombParams* p, const float* raw, const OCombLine* lines,
+                        const float* table, float* proj) {
+    if (p->sub_blocks == 0 || p->count % p->sub_blocks) return -1;
+    FlushGuard flush;
+    const uint32_t length = p->count / p->sub_blocks;
+    for (uint32_t l = 0; l < p->lines; ++l) {
+        for (uint32_t s = 0; s < p->sub_blocks; ++s) {
+            const uint32_t begin = s * length;
+            float tr = 0.0f, ti = 0.0f;
+            for (uint32_t c = 0; c < length; c += 64) {
+                const uint32_t end = std::min(c + 64, length);
+                float cr = 0.0f, ci = 0.0f;
+                for (uint32_t m = c; m < end; ++m) {
+                    const uint32_t n = begin + m;
+                    const uint64_t k = (p->first + n) & p->ring_mask;
+                    const uint32_t t = (lines[l].phase0 + n * lines[l].step) >> 20;
+                    const float xr = raw[2 * k], xi = raw[2 * k + 1];
+                    const float c0 = table[2 * t], s0 = table[2 * t + 1];
+                    const float re = xr * c0 + xi * s0;
+                    const float im = xi * c0 - xr * s0;
+                    cr = cr + re;
+                    ci = ci + im;
+                }
+                tr = tr + cr;
+                ti = ti + ci;
+            }
+            proj[2 * (uint64_t(l) * p->sub_blocks + s)] = tr;
+            proj[2 * (uint64_t(l) * p->sub_blocks + s) + 1] = ti;
+        }
+    }
+    return 0;
+}
+
+int oracle_comb_subtract(const OCombParams* p, const float* raw, const OCombLine* lines,
+                         const float* table, float* cleaned, float* removed) {
+    FlushGuard flush;
+    for (uint32_t n = 0; n < p->count; ++n) {
+        const uint64_t slot = (p->first + n) & p->ring_mask;
+        const float xr = raw[2 * slot], xi = raw[2 * slot + 1];
+        float sr = -0.0f, si = -0.0f;
+        for (uint32_t l = 0; l < p->lines; ++l) {
+            const uint32_t t = (lines[l].phase0 + n * lines[l].step) >> 20;
+            const float c0 = table[2 * t], s0 = table[2 * t + 1];
+            const float re = lines[l].amp_re * c0 - lines[l].amp_im * s0;
+            const float im = lines[l].amp_re * s0 + lines[l].amp_im * c0;
+            sr = sr + re;
+            si = si + im;
+        }
+        if (p->lines == 0) {
+            cleaned[2 * slot] = xr;
+            cleaned[2 * slot + 1] = xi;
+        } else {
+            cleaned[2 * slot] = xr - sr;
+            cleaned[2 * slot + 1] = xi - si;
+        }
+        const uint64_t r = (p->first + n) & p->removed_mask;
+        removed[2 * r] = sr;
+        removed[2 * r + 1] = si;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // The carrier loop. core/dsp/vrx_reference.cpp:1052-1185
 // (reference_vrx_carrier) and its validate(), :986-1050: PLL (power 1) for
 // SAM, Costas (power 2) for DSB, one serial pass over the fine ring rotating

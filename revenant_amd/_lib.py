@@ -118,6 +118,16 @@ class ExcisionPlan(C.Structure):
                 ("restarted", i32), ("reserved", u32)]
 
 
+class CombLine(C.Structure):
+    _fields_ = [("phase0", u32), ("step", u32), ("pad0", u32), ("pad1", u32),
+                ("amplitude", f32 * 2)]
+
+
+class CombParams(C.Structure):
+    _fields_ = [(n, u32) for n in ("ring_mask", "first", "count", "lines", "sub_blocks",
+                                   "removed_mask", "pad0", "pad1")]
+
+
 class VrxCarrierConfig(C.Structure):
     _fields_ = [("power", u32), ("nco_log2", u32)]
 
@@ -299,6 +309,12 @@ _SIGNATURES = {
                               vp, u64, vp, u64]),
     "rvx_excise_apply": (i32, [vp, C.POINTER(ExcisionConfig), C.POINTER(ExcisionParams), vp, u64,
                                vp, u64, vp, u64, vp, u64, vp]),
+    "rvx_comb_step": (u32, [f64, i64]),
+    "rvx_comb_table": (i32, [fptr, u64]),
+    "rvx_comb_check": (i32, [C.POINTER(CombParams)]),
+    "rvx_comb_project": (i32, [vp, C.POINTER(CombParams), vp, u64, vp, u64, vp, u64, vp, u64]),
+    "rvx_comb_subtract": (i32, [vp, C.POINTER(CombParams), vp, u64, vp, u64, vp, u64, vp, u64,
+                                vp, u64]),
     "rvx_iq_moments": (i32, [vp, C.POINTER(IqMomentsParams), vp, u64, vp, u64]),
     "rvx_iq_moments_codes": (i32, [vp, i32, vp, u64, u64, u32, vp, u64]),
     "rvx_iq_correct": (i32, [vp, C.POINTER(IqCorrectParams), vp, u64, vp, u64]),

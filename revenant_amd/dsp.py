@@ -258,3 +258,24 @@ def prescale_taps(taps: np.ndarray) -> np.ndarray:
     out = np.empty_like(t)
     _lib.check(_lib.lib().rvx_prescale_taps(_fptr(t), t.size, _fptr(out)))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Comb cancellation (core/dsp/comb_cancel.h): the device half's vocabulary
+# ---------------------------------------------------------------------------
+
+def comb_step(frequency_hz: float, rate: int) -> int:
+    """comb_kernels_reference.cpp:44 comb_step."""
+    return int(_lib.lib().rvx_comb_step(frequency_hz, rate))
+
+
+def comb_table() -> np.ndarray:
+    """comb_kernels_reference.cpp:58 comb_table: 4096 complex64 (cos, sin)."""
+    out = np.zeros(2 * 4096, dtype=np.float32)
+    _lib.check(_lib.lib().rvx_comb_table(_fptr(out), out.size))
+    return out.view(np.complex64)
+
+
+def comb_check(params: _lib.CombParams) -> None:
+    _lib.check(_lib.lib().rvx_comb_check(C.byref(params)))
+

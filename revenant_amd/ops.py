@@ -167,6 +167,21 @@ class Device:
         _lib.check(self.lib.rvx_iq_correct(self.ctx, C.byref(params), _p(ring_in),
                                            _csamples(ring_in), _p(out), _csamples(out)))
 
+    # -- comb cancellation (comb_project.comp, comb_subtract.comp) --------
+    def comb_project(self, params, raw: torch.Tensor, lines: torch.Tensor, n_lines: int,
+                     table: torch.Tensor, projections: torch.Tensor) -> None:
+        """lines: a device copy of rvx_comb_line records."""
+        _lib.check(self.lib.rvx_comb_project(self.ctx, C.byref(params), _p(raw), _csamples(raw),
+                                             _p(lines), n_lines, _p(table), _csamples(table),
+                                             _p(projections), _csamples(projections)))
+
+    def comb_subtract(self, params, raw: torch.Tensor, lines: torch.Tensor, n_lines: int,
+                      table: torch.Tensor, cleaned: torch.Tensor, removed: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_comb_subtract(self.ctx, C.byref(params), _p(raw), _csamples(raw),
+                                              _p(lines), n_lines, _p(table), _csamples(table),
+                                              _p(cleaned), _csamples(cleaned), _p(removed),
+                                              _csamples(removed)))
+
     # -- impulse excision (impulse_excise.comp's four passes) -------------
     def excise_segment(self, config, params, raw: torch.Tensor, seg_power: torch.Tensor) -> None:
         _lib.check(self.lib.rvx_excise_segment(self.ctx, C.byref(config), C.byref(params),
