@@ -20,6 +20,13 @@ graph and replayed:
 `value` is input MS/s through all four. `--workload plain` times steps 1-2
 alone (the plain chain).
 
+A second leg (not in the step; `--no-conditioning` skips it) times the stages
+around the path on the same input: the front-end moments over the resident
+codes and the channelizer with the front-end correction fused in, the four
+impulse-excision passes and the comb canceller's two kernels on a 2^24-sample
+block of the cf32 ring, and the carrier loops of 256 SAM/DSB receivers over a
+second of 48 kHz audio each.
+
 Prints one JSON line. The timed region brackets exactly K steps with device
 synchronisation; each kernel's average duration is measured with HIP events
 on the stream it is launched on. The CPU baseline times the oracle (a C++
@@ -71,6 +78,8 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="default tiles, no autotuning")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-detector leg (scenes, training, recall and speed)")
+    ap.add_argument("--no-conditioning", action="store_true",
+                    help="skip the ring-conditioning and carrier-loop leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the run, copy 1 GiB device to device three times: the "
                          "known-size kernel that calibrates FETCH_SIZE / WRITE_SIZE")
@@ -342,6 +351,12 @@ def main() -> int:
                 learned["speedup_vs_cpu_twin"] = round(
                     learned["ms_plain_cpu_twin_per_block"] / learned["ms_learned_per_block"], 1)
 
+    conditioning = None
+    if not args.no_conditioning:
+        log("timing the ring-conditioning stages and the carrier loops")
+        conditioning = conditioning_leg(dev, torch, packed, main_grid, partial, kernel_ms,
+                                        args.steps)
+
     workload = ("PFB + 256 VRX: 2^27 cs16 @ 122.88 MS/s (400 carriers + AWGN) -> 4096-ch PFB "
                 "(D=2048, 16 taps/branch, Kaiser beta 8) -> channel ring + |X|^2 EMA "
                 "alpha=1/16; 512-ch PFB (D=256) for the WFM receivers; 256 VRX "
@@ -397,6 +412,7 @@ def main() -> int:
             "precision_vs_fp64": precision_from_profiles(),
         },
         "learned_detector": learned,
+        "conditioning": conditioning,
         "cpu_baseline": cpu,
         "parity": parity,
     }
@@ -409,6 +425,129 @@ def main() -> int:
         torch.cuda.synchronize()
     dev.bind_stream()
     return 0
+
+
+def conditioning_leg(dev, torch, packed, g: Grid, partial, kernel_ms, steps) -> dict:
+    """The stages either side of the path, each timed with HIP events on its
+    stream over `steps` launches: front-end moments (HBM-bound, 4 B read per
+    sample), the chain with the correction fused in against the plain chain,
+    the excision passes and the comb kernels on a 2^24-sample cf32 block, and
+    256 carrier loops (a serial recurrence per receiver: latency-bound)."""
+    import ctypes as C
+
+    from revenant_amd import _lib as L
+    from revenant_amd.dsp import build_nco_table, comb_step, comb_table
+    from revenant_amd.excision import (ExcisionCursor, design_excision, excision_threshold,
+                                       size_excision_rings)
+    from revenant_amd.front_end import moments_chunks
+
+    def timed(fn):
+        fn()
+        dev.timer_begin()
+        for _ in range(steps):
+            fn()
+        return dev.timer_end() / steps
+
+    out = {}
+    # front end: moments over the whole resident input, and the corrected chain
+    sums = torch.empty(moments_chunks(SAMPLES) * 5, dtype=torch.float32, device="cuda")
+    ms = timed(lambda: dev.iq_moments_codes(2, packed, 0, SAMPLES, sums))
+    moments_bytes = 4 * SAMPLES + 4 * sums.numel()
+    corr = L.IqCorrectParams(0, 0, 0, 0.0213, -0.0097, -0.0524078, 0.9536746)
+    before = L.IqCorrectParams(0, 0, 0, 0.0, 0.0, 0.0, 1.0)
+    cdesc = _lib_copy(g.desc)
+    cdesc.taps_scaled = 0
+    cdesc.correction = C.cast(C.pointer(corr), C.c_void_p)
+    cdesc.correction_before = C.cast(C.pointer(before), C.c_void_p)
+    cdesc.correction_from = SAMPLES // 2
+    chain_corr = timed(lambda: dev.chain(cdesc, packed, g.taps, g.tw, g.ring, partial))
+    out["front_end"] = {
+        "moments_ms": round(ms, 4),
+        "moments_gbs": round(moments_bytes / (ms * 1e-3) / 1e9, 1),
+        "moments_hbm_frac": round(moments_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "chain_m4096_corrected_ms": round(chain_corr, 4),
+        "chain_m4096_plain_ms": round(kernel_ms["chain_m4096"], 4),
+    }
+
+    # a cf32 block with noise and impulses for excision and comb
+    n_block = 1 << 24
+    ring_n = 1 << 26
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    ring = (torch.randn(2 * ring_n, generator=gen, device="cuda") * 0.01).contiguous()
+    hits = torch.randint(0, ring_n, (4096,), generator=gen, device="cuda")
+    ring.view(-1, 2)[hits] += 1.5
+    cfg = design_excision(RATE)
+    rings = size_excision_rings(cfg, n_block, 1, ring_n - 1)
+    power = torch.zeros(rings.seg_mask + 1, dtype=torch.float32, device="cuda")
+    level = torch.zeros_like(power)
+    flags = torch.zeros(rings.flag_mask + 1, dtype=torch.int32, device="cuda")
+    removed = torch.zeros(2 * (rings.removed_mask + 1), dtype=torch.float32, device="cuda")
+    counters = torch.zeros(4, dtype=torch.int32, device="cuda")
+    cursor = ExcisionCursor(cfg, ring_n - 1, rings, excision_threshold(15.0))
+    p0 = cursor.plan(0, n_block)  # fills the side rings; the second block is steady state
+    dev.excise_segment(cfg, p0.segment, ring, power)
+    dev.excise_reference(cfg, p0.reference, power, level)
+    dev.excise_flag(cfg, p0.flag, ring, level, flags)
+    plan = cursor.plan(n_block, n_block)
+    copy = ring.clone()  # the apply pass cleans in place: time it on a copy each launch
+    seg_ms = timed(lambda: dev.excise_segment(cfg, plan.segment, ring, power))
+    ref_ms = timed(lambda: dev.excise_reference(cfg, plan.reference, power, level))
+    flag_ms = timed(lambda: dev.excise_flag(cfg, plan.flag, ring, level, flags))
+    apply_ms = timed(lambda: dev.excise_apply(cfg, plan.apply, copy, flags, copy, removed,
+                                              counters))
+    cursor.close()
+    out["excision"] = {
+        "block_samples": n_block,
+        "segment_ms": round(seg_ms, 4), "reference_ms": round(ref_ms, 4),
+        "flag_ms": round(flag_ms, 4), "apply_ms": round(apply_ms, 4),
+        "msps": round(n_block / ((seg_ms + ref_ms + flag_ms + apply_ms) * 1e-3) / 1e6, 1),
+        "segment_gbs": round(8 * n_block / (seg_ms * 1e-3) / 1e9, 1),
+        "flag_gbs": round(12 * n_block / (flag_ms * 1e-3) / 1e9, 1),
+    }
+
+    # comb: 64 lines over the block, 16 sub-blocks
+    table = torch.from_numpy(comb_table().view(np.float32)).cuda()
+    lines = (L.CombLine * 64)()
+    for k in range(64):
+        lines[k].phase0 = (k * 2654435761) & 0xFFFFFFFF
+        lines[k].step = comb_step(1234.5 + (k - 32) * 45_678.25, RATE)
+        lines[k].amplitude[0], lines[k].amplitude[1] = 0.01, -0.005
+    lines_d = torch.frombuffer(bytearray(bytes(lines)), dtype=torch.uint8).cuda()
+    cp = L.CombParams(ring_n - 1, 0, n_block, 64, 16, rings.removed_mask, 0, 0)
+    proj = torch.zeros(2 * 64 * 16, dtype=torch.float32, device="cuda")
+    proj_ms = timed(lambda: dev.comb_project(cp, ring, lines_d, 64, table, proj))
+    sub_ms = timed(lambda: dev.comb_subtract(cp, ring, lines_d, 64, table, copy, removed))
+    out["comb"] = {"lines": 64, "block_samples": n_block, "project_ms": round(proj_ms, 4),
+                   "subtract_ms": round(sub_ms, 4),
+                   "subtract_msps": round(n_block / (sub_ms * 1e-3) / 1e6, 1)}
+    del ring, copy
+
+    # carrier loops: 256 receivers (SAM and DSB alternating), one second of
+    # 48 kHz audio each, one lane per receiver
+    n_rx, count, cap = 256, 48_000, 1 << 16
+    rings_c = (torch.randn(n_rx, 2 * cap, generator=gen, device="cuda") * 0.3).contiguous()
+    states = torch.zeros(n_rx, 8, dtype=torch.int32, device="cuda")
+    nco = torch.from_numpy(build_nco_table(16).view(np.float32)).cuda()
+    from revenant_amd.dsp import DEMOD_DSB, DEMOD_SAM, GridParams, VrxRequest, plan_vrx
+    plans = [plan_vrx(GridParams(64, 17, 32), 2_400_000, VrxRequest(196_500, m, bw))
+             for m, bw in ((DEMOD_SAM, 10_000), (DEMOD_DSB, 6_000))]
+    items = []
+    for k in range(n_rx):
+        pl = plans[k % 2]
+        it = L.VrxCarrierItem(pl.plan.carrier, pl.carrier_block(cap - 1, 0, count),
+                              rings_c[k].data_ptr(), cap, states[k].data_ptr(), 0)
+        L.check(L.lib().rvx_vrx_carrier_item_check(C.byref(it), 1 << 16))
+        items.append(it)
+    items_d = torch.frombuffer(bytearray(b"".join(bytes(i) for i in items)),
+                               dtype=torch.uint8).cuda()
+    car_ms = timed(lambda: dev.vrx_carrier_batch(items_d, n_rx, nco))
+    out["carrier"] = {"receivers": n_rx, "samples_per_receiver": count,
+                      "ms": round(car_ms, 4),
+                      "ns_per_sample_serial": round(car_ms * 1e6 / count, 1),
+                      "aggregate_msps": round(n_rx * count / (car_ms * 1e-3) / 1e6, 1),
+                      "realtime_factor_48k": round(1000.0 / car_ms, 1)}
+    return out
 
 
 def spot_check(torch, packed, g: Grid, partial, frames, alpha, batch, starts, counts, audio):
