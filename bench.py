@@ -355,7 +355,7 @@ def main() -> int:
     if not args.no_conditioning:
         log("timing the ring-conditioning stages and the carrier loops")
         conditioning = conditioning_leg(dev, torch, packed, main_grid, partial, kernel_ms,
-                                        args.steps)
+                                        args.steps, cpu=not args.no_cpu)
 
     workload = ("PFB + 256 VRX: 2^27 cs16 @ 122.88 MS/s (400 carriers + AWGN) -> 4096-ch PFB "
                 "(D=2048, 16 taps/branch, Kaiser beta 8) -> channel ring + |X|^2 EMA "
@@ -427,12 +427,15 @@ def main() -> int:
     return 0
 
 
-def conditioning_leg(dev, torch, packed, g: Grid, partial, kernel_ms, steps) -> dict:
+def conditioning_leg(dev, torch, packed, g: Grid, partial, kernel_ms, steps,
+                     cpu: bool) -> dict:
     """The stages either side of the path, each timed with HIP events on its
     stream over `steps` launches: front-end moments (HBM-bound, 4 B read per
     sample), the chain with the correction fused in against the plain chain,
     the excision passes and the comb kernels on a 2^24-sample cf32 block, and
-    256 carrier loops (a serial recurrence per receiver: latency-bound)."""
+    256 carrier loops (a serial recurrence per receiver: latency-bound). With
+    `cpu`, the oracle is timed beside each on a bounded sample of the same
+    input on one host thread (the CPU path each stage replaces)."""
     import ctypes as C
 
     from revenant_amd import _lib as L
@@ -463,11 +466,28 @@ def conditioning_leg(dev, torch, packed, g: Grid, partial, kernel_ms, steps) -> 
     chain_corr = timed(lambda: dev.chain(cdesc, packed, g.taps, g.tw, g.ring, partial))
     out["front_end"] = {
         "moments_ms": round(ms, 4),
-        "moments_gbs": round(moments_bytes / (ms * 1e-3) / 1e9, 1),
-        "moments_hbm_frac": round(moments_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "moments_roofline": {"bound": "hbm", "achieved_gbs": round(moments_bytes / (ms * 1e-3) / 1e9, 1),
+                             "frac": round(moments_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                             "bytes": moments_bytes},
+        "moments_gsps": round(SAMPLES / (ms * 1e-3) / 1e9, 2),
         "chain_m4096_corrected_ms": round(chain_corr, 4),
         "chain_m4096_plain_ms": round(kernel_ms["chain_m4096"], 4),
     }
+    if cpu:
+        from oracle import oracle
+        n_cpu = 1 << 22
+        words = packed[:4 * n_cpu].cpu().numpy()
+        ring_h = np.zeros(2 * n_cpu, np.float32)
+        oracle.convert(2, words, L.ConvertParams(n_cpu - 1, 0, 0, n_cpu), ring_h)
+        t0 = time.perf_counter()
+        oracle.iq_moments(n_cpu - 1, 0, n_cpu, ring_h)
+        t_m = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        oracle.iq_correct(n_cpu - 1, 0, n_cpu, 0.0213, -0.0097, -0.0524078, 0.9536746, ring_h)
+        t_c = time.perf_counter() - t0
+        out["front_end"]["cpu_oracle"] = {
+            "moments_msps": round(n_cpu / t_m / 1e6, 1), "correct_msps": round(n_cpu / t_c / 1e6, 1),
+            "cores": 1, "sample": f"{n_cpu} samples of the resident input (converted)"}
 
     # a cf32 block with noise and impulses for excision and comb
     n_block = 1 << 24
@@ -497,14 +517,52 @@ def conditioning_leg(dev, torch, packed, g: Grid, partial, kernel_ms, steps) -> 
     apply_ms = timed(lambda: dev.excise_apply(cfg, plan.apply, copy, flags, copy, removed,
                                               counters))
     cursor.close()
+    total_ms = seg_ms + ref_ms + flag_ms + apply_ms
+    # algorithmic bytes: segment reads 8 B/sample; flag reads 8 B and writes
+    # 4 B; apply reads 8 B + 4 B of flags and writes 8 + 8 B (cleaned, removed)
+    algo = 8 * n_block + 12 * n_block + 28 * n_block
     out["excision"] = {
         "block_samples": n_block,
         "segment_ms": round(seg_ms, 4), "reference_ms": round(ref_ms, 4),
         "flag_ms": round(flag_ms, 4), "apply_ms": round(apply_ms, 4),
-        "msps": round(n_block / ((seg_ms + ref_ms + flag_ms + apply_ms) * 1e-3) / 1e6, 1),
-        "segment_gbs": round(8 * n_block / (seg_ms * 1e-3) / 1e9, 1),
-        "flag_gbs": round(12 * n_block / (flag_ms * 1e-3) / 1e9, 1),
+        "msps": round(n_block / (total_ms * 1e-3) / 1e6, 1),
+        "roofline": {"bound": "hbm", "achieved_gbs": round(algo / (total_ms * 1e-3) / 1e9, 1),
+                     "frac": round(algo / (total_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "bytes": algo},
+        "counters": [int(v) for v in counters.cpu().numpy()],
     }
+    if cpu:
+        from oracle import oracle
+        n_cpu = 1 << 21
+        c_ring = copy[:2 * (1 << 22)].cpu().numpy().copy()
+        c_cfg = design_excision(RATE)
+        c_rings = size_excision_rings(c_cfg, n_cpu, 1, (1 << 22) - 1)
+        c_cur = ExcisionCursor(c_cfg, (1 << 22) - 1, c_rings, excision_threshold(15.0))
+        c_pow = np.zeros(c_rings.seg_mask + 1, np.float32)
+        c_lev = np.zeros_like(c_pow)
+        c_flags = np.zeros(c_rings.flag_mask + 1, np.uint32)
+        c_rem = np.zeros(2 * (c_rings.removed_mask + 1), np.float32)
+        c_cnt = np.zeros(4, np.uint32)
+        t_total = 0.0
+        for k in range(2):
+            pl = c_cur.plan(k * n_cpu, n_cpu)
+            t0 = time.perf_counter()
+            for which, fn, args_ in ((0, oracle.excise_segment, (pl.segment, c_ring, c_pow)),
+                                     (1, oracle.excise_reference, (pl.reference, c_pow, c_lev)),
+                                     (2, oracle.excise_flag, (pl.flag, c_ring, c_lev, c_flags))):
+                if args_[0].count:
+                    cc = L.ExcisionConfig.from_buffer_copy(bytes(c_cfg))
+                    cc.pass_ = which
+                    fn(cc, *args_)
+            if pl.apply.count:
+                cc = L.ExcisionConfig.from_buffer_copy(bytes(c_cfg))
+                cc.pass_ = 3
+                oracle.excise_apply(cc, pl.apply, c_ring, c_flags, c_ring, c_rem, c_cnt)
+            if k == 1:
+                t_total = time.perf_counter() - t0
+        c_cur.close()
+        out["excision"]["cpu_oracle"] = {"msps": round(n_cpu / t_total / 1e6, 1), "cores": 1,
+                                         "sample": f"the second of two {n_cpu}-sample blocks"}
 
     # comb: 64 lines over the block, 16 sub-blocks
     table = torch.from_numpy(comb_table().view(np.float32)).cuda()
@@ -518,9 +576,29 @@ def conditioning_leg(dev, torch, packed, g: Grid, partial, kernel_ms, steps) -> 
     proj = torch.zeros(2 * 64 * 16, dtype=torch.float32, device="cuda")
     proj_ms = timed(lambda: dev.comb_project(cp, ring, lines_d, 64, table, proj))
     sub_ms = timed(lambda: dev.comb_subtract(cp, ring, lines_d, 64, table, copy, removed))
+    sub_bytes = 8 * n_block * 3  # read x, write cleaned and removed
     out["comb"] = {"lines": 64, "block_samples": n_block, "project_ms": round(proj_ms, 4),
                    "subtract_ms": round(sub_ms, 4),
-                   "subtract_msps": round(n_block / (sub_ms * 1e-3) / 1e6, 1)}
+                   "subtract_msps": round(n_block / (sub_ms * 1e-3) / 1e6, 1),
+                   "subtract_roofline": {
+                       "bound": "hbm", "achieved_gbs": round(sub_bytes / (sub_ms * 1e-3) / 1e9, 1),
+                       "frac": round(sub_bytes / (sub_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "bytes": sub_bytes}}
+    if cpu:
+        from oracle import oracle
+        n_cpu = 1 << 18
+        x = ring[:2 * n_cpu].cpu().numpy().copy()
+        tab = comb_table()
+        cpp = L.CombParams(n_cpu - 1, 0, n_cpu, 64, 16, n_cpu - 1, 0, 0)
+        t0 = time.perf_counter()
+        oracle.comb_subtract(cpp, x, lines, tab, n_cpu)
+        t_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        oracle.comb_project(cpp, x, lines, tab)
+        t_p = time.perf_counter() - t0
+        out["comb"]["cpu_oracle"] = {"subtract_msps": round(n_cpu / t_s / 1e6, 1),
+                                     "project_msps": round(n_cpu / t_p / 1e6, 1), "cores": 1,
+                                     "sample": f"{n_cpu} samples, 64 lines"}
     del ring, copy
 
     # carrier loops: 256 receivers (SAM and DSB alternating), one second of
@@ -546,7 +624,24 @@ def conditioning_leg(dev, torch, packed, g: Grid, partial, kernel_ms, steps) -> 
                       "ms": round(car_ms, 4),
                       "ns_per_sample_serial": round(car_ms * 1e6 / count, 1),
                       "aggregate_msps": round(n_rx * count / (car_ms * 1e-3) / 1e6, 1),
-                      "realtime_factor_48k": round(1000.0 / car_ms, 1)}
+                      "realtime_factor_48k": round(1000.0 / car_ms, 1),
+                      "bound": "latency (one serial recurrence per receiver)"}
+    if cpu:
+        from oracle import oracle
+        n_cpu_rx = 8
+        host = rings_c[:n_cpu_rx].cpu().numpy().copy()
+        st = np.zeros((n_cpu_rx, 8), np.uint32)
+        nco_h = build_nco_table(16)
+        t0 = time.perf_counter()
+        for k in range(n_cpu_rx):
+            pl = plans[k % 2]
+            oracle.vrx_carrier(pl.plan.carrier, pl.carrier_block(cap - 1, 0, count), nco_h,
+                               host[k], st[k])
+        t_c = time.perf_counter() - t0
+        out["carrier"]["cpu_oracle"] = {
+            "ns_per_sample_serial": round(t_c * 1e9 / (n_cpu_rx * count), 1),
+            "msps_per_core": round(n_cpu_rx * count / t_c / 1e6, 1), "cores": 1,
+            "sample": f"{n_cpu_rx} receivers x {count} samples"}
     return out
 
 

@@ -8,10 +8,11 @@
 //
 // The moments are chunk sums in sample order from zero (64 samples a chunk,
 // five sums: I, Q, II, QQ, IQ), one rounding per operation, so the order is
-// fixed by the format and not by the launch. A wave stages its 64 chunks in
-// LDS with coalesced loads (each chunk row padded by one sample so the
-// per-lane reads down a row fall in different banks), then each lane sums its
-// own chunk from LDS. Every operation is a single IEEE add or multiply; no
+// fixed by the format and not by the launch. On a ring, a wave stages its 64
+// chunks in LDS with coalesced loads (each chunk row padded by one sample so
+// the per-lane reads down a row fall in different banks), then each lane sums
+// its own chunk from LDS; on native codes each lane loads its chunk straight
+// into registers. Every operation is a single IEEE add or multiply; no
 // contraction, denormals flushed (the build's flags).
 
 #pragma clang fp contract(off)
@@ -88,6 +89,70 @@ iq_moments_kernel(Source src, uint32_t count, float* __restrict__ sums) {
     out[4] = siq;
 }
 
+// The native-code form reads each lane's chunk straight into registers:
+// one lane per chunk, its 64 codes loaded as sixteen 16-byte vectors (cs16)
+// or eight (cu8) from a linear buffer, no LDS, so occupancy is set by
+// registers alone and a wave keeps a whole 16 KiB span of loads in flight.
+template <int FORMAT>
+__global__ void __launch_bounds__(256)
+iq_moments_codes_kernel(const void* __restrict__ codes, uint64_t first, uint32_t count,
+                        float* __restrict__ sums) {
+    const uint32_t chunk = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c_first = chunk * kChunk;
+    if (c_first >= count) return;
+    const uint32_t len = min(kChunk, count - c_first);
+    CodeSource<FORMAT> src{codes, first};
+    float si = 0.0f, sq = 0.0f, sii = 0.0f, sqq = 0.0f, siq = 0.0f;
+    auto add = [&](float2 v) {
+        const float i = v.x, q = v.y;
+        si = si + i;
+        sq = sq + q;
+        sii = sii + i * i;
+        sqq = sqq + q * q;
+        siq = siq + i * q;
+    };
+    constexpr uint32_t kBytes = FORMAT == RVX_FORMAT_CS16 ? 4u : 2u;
+    const uint64_t s0 = first + c_first;
+    const uint64_t at = reinterpret_cast<uintptr_t>(codes) + s0 * kBytes;
+    if (len == kChunk && (at & 15u) == 0) {
+        // whole, 16-byte-aligned chunk: vector loads, then the sums in order
+        const uint4* p = reinterpret_cast<const uint4*>(static_cast<const uint8_t*>(codes) +
+                                                        s0 * kBytes);
+        constexpr uint32_t kVec = kChunk * kBytes / 16;
+        uint4 w[kVec];
+#pragma unroll
+        for (uint32_t k = 0; k < kVec; ++k) w[k] = p[k];
+#pragma unroll
+        for (uint32_t k = 0; k < kVec; ++k) {
+            const uint32_t words[4] = {w[k].x, w[k].y, w[k].z, w[k].w};
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+                if constexpr (FORMAT == RVX_FORMAT_CS16) {
+                    const float scale = __uint_as_float(0x38000000u);  // 2^-15
+                    add(make_float2(float(int32_t(words[j] << 16) >> 16) * scale,
+                                    float(int32_t(words[j]) >> 16) * scale));
+                } else {
+                    const float bias = __uint_as_float(0x42FF0000u);   // 127.5
+                    const float scale = __uint_as_float(0x3C008081u);  // 1/127.5
+                    for (uint32_t h = 0; h < 2; ++h) {
+                        const uint32_t c = words[j] >> (16 * h);
+                        add(make_float2((float(c & 0xFFu) - bias) * scale,
+                                        (float((c >> 8) & 0xFFu) - bias) * scale));
+                    }
+                }
+            }
+        }
+    } else {
+        for (uint32_t k = 0; k < len; ++k) add(src(c_first + k));
+    }
+    float* out = sums + uint64_t(chunk) * kPerChunk;
+    out[0] = si;
+    out[1] = sq;
+    out[2] = sii;
+    out[3] = sqq;
+    out[4] = siq;
+}
+
 __global__ void iq_correct_kernel(const float2* in, rvx_iq_correct_params p, float2* out) {
     const uint32_t index = blockIdx.x * blockDim.x + threadIdx.x;
     if (index >= p.count) return;
@@ -153,13 +218,13 @@ int32_t rvx_iq_moments_codes(rvx_ctx* ctx, int32_t format, const void* codes,
                          chunks * kPerChunk);
     if (chunks == 0) return RVX_OK;
     if (!codes || !sums) return rvx::fail(RVX_EINVAL, "%s: null buffer", who);
-    const dim3 grid((chunks + kLanes - 1) / kLanes), block(kLanes);
+    const dim3 grid((chunks + 255) / 256), block(256);
     if (format == RVX_FORMAT_CS16)
-        iq_moments_kernel<<<grid, block, 0, ctx->stream>>>(
-            CodeSource<RVX_FORMAT_CS16>{codes, first}, count, sums);
+        iq_moments_codes_kernel<RVX_FORMAT_CS16><<<grid, block, 0, ctx->stream>>>(codes, first,
+                                                                                  count, sums);
     else
-        iq_moments_kernel<<<grid, block, 0, ctx->stream>>>(
-            CodeSource<RVX_FORMAT_CU8>{codes, first}, count, sums);
+        iq_moments_codes_kernel<RVX_FORMAT_CU8><<<grid, block, 0, ctx->stream>>>(codes, first,
+                                                                                 count, sums);
     return rvx::check_launch(who);
 }
 

@@ -303,7 +303,7 @@ struct CarrierJob {
 // The fine stage of one carrier receiver over its new fine samples, in
 // dispatches the resampler recurrence allows, and the loop's item.
 int32_t carrier_fine(rvx_engine& e, EngineRx& r, uint64_t a0, uint64_t count,
-                     rvx_vrx_carrier_item* item, const char* who) {
+                     rvx_vrx_carrier_item* item) {
     const uint64_t R = r.plan.demod.decimation, hist = rvx::demod_fine_history(r.plan.demod);
     const uint64_t need = a0 * R - hist;  // audio_range keeps a0*R >= hist
     const uint64_t last = (a0 + count - 1) * R;
@@ -341,7 +341,6 @@ int32_t carrier_fine(rvx_engine& e, EngineRx& r, uint64_t a0, uint64_t count,
     if (int32_t rc = rvx_vrx_carrier_item_check(&it, e.nco_entries)) return rc;
     *item = it;
     r.next_fine = last + 1;
-    (void)who;
     return RVX_OK;
 }
 
@@ -471,7 +470,7 @@ int32_t engine_receivers(rvx_engine& e, const char* who) {
         }
         if (r.carrier) {
             rvx_vrx_carrier_item it{};
-            if (int32_t rc = carrier_fine(e, r, r.next_audio, count, &it, who)) return rc;
+            if (int32_t rc = carrier_fine(e, r, r.next_audio, count, &it)) return rc;
             jobs.push_back({&r, r.next_audio, count});
             items.push_back(it);
             r.next_audio += count;

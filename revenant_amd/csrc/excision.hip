@@ -115,17 +115,32 @@ __global__ void flag_kernel(rvx_excision_config c, rvx_excision_params p,
     flags[(p.first + j) & p.flag_mask] = (level > 0.0f && lhs > rhs) ? 1u : 0u;
 }
 
+constexpr uint32_t kThreads = 256;
+
 // raw and cleaned may be the same ring: a lane reads and writes only its slot.
-__global__ void apply_kernel(rvx_excision_config c, rvx_excision_params p, const float2* raw,
-                             const uint32_t* __restrict__ flags, float2* cleaned,
-                             float2* __restrict__ removed, uint32_t* __restrict__ counters) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+// The workgroup first stages the flags its samples' marks read ([j0 - hang,
+// j0 + 256 + lead), each already tested against the restart as flag_at
+// tests it) in LDS, so the common case, an unmarked sample, reads LDS only;
+// the run scans of a marked sample read the flag ring.
+__global__ void __launch_bounds__(256)
+apply_kernel(rvx_excision_config c, rvx_excision_params p, const float2* raw,
+             const uint32_t* __restrict__ flags, float2* cleaned,
+             float2* __restrict__ removed, uint32_t* __restrict__ counters) {
+    __shared__ uint32_t tile[256 + 2 * kMaxGuard + 1];
+    const uint32_t j0 = blockIdx.x * 256u;
+    const uint32_t span = 256u + c.hang + c.lead;
+    for (uint32_t k = threadIdx.x; k < span; k += 256u)
+        tile[k] = flag_at(p, flags, int64_t(j0) - int64_t(c.hang) + int64_t(k)) ? 1u : 0u;
+    __syncthreads();
+    const uint32_t j = j0 + threadIdx.x;
     if (j >= p.count) return;
     const int64_t rel = j;
     const uint32_t slot = (p.first + j) & p.ring_mask;
     const float2 x = raw[slot];
+    bool here = false;
+    for (uint32_t m = 0; m <= c.hang + c.lead; ++m) here = here || tile[threadIdx.x + m] != 0u;
     bool excise = false;
-    if (marked(c, p, flags, rel)) {
+    if (here) {
         uint32_t left = 0, right = 0;
         for (uint32_t k = 1; k <= c.max_width; ++k) {
             if (!marked(c, p, flags, rel - int64_t(k))) break;
@@ -144,7 +159,6 @@ __global__ void apply_kernel(rvx_excision_config c, rvx_excision_params p, const
     removed[(p.first + j) & p.removed_mask] = excise ? x : nothing;
 }
 
-constexpr uint32_t kThreads = 256;
 dim3 grid_for(uint32_t n) { return dim3((n + kThreads - 1) / kThreads); }
 
 }  // namespace
