@@ -10,9 +10,10 @@
 // x[n] * conj(tone_l(n)), in chunks of 64 samples summed from zero in sample
 // order, the chunk sums added from zero in chunk order (the kernel's two
 // levels). tone_l(n) = table[(phase0_l + n * step_l) >> 20] with 32-bit
-// wrapping phase. A workgroup takes one (line, sub-block): it stages 64
-// chunks of samples at a time in LDS with coalesced loads (rows padded by one
-// sample), each lane sums its chunk, and lane 0 folds the chunk sums in order.
+// wrapping phase. A workgroup takes one sub-block and up to 16 lines, a wave
+// per line: it stages 64 chunks of samples at a time in LDS with coalesced
+// loads (rows padded by one sample), each lane sums its chunk for its wave's
+// line, and the wave's lane 0 folds the chunk sums in order.
 //
 // subtract: per sample, removed = sum over lines ascending of a_l * tone_l(n)
 // (from -0, which adds to anything unchanged), cleaned = x - removed; with no
@@ -46,32 +47,38 @@ __device__ __forceinline__ float2 tone(const float2* __restrict__ table, const r
     return table[(l.phase0 + n * l.step) >> kTableShift];
 }
 
-__global__ void __launch_bounds__(kLanes)
-project_kernel(rvx_comb_params p, const float2* __restrict__ raw,
+// One workgroup per (sub-block, group of `group` lines): a wave per line. The
+// waves share each staged tile of 64 chunks, so the block's samples are read
+// from memory once per group rather than once per line.
+__global__ void __launch_bounds__(1024)
+project_kernel(rvx_comb_params p, uint32_t group, const float2* __restrict__ raw,
                const rvx_comb_line* __restrict__ lines, const float2* __restrict__ table,
                float2* __restrict__ proj) {
     __shared__ float2 stage[kLanes * kRow];
-    __shared__ float2 sums[kLanes];
-    const uint32_t l = blockIdx.x / p.sub_blocks;
-    const uint32_t s = blockIdx.x % p.sub_blocks;
-    const rvx_comb_line line = lines[l];
+    __shared__ float2 sums[16][kLanes];
+    const uint32_t groups = (p.lines + group - 1) / group;
+    const uint32_t s = blockIdx.x / groups;
+    const uint32_t wave = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
+    const uint32_t l = (blockIdx.x % groups) * group + wave;
+    const bool live = l < p.lines;
+    const rvx_comb_line line = lines[live ? l : 0];
     const uint32_t length = p.count / p.sub_blocks;
     const uint32_t begin = s * length;
     const uint32_t chunks = (length + kChunk - 1) / kChunk;
-    float total_re = 0.0f, total_im = 0.0f;  // lane 0's running fold
+    float total_re = 0.0f, total_im = 0.0f;  // lane 0's running fold for this wave's line
     for (uint32_t c0 = 0; c0 < chunks; c0 += kLanes) {
         const uint32_t first = c0 * kChunk;  // offset of this tile in the sub-block
         const uint32_t n_tile = min(kLanes * kChunk, length - first);
-        for (uint32_t k = threadIdx.x; k < n_tile; k += kLanes)
+        for (uint32_t k = threadIdx.x; k < n_tile; k += blockDim.x)
             stage[(k / kChunk) * kRow + (k % kChunk)] =
                 raw[(p.first + begin + first + k) & p.ring_mask];
         __syncthreads();
-        const uint32_t c = c0 + threadIdx.x;
+        const uint32_t c = c0 + lane;
         float chunk_re = 0.0f, chunk_im = 0.0f;
-        if (c < chunks) {
+        if (live && c < chunks) {
             const uint32_t m0 = c * kChunk;
             const uint32_t m1 = min(m0 + kChunk, length);
-            const float2* row = stage + threadIdx.x * kRow;
+            const float2* row = stage + lane * kRow;
             for (uint32_t m = m0; m < m1; ++m) {
                 const float2 x = row[m - m0];
                 const float2 t = tone(table, line, begin + m);
@@ -81,18 +88,18 @@ project_kernel(rvx_comb_params p, const float2* __restrict__ raw,
                 chunk_im = chunk_im + im;
             }
         }
-        sums[threadIdx.x] = make_float2(chunk_re, chunk_im);
+        sums[wave][lane] = make_float2(chunk_re, chunk_im);
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (lane == 0 && live) {
             const uint32_t here = min(kLanes, chunks - c0);
             for (uint32_t k = 0; k < here; ++k) {
-                total_re = total_re + sums[k].x;
-                total_im = total_im + sums[k].y;
+                total_re = total_re + sums[wave][k].x;
+                total_im = total_im + sums[wave][k].y;
             }
         }
         __syncthreads();  // stage and sums are rewritten by the next tile
     }
-    if (threadIdx.x == 0) proj[uint64_t(l) * p.sub_blocks + s] = make_float2(total_re, total_im);
+    if (lane == 0 && live) proj[uint64_t(l) * p.sub_blocks + s] = make_float2(total_re, total_im);
 }
 
 // raw and cleaned may be the same ring: a lane reads and writes only its slot.
@@ -181,8 +188,12 @@ int32_t rvx_comb_project(rvx_ctx* ctx, const rvx_comb_params* params, const void
     if (p.lines == 0) return RVX_OK;
     if (!raw || !lines || !table || !projections)
         return rvx::fail(RVX_EINVAL, "%s: null buffer", who);
-    project_kernel<<<dim3(p.lines * p.sub_blocks), dim3(kLanes), 0, ctx->stream>>>(
-        p, static_cast<const float2*>(raw), lines, static_cast<const float2*>(table),
+    // lines per workgroup: as many (up to 16) as keep 256+ workgroups in flight
+    uint32_t group = 16;
+    while (group > 1 && uint64_t((p.lines + group - 1) / group) * p.sub_blocks < 256) group /= 2;
+    const uint32_t groups = (p.lines + group - 1) / group;
+    project_kernel<<<dim3(groups * p.sub_blocks), dim3(group * kLanes), 0, ctx->stream>>>(
+        p, group, static_cast<const float2*>(raw), lines, static_cast<const float2*>(table),
         static_cast<float2*>(projections));
     return rvx::check_launch(who);
 }
