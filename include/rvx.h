@@ -841,6 +841,47 @@ int32_t rvx_comb_subtract(rvx_ctx* ctx, const rvx_comb_params* params, const voi
                           const void* table, uint64_t n_table, void* cleaned,
                           uint64_t cleaned_samples, void* removed, uint64_t removed_samples);
 
+/* ---- Per-receiver impulse blanker --------------------------------------------
+ * core/dsp/noise_reference.h:59-190: impulses blanked on a receiver's coarse
+ * channel at the channel rate, before its fine stage narrows them into
+ * ringing bursts. The fine stage then reads the blanked ring (chan_base 0,
+ * out_mask as its channel mask). */
+#define RVX_BLANK_DETECT 0
+#define RVX_BLANK_APPLY 1
+
+/* noise_reference.h:131-140 BlankerConfig */
+typedef struct rvx_blanker_config {
+    uint32_t pass;
+    uint32_t window;     /* reference window, 16..256 samples */
+    uint32_t guard;      /* samples between the window and the tested sample */
+    uint32_t hang, lead; /* blanked after the last / before the first flag */
+} rvx_blanker_config;
+
+/* noise_reference.h:160-175 BlankerParams (seven words) */
+typedef struct rvx_blanker_params {
+    uint32_t chan_base, chan_mask, chan_first; /* the channel ring, as vrx_fine reads it */
+    uint32_t out_mask, out_first;              /* the receiver's flag / blanked rings */
+    uint32_t count;
+    float threshold;                           /* T as a power ratio */
+} rvx_blanker_params;
+
+/* noise_reference.cpp:112 design_blanker: 0.5 ms window clamped to [16, 256],
+ * hang = lead = 2, guard = lead + 2. */
+int32_t rvx_design_blanker(int64_t channel_rate, rvx_blanker_config* out);
+/* :123 blanker_threshold: 10^(dB/10) rounded to float once. */
+float rvx_blanker_threshold(double threshold_db);
+/* :127 validate(BlankerConfig, BlankerParams). */
+int32_t rvx_blanker_check(const rvx_blanker_config* config, const rvx_blanker_params* params);
+/* core/shaders/noise_blank.comp; twins noise_reference.cpp:158 (detect: flags,
+ * out_mask + 1 floats) and :197 (apply: blanked, out_mask + 1 cf32). */
+int32_t rvx_blank_detect(rvx_ctx* ctx, const rvx_blanker_config* config,
+                         const rvx_blanker_params* params, const void* channel_ring,
+                         uint64_t ring_samples, float* flags, uint64_t n_flags);
+int32_t rvx_blank_apply(rvx_ctx* ctx, const rvx_blanker_config* config,
+                        const rvx_blanker_params* params, const void* channel_ring,
+                        uint64_t ring_samples, const float* flags, uint64_t n_flags,
+                        void* blanked, uint64_t blanked_samples);
+
 /* ---- Plain detector ----------------------------------------------------------
  * The decision of core/detect/detector.cpp on an averaged linear spectrum:
  * estimate_noise_floor (:794-876), then find_candidates (:879-1120) through

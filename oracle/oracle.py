@@ -62,7 +62,8 @@ def lib() -> C.CDLL:
                      "fp64_channelize", "fp64_dft", "oracle_vrx_fine", "oracle_vrx_demod",
                      "oracle_vrx_carrier", "oracle_iq_moments", "oracle_iq_correct",
                      "oracle_excise_segment", "oracle_excise_reference", "oracle_excise_flag",
-                     "oracle_excise_apply", "oracle_comb_project", "oracle_comb_subtract"):
+                     "oracle_excise_apply", "oracle_comb_project", "oracle_comb_subtract",
+                     "oracle_blank_detect", "oracle_blank_apply"):
             getattr(_lib, name).restype = i32
     return _lib
 
@@ -322,6 +323,24 @@ def comb_subtract(params, raw: np.ndarray, lines, table: np.ndarray,
                                       None else None, _a(t), _a(cleaned), _a(removed)),
            "comb_subtract")
     return cleaned, removed
+
+
+def blank_detect(config, params, channel: np.ndarray, flags: np.ndarray) -> None:
+    """reference_blank_detect into flags (float32, out_mask + 1) in place. The
+    library's rvx_blanker_check is the caller's to run first."""
+    if flags.dtype != np.float32 or not flags.flags.c_contiguous:
+        raise ValueError("flags are contiguous float32")
+    _check(lib().oracle_blank_detect(C.byref(config), C.byref(params), _a(f32c(channel)),
+                                     _a(flags)), "blank_detect")
+
+
+def blank_apply(config, params, channel: np.ndarray, flags: np.ndarray,
+                blanked: np.ndarray) -> None:
+    if blanked.dtype != np.float32 or not blanked.flags.c_contiguous:
+        raise ValueError("blanked is contiguous float32 pairs")
+    _check(lib().oracle_blank_apply(C.byref(config), C.byref(params), _a(f32c(channel)),
+                                    _a(np.ascontiguousarray(flags, dtype=np.float32)),
+                                    _a(blanked)), "blank_apply")
 
 
 def detect(config, average: np.ndarray):

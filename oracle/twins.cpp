@@ -1169,6 +1169,63 @@ int oracle_comb_subtract(const OCombParams* p, const float* raw, const OCombLine
 }
 
 // ---------------------------------------------------------------------------
+// The per-receiver impulse blanker. core/dsp/noise_reference.cpp:158-226:
+// detect (flag when |x|^2 * W > T * the window's power sum, oldest first, a
+// zero sum never flags) and apply (zero when a flag lies in [i - hang,
+// i + lead]). Flags are floats, as the twin's.
+// Pinned by (tests/test_oracle_noise.py): the window sums against float64
+// sums and the flag inequality against numpy away from ties; one five-sample
+// impulse 30 dB over noise blanked with its hang and lead and nothing else
+// (tests/reference/test_noise.cpp:270-322).
+// ---------------------------------------------------------------------------
+
+struct OBlankerConfig {  // core/dsp/noise_reference.h:131-140
+    uint32_t pass, window, guard, hang, lead;
+};
+struct OBlankerParams {  // :160-175
+    uint32_t chan_base, chan_mask, chan_first, out_mask, out_first, count;
+    float threshold;
+};
+
+int oracle_blank_detect(const OBlankerConfig* c, const OBlankerParams* p, const float* chan,
+                        float* flags) {
+    FlushGuard flush;
+    const float window = float(c->window);
+    for (uint32_t i = 0; i < p->count; ++i) {
+        const uint32_t ci = (p->chan_first + i) & p->chan_mask;
+        const uint32_t o = (p->out_first + i) & p->out_mask;
+        const uint64_t k = uint64_t(p->chan_base) + ci;
+        const float pw = chan[2 * k] * chan[2 * k] + chan[2 * k + 1] * chan[2 * k + 1];
+        float sum = 0.0f;
+        for (uint32_t m = 0; m < c->window; ++m) {
+            const uint32_t at = (ci - c->guard - c->window + m) & p->chan_mask;
+            const uint64_t r = uint64_t(p->chan_base) + at;
+            const float q = chan[2 * r] * chan[2 * r] + chan[2 * r + 1] * chan[2 * r + 1];
+            sum = sum + q;
+        }
+        const float lhs = pw * window;
+        const float rhs = p->threshold * sum;
+        flags[o] = (sum > 0.0f && lhs > rhs) ? 1.0f : 0.0f;
+    }
+    return 0;
+}
+
+int oracle_blank_apply(const OBlankerConfig* c, const OBlankerParams* p, const float* chan,
+                       const float* flags, float* blanked) {
+    for (uint32_t i = 0; i < p->count; ++i) {
+        const uint32_t ci = (p->chan_first + i) & p->chan_mask;
+        const uint32_t o = (p->out_first + i) & p->out_mask;
+        bool hit = false;
+        for (uint32_t m = 0; m <= c->hang + c->lead; ++m)
+            if (flags[(o - c->hang + m) & p->out_mask] != 0.0f) hit = true;
+        const uint64_t k = uint64_t(p->chan_base) + ci;
+        blanked[2 * uint64_t(o)] = hit ? 0.0f : chan[2 * k];
+        blanked[2 * uint64_t(o) + 1] = hit ? 0.0f : chan[2 * k + 1];
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // The carrier loop. core/dsp/vrx_reference.cpp:1052-1185
 // (reference_vrx_carrier) and its validate(), :986-1050: PLL (power 1) for
 // SAM, Costas (power 2) for DSB, one serial pass over the fine ring rotating

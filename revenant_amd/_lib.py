@@ -128,6 +128,15 @@ class CombParams(C.Structure):
                                    "removed_mask", "pad0", "pad1")]
 
 
+class BlankerConfig(C.Structure):
+    _fields_ = [(n, u32) for n in ("pass_", "window", "guard", "hang", "lead")]
+
+
+class BlankerParams(C.Structure):
+    _fields_ = [(n, u32) for n in ("chan_base", "chan_mask", "chan_first", "out_mask",
+                                   "out_first", "count")] + [("threshold", f32)]
+
+
 class VrxCarrierConfig(C.Structure):
     _fields_ = [("power", u32), ("nco_log2", u32)]
 
@@ -309,6 +318,13 @@ _SIGNATURES = {
                               vp, u64, vp, u64]),
     "rvx_excise_apply": (i32, [vp, C.POINTER(ExcisionConfig), C.POINTER(ExcisionParams), vp, u64,
                                vp, u64, vp, u64, vp, u64, vp]),
+    "rvx_design_blanker": (i32, [i64, C.POINTER(BlankerConfig)]),
+    "rvx_blanker_threshold": (f32, [f64]),
+    "rvx_blanker_check": (i32, [C.POINTER(BlankerConfig), C.POINTER(BlankerParams)]),
+    "rvx_blank_detect": (i32, [vp, C.POINTER(BlankerConfig), C.POINTER(BlankerParams), vp, u64,
+                               vp, u64]),
+    "rvx_blank_apply": (i32, [vp, C.POINTER(BlankerConfig), C.POINTER(BlankerParams), vp, u64,
+                              vp, u64, vp, u64]),
     "rvx_comb_step": (u32, [f64, i64]),
     "rvx_comb_table": (i32, [fptr, u64]),
     "rvx_comb_check": (i32, [C.POINTER(CombParams)]),

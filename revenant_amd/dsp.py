@@ -279,3 +279,27 @@ def comb_table() -> np.ndarray:
 def comb_check(params: _lib.CombParams) -> None:
     _lib.check(_lib.lib().rvx_comb_check(C.byref(params)))
 
+
+# ---------------------------------------------------------------------------
+# The per-receiver impulse blanker (core/dsp/noise_reference.h)
+# ---------------------------------------------------------------------------
+
+BLANK_DETECT, BLANK_APPLY = 0, 1
+
+
+def design_blanker(channel_rate: int) -> _lib.BlankerConfig:
+    """noise_reference.cpp:112 design_blanker."""
+    out = _lib.BlankerConfig()
+    _lib.check(_lib.lib().rvx_design_blanker(channel_rate, C.byref(out)))
+    return out
+
+
+def blanker_threshold(threshold_db: float = 12.0) -> float:
+    return float(_lib.lib().rvx_blanker_threshold(threshold_db))
+
+
+def blanker_check(config: _lib.BlankerConfig, params: _lib.BlankerParams, pass_: int) -> None:
+    c = _lib.BlankerConfig.from_buffer_copy(bytes(config))
+    c.pass_ = pass_
+    _lib.check(_lib.lib().rvx_blanker_check(C.byref(c), C.byref(params)))
+

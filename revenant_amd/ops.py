@@ -167,6 +167,19 @@ class Device:
         _lib.check(self.lib.rvx_iq_correct(self.ctx, C.byref(params), _p(ring_in),
                                            _csamples(ring_in), _p(out), _csamples(out)))
 
+    # -- per-receiver impulse blanker (noise_blank.comp) -------------------
+    def blank_detect(self, config, params, channel_ring: torch.Tensor,
+                     flags: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_blank_detect(self.ctx, C.byref(config), C.byref(params),
+                                             _p(channel_ring), _csamples(channel_ring), _p(flags),
+                                             _n(flags)))
+
+    def blank_apply(self, config, params, channel_ring: torch.Tensor, flags: torch.Tensor,
+                    blanked: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_blank_apply(self.ctx, C.byref(config), C.byref(params),
+                                            _p(channel_ring), _csamples(channel_ring), _p(flags),
+                                            _n(flags), _p(blanked), _csamples(blanked)))
+
     # -- comb cancellation (comb_project.comp, comb_subtract.comp) --------
     def comb_project(self, params, raw: torch.Tensor, lines: torch.Tensor, n_lines: int,
                      table: torch.Tensor, projections: torch.Tensor) -> None:
