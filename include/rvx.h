@@ -882,6 +882,117 @@ int32_t rvx_blank_apply(rvx_ctx* ctx, const rvx_blanker_config* config,
                         uint64_t ring_samples, const float* flags, uint64_t n_flags,
                         void* blanked, uint64_t blanked_samples);
 
+/* ---- The line kernel: manual notch cascade and adaptive line enhancer ----------
+ * core/dsp/noise_reference.h:190-418: per receiver, over its mono audio, a
+ * cascade of up to four notch sections, then an NLMS predictor (taps split
+ * over `lanes` partial sums, summed in lane order) whose error is the output,
+ * optionally with a guard section on the subtraction (CW). */
+#define RVX_LINE_NOTCH 1u
+#define RVX_LINE_ALE 2u
+#define RVX_LINE_GUARD 4u
+#define RVX_LINE_NOTCH_COUNT_SHIFT 8
+#define RVX_LINE_NOTCH_COUNT_MASK (3u << RVX_LINE_NOTCH_COUNT_SHIFT)
+#define RVX_MAX_NOTCHES 4
+#define RVX_MAX_LINE_STRIDE 8
+/* state layout, floats (noise_reference.h:306-318) */
+#define RVX_LINE_STATE_POSITION 4
+#define RVX_LINE_STATE_GUARD 8
+#define RVX_LINE_STATE_MORE 12
+#define RVX_LINE_STATE_WEIGHTS 24
+
+/* noise_reference.h:296-304 LineConfig. Here lanes <= 64 (one wavefront) and
+ * taps + history <= 16128 floats (the workgroup's LDS copy). */
+typedef struct rvx_line_config {
+    uint32_t lanes, taps, history;
+} rvx_line_config;
+
+typedef struct rvx_line_section {
+    float b0, b1, b2, a1, a2; /* y = b0 x + b1 x1 + b2 x2 - a1 y1 - a2 y2 */
+} rvx_line_section;
+
+/* noise_reference.h:330-366 LineParams (thirty-two words) */
+typedef struct rvx_line_params {
+    uint32_t count, flags, stride, delay;
+    float b0, b1, b2, a1, a2;
+    float mu, leak, eps;
+    float gb0, gb1, gb2, ga1, ga2;
+    rvx_line_section more[RVX_MAX_NOTCHES - 1];
+} rvx_line_params;
+
+/* One receiver of a batched launch: its parameters, its audio (count floats,
+ * in place) and its state (rvx_line_state_size floats). 144 bytes; the
+ * array lives in device memory. */
+typedef struct rvx_noise_line_item {
+    rvx_line_params params;
+    float* audio;
+    float* state;
+} rvx_noise_line_item;
+
+/* noise_reference.h:396 NotchDesign, as five doubles b0 b1 b2 a1 a2. */
+int32_t rvx_design_notch(double audio_hz, double depth_db, double width_hz, int64_t audio_rate,
+                         double* out5);
+/* :409 notch_section_db: the power response of the float-rounded section. */
+double rvx_notch_section_db(const double* design5, double audio_hz, int64_t audio_rate);
+uint64_t rvx_line_state_size(const rvx_line_config* config);
+/* :371 validate(LineConfig, LineParams), plus this kernel's lane and LDS limits. */
+int32_t rvx_line_check(const rvx_line_config* config, const rvx_line_params* params);
+/* core/shaders/noise_line.comp; twin noise_reference.cpp:262 reference_line.
+ * audio: count floats processed in place; state: rvx_line_state_size floats. */
+int32_t rvx_noise_line(rvx_ctx* ctx, const rvx_line_config* config, const rvx_line_params* params,
+                       float* audio, uint64_t n_audio, float* state, uint64_t n_state);
+/* Many receivers' line kernels in one launch, one workgroup each. The items
+ * are in device memory and the launch trusts them: the caller checks each
+ * with rvx_noise_line_item_check (its audio and state lengths) before
+ * uploading it, as for rvx_vrx_carrier_batch. */
+int32_t rvx_noise_line_item_check(const rvx_line_config* config, const rvx_noise_line_item* item,
+                                  uint64_t n_audio, uint64_t n_state);
+int32_t rvx_noise_line_batch(rvx_ctx* ctx, const rvx_line_config* config,
+                             const rvx_noise_line_item* items, uint32_t n_items);
+
+/* ---- Spectral noise reduction -------------------------------------------------
+ * noise_reference.h:420-520: 50% overlap-add with a square-root Hann window,
+ * a direct DFT, power spectral subtraction with a floor, and Doblinger
+ * minimum tracking of the noise floor. */
+#define RVX_MIN_SPECTRAL_FRAME 64
+#define RVX_MAX_SPECTRAL_FRAME 512
+
+/* noise_reference.h:489-507 SpectralParams (ten words) */
+typedef struct rvx_spectral_params {
+    uint32_t count;
+    float smoothing, smoothing_complement;
+    float gamma, beta, rise;
+    float bias, alpha, floor_gain, eps;
+} rvx_spectral_params;
+
+typedef struct rvx_noise_spectral_item {
+    rvx_spectral_params params;
+    uint32_t reserved[2];
+    float* audio;
+    float* state;
+} rvx_noise_spectral_item; /* 64 bytes */
+
+/* :523 spectral_config_for: the frame for an audio rate. */
+uint32_t rvx_spectral_frame_for(int64_t audio_rate);
+uint64_t rvx_spectral_state_size(uint32_t frame);
+/* :529 design_spectral_tables: 4 * frame floats. */
+int32_t rvx_design_spectral_tables(uint32_t frame, float* out, uint64_t n_floats);
+/* :532 spectral_params(strength), count left at zero. */
+int32_t rvx_spectral_params_for(double strength, rvx_spectral_params* out);
+int32_t rvx_spectral_check(uint32_t frame, const rvx_spectral_params* params);
+/* core/shaders/noise_spectral.comp; twin noise_reference.cpp:600
+ * reference_spectral. tables: the designed 4 * frame floats on the device. */
+int32_t rvx_noise_spectral(rvx_ctx* ctx, uint32_t frame, const rvx_spectral_params* params,
+                           const float* tables, uint64_t n_tables, float* audio, uint64_t n_audio,
+                           float* state, uint64_t n_state);
+/* Many receivers in one launch, one workgroup each, sharing one frame and
+ * its tables; items in device memory, each checked by the caller with
+ * rvx_noise_spectral_item_check before uploading. */
+int32_t rvx_noise_spectral_item_check(uint32_t frame, const rvx_noise_spectral_item* item,
+                                      uint64_t n_audio, uint64_t n_state);
+int32_t rvx_noise_spectral_batch(rvx_ctx* ctx, uint32_t frame, const float* tables,
+                                 uint64_t n_tables, const rvx_noise_spectral_item* items,
+                                 uint32_t n_items);
+
 /* ---- Plain detector ----------------------------------------------------------
  * The decision of core/detect/detector.cpp on an averaged linear spectrum:
  * estimate_noise_floor (:794-876), then find_candidates (:879-1120) through

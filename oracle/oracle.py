@@ -63,7 +63,8 @@ def lib() -> C.CDLL:
                      "oracle_vrx_carrier", "oracle_iq_moments", "oracle_iq_correct",
                      "oracle_excise_segment", "oracle_excise_reference", "oracle_excise_flag",
                      "oracle_excise_apply", "oracle_comb_project", "oracle_comb_subtract",
-                     "oracle_blank_detect", "oracle_blank_apply"):
+                     "oracle_blank_detect", "oracle_blank_apply", "oracle_noise_line",
+                     "oracle_noise_spectral"):
             getattr(_lib, name).restype = i32
     return _lib
 
@@ -341,6 +342,34 @@ def blank_apply(config, params, channel: np.ndarray, flags: np.ndarray,
     _check(lib().oracle_blank_apply(C.byref(config), C.byref(params), _a(f32c(channel)),
                                     _a(np.ascontiguousarray(flags, dtype=np.float32)),
                                     _a(blanked)), "blank_apply")
+
+
+def noise_line(config, params, audio: np.ndarray, state: np.ndarray) -> None:
+    """reference_line in place on audio (params.count floats) and state
+    (float32, 24 + taps + history). The library's rvx_line_check is the
+    caller's to run first."""
+    for a in (audio, state):
+        if a.dtype != np.float32 or not a.flags.c_contiguous:
+            raise ValueError("audio and state are contiguous float32")
+    if audio.size < params.count or state.size < 24 + config.taps + config.history:
+        raise ValueError("audio or state shorter than the parameters")
+    _check(lib().oracle_noise_line(C.byref(config), C.byref(params), _a(audio), _a(state)),
+           "noise_line")
+
+
+def noise_spectral(frame: int, params, tables: np.ndarray, audio: np.ndarray,
+                   state: np.ndarray) -> None:
+    """reference_spectral in place on audio and state (float32,
+    2 + 3 frame + 2 (frame / 2 + 1))."""
+    for a in (audio, state):
+        if a.dtype != np.float32 or not a.flags.c_contiguous:
+            raise ValueError("audio and state are contiguous float32")
+    n = frame
+    if audio.size < params.count or state.size < 2 + 2 * n + 2 * (n // 2 + 1) or \
+            tables.size < 4 * n:
+        raise ValueError("a buffer is shorter than the parameters")
+    _check(lib().oracle_noise_spectral(C.c_uint32(frame), C.byref(params), _a(f32c(tables)),
+                                       _a(audio), _a(state)), "noise_spectral")
 
 
 def detect(config, average: np.ndarray):

@@ -1226,6 +1226,245 @@ int oracle_blank_apply(const OBlankerConfig* c, const OBlankerParams* p, const f
 }
 
 // ---------------------------------------------------------------------------
+// The line kernel. core/dsp/noise_reference.cpp:262-421: a cascade of notch
+// sections (y = b0 x + b1 x1 + b2 x2 - a1 y1 - a2 y2, summed in that order),
+// then a normalized leaky LMS predictor over the history ring, its taps split
+// over `lanes` partial sums (lane l: taps l, l + lanes, ..., ascending) added
+// in lane order; the error is the output, or with the guard, v less the
+// guard section run on the prediction. State: notch [0,4), position [4],
+// guard [8,12), sections 1..3 [12,24), weights, history.
+// Pinned by (tests/test_oracle_noise.py): a tone through a notch loses what
+// the float-rounded section's analytic response says, at the centre and at
+// the half-width edges (tests/reference/test_noise.cpp:517-567); a steady
+// heterodyne is learned down past 10 dB while a gliding tone keeps within
+// 1 dB (:618-676); on CW the guard keeps the pitch and the predictor still
+// takes a heterodyne, and without the guard takes the pitch (:782-857); the
+// cascade state equals sections run one after another; lanes = taps (one tap
+// per lane) and lanes = 1 give the same output within float rounding.
+// ---------------------------------------------------------------------------
+
+struct OLineConfig {
+    uint32_t lanes, taps, history;
+};
+struct OLineSection {
+    float b0, b1, b2, a1, a2;
+};
+struct OLineParams {
+    uint32_t count, flags, stride, delay;
+    float b0, b1, b2, a1, a2;
+    float mu, leak, eps;
+    float gb0, gb1, gb2, ga1, ga2;
+    OLineSection more[3];
+};
+
+static float o_section(float x, float b0, float b1, float b2, float a1, float a2, float* st) {
+    // st: x1, x2, y1, y2
+    const float t0 = b0 * x;
+    const float t1 = b1 * st[0];
+    const float t2 = b2 * st[1];
+    const float t3 = a1 * st[2];
+    const float t4 = a2 * st[3];
+    float acc = t0 + t1;
+    acc = acc + t2;
+    acc = acc - t3;
+    acc = acc - t4;
+    st[1] = st[0];
+    st[0] = x;
+    st[3] = st[2];
+    st[2] = acc;
+    return acc;
+}
+
+int oracle_noise_line(const OLineConfig* c, const OLineParams* p, float* audio, float* state) {
+    FlushGuard flush;
+    const bool notch = (p->flags & 1u) != 0;
+    const bool ale = (p->flags & 2u) != 0;
+    const bool guard = ale && (p->flags & 4u) != 0;
+    const uint32_t sections = notch ? 1 + ((p->flags >> 8) & 3u) : 0;
+    const uint32_t lanes = c->lanes, per_lane = c->taps / c->lanes, hmask = c->history - 1;
+    float notch_st[4] = {state[0], state[1], state[2], state[3]};
+    float more_st[3][4] = {};
+    for (uint32_t s = 0; s + 1 < sections; ++s)
+        for (int k = 0; k < 4; ++k) more_st[s][k] = state[12 + 4 * s + k];
+    float guard_st[4] = {state[8], state[9], state[10], state[11]};
+    uint32_t position = static_cast<uint32_t>(state[4]);
+    float* weights = state + 24;
+    float* history = weights + c->taps;
+    std::vector<float> part_y(lanes), part_p(lanes), ref(c->taps);
+    for (uint32_t n = 0; n < p->count; ++n) {
+        float v = audio[n];
+        if (notch) {
+            v = o_section(v, p->b0, p->b1, p->b2, p->a1, p->a2, notch_st);
+            for (uint32_t s = 0; s + 1 < sections; ++s) {
+                const OLineSection& q = p->more[s];
+                v = o_section(v, q.b0, q.b1, q.b2, q.a1, q.a2, more_st[s]);
+            }
+        }
+        float out = v;
+        if (ale) {
+            for (uint32_t l = 0; l < lanes; ++l) {
+                float py = 0.0f, pp = 0.0f;
+                for (uint32_t j = 0; j < per_lane; ++j) {
+                    const uint32_t k = l + j * lanes;
+                    const float r = history[(position - p->delay - k * p->stride) & hmask];
+                    ref[k] = r;
+                    const float wy = weights[k] * r;
+                    py = py + wy;
+                    const float rr = r * r;
+                    pp = pp + rr;
+                }
+                part_y[l] = py;
+                part_p[l] = pp;
+            }
+            float y = 0.0f, pw = 0.0f;
+            for (uint32_t l = 0; l < lanes; ++l) {
+                y = y + part_y[l];
+                pw = pw + part_p[l];
+            }
+            const float e = v - y;
+            const float g = p->mu * det_recip(pw + p->eps);
+            const float ge = g * e;
+            for (uint32_t k = 0; k < c->taps; ++k) {
+                const float kept = p->leak * weights[k];
+                const float step = ge * ref[k];
+                weights[k] = kept + step;
+            }
+            history[position & hmask] = v;
+            out = e;
+            if (guard) {
+                const float acc = o_section(y, p->gb0, p->gb1, p->gb2, p->ga1, p->ga2, guard_st);
+                out = v - acc;
+            }
+        }
+        audio[n] = out;
+        position = position + 1u;
+    }
+    for (int k = 0; k < 4; ++k) state[k] = notch_st[k];
+    for (uint32_t s = 0; s + 1 < sections; ++s)
+        for (int k = 0; k < 4; ++k) state[12 + 4 * s + k] = more_st[s][k];
+    state[4] = static_cast<float>(position & hmask);
+    for (int k = 0; k < 4; ++k) state[8 + k] = guard_st[k];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Spectral noise reduction. core/dsp/noise_reference.cpp:600-735: hop H = N/2
+// of input into the history while the queued output goes out; at a full hop
+// the analysis window, a direct DFT per bin (ascending time, tables indexed
+// (k t) mod N, im accumulated by subtraction), power smoothing and Doblinger's
+// minimum (the first frame seeds both), gain 1 - alpha bias minimum / (level
+// + eps) floored, the inverse sum over bins 1..H-1 in ascending order plus DC
+// and the alternating Nyquist, the synthesis window, and the overlap-add.
+// State: phase, frames seen (saturating at 2), history N, tail H, queue H,
+// smoothed N/2+1, minimum N/2+1.
+// Pinned by (tests/test_oracle_noise.py): at unit gain the output is the input
+// one frame late within 2e-5 and the first hop is zero (the Allen overlap-add
+// identity; test_noise.cpp:911-938); on four seconds of white noise the bias
+// times the tracked minimum lands within 1.5 dB of sigma^2 N / 2 (:940-976);
+// the tables against numpy's cos/sin/Hann; a stationary tone is passed at a
+// gain within 0.5 dB of unity while noise-only bins are floored.
+// ---------------------------------------------------------------------------
+
+struct OSpectralParams {
+    uint32_t count;
+    float smoothing, smoothing_complement, gamma, beta, rise, bias, alpha, floor_gain, eps;
+};
+
+int oracle_noise_spectral(uint32_t n, const OSpectralParams* p, const float* tables, float* audio,
+                          float* state) {
+    FlushGuard flush;
+    const uint32_t h = n / 2, bins = h + 1, mask = n - 1;
+    const float* cosine = tables;
+    const float* sine = tables + n;
+    const float* analysis = tables + 2 * n;
+    const float* synthesis = tables + 3 * n;
+    float* hist = state + 2;
+    float* tail = hist + n;
+    float* queue = tail + h;
+    float* smoothed = queue + h;
+    float* minimum = smoothed + bins;
+    uint32_t phase = static_cast<uint32_t>(state[0]);
+    float seen = state[1];
+    std::vector<float> work(n), yr(bins), yi(bins);
+    uint32_t s = 0;
+    while (s < p->count) {
+        const uint32_t length = std::min(h - phase, p->count - s);
+        for (uint32_t i = 0; i < length; ++i) {
+            const float x = audio[s + i];
+            hist[h + phase + i] = x;
+            audio[s + i] = queue[phase + i];
+        }
+        phase += length;
+        s += length;
+        if (phase != h) continue;
+        for (uint32_t t = 0; t < n; ++t) work[t] = hist[t] * analysis[t];
+        for (uint32_t k = 0; k < bins; ++k) {
+            float re = 0.0f, im = 0.0f;
+            for (uint32_t t = 0; t < n; ++t) {
+                const uint32_t at = (k * t) & mask;
+                const float xc = work[t] * cosine[at];
+                re = re + xc;
+                const float xs = work[t] * sine[at];
+                im = im - xs;
+            }
+            const float rr = re * re;
+            const float ii = im * im;
+            const float power = rr + ii;
+            float level = power, floor_power = power;
+            if (seen != 0.0f) {
+                const float previous = smoothed[k];
+                const float kept = p->smoothing * previous;
+                const float fresh = p->smoothing_complement * power;
+                level = kept + fresh;
+                const float held = minimum[k];
+                if (held < level) {
+                    const float decayed = p->gamma * held;
+                    const float lagged = p->beta * previous;
+                    const float lift = level - lagged;
+                    const float risen = p->rise * lift;
+                    floor_power = decayed + risen;
+                } else {
+                    floor_power = level;
+                }
+            }
+            smoothed[k] = level;
+            minimum[k] = floor_power;
+            const float noise = p->bias * floor_power;
+            const float ratio = noise * det_recip(level + p->eps);
+            const float cut = p->alpha * ratio;
+            const float gain = std::max(1.0f - cut, p->floor_gain);
+            yr[k] = gain * re;
+            yi[k] = gain * im;
+        }
+        for (uint32_t t = 0; t < n; ++t) {
+            float acc = 0.0f;
+            for (uint32_t k = 1; k < h; ++k) {
+                const uint32_t at = (k * t) & mask;
+                const float a = yr[k] * cosine[at];
+                const float b = yi[k] * sine[at];
+                const float term = a - b;
+                acc = acc + term;
+            }
+            const float nyquist = (t & 1u) ? -yr[h] : yr[h];
+            const float doubled = 2.0f * acc;
+            float sum = yr[0] + nyquist;
+            sum = sum + doubled;
+            work[t] = sum * synthesis[t];
+        }
+        for (uint32_t t = 0; t < h; ++t) {
+            queue[t] = tail[t] + work[t];
+            tail[t] = work[h + t];
+            hist[t] = hist[h + t];
+        }
+        phase = 0;
+        seen = std::min(seen + 1.0f, 2.0f);
+    }
+    state[0] = static_cast<float>(phase);
+    state[1] = seen;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // The carrier loop. core/dsp/vrx_reference.cpp:1052-1185
 // (reference_vrx_carrier) and its validate(), :986-1050: PLL (power 1) for
 // SAM, Costas (power 2) for DSB, one serial pass over the fine ring rotating

@@ -137,6 +137,50 @@ class BlankerParams(C.Structure):
                                    "out_first", "count")] + [("threshold", f32)]
 
 
+class LineConfig(C.Structure):
+    _fields_ = [("lanes", u32), ("taps", u32), ("history", u32)]
+
+    def __init__(self, lanes=32, taps=128, history=2048):
+        super().__init__(lanes, taps, history)
+
+
+class LineSection(C.Structure):
+    _fields_ = [(n, f32) for n in ("b0", "b1", "b2", "a1", "a2")]
+
+    def __init__(self, b0=1.0, b1=0.0, b2=0.0, a1=0.0, a2=0.0):
+        super().__init__(b0, b1, b2, a1, a2)
+
+
+class LineParams(C.Structure):
+    _fields_ = ([(n, u32) for n in ("count", "flags", "stride", "delay")] +
+                [(n, f32) for n in ("b0", "b1", "b2", "a1", "a2", "mu", "leak", "eps",
+                                    "gb0", "gb1", "gb2", "ga1", "ga2")] +
+                [("more", LineSection * 3)])
+
+    def __init__(self, **kw):
+        super().__init__()
+        defaults = dict(stride=1, delay=1, b0=1.0, leak=1.0, eps=1.0e-6, gb0=1.0)
+        for k, v in {**defaults, **kw}.items():
+            setattr(self, k, v)
+        for i in range(3):
+            if "more" not in kw:
+                self.more[i] = LineSection()
+
+
+class NoiseLineItem(C.Structure):
+    _fields_ = [("params", LineParams), ("audio", vp), ("state", vp)]
+
+
+class SpectralParams(C.Structure):
+    _fields_ = [("count", u32)] + [(n, f32) for n in (
+        "smoothing", "smoothing_complement", "gamma", "beta", "rise", "bias", "alpha",
+        "floor_gain", "eps")]
+
+
+class NoiseSpectralItem(C.Structure):
+    _fields_ = [("params", SpectralParams), ("reserved", u32 * 2), ("audio", vp), ("state", vp)]
+
+
 class VrxCarrierConfig(C.Structure):
     _fields_ = [("power", u32), ("nco_log2", u32)]
 
@@ -325,6 +369,23 @@ _SIGNATURES = {
                                vp, u64]),
     "rvx_blank_apply": (i32, [vp, C.POINTER(BlankerConfig), C.POINTER(BlankerParams), vp, u64,
                               vp, u64, vp, u64]),
+    "rvx_design_notch": (i32, [f64, f64, f64, i64, vp]),
+    "rvx_notch_section_db": (f64, [vp, f64, i64]),
+    "rvx_line_state_size": (u64, [C.POINTER(LineConfig)]),
+    "rvx_line_check": (i32, [C.POINTER(LineConfig), C.POINTER(LineParams)]),
+    "rvx_noise_line_item_check": (i32, [C.POINTER(LineConfig), C.POINTER(NoiseLineItem), u64,
+                                        u64]),
+    "rvx_noise_line": (i32, [vp, C.POINTER(LineConfig), C.POINTER(LineParams), vp, u64, vp,
+                             u64]),
+    "rvx_noise_line_batch": (i32, [vp, C.POINTER(LineConfig), vp, u32]),
+    "rvx_spectral_frame_for": (u32, [i64]),
+    "rvx_spectral_state_size": (u64, [u32]),
+    "rvx_design_spectral_tables": (i32, [u32, vp, u64]),
+    "rvx_spectral_params_for": (i32, [f64, C.POINTER(SpectralParams)]),
+    "rvx_spectral_check": (i32, [u32, C.POINTER(SpectralParams)]),
+    "rvx_noise_spectral_item_check": (i32, [u32, C.POINTER(NoiseSpectralItem), u64, u64]),
+    "rvx_noise_spectral": (i32, [vp, u32, C.POINTER(SpectralParams), vp, u64, vp, u64, vp, u64]),
+    "rvx_noise_spectral_batch": (i32, [vp, u32, vp, u64, vp, u32]),
     "rvx_comb_step": (u32, [f64, i64]),
     "rvx_comb_table": (i32, [fptr, u64]),
     "rvx_comb_check": (i32, [C.POINTER(CombParams)]),

@@ -303,3 +303,89 @@ def blanker_check(config: _lib.BlankerConfig, params: _lib.BlankerParams, pass_:
     c.pass_ = pass_
     _lib.check(_lib.lib().rvx_blanker_check(C.byref(c), C.byref(params)))
 
+
+# ---------------------------------------------------------------------------
+# The line kernel and spectral noise reduction (core/dsp/noise_reference.h)
+# ---------------------------------------------------------------------------
+
+LINE_NOTCH, LINE_ALE, LINE_GUARD = 1, 2, 4
+LINE_NOTCH_COUNT_SHIFT = 8
+MAX_NOTCHES = 4
+LINE_STATE_POSITION, LINE_STATE_GUARD, LINE_STATE_MORE, LINE_STATE_WEIGHTS = 4, 8, 12, 24
+CW_GUARD_MIN_HZ, CW_GUARD_DEPTH_DB = 50.0, 40.0
+
+
+def line_notch_flags(sections: int) -> int:
+    """noise_reference.h:280 line_notch_flags."""
+    return 0 if sections == 0 else LINE_NOTCH | ((sections - 1) << LINE_NOTCH_COUNT_SHIFT)
+
+
+def line_notch_sections(flags: int) -> int:
+    return 0 if not flags & LINE_NOTCH else 1 + ((flags >> LINE_NOTCH_COUNT_SHIFT) & 3)
+
+
+def design_notch(audio_hz: float, depth_db: float, width_hz: float,
+                 audio_rate: int) -> tuple[float, float, float, float, float]:
+    """noise_reference.cpp:424 design_notch: (b0, b1, b2, a1, a2) in double."""
+    out = (C.c_double * 5)()
+    _lib.check(_lib.lib().rvx_design_notch(audio_hz, depth_db, width_hz, audio_rate, out))
+    return tuple(out)
+
+
+def notch_section_db(design, audio_hz: float, audio_rate: int) -> float:
+    d = (C.c_double * 5)(*design)
+    return float(_lib.lib().rvx_notch_section_db(d, audio_hz, audio_rate))
+
+
+def notch_response_db(centre_hz, depth_db, width_hz, audio_rate, audio_hz) -> float:
+    return notch_section_db(design_notch(centre_hz, depth_db, width_hz, audio_rate), audio_hz,
+                            audio_rate)
+
+
+def set_notch(params: _lib.LineParams, index: int, design, guard: bool = False) -> None:
+    """Section `index` (0 the first, 1..3 the cascade) or the guard from a
+    design, each coefficient rounded to float once."""
+    b0, b1, b2, a1, a2 = (float(np.float32(v)) for v in design)
+    if guard:
+        params.gb0, params.gb1, params.gb2, params.ga1, params.ga2 = b0, b1, b2, a1, a2
+    elif index == 0:
+        params.b0, params.b1, params.b2, params.a1, params.a2 = b0, b1, b2, a1, a2
+    else:
+        params.more[index - 1] = _lib.LineSection(b0, b1, b2, a1, a2)
+
+
+def line_state_size(config: _lib.LineConfig) -> int:
+    return int(_lib.lib().rvx_line_state_size(C.byref(config)))
+
+
+def line_check(config: _lib.LineConfig, params: _lib.LineParams) -> None:
+    _lib.check(_lib.lib().rvx_line_check(C.byref(config), C.byref(params)))
+
+
+def spectral_frame_for(audio_rate: int) -> int:
+    """noise_reference.cpp:492 spectral_config_for."""
+    return int(_lib.lib().rvx_spectral_frame_for(audio_rate))
+
+
+def spectral_state_size(frame: int) -> int:
+    return int(_lib.lib().rvx_spectral_state_size(frame))
+
+
+def spectral_tables(frame: int) -> np.ndarray:
+    """noise_reference.cpp:502 design_spectral_tables: cos, sin, analysis and
+    synthesis (1/N folded in) windows, frame floats each."""
+    out = np.zeros(4 * frame, np.float32)
+    _lib.check(_lib.lib().rvx_design_spectral_tables(frame, out.ctypes.data_as(C.c_void_p),
+                                                      out.size))
+    return out
+
+
+def spectral_params(strength: float) -> _lib.SpectralParams:
+    out = _lib.SpectralParams()
+    _lib.check(_lib.lib().rvx_spectral_params_for(strength, C.byref(out)))
+    return out
+
+
+def spectral_check(frame: int, params: _lib.SpectralParams) -> None:
+    _lib.check(_lib.lib().rvx_spectral_check(frame, C.byref(params)))
+

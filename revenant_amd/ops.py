@@ -180,6 +180,27 @@ class Device:
                                             _p(channel_ring), _csamples(channel_ring), _p(flags),
                                             _n(flags), _p(blanked), _csamples(blanked)))
 
+    # -- line kernel and spectral noise reduction (noise_line/spectral.comp) --
+    def noise_line(self, config, params, audio: torch.Tensor, state: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_noise_line(self.ctx, C.byref(config), C.byref(params), _p(audio),
+                                           _n(audio), _p(state), _n(state)))
+
+    def noise_line_batch(self, config, items: torch.Tensor, n_items: int) -> None:
+        """items: a device byte tensor of n_items rvx_noise_line_item records,
+        each checked with rvx_noise_line_item_check by the caller."""
+        _lib.check(self.lib.rvx_noise_line_batch(self.ctx, C.byref(config), _p(items), n_items))
+
+    def noise_spectral(self, frame: int, params, tables: torch.Tensor, audio: torch.Tensor,
+                       state: torch.Tensor) -> None:
+        _lib.check(self.lib.rvx_noise_spectral(self.ctx, frame, C.byref(params), _p(tables),
+                                               _n(tables), _p(audio), _n(audio), _p(state),
+                                               _n(state)))
+
+    def noise_spectral_batch(self, frame: int, tables: torch.Tensor, items: torch.Tensor,
+                             n_items: int) -> None:
+        _lib.check(self.lib.rvx_noise_spectral_batch(self.ctx, frame, _p(tables), _n(tables),
+                                                     _p(items), n_items))
+
     # -- comb cancellation (comb_project.comp, comb_subtract.comp) --------
     def comb_project(self, params, raw: torch.Tensor, lines: torch.Tensor, n_lines: int,
                      table: torch.Tensor, projections: torch.Tensor) -> None:
