@@ -642,6 +642,129 @@ def conditioning_leg(dev, torch, packed, g: Grid, partial, kernel_ms, steps,
             "ns_per_sample_serial": round(t_c * 1e9 / (n_cpu_rx * count), 1),
             "msps_per_core": round(n_cpu_rx * count / t_c / 1e6, 1), "cores": 1,
             "sample": f"{n_cpu_rx} receivers x {count} samples"}
+    del rings_c
+    out["noise"] = noise_leg(dev, torch, gen, timed, cpu)
+    return out
+
+
+def noise_leg(dev, torch, gen, timed, cpu: bool) -> dict:
+    """The per-receiver noise stages: the impulse blanker's two passes over a
+    2^24-sample channel span (HBM-bound: detect reads 8 B and writes a 4 B
+    flag per sample, apply reads 8 B and a flag and writes 8 B), and the line
+    kernel (two notches and the line enhancer) and spectral noise reduction
+    (frame 512) for 256 receivers, 10 ms of 48 kS/s audio each, one workgroup
+    per receiver (serial recurrences: latency-bound)."""
+    import ctypes as C
+
+    from revenant_amd import _lib as L
+    from revenant_amd.dsp import (LINE_ALE, blanker_threshold, design_blanker, design_notch,
+                                  line_notch_flags, line_state_size, set_notch,
+                                  spectral_params, spectral_state_size, spectral_tables)
+    out = {}
+    # blanker over one long span of a channel ring
+    n, ring = 1 << 24, 1 << 25
+    chan = (torch.randn(2 * ring, generator=gen, device="cuda") * 0.01).contiguous()
+    hits = torch.randint(0, ring, (8192,), generator=gen, device="cuda")
+    chan.view(-1, 2)[hits] += 0.5
+    flags = torch.zeros(ring, dtype=torch.float32, device="cuda")
+    blanked = torch.zeros(2 * ring, dtype=torch.float32, device="cuda")
+    cfg = design_blanker(144_000)
+    bp = L.BlankerParams(0, ring - 1, 1000, ring - 1, 1000, n, blanker_threshold(12.0))
+    ap = L.BlankerParams(0, ring - 1, 1000 + cfg.hang, ring - 1, 1000 + cfg.hang,
+                         n - cfg.hang - cfg.lead, bp.threshold)
+    det_ms = timed(lambda: dev.blank_detect(cfg, bp, chan, flags))
+    app_ms = timed(lambda: dev.blank_apply(cfg, ap, chan, flags, blanked))
+    det_bytes, app_bytes = 12 * n, 20 * n
+    out["blanker"] = {
+        "samples": n, "window": cfg.window,
+        "detect_ms": round(det_ms, 4), "apply_ms": round(app_ms, 4),
+        "msps": round(n / ((det_ms + app_ms) * 1e-3) / 1e6, 1),
+        "detect_roofline": {"bound": "hbm",
+                            "achieved_gbs": round(det_bytes / (det_ms * 1e-3) / 1e9, 1),
+                            "frac": round(det_bytes / (det_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "bytes": det_bytes},
+        "apply_roofline": {"bound": "hbm",
+                           "achieved_gbs": round(app_bytes / (app_ms * 1e-3) / 1e9, 1),
+                           "frac": round(app_bytes / (app_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                           "bytes": app_bytes},
+        "flagged": int((flags[1000:1000 + n] != 0).sum().item())}
+    if cpu:
+        from oracle import oracle
+        n_cpu = 1 << 21
+        h_chan = chan[:2 * (1 << 22)].cpu().numpy().reshape(-1, 2).copy()
+        h_flags = np.zeros(1 << 22, np.float32)
+        h_out = np.zeros((1 << 22, 2), np.float32)
+        c_bp = L.BlankerParams(0, (1 << 22) - 1, 1000, (1 << 22) - 1, 1000, n_cpu, bp.threshold)
+        t0 = time.perf_counter()
+        oracle.blank_detect(cfg, c_bp, h_chan, h_flags)
+        oracle.blank_apply(cfg, c_bp, h_chan, h_flags, h_out)
+        t_b = time.perf_counter() - t0
+        out["blanker"]["cpu_oracle"] = {"msps": round(n_cpu / t_b / 1e6, 1), "cores": 1,
+                                        "sample": f"{n_cpu} samples, both passes"}
+    del chan, flags, blanked
+
+    # line kernel and spectral stage: 256 receivers x 480 samples
+    n_rx, count = 256, 480
+    audio = (torch.randn(n_rx, count, generator=gen, device="cuda") * 0.1).contiguous()
+    lc = L.LineConfig()
+    lp = L.LineParams(flags=line_notch_flags(2) | LINE_ALE, stride=7, delay=240, mu=1.0e-4,
+                      leak=float(np.float32(1.0 - 3.0e-6)), eps=1.0e-10, count=count)
+    set_notch(lp, 0, design_notch(1000.0, 40.0, 100.0, 48_000))
+    set_notch(lp, 1, design_notch(2200.0, 30.0, 80.0, 48_000))
+    l_state = torch.zeros(n_rx, line_state_size(lc), dtype=torch.float32, device="cuda")
+    items = []
+    for k in range(n_rx):
+        it = L.NoiseLineItem(lp, audio[k].data_ptr(), l_state[k].data_ptr())
+        L.check(L.lib().rvx_noise_line_item_check(C.byref(lc), C.byref(it), count,
+                                                  l_state.shape[1]))
+        items.append(it)
+    l_items = torch.frombuffer(bytearray(b"".join(bytes(i) for i in items)),
+                               dtype=torch.uint8).cuda()
+    line_ms = timed(lambda: dev.noise_line_batch(lc, l_items, n_rx))
+    frame = 512
+    tables = torch.from_numpy(spectral_tables(frame)).cuda()
+    sp = spectral_params(0.5)
+    sp.count = count
+    s_state = torch.zeros(n_rx, spectral_state_size(frame), dtype=torch.float32, device="cuda")
+    items = []
+    for k in range(n_rx):
+        it = L.NoiseSpectralItem(sp, (C.c_uint32 * 2)(), audio[k].data_ptr(),
+                                 s_state[k].data_ptr())
+        L.check(L.lib().rvx_noise_spectral_item_check(frame, C.byref(it), count,
+                                                      s_state.shape[1]))
+        items.append(it)
+    s_items = torch.frombuffer(bytearray(b"".join(bytes(i) for i in items)),
+                               dtype=torch.uint8).cuda()
+    spec_ms = timed(lambda: dev.noise_spectral_batch(frame, tables, s_items, n_rx))
+    block_ms = 1000.0 * count / 48_000
+    out["line"] = {"receivers": n_rx, "samples_per_receiver": count, "ms": round(line_ms, 4),
+                   "ns_per_sample_serial": round(line_ms * 1e6 / count, 1),
+                   "realtime_factor_48k": round(block_ms / line_ms, 1),
+                   "bound": "latency (one serial recurrence per receiver)"}
+    out["spectral"] = {"receivers": n_rx, "samples_per_receiver": count, "frame": frame,
+                       "ms": round(spec_ms, 4), "realtime_factor_48k": round(block_ms / spec_ms, 1),
+                       "bound": "latency (direct DFT per hop, one workgroup per receiver)"}
+    if cpu:
+        from oracle import oracle
+        n_cpu_rx = 16
+        host = audio[:n_cpu_rx].cpu().numpy().copy()
+        st = np.zeros((n_cpu_rx, line_state_size(lc)), np.float32)
+        t0 = time.perf_counter()
+        for k in range(n_cpu_rx):
+            oracle.noise_line(lc, lp, host[k], st[k])
+        t_l = time.perf_counter() - t0
+        host = audio[:n_cpu_rx].cpu().numpy().copy()
+        st = np.zeros((n_cpu_rx, spectral_state_size(frame)), np.float32)
+        tab = spectral_tables(frame)
+        t0 = time.perf_counter()
+        for k in range(n_cpu_rx):
+            oracle.noise_spectral(frame, sp, tab, host[k], st[k])
+        t_s = time.perf_counter() - t0
+        sample = f"{n_cpu_rx} receivers x {count} samples"
+        out["line"]["cpu_oracle"] = {"ms_for_256": round(t_l / n_cpu_rx * n_rx * 1e3, 2),
+                                     "cores": 1, "sample": sample}
+        out["spectral"]["cpu_oracle"] = {"ms_for_256": round(t_s / n_cpu_rx * n_rx * 1e3, 2),
+                                         "cores": 1, "sample": sample}
     return out
 
 

@@ -77,6 +77,9 @@ __device__ __forceinline__ float lane_value(float v, uint32_t lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), int(lane)));
 }
 
+// kLanes / kPerLane nonzero: the reference's default shape compiled in, so
+// the per-tap loops and the lane fold unroll; zero: read from the config.
+template <uint32_t kLanes, uint32_t kPerLane>
 __global__ void __launch_bounds__(kWave)
 line_kernel(rvx_line_config c, const rvx_noise_line_item* __restrict__ items,
             rvx_noise_line_item one) {
@@ -92,7 +95,8 @@ line_kernel(rvx_line_config c, const rvx_noise_line_item* __restrict__ items,
     const bool ale = (p.flags & RVX_LINE_ALE) != 0;
     const bool guard = ale && (p.flags & RVX_LINE_GUARD) != 0;
     const uint32_t sections = notch_sections(p.flags);
-    const uint32_t per_lane = c.taps / c.lanes;
+    const uint32_t lanes = kLanes ? kLanes : c.lanes;
+    const uint32_t per_lane = kPerLane ? kPerLane : c.taps / c.lanes;
     const uint32_t hmask = c.history - 1;
 
     if (ale) {
@@ -131,9 +135,10 @@ line_kernel(rvx_line_config c, const rvx_noise_line_item* __restrict__ items,
             float out = v;
             if (ale) {
                 float py = 0.0f, pp = 0.0f;
-                if (lane < c.lanes) {
+                if (lane < lanes) {
+#pragma unroll
                     for (uint32_t j = 0; j < per_lane; ++j) {
-                        const uint32_t k = lane + j * c.lanes;
+                        const uint32_t k = lane + j * lanes;
                         const float r = history[(position - p.delay - k * p.stride) & hmask];
                         const float wy = weights[k] * r;
                         py = py + wy;
@@ -142,16 +147,18 @@ line_kernel(rvx_line_config c, const rvx_noise_line_item* __restrict__ items,
                     }
                 }
                 float y = 0.0f, pw = 0.0f;
-                for (uint32_t l = 0; l < c.lanes; ++l) {
+#pragma unroll
+                for (uint32_t l = 0; l < lanes; ++l) {
                     y = y + lane_value(py, l);
                     pw = pw + lane_value(pp, l);
                 }
                 const float e = v - y;
                 const float g = p.mu * rvx::det_recip(pw + p.eps);
                 const float ge = g * e;
-                if (lane < c.lanes) {
+                if (lane < lanes) {
+#pragma unroll
                     for (uint32_t j = 0; j < per_lane; ++j) {
-                        const uint32_t k = lane + j * c.lanes;
+                        const uint32_t k = lane + j * lanes;
                         const float r = history[(position - p.delay - k * p.stride) & hmask];
                         const float kept = p.leak * weights[k];
                         const float step = ge * r;
@@ -387,6 +394,15 @@ size_t line_lds_bytes(const rvx_line_config& c) {
     return (size_t(c.taps) + c.history + kChunk) * sizeof(float);
 }
 
+void launch_line(const rvx_line_config& c, uint32_t groups, const rvx_noise_line_item* items,
+                 const rvx_noise_line_item& one, hipStream_t stream) {
+    const size_t lds = line_lds_bytes(c);
+    if (c.lanes == 32 && c.taps == 128)
+        line_kernel<32, 4><<<dim3(groups), dim3(kWave), lds, stream>>>(c, items, one);
+    else
+        line_kernel<0, 0><<<dim3(groups), dim3(kWave), lds, stream>>>(c, items, one);
+}
+
 int32_t spectral_valid(uint32_t frame, const rvx_spectral_params& p, const char* who) {
     if (!rvx::pow2(frame) || frame < RVX_MIN_SPECTRAL_FRAME || frame > RVX_MAX_SPECTRAL_FRAME)
         return rvx::fail(RVX_EINVAL, "%s: a frame of %u is not a power of two in [%u, %u]", who,
@@ -489,8 +505,7 @@ int32_t rvx_noise_line(rvx_ctx* ctx, const rvx_line_config* config, const rvx_li
     one.audio = audio;
     one.state = state;
     if (int32_t rc = rvx_noise_line_item_check(config, &one, n_audio, n_state)) return rc;
-    line_kernel<<<dim3(1), dim3(kWave), line_lds_bytes(*config), ctx->stream>>>(*config, nullptr,
-                                                                                 one);
+    launch_line(*config, 1, nullptr, one, ctx->stream);
     return rvx::check_launch(who);
 }
 
@@ -506,8 +521,7 @@ int32_t rvx_noise_line_batch(rvx_ctx* ctx, const rvx_line_config* config,
     probe.delay = 1;
     probe.eps = 1.0f;
     if (int32_t rc = line_valid(*config, probe, who)) return rc;  // the shared config
-    line_kernel<<<dim3(n_items), dim3(kWave), line_lds_bytes(*config), ctx->stream>>>(
-        *config, items, rvx_noise_line_item{});
+    launch_line(*config, n_items, items, rvx_noise_line_item{}, ctx->stream);
     return rvx::check_launch(who);
 }
 

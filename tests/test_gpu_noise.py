@@ -197,3 +197,25 @@ def test_spectral_batch_runs_each_receiver_as_alone(dev):
     for i, (w_a, w_s) in enumerate(wants):
         assert_bit_exact(from_dev(d_audio[i]), w_a, i, f"batched spectral audio {i}")
         assert_bit_exact(from_dev(d_state[i]), w_s, i, f"batched spectral state {i}")
+
+
+def test_line_matches_twin_in_a_shape_of_its_own(dev):
+    """A config other than the default 32 lanes x 128 taps (which runs a
+    specialised kernel): 16 lanes of 6 taps over a 1024-sample history."""
+    c = _lib.LineConfig(lanes=16, taps=96, history=1024)
+    p = line_params(LINE_NOTCH | LINE_ALE | LINE_GUARD)
+    audio = voice_audio(5000, 77, 48000.0)
+    o_state = np.zeros(line_state_size(c), np.float32)
+    g_state = to_dev(o_state)
+    at = 0
+    for n in (1000, 1, 2999, 1000):
+        p.count = n
+        line_check(c, p)
+        want = audio[at:at + n].copy()
+        oracle.noise_line(c, p, want, o_state)
+        got = to_dev(audio[at:at + n].copy())
+        print(f"noise line 16x96 dispatch {n} at {at}", flush=True)
+        dev.noise_line(c, p, got, g_state)
+        assert_bit_exact(from_dev(got), want, 77, f"line audio ({n} at {at})")
+        assert_bit_exact(from_dev(g_state), o_state, 77, f"line state ({n} at {at})")
+        at += n
