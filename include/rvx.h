@@ -338,6 +338,42 @@ int32_t rvx_demod_block(const rvx_vrx_plan* plan, uint32_t fine_mask, uint64_t f
 /* core/dsp/vrx_reference.h:555 demod_fine_history. */
 uint32_t rvx_demod_history(const rvx_vrx_demod_config* config);
 
+/* ---- The display tap: vrx_reference.h:1639-1768 ---------------------------------
+ * A receiver's passband pane: the fine kernel (rvx_vrx_fine) run a second time
+ * with a 256-tap, one-phase anti-alias table, reading the channel before any
+ * receiver filter, mixing by the fine stage's own frequency and decimating by
+ * an integer R from the channel rate's ladder, so the pane is flat across its
+ * width and the client draws the receiver's filter over it. */
+#define RVX_DISPLAY_TAPS 256
+
+/* vrx_reference.h:1727-1754 VrxDisplayPlan (the tap table is separate) */
+typedef struct rvx_vrx_display_plan {
+    int64_t channel_rate, display_rate;
+    uint32_t decimation, reserved;
+    int64_t reach_hz;                 /* the passband's further edge from DC */
+    rvx_vrx_fine_config fine;         /* taps 256, phases 1, the plan's NCO */
+    uint32_t reserved2;
+    uint64_t nco_delta;               /* the fine mix at the display rate */
+    double pass_hz, stop_hz, attenuation_db;
+    double group_delay_channel_samples;
+} rvx_vrx_display_plan;
+
+/* :1721 display_max_decimation (25). */
+uint32_t rvx_display_max_decimation(void);
+/* :1716 display_ladder: the rungs for a channel rate, ascending from 1 (writes
+ * up to max_rungs, returns how many there are). */
+uint32_t rvx_display_ladder(int64_t channel_rate, uint32_t* out, uint32_t max_rungs);
+/* :1725 display_decimation. */
+uint32_t rvx_display_decimation(int64_t channel_rate, int64_t reach_hz);
+/* :1758 plan_vrx_display. taps: 2 * (RVX_DISPLAY_TAPS + 1) floats, or NULL. */
+int32_t rvx_plan_vrx_display(const rvx_vrx_plan* plan, rvx_vrx_display_plan* out, float* taps,
+                             uint64_t n_floats);
+/* :1762 display_block: rvx_vrx_fine params for display outputs
+ * [first_output, first_output + count). */
+int32_t rvx_display_block(const rvx_vrx_display_plan* plan, uint32_t channel_base,
+                          uint32_t channel_mask, uint32_t display_mask, uint64_t first_output,
+                          uint32_t count, rvx_vrx_fine_params* out, uint64_t* first_input);
+
 /* Carrier loop design, core/dsp/vrx_reference.cpp:918-1185. */
 /* :918 carrier_power: RVX_CARRIER_* for a demod mode. */
 uint32_t rvx_carrier_power(uint32_t mode);

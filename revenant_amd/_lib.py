@@ -181,6 +181,13 @@ class NoiseSpectralItem(C.Structure):
     _fields_ = [("params", SpectralParams), ("reserved", u32 * 2), ("audio", vp), ("state", vp)]
 
 
+class VrxDisplayPlan(C.Structure):
+    _fields_ = [("channel_rate", i64), ("display_rate", i64), ("decimation", u32),
+                ("reserved", u32), ("reach_hz", i64), ("fine", VrxFineConfig),
+                ("reserved2", u32), ("nco_delta", u64), ("pass_hz", f64), ("stop_hz", f64),
+                ("attenuation_db", f64), ("group_delay_channel_samples", f64)]
+
+
 class VrxCarrierConfig(C.Structure):
     _fields_ = [("power", u32), ("nco_log2", u32)]
 
@@ -386,6 +393,12 @@ _SIGNATURES = {
     "rvx_noise_spectral_item_check": (i32, [u32, C.POINTER(NoiseSpectralItem), u64, u64]),
     "rvx_noise_spectral": (i32, [vp, u32, C.POINTER(SpectralParams), vp, u64, vp, u64, vp, u64]),
     "rvx_noise_spectral_batch": (i32, [vp, u32, vp, u64, vp, u32]),
+    "rvx_display_max_decimation": (u32, []),
+    "rvx_display_ladder": (u32, [i64, vp, u32]),
+    "rvx_display_decimation": (u32, [i64, i64]),
+    "rvx_plan_vrx_display": (i32, [C.POINTER(VrxPlan), C.POINTER(VrxDisplayPlan), vp, u64]),
+    "rvx_display_block": (i32, [C.POINTER(VrxDisplayPlan), u32, u32, u32, u64, u32,
+                                C.POINTER(VrxFineParams), C.POINTER(u64)]),
     "rvx_comb_step": (u32, [f64, i64]),
     "rvx_comb_table": (i32, [fptr, u64]),
     "rvx_comb_check": (i32, [C.POINTER(CombParams)]),

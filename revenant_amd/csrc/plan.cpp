@@ -180,6 +180,86 @@ bool complex_output(uint32_t mode) { return mode == RVX_DEMOD_RAW; }
 
 }  // namespace
 
+// Kaiser-windowed lowpass of cutoff half_width over phases * taps points,
+// each phase normalised to unit DC gain, then modulated to centre_num /
+// centre_den Hz about the filter's own centre with the angle reduced in
+// integers (vrx_reference.cpp:1808 design_fine_taps). out: 2 * (length + 1)
+// floats, the last pair zero.
+static int32_t design_fine_table(const rvx_vrx_fine_config& fine, int64_t channel_rate,
+                          int64_t half_width, int64_t centre_num, int64_t centre_den,
+                          double attenuation_db, float* out, const char* who) {
+    const size_t phases = fine.phases, taps = fine.taps, length = phases * taps;
+    if (half_width <= 0 || 2 * half_width >= channel_rate || centre_den == 0)
+        return rvx::fail(RVX_EINVAL, "%s: half width %lld at %lld S/s", who,
+                         (long long)half_width, (long long)channel_rate);
+    const double centre = (double(length) - 1.0) / 2.0;
+    const double cutoff = double(half_width) / double(channel_rate);
+    const double beta = kaiser_shape(attenuation_db), norm = i0(beta);
+    std::vector<double> proto(length);
+    for (size_t i = 0; i < length; ++i) {
+        const double pos = (double(i) - centre) / double(phases);
+        proto[i] = 2.0 * cutoff * sinc_pi(2.0 * cutoff * pos) *
+                   kaiser_window(i, length, beta, norm);
+    }
+    for (size_t ph = 0; ph < phases; ++ph) {
+        double sum = 0.0;
+        for (size_t k = 0; k < taps; ++k) sum += proto[ph + k * phases];
+        if (std::abs(sum) < 1e-12) continue;
+        for (size_t k = 0; k < taps; ++k) proto[ph + k * phases] /= sum;
+    }
+    const int64_t period = 2 * iabs(centre_den) * int64_t(phases) * channel_rate;
+    const int64_t num = centre_den < 0 ? -centre_num : centre_num;
+    for (size_t i = 0; i < length; ++i) {
+        const int64_t twice = 2 * int64_t(i) - (int64_t(length) - 1);
+        int64_t turns = (num * twice) % period;
+        if (turns < 0) turns += period;
+        const double ang = kTwoPi * double(turns) / double(period);
+        out[2 * i] = float(proto[i] * std::cos(ang));
+        out[2 * i + 1] = float(proto[i] * std::sin(ang));
+    }
+    out[2 * length] = 0.0f;
+    out[2 * length + 1] = 0.0f;
+    return RVX_OK;
+}
+
+// The exact resampler recurrence for outputs [first_output, first_output +
+// count) from Fc to Fd: n0 = floor(j0 Fc / Fd) and its remainder without
+// forming j0 Fc, and the NCO phase j0 * delta mod 2^64.
+static int32_t resample_block(const rvx_vrx_fine_config& fine, int64_t channel_rate, int64_t out_rate,
+                       uint64_t nco_delta, uint32_t channel_base, uint32_t channel_mask,
+                       uint32_t out_mask, uint64_t first_output, uint32_t count,
+                       rvx_vrx_fine_params* out, uint64_t* first_input, const char* who) {
+    if (channel_rate <= 0 || out_rate <= 0)
+        return rvx::fail(RVX_EINVAL, "%s: the plan carries no rates", who);
+    if (count == 0) return rvx::fail(RVX_EINVAL, "%s: zero outputs", who);
+    const uint64_t fc = uint64_t(channel_rate), fd = uint64_t(out_rate);
+    const uint64_t sw = fc / fd, sr = fc % fd;
+    const uint64_t whole = first_output / fd, part = first_output % fd;
+    const uint64_t cp = part * sr;
+    const uint64_t n0 = first_output * sw + whole * sr + cp / fd;
+    rvx_vrx_fine_params p{};
+    p.chan_base = channel_base;
+    p.chan_mask = channel_mask;
+    p.in_offset = uint32_t(n0 & channel_mask);
+    p.out_offset = uint32_t(first_output & out_mask);
+    p.out_mask = out_mask;
+    p.count = count;
+    p.step_whole = uint32_t(sw);
+    p.step_rem = uint32_t(sr);
+    p.out_rate = uint32_t(fd);
+    p.frac0 = uint32_t(cp % fd);
+    p.inv_out_rate = 1.0f / float(fd);
+    const uint64_t phase = nco_delta * first_output;  // wraps mod 2^64
+    p.nco_phase_high = uint32_t(phase >> 32);
+    p.nco_phase_low = uint32_t(phase);
+    p.nco_delta_high = uint32_t(nco_delta >> 32);
+    p.nco_delta_low = uint32_t(nco_delta);
+    if (int32_t rc = rvx::vrx_fine_valid(fine, p, who)) return rc;
+    *out = p;
+    if (first_input) *first_input = n0;
+    return RVX_OK;
+}
+
 extern "C" {
 
 int32_t rvx_place(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* request,
@@ -335,39 +415,9 @@ int32_t rvx_plan_vrx_tables(const rvx_vrx_plan* plan, float* fine_taps, uint64_t
         if (n_fine_floats != 2 * (length + 1))
             return rvx::fail(RVX_EINVAL, "%s: fine table wants %zu floats", who, 2 * (length + 1));
         const int64_t half_width = p.bandwidth / 2;
-        if (half_width <= 0 || 2 * half_width >= p.channel_rate || p.filter_den == 0)
-            return rvx::fail(RVX_EINVAL, "%s: half width %lld at %lld S/s", who,
-                             (long long)half_width, (long long)p.channel_rate);
-        // Kaiser-windowed lowpass of cutoff half_width over P*T taps, each
-        // phase normalised to unit DC gain, then modulated to the passband's
-        // centre filter_num/filter_den Hz with the angle reduced in integers.
-        const double centre = (double(length) - 1.0) / 2.0;
-        const double cutoff = double(half_width) / double(p.channel_rate);
-        const double beta = kaiser_shape(kFineAttenuationDb), norm = i0(beta);
-        std::vector<double> proto(length);
-        for (size_t i = 0; i < length; ++i) {
-            const double pos = (double(i) - centre) / double(phases);
-            proto[i] = 2.0 * cutoff * sinc_pi(2.0 * cutoff * pos) *
-                       kaiser_window(i, length, beta, norm);
-        }
-        for (size_t ph = 0; ph < phases; ++ph) {
-            double sum = 0.0;
-            for (size_t k = 0; k < taps; ++k) sum += proto[ph + k * phases];
-            if (std::abs(sum) < 1e-12) continue;
-            for (size_t k = 0; k < taps; ++k) proto[ph + k * phases] /= sum;
-        }
-        const int64_t period = 2 * iabs(p.filter_den) * int64_t(phases) * p.channel_rate;
-        const int64_t num = p.filter_den < 0 ? -p.filter_num : p.filter_num;
-        for (size_t i = 0; i < length; ++i) {
-            const int64_t twice = 2 * int64_t(i) - (int64_t(length) - 1);
-            int64_t turns = (num * twice) % period;
-            if (turns < 0) turns += period;
-            const double ang = kTwoPi * double(turns) / double(period);
-            fine_taps[2 * i] = float(proto[i] * std::cos(ang));
-            fine_taps[2 * i + 1] = float(proto[i] * std::sin(ang));
-        }
-        fine_taps[2 * length] = 0.0f;
-        fine_taps[2 * length + 1] = 0.0f;
+        if (int32_t rc = design_fine_table(p.fine, p.channel_rate, half_width, p.filter_num,
+                                           p.filter_den, kFineAttenuationDb, fine_taps, who))
+            return rc;
     }
     if (weights != nullptr) {
         const uint32_t at = p.demod.audio_taps, dc = p.demod.dc_taps;
@@ -412,36 +462,9 @@ int32_t rvx_fine_block(const rvx_vrx_plan* plan, uint32_t channel_base, uint32_t
                        rvx_vrx_fine_params* out, uint64_t* first_input) {
     static const char* who = "rvx_fine_block";
     if (!plan || !out) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
-    if (plan->channel_rate <= 0 || plan->demod_rate <= 0)
-        return rvx::fail(RVX_EINVAL, "%s: the plan carries no rates", who);
-    if (count == 0) return rvx::fail(RVX_EINVAL, "%s: zero outputs", who);
-    const uint64_t fc = uint64_t(plan->channel_rate), fd = uint64_t(plan->demod_rate);
-    const uint64_t sw = fc / fd, sr = fc % fd;
-    // n0 = floor(j0*Fc/Fd), frac0 = (j0*Fc) mod Fd without forming j0*Fc.
-    const uint64_t whole = first_output / fd, part = first_output % fd;
-    const uint64_t cp = part * sr;
-    const uint64_t n0 = first_output * sw + whole * sr + cp / fd;
-    rvx_vrx_fine_params p{};
-    p.chan_base = channel_base;
-    p.chan_mask = channel_mask;
-    p.in_offset = uint32_t(n0 & channel_mask);
-    p.out_offset = uint32_t(first_output & fine_mask);
-    p.out_mask = fine_mask;
-    p.count = count;
-    p.step_whole = uint32_t(sw);
-    p.step_rem = uint32_t(sr);
-    p.out_rate = uint32_t(fd);
-    p.frac0 = uint32_t(cp % fd);
-    p.inv_out_rate = 1.0f / float(fd);
-    const uint64_t phase = plan->fine_nco_delta * first_output;  // wraps mod 2^64
-    p.nco_phase_high = uint32_t(phase >> 32);
-    p.nco_phase_low = uint32_t(phase);
-    p.nco_delta_high = uint32_t(plan->fine_nco_delta >> 32);
-    p.nco_delta_low = uint32_t(plan->fine_nco_delta);
-    if (int32_t rc = rvx::vrx_fine_valid(plan->fine, p, who)) return rc;
-    *out = p;
-    if (first_input) *first_input = n0;
-    return RVX_OK;
+    return resample_block(plan->fine, plan->channel_rate, plan->demod_rate, plan->fine_nco_delta,
+                          channel_base, channel_mask, fine_mask, first_output, count, out,
+                          first_input, who);
 }
 
 int32_t rvx_demod_block(const rvx_vrx_plan* plan, uint32_t fine_mask, uint64_t first_audio,
@@ -553,6 +576,131 @@ int32_t rvx_carrier_block(const rvx_vrx_plan* plan, uint32_t fine_mask, uint64_t
     if (int32_t rc = rvx::vrx_carrier_valid(plan->carrier, p, who)) return rc;
     *out = p;
     return RVX_OK;
+}
+
+}  // extern "C"
+
+// ---- The display tap (vrx_reference.cpp:2938-3060) ---------------------------
+
+namespace {
+constexpr uint32_t kDisplayTaps = 256;
+constexpr uint32_t kMaxFinePhasesDisplay = 256;
+constexpr double kDisplayAttenuationDb = 80.0;
+constexpr double kDisplayAttenuationCeilingDb = 100.0;
+
+// Kaiser's order estimate read the other way: the depth `taps` reach across
+// a transition given as a fraction of the rate.
+double attenuation_reachable(uint32_t taps, double transition_fraction) {
+    if (taps < 2 || !(transition_fraction > 0.0)) return 0.0;
+    return 2.285 * (kTwoPi * transition_fraction) * double(taps - 1u) + 8.0;
+}
+}  // namespace
+
+extern "C" {
+
+uint32_t rvx_display_max_decimation(void) {
+    // the transition is Fdisp / 2 = Fc / (2R), a falling fraction of the rate
+    uint32_t best = 1;
+    for (uint32_t r = 1; r <= kMaxFinePhasesDisplay; ++r) {
+        if (attenuation_reachable(kDisplayTaps, 1.0 / (2.0 * double(r))) < kDisplayAttenuationDb)
+            break;
+        best = r;
+    }
+    return best;
+}
+
+uint32_t rvx_display_ladder(int64_t channel_rate, uint32_t* out, uint32_t max_rungs) {
+    uint32_t rungs[64];
+    uint32_t n = 0;
+    rungs[n++] = 1;
+    if (channel_rate > 0) {
+        const uint32_t cap = rvx_display_max_decimation();
+        for (;;) {
+            uint32_t next = 0;
+            for (uint32_t d = 2 * rungs[n - 1]; d <= cap; ++d)
+                if (channel_rate % int64_t(d) == 0) {
+                    next = d;
+                    break;
+                }
+            if (next == 0 || n == 64) break;
+            rungs[n++] = next;
+        }
+    }
+    if (out)
+        for (uint32_t i = 0; i < std::min(n, max_rungs); ++i) out[i] = rungs[i];
+    return n;
+}
+
+uint32_t rvx_display_decimation(int64_t channel_rate, int64_t reach_hz) {
+    if (channel_rate <= 0 || reach_hz <= 0) return 1;
+    // Fdisp / 4 >= 2 reach: the pane is at least four reaches across
+    const int64_t wanted = channel_rate / (8 * reach_hz);
+    uint32_t rungs[64];
+    const uint32_t n = rvx_display_ladder(channel_rate, rungs, 64);
+    uint32_t chosen = 1;
+    for (uint32_t i = 0; i < n; ++i)
+        if (int64_t(rungs[i]) <= wanted) chosen = rungs[i];
+    return chosen;
+}
+
+int32_t rvx_plan_vrx_display(const rvx_vrx_plan* plan, rvx_vrx_display_plan* out, float* taps,
+                             uint64_t n_floats) {
+    static const char* who = "rvx_plan_vrx_display";
+    if (!plan || !out) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    const rvx_vrx_plan& p = *plan;
+    if (p.channel_rate <= 0) return rvx::fail(RVX_EINVAL, "%s: the plan carries no channel rate",
+                                              who);
+    if (p.mix_den == 0 || p.placement.residual_den == 0)
+        return rvx::fail(RVX_EINVAL, "%s: the plan carries no mix frequency", who);
+    // the passband in the frame the fine stage mixes to DC: its edges are about
+    // the tuned frequency, (residual - mix) above the mix (the pitch for CW)
+    const double shift = double(p.placement.residual_num) / double(p.placement.residual_den) -
+                         double(p.mix_num) / double(p.mix_den);
+    const double reach = std::max(std::abs(double(p.passband_low) + shift),
+                                  std::abs(double(p.passband_high) + shift));
+    rvx_vrx_display_plan d{};
+    d.channel_rate = p.channel_rate;
+    d.reach_hz = std::max<int64_t>(1, int64_t(std::ceil(reach)));
+    d.decimation = rvx_display_decimation(p.channel_rate, d.reach_hz);
+    d.display_rate = p.channel_rate / int64_t(d.decimation);
+    d.fine.taps = kDisplayTaps;
+    d.fine.phases = 1;
+    d.fine.nco_log2 = p.fine.nco_log2;
+    d.group_delay_channel_samples = (double(kDisplayTaps) - 1.0) / 2.0;
+    // flat to the pane's edge Fdisp / 4, stopband by 3 Fdisp / 4 (the nearest
+    // fold into the pane), or Fc / 2 at R = 1
+    const double fc = double(p.channel_rate), fdisp = double(d.display_rate);
+    d.pass_hz = 0.25 * fdisp;
+    d.stop_hz = std::min(0.75 * fdisp, 0.5 * fc);
+    const double reachable = attenuation_reachable(kDisplayTaps, (d.stop_hz - d.pass_hz) / fc);
+    if (reachable < kDisplayAttenuationDb)
+        return rvx::fail(RVX_EINVAL, "%s: %u taps reach %.1f dB across %.0f to %.0f Hz at %lld "
+                         "S/s, short of %.0f dB", who, kDisplayTaps, reachable, d.pass_hz,
+                         d.stop_hz, (long long)p.channel_rate, kDisplayAttenuationDb);
+    d.attenuation_db = std::min(kDisplayAttenuationCeilingDb, reachable);
+    if (int32_t rc = rvx_nco_delta(p.mix_num, p.mix_den, d.display_rate, &d.nco_delta)) return rc;
+    if (taps) {
+        if (n_floats != 2ull * (kDisplayTaps + 1))
+            return rvx::fail(RVX_EINVAL, "%s: the display table wants %u floats", who,
+                             2 * (kDisplayTaps + 1));
+        // the cutoff midway between the edges
+        const int64_t half_width = int64_t(0.5 * (d.pass_hz + d.stop_hz));
+        if (int32_t rc = design_fine_table(d.fine, p.channel_rate, half_width, p.mix_num,
+                                           p.mix_den, d.attenuation_db, taps, who))
+            return rc;
+    }
+    *out = d;
+    return RVX_OK;
+}
+
+int32_t rvx_display_block(const rvx_vrx_display_plan* plan, uint32_t channel_base,
+                          uint32_t channel_mask, uint32_t display_mask, uint64_t first_output,
+                          uint32_t count, rvx_vrx_fine_params* out, uint64_t* first_input) {
+    static const char* who = "rvx_display_block";
+    if (!plan || !out) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    return resample_block(plan->fine, plan->channel_rate, plan->display_rate, plan->nco_delta,
+                          channel_base, channel_mask, display_mask, first_output, count, out,
+                          first_input, who);
 }
 
 }  // extern "C"

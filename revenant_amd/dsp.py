@@ -389,3 +389,50 @@ def spectral_params(strength: float) -> _lib.SpectralParams:
 def spectral_check(frame: int, params: _lib.SpectralParams) -> None:
     _lib.check(_lib.lib().rvx_spectral_check(frame, C.byref(params)))
 
+
+# ---------------------------------------------------------------------------
+# The display tap (core/dsp/vrx_reference.h:1639-1768)
+# ---------------------------------------------------------------------------
+
+DISPLAY_TAPS = 256
+
+
+def display_max_decimation() -> int:
+    return int(_lib.lib().rvx_display_max_decimation())
+
+
+def display_ladder(channel_rate: int) -> list[int]:
+    out = (C.c_uint32 * 64)()
+    n = _lib.lib().rvx_display_ladder(channel_rate, out, 64)
+    return list(out[:n])
+
+
+def display_decimation(channel_rate: int, reach_hz: int) -> int:
+    return int(_lib.lib().rvx_display_decimation(channel_rate, reach_hz))
+
+
+@dataclass
+class VrxDisplay:
+    """A receiver's display tap: the plan block and its 256-tap table."""
+
+    plan: _lib.VrxDisplayPlan
+    taps: np.ndarray  # complex64, DISPLAY_TAPS + 1
+
+    def block(self, channel_base: int, channel_mask: int, display_mask: int,
+              first_output: int, count: int) -> tuple[_lib.VrxFineParams, int]:
+        out = _lib.VrxFineParams()
+        first_input = C.c_uint64()
+        _lib.check(_lib.lib().rvx_display_block(C.byref(self.plan), channel_base, channel_mask,
+                                                display_mask, first_output, count, C.byref(out),
+                                                C.byref(first_input)))
+        return out, first_input.value
+
+
+def plan_vrx_display(plan: VrxPlan) -> VrxDisplay:
+    """vrx_reference.cpp:2988 plan_vrx_display."""
+    out = _lib.VrxDisplayPlan()
+    taps = np.zeros(2 * (DISPLAY_TAPS + 1), np.float32)
+    _lib.check(_lib.lib().rvx_plan_vrx_display(C.byref(plan.plan), C.byref(out),
+                                               taps.ctypes.data_as(C.c_void_p), taps.size))
+    return VrxDisplay(out, taps.view(np.complex64))
+
