@@ -311,6 +311,7 @@ def blanker_check(config: _lib.BlankerConfig, params: _lib.BlankerParams, pass_:
 LINE_NOTCH, LINE_ALE, LINE_GUARD = 1, 2, 4
 LINE_NOTCH_COUNT_SHIFT = 8
 MAX_NOTCHES = 4
+MAX_LINE_STRIDE = 8
 LINE_STATE_POSITION, LINE_STATE_GUARD, LINE_STATE_MORE, LINE_STATE_WEIGHTS = 4, 8, 12, 24
 CW_GUARD_MIN_HZ, CW_GUARD_DEPTH_DB = 50.0, 40.0
 
