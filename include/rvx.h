@@ -885,7 +885,7 @@ int32_t rvx_comb_subtract(rvx_ctx* ctx, const rvx_comb_params* params, const voi
 #define RVX_BLANK_DETECT 0
 #define RVX_BLANK_APPLY 1
 
-/* noise_reference.h:131-140 BlankerConfig */
+/* noise_reference.h:117 BlankerConfig */
 typedef struct rvx_blanker_config {
     uint32_t pass;
     uint32_t window;     /* reference window, 16..256 samples */
@@ -893,7 +893,7 @@ typedef struct rvx_blanker_config {
     uint32_t hang, lead; /* blanked after the last / before the first flag */
 } rvx_blanker_config;
 
-/* noise_reference.h:160-175 BlankerParams (seven words) */
+/* noise_reference.h:152 BlankerParams (seven words) */
 typedef struct rvx_blanker_params {
     uint32_t chan_base, chan_mask, chan_first; /* the channel ring, as vrx_fine reads it */
     uint32_t out_mask, out_first;              /* the receiver's flag / blanked rings */
@@ -919,7 +919,7 @@ int32_t rvx_blank_apply(rvx_ctx* ctx, const rvx_blanker_config* config,
                         void* blanked, uint64_t blanked_samples);
 
 /* ---- The line kernel: manual notch cascade and adaptive line enhancer ----------
- * core/dsp/noise_reference.h:190-418: per receiver, over its mono audio, a
+ * core/dsp/noise_reference.h:190-420: per receiver, over its mono audio, a
  * cascade of up to four notch sections, then an NLMS predictor (taps split
  * over `lanes` partial sums, summed in lane order) whose error is the output,
  * optionally with a guard section on the subtraction (CW). */
@@ -930,13 +930,13 @@ int32_t rvx_blank_apply(rvx_ctx* ctx, const rvx_blanker_config* config,
 #define RVX_LINE_NOTCH_COUNT_MASK (3u << RVX_LINE_NOTCH_COUNT_SHIFT)
 #define RVX_MAX_NOTCHES 4
 #define RVX_MAX_LINE_STRIDE 8
-/* state layout, floats (noise_reference.h:306-318) */
+/* state layout, floats (noise_reference.h:318-333) */
 #define RVX_LINE_STATE_POSITION 4
 #define RVX_LINE_STATE_GUARD 8
 #define RVX_LINE_STATE_MORE 12
 #define RVX_LINE_STATE_WEIGHTS 24
 
-/* noise_reference.h:296-304 LineConfig. Here lanes <= 64 (one wavefront) and
+/* noise_reference.h:277 LineConfig. Here lanes <= 64 (one wavefront) and
  * taps + history <= 16128 floats (the workgroup's LDS copy). */
 typedef struct rvx_line_config {
     uint32_t lanes, taps, history;
@@ -946,7 +946,7 @@ typedef struct rvx_line_section {
     float b0, b1, b2, a1, a2; /* y = b0 x + b1 x1 + b2 x2 - a1 y1 - a2 y2 */
 } rvx_line_section;
 
-/* noise_reference.h:330-366 LineParams (thirty-two words) */
+/* noise_reference.h:346 LineParams (thirty-two words) */
 typedef struct rvx_line_params {
     uint32_t count, flags, stride, delay;
     float b0, b1, b2, a1, a2;
@@ -964,15 +964,15 @@ typedef struct rvx_noise_line_item {
     float* state;
 } rvx_noise_line_item;
 
-/* noise_reference.h:396 NotchDesign, as five doubles b0 b1 b2 a1 a2. */
+/* noise_reference.h:395 NotchDesign, as five doubles b0 b1 b2 a1 a2. */
 int32_t rvx_design_notch(double audio_hz, double depth_db, double width_hz, int64_t audio_rate,
                          double* out5);
-/* :409 notch_section_db: the power response of the float-rounded section. */
+/* :410 notch_section_db: the power response of the float-rounded section. */
 double rvx_notch_section_db(const double* design5, double audio_hz, int64_t audio_rate);
 uint64_t rvx_line_state_size(const rvx_line_config* config);
-/* :371 validate(LineConfig, LineParams), plus this kernel's lane and LDS limits. */
+/* :390 validate(LineConfig, LineParams), plus this kernel's lane and LDS limits. */
 int32_t rvx_line_check(const rvx_line_config* config, const rvx_line_params* params);
-/* core/shaders/noise_line.comp; twin noise_reference.cpp:262 reference_line.
+/* core/shaders/noise_line.comp; twin noise_reference.cpp:266 reference_line.
  * audio: count floats processed in place; state: rvx_line_state_size floats. */
 int32_t rvx_noise_line(rvx_ctx* ctx, const rvx_line_config* config, const rvx_line_params* params,
                        float* audio, uint64_t n_audio, float* state, uint64_t n_state);
@@ -992,7 +992,7 @@ int32_t rvx_noise_line_batch(rvx_ctx* ctx, const rvx_line_config* config,
 #define RVX_MIN_SPECTRAL_FRAME 64
 #define RVX_MAX_SPECTRAL_FRAME 512
 
-/* noise_reference.h:489-507 SpectralParams (ten words) */
+/* noise_reference.h:495 SpectralParams (ten words) */
 typedef struct rvx_spectral_params {
     uint32_t count;
     float smoothing, smoothing_complement;
@@ -1007,15 +1007,15 @@ typedef struct rvx_noise_spectral_item {
     float* state;
 } rvx_noise_spectral_item; /* 64 bytes */
 
-/* :523 spectral_config_for: the frame for an audio rate. */
+/* :535 spectral_config_for: the frame for an audio rate. */
 uint32_t rvx_spectral_frame_for(int64_t audio_rate);
 uint64_t rvx_spectral_state_size(uint32_t frame);
-/* :529 design_spectral_tables: 4 * frame floats. */
+/* :540 design_spectral_tables: 4 * frame floats. */
 int32_t rvx_design_spectral_tables(uint32_t frame, float* out, uint64_t n_floats);
-/* :532 spectral_params(strength), count left at zero. */
+/* :543 spectral_params(strength), count left at zero. */
 int32_t rvx_spectral_params_for(double strength, rvx_spectral_params* out);
 int32_t rvx_spectral_check(uint32_t frame, const rvx_spectral_params* params);
-/* core/shaders/noise_spectral.comp; twin noise_reference.cpp:600
+/* core/shaders/noise_spectral.comp; twin noise_reference.cpp:568
  * reference_spectral. tables: the designed 4 * frame floats on the device. */
 int32_t rvx_noise_spectral(rvx_ctx* ctx, uint32_t frame, const rvx_spectral_params* params,
                            const float* tables, uint64_t n_tables, float* audio, uint64_t n_audio,

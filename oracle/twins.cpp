@@ -1179,10 +1179,10 @@ int oracle_comb_subtract(const OCombParams* p, const float* raw, const OCombLine
 // (tests/reference/test_noise.cpp:270-322).
 // ---------------------------------------------------------------------------
 
-struct OBlankerConfig {  // core/dsp/noise_reference.h:131-140
+struct OBlankerConfig {  // core/dsp/noise_reference.h:117
     uint32_t pass, window, guard, hang, lead;
 };
-struct OBlankerParams {  // :160-175
+struct OBlankerParams {  // :152
     uint32_t chan_base, chan_mask, chan_first, out_mask, out_first, count;
     float threshold;
 };
@@ -1226,7 +1226,7 @@ int oracle_blank_apply(const OBlankerConfig* c, const OBlankerParams* p, const f
 }
 
 // ---------------------------------------------------------------------------
-// The line kernel. core/dsp/noise_reference.cpp:262-421: a cascade of notch
+// The line kernel. core/dsp/noise_reference.cpp:266-421: a cascade of notch
 // sections (y = b0 x + b1 x1 + b2 x2 - a1 y1 - a2 y2, summed in that order),
 // then a normalized leaky LMS predictor over the history ring, its taps split
 // over `lanes` partial sums (lane l: taps l, l + lanes, ..., ascending) added
@@ -1348,7 +1348,7 @@ int oracle_noise_line(const OLineConfig* c, const OLineParams* p, float* audio, 
 }
 
 // ---------------------------------------------------------------------------
-// Spectral noise reduction. core/dsp/noise_reference.cpp:600-735: hop H = N/2
+// Spectral noise reduction. core/dsp/noise_reference.cpp:568-700: hop H = N/2
 // of input into the history while the queued output goes out; at a full hop
 // the analysis window, a direct DFT per bin (ascending time, tables indexed
 // (k t) mod N, im accumulated by subtraction), power smoothing and Doblinger's

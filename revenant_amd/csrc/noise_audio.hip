@@ -1,10 +1,10 @@
 // The per-receiver audio noise stages: the line kernel (manual notch cascade
 // and adaptive line enhancer) and spectral noise reduction.
 //
-//   rvx_noise_line       noise_line.comp / noise_reference.cpp:262-421
-//   rvx_noise_spectral   noise_spectral.comp / :600-735
+//   rvx_noise_line       noise_line.comp / noise_reference.cpp:231-421
+//   rvx_noise_spectral   noise_spectral.comp / :556-700
 //   rvx_design_notch, rvx_notch_section_db      :424-478
-//   rvx_spectral_frame_for, rvx_design_spectral_tables, rvx_spectral_params_for  :492-558
+//   rvx_spectral_frame_for, rvx_design_spectral_tables, rvx_spectral_params_for  :493-553
 //
 // Both are recursions over one receiver's audio, so a receiver is one
 // workgroup and a batch launch runs many receivers side by side.
@@ -42,7 +42,7 @@ constexpr uint32_t kMaxLineLds = 16384 - kChunk;
 constexpr uint32_t kMoreSections = RVX_MAX_NOTCHES - 1;
 constexpr double kPi = 3.141592653589793238462643383279502884;
 
-// noise_reference.cpp:61-72, the spectral stage's fixed figures
+// noise_reference.cpp:67-80, the spectral stage's fixed figures
 constexpr double kSpectralSmoothing = 0.7;
 constexpr double kSpectralGamma = 0.998;
 constexpr double kSpectralBeta = 0.96;

@@ -317,7 +317,7 @@ CW_GUARD_MIN_HZ, CW_GUARD_DEPTH_DB = 50.0, 40.0
 
 
 def line_notch_flags(sections: int) -> int:
-    """noise_reference.h:280 line_notch_flags."""
+    """noise_reference.h:303 line_notch_flags."""
     return 0 if sections == 0 else LINE_NOTCH | ((sections - 1) << LINE_NOTCH_COUNT_SHIFT)
 
 
@@ -364,7 +364,7 @@ def line_check(config: _lib.LineConfig, params: _lib.LineParams) -> None:
 
 
 def spectral_frame_for(audio_rate: int) -> int:
-    """noise_reference.cpp:492 spectral_config_for."""
+    """noise_reference.cpp:493 spectral_config_for."""
     return int(_lib.lib().rvx_spectral_frame_for(audio_rate))
 
 
@@ -373,7 +373,7 @@ def spectral_state_size(frame: int) -> int:
 
 
 def spectral_tables(frame: int) -> np.ndarray:
-    """noise_reference.cpp:502 design_spectral_tables: cos, sin, analysis and
+    """noise_reference.cpp:503 design_spectral_tables: cos, sin, analysis and
     synthesis (1/N folded in) windows, frame floats each."""
     out = np.zeros(4 * frame, np.float32)
     _lib.check(_lib.lib().rvx_design_spectral_tables(frame, out.ctypes.data_as(C.c_void_p),
