@@ -1156,6 +1156,15 @@ int32_t rvx_engine_add_vrx(rvx_engine* engine, const rvx_vrx_plan* plan, const f
                            uint64_t n_fine_floats, const float* weights, uint64_t n_weights,
                            uint32_t audio_capacity, uint32_t tile, uint32_t* id);
 int32_t rvx_engine_remove_vrx(rvx_engine* engine, uint32_t id);
+/* A mono receiver's audio noise stages (the reference's per-receiver stage,
+ * noise_reference.h:190-520, recorded after its demodulator): the line
+ * kernel with line_config / line (count ignored) and spectral reduction at
+ * spectral_frame with spectral (count ignored), each NULL for off. Every push
+ * then runs them in place on the audio it produced, all receivers in batched
+ * launches. A call restarts both stages' state (it waits for the stream). */
+int32_t rvx_engine_set_noise(rvx_engine* engine, uint32_t id, const rvx_line_config* line_config,
+                             const rvx_line_params* line, uint32_t spectral_frame,
+                             const rvx_spectral_params* spectral);
 /* codes: `samples` native samples, host memory (on_device = 0) or device. */
 int32_t rvx_engine_push(rvx_engine* engine, const void* codes, uint64_t samples,
                         int32_t on_device);

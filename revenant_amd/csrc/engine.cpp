@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <tuple>
 #include <vector>
 
@@ -198,14 +199,27 @@ struct EngineRx {
     uint32_t* cstate = nullptr;  // the loop's RVX_CARRIER_STATE_WORDS
     bool fine_started = false;
     uint64_t next_fine = 0;      // first fine sample not yet produced and rotated
+    // audio noise stages (mono receivers): the line kernel, then spectral
+    // reduction, in place on the audio each push produces
+    bool line_on = false, spectral_on = false;
+    rvx_line_config line_cfg{};
+    rvx_line_params line{};
+    float* line_state = nullptr;
+    uint32_t spec_frame = 0;
+    rvx_spectral_params spec{};
+    float* spec_state = nullptr;
+    float* spec_tables = nullptr;
+    uint64_t pushed_a0 = 0, pushed_n = 0;  // the audio this push produced
 };
 
 void free_rx(EngineRx& r) {
-    for (void* q : {r.taps, (void*)r.weights, (void*)r.audio, r.fine, (void*)r.cstate})
+    for (void* q : {r.taps, (void*)r.weights, (void*)r.audio, r.fine, (void*)r.cstate,
+                    (void*)r.line_state, (void*)r.spec_state, (void*)r.spec_tables})
         if (q) (void)hipFree(q);
     r.taps = r.fine = nullptr;
     r.weights = r.audio = nullptr;
     r.cstate = nullptr;
+    r.line_state = r.spec_state = r.spec_tables = nullptr;
 }
 
 template <class T>
@@ -255,6 +269,10 @@ struct rvx_engine {
     rvx_vrx_carrier_item* items_h = nullptr;  // carrier loops of one push
     void* items_d = nullptr;
     uint32_t items_cap = 0;
+    uint8_t* noise_h = nullptr;  // line and spectral items of one push
+    void* noise_d = nullptr;
+    uint64_t noise_cap = 0;      // bytes
+    hipEvent_t noise_staged = nullptr;
     // front-end correction (calibration on)
     static constexpr int kInFlight = 3;
     rvx_front_end* fe = nullptr;
@@ -281,6 +299,9 @@ struct rvx_engine {
         if (descs_h) (void)hipHostFree(descs_h);
         if (work_h) (void)hipHostFree(work_h);
         if (items_h) (void)hipHostFree(items_h);
+        if (noise_h) (void)hipHostFree(noise_h);
+        if (noise_d) (void)hipFree(noise_d);
+        if (noise_staged) (void)hipEventDestroy(noise_staged);
         if (staged) (void)hipEventDestroy(staged);
         for (int k = 0; k < kInFlight; ++k) {
             if (sums_d[k]) (void)hipFree(sums_d[k]);
@@ -441,6 +462,9 @@ int32_t front_end_push(rvx_engine& e, uint64_t push_first, uint64_t samples, con
     return RVX_OK;
 }
 
+int32_t batch_receivers(rvx_engine& e, const std::vector<rvx_vrx_span>& spans, const char* who);
+int32_t noise_stages(rvx_engine& e, const char* who);
+
 int32_t engine_receivers(rvx_engine& e, const char* who) {
     const uint64_t newest = e.next_block - 1;
     const uint64_t ring = e.cfg.ring_blocks;
@@ -448,6 +472,7 @@ int32_t engine_receivers(rvx_engine& e, const char* who) {
     std::vector<rvx_vrx_span> spans;
     std::vector<CarrierJob> jobs;
     std::vector<rvx_vrx_carrier_item> items;
+    for (auto& r : e.rx) r.pushed_n = 0;
     for (auto& r : e.rx) {
         uint64_t a_lo = 0, n = 0;
         audio_range(r.plan, oldest, newest, &a_lo, &n);
@@ -468,6 +493,8 @@ int32_t engine_receivers(rvx_engine& e, const char* who) {
             r.next_audio += count - r.cap;
             count = r.cap;
         }
+        r.pushed_a0 = r.next_audio;
+        r.pushed_n = count;
         if (r.carrier) {
             rvx_vrx_carrier_item it{};
             if (int32_t rc = carrier_fine(e, r, r.next_audio, count, &it)) return rc;
@@ -501,6 +528,11 @@ int32_t engine_receivers(rvx_engine& e, const char* who) {
         if (r.next_audio - r.read_audio > r.cap) r.read_audio = r.next_audio - r.cap;
     }
     if (int32_t rc = carrier_receivers(e, jobs, items, who)) return rc;
+    if (int32_t rc = batch_receivers(e, spans, who)) return rc;
+    return noise_stages(e, who);
+}
+
+int32_t batch_receivers(rvx_engine& e, const std::vector<rvx_vrx_span>& spans, const char* who) {
     if (spans.empty()) return RVX_OK;
     uint32_t nd = 0, nw = 0;
     if (int32_t rc = rvx_vrx_batch_plan(spans.data(), uint32_t(spans.size()), nullptr, 0, &nd,
@@ -548,6 +580,112 @@ int32_t engine_receivers(rvx_engine& e, const char* who) {
     return rvx_vrx_batch(e.ctx, static_cast<const rvx_vrx_desc*>(e.descs_d), nd,
                          static_cast<const rvx_vrx_work*>(e.work_d), nw, e.counter, e.nco,
                          e.nco_entries, 0);
+}
+
+// The audio noise stages of one push (the reference records them after the
+// demodulator, noise_reference.h:190-520): every receiver's line kernel over
+// the audio this push produced, then every receiver's spectral reduction, in
+// place in the audio ring. Audio that wraps the ring is two pieces of one
+// receiver, which must run in order, so the second pieces go in a second
+// launch. Spectral items are launched per frame size.
+int32_t noise_stages(rvx_engine& e, const char* who) {
+    // launch groups: the line kernel per config, spectral reduction per frame
+    using LineKey = std::tuple<uint32_t, uint32_t, uint32_t>;
+    struct SpecGroup {
+        const float* tables = nullptr;
+        std::vector<rvx_noise_spectral_item> items;
+    };
+    std::map<LineKey, std::pair<rvx_line_config, std::vector<rvx_noise_line_item>>> line[2];
+    std::map<uint32_t, SpecGroup> spec[2];
+    for (auto& r : e.rx) {
+        if (!r.pushed_n || (!r.line_on && !r.spectral_on)) continue;
+        const uint64_t pos = r.pushed_a0 % r.cap;
+        const uint64_t first = std::min(r.pushed_n, r.cap - pos);
+        const uint64_t piece[2] = {first, r.pushed_n - first};
+        float* at[2] = {r.audio + pos, r.audio};
+        for (int k = 0; k < 2; ++k) {
+            if (!piece[k]) continue;
+            if (r.line_on) {
+                rvx_noise_line_item it{};
+                it.params = r.line;
+                it.params.count = uint32_t(piece[k]);
+                it.audio = at[k];
+                it.state = r.line_state;
+                auto& g = line[k][LineKey{r.line_cfg.lanes, r.line_cfg.taps, r.line_cfg.history}];
+                g.first = r.line_cfg;
+                g.second.push_back(it);
+            }
+            if (r.spectral_on) {
+                rvx_noise_spectral_item it{};
+                it.params = r.spec;
+                it.params.count = uint32_t(piece[k]);
+                it.audio = at[k];
+                it.state = r.spec_state;
+                SpecGroup& g = spec[k][r.spec_frame];
+                g.tables = r.spec_tables;
+                g.items.push_back(it);
+            }
+        }
+    }
+    struct Launch {
+        const rvx_line_config* line_cfg;  // null: spectral
+        uint32_t frame;
+        const float* tables;
+        uint64_t offset;
+        uint32_t n;
+    };
+    std::vector<Launch> launches;
+    uint64_t bytes = 0;
+    for (int k = 0; k < 2; ++k) {
+        for (auto& g : line[k]) bytes += g.second.second.size() * sizeof(rvx_noise_line_item);
+        for (auto& g : spec[k]) bytes += g.second.items.size() * sizeof(rvx_noise_spectral_item);
+    }
+    if (bytes == 0) return RVX_OK;
+    RVX_TRY_HIP(hipEventSynchronize(e.noise_staged), who);
+    if (bytes > e.noise_cap) {
+        RVX_TRY_HIP(hipStreamSynchronize(e.ctx->stream), who);
+        (void)hipHostFree(e.noise_h);
+        (void)hipFree(e.noise_d);
+        e.noise_h = nullptr;
+        e.noise_d = nullptr;
+        e.noise_cap = std::max(bytes, 2 * e.noise_cap);
+        RVX_TRY_HIP(hipHostMalloc((void**)&e.noise_h, e.noise_cap, hipHostMallocDefault), who);
+        if (int32_t rc = device_alloc(&e.noise_d, e.noise_cap, who)) return rc;
+    }
+    // order: every line piece (first pieces, then the wrapped remainders),
+    // then every spectral piece the same way; a receiver's stages and pieces
+    // therefore run in stream order
+    uint64_t off = 0;
+    for (int k = 0; k < 2; ++k)
+        for (auto& g : line[k]) {
+            const auto& items = g.second.second;
+            std::memcpy(e.noise_h + off, items.data(), items.size() * sizeof(rvx_noise_line_item));
+            launches.push_back({&g.second.first, 0, nullptr, off, uint32_t(items.size())});
+            off += items.size() * sizeof(rvx_noise_line_item);
+        }
+    for (int k = 0; k < 2; ++k)
+        for (auto& g : spec[k]) {
+            const auto& items = g.second.items;
+            std::memcpy(e.noise_h + off, items.data(),
+                        items.size() * sizeof(rvx_noise_spectral_item));
+            launches.push_back({nullptr, g.first, g.second.tables, off, uint32_t(items.size())});
+            off += items.size() * sizeof(rvx_noise_spectral_item);
+        }
+    RVX_TRY_HIP(hipMemcpyAsync(e.noise_d, e.noise_h, off, hipMemcpyHostToDevice, e.ctx->stream),
+                who);
+    RVX_TRY_HIP(hipEventRecord(e.noise_staged, e.ctx->stream), who);
+    for (const Launch& l : launches) {
+        const uint8_t* base = static_cast<const uint8_t*>(e.noise_d) + l.offset;
+        const int32_t rc =
+            l.line_cfg
+                ? rvx_noise_line_batch(e.ctx, l.line_cfg,
+                                       reinterpret_cast<const rvx_noise_line_item*>(base), l.n)
+                : rvx_noise_spectral_batch(e.ctx, l.frame, l.tables, 4ull * l.frame,
+                                           reinterpret_cast<const rvx_noise_spectral_item*>(base),
+                                           l.n);
+        if (rc) return rc;
+    }
+    return RVX_OK;
 }
 
 }  // namespace
@@ -659,6 +797,8 @@ int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const f
     if (err == hipSuccess) err = hipEventRecord(e->staged, s);
     if (err == hipSuccess) err = hipEventCreateWithFlags(&e->pushed, hipEventDisableTiming);
     if (err == hipSuccess) err = hipEventRecord(e->pushed, s);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&e->noise_staged, hipEventDisableTiming);
+    if (err == hipSuccess) err = hipEventRecord(e->noise_staged, s);
     if (err == hipSuccess)
         err = hipHostMalloc((void**)&e->stage_h, uint64_t(c.max_push) * e->bps,
                             hipHostMallocDefault);
@@ -764,6 +904,60 @@ int32_t rvx_engine_remove_vrx(rvx_engine* e, uint32_t id) {
     RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
     free_rx(*it);
     e->rx.erase(it);
+    return RVX_OK;
+}
+
+int32_t rvx_engine_set_noise(rvx_engine* e, uint32_t id, const rvx_line_config* line_config,
+                             const rvx_line_params* line, uint32_t spectral_frame,
+                             const rvx_spectral_params* spectral) {
+    static const char* who = "rvx_engine_set_noise";
+    if (!e) return rvx::fail(RVX_EINVAL, "%s: null engine", who);
+    RVX_ENTER(e->ctx, who);
+    auto it = std::find_if(e->rx.begin(), e->rx.end(), [&](const EngineRx& r) { return r.id == id; });
+    if (it == e->rx.end()) return rvx::fail(RVX_EINVAL, "%s: no receiver %u", who, id);
+    EngineRx& r = *it;
+    if ((line || spectral) && r.plan.demod.channels != 1)
+        return rvx::fail(RVX_EINVAL, "%s: receiver %u hands out %u channels; the audio stages "
+                         "run on one channel of audio", who, id, r.plan.demod.channels);
+    if (line) {
+        if (!line_config) return rvx::fail(RVX_EINVAL, "%s: a line kernel needs its config", who);
+        if (int32_t rc = rvx_line_check(line_config, line)) return rc;
+    }
+    if (spectral) {
+        if (int32_t rc = rvx_spectral_check(spectral_frame, spectral)) return rc;
+    }
+    // the stages' state restarts: wait for the launches that use the old one
+    RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
+    for (float** q : {&r.line_state, &r.spec_state, &r.spec_tables})
+        if (*q) {
+            (void)hipFree(*q);
+            *q = nullptr;
+        }
+    r.line_on = r.spectral_on = false;
+    if (line) {
+        const uint64_t n = rvx_line_state_size(line_config);
+        if (int32_t rc = device_alloc(&r.line_state, n * sizeof(float), who)) return rc;
+        RVX_TRY_HIP(hipMemset(r.line_state, 0, n * sizeof(float)), who);
+        r.line_cfg = *line_config;
+        r.line = *line;
+        r.line_on = true;
+    }
+    if (spectral) {
+        const uint64_t n = rvx_spectral_state_size(spectral_frame);
+        std::vector<float> tables(4ull * spectral_frame);
+        if (int32_t rc = rvx_design_spectral_tables(spectral_frame, tables.data(), tables.size()))
+            return rc;
+        if (int32_t rc = device_alloc(&r.spec_state, n * sizeof(float), who)) return rc;
+        if (int32_t rc = device_alloc(&r.spec_tables, tables.size() * sizeof(float), who))
+            return rc;
+        RVX_TRY_HIP(hipMemset(r.spec_state, 0, n * sizeof(float)), who);
+        RVX_TRY_HIP(hipMemcpy(r.spec_tables, tables.data(), tables.size() * sizeof(float),
+                              hipMemcpyHostToDevice),
+                    who);
+        r.spec_frame = spectral_frame;
+        r.spec = *spectral;
+        r.spectral_on = true;
+    }
     return RVX_OK;
 }
 

@@ -99,6 +99,28 @@ class Engine:
         except Exception:
             pass
 
+    def set_noise(self, rid: int, line_config: _lib.LineConfig | None = None,
+                  line_params: _lib.LineParams | None = None, spectral_frame: int = 0,
+                  spectral_params: _lib.SpectralParams | None = None) -> None:
+        """A mono receiver's audio noise stages (rvx_engine_set_noise): the
+        line kernel and / or spectral reduction, run on every push's audio;
+        None turns a stage off. Restarts both stages' state."""
+        if line_params is not None and line_config is None:
+            line_config = _lib.LineConfig()
+        _lib.check(self.lib.rvx_engine_set_noise(
+            self.handle, rid, C.byref(line_config) if line_config is not None else None,
+            C.byref(line_params) if line_params is not None else None, spectral_frame,
+            C.byref(spectral_params) if spectral_params is not None else None))
+
+    def set_noise_plan(self, rid: int, plan, line_config: _lib.LineConfig | None = None,
+                       spectral_frame: int = 0) -> None:
+        """From revenant_amd.noise.plan_noise's NoisePlan (the blanker runs on
+        the channel, outside this call)."""
+        self.set_noise(rid, line_config if plan.line else None,
+                       plan.line_params if plan.line else None,
+                       spectral_frame if plan.spectral else 0,
+                       plan.spectral_params if plan.spectral else None)
+
     def front_end(self) -> tuple[_lib.FrontEndEstimate, _lib.IqCorrectParams]:
         """The front-end estimate and the correction the newest push took."""
         est, applied = _lib.FrontEndEstimate(), _lib.IqCorrectParams()
