@@ -1162,6 +1162,15 @@ int32_t rvx_engine_remove_vrx(rvx_engine* engine, uint32_t id);
  * spectral_frame with spectral (count ignored), each NULL for off. Every push
  * then runs them in place on the audio it produced, all receivers in batched
  * launches. A call restarts both stages' state (it waits for the stream). */
+/* The receiver's impulse blanker on its coarse channel (noise_reference.h:
+ * 59-190, recorded before the fine stage): config (NULL for off; pass is
+ * ignored) and the threshold as a power ratio (rvx_blanker_threshold). Each
+ * push detects through the newest channel sample and applies through `lead`
+ * before it; the receiver's fine stage reads the blanked samples, so its audio
+ * runs lead channel samples behind. Enabling restarts the receiver's audio
+ * cursor from the blanked stream. */
+int32_t rvx_engine_set_blanker(rvx_engine* engine, uint32_t id, const rvx_blanker_config* config,
+                               float threshold);
 int32_t rvx_engine_set_noise(rvx_engine* engine, uint32_t id, const rvx_line_config* line_config,
                              const rvx_line_params* line, uint32_t spectral_frame,
                              const rvx_spectral_params* spectral);

@@ -210,12 +210,25 @@ struct EngineRx {
     float* spec_state = nullptr;
     float* spec_tables = nullptr;
     uint64_t pushed_a0 = 0, pushed_n = 0;  // the audio this push produced
+    // the impulse blanker on the receiver's coarse channel: the fine stage
+    // then reads the blanked ring (same slots as the channel ring), which
+    // runs `lead` channel samples behind the channel
+    bool blank_on = false;
+    rvx_blanker_config blank_cfg{};
+    float blank_threshold = 0.0f;
+    float* flags = nullptr;   // ring_blocks floats
+    void* blanked = nullptr;  // ring_blocks cf32
+    bool blank_started = false;
+    uint64_t detect_next = 0, apply_next = 0, apply_first = 0;
 };
 
 void free_rx(EngineRx& r) {
     for (void* q : {r.taps, (void*)r.weights, (void*)r.audio, r.fine, (void*)r.cstate,
-                    (void*)r.line_state, (void*)r.spec_state, (void*)r.spec_tables})
+                    (void*)r.line_state, (void*)r.spec_state, (void*)r.spec_tables,
+                    (void*)r.flags, r.blanked})
         if (q) (void)hipFree(q);
+    r.flags = nullptr;
+    r.blanked = nullptr;
     r.taps = r.fine = nullptr;
     r.weights = r.audio = nullptr;
     r.cstate = nullptr;
@@ -342,11 +355,13 @@ int32_t carrier_fine(rvx_engine& e, EngineRx& r, uint64_t a0, uint64_t count,
         const uint32_t n = uint32_t(std::min<uint64_t>(last + 1 - at, limit));
         rvx_vrx_fine_params fp{};
         uint64_t first_input = 0;
-        if (int32_t rc = rvx_fine_block(&r.plan, r.plan.placement.channel * e.cfg.ring_blocks,
+        const bool b = r.blank_on;
+        if (int32_t rc = rvx_fine_block(&r.plan, b ? 0u : r.plan.placement.channel * e.cfg.ring_blocks,
                                         e.cfg.ring_blocks - 1, mask, at, n, &fp, &first_input))
             return rc;
-        if (int32_t rc = rvx_vrx_fine(e.ctx, &r.plan.fine, &fp, e.ring,
-                                      uint64_t(e.cfg.channels) * e.cfg.ring_blocks, r.taps, n_taps,
+        if (int32_t rc = rvx_vrx_fine(e.ctx, &r.plan.fine, &fp, b ? r.blanked : e.ring,
+                                      uint64_t(b ? 1u : e.cfg.channels) * e.cfg.ring_blocks,
+                                      r.taps, n_taps,
                                       e.nco, e.nco_entries, r.fine, r.fine_cap))
             return rc;
         at += n;
@@ -465,6 +480,56 @@ int32_t front_end_push(rvx_engine& e, uint64_t push_first, uint64_t samples, con
 int32_t batch_receivers(rvx_engine& e, const std::vector<rvx_vrx_span>& spans, const char* who);
 int32_t noise_stages(rvx_engine& e, const char* who);
 
+// The blanker over a receiver's new channel samples (noise_reference.h:59-190):
+// detect through the newest sample, apply through `lead` before it (a sample's
+// verdict waits for the flags up to lead after it), in pieces that keep each
+// dispatch's reach inside the ring. The first push starts at the stream's first
+// block, whose window reads the ring's cleared slots (a zero window never
+// flags); a blanker enabled mid-stream starts a window past the oldest sample
+// the ring still holds.
+int32_t blank_channel(rvx_engine& e, EngineRx& r, uint64_t oldest, uint64_t newest,
+                      const char* who) {
+    const rvx_blanker_config& c = r.blank_cfg;
+    const uint32_t mask = e.cfg.ring_blocks - 1;
+    const uint64_t below = uint64_t(c.hang) + c.guard + c.window;
+    if (!r.blank_started) {
+        r.blank_started = true;
+        const uint64_t start =
+            oldest == e.first_block ? e.first_block : oldest + c.guard + c.window;
+        r.detect_next = start;
+        r.apply_next = r.apply_first = start;
+    }
+    const uint64_t piece_max = uint64_t(mask) + 1 - below - c.lead - 1;
+    rvx_blanker_params p{};
+    p.chan_base = r.plan.placement.channel * e.cfg.ring_blocks;
+    p.chan_mask = mask;
+    p.out_mask = mask;
+    p.threshold = r.blank_threshold;
+    const uint64_t ring_n = uint64_t(e.cfg.channels) * e.cfg.ring_blocks;
+    while (r.detect_next <= newest) {
+        const uint64_t n = std::min(newest + 1 - r.detect_next, piece_max);
+        p.chan_first = uint32_t(r.detect_next & mask);
+        p.out_first = p.chan_first;
+        p.count = uint32_t(n);
+        if (int32_t rc = rvx_blank_detect(e.ctx, &c, &p, e.ring, ring_n, r.flags, mask + 1ull))
+            return rc;
+        r.detect_next += n;
+    }
+    const uint64_t apply_end = r.detect_next >= c.lead ? r.detect_next - c.lead : 0;
+    while (r.apply_next < apply_end) {
+        const uint64_t n = std::min(apply_end - r.apply_next, piece_max);
+        p.chan_first = uint32_t(r.apply_next & mask);
+        p.out_first = p.chan_first;
+        p.count = uint32_t(n);
+        if (int32_t rc = rvx_blank_apply(e.ctx, &c, &p, e.ring, ring_n, r.flags, mask + 1ull,
+                                         r.blanked, mask + 1ull))
+            return rc;
+        r.apply_next += n;
+    }
+    (void)who;
+    return RVX_OK;
+}
+
 int32_t engine_receivers(rvx_engine& e, const char* who) {
     const uint64_t newest = e.next_block - 1;
     const uint64_t ring = e.cfg.ring_blocks;
@@ -475,7 +540,14 @@ int32_t engine_receivers(rvx_engine& e, const char* who) {
     for (auto& r : e.rx) r.pushed_n = 0;
     for (auto& r : e.rx) {
         uint64_t a_lo = 0, n = 0;
-        audio_range(r.plan, oldest, newest, &a_lo, &n);
+        if (r.blank_on) {
+            if (int32_t rc = blank_channel(e, r, oldest, newest, who)) return rc;
+            if (r.apply_next == r.apply_first) continue;  // nothing blanked yet
+            const uint64_t lo = std::max(oldest, r.apply_first);
+            audio_range(r.plan, lo, r.apply_next - 1, &a_lo, &n);
+        } else {
+            audio_range(r.plan, oldest, newest, &a_lo, &n);
+        }
         if (n == 0) continue;
         const uint64_t a_hi = a_lo + n - 1;
         if (!r.started) {
@@ -511,9 +583,9 @@ int32_t engine_receivers(rvx_engine& e, const char* who) {
             const uint64_t seg = std::min<uint64_t>({left, r.cap - pos, 0x7FFFFFFFull});
             rvx_vrx_span s{};
             s.plan = &r.plan;
-            s.channel_base = r.plan.placement.channel * e.cfg.ring_blocks;
+            s.channel_base = r.blank_on ? 0u : r.plan.placement.channel * e.cfg.ring_blocks;
             s.channel_mask = e.cfg.ring_blocks - 1;
-            s.channel_ring = e.ring;
+            s.channel_ring = r.blank_on ? r.blanked : e.ring;
             s.taps = r.taps;
             s.weights = r.weights;
             s.audio = r.audio + pos * ch;
@@ -751,6 +823,9 @@ int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const f
     if (!rc) rc = device_alloc(&e->win, e->win_cap * e->bps, who);
     if (!rc) rc = device_alloc(&e->bounce, (e->N + uint64_t(e->max_blocks + 1) * D) * e->bps, who);
     if (!rc) rc = device_alloc(&e->ring, uint64_t(M) * c.ring_blocks * 8, who);
+    // cleared, so a window reaching below the stream's first block reads zeros
+    if (!rc && hipMemset(e->ring, 0, uint64_t(M) * c.ring_blocks * 8) != hipSuccess)
+        rc = rvx::fail(RVX_EDEVICE, "%s: clearing the channel ring failed", who);
     if (!rc) rc = device_alloc(&e->counter, sizeof(uint32_t), who);
     if (!rc && c.calibration) {
         rc = rvx_front_end_create(nullptr, &e->fe);
@@ -904,6 +979,54 @@ int32_t rvx_engine_remove_vrx(rvx_engine* e, uint32_t id) {
     RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
     free_rx(*it);
     e->rx.erase(it);
+    return RVX_OK;
+}
+
+int32_t rvx_engine_set_blanker(rvx_engine* e, uint32_t id, const rvx_blanker_config* config,
+                               float threshold) {
+    static const char* who = "rvx_engine_set_blanker";
+    if (!e) return rvx::fail(RVX_EINVAL, "%s: null engine", who);
+    RVX_ENTER(e->ctx, who);
+    auto it = std::find_if(e->rx.begin(), e->rx.end(), [&](const EngineRx& r) { return r.id == id; });
+    if (it == e->rx.end()) return rvx::fail(RVX_EINVAL, "%s: no receiver %u", who, id);
+    EngineRx& r = *it;
+    if (config) {
+        rvx_blanker_params probe{};
+        probe.chan_mask = probe.out_mask = e->cfg.ring_blocks - 1;
+        probe.count = 1;
+        probe.threshold = threshold;
+        rvx_blanker_config c = *config;
+        c.pass = RVX_BLANK_DETECT;
+        if (int32_t rc = rvx_blanker_check(&c, &probe)) return rc;
+        // a push's new blocks, the window below them and the lead above must
+        // all still be in the ring when the blanker reads them
+        const uint64_t reach = uint64_t(c.hang) + c.guard + c.window + c.lead;
+        if (uint64_t(e->max_blocks) + reach >= e->cfg.ring_blocks)
+            return rvx::fail(RVX_EINVAL, "%s: a push of up to %u blocks plus the blanker's reach "
+                             "of %llu does not fit a %u-block ring", who, e->max_blocks,
+                             (unsigned long long)reach, e->cfg.ring_blocks);
+    }
+    RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
+    for (void** q : {(void**)&r.flags, &r.blanked})
+        if (*q) {
+            (void)hipFree(*q);
+            *q = nullptr;
+        }
+    r.blank_on = false;
+    r.blank_started = false;
+    if (config) {
+        const uint64_t n = e->cfg.ring_blocks;
+        if (int32_t rc = device_alloc(&r.flags, n * sizeof(float), who)) return rc;
+        if (int32_t rc = device_alloc(&r.blanked, n * 8, who)) return rc;
+        RVX_TRY_HIP(hipMemset(r.flags, 0, n * sizeof(float)), who);  // no flags yet
+        RVX_TRY_HIP(hipMemset(r.blanked, 0, n * 8), who);
+        r.blank_cfg = *config;
+        r.blank_threshold = threshold;
+        r.blank_on = true;
+        // the fine stage changes source: its audio restarts from what is blanked
+        r.started = false;
+        r.fine_started = false;
+    }
     return RVX_OK;
 }
 

@@ -99,6 +99,13 @@ class Engine:
         except Exception:
             pass
 
+    def set_blanker(self, rid: int, config: _lib.BlankerConfig | None,
+                    threshold: float = 0.0) -> None:
+        """The receiver's impulse blanker on its coarse channel
+        (rvx_engine_set_blanker); None turns it off."""
+        _lib.check(self.lib.rvx_engine_set_blanker(
+            self.handle, rid, C.byref(config) if config is not None else None, threshold))
+
     def set_noise(self, rid: int, line_config: _lib.LineConfig | None = None,
                   line_params: _lib.LineParams | None = None, spectral_frame: int = 0,
                   spectral_params: _lib.SpectralParams | None = None) -> None:
@@ -113,9 +120,15 @@ class Engine:
             C.byref(spectral_params) if spectral_params is not None else None))
 
     def set_noise_plan(self, rid: int, plan, line_config: _lib.LineConfig | None = None,
-                       spectral_frame: int = 0) -> None:
-        """From revenant_amd.noise.plan_noise's NoisePlan (the blanker runs on
-        the channel, outside this call)."""
+                       spectral_frame: int = 0, blanker: _lib.BlankerConfig | None = None) -> None:
+        """From revenant_amd.noise.plan_noise's NoisePlan; the blanker needs
+        its design for the receiver's channel rate (dsp.design_blanker)."""
+        if plan.blank:
+            if blanker is None:
+                raise ValueError("the plan blanks: pass design_blanker(channel_rate)")
+            self.set_blanker(rid, blanker, plan.blank_threshold)
+        else:
+            self.set_blanker(rid, None)
         self.set_noise(rid, line_config if plan.line else None,
                        plan.line_params if plan.line else None,
                        spectral_frame if plan.spectral else 0,

@@ -250,3 +250,69 @@ def test_engine_front_end_correction_removes_spike_and_image(dev):
     assert abs(applied.cross - (-np.tan(phi))) < 2e-3
     assert abs(applied.scale - 1.0 / (1.05 * np.cos(phi))) < 2e-3
     assert abs(applied.dc_i - 0.02) < 1e-3 and abs(applied.dc_q - 0.01) < 1e-3
+
+
+def test_engine_blanker_feeds_the_fine_stage(dev, stream, oracle_run):
+    """Impulses on two receivers' channels: with the blanker on, each
+    receiver's audio equals the blanker twins over its whole channel (zeros
+    below the first block, as a cleared ring reads) followed by the fine and
+    demod twins over the blanked channel; and it differs from the unblanked
+    audio where the impulses were."""
+    from revenant_amd import _lib
+    from revenant_amd.dsp import BLANK_APPLY, BLANK_DETECT, blanker_threshold, design_blanker
+    from revenant_amd.engine import Engine
+
+    codes, proto, plans = stream
+    first, blocks_all, cap, ring, state = oracle_run
+    # impulses: a few strong clicks in the raw stream (every channel sees them)
+    g = rng(0x424C414E4B)
+    clicked = codes.copy().view(np.int16)
+    for at in g.integers(100_000, SAMPLES - 100_000, size=12):
+        clicked[2 * at:2 * at + 32] = 30000  # 16 samples
+    clicked = clicked.view(np.uint32)
+    od = oracle.OChain(FORMAT_CS16, GRID.channels, GRID.taps_per_branch, GRID.decimation, first,
+                       blocks_all, cap, FRAME, ALPHA, ema_frame_decay(ALPHA, FRAME))
+    cring, _ = oracle.chain(od, clicked.view(np.uint8), proto, build_twiddles(GRID.channels),
+                            threads=4)
+    cring = cring.reshape(GRID.channels, cap, 2)
+    sel = [0, 1]  # an NFM and an AM receiver
+    cfg = design_blanker(plans[0].plan.channel_rate)
+    # 6 dB: a click the channelizer has smeared over its impulse response
+    # seldom stands 12 dB over its own window
+    thr = blanker_threshold(6.0)
+    eng = Engine(dev, GRID, proto, FORMAT_CS16, max_push=65536, ring_blocks=4096,
+                 frame_blocks=FRAME, row_frames=ROWS, alpha=float(ALPHA))
+    ids = [eng.add(plans[k], audio_capacity=1 << 16) for k in sel]
+    for rid in ids:
+        eng.set_blanker(rid, cfg, thr)
+    for pos in range(0, SAMPLES, 60000):
+        eng.push(clicked[pos:min(pos + 60000, SAMPLES)])
+    assert eng.stats().audio_lost == 0
+    for k, rid in zip(sel, ids):
+        p = plans[k]
+        a0, got = eng.read_audio(rid)
+        # the twin: the channel linear from `first`, zeros below it
+        lin = 1 << 16
+        P = 1024
+        chan = np.zeros((lin, 2), np.float32)
+        chan[P:P + blocks_all] = cring[p.plan.placement.channel][
+            (first + np.arange(blocks_all)) & (cap - 1)]
+        flags = np.zeros(lin, np.float32)
+        blanked = np.zeros((lin, 2), np.float32)
+        bp = _lib.BlankerParams(0, lin - 1, P, lin - 1, P, blocks_all, thr)
+        c = _lib.BlankerConfig.from_buffer_copy(bytes(cfg))
+        c.pass_ = BLANK_DETECT
+        oracle.blank_detect(c, bp, chan, flags)
+        ap = _lib.BlankerParams(0, lin - 1, P, lin - 1, P, blocks_all - cfg.lead, thr)
+        c.pass_ = BLANK_APPLY
+        oracle.blank_apply(c, ap, chan, flags, blanked)
+        assert (flags != 0).sum() >= 1
+        assert not np.array_equal(blanked[P:P + blocks_all - cfg.lead],
+                                  chan[P:P + blocks_all - cfg.lead])
+        # back onto the twin ring's slots, so the fine/demod composition reads it
+        bring = np.zeros((GRID.channels, cap, 2), np.float32)
+        m = first + np.arange(blocks_all - cfg.lead)
+        bring[p.plan.placement.channel][m & (cap - 1)] = blanked[P:P + blocks_all - cfg.lead]
+        ref = _oracle_audio(p, bring, cap, a0, got.size // p.demod.channels)
+        assert_bit_exact(got, ref, what=f"blanked receiver {k}")
+    eng.close()
