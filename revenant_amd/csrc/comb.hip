@@ -17,8 +17,9 @@
 //
 // subtract: per sample, removed = sum over lines ascending of a_l * tone_l(n)
 // (from -0, which adds to anything unchanged), cleaned = x - removed; with no
-// lines the sample is copied and -0 is removed. One lane per sample; the line
-// records are uniform across the wave.
+// lines the sample is copied and -0 is removed. Sixteen samples per lane over
+// an LDS copy of the tone table; the line records are uniform across the wave.
+// Both kernels gather the tone table from LDS.
 
 #pragma clang fp contract(off)
 
@@ -52,10 +53,12 @@ __device__ __forceinline__ float2 tone(const float2* __restrict__ table, const r
 // from memory once per group rather than once per line.
 __global__ void __launch_bounds__(1024)
 project_kernel(rvx_comb_params p, uint32_t group, const float2* __restrict__ raw,
-               const rvx_comb_line* __restrict__ lines, const float2* __restrict__ table,
+               const rvx_comb_line* __restrict__ lines, const float2* __restrict__ table_g,
                float2* __restrict__ proj) {
     __shared__ float2 stage[kLanes * kRow];
     __shared__ float2 sums[16][kLanes];
+    __shared__ float2 table[kTableSize];  // the tone table, gathered from LDS
+    for (uint32_t i = threadIdx.x; i < kTableSize; i += blockDim.x) table[i] = table_g[i];
     const uint32_t groups = (p.lines + group - 1) / group;
     const uint32_t s = blockIdx.x / groups;
     const uint32_t wave = threadIdx.x / kLanes, lane = threadIdx.x % kLanes;
@@ -102,26 +105,45 @@ project_kernel(rvx_comb_params p, uint32_t group, const float2* __restrict__ raw
     if (lane == 0 && live) proj[uint64_t(l) * p.sub_blocks + s] = make_float2(total_re, total_im);
 }
 
-// raw and cleaned may be the same ring: a lane reads and writes only its slot.
-__global__ void subtract_kernel(rvx_comb_params p, const float2* raw,
-                                const rvx_comb_line* __restrict__ lines,
-                                const float2* __restrict__ table, float2* cleaned,
-                                float2* __restrict__ removed) {
-    const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= p.count) return;
-    const uint32_t slot = (p.first + n) & p.ring_mask;
-    const float2 x = raw[slot];
-    float sum_re = -0.0f, sum_im = -0.0f;
+// raw and cleaned may be the same ring: a lane reads and writes only its
+// slots. A workgroup takes kSubPer * 256 samples (a lane's are 256 apart, so
+// every pass over them is coalesced) and stages the tone table in LDS once
+// for all of them; each lane runs its samples' sums side by side, lines in
+// ascending order for every one.
+constexpr uint32_t kSubThreads = 256, kSubPer = 16;
+
+__global__ void __launch_bounds__(kSubThreads)
+subtract_kernel(rvx_comb_params p, const float2* raw, const rvx_comb_line* __restrict__ lines,
+                const float2* __restrict__ table_g, float2* cleaned,
+                float2* __restrict__ removed) {
+    __shared__ float2 table[kTableSize];
+    for (uint32_t i = threadIdx.x; i < kTableSize; i += kSubThreads) table[i] = table_g[i];
+    __syncthreads();
+    const uint32_t n0 = blockIdx.x * (kSubThreads * kSubPer) + threadIdx.x;
+    float sum_re[kSubPer], sum_im[kSubPer];
+#pragma unroll
+    for (uint32_t j = 0; j < kSubPer; ++j) sum_re[j] = sum_im[j] = -0.0f;
     for (uint32_t l = 0; l < p.lines; ++l) {
         const rvx_comb_line line = lines[l];
-        const float2 t = tone(table, line, n);
-        const float re = line.amplitude[0] * t.x - line.amplitude[1] * t.y;
-        const float im = line.amplitude[0] * t.y + line.amplitude[1] * t.x;
-        sum_re = sum_re + re;
-        sum_im = sum_im + im;
+#pragma unroll
+        for (uint32_t j = 0; j < kSubPer; ++j) {
+            const uint32_t n = n0 + j * kSubThreads;
+            const float2 t = table[(line.phase0 + n * line.step) >> kTableShift];
+            const float re = line.amplitude[0] * t.x - line.amplitude[1] * t.y;
+            const float im = line.amplitude[0] * t.y + line.amplitude[1] * t.x;
+            sum_re[j] = sum_re[j] + re;
+            sum_im[j] = sum_im[j] + im;
+        }
     }
-    cleaned[slot] = p.lines == 0 ? x : make_float2(x.x - sum_re, x.y - sum_im);
-    removed[(p.first + n) & p.removed_mask] = make_float2(sum_re, sum_im);
+#pragma unroll
+    for (uint32_t j = 0; j < kSubPer; ++j) {
+        const uint32_t n = n0 + j * kSubThreads;
+        if (n >= p.count) break;
+        const uint32_t slot = (p.first + n) & p.ring_mask;
+        const float2 x = raw[slot];
+        cleaned[slot] = p.lines == 0 ? x : make_float2(x.x - sum_re[j], x.y - sum_im[j]);
+        removed[(p.first + n) & p.removed_mask] = make_float2(sum_re[j], sum_im[j]);
+    }
 }
 
 }  // namespace
@@ -211,7 +233,9 @@ int32_t rvx_comb_subtract(rvx_ctx* ctx, const rvx_comb_params* params, const voi
         return rvx::fail(RVX_EINVAL, "%s: a buffer is shorter than its parameters", who);
     if (!raw || !table || !cleaned || !removed || (p.lines && !lines))
         return rvx::fail(RVX_EINVAL, "%s: null buffer", who);
-    subtract_kernel<<<dim3((p.count + 255) / 256), dim3(256), 0, ctx->stream>>>(
+    constexpr uint32_t per_group = kSubThreads * kSubPer;
+    subtract_kernel<<<dim3((p.count + per_group - 1) / per_group), dim3(kSubThreads), 0,
+                      ctx->stream>>>(
         p, static_cast<const float2*>(raw), lines, static_cast<const float2*>(table),
         static_cast<float2*>(cleaned), static_cast<float2*>(removed));
     return rvx::check_launch(who);
