@@ -35,6 +35,7 @@ extern "C" {
 #define RVX_EINVAL (-1)   /* parameters refused (the twin's validate() rules) */
 #define RVX_EDEVICE (-2)  /* a HIP call failed */
 #define RVX_ENOMEM (-3)   /* allocation failed */
+#define RVX_EAGAIN (-4)   /* not yet: the call needs more stream first */
 
 #define RVX_ABI_VERSION 1
 
@@ -1162,6 +1163,18 @@ int32_t rvx_engine_remove_vrx(rvx_engine* engine, uint32_t id);
  * spectral_frame with spectral (count ignored), each NULL for off. Every push
  * then runs them in place on the audio it produced, all receivers in batched
  * launches. A call restarts both stages' state (it waits for the stream). */
+/* The receiver's display tap (vrx_reference.h:1639-1768, recorded beside its
+ * fine stage): every push extends the display stream (the fine kernel with
+ * the 256-tap display table) over the samples the fine stage reads, into a
+ * display ring of display_ring samples (a power of two, 0 turns it off).
+ * rvx_engine_passband transforms the newest `transform` display samples into
+ * the pane: transform / 2 dB values, ascending from the display's DC minus a
+ * quarter of the display rate (graph.cpp:314-400); *end_output is one past
+ * the display output the window ends at. RVX_EAGAIN until enough samples. */
+int32_t rvx_engine_set_display(rvx_engine* engine, uint32_t id, uint32_t display_ring,
+                               uint32_t transform);
+int32_t rvx_engine_passband(rvx_engine* engine, uint32_t id, float* out, uint64_t n_out,
+                            uint64_t* end_output, int64_t* display_rate);
 /* The receiver's impulse blanker on its coarse channel (noise_reference.h:
  * 59-190, recorded before the fine stage): config (NULL for off; pass is
  * ignored) and the threshold as a power ratio (rvx_blanker_threshold). Each

@@ -399,6 +399,8 @@ _SIGNATURES = {
     "rvx_plan_vrx_display": (i32, [C.POINTER(VrxPlan), C.POINTER(VrxDisplayPlan), vp, u64]),
     "rvx_display_block": (i32, [C.POINTER(VrxDisplayPlan), u32, u32, u32, u64, u32,
                                 C.POINTER(VrxFineParams), C.POINTER(u64)]),
+    "rvx_engine_set_display": (i32, [vp, u32, u32, u32]),
+    "rvx_engine_passband": (i32, [vp, u32, vp, u64, C.POINTER(u64), C.POINTER(i64)]),
     "rvx_engine_set_blanker": (i32, [vp, u32, C.POINTER(BlankerConfig), f32]),
     "rvx_engine_set_noise": (i32, [vp, u32, C.POINTER(LineConfig), C.POINTER(LineParams), u32,
                                    C.POINTER(SpectralParams)]),

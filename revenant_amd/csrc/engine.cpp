@@ -220,15 +220,32 @@ struct EngineRx {
     void* blanked = nullptr;  // ring_blocks cf32
     bool blank_started = false;
     uint64_t detect_next = 0, apply_next = 0, apply_first = 0;
+    // the display tap: the fine kernel with the display table into a display
+    // ring (channel 0 of two; channel 1 stays zero for the pane's transform)
+    bool display_on = false;
+    rvx_vrx_display_plan display{};
+    void* display_taps = nullptr;     // RVX_DISPLAY_TAPS + 1 cf32
+    void* display_ring = nullptr;     // 2 * display_cap cf32
+    uint64_t display_cap = 0;
+    uint32_t transform = 0;
+    void* pane_tw = nullptr;          // transform cf32
+    float* pane_window = nullptr;     // transform + transform / 2 floats
+    float* pane = nullptr;            // transform floats (two channels' central halves)
+    bool display_started = false;
+    uint64_t display_next = 0;        // first display output not yet produced
+    uint64_t display_from = 0;        // first display output of this run
 };
 
 void free_rx(EngineRx& r) {
     for (void* q : {r.taps, (void*)r.weights, (void*)r.audio, r.fine, (void*)r.cstate,
                     (void*)r.line_state, (void*)r.spec_state, (void*)r.spec_tables,
-                    (void*)r.flags, r.blanked})
+                    (void*)r.flags, r.blanked, r.display_taps, r.display_ring, r.pane_tw,
+                    (void*)r.pane_window, (void*)r.pane})
         if (q) (void)hipFree(q);
     r.flags = nullptr;
     r.blanked = nullptr;
+    r.display_taps = r.display_ring = r.pane_tw = nullptr;
+    r.pane_window = r.pane = nullptr;
     r.taps = r.fine = nullptr;
     r.weights = r.audio = nullptr;
     r.cstate = nullptr;
@@ -480,6 +497,44 @@ int32_t front_end_push(rvx_engine& e, uint64_t push_first, uint64_t samples, con
 int32_t batch_receivers(rvx_engine& e, const std::vector<rvx_vrx_span>& spans, const char* who);
 int32_t noise_stages(rvx_engine& e, const char* who);
 
+// The display stream over the receiver's source samples lo .. hi (the channel,
+// or the blanked channel the fine stage reads): output j reads source samples
+// j*R - 255 .. j*R, so outputs up to hi / R are produced, from the first whose
+// history is inside [lo, ...] on a new run (vrx_reference.h:1639-1768).
+int32_t display_stream(rvx_engine& e, EngineRx& r, uint64_t lo, uint64_t hi, const char* who) {
+    const uint64_t R = r.display.decimation;
+    const uint64_t hist = RVX_DISPLAY_TAPS - 1;
+    const uint64_t j_hi = hi / R;
+    if (!r.display_started) {
+        r.display_started = true;
+        r.display_next = r.display_from = (lo + hist + R - 1) / R;
+    }
+    if (r.display_next * R < lo + hist) {  // the source moved past the stream: a new run
+        r.display_next = r.display_from = (lo + hist + R - 1) / R;
+    }
+    const bool b = r.blank_on;
+    const void* src = b ? r.blanked : e.ring;
+    const uint32_t base = b ? 0u : r.plan.placement.channel * e.cfg.ring_blocks;
+    const uint64_t src_n = uint64_t(b ? 1u : e.cfg.channels) * e.cfg.ring_blocks;
+    const uint64_t n_taps = RVX_DISPLAY_TAPS + 1;
+    while (r.display_next <= j_hi) {
+        const uint32_t n = uint32_t(std::min<uint64_t>(j_hi + 1 - r.display_next, r.display_cap));
+        rvx_vrx_fine_params fp{};
+        uint64_t first_input = 0;
+        if (int32_t rc = rvx_display_block(&r.display, base, e.cfg.ring_blocks - 1,
+                                           uint32_t(r.display_cap - 1), r.display_next, n, &fp,
+                                           &first_input))
+            return rc;
+        if (int32_t rc = rvx_vrx_fine(e.ctx, &r.display.fine, &fp, src, src_n, r.display_taps,
+                                      n_taps, e.nco, e.nco_entries, r.display_ring,
+                                      r.display_cap))
+            return rc;
+        r.display_next += n;
+    }
+    (void)who;
+    return RVX_OK;
+}
+
 // The blanker over a receiver's new channel samples (noise_reference.h:59-190):
 // detect through the newest sample, apply through `lead` before it (a sample's
 // verdict waits for the flags up to lead after it), in pieces that keep each
@@ -540,8 +595,21 @@ int32_t engine_receivers(rvx_engine& e, const char* who) {
     for (auto& r : e.rx) r.pushed_n = 0;
     for (auto& r : e.rx) {
         uint64_t a_lo = 0, n = 0;
+        if (r.display_on) {
+            if (r.blank_on) {
+                if (int32_t rc = blank_channel(e, r, oldest, newest, who)) return rc;
+                if (r.apply_next > r.apply_first) {
+                    if (int32_t rc = display_stream(e, r, std::max(oldest, r.apply_first),
+                                                    r.apply_next - 1, who))
+                        return rc;
+                }
+            } else if (int32_t rc = display_stream(e, r, oldest, newest, who)) {
+                return rc;
+            }
+        }
         if (r.blank_on) {
-            if (int32_t rc = blank_channel(e, r, oldest, newest, who)) return rc;
+            if (!r.display_on)
+                if (int32_t rc = blank_channel(e, r, oldest, newest, who)) return rc;
             if (r.apply_next == r.apply_first) continue;  // nothing blanked yet
             const uint64_t lo = std::max(oldest, r.apply_first);
             audio_range(r.plan, lo, r.apply_next - 1, &a_lo, &n);
@@ -979,6 +1047,101 @@ int32_t rvx_engine_remove_vrx(rvx_engine* e, uint32_t id) {
     RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
     free_rx(*it);
     e->rx.erase(it);
+    return RVX_OK;
+}
+
+int32_t rvx_engine_set_display(rvx_engine* e, uint32_t id, uint32_t display_ring,
+                               uint32_t transform) {
+    static const char* who = "rvx_engine_set_display";
+    if (!e) return rvx::fail(RVX_EINVAL, "%s: null engine", who);
+    RVX_ENTER(e->ctx, who);
+    auto it = std::find_if(e->rx.begin(), e->rx.end(), [&](const EngineRx& r) { return r.id == id; });
+    if (it == e->rx.end()) return rvx::fail(RVX_EINVAL, "%s: no receiver %u", who, id);
+    EngineRx& r = *it;
+    rvx_vrx_display_plan plan{};
+    std::vector<float> taps(2 * (RVX_DISPLAY_TAPS + 1));
+    if (display_ring) {
+        if (!rvx::pow2(display_ring) || !rvx::pow2(transform) || transform < 4 ||
+            transform > display_ring)
+            return rvx::fail(RVX_EINVAL, "%s: a display ring of %u and a transform of %u (powers "
+                             "of two, the ring holding the transform)", who, display_ring,
+                             transform);
+        if (int32_t rc = rvx_plan_vrx_display(&r.plan, &plan, taps.data(), taps.size()))
+            return rc;
+        if ((uint64_t(1) << plan.fine.nco_log2) != e->nco_entries)
+            return rvx::fail(RVX_EINVAL, "%s: display NCO of 2^%u entries", who,
+                             plan.fine.nco_log2);
+    }
+    RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
+    for (void** q : {&r.display_taps, &r.display_ring, &r.pane_tw, (void**)&r.pane_window,
+                     (void**)&r.pane})
+        if (*q) {
+            (void)hipFree(*q);
+            *q = nullptr;
+        }
+    r.display_on = false;
+    r.display_started = false;
+    if (!display_ring) return RVX_OK;
+    std::vector<float> tw(2ull * transform), win(transform + transform / 2);
+    if (int32_t rc = rvx_build_twiddles(transform, tw.data(), tw.size())) return rc;
+    if (int32_t rc = rvx_build_spectrum_window(transform, e->cfg.taps_per_branch, 120.0,
+                                               win.data(), win.size()))
+        return rc;
+    // the pane has no channel seam: unity in place of the droop correction
+    for (uint32_t b = 0; b < transform / 2; ++b) win[transform + b] = 1.0f;
+    int32_t rc = device_alloc(&r.display_taps, taps.size() * sizeof(float), who);
+    if (!rc) rc = device_alloc(&r.display_ring, 2ull * display_ring * 8, who);
+    if (!rc) rc = device_alloc(&r.pane_tw, tw.size() * sizeof(float), who);
+    if (!rc) rc = device_alloc(&r.pane_window, win.size() * sizeof(float), who);
+    if (!rc) rc = device_alloc(&r.pane, uint64_t(transform) * sizeof(float), who);
+    if (rc) return rc;
+    RVX_TRY_HIP(hipMemset(r.display_ring, 0, 2ull * display_ring * 8), who);
+    RVX_TRY_HIP(hipMemcpy(r.display_taps, taps.data(), taps.size() * sizeof(float),
+                          hipMemcpyHostToDevice),
+                who);
+    RVX_TRY_HIP(hipMemcpy(r.pane_tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice),
+                who);
+    RVX_TRY_HIP(hipMemcpy(r.pane_window, win.data(), win.size() * sizeof(float),
+                          hipMemcpyHostToDevice),
+                who);
+    r.display = plan;
+    r.display_cap = display_ring;
+    r.transform = transform;
+    r.display_on = true;
+    return RVX_OK;
+}
+
+int32_t rvx_engine_passband(rvx_engine* e, uint32_t id, float* out, uint64_t n_out,
+                            uint64_t* end_output, int64_t* display_rate) {
+    static const char* who = "rvx_engine_passband";
+    if (!e) return rvx::fail(RVX_EINVAL, "%s: null engine", who);
+    RVX_ENTER(e->ctx, who);
+    auto it = std::find_if(e->rx.begin(), e->rx.end(), [&](const EngineRx& r) { return r.id == id; });
+    if (it == e->rx.end()) return rvx::fail(RVX_EINVAL, "%s: no receiver %u", who, id);
+    EngineRx& r = *it;
+    if (!r.display_on) return rvx::fail(RVX_EINVAL, "%s: receiver %u has no display tap", who, id);
+    const uint32_t n = r.transform;
+    if (!out || n_out < n / 2) return rvx::fail(RVX_EINVAL, "%s: the pane is %u floats", who, n / 2);
+    if (!r.display_started || r.display_next - r.display_from < n)
+        return rvx::fail(RVX_EAGAIN, "%s: fewer than %u display samples so far", who, n);
+    const uint64_t first = r.display_next - n;
+    rvx_spectrum_params p{};
+    p.channels = 2;
+    p.transform = n;
+    p.stages = rvx::log2u(n);
+    p.chan_blocks = uint32_t(r.display_cap);
+    p.chan_mask = uint32_t(r.display_cap - 1);
+    p.in_offset = uint32_t(first & (r.display_cap - 1));
+    if (int32_t rc = rvx_spectrum(e->ctx, &p, r.display_ring, 2 * r.display_cap, r.pane_tw, n,
+                                  r.pane_window, n + n / 2, r.pane, n))
+        return rc;
+    // channel 0's central half is the frame's upper half
+    RVX_TRY_HIP(hipMemcpyAsync(out, r.pane + n / 2, (n / 2) * sizeof(float),
+                               hipMemcpyDeviceToHost, e->ctx->stream),
+                who);
+    RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
+    if (end_output) *end_output = r.display_next;
+    if (display_rate) *display_rate = r.display.display_rate;
     return RVX_OK;
 }
 

@@ -99,6 +99,24 @@ class Engine:
         except Exception:
             pass
 
+    def set_display(self, rid: int, display_ring: int = 1 << 14, transform: int = 1024) -> None:
+        """The receiver's display tap (rvx_engine_set_display); display_ring
+        0 turns it off."""
+        _lib.check(self.lib.rvx_engine_set_display(self.handle, rid, display_ring, transform))
+
+    def passband(self, rid: int, transform: int = 1024):
+        """The pane over the newest `transform` display samples:
+        (dB values ascending in frequency, one past the window's last display
+        output, the display rate); None before there are enough samples."""
+        out = np.empty(transform // 2, dtype=np.float32)
+        end, rate = _lib.u64(), _lib.i64()
+        rc = self.lib.rvx_engine_passband(self.handle, rid, out.ctypes.data, out.size,
+                                          C.byref(end), C.byref(rate))
+        if rc == -4:
+            return None
+        _lib.check(rc)
+        return out, end.value, rate.value
+
     def set_blanker(self, rid: int, config: _lib.BlankerConfig | None,
                     threshold: float = 0.0) -> None:
         """The receiver's impulse blanker on its coarse channel

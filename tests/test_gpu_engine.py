@@ -316,3 +316,41 @@ def test_engine_blanker_feeds_the_fine_stage(dev, stream, oracle_run):
         ref = _oracle_audio(p, bring, cap, a0, got.size // p.demod.channels)
         assert_bit_exact(got, ref, what=f"blanked receiver {k}")
     eng.close()
+
+
+def test_engine_display_tap_and_pane(dev, stream, oracle_run):
+    """A receiver's display tap in the engine: after the stream, its pane is
+    the spectrum twin (two channels, unity gains, channel 0's half) over the
+    newest 1024 display samples, and those samples are the fine twin with
+    the display table over the twin's channel, bit for bit."""
+    from revenant_amd import _lib
+    from revenant_amd.dsp import build_spectrum_window, plan_vrx_display
+    from revenant_amd.engine import Engine
+
+    codes, proto, plans = stream
+    first, blocks_all, cap, ring, state = oracle_run
+    p = plans[0]  # NFM 12.5 kHz
+    eng = Engine(dev, GRID, proto, FORMAT_CS16, max_push=65536, ring_blocks=4096,
+                 frame_blocks=FRAME, row_frames=ROWS, alpha=float(ALPHA))
+    rid = eng.add(p, audio_capacity=1 << 16)
+    n, dcap = 1024, 1 << 14
+    eng.set_display(rid, dcap, n)
+    assert eng.passband(rid, n) is None
+    for pos in range(0, SAMPLES, 50000):
+        eng.push(codes[pos:min(pos + 50000, SAMPLES)])
+    pane, end, rate = eng.passband(rid, n)
+    d = plan_vrx_display(p)
+    assert rate == d.plan.display_rate and end * d.plan.decimation <= first + blocks_all
+    chan = np.ascontiguousarray(ring[p.plan.placement.channel]).ravel()
+    fp, _ = d.block(0, cap - 1, dcap - 1, end - n, n)
+    disp = oracle.vrx_fine(d.plan.fine, fp, chan, d.taps, build_nco_table(16), dcap)
+    two = np.zeros(4 * dcap, np.float32)
+    two[:2 * dcap] = disp
+    win = build_spectrum_window(n, GRID.taps_per_branch).copy()
+    win[n:n + n // 2] = 1.0
+    sp = _lib.SpectrumParams(2, n, 10, dcap, dcap - 1, (end - n) & (dcap - 1))
+    want = oracle.spectrum(sp, two, build_twiddles(n), win)[n // 2:]
+    assert_bit_exact(pane, want, what="engine passband pane")
+    # the receiver's own tone (200 kHz sits at the receiver's centre) is the pane's peak
+    assert abs(int(np.argmax(pane)) - n // 4) <= 2
+    eng.close()
