@@ -34,6 +34,13 @@ def test_abi_version_and_struct_sizes():
     assert C.sizeof(_lib.VrxDemodParams) == 4 * 4
     assert C.sizeof(_lib.PfbFftParams) == 7 * 4
     assert C.sizeof(_lib.SpectrumParams) == 6 * 4
+    # the noise stages' parameter blocks, word for word the reference's push constants
+    assert C.sizeof(_lib.BlankerParams) == 7 * 4      # noise_reference.h:165
+    assert C.sizeof(_lib.LineParams) == 32 * 4        # :381
+    assert C.sizeof(_lib.SpectralParams) == 10 * 4    # :514
+    assert C.sizeof(_lib.NoiseLineItem) == 144 and C.sizeof(_lib.NoiseSpectralItem) == 64
+    assert C.sizeof(_lib.ExcisionParams) == 8 * 4 and C.sizeof(_lib.CombParams) == 8 * 4
+    assert C.sizeof(_lib.VrxDisplayPlan) == 88
 
 
 def test_design_refuses_bad_grids():
