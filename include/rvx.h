@@ -275,13 +275,19 @@ int32_t rvx_nco_delta(int64_t hz_numerator, int64_t hz_denominator, int64_t rate
 /* ------------------------------------------------------------------------ */
 
 /* core/engine/vrx.h VrxParams, the part planning reads. */
+/* core/engine/vrx.h:213-240 Deemphasis */
+#define RVX_DEEMPH_DEFAULT 0
+#define RVX_DEEMPH_NONE 1
+#define RVX_DEEMPH_US75 2
+#define RVX_DEEMPH_EU50 3
+
 typedef struct rvx_vrx_request {
     int64_t center_hz;     /* relative to the source's tuned centre */
     int64_t bandwidth_hz;  /* 0: the mode's default passband */
     int64_t passband_low;  /* both 0: derived from bandwidth_hz */
     int64_t passband_high;
     uint32_t mode;         /* RVX_DEMOD_* */
-    uint32_t reserved;
+    uint32_t deemphasis;   /* RVX_DEEMPH_*: DEFAULT takes the mode's own */
     int64_t audio_rate;    /* 0: 48000 */
     int64_t cw_pitch;
 } rvx_vrx_request;
@@ -305,7 +311,7 @@ typedef struct rvx_vrx_plan {
     int64_t passband_low, passband_high, bandwidth, deviation;
     int64_t filter_num, filter_den, mix_num, mix_den;
     rvx_vrx_fine_config fine;
-    uint32_t reserved;
+    uint32_t deemphasis;  /* the curve resolved for the mode and rate (RVX_DEEMPH_*) */
     uint64_t fine_nco_delta;
     rvx_vrx_demod_config demod;
     float demod_gain;
@@ -320,13 +326,21 @@ typedef struct rvx_vrx_plan {
 int32_t rvx_place(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* request,
                   rvx_vrx_placement* out);
 /* core/dsp/vrx_reference.cpp:2568 plan_vrx (AM, NFM, WFM mono, USB, LSB,
- * DSB, CW, SAM, raw; no de-emphasis or stereo). SAM and DSB plans carry
- * their carrier loop (:2550-2561). */
+ * DSB, CW, SAM, raw; no stereo). The de-emphasis curve (resolve_deemphasis,
+ * core/engine/vrx.h:302: 75 us on WFM programme audio by default, none
+ * elsewhere) is folded into the audio filter, so demod.audio_taps is the
+ * folded length (:2462-2509). SAM and DSB plans carry their carrier loop
+ * (:2550-2561). */
 int32_t rvx_plan_vrx(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* request,
                      const rvx_vrx_placement* placement, rvx_vrx_plan* out);
 /* The plan's tables: fine taps (2*(phases*taps+1) floats, design_fine_taps
- * :1808) and demod weights (audio_taps + dc_taps floats, design_audio_taps
- * :1921 and design_dc_weights :1983). Either pointer may be NULL. */
+ * :1808) and demod weights (audio_taps + dc_taps floats: design_audio_taps
+ * :1921 folded with design_deemphasis_taps :2011 by fold_deemphasis :2079,
+ * then design_dc_weights :1983). Either pointer may be NULL. */
+/* :2011 design_deemphasis_taps: the curve's taps for an audio rate (out may
+ * be NULL to ask the length, returned in *n). */
+int32_t rvx_design_deemphasis(uint32_t curve, int64_t audio_rate, float* out, uint32_t max_taps,
+                              uint32_t* n);
 int32_t rvx_plan_vrx_tables(const rvx_vrx_plan* plan, float* fine_taps, uint64_t n_fine_floats,
                             float* weights, uint64_t n_weights);
 /* core/dsp/vrx_reference.cpp:2855 fine_block and :2873 demod_block: one

@@ -225,7 +225,7 @@ class ChainDesc(C.Structure):
 
 class VrxRequest(C.Structure):
     _fields_ = [("center_hz", i64), ("bandwidth_hz", i64), ("passband_low", i64),
-                ("passband_high", i64), ("mode", u32), ("reserved", u32), ("audio_rate", i64),
+                ("passband_high", i64), ("mode", u32), ("deemphasis", u32), ("audio_rate", i64),
                 ("cw_pitch", i64)]
 
 
@@ -240,7 +240,7 @@ class VrxPlan(C.Structure):
                 ("channel_rate", i64), ("audio_rate", i64), ("demod_rate", i64),
                 ("output_rate", i64), ("passband_low", i64), ("passband_high", i64),
                 ("bandwidth", i64), ("deviation", i64), ("filter_num", i64), ("filter_den", i64),
-                ("mix_num", i64), ("mix_den", i64), ("fine", VrxFineConfig), ("reserved", u32),
+                ("mix_num", i64), ("mix_den", i64), ("fine", VrxFineConfig), ("deemphasis", u32),
                 ("fine_nco_delta", u64), ("demod", VrxDemodConfig), ("demod_gain", f32),
                 ("audio_pass_hz", f64), ("audio_stop_hz", f64),
                 ("carrier", VrxCarrierConfig), ("carrier_loop", VrxCarrierParams),
@@ -404,6 +404,7 @@ _SIGNATURES = {
     "rvx_engine_set_blanker": (i32, [vp, u32, C.POINTER(BlankerConfig), f32]),
     "rvx_engine_set_noise": (i32, [vp, u32, C.POINTER(LineConfig), C.POINTER(LineParams), u32,
                                    C.POINTER(SpectralParams)]),
+    "rvx_design_deemphasis": (i32, [u32, i64, vp, u32, C.POINTER(u32)]),
     "rvx_comb_step": (u32, [f64, i64]),
     "rvx_comb_table": (i32, [fptr, u64]),
     "rvx_comb_check": (i32, [C.POINTER(CombParams)]),

@@ -178,6 +178,54 @@ int64_t minimum_rate(uint32_t mode, Band b) {
 
 bool complex_output(uint32_t mode) { return mode == RVX_DEMOD_RAW; }
 
+// core/engine/vrx.h:302 resolve_deemphasis: an explicit curve is kept except
+// on the complex tap; the default is 75 us for WFM programme audio (below
+// the composite rate) and none for every other mode.
+uint32_t resolve_deemphasis(uint32_t mode, uint32_t requested, int64_t audio_rate) {
+    if (requested != RVX_DEEMPH_DEFAULT)
+        return complex_output(mode) ? uint32_t(RVX_DEEMPH_NONE) : requested;
+    if (mode == RVX_DEMOD_WFM && audio_rate < kCompositeAudioRateHz) return RVX_DEEMPH_US75;
+    return RVX_DEEMPH_NONE;
+}
+
+double deemphasis_seconds(uint32_t curve) {
+    return curve == RVX_DEEMPH_US75 ? 75e-6 : curve == RVX_DEEMPH_EU50 ? 50e-6 : 0.0;
+}
+
+constexpr uint32_t kMaxDeemphasisTaps = 1024;  // vrx_reference.h:471
+constexpr uint32_t kMaxAudioTaps = 4096;       // vrx_reference.h:462
+
+// vrx_reference.cpp:2011 design_deemphasis_taps: the impulse-invariant
+// one-pole a = exp(-1/(tau*rate)) as its geometric series a^k, cut where a
+// longer series cannot move a float (a^N <= 2^-24 (1 - a)), at most 1024
+// terms, renormalised to unit sum in double and rounded once. No curve is
+// one unit tap.
+std::vector<float> deemphasis_taps(uint32_t curve, int64_t audio_rate) {
+    const double tau = deemphasis_seconds(curve);
+    if (!(tau > 0.0) || audio_rate <= 0) return {1.0f};
+    const double pole = std::exp(-(1.0 / double(audio_rate)) / tau);
+    const double exact = std::log((1.0 / 16777216.0) * (1.0 - pole)) / std::log(pole);
+    const uint32_t n =
+        std::min(uint32_t(std::ceil(std::max(1.0, exact))) + 1u, kMaxDeemphasisTaps);
+    std::vector<double> series(n);
+    double sum = 0.0, term = 1.0;
+    for (uint32_t k = 0; k < n; ++k) {
+        series[k] = term;
+        sum += term;
+        term *= pole;
+    }
+    std::vector<float> taps(n);
+    const double scale = 1.0 / sum;
+    for (uint32_t k = 0; k < n; ++k) taps[k] = float(series[k] * scale);
+    return taps;
+}
+
+// Taps of the decimation filter with the curve folded in at the audio stride
+// (vrx_reference.cpp:2079 fold_deemphasis).
+uint64_t folded_taps(uint32_t decimation_taps, uint32_t curve_taps, uint32_t decimation) {
+    return uint64_t(decimation_taps) + uint64_t(curve_taps - 1) * decimation;
+}
+
 }  // namespace
 
 // Kaiser-windowed lowpass of cutoff half_width over phases * taps points,
@@ -320,6 +368,8 @@ int32_t rvx_plan_vrx(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* 
         return rvx::fail(RVX_EINVAL, "%s: placement carries no rates", who);
     if (r.mode > RVX_DEMOD_CW && r.mode != RVX_DEMOD_SAM)
         return rvx::fail(RVX_EINVAL, "%s: mode %u is not planned by this build", who, r.mode);
+    if (r.deemphasis > RVX_DEEMPH_EU50)
+        return rvx::fail(RVX_EINVAL, "%s: de-emphasis %u is not a curve", who, r.deemphasis);
     rvx_vrx_plan plan{};
     plan.placement = pl;
     plan.mode = r.mode;
@@ -379,7 +429,20 @@ int32_t rvx_plan_vrx(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* 
         if ((taps % 2u) == 0u) ++taps;
         plan.audio_decimation_taps = std::min(taps, 1024u);
     }
-    plan.demod.audio_taps = plan.audio_decimation_taps;
+    // The de-emphasis curve is folded into the audio filter at the audio
+    // stride, so the kernel still runs one FIR (vrx_reference.cpp:2462-2509).
+    plan.deemphasis = resolve_deemphasis(r.mode, r.deemphasis, plan.audio_rate);
+    const uint64_t audio_taps =
+        folded_taps(plan.audio_decimation_taps,
+                    uint32_t(deemphasis_taps(plan.deemphasis, plan.audio_rate).size()),
+                    plan.demod.decimation);
+    if (audio_taps > kMaxAudioTaps)
+        return rvx::fail(RVX_EINVAL,
+                         "%s: a %u-tap audio filter with its de-emphasis folded in at a "
+                         "decimation of %u is %llu taps; the demodulator holds %u",
+                         who, plan.audio_decimation_taps, plan.demod.decimation,
+                         (unsigned long long)audio_taps, kMaxAudioTaps);
+    plan.demod.audio_taps = uint32_t(audio_taps);
     plan.demod.channels = cx ? 2u : 1u;
     plan.demod.pilot_taps = 0;
     plan.demod.dc_taps =
@@ -405,6 +468,25 @@ int32_t rvx_plan_vrx(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* 
     return RVX_OK;
 }
 
+int32_t rvx_design_deemphasis(uint32_t curve, int64_t audio_rate, float* out, uint32_t max_taps,
+                              uint32_t* n) {
+    static const char* who = "rvx_design_deemphasis";
+    if (!n) return rvx::fail(RVX_EINVAL, "%s: null length", who);
+    if (curve > RVX_DEEMPH_EU50 || curve == RVX_DEEMPH_DEFAULT)
+        return rvx::fail(RVX_EINVAL, "%s: %u is not a resolved curve", who, curve);
+    if (deemphasis_seconds(curve) > 0.0 && audio_rate <= 0)
+        return rvx::fail(RVX_EINVAL, "%s: audio rate must be positive, got %lld", who,
+                         (long long)audio_rate);
+    const std::vector<float> taps = deemphasis_taps(curve, audio_rate);
+    *n = uint32_t(taps.size());
+    if (out != nullptr) {
+        if (max_taps < taps.size())
+            return rvx::fail(RVX_EINVAL, "%s: %zu taps do not fit %u", who, taps.size(), max_taps);
+        std::copy(taps.begin(), taps.end(), out);
+    }
+    return RVX_OK;
+}
+
 int32_t rvx_plan_vrx_tables(const rvx_vrx_plan* plan, float* fine_taps, uint64_t n_fine_floats,
                             float* weights, uint64_t n_weights) {
     static const char* who = "rvx_plan_vrx_tables";
@@ -423,22 +505,47 @@ int32_t rvx_plan_vrx_tables(const rvx_vrx_plan* plan, float* fine_taps, uint64_t
         const uint32_t at = p.demod.audio_taps, dc = p.demod.dc_taps;
         if (n_weights != uint64_t(at) + dc)
             return rvx::fail(RVX_EINVAL, "%s: weights want %u floats", who, at + dc);
-        // Audio decimation filter: Kaiser lowpass, unit DC gain.
-        if (at == 1) {
-            weights[0] = 1.0f;
+        if (p.deemphasis > RVX_DEEMPH_EU50)
+            return rvx::fail(RVX_EINVAL, "%s: de-emphasis %u is not a curve", who, p.deemphasis);
+        const std::vector<float> curve = deemphasis_taps(p.deemphasis, p.audio_rate);
+        const uint32_t dt = p.audio_decimation_taps;
+        if (dt == 0 || folded_taps(dt, uint32_t(curve.size()), p.demod.decimation) != at)
+            return rvx::fail(RVX_EINVAL,
+                             "%s: %u audio taps are not %u decimation taps with a %zu-tap "
+                             "curve folded in at %u",
+                             who, at, dt, curve.size(), p.demod.decimation);
+        // Audio decimation filter: Kaiser lowpass, unit DC gain, rounded to
+        // float (design_audio_taps :1921).
+        std::vector<float> decim(dt);
+        if (dt == 1) {
+            decim[0] = 1.0f;
         } else {
             const double cut = (p.audio_pass_hz + p.audio_stop_hz) * 0.5 / double(p.demod_rate);
-            const double centre = (double(at) - 1.0) / 2.0;
+            const double centre = (double(dt) - 1.0) / 2.0;
             const double beta = kaiser_shape(kAudioAttenuationDb), norm = i0(beta);
-            std::vector<double> d(at);
+            std::vector<double> d(dt);
             double sum = 0.0;
-            for (uint32_t i = 0; i < at; ++i) {
+            for (uint32_t i = 0; i < dt; ++i) {
                 d[i] = 2.0 * cut * sinc_pi(2.0 * cut * (double(i) - centre)) *
-                       kaiser_window(i, at, beta, norm);
+                       kaiser_window(i, dt, beta, norm);
                 sum += d[i];
             }
             const double scale = std::abs(sum) < 1e-12 ? 1.0 : 1.0 / sum;
-            for (uint32_t i = 0; i < at; ++i) weights[i] = float(d[i] * scale);
+            for (uint32_t i = 0; i < dt; ++i) decim[i] = float(d[i] * scale);
+        }
+        // The curve folded in (fold_deemphasis :2079): the unit tap leaves the
+        // floats as they are; otherwise the products of the two float tables
+        // are summed in double at the audio stride and rounded once.
+        if (curve.size() == 1 && curve[0] == 1.0f) {
+            std::copy(decim.begin(), decim.end(), weights);
+        } else {
+            std::vector<double> folded(at, 0.0);
+            for (size_t k = 0; k < curve.size(); ++k) {
+                const double w = double(curve[k]);
+                const size_t offset = k * p.demod.decimation;
+                for (uint32_t j = 0; j < dt; ++j) folded[offset + j] += w * double(decim[j]);
+            }
+            for (uint32_t i = 0; i < at; ++i) weights[i] = float(folded[i]);
         }
         // DC window: raised cosine over dc taps, unit sum.
         if (dc <= 1) {

@@ -19,6 +19,8 @@ BYTES_PER_SAMPLE = {FORMAT_CU8: 2, FORMAT_CS8: 2, FORMAT_CS16: 4, FORMAT_CS24: 6
 
 DEMOD_RAW, DEMOD_AM, DEMOD_NFM, DEMOD_WFM, DEMOD_USB, DEMOD_LSB, DEMOD_DSB, DEMOD_CW = range(8)
 DEMOD_SAM = 12
+# core/engine/vrx.h:213 Deemphasis: DEFAULT takes the mode's own curve.
+DEEMPH_DEFAULT, DEEMPH_NONE, DEEMPH_US75, DEEMPH_EU50 = range(4)
 CARRIER_NONE, CARRIER_PLL, CARRIER_COSTAS = 0, 1, 2  # core/dsp/vrx_reference.h:674-676
 CARRIER_FLAG_RESET = 1
 CARRIER_STATE_WORDS = 8
@@ -137,11 +139,12 @@ class VrxRequest:
     audio_rate: int = 48000
     cw_pitch: int = 0
     passband: tuple[int, int] | None = None
+    deemphasis: int = DEEMPH_DEFAULT
 
     def c(self) -> _lib.VrxRequest:
         lo, hi = self.passband if self.passband else (0, 0)
-        return _lib.VrxRequest(self.center_hz, self.bandwidth_hz, lo, hi, self.mode, 0,
-                               self.audio_rate, self.cw_pitch)
+        return _lib.VrxRequest(self.center_hz, self.bandwidth_hz, lo, hi, self.mode,
+                               self.deemphasis, self.audio_rate, self.cw_pitch)
 
 
 @dataclass
@@ -249,6 +252,16 @@ def plan_vrx(grid: GridParams, rate: int, request: VrxRequest,
     _lib.check(_lib.lib().rvx_plan_vrx_tables(C.byref(out), _fptr(taps), taps.size, _fptr(w),
                                               w.size))
     return VrxPlan(out, taps.view(np.complex64), w)
+
+
+def design_deemphasis(curve: int, audio_rate: int) -> np.ndarray:
+    """vrx_reference.cpp:2011 design_deemphasis_taps: the resolved curve's float taps."""
+    n = C.c_uint32()
+    _lib.check(_lib.lib().rvx_design_deemphasis(curve, audio_rate, None, 0, C.byref(n)))
+    taps = np.zeros(n.value, np.float32)
+    _lib.check(_lib.lib().rvx_design_deemphasis(curve, audio_rate, _fptr(taps), taps.size,
+                                                C.byref(n)))
+    return taps
 
 
 def prescale_taps(taps: np.ndarray) -> np.ndarray:

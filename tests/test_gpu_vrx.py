@@ -2,7 +2,8 @@
 
 After tests/reference/test_vrx.cpp:1206-1690 (conformance of vrx_fine and
 vrx_demod against their twins), with the receivers of the batching workload:
-NFM, AM, WFM, USB, LSB, CW and a raw tap.
+NFM, AM, WFM (75 us de-emphasis folded in, 50 us, and none), USB, LSB, CW
+and a raw tap.
 """
 import numpy as np
 import pytest
@@ -10,8 +11,9 @@ import pytest
 from helpers import assert_bit_exact, from_dev, rng, to_dev
 from oracle import oracle
 from revenant_amd import _lib
-from revenant_amd.dsp import (DEMOD_AM, DEMOD_CW, DEMOD_LSB, DEMOD_NFM, DEMOD_RAW, DEMOD_USB,
-                              DEMOD_WFM, GridParams, VrxRequest, build_nco_table, plan_vrx)
+from revenant_amd.dsp import (DEEMPH_EU50, DEEMPH_NONE, DEMOD_AM, DEMOD_CW, DEMOD_LSB,
+                              DEMOD_NFM, DEMOD_RAW, DEMOD_USB, DEMOD_WFM, GridParams, VrxRequest,
+                              build_nco_table, plan_vrx)
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -27,8 +29,10 @@ CASES = [
     (FINE_GRID, VrxRequest(33_333_333, DEMOD_CW, 500, cw_pitch=700)),
     (FINE_GRID, VrxRequest(5_000_000, DEMOD_RAW, 12000)),
     (WIDE_GRID, VrxRequest(20_000_000, DEMOD_WFM, 200_000)),
+    (WIDE_GRID, VrxRequest(20_000_000, DEMOD_WFM, 200_000, deemphasis=DEEMPH_EU50)),
+    (WIDE_GRID, VrxRequest(20_000_000, DEMOD_WFM, 200_000, deemphasis=DEEMPH_NONE)),
 ]
-IDS = ["nfm", "am", "usb", "lsb", "cw", "raw", "wfm"]
+IDS = ["nfm", "am", "usb", "lsb", "cw", "raw", "wfm", "wfm-50us", "wfm-flat"]
 
 
 @pytest.fixture(scope="module")
