@@ -274,13 +274,13 @@ int32_t rvx_nco_delta(int64_t hz_numerator, int64_t hz_denominator, int64_t rate
 /* Receiver planning (core/engine/vrx_place.cpp, core/dsp/vrx_reference.cpp) */
 /* ------------------------------------------------------------------------ */
 
-/* core/engine/vrx.h VrxParams, the part planning reads. */
 /* core/engine/vrx.h:213-240 Deemphasis */
 #define RVX_DEEMPH_DEFAULT 0
 #define RVX_DEEMPH_NONE 1
 #define RVX_DEEMPH_US75 2
 #define RVX_DEEMPH_EU50 3
 
+/* core/engine/vrx.h VrxParams, the part planning reads. */
 typedef struct rvx_vrx_request {
     int64_t center_hz;     /* relative to the source's tuned centre */
     int64_t bandwidth_hz;  /* 0: the mode's default passband */
@@ -337,12 +337,12 @@ int32_t rvx_plan_vrx(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* 
  * :1808) and demod weights (audio_taps + dc_taps floats: design_audio_taps
  * :1921 folded with design_deemphasis_taps :2011 by fold_deemphasis :2079,
  * then design_dc_weights :1983). Either pointer may be NULL. */
-/* :2011 design_deemphasis_taps: the curve's taps for an audio rate (out may
- * be NULL to ask the length, returned in *n). */
-int32_t rvx_design_deemphasis(uint32_t curve, int64_t audio_rate, float* out, uint32_t max_taps,
-                              uint32_t* n);
 int32_t rvx_plan_vrx_tables(const rvx_vrx_plan* plan, float* fine_taps, uint64_t n_fine_floats,
                             float* weights, uint64_t n_weights);
+/* :2011 design_deemphasis_taps: a resolved curve's taps at an audio rate
+ * (out may be NULL to ask the length, returned in *n). */
+int32_t rvx_design_deemphasis(uint32_t curve, int64_t audio_rate, float* out, uint32_t max_taps,
+                              uint32_t* n);
 /* core/dsp/vrx_reference.cpp:2855 fine_block and :2873 demod_block: one
  * dispatch's params from the absolute index of its first output. */
 int32_t rvx_fine_block(const rvx_vrx_plan* plan, uint32_t channel_base, uint32_t channel_mask,
