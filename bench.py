@@ -16,9 +16,9 @@ graph and replayed:
   4. 256 receivers in one persistent launch: 64 NFM 12.5 kHz, 64 AM 10 kHz,
      32 WFM 200 kHz, 64 USB 2.7 kHz, 32 CW 500 Hz at seeded uniform offsets,
      each resampled to 48 kHz and demodulated to FP32 audio over the whole
-     step (the WFM receivers with the planner's default 75 us de-emphasis
-     folded into their audio filter); tiles picked by the autotuner before
-     capture.
+     step (the WFM receivers as the planner makes them by default: FM stereo,
+     L and R, with the 75 us de-emphasis folded into their audio filter);
+     tiles picked by the autotuner before capture.
 `value` is input MS/s through all four. `--workload plain` times steps 1-2
 alone (the plain chain).
 
@@ -362,7 +362,7 @@ def main() -> int:
     workload = ("PFB + 256 VRX: 2^27 cs16 @ 122.88 MS/s (400 carriers + AWGN) -> 4096-ch PFB "
                 "(D=2048, 16 taps/branch, Kaiser beta 8) -> channel ring + |X|^2 EMA "
                 "alpha=1/16; 512-ch PFB (D=256) for the WFM receivers; 256 VRX "
-                "(64 NFM 12.5k, 64 AM 10k, 32 WFM 200k + 75us de-emphasis, 64 USB 2.7k, "
+                "(64 NFM 12.5k, 64 AM 10k, 32 WFM 200k stereo + 75us de-emphasis, 64 USB 2.7k, "
                 "32 CW 500) -> 48 kHz "
                 "FP32 audio, one persistent launch; the step is one HIP graph") if vrx else (
                 "plain chain: 2^27 cs16 @ 122.88 MS/s, 400 carriers + AWGN -> 4096-ch PFB "

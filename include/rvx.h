@@ -290,6 +290,9 @@ typedef struct rvx_vrx_request {
     uint32_t deemphasis;   /* RVX_DEEMPH_*: DEFAULT takes the mode's own */
     int64_t audio_rate;    /* 0: 48000 */
     int64_t cw_pitch;
+    uint32_t mono;         /* nonzero: VrxParams::stereo = false. 0 (the default) decodes
+                              FM stereo where resolve_stereo allows (WFM programme audio) */
+    uint32_t reserved;
 } rvx_vrx_request;
 
 /* core/engine/vrx.h:746-783 VrxPlacement */
@@ -318,7 +321,7 @@ typedef struct rvx_vrx_plan {
     double audio_pass_hz, audio_stop_hz;
     rvx_vrx_carrier_config carrier;      /* power RVX_CARRIER_NONE: no loop */
     rvx_vrx_carrier_params carrier_loop; /* design_carrier_loop; per-dispatch words zero */
-    uint32_t reserved2;
+    uint32_t stereo;  /* resolve_stereo (core/engine/vrx.h:381): L and R, a pilot filter */
 } rvx_vrx_plan;
 
 /* core/engine/vrx_place.cpp:216 place: nearest channel round(center*M/rate),

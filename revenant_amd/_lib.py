@@ -226,7 +226,7 @@ class ChainDesc(C.Structure):
 class VrxRequest(C.Structure):
     _fields_ = [("center_hz", i64), ("bandwidth_hz", i64), ("passband_low", i64),
                 ("passband_high", i64), ("mode", u32), ("deemphasis", u32), ("audio_rate", i64),
-                ("cw_pitch", i64)]
+                ("cw_pitch", i64), ("mono", u32), ("reserved", u32)]
 
 
 class VrxPlacement(C.Structure):
@@ -244,7 +244,7 @@ class VrxPlan(C.Structure):
                 ("fine_nco_delta", u64), ("demod", VrxDemodConfig), ("demod_gain", f32),
                 ("audio_pass_hz", f64), ("audio_stop_hz", f64),
                 ("carrier", VrxCarrierConfig), ("carrier_loop", VrxCarrierParams),
-                ("reserved2", u32)]
+                ("stereo", u32)]
 
 
 class VrxDesc(C.Structure):

@@ -88,7 +88,8 @@ double kaiser_shape(double a) {
 // Kaiser's order estimate for a transition given as a fraction of the rate.
 uint32_t kaiser_taps(double attenuation_db, double transition_fraction) {
     if (!(transition_fraction > 0.0)) return 0;
-    const double order = (attenuation_db - 8.0) / (2.285 * kTwoPi * transition_fraction);
+    const double radians = kTwoPi * transition_fraction;
+    const double order = (attenuation_db - 8.0) / (2.285 * radians);
     if (!(order > 0.0)) return 2;
     return uint32_t(std::ceil(order)) + 1u;
 }
@@ -193,6 +194,67 @@ double deemphasis_seconds(uint32_t curve) {
 }
 
 constexpr uint32_t kMaxDeemphasisTaps = 1024;  // vrx_reference.h:471
+// FM stereo's pilot bandpass (vrx_reference.h:476-502).
+constexpr uint32_t kMaxPilotTaps = 2048;
+constexpr int64_t kStereoPilotHz = 19000;
+constexpr double kStereoPilotHalfWidthHz = 1000.0;
+constexpr double kStereoPilotTransitionHz = 3000.0;
+constexpr double kStereoPilotAttenuationDb = 70.0;
+
+// core/engine/vrx.h:381 resolve_stereo: a request, kept only by a WFM
+// receiver delivering programme audio (below the composite rate).
+bool resolve_stereo(uint32_t mode, bool requested, int64_t audio_rate) {
+    return requested && mode == RVX_DEMOD_WFM && audio_rate < kCompositeAudioRateHz;
+}
+
+// vrx_reference.cpp:2129 stereo_pilot_taps: a 70 dB Kaiser bandpass with a
+// 3 kHz transition at the demodulation rate, odd, in [16, 2048].
+uint32_t stereo_pilot_taps(int64_t demod_rate) {
+    if (demod_rate <= 0) return 0;
+    const double transition = kStereoPilotTransitionHz / double(demod_rate);
+    uint32_t taps = std::clamp(kaiser_taps(kStereoPilotAttenuationDb, transition), 16u,
+                               kMaxPilotTaps);
+    if ((taps % 2u) == 0u) ++taps;
+    return std::min(taps, kMaxPilotTaps);
+}
+
+// vrx_reference.cpp:2147 design_stereo_tables. The pilot filter: a 1 kHz
+// half-width Kaiser lowpass of unit DC gain modulated by exp(+j 2 pi 19k k /
+// rate) on the convolution's own k, so a 19 kHz tone comes out with the
+// prototype's gain and no delay (pilot_taps complex pairs). The rotation:
+// twice the cosine and sine of the 38 kHz subcarrier's advance from the
+// audio filter's centre tap (audio_taps pairs), which carries the pilot
+// phase measured there to every tap.
+void design_stereo_tables(int64_t demod_rate, uint32_t audio_taps, uint32_t pilot_taps,
+                          float* pilot, float* rotation) {
+    const double rate = double(demod_rate);
+    const double cutoff = kStereoPilotHalfWidthHz / rate;
+    const double centre = (double(pilot_taps) - 1.0) / 2.0;
+    const double beta = kaiser_shape(kStereoPilotAttenuationDb), norm = i0(beta);
+    std::vector<double> proto(pilot_taps);
+    double sum = 0.0;
+    for (uint32_t k = 0; k < pilot_taps; ++k) {
+        proto[k] = 2.0 * cutoff * sinc_pi(2.0 * cutoff * (double(k) - centre)) *
+                   kaiser_window(k, pilot_taps, beta, norm);
+        sum += proto[k];
+    }
+    const double scale = std::abs(sum) < 1e-12 ? 1.0 : 1.0 / sum;
+    const double pilot_turns = double(kStereoPilotHz) / rate;
+    for (uint32_t k = 0; k < pilot_taps; ++k) {
+        const double angle = kTwoPi * std::fmod(pilot_turns * double(k), 1.0);
+        const double w = proto[k] * scale;
+        pilot[2 * k] = float(w * std::cos(angle));
+        pilot[2 * k + 1] = float(w * std::sin(angle));
+    }
+    const double sub_turns = 2.0 * pilot_turns;
+    const int64_t pivot = int64_t(audio_taps / 2u);
+    for (uint32_t t = 0; t < audio_taps; ++t) {
+        const double offset = double(int64_t(t) - pivot);
+        const double angle = kTwoPi * std::fmod(sub_turns * offset, 1.0);
+        rotation[2 * t] = float(2.0 * std::cos(angle));
+        rotation[2 * t + 1] = float(2.0 * std::sin(angle));
+    }
+}
 constexpr uint32_t kMaxAudioTaps = 4096;       // vrx_reference.h:462
 
 // vrx_reference.cpp:2011 design_deemphasis_taps: the impulse-invariant
@@ -443,8 +505,15 @@ int32_t rvx_plan_vrx(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* 
                          who, plan.audio_decimation_taps, plan.demod.decimation,
                          (unsigned long long)audio_taps, kMaxAudioTaps);
     plan.demod.audio_taps = uint32_t(audio_taps);
-    plan.demod.channels = cx ? 2u : 1u;
-    plan.demod.pilot_taps = 0;
+    // Stereo (vrx_reference.cpp:2515-2535): two channels and a pilot filter.
+    plan.stereo = resolve_stereo(r.mode, r.mono == 0, plan.audio_rate) ? 1u : 0u;
+    plan.demod.channels = (cx || plan.stereo) ? 2u : 1u;
+    plan.demod.pilot_taps = plan.stereo ? stereo_pilot_taps(plan.demod_rate) : 0u;
+    if (plan.stereo && (plan.demod.pilot_taps == 0 || 2 * kStereoPilotHz >= plan.demod_rate))
+        return rvx::fail(RVX_EINVAL,
+                         "%s: a stereo receiver needs a 19 kHz pilot filter and a 38 kHz "
+                         "subcarrier, and %lld S/s carries neither",
+                         who, (long long)plan.demod_rate);
     plan.demod.dc_taps =
         (r.mode == RVX_DEMOD_AM || r.mode == RVX_DEMOD_SAM)
             ? uint32_t(std::clamp<int64_t>(plan.demod_rate / kAmDcCornerHz, 16, 4096))
@@ -502,9 +571,15 @@ int32_t rvx_plan_vrx_tables(const rvx_vrx_plan* plan, float* fine_taps, uint64_t
             return rc;
     }
     if (weights != nullptr) {
-        const uint32_t at = p.demod.audio_taps, dc = p.demod.dc_taps;
-        if (n_weights != uint64_t(at) + dc)
-            return rvx::fail(RVX_EINVAL, "%s: weights want %u floats", who, at + dc);
+        const uint32_t at = p.demod.audio_taps, dc = p.demod.dc_taps, pt = p.demod.pilot_taps;
+        // audio taps, DC window, then (stereo) pilot pairs and rotation pairs
+        const uint64_t want = uint64_t(at) + dc + (pt ? 2ull * pt + 2ull * at : 0ull);
+        if (n_weights != want)
+            return rvx::fail(RVX_EINVAL, "%s: weights want %llu floats", who,
+                             (unsigned long long)want);
+        if (pt != 0 && (pt > kMaxPilotTaps || 2 * kStereoPilotHz >= p.demod_rate))
+            return rvx::fail(RVX_EINVAL, "%s: %u pilot taps at %lld S/s", who, pt,
+                             (long long)p.demod_rate);
         if (p.deemphasis > RVX_DEEMPH_EU50)
             return rvx::fail(RVX_EINVAL, "%s: de-emphasis %u is not a curve", who, p.deemphasis);
         const std::vector<float> curve = deemphasis_taps(p.deemphasis, p.audio_rate);
@@ -560,6 +635,9 @@ int32_t rvx_plan_vrx_tables(const rvx_vrx_plan* plan, float* fine_taps, uint64_t
             const double scale = sum < 1e-12 ? 1.0 : 1.0 / sum;
             for (uint32_t i = 0; i < dc; ++i) weights[at + i] = float(wdc[i] * scale);
         }
+        if (pt != 0)
+            design_stereo_tables(p.demod_rate, at, pt, weights + at + dc,
+                                 weights + at + dc + 2ull * pt);
     }
     return RVX_OK;
 }

@@ -140,11 +140,13 @@ class VrxRequest:
     cw_pitch: int = 0
     passband: tuple[int, int] | None = None
     deemphasis: int = DEEMPH_DEFAULT
+    stereo: bool = True   # a request: kept by WFM programme audio only
 
     def c(self) -> _lib.VrxRequest:
         lo, hi = self.passband if self.passband else (0, 0)
         return _lib.VrxRequest(self.center_hz, self.bandwidth_hz, lo, hi, self.mode,
-                               self.deemphasis, self.audio_rate, self.cw_pitch)
+                               self.deemphasis, self.audio_rate, self.cw_pitch,
+                               0 if self.stereo else 1, 0)
 
 
 @dataclass
@@ -153,7 +155,7 @@ class VrxPlan:
 
     plan: _lib.VrxPlan
     fine_taps: np.ndarray  # complex64, phases*taps + 1
-    weights: np.ndarray    # float32, audio_taps + dc_taps
+    weights: np.ndarray    # float32: audio_taps + dc_taps (+ stereo pilot and rotation pairs)
 
     @property
     def fine(self) -> _lib.VrxFineConfig:
@@ -248,7 +250,9 @@ def plan_vrx(grid: GridParams, rate: int, request: VrxRequest,
                                        C.byref(out)))
     n = out.fine.phases * out.fine.taps + 1
     taps = np.zeros(2 * n, dtype=np.float32)
-    w = np.zeros(out.demod.audio_taps + out.demod.dc_taps, dtype=np.float32)
+    d = out.demod
+    stereo = 2 * d.pilot_taps + 2 * d.audio_taps if d.pilot_taps else 0
+    w = np.zeros(d.audio_taps + d.dc_taps + stereo, dtype=np.float32)
     _lib.check(_lib.lib().rvx_plan_vrx_tables(C.byref(out), _fptr(taps), taps.size, _fptr(w),
                                               w.size))
     return VrxPlan(out, taps.view(np.complex64), w)

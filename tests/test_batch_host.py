@@ -97,14 +97,17 @@ def test_batch_plan_splits_tiles_and_orders_largest_first():
         covered.setdefault(d.fine_config.taps, 0)
         covered[d.fine_config.taps] += d.demod.count
     assert covered == {nfm.fine.taps: 3000, wfm.fine.taps: 20000}
-    # WFM descriptors continue each other: audio pointers step by count floats
+    # WFM descriptors continue each other: audio pointers step by count frames
+    # (two floats a frame: the planner's WFM receiver is stereo)
+    assert wfm.demod.channels == 2
     w = [d for d in descs if d.fine_config.taps == wfm.fine.taps]
     for a, b in zip(w, w[1:]):
-        assert b.audio - a.audio == 4 * a.demod.count
+        assert b.audio - a.audio == 4 * a.demod_config.channels * a.demod.count
     # every (descriptor, tile) exactly once
     want = {(v, j) for v, d in enumerate(descs) for j in range(-(-d.demod.count // d.tile))}
     assert sorted(work) == sorted(want)
-    # WFM tiles (423-tap audio filter at R = 7) cost more than NFM ones: first
+    # WFM tiles (an 878-tap audio filter and a pilot filter at R = 7) cost more
+    # than NFM ones: first
     first_kind = descs[work[0][0]].fine_config.taps
     last_kind = descs[work[-1][0]].fine_config.taps
     assert first_kind == wfm.fine.taps and last_kind == nfm.fine.taps

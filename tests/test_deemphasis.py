@@ -26,6 +26,13 @@ WIDE = GridParams(512, 16, 256)   # 480 kS/s channels: a whole 200 kHz broadcast
 TAU = {DEEMPH_US75: 75e-6, DEEMPH_EU50: 50e-6}
 FLOOR = 2.0 ** -24
 WARP_BUDGET_DB = 0.5              # test_vrx.cpp:2604
+# The WFM receivers here are mono (stereo=False), as in test_vrx.cpp:2613, so
+# the curve reads straight off one channel.
+
+
+def _wfm(**kw):
+    """A mono broadcast FM receiver on the wide grid."""
+    return plan_vrx(WIDE, WIDE_RATE, VrxRequest(STATION, DEMOD_WFM, 200_000, stereo=False, **kw))
 
 
 def _response(taps, f, rate):
@@ -85,7 +92,7 @@ def test_curve_follows_the_one_pole_within_the_warp_budget(curve):
 
 
 def test_broadcast_fm_takes_the_north_american_curve():
-    p = plan_vrx(WIDE, WIDE_RATE, VrxRequest(STATION, DEMOD_WFM, 200_000))
+    p = _wfm()
     pl = p.plan
     assert pl.deemphasis == DEEMPH_US75
     n = design_deemphasis(DEEMPH_US75, pl.audio_rate).size
@@ -97,7 +104,7 @@ def test_broadcast_fm_takes_the_north_american_curve():
 
 
 def test_the_composite_tap_takes_no_curve():
-    p = plan_vrx(WIDE, WIDE_RATE, VrxRequest(STATION, DEMOD_WFM, 200_000, audio_rate=171_000))
+    p = _wfm(audio_rate=171_000)
     assert p.plan.deemphasis == DEEMPH_NONE
     assert p.plan.demod.audio_taps == p.plan.audio_decimation_taps
     assert p.plan.audio_pass_hz == pytest.approx(0.4 * 171_000)
@@ -118,7 +125,7 @@ def test_the_raw_tap_never_carries_a_curve():
 
 
 def test_an_explicit_curve_is_honoured():
-    eu = plan_vrx(WIDE, WIDE_RATE, VrxRequest(STATION, DEMOD_WFM, 200_000, deemphasis=DEEMPH_EU50))
+    eu = _wfm(deemphasis=DEEMPH_EU50)
     assert eu.plan.deemphasis == DEEMPH_EU50
     nfm = plan_vrx(GRID, RATE, VrxRequest(196_500, DEMOD_NFM, 12000, deemphasis=DEEMPH_US75))
     assert nfm.plan.deemphasis == DEEMPH_US75
@@ -126,22 +133,21 @@ def test_an_explicit_curve_is_honoured():
     assert nfm.plan.demod.audio_taps == (nfm.plan.audio_decimation_taps +
                                          (n - 1) * nfm.plan.demod.decimation)
     # an explicit curve on the composite tap is honoured too
-    comp = plan_vrx(WIDE, WIDE_RATE, VrxRequest(STATION, DEMOD_WFM, 200_000, audio_rate=171_000,
-                                           deemphasis=DEEMPH_US75))
+    comp = _wfm(audio_rate=171_000, deemphasis=DEEMPH_US75)
     assert comp.plan.deemphasis == DEEMPH_US75
 
 
 def test_an_unknown_curve_is_refused():
     with pytest.raises(_lib.RvxError):
-        plan_vrx(WIDE, WIDE_RATE, VrxRequest(STATION, DEMOD_WFM, 200_000, deemphasis=4))
+        _wfm(deemphasis=4)
 
 
 def test_a_flat_curve_leaves_the_audio_filter_as_it_was():
     """NONE is the decimation filter itself, the table a receiver had before
     de-emphasis; the curve's own table is the same decimation filter with the curve
     folded in, whose response is the product of the two (convolution theorem)."""
-    off = plan_vrx(WIDE, WIDE_RATE, VrxRequest(STATION, DEMOD_WFM, 200_000, deemphasis=DEEMPH_NONE))
-    on = plan_vrx(WIDE, WIDE_RATE, VrxRequest(STATION, DEMOD_WFM, 200_000))
+    off = _wfm(deemphasis=DEEMPH_NONE)
+    on = _wfm()
     dt = off.plan.audio_decimation_taps
     assert off.plan.demod.audio_taps == dt and on.plan.audio_decimation_taps == dt
     # the DC window after the audio filter is untouched by the fold
@@ -182,9 +188,8 @@ def _fm_tone_amplitude(p, modulation_hz):
 
 @pytest.mark.parametrize("curve", [DEEMPH_US75, DEEMPH_EU50])
 def test_a_deemphasised_receiver_follows_the_one_pole(curve):
-    on = plan_vrx(WIDE, WIDE_RATE, VrxRequest(STATION, DEMOD_WFM, 200_000, deemphasis=curve))
-    flat = plan_vrx(WIDE, WIDE_RATE,
-                    VrxRequest(STATION, DEMOD_WFM, 200_000, deemphasis=DEEMPH_NONE))
+    on = _wfm(deemphasis=curve)
+    flat = _wfm(deemphasis=DEEMPH_NONE)
     for f in (500.0, 1000.0, 2122.0, 5000.0, 8000.0):
         amp, purity = _fm_tone_amplitude(on, f)
         ref, _ = _fm_tone_amplitude(flat, f)

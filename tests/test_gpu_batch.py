@@ -29,15 +29,15 @@ REQUESTS = [
     (FINE_GRID, VrxRequest(-20_000_000, DEMOD_LSB, 2700)),
     (FINE_GRID, VrxRequest(33_333_333, DEMOD_CW, 500, cw_pitch=700)),
     (FINE_GRID, VrxRequest(5_000_000, DEMOD_RAW, 12000)),
-    (WIDE_GRID, VrxRequest(20_000_000, DEMOD_WFM, 200_000)),
+    (WIDE_GRID, VrxRequest(20_000_000, DEMOD_WFM, 200_000, stereo=False)),
 ]
 CAP = 1 << 14          # channel length in the test ring
 CHANNELS = 8
 
 
 def stereo_plan(base: VrxPlan, seed: int) -> VrxPlan:
-    """The WFM plan turned into a two-channel receiver with a pilot filter and
-    seeded weights (the planner keeps 48 kHz audio mono, vrx.h:275)."""
+    """The mono WFM plan turned into a two-channel receiver with a short pilot
+    filter and seeded weights (every weight of the stereo branch exercised)."""
     plan = type(base.plan).from_buffer_copy(base.plan)
     plan.demod.channels = 2
     plan.demod.pilot_taps = 33
@@ -56,6 +56,8 @@ def scene():
         np.complex64)
     plans = [plan_vrx(grid, RATE, req) for grid, req in REQUESTS]
     plans.append(stereo_plan(plans[-1], 0x5652580000000C01))
+    # the planner's own stereo receiver: pilot bandpass, rotation and 75 us curve
+    plans.append(plan_vrx(WIDE_GRID, RATE, VrxRequest(-20_000_000, DEMOD_WFM, 200_000)))
     return nco, ring, plans
 
 

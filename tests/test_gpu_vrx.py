@@ -85,3 +85,25 @@ def test_demod_refuses_short_ring(dev):
     dp = _lib.VrxDemodParams(255, 0, 4096, 1.0)
     with pytest.raises(_lib.RvxError):
         dev.vrx_demod(p.demod, dp, fine, to_dev(p.weights), audio)
+
+
+def test_stereo_station_matches_twin_and_separates(dev):
+    """A pilot-tone stereo multiplex through the planner's stereo receiver:
+    bit-exact with the oracle, and the two tones land in their own channels."""
+    import test_stereo as st
+
+    p = st._wfm(deemphasis=DEEMPH_NONE)
+    ring = st._station(p, 1000.0, 3000.0)
+    cap, R = ring.size, p.demod.decimation
+    first = -(-p.history // R) + 8
+    count = (cap - 1) // R - first
+    dp = p.demod_block(cap - 1, first, count)
+    cpu = oracle.vrx_demod(p.demod, dp, ring, p.weights)
+    audio = torch.zeros(2 * count, dtype=torch.float32, device="cuda")
+    dev.vrx_demod(p.demod, dp, to_dev(ring.view(np.float32)), to_dev(p.weights), audio)
+    got = from_dev(audio)
+    assert_bit_exact(got, cpu, what="vrx_demod stereo station")
+    t = (first + np.arange(count)) / float(p.plan.output_rate)
+    l_want, l_leak = st._tones(got[0::2], t, (1000.0, 3000.0))
+    r_leak, r_want = st._tones(got[1::2], t, (1000.0, 3000.0))
+    assert 20 * np.log10(l_want / l_leak) > 30.0 and 20 * np.log10(r_want / r_leak) > 30.0
