@@ -28,6 +28,9 @@ constexpr uint32_t kBatchThreads = 256;
 constexpr uint32_t kMaxSpan = 2048;     // fine samples per tile: fine (8 B) + detector (4 B)
 constexpr uint32_t kTapEntries = 2040;  // interpolated taps of every phase class (8 B each)
 constexpr uint32_t kPerThread = kMaxSpan / kBatchThreads;
+// Global memory seen through the constant address space: the caller's
+// promise that nothing writes it while the kernel runs.
+typedef const __attribute__((address_space(4))) float kconst_float;
 // 40,900 bytes of LDS: four workgroups fit a CU's 160 KiB.
 static_assert(sizeof(rvx_vrx_desc) == 160, "descriptor layout is part of the ABI");
 
@@ -81,7 +84,11 @@ vrx_batch_kernel(const rvx_vrx_desc* __restrict__ descs, uint32_t n_desc,
         const uint32_t dh = detector_history(dc);
         const float2* ch = static_cast<const float2*>(d.channel_ring) + fp.chan_base;
         const float2* taps = static_cast<const float2*>(d.taps);
-        const float* wts = d.weights;
+        // The demod tables are read-only for the launch and every lane of a
+        // workgroup reads the same tap at once: through the constant address
+        // space those reads are scalar loads, issued ahead of the taps that
+        // use them, instead of a vector load and a full wait per tap.
+        const kconst_float* wts = reinterpret_cast<const kconst_float*>(uintptr_t(d.weights));
         float* audio = d.audio;
 
         // 1. fine samples a0*R .. a0*R + span - 1 of the receiver's fine block.
@@ -144,11 +151,13 @@ vrx_batch_kernel(const rvx_vrx_desc* __restrict__ descs, uint32_t n_desc,
                 if (s < span) mag[s] = m[q];
             }
             __syncthreads();
-            const float* dcw = wts + dc.audio_taps;
+            const kconst_float* dcw = wts + dc.audio_taps;
             for (uint32_t s = tid; s < det_span; s += kBatchThreads) {
                 float acc = 0.0f;
-                for (uint32_t k = 0; k < dc.dc_taps; ++k) {
-                    const float term = mag[s + dh - k] * dcw[k];
+                const kconst_float* dw = dcw;
+#pragma unroll 8
+                for (uint32_t k = 0; k < dc.dc_taps; ++k, ++dw) {
+                    const float term = mag[s + dh - k] * dw[0];
                     acc = acc + term;
                 }
                 det[s] = mag[s + dh] - acc;
@@ -176,10 +185,12 @@ vrx_batch_kernel(const rvx_vrx_desc* __restrict__ descs, uint32_t n_desc,
             if (dc.channels == 2) {
                 const uint32_t at = rel - dc.audio_taps / 2;
                 float pr = 0.0f, pi = 0.0f;
-                for (uint32_t k = 0; k < dc.pilot_taps; ++k) {
+                const kconst_float* pw = wts + pilot_base;
+#pragma unroll 8
+                for (uint32_t k = 0; k < dc.pilot_taps; ++k, pw += 2) {
                     const float comp = det[at - k];
-                    const float tr = comp * wts[pilot_base + 2 * k];
-                    const float ti = comp * wts[pilot_base + 2 * k + 1];
+                    const float tr = comp * pw[0];
+                    const float ti = comp * pw[1];
                     pr = pr + tr;
                     pi = pi + ti;
                 }
@@ -192,10 +203,13 @@ vrx_batch_kernel(const rvx_vrx_desc* __restrict__ descs, uint32_t n_desc,
                 const float sin2 = (-2.0f * cross) * inv;
                 const float locked = power >= 2.0e-5f ? 1.0f : 0.0f;
                 float sum = 0.0f, diff = 0.0f;
-                for (uint32_t t = 0; t < dc.audio_taps; ++t) {
-                    const float weighted = det[rel - t] * wts[t];
+                const kconst_float* aw = wts;
+                const kconst_float* rw = wts + rot_base;
+#pragma unroll 8
+                for (uint32_t t = 0; t < dc.audio_taps; ++t, ++aw, rw += 2) {
+                    const float weighted = det[rel - t] * aw[0];
                     sum = sum + weighted;
-                    const float ref = sin2 * wts[rot_base + 2 * t] - cos2 * wts[rot_base + 2 * t + 1];
+                    const float ref = sin2 * rw[0] - cos2 * rw[1];
                     const float term = weighted * ref;
                     diff = diff + term;
                 }
@@ -204,8 +218,10 @@ vrx_batch_kernel(const rvx_vrx_desc* __restrict__ descs, uint32_t n_desc,
                 audio[2 * uint64_t(o) + 1] = (sum - gated) * dp.gain;
             } else {
                 float acc = 0.0f;
-                for (uint32_t t = 0; t < dc.audio_taps; ++t) {
-                    const float term = det[rel - t] * wts[t];
+                const kconst_float* aw = wts;
+#pragma unroll 8
+                for (uint32_t t = 0; t < dc.audio_taps; ++t, ++aw) {
+                    const float term = det[rel - t] * aw[0];
                     acc = acc + term;
                 }
                 audio[o] = acc * dp.gain;
