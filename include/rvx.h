@@ -644,7 +644,10 @@ int32_t rvx_synth(rvx_ctx* ctx, const rvx_carrier* carriers, uint32_t ncarriers,
  * channel_rows[k] of a ring of ring_rows rows, and not stored when that is
  * >= ring_rows; null stores every channel in row k (ring_rows is then
  * ignored). A coarse grid that exists only to feed a few wide receivers keeps
- * just their channels; the averaged spectrum still covers every channel. */
+ * just their channels; the averaged spectrum still covers every channel.
+ * twiddles: rvx_build_twiddles(M). The kernel relies on that table's exact
+ * quadrant symmetry (entry k + M/4 is entry k turned by -i, bit for bit) and
+ * reads only the entries it needs to form the rest. */
 typedef struct rvx_chain_desc {
     int32_t format;           /* RVX_FORMAT_CU8 or RVX_FORMAT_CS16 */
     uint32_t channels;

@@ -241,6 +241,13 @@ chain_kernel(ChainArgs a) {
 #pragma unroll
     for (int r = 0; r < V; ++r) ema[r] = 0.0f;
 
+    // The output correction reads w[(k * dm) mod M]; when dm is 0 or M/2 (a
+    // decimation of M/2, every block) that is w[0] or w[M/2], read once here.
+    const __attribute__((address_space(4))) float* tw_c =
+        reinterpret_cast<const __attribute__((address_space(4))) float*>(uintptr_t(a.tw));
+    const float2 tw_zero = make_float2(tw_c[0], tw_c[1]);
+    const float2 tw_half = make_float2(tw_c[M], tw_c[M + 1]);
+
     const uint32_t first = chunk * a.chunk;
     constexpr int PF = (TAPS > 0 && TAPS < kPrefetch) ? TAPS : kPrefetch;
     auto block_base = [&](uint32_t b0) {
@@ -346,6 +353,13 @@ chain_kernel(ChainArgs a) {
 #pragma unroll
             for (int i = 0; i < V; ++i) v[i] = make_float2(0.0f, 0.0f);
         }
+        // The transform's twiddles are read before the next block's codes are
+        // requested: memory operations of a wave complete in issue order, so a
+        // twiddle read issued after the prefetch would make the transform wait
+        // for the prefetch as well. (The output correction's few reads come
+        // after the transform, by which time the prefetch has landed.)
+        float2 wt[F::PASSES][F::WPP];
+        F::load_all_tw(tw, t, wt);
         if constexpr (TAPS > 0) {
             const uint32_t nb = b0 + S::BPW;
             if (nb < a.chunk && block_active(nb)) {
@@ -355,7 +369,7 @@ chain_kernel(ChainArgs a) {
             }
         }
 
-        F::template run_pass<0>(v, tw, 0);
+        F::template run_pass_w<0>(v, wt[0]);
         if constexpr (F::PASSES > 1) {
             // Everyone has finished reading the previous block's last pass.
             __syncthreads();
@@ -364,8 +378,8 @@ chain_kernel(ChainArgs a) {
             for (int r = 0; r < V; ++r) my[F::phys(g0 + r)] = v[r];
             __syncthreads();
             F::template load<1>(my, v, t);
-            F::template run_pass<1>(v, tw, t);
-            F::template rest<2>(my, v, tw, t);
+            F::template run_pass_w<1>(v, wt[1]);
+            F::template rest_w<2>(my, v, wt, t);
         }
 
         const uint32_t m_lo = uint32_t(m);
@@ -376,7 +390,17 @@ chain_kernel(ChainArgs a) {
         for (int r = 0; r < V; ++r) {
             const uint32_t k = F::template pos<LAST>(t, r);
             float2 value = v[r];
-            if (a.decimation != M) value = rvx::cmul_exact(value, tw[(k * dm) & (M - 1)]);
+            if (a.decimation != M) {
+                float2 c;
+                if ((dm & (M / 2 - 1)) == 0) {
+                    const bool half = dm != 0 && (k & 1u) != 0u;
+                    c.x = half ? tw_half.x : tw_zero.x;
+                    c.y = half ? tw_half.y : tw_zero.y;
+                } else {
+                    c = tw[(k * dm) & (M - 1)];
+                }
+                value = rvx::cmul_exact(value, c);
+            }
             if constexpr (STAGE > 1) {
                 stage[(b0 % STAGE) * M + k] = value;
             } else {

@@ -124,6 +124,79 @@ struct Fft {
         }
     }
 
+    // Twiddles of pass P for thread t held in registers: for each group u and
+    // stage l, the 2^l distinct values w[(j + (m << S0)) * (N >> (s + 1))],
+    // m = 0 .. 2^l - 1, the ones run_pass reads at i & (2^l - 1). A pass has
+    // at most V - 1 of them. Loading them apart from the butterflies lets a
+    // caller issue the reads before other memory traffic of its own.
+    //
+    // The upper half of a stage's values sits N/4 entries past the lower half
+    // (below N/4), and the table is exactly quadrant-symmetric: entry k + N/4
+    // is entry k turned by -i, (x, y) -> (y, -x), bit for bit (the reference
+    // builds it from the first octant with exact negations). So only the lower
+    // half is read and the upper half is formed in registers: the same values
+    // the table holds, in 4 reads instead of 7 for a radix-8 pass.
+    static constexpr int WPP = V - 1;
+    template <int P>
+    __device__ __forceinline__ static void load_tw(const float2* __restrict__ tw, uint32_t t,
+                                                   float2 (&w)[WPP]) {
+        constexpr int S0 = P * LOGV;
+        constexpr int LS = stages_in(P);
+        constexpr int G = 1 << LS;
+#pragma unroll
+        for (int u = 0; u < V / G; ++u) {
+            const uint32_t g = t * (V / G) + u;
+            const uint32_t j = g & ((1u << S0) - 1u);
+#pragma unroll
+            for (int l = 0; l < LS; ++l) {
+                const int s = S0 + l;
+#pragma unroll
+                for (int m = 0; m < (1 << l); ++m) {
+                    const uint32_t idx = (j + (uint32_t(m) << S0)) * uint32_t(N >> (s + 1));
+                    if (l >= 1 && m >= (1 << (l - 1))) {
+                        const float2 lo = w[u * (G - 1) + (1 << l) - 1 + m - (1 << (l - 1))];
+                        w[u * (G - 1) + (1 << l) - 1 + m] = make_float2(lo.y, -lo.x);
+                    } else if constexpr (P == 0) {
+                        // pass 0 runs at t = 0: every lane reads the same entries,
+                        // which scalar loads bring into scalar registers
+                        const __attribute__((address_space(4))) float* c =
+                            reinterpret_cast<const __attribute__((address_space(4))) float*>(
+                                uintptr_t(tw));
+                        w[u * (G - 1) + (1 << l) - 1 + m] = make_float2(c[2 * idx], c[2 * idx + 1]);
+                    } else {
+                        w[u * (G - 1) + (1 << l) - 1 + m] = tw[idx];
+                    }
+                }
+            }
+        }
+    }
+    template <int P>
+    __device__ __forceinline__ static void run_pass_w(float2 (&v)[V], const float2 (&w)[WPP]) {
+        constexpr int LS = stages_in(P);
+        constexpr int G = 1 << LS;
+#pragma unroll
+        for (int u = 0; u < V / G; ++u) {
+#pragma unroll
+            for (int l = 0; l < LS; ++l) {
+#pragma unroll
+                for (int i = 0; i < G; ++i) {
+                    if (i & (1 << l)) continue;
+                    butterfly(v[u * G + i], v[u * G + i + (1 << l)],
+                              w[u * (G - 1) + (1 << l) - 1 + (i & ((1 << l) - 1))]);
+                }
+            }
+        }
+    }
+    // Every pass's twiddles for thread t (pass 0 at t = 0, as the callers run it).
+    template <int P = 0>
+    __device__ __forceinline__ static void load_all_tw(const float2* __restrict__ tw, uint32_t t,
+                                                       float2 (&w)[PASSES][WPP]) {
+        if constexpr (P < PASSES) {
+            load_tw<P>(tw, P == 0 ? 0u : t, w[P]);
+            load_all_tw<P + 1>(tw, t, w);
+        }
+    }
+
     template <int P>
     __device__ __forceinline__ static void store(float2* lds, const float2 (&v)[V], uint32_t t) {
 #pragma unroll
@@ -148,6 +221,19 @@ struct Fft {
             load<P>(lds, v, t);
             run_pass<P>(v, tw, t);
             rest<P + 1>(lds, v, tw, t);
+        }
+    }
+
+    // rest() with the twiddles already in registers.
+    template <int P = 1>
+    __device__ __forceinline__ static void rest_w(float2* lds, float2 (&v)[V],
+                                                  const float2 (&w)[PASSES][WPP], uint32_t t) {
+        if constexpr (P < PASSES) {
+            store<P - 1>(lds, v, t);
+            __syncthreads();
+            load<P>(lds, v, t);
+            run_pass_w<P>(v, w[P]);
+            rest_w<P + 1>(lds, v, w, t);
         }
     }
 

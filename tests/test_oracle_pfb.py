@@ -29,6 +29,17 @@ def test_twiddles_closed_form(M):
         assert (w[j].real, w[j].imag) == want[j], j
 
 
+@pytest.mark.parametrize("M", [4, 8, 64, 512, 2048, 4096, 16384])
+def test_twiddles_are_exactly_quadrant_symmetric(M):
+    """Entry k + M/4 is entry k turned by -i, (x, y) -> (y, -x), bit for bit,
+    for k below M/4: the chain kernel reads only the lower half of each
+    stage's twiddles and forms the upper half this way."""
+    t = build_twiddles(M).view(np.float32).reshape(M, 2)
+    lo, hi = t[:M // 4], t[M // 4:M // 2]
+    assert np.array_equal(hi[:, 0].view(np.uint32), lo[:, 1].view(np.uint32))
+    assert np.array_equal(hi[:, 1].view(np.uint32), (-lo[:, 0]).view(np.uint32))
+
+
 @pytest.mark.parametrize("M,L1", [(64, 17), (64, 16), (4096, 16), (8, 5)])
 def test_prototype_invariants(M, L1):
     g = GridParams(M, L1, M // 2)
