@@ -180,6 +180,12 @@ constexpr int kPrefetch = 4;
 // blocks make a 32-byte run per channel, a whole memory sector.
 constexpr int kStage = 4;
 
+// Transform passes whose twiddles are read once per chunk rather than per
+// block (pass 0's by scalar loads). Three of the 4096-point transform's four
+// fit beside the filter's registers without spilling and measured fastest
+// (all four: no faster; per block: 3% slower).
+constexpr int kHoistPasses = 3;
+
 // TAPS > 0: the tap count is a compile-time constant; REGTAPS then keeps the
 // thread's V*TAPS taps in registers for the whole chunk, otherwise they are
 // read (L1/L2-resident) alongside the samples, through the same prefetch.
@@ -247,6 +253,13 @@ chain_kernel(ChainArgs a) {
         reinterpret_cast<const __attribute__((address_space(4))) float*>(uintptr_t(a.tw));
     const float2 tw_zero = make_float2(tw_c[0], tw_c[1]);
     const float2 tw_half = make_float2(tw_c[M], tw_c[M + 1]);
+
+    // The transform's twiddles depend on the thread, not the block: the first
+    // kHoist passes' stay in registers for the whole chunk, the rest are read
+    // per block (registers do not hold them all beside the filter's).
+    constexpr int kHoist = kHoistPasses < F::PASSES ? kHoistPasses : F::PASSES;
+    float2 wt[F::PASSES][F::WPP];
+    F::template load_all_tw<0, kHoist>(a.tw, t, wt);
 
     const uint32_t first = chunk * a.chunk;
     constexpr int PF = (TAPS > 0 && TAPS < kPrefetch) ? TAPS : kPrefetch;
@@ -358,8 +371,7 @@ chain_kernel(ChainArgs a) {
         // twiddle read issued after the prefetch would make the transform wait
         // for the prefetch as well. (The output correction's few reads come
         // after the transform, by which time the prefetch has landed.)
-        float2 wt[F::PASSES][F::WPP];
-        F::load_all_tw(tw, t, wt);
+        F::template load_all_tw<kHoist>(tw, t, wt);
         if constexpr (TAPS > 0) {
             const uint32_t nb = b0 + S::BPW;
             if (nb < a.chunk && block_active(nb)) {

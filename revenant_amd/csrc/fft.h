@@ -187,13 +187,14 @@ struct Fft {
             }
         }
     }
-    // Every pass's twiddles for thread t (pass 0 at t = 0, as the callers run it).
-    template <int P = 0>
+    // The twiddles of passes P .. END - 1 for thread t (pass 0 at t = 0, as
+    // the callers run it).
+    template <int P = 0, int END = PASSES>
     __device__ __forceinline__ static void load_all_tw(const float2* __restrict__ tw, uint32_t t,
                                                        float2 (&w)[PASSES][WPP]) {
-        if constexpr (P < PASSES) {
+        if constexpr (P < END && P < PASSES) {
             load_tw<P>(tw, P == 0 ? 0u : t, w[P]);
-            load_all_tw<P + 1>(tw, t, w);
+            load_all_tw<P + 1, END>(tw, t, w);
         }
     }
 
