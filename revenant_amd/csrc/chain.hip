@@ -181,9 +181,10 @@ constexpr int kPrefetch = 4;
 constexpr int kStage = 4;
 
 // Transform passes whose twiddles are read once per chunk rather than per
-// block (pass 0's by scalar loads). Three of the 4096-point transform's four
-// fit beside the filter's registers without spilling and measured fastest
-// (all four: no faster; per block: 3% slower).
+// block (pass 0's by scalar loads). Three of a radix-8 transform's passes fit
+// beside the filter's registers without spilling and measured fastest at 4096
+// points (all four: no faster; per block: 3% slower); the radix-4 shapes hold
+// every pass's.
 constexpr int kHoistPasses = 3;
 
 // TAPS > 0: the tap count is a compile-time constant; REGTAPS then keeps the
@@ -257,7 +258,8 @@ chain_kernel(ChainArgs a) {
     // The transform's twiddles depend on the thread, not the block: the first
     // kHoist passes' stay in registers for the whole chunk, the rest are read
     // per block (registers do not hold them all beside the filter's).
-    constexpr int kHoist = kHoistPasses < F::PASSES ? kHoistPasses : F::PASSES;
+    constexpr int kHoist =
+        (F::LOG_V < 3 || kHoistPasses >= F::PASSES) ? F::PASSES : kHoistPasses;
     float2 wt[F::PASSES][F::WPP];
     F::template load_all_tw<0, kHoist>(a.tw, t, wt);
 
