@@ -146,17 +146,24 @@ __device__ __forceinline__ float2 fine_output_row(const rvx_vrx_fine_config& c,
     const uint32_t carry = acc_idx / p.out_rate;
     const uint32_t n_rel = i * p.step_whole + carry;
     const uint32_t base = p.in_offset + n_rel;
+    // The ring through the global address space: its reads then wait on the
+    // vector-memory counter alone, not on the LDS reads of the taps as well,
+    // and eight are in flight per wait.
+    typedef float ring_f2 __attribute__((ext_vector_type(2)));
+    const __attribute__((address_space(1))) ring_f2* gch =
+        reinterpret_cast<const __attribute__((address_space(1))) ring_f2*>(uintptr_t(ch));
     float ar = 0.0f, ai = 0.0f;
     uint32_t k = 0;
-    for (; k + 4 <= c.taps; k += 4) {
-        float2 s[4], t[4];
+    for (; k + 8 <= c.taps; k += 8) {
+        float2 s[8], t[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            s[u] = ch[(base - (k + u)) & p.chan_mask];
+        for (int u = 0; u < 8; ++u) {
+            const ring_f2 v = gch[(base - (k + u)) & p.chan_mask];
+            s[u] = make_float2(v.x, v.y);
             t[u] = row[k + u];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 8; ++u) {
             ar = ar + (s[u].x * t[u].x - s[u].y * t[u].y);
             ai = ai + (s[u].x * t[u].y + s[u].y * t[u].x);
         }
