@@ -17,7 +17,7 @@ import pytest
 from helpers import assert_bit_exact, rng
 from oracle import oracle
 from revenant_amd.dsp import (CARRIER_NONE, DEMOD_AM, DEMOD_CW, DEMOD_DSB, DEMOD_NFM, DEMOD_SAM,
-                              DEMOD_USB, FORMAT_CS16, GridParams,
+                              DEMOD_USB, DEMOD_WFM, FORMAT_CS16, GridParams,
                               VrxRequest, build_nco_table, build_twiddles, design_prototype,
                               ema_frame_decay, ema_params, plan_vrx)
 
@@ -32,7 +32,9 @@ ALPHA = np.float32(1.0 / 16.0)
 REQUESTS = [VrxRequest(200_000, DEMOD_NFM, 12500), VrxRequest(-333_333, DEMOD_AM, 10000),
             VrxRequest(510_000, DEMOD_USB, 2700), VrxRequest(-1_000_500, DEMOD_CW, 500,
                                                                  cw_pitch=700),
-            VrxRequest(850_000, DEMOD_SAM, 10000), VrxRequest(-650_000, DEMOD_DSB, 6000)]
+            VrxRequest(850_000, DEMOD_SAM, 10000), VrxRequest(-650_000, DEMOD_DSB, 6000),
+            VrxRequest(-150_000, DEMOD_WFM, 60_000)]  # FM stereo, 75 us, as planned by default
+STEREO = 6
 
 
 @pytest.fixture(scope="module")
@@ -54,6 +56,16 @@ def stream():
         codes[1::2] = np.clip(codes[1::2] + z.imag, -32768, 32767).astype(np.int16)
     proto = design_prototype(GRID, beta=8.0).taps
     plans = [plan_vrx(GRID, RATE, r) for r in REQUESTS]
+    # a pilot-tone stereo multiplex (1 kHz left, 3 kHz right) on the WFM
+    # receiver, at its deviation
+    wfm = plans[STEREO].plan
+    left, right = np.sin(2 * np.pi * 1000.0 * t / RATE), np.sin(2 * np.pi * 3000.0 * t / RATE)
+    wp = 2 * np.pi * 19_000.0 * t / RATE
+    m = 0.4 * ((left + right) / 2 + (left - right) / 2 * np.sin(2 * wp)) + 0.1 * np.sin(wp)
+    phase = 2 * np.pi * wfm.deviation * np.cumsum(m) / RATE
+    z = 5000 * np.exp(1j * (2 * np.pi * -150_000 * t / RATE + phase))
+    codes[0::2] = np.clip(codes[0::2] + z.real, -32768, 32767).astype(np.int16)
+    codes[1::2] = np.clip(codes[1::2] + z.imag, -32768, 32767).astype(np.int16)
     return codes.view(np.uint32), proto, plans
 
 
@@ -158,6 +170,14 @@ def test_engine_matches_oracle(dev, stream, oracle_run):
         ref = _oracle_audio(p, ring, cap, a0, got.size // p.demod.channels)
         assert_bit_exact(got, ref, what=f"engine receiver {v} vs oracle")
         assert np.abs(got).max() > 0
+    # The stereo receiver runs the stereo branch with its pilot gate open (L and
+    # R differ). A 60 kHz receiver passes +-30 kHz and so not the 38 kHz
+    # subcarrier: separation is checked where the whole station reaches the
+    # demodulator (test_stereo.py, test_gpu_vrx.py).
+    p, (a0, got) = plans[STEREO], audio[STEREO]
+    assert p.demod.channels == 2 and p.demod.pilot_taps > 0
+    lch, rch = got[0::2], got[1::2]
+    assert np.count_nonzero(lch == rch) * 20 < lch.size
 
 
 def test_engine_late_start_and_receiver_removal(dev, stream, oracle_run):
