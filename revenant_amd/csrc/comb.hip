@@ -82,7 +82,25 @@ project_kernel(rvx_comb_params p, uint32_t group, const float2* __restrict__ raw
             const uint32_t m0 = c * kChunk;
             const uint32_t m1 = min(m0 + kChunk, length);
             const float2* row = stage + lane * kRow;
-            for (uint32_t m = m0; m < m1; ++m) {
+            // eight samples' LDS reads in flight per wait; the sums still run
+            // in sample order
+            uint32_t m = m0;
+            for (; m + 8 <= m1; m += 8) {
+                float2 xs[8], ts[8];
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) {
+                    xs[u] = row[m - m0 + u];
+                    ts[u] = tone(table, line, begin + m + u);
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) {
+                    const float re = xs[u].x * ts[u].x + xs[u].y * ts[u].y;
+                    const float im = xs[u].y * ts[u].x - xs[u].x * ts[u].y;
+                    chunk_re = chunk_re + re;
+                    chunk_im = chunk_im + im;
+                }
+            }
+            for (; m < m1; ++m) {
                 const float2 x = row[m - m0];
                 const float2 t = tone(table, line, begin + m);
                 const float re = x.x * t.x + x.y * t.y;
