@@ -137,8 +137,12 @@ apply_kernel(rvx_excision_config c, rvx_excision_params p, const float2* raw,
     const int64_t rel = j;
     const uint32_t slot = (p.first + j) & p.ring_mask;
     const float2 x = raw[slot];
-    bool here = false;
-    for (uint32_t m = 0; m <= c.hang + c.lead; ++m) here = here || tile[threadIdx.x + m] != 0u;
+    // any flag in [j - hang, j + lead]: a bitwise OR over the window, so the
+    // reads go out together rather than one per wait as a short-circuit would
+    uint32_t any = 0u;
+#pragma unroll 8
+    for (uint32_t m = 0; m <= c.hang + c.lead; ++m) any |= tile[threadIdx.x + m];
+    const bool here = any != 0u;
     bool excise = false;
     if (here) {
         uint32_t left = 0, right = 0;
