@@ -63,9 +63,6 @@ ALPHA = 1.0 / 16.0
 SEED = 0x5743454E45000002
 RX_SEED = 0x5652584241544348
 AUDIO_RATE = 48000
-# (mode name, count, bandwidth Hz) of the 256-receiver mix
-MIX = [("nfm", 64, 12500), ("am", 64, 10000), ("wfm", 32, 200_000), ("usb", 64, 2700),
-       ("cw", 32, 500)]
 
 
 def parse():
@@ -139,27 +136,11 @@ class Grid:
 
 
 def build_receivers(main: Grid):
-    """The 256-receiver mix, placed and planned; returns (receivers, ranges)."""
-    from revenant_amd.dsp import (DEMOD_AM, DEMOD_CW, DEMOD_NFM, DEMOD_USB, DEMOD_WFM,
-                                  VrxRequest, place, plan_vrx)
-
-    modes = {"nfm": DEMOD_NFM, "am": DEMOD_AM, "wfm": DEMOD_WFM, "usb": DEMOD_USB,
-             "cw": DEMOD_CW}
-    g = np.random.default_rng(RX_SEED)
-    reqs = []
-    for name, count, bw in MIX:
-        for _ in range(count):
-            f = int(g.integers(-int(0.45 * RATE), int(0.45 * RATE)))
-            reqs.append((name, VrxRequest(f, modes[name], bw, AUDIO_RATE,
-                                          cw_pitch=700 if name == "cw" else 0)))
+    """The 256-receiver mix, placed and planned; returns (receivers, wide channels)."""
     from revenant_amd.dsp import GridParams
-    wide = GridParams(WM, TAPS, WD)
-    plans = []
-    for name, req in reqs:
-        grid = wide if name == "wfm" else main.params
-        plans.append((name, plan_vrx(grid, RATE, req, place(grid, RATE, req))))
-    wide_set = sorted({p.plan.placement.channel for n, p in plans if n == "wfm"})
-    return plans, wide_set
+    from revenant_amd.receivers import standard_mix
+
+    return standard_mix(RATE, main.params, GridParams(WM, TAPS, WD), RX_SEED, AUDIO_RATE)
 
 
 def main() -> int:

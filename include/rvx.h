@@ -324,6 +324,19 @@ typedef struct rvx_vrx_plan {
     uint32_t stereo;  /* resolve_stereo (core/engine/vrx.h:381): L and R, a pilot filter */
 } rvx_vrx_plan;
 
+/* core/engine/vrx_place.cpp:187 channel_count_for: the largest power of two M
+ * with rate/M >= widest_receiver_hz, at least 2 (also for a width <= 0). The
+ * reference narrows the 64-bit answer to 32 bits; this one stops at 2^31. */
+uint32_t rvx_channel_count_for(int64_t rate, int64_t widest_receiver_hz);
+/* core/dsp/pfb_design.cpp:349 channel_centre: channel k's centre k*rate/M as a
+ * reduced rational, channels from M/2 up negative (DC is 0/1). */
+int32_t rvx_channel_centre(const rvx_grid* grid, int64_t rate, uint32_t channel,
+                           int64_t* numerator, int64_t* denominator);
+/* core/dsp/pfb_design.cpp:593 channel_sample_to_input: base + m*D minus the
+ * prototype's group delay (rvx_design_prototype's *group_delay, M*L/2),
+ * modulo 2^64. */
+uint64_t rvx_channel_sample_to_input(const rvx_grid* grid, uint64_t group_delay_samples,
+                                     uint64_t base, uint64_t m);
 /* core/engine/vrx_place.cpp:216 place: nearest channel round(center*M/rate),
  * exact rational residual, refusal when the channel cannot carry the band. */
 int32_t rvx_place(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* request,
@@ -1159,6 +1172,10 @@ typedef struct rvx_engine_stats {
     uint64_t spectrum_rows;    /* averaged-spectrum rows folded into the state */
     uint64_t audio_lost;       /* receiver audio outputs skipped because their
                                 * channel samples had left the ring */
+    uint64_t audio_overrun;    /* audio outputs computed but overwritten in a
+                                * receiver's audio ring before rvx_engine_read_audio
+                                * copied them (the reader fell more than the ring
+                                * behind) */
 } rvx_engine_stats;
 
 /* The front-end estimate and the correction the newest push's samples took
@@ -1212,7 +1229,10 @@ int32_t rvx_engine_push(rvx_engine* engine, const void* codes, uint64_t samples,
                         int32_t on_device);
 /* Copies the receiver's unread audio (oldest first, at most max_values floats)
  * to host memory and waits for it; *first_audio is the index of the first
- * output copied, *values the floats copied. */
+ * output copied, *values the floats copied. It moves the same per-receiver
+ * cursors rvx_engine_push moves: calls of the two on one engine must be
+ * serialised by the caller (one lock around both), like every call on a
+ * context. */
 int32_t rvx_engine_read_audio(rvx_engine* engine, uint32_t id, float* out, uint64_t max_values,
                               uint64_t* first_audio, uint64_t* values);
 /* The averaged spectrum state (M floats, host) after the last row; waits. */

@@ -273,7 +273,7 @@ class EngineConfig(C.Structure):
 
 class EngineStats(C.Structure):
     _fields_ = [(n, u64) for n in ("samples_in", "next_block", "blocks", "spectrum_rows",
-                                   "audio_lost")]
+                                   "audio_lost", "audio_overrun")]
 
 
 class DetectConfig(C.Structure):
@@ -333,6 +333,9 @@ _SIGNATURES = {
     "rvx_vrx_demod": (i32, [vp, C.POINTER(VrxDemodConfig), C.POINTER(VrxDemodParams), vp, u64,
                             vp, u64, vp, u64]),
     "rvx_place": (i32, [C.POINTER(Grid), i64, C.POINTER(VrxRequest), C.POINTER(VrxPlacement)]),
+    "rvx_channel_count_for": (u32, [i64, i64]),
+    "rvx_channel_centre": (i32, [C.POINTER(Grid), i64, u32, C.POINTER(i64), C.POINTER(i64)]),
+    "rvx_channel_sample_to_input": (u64, [C.POINTER(Grid), u64, u64, u64]),
     "rvx_plan_vrx": (i32, [C.POINTER(Grid), i64, C.POINTER(VrxRequest), C.POINTER(VrxPlacement),
                            C.POINTER(VrxPlan)]),
     "rvx_plan_vrx_tables": (i32, [C.POINTER(VrxPlan), fptr, u64, fptr, u64]),

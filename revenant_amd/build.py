@@ -10,7 +10,8 @@ are part of the contract the kernels keep with the reference twins:
   output, the twins' FTZ|DAZ (core/dsp/denormal_mode.h).
 * ``-Xclang -target-feature -Xclang -packed-fp32-ops``: no packed FP32 ops.
 
-``build_oracle()`` runs ``make`` in ``oracle/`` (test infrastructure).
+``build_oracle()`` runs ``make`` in ``oracle/`` and ``build_negative_control()``
+compiles the suite's deliberately wrong kernel (both test infrastructure).
 """
 from __future__ import annotations
 
@@ -87,9 +88,27 @@ def build_oracle() -> Path:
     return ORACLE / "liboracle.so"
 
 
+NEGATIVE = ROOT / "tests" / "negative_control"
+NEGATIVE_LIB = NEGATIVE / "libcmul_wrong.so"
+
+
+def build_negative_control() -> Path:
+    """The conformance suite's deliberately wrong kernel, in a library of its
+    own under tests/ (the product library never contains it)."""
+    src = NEGATIVE / "cmul_wrong.hip"
+    tmp = NEGATIVE_LIB.with_suffix(".so.tmp")
+    cmd = [hipcc(), *HIP_FLAGS, "-x", "hip", "-shared", str(src), "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, NEGATIVE_LIB)
+    return NEGATIVE_LIB
+
+
 def build_all() -> None:
     build_oracle()
     build_library()
+    build_negative_control()
 
 
 if __name__ == "__main__":

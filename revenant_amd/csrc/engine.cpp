@@ -585,6 +585,16 @@ int32_t blank_channel(rvx_engine& e, EngineRx& r, uint64_t oldest, uint64_t newe
     return RVX_OK;
 }
 
+// The audio ring holds the last cap outputs: once the writer is more than cap
+// ahead of the reader, the oldest unread outputs have been overwritten. They
+// are counted, and the reader moves to the oldest output still held.
+void reader_follows(rvx_engine& e, EngineRx& r) {
+    if (r.next_audio - r.read_audio > r.cap) {
+        e.stats.audio_overrun += (r.next_audio - r.cap) - r.read_audio;
+        r.read_audio = r.next_audio - r.cap;
+    }
+}
+
 int32_t engine_receivers(rvx_engine& e, const char* who) {
     const uint64_t newest = e.next_block - 1;
     const uint64_t ring = e.cfg.ring_blocks;
@@ -641,7 +651,7 @@ int32_t engine_receivers(rvx_engine& e, const char* who) {
             jobs.push_back({&r, r.next_audio, count});
             items.push_back(it);
             r.next_audio += count;
-            if (r.next_audio - r.read_audio > r.cap) r.read_audio = r.next_audio - r.cap;
+            reader_follows(e, r);
             continue;
         }
         const uint32_t ch = r.plan.demod.channels;
@@ -665,7 +675,7 @@ int32_t engine_receivers(rvx_engine& e, const char* who) {
             left -= seg;
         }
         r.next_audio += count;
-        if (r.next_audio - r.read_audio > r.cap) r.read_audio = r.next_audio - r.cap;
+        reader_follows(e, r);
     }
     if (int32_t rc = carrier_receivers(e, jobs, items, who)) return rc;
     if (int32_t rc = batch_receivers(e, spans, who)) return rc;

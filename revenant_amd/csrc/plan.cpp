@@ -372,6 +372,47 @@ static int32_t resample_block(const rvx_vrx_fine_config& fine, int64_t channel_r
 
 extern "C" {
 
+uint32_t rvx_channel_count_for(int64_t rate, int64_t widest_receiver_hz) {
+    // The largest power-of-two channel count whose channels are each at least
+    // as wide as the widest receiver: rate / M >= widest. Two is the floor (a
+    // single channel channelizes nothing), also for a zero or negative width.
+    if (rate <= 0 || widest_receiver_hz <= 0) return 2;
+    const uint64_t fits = uint64_t(rate) / uint64_t(widest_receiver_hz);
+    if (fits < 2) return 2;
+    uint64_t m = 1;
+    while (m <= fits / 2 && m < (uint64_t(1) << 31)) m <<= 1;
+    return uint32_t(m);
+}
+
+int32_t rvx_channel_centre(const rvx_grid* grid, int64_t rate, uint32_t channel,
+                           int64_t* numerator, int64_t* denominator) {
+    static const char* who = "rvx_channel_centre";
+    if (!grid || !numerator || !denominator) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    int64_t num = 0, den = 1;
+    const int64_t M = grid->channels;
+    if (M != 0) {
+        // k*rate/M folded so that channels from M/2 up are negative frequencies,
+        // reduced so that equal frequencies compare equal (DC is 0/1).
+        const int64_t k = int64_t(channel % uint32_t(M));
+        num = (k < M / 2 ? k : k - M) * rate;
+        den = M;
+        reduce(num, den);
+    }
+    *numerator = num;
+    *denominator = den;
+    return RVX_OK;
+}
+
+uint64_t rvx_channel_sample_to_input(const rvx_grid* grid, uint64_t group_delay_samples,
+                                     uint64_t base, uint64_t m) {
+    // Channel sample m is taken at input base + m*D and the linear-phase
+    // prototype delays by its group delay, so it is centred on
+    // base + m*D - delay. Modulo 2^64: a sample from before the filter filled
+    // maps near 2^64, visibly impossible, rather than clamping to zero.
+    const uint64_t d = grid ? grid->decimation : 0;
+    return base + m * d - group_delay_samples;
+}
+
 int32_t rvx_place(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* request,
                   rvx_vrx_placement* out) {
     static const char* who = "rvx_place";
@@ -396,10 +437,7 @@ int32_t rvx_place(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* req
     rvx_vrx_placement p{};
     p.channel = uint32_t(idx);
     // Channel k's centre k*rate/M folded into [-rate/2, rate/2), reduced.
-    const int64_t signed_k = idx < M / 2 ? idx : idx - M;
-    p.centre_num = signed_k * rate;
-    p.centre_den = M;
-    reduce(p.centre_num, p.centre_den);
+    rvx_channel_centre(&g, rate, p.channel, &p.centre_num, &p.centre_den);
     p.channel_rate = rate / int64_t(g.decimation);
     p.residual_num = request->center_hz * p.centre_den - p.centre_num;
     p.residual_den = p.centre_den;

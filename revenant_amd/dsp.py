@@ -231,6 +231,30 @@ def carrier_check(config: _lib.VrxCarrierConfig, params: _lib.VrxCarrierParams) 
     _lib.check(_lib.lib().rvx_vrx_carrier_check(C.byref(config), C.byref(params)))
 
 
+def channel_count_for(rate: int, widest_receiver_hz: int) -> int:
+    """core/engine/vrx_place.cpp:187: the channel count whose channels carry a
+    receiver of the given width wherever it lands."""
+    return int(_lib.lib().rvx_channel_count_for(int(rate), int(widest_receiver_hz)))
+
+
+def channel_centre(grid: GridParams, rate: int, channel: int) -> tuple[int, int]:
+    """core/dsp/pfb_design.cpp:349: (numerator, denominator) Hz, reduced."""
+    num, den = C.c_int64(), C.c_int64()
+    g = grid.c()
+    _lib.check(_lib.lib().rvx_channel_centre(C.byref(g), int(rate), int(channel), C.byref(num),
+                                             C.byref(den)))
+    return num.value, den.value
+
+
+def channel_sample_to_input(grid: GridParams, prototype: PrototypeFilter, base: int,
+                            m: int) -> int:
+    """core/dsp/pfb_design.cpp:593: the input index channel sample m is centred
+    on (modulo 2^64)."""
+    g = grid.c()
+    return int(_lib.lib().rvx_channel_sample_to_input(
+        C.byref(g), int(prototype.group_delay_samples), int(base) % 2**64, int(m) % 2**64))
+
+
 def place(grid: GridParams, rate: int, request: VrxRequest) -> _lib.VrxPlacement:
     """core/engine/vrx_place.cpp:216."""
     out = _lib.VrxPlacement()

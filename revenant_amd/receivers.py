@@ -27,6 +27,33 @@ from . import _lib
 from .dsp import VrxPlan
 from .ops import Device, _csamples, _p
 
+# The batching workload's receiver mix: (mode name, count, bandwidth Hz).
+STANDARD_MIX = (("nfm", 64, 12500), ("am", 64, 10000), ("wfm", 32, 200_000),
+                ("usb", 64, 2700), ("cw", 32, 500))
+
+
+def standard_mix(rate: int, narrow_grid, wide_grid, seed: int, audio_rate: int = 48000,
+                 mix=STANDARD_MIX):
+    """The mix's receivers at seeded uniform offsets within +-0.45 rate, placed
+    and planned: 200 kHz receivers on ``wide_grid`` (the grid channel_count_for
+    picks for them), the rest on ``narrow_grid``. Returns ([(name, plan)], the
+    sorted wide-grid channels the wide receivers use)."""
+    from .dsp import (DEMOD_AM, DEMOD_CW, DEMOD_NFM, DEMOD_USB, DEMOD_WFM, VrxRequest, place,
+                      plan_vrx)
+
+    modes = {"nfm": DEMOD_NFM, "am": DEMOD_AM, "wfm": DEMOD_WFM, "usb": DEMOD_USB,
+             "cw": DEMOD_CW}
+    g = np.random.default_rng(seed)
+    plans = []
+    for name, count, bw in mix:
+        for _ in range(count):
+            f = int(g.integers(-int(0.45 * rate), int(0.45 * rate)))
+            req = VrxRequest(f, modes[name], bw, audio_rate, cw_pitch=700 if name == "cw" else 0)
+            grid = wide_grid if name == "wfm" else narrow_grid
+            plans.append((name, plan_vrx(grid, rate, req, place(grid, rate, req))))
+    wide_set = sorted({p.plan.placement.channel for n, p in plans if n == "wfm"})
+    return plans, wide_set
+
 MAX_SPAN = _lib.VRX_TILE_SPAN  # fine samples one tile holds on chip
 _UNBOUNDED = 0xFFFFFFFF         # ring masks of the on-chip fine block: no wrap
 

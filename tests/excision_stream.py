@@ -70,8 +70,11 @@ class OraclePasses:
         return st[0]
 
 
-def run_stream(capture: np.ndarray, rate: int, block: int, threshold_db: float, passes):
+def run_stream(capture: np.ndarray, rate: int, block: int, threshold_db: float, passes,
+               guard: int | None = None):
     cfg = design_excision(rate)
+    if guard is not None:  # hang and lead overridden (the kernels take up to 16)
+        cfg.hang = cfg.lead = guard
     ring_n = 1 << (capture.size - 1).bit_length()
     mask = ring_n - 1
     rings = size_excision_rings(cfg, block, 1, mask)

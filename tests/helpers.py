@@ -1,8 +1,9 @@
 """Shared test plumbing: seeded inputs and the bit-exact diff.
 
 The diff follows tests/reference/reference_diff.h:98-152 of the reference:
-compare bit patterns (so -0.0 != +0.0 and NaN payloads count), and on failure
-name the seed, the first divergent index and both patterns.
+compare bit patterns (so -0.0 != +0.0), treat a NaN as matching any NaN (its
+sign and payload are the vendor's choice), and on failure name the seed, the
+first divergent index and both patterns.
 """
 from __future__ import annotations
 
@@ -45,38 +46,125 @@ def bits(a) -> np.ndarray:
     return a.view(np.uint32).ravel()
 
 
-def assert_bit_exact(gpu, cpu, seed: int | None = None, what: str = "") -> None:
-    """Bit-identical, except that any NaN equals any NaN: an invalid operation's
-    NaN has a vendor-chosen sign and payload (x86 gives 0xFFC00000, gfx950
-    0x7FC00000), and the reference's harness compares NaNs the same way
-    (tests/reference/reference_diff.h:71-75). Signed zeros still differ."""
-    ga, ca = np.asarray(gpu), np.asarray(cpu)
-    if ga.dtype == np.float64 or ca.dtype == np.float64:
-        assert ga.dtype == ca.dtype == np.float64, (ga.dtype, ca.dtype)
-        g64, c64 = ga.view(np.uint64).ravel(), ca.view(np.uint64).ravel()
-        assert g64.size == c64.size, f"{what}: sizes differ {g64.size} vs {c64.size}"
-        nan = np.isnan(ga.ravel()) & np.isnan(ca.ravel())
-        bad = np.nonzero((g64 != c64) & ~nan)[0]
-        if bad.size:
-            i = int(bad[0])
-            raise AssertionError(f"{what}: {bad.size} of {g64.size} doubles differ; first at "
-                                 f"{i}: gpu {ga.ravel()[i]!r} vs twin {ca.ravel()[i]!r}")
-        return
-    g, c = bits(gpu), bits(cpu)
-    assert g.size == c.size, f"{what}: sizes differ {g.size} vs {c.size}"
-    nan_g = (g & 0x7F800000) == 0x7F800000
-    nan_g &= (g & 0x007FFFFF) != 0
-    nan_c = (c & 0x7F800000) == 0x7F800000
-    nan_c &= (c & 0x007FFFFF) != 0
-    bad = np.nonzero((g != c) & ~(nan_g & nan_c))[0]
+class DiffResult:
+    """The outcome of one comparison (tests/reference/reference_diff.h:45-62):
+    whether the buffers agree within the tolerance, how many elements were
+    compared, the first index that did not, the worst distance in units in the
+    last place and in absolute terms, and a report naming the seed, the index
+    and both bit patterns."""
+
+    def __init__(self) -> None:
+        self.identical = True
+        self.compared = 0
+        self.first_divergence = 0
+        self.max_ulp_error = 0
+        self.max_absolute_error = 0.0
+        self.report = ""
+
+    def __bool__(self) -> bool:
+        return self.identical
+
+
+def _flat(a) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(a))
+    if a.dtype == np.complex64:
+        a = a.view(np.float32)
+    elif a.dtype == np.complex128:
+        a = a.view(np.float64)
+    return a.ravel()
+
+
+def _ordered(v: np.ndarray) -> np.ndarray:
+    """Float bit patterns mapped onto one increasing integer line: +0 is 0, the
+    next float up is 1, -0 is -1, so the distance between two keys counts the
+    representable values between them and +0 and -0 are one step apart."""
+    if v.dtype.itemsize == 4:
+        raw = v.view(np.int32).astype(np.int64)
+        return np.where(raw < 0, -(raw & 0x7FFFFFFF) - 1, raw)
+    raw = v.view(np.int64)
+    return np.where(raw < 0, -(raw & np.int64(0x7FFFFFFFFFFFFFFF)) - 1, raw)
+
+
+def _distance(kg: np.ndarray, kc: np.ndarray) -> np.ndarray:
+    """|kg - kc| without int64 overflow: exact when the keys are equal or close
+    (below 2^53 apart), rounded beyond that, never 0 for different keys."""
+    far = np.abs(kg.astype(np.float64) - kc.astype(np.float64))
+    near = np.abs(np.where(np.sign(kg) == np.sign(kc), kg - kc, 0))
+    d = np.where(far < 2.0 ** 53, near.astype(np.float64), far)
+    return np.where(kg == kc, 0.0, np.maximum(d, 1.0))
+
+
+def diff(gpu, cpu, seed: int | None = None, tolerance_ulp: int = 0, what: str = "") -> DiffResult:
+    """Compare two buffers element by element.
+
+    Floating-point buffers are compared as bit patterns: the distance is the
+    number of representable values between the two (so +0 and -0 differ, and
+    one rounding decision is 1 ulp); a NaN matches only a NaN, whatever its
+    sign and payload, since an invalid operation's NaN is vendor-chosen (x86
+    gives 0xFFC00000, gfx950 0x7FC00000). Integer buffers must match exactly.
+    ``tolerance_ulp`` of 0 demands bit-identical output; a larger one must be
+    asked for by the caller, who says why (reference_diff.h:92-152)."""
+    g, c = _flat(gpu), _flat(cpu)
+    res = DiffResult()
+    if g.dtype != c.dtype:
+        res.identical = False
+        res.report = f"{what}: dtype mismatch: gpu {g.dtype}, reference {c.dtype}\n"
+        return res
+    if g.size != c.size:
+        res.identical = False
+        res.report = (f"{what}: size mismatch: gpu produced {g.size}, "
+                      f"reference produced {c.size}\n")
+        return res
+    res.compared = int(g.size)
+    if g.size == 0:
+        return res
+    if g.dtype.kind == "f":
+        nan_g, nan_c = np.isnan(g), np.isnan(c)
+        ulp = _distance(_ordered(g), _ordered(c))
+        ulp = np.where(nan_g & nan_c, 0.0, ulp)
+        ulp = np.where(nan_g ^ nan_c, np.inf, ulp)
+        both_finite = ~(nan_g | nan_c)
+        with np.errstate(invalid="ignore", over="ignore"):
+            absolute = np.where(both_finite, np.abs(g.astype(np.float64) - c.astype(np.float64)),
+                                0.0)
+        width = 8 if g.dtype.itemsize == 4 else 16
+        pattern = lambda x: f"0x{int(np.asarray(x).view(np.uint32 if width == 8 else np.uint64)):0{width}X}"
+    else:
+        wide = g.astype(np.float64) - c.astype(np.float64)
+        ulp = np.where(g == c, 0.0, np.maximum(np.abs(wide), 1.0))
+        absolute = np.abs(wide)
+        pattern = lambda x: str(x)
+    ulp = np.asarray(ulp, dtype=np.float64)
+    worst = float(ulp.max())
+    res.max_ulp_error = int(worst) if np.isfinite(worst) else worst
+    finite_abs = absolute[np.isfinite(absolute)]
+    res.max_absolute_error = float(finite_abs.max()) if finite_abs.size else 0.0
+    bad = np.nonzero(ulp > tolerance_ulp)[0]
     if bad.size:
-        i = int(bad[0])
-        gf = g[i:i + 1].view(np.float32)[0]
-        cf = c[i:i + 1].view(np.float32)[0]
-        tag = f" (seed {seed:#x})" if seed is not None else ""
-        raise AssertionError(
-            f"{what}{tag}: {bad.size} of {g.size} words differ; first at float {i}: "
-            f"gpu {gf!r} (0x{g[i]:08x}) vs twin {cf!r} (0x{c[i]:08x})")
+        res.identical = False
+        res.first_divergence = int(bad[0])
+        lines = []
+        for i in bad[:8]:
+            lines.append(f"  [{int(i)}] gpu={g[i]!r} ({pattern(g[i])})  cpu={c[i]!r} "
+                         f"({pattern(c[i])})  ulp={ulp[i]:.0f}")
+        tag = f"{seed:#x}" if seed is not None else "none"
+        res.report = (f"{what}: GPU output diverges from the oracle\n"
+                      f"  seed              {tag}\n"
+                      f"  elements compared {res.compared}\n"
+                      f"  divergent         {bad.size}\n"
+                      f"  first divergence  index {res.first_divergence}\n"
+                      f"  worst error       {res.max_ulp_error} ulp, "
+                      f"{res.max_absolute_error:.9g} absolute\n"
+                      f"  tolerance         {tolerance_ulp} ulp\n" + "\n".join(lines))
+    return res
+
+
+def assert_bit_exact(gpu, cpu, seed: int | None = None, what: str = "") -> None:
+    """Raise unless ``diff`` finds the buffers bit-identical (NaN matching any
+    NaN, signed zeros distinct)."""
+    res = diff(gpu, cpu, seed=seed, what=what)
+    if not res.identical:
+        raise AssertionError(res.report)
 
 
 def to_dev(a: np.ndarray):

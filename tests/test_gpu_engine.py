@@ -374,3 +374,39 @@ def test_engine_display_tap_and_pane(dev, stream, oracle_run):
     # the receiver's own tone (200 kHz sits at the receiver's centre) is the pane's peak
     assert abs(int(np.argmax(pane)) - n // 4) <= 2
     eng.close()
+
+
+def test_a_lagging_reader_loses_the_oldest_audio_and_it_is_counted(dev, stream):
+    """Audio the reader did not copy before the ring wrapped over it is
+    counted in audio_overrun; the reader resumes at the oldest output held,
+    which is bit-identical to the same output read in time."""
+    from revenant_amd.engine import Engine
+
+    codes, proto, plans = stream
+    plan = plans[0]  # NFM
+    cap = 1 << 12
+    runs = {}
+    for lag in (False, True):
+        eng = Engine(dev, GRID, proto, FORMAT_CS16, max_push=65536, ring_blocks=4096,
+                     frame_blocks=FRAME, row_frames=ROWS, alpha=float(ALPHA))
+        rid = eng.add(plan, audio_capacity=cap)
+        got, pos = [], 0
+        for n in [65536] * (SAMPLES // 65536):
+            eng.push(codes[pos:pos + n])
+            pos += n
+            if not lag:
+                got.append(eng.read_audio(rid))
+        if lag:
+            got.append(eng.read_audio(rid))
+        st = eng.stats()
+        runs[lag] = (got, st.audio_overrun, st.audio_lost)
+        eng.close()
+    in_time, overrun, lost = runs[False]
+    assert overrun == 0 and lost == 0
+    produced = sum(x.size for _, x in in_time)
+    (first, tail), = runs[True][0]
+    assert tail.size == cap
+    assert runs[True][1] == produced - cap
+    assert first == in_time[0][0] + produced - cap
+    whole = np.concatenate([x for _, x in in_time])
+    assert_bit_exact(tail, whole[-cap:], what="the audio the lagging reader still gets")

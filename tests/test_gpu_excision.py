@@ -29,12 +29,30 @@ def _pass(cfg, which):
     return c
 
 
-def test_four_passes_match_twins_through_a_wrap(dev):
-    ring_n, block = 1 << 15, 4096
+# (source rate, guard override): the narrowband design (hang = lead = 2),
+# the wideband one (hang = 12 at 122.88 MS/s) and the widest guards the
+# kernels take (hang = lead = 16, the apply pass's LDS window at its largest).
+DESIGNS = [(RATE, None), (122_880_000, None), (RATE, 16)]
+DESIGN_IDS = ["2.4MS/s", "122.88MS/s", "guard16"]
+
+
+def _design(rate, guard):
+    cfg = design_excision(rate)
+    if guard is not None:
+        cfg.hang = cfg.lead = guard
+    return cfg
+
+
+@pytest.mark.parametrize("rate,guard", DESIGNS, ids=DESIGN_IDS)
+def test_four_passes_match_twins_through_a_wrap(dev, rate, guard):
+    wide = rate > RATE
+    ring_n, block = (1 << 18, 1 << 15) if wide else (1 << 15, 4096)
     mask = ring_n - 1
-    cfg = design_excision(RATE)
+    cfg = _design(rate, guard)
+    print(f"excision design rate={rate} hang={cfg.hang} lead={cfg.lead} "
+          f"seg=2^{cfg.seg_log2} max_width={cfg.max_width}", flush=True)
     rings = size_excision_rings(cfg, block, 1, mask)
-    capture = impulsive_capture(ring_n, 0x455843495345, 0.05, 0.0123, 120, 6)
+    capture = impulsive_capture(ring_n, 0x455843495345, 0.05, 0.0123, 120 * (ring_n >> 15), 6)
     raw = np.ascontiguousarray(capture.view(np.float32).copy())
     power = np.zeros(rings.seg_mask + 1, np.float32)
     level = np.zeros_like(power)
@@ -133,14 +151,15 @@ class DevicePasses:
         return from_dev(st["raw"])
 
 
-def test_stream_on_device_matches_the_oracle(dev):
+@pytest.mark.parametrize("rate,guard", DESIGNS, ids=DESIGN_IDS)
+def test_stream_on_device_matches_the_oracle(dev, rate, guard):
     length = 1 << 19
     dirty = impulsive_capture(length, 20261009, 0.1, 0.0371, 40, 4)
-    print("excision stream on the device", flush=True)
-    g_clean, g_removed, g_counters, g_end, _ = run_stream(dirty, RATE, 65536, 15.0,
-                                                          DevicePasses(dev))
-    o_clean, o_removed, o_counters, o_end, _ = run_stream(dirty, RATE, 65536, 15.0,
-                                                          OraclePasses())
+    print(f"excision stream on the device rate={rate} guard={guard}", flush=True)
+    g_clean, g_removed, g_counters, g_end, _ = run_stream(dirty, rate, 65536, 15.0,
+                                                          DevicePasses(dev), guard=guard)
+    o_clean, o_removed, o_counters, o_end, _ = run_stream(dirty, rate, 65536, 15.0,
+                                                          OraclePasses(), guard=guard)
     assert g_end == o_end
     assert np.array_equal(g_counters, o_counters) and o_counters[1] >= 38
     assert_bit_exact(g_clean, o_clean, what="cleaned stream")
