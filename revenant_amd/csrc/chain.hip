@@ -480,13 +480,266 @@ chain_kernel(ChainArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// The two-times-oversampled grid (D = M/2) with the sample stream shared
+// between a branch pair.
+//
+// At D = M/2 the sample branch r reads for tap q of block m is
+// x[(m - 2q) D - r] and branch r + D reads x[(m - 2q - 1) D - r]: both walk the
+// one sequence s_r[j] = x[j D - r], one step apart. A thread owns its V
+// branches as V/2 such pairs (slots t + T i and t + T (i + V/2) are branches D
+// apart), so for a chunk of C blocks it needs C + 2 TAPS - 1 samples per pair
+// instead of 2 C TAPS: it loads and converts each once, newest first, and
+// adds it into every (branch, block) sum whose tap reaches it. Streaming the
+// samples newest first visits each sum's terms in ascending q, the twin's
+// order, so every sum is the same chain of roundings as before.
+//
+// All C blocks' branch sums stay in registers; each block's transform then
+// runs in place, so after the chunk a thread holds its channels' outputs of
+// all C blocks and writes each channel's run straight from registers (no LDS
+// staging). LDS holds one transform's exchange only.
+// ---------------------------------------------------------------------------
+template <int LOGM, int LOGV_, int FORMAT, int TAPS, int C, int SUBS>
+__global__ void __launch_bounds__((ChainShape<LOGM, LOGV_>::THREADS))
+chain_pair_kernel(ChainArgs a) {
+    using S = ChainShape<LOGM, LOGV_>;
+    using F = typename S::F;
+    constexpr uint32_t M = F::N;
+    constexpr uint32_t D = M / 2;
+    constexpr int V = F::V;
+    constexpr int P = V / 2;  // branch pairs per thread
+    constexpr int T = F::T;
+    constexpr int LOGT = F::LOGT;
+    constexpr int LAST = F::PASSES - 1;
+    constexpr int NS = C + 2 * TAPS - 1;  // samples per pair per sub-chunk
+    constexpr uint32_t BPS = is_cs16(FORMAT) ? 4u : 2u;
+
+    __shared__ float2 smem[S::BPW * M];
+
+    const uint32_t w = blockIdx.x;
+    const uint32_t per_xcd = gridDim.x / 8;
+    const uint32_t group = (w % 8) * per_xcd + w / 8;  // XCD-major numbering
+    const uint32_t sub = threadIdx.x / T;
+    const uint32_t t = threadIdx.x % T;
+    const uint32_t chunk = group * S::BPW + sub;
+    // A sub-transform past the last chunk still runs every barrier.
+    const bool live = chunk < a.chunks;
+    float2* my = smem + sub * M;
+
+    uint32_t branch[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) branch[i] = (M - (t + uint32_t(T) * i)) & (M - 1);
+    // Pair i is branches lo and lo + D. Slot t + T i holds lo only at t = 0,
+    // i = 0 (branch 0); every other pair's first slot holds lo + D.
+    uint32_t lo[P], voff[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        lo[i] = branch[i] < D ? branch[i] : branch[i + P];
+        voff[i] = (D - 1u - lo[i]) * BPS;
+    }
+    const bool lo_first = t == 0;
+    const __amdgpu_buffer_rsrc_t tap_rsrc = make_rsrc(a.taps, uint64_t(TAPS) * M * 4);
+
+    float2 wt[F::PASSES][F::WPP];
+    const __attribute__((address_space(4))) float* tw_c =
+        reinterpret_cast<const __attribute__((address_space(4))) float*>(uintptr_t(a.tw));
+    const float2 tw_zero = make_float2(tw_c[0], tw_c[1]);
+    const float2 tw_half = make_float2(tw_c[M], tw_c[M + 1]);
+
+    float ema[V];
+#pragma unroll
+    for (int r = 0; r < V; ++r) ema[r] = 0.0f;
+
+    for (int sc = 0; sc < SUBS; ++sc) {
+        const uint32_t first = chunk * (C * SUBS) + uint32_t(sc * C);  // block within the launch
+        // Oldest sample of the sub-chunk: j_min D - (D - 1), j_min = m0 - 2 TAPS + 1.
+        const uint32_t base0 = uint32_t((a.first_block + first) * a.decimation - a.input_origin);
+        const uint32_t window0 = base0 - uint32_t(TAPS) * M + 1u;
+        const __amdgpu_buffer_rsrc_t in_rsrc = make_rsrc(
+            static_cast<const uint8_t*>(a.input) + uint64_t(window0) * BPS,
+            live ? (a.input_samples - window0) * BPS : 0);
+
+        // acc[i][b]: pair i's sums for block b, (x, y) of lo then of lo + D
+        float4 acc[P][C];
+#pragma unroll
+        for (int i = 0; i < P; ++i)
+#pragma unroll
+            for (int b = 0; b < C; ++b) acc[i][b] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+
+        // Each pair's samples in segments of SEG, a segment's codes and the taps
+        // it first reaches loaded one segment ahead of its arithmetic. The
+        // scheduling barriers stop later loads from being hoisted further, so
+        // two segments of loads are in registers beside the sums.
+        constexpr int SEG = 8;
+        constexpr int NSEG = (NS + SEG - 1) / SEG;
+        uint32_t code[P][NS];
+        float h_lo[P][TAPS], h_hi[P][TAPS];
+        auto load_seg = [&](int i, int sg) {
+#pragma unroll
+            for (int n = sg * SEG; n < (sg + 1) * SEG && n < NS; ++n)
+                code[i][n] = buffer_code<FORMAT>(in_rsrc, voff[i], uint32_t(NS - 1 - n) * D * BPS);
+#pragma unroll
+            for (int q = sg * (SEG / 2); q < (sg + 1) * (SEG / 2) && q < TAPS; ++q) {
+                h_lo[i][q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                    tap_rsrc, lo[i] * 4u, uint32_t(q) * M * 4u, 0));
+                h_hi[i][q] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                    tap_rsrc, (lo[i] + D) * 4u, uint32_t(q) * M * 4u, 0));
+            }
+        };
+        // Sample n of a segment is j = m0 + C - 1 - n; it meets block m0 + b of
+        // branch lo at tap q = (b - C + 1 + n) / 2 and of lo + D at
+        // q = (b - C + n) / 2 (whole, in range), first at n = 2q, so a tap is
+        // loaded in segment q / (SEG/2) at the latest.
+        auto run_seg = [&](int i, int sg) {
+#pragma unroll
+            for (int n = sg * SEG; n < (sg + 1) * SEG && n < NS; ++n) {
+                const uint32_t idx = base0 + uint32_t(C - 1 - n) * D - lo[i];
+                const float2 x =
+                    correct_sample<FORMAT>(convert_code<FORMAT>(code[i][n]), idx, a);
+#pragma unroll
+                for (int b = 0; b < C; ++b) {
+                    const int dl = b - C + 1 + n;
+                    if (dl >= 0 && (dl & 1) == 0 && dl / 2 < TAPS) {
+                        const float h = h_lo[i][dl / 2 < TAPS ? dl / 2 : 0];
+                        acc[i][b].x = acc[i][b].x + x.x * h;
+                        acc[i][b].y = acc[i][b].y + x.y * h;
+                    }
+                    const int dh = b - C + n;
+                    if (dh >= 0 && (dh & 1) == 0 && dh / 2 < TAPS) {
+                        const float h = h_hi[i][dh / 2 < TAPS ? dh / 2 : 0];
+                        acc[i][b].z = acc[i][b].z + x.x * h;
+                        acc[i][b].w = acc[i][b].w + x.y * h;
+                    }
+                }
+            }
+        };
+        load_seg(0, 0);
+#pragma unroll
+        for (int g = 0; g < P * NSEG; ++g) {
+            __builtin_amdgcn_sched_barrier(0);
+            if (g + 1 < P * NSEG) load_seg((g + 1) / NSEG, (g + 1) % NSEG);
+            run_seg(g / NSEG, g % NSEG);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+
+        // The transform's twiddles (pass 0's by scalar loads), read after the
+        // filter so that they do not hold registers through it.
+        F::template load_all_tw<0, F::PASSES>(a.tw, t, wt);
+        float2 out[C][V];
+#pragma unroll
+        for (int b = 0; b < C; ++b) {
+            float2 v[V];
+#pragma unroll
+            for (int i = 0; i < P; ++i) {
+                const float2 s_lo = make_float2(acc[i][b].x, acc[i][b].y);
+                const float2 s_hi = make_float2(acc[i][b].z, acc[i][b].w);
+                // slot t + T i is the pair's lo + D branch except at t = 0, i = 0
+                v[rvx::bitrev<F::LOG_V>(i)] = (i == 0 && lo_first) ? s_lo : s_hi;
+                v[rvx::bitrev<F::LOG_V>(i + P)] = (i == 0 && lo_first) ? s_hi : s_lo;
+            }
+            F::template run_pass_w<0>(v, wt[0]);
+            if constexpr (F::PASSES > 1) {
+                __syncthreads();  // the previous block's last pass has been read
+                const uint32_t g0 = rvx::bitrev<LOGT>(t) * V;
+#pragma unroll
+                for (int r = 0; r < V; ++r) my[F::phys(g0 + r)] = v[r];
+                __syncthreads();
+                F::template load<1>(my, v, t);
+                F::template run_pass_w<1>(v, wt[1]);
+                F::template rest_w<2>(my, v, wt, t);
+            }
+            // Output correction: at D = M/2 block m turns channel k by
+            // w[(k m D) mod M], which is w[M/2] for odd k of odd m and w[0] otherwise.
+            const bool odd_block = (uint32_t(a.first_block + first + b) & 1u) != 0u;
+#pragma unroll
+            for (int r = 0; r < V; ++r) {
+                const uint32_t k = F::template pos<LAST>(t, r);
+                const bool half = odd_block && (k & 1u) != 0u;
+                float2 c;
+                c.x = half ? tw_half.x : tw_zero.x;
+                c.y = half ? tw_half.y : tw_zero.y;
+                out[b][r] = rvx::cmul_exact(v[r], c);
+            }
+        }
+
+        // Powers into the frame's average, and each channel's run of C outputs.
+        const uint32_t active =
+            live && first < a.block_count ? min(uint32_t(C), a.block_count - first) : 0u;
+        const uint32_t m0 = uint32_t(a.first_block + first);
+        const uint32_t s0 = m0 & a.ring_mask;
+        const bool whole = active == uint32_t(C) && (s0 % C) == 0 && (C % 2) == 0;
+#pragma unroll
+        for (int r = 0; r < V; ++r) {
+            const uint32_t k = F::template pos<LAST>(t, r);
+            if (a.average) {
+#pragma unroll
+                for (int b = 0; b < C; ++b) {
+                    const float x = out[b][r].x * out[b][r].x + out[b][r].y * out[b][r].y;
+                    ema[r] = ema[r] + (x - ema[r]) * a.alpha;
+                }
+            }
+            const uint32_t ring_r = ring_row(a, k);
+            if (ring_r >= a.ring_rows || active == 0) continue;
+            float2* row = a.ring + uint64_t(ring_r) * a.ring_blocks;
+            if (whole) {
+#pragma unroll
+                for (int b = 0; b < C; b += 2) {
+                    float4* dst = reinterpret_cast<float4*>(row + s0 + b);
+                    *dst = make_float4(out[b][r].x, out[b][r].y, out[b + 1][r].x,
+                                       out[b + 1][r].y);
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < C; ++b)
+                    if (uint32_t(b) < active) row[(m0 + b) & a.ring_mask] = out[b][r];
+            }
+        }
+    }
+    if (a.average && live) {
+#pragma unroll
+        for (int r = 0; r < V; ++r)
+            a.partial[uint64_t(chunk) * M + F::template pos<LAST>(t, r)] = ema[r];
+    }
+}
+
 // Eight values per thread: radix-8 register passes (four for 4096 points).
 // The 512-point grid runs radix-4 passes: two transforms per workgroup at
 // half the registers per thread, 1.6x faster than radix-8 there.
 constexpr int kChainLogV = 3;
 
+// Blocks per chunk of the pair-streaming kernel (and its averaging frame),
+// filtered kPairSub blocks at a time.
+constexpr uint32_t kPairChunk = 4;
+constexpr int kPairSub = 4;
+
+// Whether a grid runs on chain_pair_kernel: D = M/2 with a compile-time tap
+// count, and an averaging frame (if any) of one chunk.
+bool pair_eligible(uint32_t M, uint32_t taps, uint32_t D, uint32_t frame) {
+    return M >= 4 && 2 * D == M && (taps == 16 || taps == 17) &&
+           (frame == 0 || frame == kPairChunk);
+}
+
+template <int LOGM, int FORMAT>
+void launch_chain_pair(const ChainArgs& a, uint32_t taps, hipStream_t stream) {
+    constexpr int kLogV = LOGM == 9 ? 2 : kChainLogV;
+    using S = ChainShape<LOGM, kLogV>;
+    const uint32_t groups = (a.chunks + S::BPW - 1) / S::BPW;
+    const dim3 grid(8 * ((groups + 7) / 8)), block(S::THREADS);
+    constexpr int C = kPairSub, SUBS = kPairChunk / kPairSub;
+    if (taps == 16)
+        chain_pair_kernel<LOGM, kLogV, FORMAT, 16, C, SUBS><<<grid, block, 0, stream>>>(a);
+    else
+        chain_pair_kernel<LOGM, kLogV, FORMAT, 17, C, SUBS><<<grid, block, 0, stream>>>(a);
+}
+
 template <int LOGM, int FORMAT>
 void launch_chain_m(const ChainArgs& a, uint32_t taps, hipStream_t stream) {
+    if constexpr (LOGM >= 2) {
+        if (pair_eligible(1u << LOGM, taps, a.decimation, a.average ? a.chunk : 0)) {
+            launch_chain_pair<LOGM, FORMAT>(a, taps, stream);
+            return;
+        }
+    }
     // Radix-8 grids from 1024 channels up keep their taps in memory: in
     // registers they take half the register file and starve the prefetch. The
     // 512-channel grid's radix-4 threads filter four branches, whose 64 taps
@@ -579,7 +832,7 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
     // each writes a 32-byte run per channel, so the channel ring's 128-byte
     // lines (16 blocks of one channel) fill from a few workgroups while they
     // are still in that XCD's L2.
-    uint32_t chunk = 4;
+    uint32_t chunk = pair_eligible(M, T, D, d.frame_blocks) ? kPairChunk : 4;
     if (d.frame_blocks != 0) {
         if (d.block_count % d.frame_blocks)
             return rvx::fail(RVX_EINVAL, "%s: frame of %u blocks does not divide %u blocks", who,

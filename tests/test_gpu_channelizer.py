@@ -13,7 +13,7 @@ from helpers import (SEED_BRANCH, SEED_CHAIN, SEED_FFT, adversarial, assert_bit_
                      rng, to_dev, uniform_complex)
 from oracle import oracle
 from revenant_amd import _lib
-from revenant_amd.dsp import (FORMAT_CS16, FORMAT_CU8, GridParams, build_twiddles,
+from revenant_amd.dsp import (FORMAT_CS16, FORMAT_CU8, GridParams, build_twiddles, prescale_taps,
                               design_prototype, ema_frame_decay)
 
 pytestmark = pytest.mark.gpu
@@ -177,6 +177,68 @@ def test_chain_matches_composed_twins(dev, fmt, grid, blocks, frame):
     assert_bit_exact(got[mask], ring_cpu[mask], SEED_CHAIN, f"chain ring {grid}")
     if frame:
         assert_bit_exact(from_dev(partial), partial_cpu, SEED_CHAIN, f"chain ema {grid}")
+
+
+# The two-times-oversampled grids (D = M/2) with 16 or 17 taps and no frame or
+# a four-block frame run on the pair-streaming kernel: every transform shape
+# it instantiates, tails of 1..3 blocks, chunks that do not start on a
+# four-block ring boundary, cu8 and cs16.
+PAIR_CASES = [
+    (FORMAT_CU8, GridParams(4, 16, 2), 37, 0, 2),
+    (FORMAT_CS16, GridParams(8, 17, 4), 40, 4, 1),
+    (FORMAT_CU8, GridParams(16, 16, 8), 41, 0, 3),
+    (FORMAT_CS16, GridParams(64, 16, 32), 64, 4, 0),
+    (FORMAT_CU8, GridParams(64, 17, 32), 30, 0, 1),
+    (FORMAT_CS16, GridParams(256, 16, 128), 35, 0, 0),
+    (FORMAT_CS16, GridParams(512, 16, 256), 70, 0, 2),
+    (FORMAT_CU8, GridParams(512, 17, 256), 16, 4, 0),
+    (FORMAT_CS16, GridParams(1024, 16, 512), 22, 0, 3),
+    (FORMAT_CS16, GridParams(2048, 17, 1024), 12, 4, 0),
+    (FORMAT_CS16, GridParams(4096, 16, 2048), 36, 4, 0),
+    (FORMAT_CU8, GridParams(4096, 16, 2048), 9, 0, 1),
+    (FORMAT_CS16, GridParams(4096, 17, 2048), 13, 0, 2),
+]
+
+
+@pytest.mark.parametrize("fmt,grid,blocks,frame,skew", PAIR_CASES,
+                         ids=[f"{'cu8' if c[0] == FORMAT_CU8 else 'cs16'}-M{c[1].channels}"
+                              f"L{c[1].taps_per_branch}-{c[2]}b-f{c[3]}-s{c[4]}"
+                              for c in PAIR_CASES])
+def test_pair_chain_matches_composed_twins(dev, fmt, grid, blocks, frame, skew):
+    """The pair-streaming launch equals convert -> branch -> fft -> ema_partial,
+    also with prescaled cs16 taps."""
+    M, D, N = grid.channels, grid.decimation, grid.prototype_length
+    first_block = (N + D - 1) // D + skew
+    samples = (first_block + blocks) * D + 5
+    packed = _codes(fmt, samples, SEED_CHAIN + 7 * M + skew)
+    taps = design_prototype(grid, beta=8.0).taps
+    tw = build_twiddles(M)
+    ring_blocks = 1 << (blocks + 3).bit_length()
+    alpha = np.float32(1.0 / 16.0)
+    od = oracle.OChain(fmt, M, grid.taps_per_branch, D, first_block, blocks, ring_blocks, frame,
+                       alpha, ema_frame_decay(alpha, frame) if frame else 0.0)
+    ring_cpu, partial_cpu = oracle.chain(od, packed, taps, tw, threads=4)
+    written = np.zeros((M, ring_blocks), dtype=bool)
+    for b in range(blocks):
+        written[:, (first_block + b) & (ring_blocks - 1)] = True
+    mask = np.repeat(written.ravel(), 2)
+    scaled_cases = (False, True) if fmt == FORMAT_CS16 else (False,)
+    for scaled in scaled_cases:
+        desc = _lib.ChainDesc(fmt, M, grid.taps_per_branch, D, 0, samples, first_block, blocks,
+                              ring_blocks, frame, alpha)
+        desc.taps_scaled = int(scaled)
+        use = prescale_taps(taps) if scaled else taps
+        ring = torch.full((2 * M * ring_blocks,), 3.0, dtype=torch.float32, device="cuda")
+        partial = (torch.zeros((blocks // frame) * M, dtype=torch.float32, device="cuda")
+                   if frame else None)
+        print(f"pair chain M={M} taps={grid.taps_per_branch} blocks={blocks} frame={frame} "
+              f"skew={skew} scaled={scaled}", flush=True)
+        dev.chain(desc, to_dev(packed), to_dev(use), to_dev(tw.view(np.float32)), ring, partial)
+        got = from_dev(ring)
+        assert_bit_exact(got[mask], ring_cpu[mask], SEED_CHAIN, f"pair chain ring {grid}")
+        assert np.all(got[~mask] == 3.0), "a slot outside the launch's blocks was written"
+        if frame:
+            assert_bit_exact(from_dev(partial), partial_cpu, SEED_CHAIN, f"pair chain ema {grid}")
 
 
 def test_chain_refuses_missing_history(dev):
