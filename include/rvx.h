@@ -569,6 +569,18 @@ int32_t rvx_vrx_batch(rvx_ctx* ctx, const rvx_vrx_desc* descs, uint32_t n_desc,
                       const rvx_vrx_work* work, uint32_t n_work, uint32_t* counter,
                       const void* nco, uint64_t n_nco, uint32_t workgroups);
 
+/* No counterpart in the reference: rvx_vrx_batch for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid allocation). Same descriptors, work list, kernels and
+ * bits; the queue counter lives in the library's static scratch (256 bytes
+ * per device, set up by rvx_ctx_create) instead of a caller's buffer, and
+ * nothing is allocated or freed. Every call on one device shares that
+ * counter: calls that overlap on one device (two streams, or two threads)
+ * race, so the caller serialises them. At most 2^31 - 1 work items. */
+int32_t rvx_vrx_batch_noalloc(rvx_ctx* ctx, const rvx_vrx_desc* descs, uint32_t n_desc,
+                              const rvx_vrx_work* work, uint32_t n_work, const void* nco,
+                              uint64_t n_nco, uint32_t workgroups);
+
 /* Host-side check of one descriptor (sizes are the caller's to guarantee for
  * device pointers; this checks the parameter rules of rvx_vrx_fine/demod and
  * that a tile fits the LDS budget). */

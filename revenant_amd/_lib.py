@@ -430,6 +430,7 @@ _SIGNATURES = {
     "rvx_synth": (i32, [vp, C.POINTER(Carrier), u32, C.POINTER(SynthParams), vp, u64]),
     "rvx_chain": (i32, [vp, C.POINTER(ChainDesc), vp, vp, u64, vp, u64, vp, u64, vp, u64]),
     "rvx_vrx_batch": (i32, [vp, vp, u32, vp, u32, vp, vp, u64, u32]),
+    "rvx_vrx_batch_noalloc": (i32, [vp, vp, u32, vp, u32, vp, u64, u32]),
     "rvx_vrx_desc_check": (i32, [C.POINTER(VrxDesc)]),
     "rvx_vrx_batch_plan": (i32, [C.POINTER(VrxSpan), u32, C.POINTER(VrxDesc), u32,
                                  C.POINTER(u32), C.POINTER(VrxWork), u32, C.POINTER(u32)]),

@@ -35,6 +35,9 @@ inline uint32_t log2u(uint64_t v) {
 int32_t enter(rvx_ctx* ctx, const char* who);
 // Reports a launch failure as RVX_EDEVICE.
 int32_t check_launch(const char* who);
+// Resolves `device`'s address of the batch launch's static scratch (the
+// device must be current); idempotent. Called by rvx_ctx_create.
+int32_t batch_scratch_setup(int device, const char* who);
 
 }  // namespace rvx
 

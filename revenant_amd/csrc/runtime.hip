@@ -83,6 +83,10 @@ int32_t rvx_ctx_create(int32_t device, rvx_ctx** out) {
         return rvx::fail(RVX_EDEVICE, "%s: could not create the stream or its events", who);
     }
     ctx->stream = ctx->own;
+    if (int32_t rc = rvx::batch_scratch_setup(device, who)) {
+        rvx_ctx_destroy(ctx);
+        return rc;
+    }
     *out = ctx;
     return RVX_OK;
 }

@@ -17,10 +17,16 @@
 #pragma clang fp contract(off)
 
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 #include "plan.h"
 #include "vrx_math.h"
+
+// The no-allocation entry point's scratch: the persistent launch's queue
+// counter in word 0 of a 256-byte array. Each device holds its own copy until
+// the process ends; every rvx_vrx_batch_noalloc call on a device shares it.
+static __device__ __align__(256) uint32_t scratch[64];
 
 namespace {
 
@@ -262,14 +268,21 @@ int32_t rvx_vrx_desc_check(const rvx_vrx_desc* desc) {
     return RVX_OK;
 }
 
-int32_t rvx_vrx_batch(rvx_ctx* ctx, const rvx_vrx_desc* descs, uint32_t n_desc,
-                      const rvx_vrx_work* work, uint32_t n_work, uint32_t* counter,
-                      const void* nco, uint64_t n_nco, uint32_t workgroups) {
-    static const char* who = "rvx_vrx_batch";
-    RVX_ENTER(ctx, who);
-    if (n_work == 0) return RVX_OK;
+}  // extern "C"
+
+namespace {
+
+// The queue counter runs from 0 to n_work + workgroups - 1 in one uint32.
+constexpr uint32_t kMaxWorkItems = 0x7FFFFFFFu;
+
+int32_t launch_batch(rvx_ctx* ctx, const rvx_vrx_desc* descs, uint32_t n_desc,
+                     const rvx_vrx_work* work, uint32_t n_work, uint32_t* counter,
+                     const void* nco, uint64_t n_nco, uint32_t workgroups, const char* who) {
     if (!descs || !work || !counter || !nco || n_desc == 0)
         return rvx::fail(RVX_EINVAL, "%s: null buffer", who);
+    if (n_work > kMaxWorkItems)
+        return rvx::fail(RVX_EINVAL, "%s: %u work items exceed the queue's %u", who, n_work,
+                         kMaxWorkItems);
     if (!rvx::pow2(n_nco) || n_nco > (1u << 24))
         return rvx::fail(RVX_EINVAL, "%s: NCO table of %llu entries", who, (unsigned long long)n_nco);
     if (workgroups == 0) {
@@ -283,6 +296,62 @@ int32_t rvx_vrx_batch(rvx_ctx* ctx, const rvx_vrx_desc* descs, uint32_t n_desc,
     vrx_batch_kernel<<<dim3(workgroups), dim3(kBatchThreads), 0, ctx->stream>>>(
         descs, n_desc, work, n_work, counter, static_cast<const float2*>(nco), rvx::log2u(n_nco));
     return rvx::check_launch(who);
+}
+
+// Each device's address of `scratch`, resolved once with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex scratch_lock;
+uint32_t* scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+namespace rvx {
+
+int32_t batch_scratch_setup(int device, const char* who) {
+    if (device < 0 || device >= kMaxDevices)
+        return fail(RVX_EINVAL, "%s: device %d beyond the %d the library tracks", who, device,
+                    kMaxDevices);
+    std::lock_guard<std::mutex> hold(scratch_lock);
+    if (scratch_ptr[device]) return RVX_OK;
+    void* ptr = nullptr;
+    // `scratch` names a host-side stand-in; the device's copy is at the
+    // address the runtime reports for the current device.
+    const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || ptr == nullptr)
+        return fail(RVX_EDEVICE, "%s: hipGetSymbolAddress(scratch) failed: %s", who,
+                    hipGetErrorString(err));
+    scratch_ptr[device] = static_cast<uint32_t*>(ptr);
+    return RVX_OK;
+}
+
+}  // namespace rvx
+
+extern "C" {
+
+int32_t rvx_vrx_batch(rvx_ctx* ctx, const rvx_vrx_desc* descs, uint32_t n_desc,
+                      const rvx_vrx_work* work, uint32_t n_work, uint32_t* counter,
+                      const void* nco, uint64_t n_nco, uint32_t workgroups) {
+    static const char* who = "rvx_vrx_batch";
+    RVX_ENTER(ctx, who);
+    if (n_work == 0) return RVX_OK;
+    return launch_batch(ctx, descs, n_desc, work, n_work, counter, nco, n_nco, workgroups, who);
+}
+
+int32_t rvx_vrx_batch_noalloc(rvx_ctx* ctx, const rvx_vrx_desc* descs, uint32_t n_desc,
+                              const rvx_vrx_work* work, uint32_t n_work, const void* nco,
+                              uint64_t n_nco, uint32_t workgroups) {
+    static const char* who = "rvx_vrx_batch_noalloc";
+    RVX_ENTER(ctx, who);
+    if (n_work == 0) return RVX_OK;
+    // set up by rvx_ctx_create; resolved here if this device's context came
+    // from elsewhere (RVX_ENTER made the device current)
+    if (int32_t rc = rvx::batch_scratch_setup(ctx->device, who)) return rc;
+    uint32_t* counter;
+    {
+        std::lock_guard<std::mutex> hold(scratch_lock);
+        counter = scratch_ptr[ctx->device];
+    }
+    return launch_batch(ctx, descs, n_desc, work, n_work, counter, nco, n_nco, workgroups, who);
 }
 
 }  // extern "C"
