@@ -314,7 +314,10 @@ def main() -> int:
                   "GB/s": round(algo[k] / (kernel_ms[k] * 1e-3) / 1e9, 1)} for k in order}
     dominant = max(order, key=lambda k: kernel_ms[k])
     achieved = algo[dominant] / (kernel_ms[dominant] * 1e-3) / 1e9
+    valu = valu_roof(kernel_ms, main_grid, wide_grid if vrx else None)
 
+    with torch.cuda.stream(side):
+        precision = precision_in_run(dev, torch)
     parity = spot_check(torch, packed, main_grid, partial, frames, alpha,
                         batch if vrx else None, starts if vrx else None,
                         counts if vrx else None, audio if vrx else None)
@@ -382,7 +385,9 @@ def main() -> int:
             "kernel": dominant,
             "kernel_ms": round(kernel_ms[dominant], 4),
             "algorithmic_bytes": int(algo[dominant]),
+            "valu": valu.get(dominant),
         },
+        "valu_roofline": valu,
         "stages": stages,
         "batched_receivers": None if not vrx else {
             "descriptors": batch.n_desc, "work_items": batch.n_work,
@@ -393,7 +398,7 @@ def main() -> int:
         "mixed_precision": {
             "chain_m4096_ms": {"fp32_exact": round(kernel_ms["chain_m4096"], 4),
                                **{k: round(v, 4) for k, v in mfma_ms.items()}},
-            "precision_vs_fp64": precision_from_profiles(),
+            "precision_vs_fp64": precision,
         },
         "learned_detector": learned,
         "conditioning": conditioning,
@@ -833,16 +838,58 @@ def _lib_copy(desc):
     return type(desc).from_buffer_copy(desc)
 
 
-def precision_from_profiles():
-    """Error against FP64 of each form (a 100 dB dynamic-range multitone), as
-    measured by tests/test_gpu_mfma.py and committed under profiles/."""
-    p = ROOT / "profiles" / "mfma_precision.json"
-    if p.exists():
-        try:
-            return json.loads(p.read_text()).get("results")
-        except Exception:
-            return None
-    return None
+# VALU lane-operations per second: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz. The
+# bit-exact kernels run without contraction and without packed FP32, so each
+# floating-point operation is one lane-operation.
+VALU_PEAK_LANE_OPS = 256 * 4 * 16 * 2.4e9
+
+
+def chain_lane_ops(channels: int, decimation: int, taps: int) -> int:
+    """The fewest lane-operations one block of the bit-exact chain can take:
+    the branch sums (a multiply and an add per real part: 4 per tap and
+    branch), each new input sample converted once (2), the radix-2 transform
+    (5 N log2 N), the output correction (a complex multiply, 6 per channel),
+    and the power and its average (3 + 3 per channel)."""
+    n = channels
+    return (4 * taps * n + 2 * decimation + 5 * n * (n.bit_length() - 1) + 6 * n + 6 * n)
+
+
+def valu_roof(kernel_ms: dict, main: "Grid", wide: "Grid | None") -> dict:
+    """Per chain kernel: its lane-operations per launch, the time they take at
+    the VALU peak, and the fraction of that floor the measured time reaches."""
+    out = {}
+    grids = [("chain_m4096", main)] + ([("chain_m512_wfm", wide)] if wide is not None else [])
+    for name, g in grids:
+        p = g.params
+        ops = chain_lane_ops(p.channels, p.decimation, p.taps_per_branch) * g.blocks
+        if g.frames == 0:
+            ops -= 6 * p.channels * g.blocks  # no power average on this grid
+        floor = ops / VALU_PEAK_LANE_OPS * 1e3
+        out[name] = {"lane_ops": int(ops), "peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+                     "floor_ms": round(floor, 4),
+                     "frac": round(floor / kernel_ms[name], 4)}
+    return out
+
+
+def precision_in_run(dev, torch):
+    """Error against float64 of each form of the 4096-channel chain, measured
+    in this run after the timed region: the 100 dB dynamic-range multitone of
+    revenant_amd.precision through the bit-exact fp32 chain and the
+    matrix-core forms, each checked against the oracle's float64 channelizer
+    of the same codes."""
+    try:
+        from oracle import oracle
+    except Exception as exc:  # the oracle is optional on the box
+        return {"skipped": f"{type(exc).__name__}: {exc}"}
+    from revenant_amd import _lib as L
+    from revenant_amd import precision as pr
+    from revenant_amd.dsp import GridParams, design_prototype
+
+    codes, first, _, _, _ = pr.multitone()
+    taps = design_prototype(GridParams(pr.M, pr.L, pr.D), beta=8.0).taps
+    ref = oracle.fp64_chain(codes, taps, pr.M, pr.D, first, pr.BLOCKS)
+    return pr.run_forms(dev, torch, ref, (("plain_fp32", None), ("bf16x3", L.PREC_BF16X3),
+                                          ("fp16", L.PREC_FP16)))
 
 
 def traffic_from_profiles(kernel: str):
@@ -891,7 +938,7 @@ def cpu_baseline(packed, main: Grid, wide: Grid | None, batch, seconds):
         wide_ring = np.zeros(2 * WM * wcap, dtype=np.float32)
     nco = build_nco_table(16)
 
-    def receivers():
+    def receivers(nthreads):
         # every receiver over the channel samples of this sample
         jobs = []
         for rx in batch.receivers:
@@ -920,25 +967,45 @@ def cpu_baseline(packed, main: Grid, wide: Grid | None, batch, seconds):
             dp = p.demod_block(out_cap - 1, a0, n)
             return oracle.vrx_demod(p.demod, dp, fine, p.weights).size
 
-        with ThreadPoolExecutor(threads) as pool:
+        with ThreadPoolExecutor(nthreads) as pool:
             return sum(pool.map(one, jobs))
 
-    passes, total = 0, 0.0
-    while total < seconds or passes == 0:
-        t0 = time.perf_counter()
-        oracle.chain(od, sub, main.proto.taps, main.tw_np, threads=threads, ring=ring,
-                     partial=partial)
-        if wide is not None:
-            oracle.chain(wide_od, sub, wide.proto.taps, wide.tw_np, threads=threads,
-                         ring=wide_ring)
-            receivers()
-        total += time.perf_counter() - t0
-        passes += 1
+    def timed_passes(nthreads, budget):
+        passes, total = 0, 0.0
+        while total < budget or passes == 0:
+            t0 = time.perf_counter()
+            oracle.chain(od, sub, main.proto.taps, main.tw_np, threads=nthreads, ring=ring,
+                         partial=partial)
+            if wide is not None:
+                oracle.chain(wide_od, sub, wide.proto.taps, wide.tw_np, threads=nthreads,
+                             ring=wide_ring)
+                receivers(nthreads)
+            total += time.perf_counter() - t0
+            passes += 1
+        return passes, total
+
+    passes, total = timed_passes(threads, seconds)
     rate = passes * blocks * D / total / 1e6
+    one_passes, one_total = timed_passes(1, seconds / 2)
+    one_rate = one_passes * blocks * D / one_total / 1e6
     what = "4096-ch chain" + (", 512-ch chain and 256 receivers" if wide is not None else "")
     return {"value": round(rate, 2), "unit": "MS/s", "cores": threads, "kind": "port",
+            "single_thread_value": round(one_rate, 3),
+            "cpu_model": cpu_model(),
+            "host_threads": len(os.sched_getaffinity(0)),
             "sample": f"{passes} passes over {blocks * D} input samples ({blocks} blocks) of "
-                      f"the same workload ({what}), {total:.1f} s on {threads} threads"}
+                      f"the same workload ({what}), {total:.1f} s on {threads} threads; "
+                      f"{one_passes} passes, {one_total:.1f} s on one thread"}
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 if __name__ == "__main__":

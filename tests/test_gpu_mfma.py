@@ -15,49 +15,15 @@ import pytest
 
 from helpers import from_dev, rng, to_dev
 from oracle import oracle
-from revenant_amd import _lib
+from revenant_amd import _lib, precision
 from revenant_amd.dsp import FORMAT_CS16, GridParams, build_twiddles, design_prototype
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
-M, L, D = 4096, 16, 2048
-BLOCKS = 8
-RATE = 122_880_000
-
-
-def multitone(seed):
-    g = rng(seed)
-    first = (M * L - 1 + D - 1) // D
-    n = (first + BLOCKS) * D + 1
-    t = np.arange(n)
-    carrier = 37 * RATE / M + 1234.5
-    x = 10 ** (-3 / 20) * np.exp(2j * np.pi * carrier * t / RATE)
-    tones = g.choice(np.arange(100, M - 100), 50, replace=False)
-    for k in tones:
-        f = (k if k < M // 2 else k - M) * RATE / M + float(g.uniform(-3000, 3000))
-        x += 1e-5 * np.exp(2j * np.pi * f * t / RATE + 1j * g.uniform(0, 2 * np.pi))
-    x += 10 ** (-120 / 20) * (g.standard_normal(n) + 1j * g.standard_normal(n)) / np.sqrt(2)
-    iq = np.empty(2 * n)
-    iq[0::2], iq[1::2] = x.real, x.imag
-    codes = np.clip(np.round(iq * 32767), -32768, 32767).astype(np.int16)
-    return codes.view(np.uint32), first, n, 37, np.sort(tones)
-
-
-def metrics(got, ref, carrier, weak):
-    err = got - ref
-    peak = np.max(np.abs(ref[:, carrier]))
-    err_dbc = 20 * np.log10(np.max(np.abs(err)) / peak)
-    perr = np.mean(np.abs(err) ** 2, axis=0)
-    others = np.delete(perr, carrier)
-    sfdr = 10 * np.log10(np.mean(np.abs(ref[:, carrier]) ** 2) / np.max(others))
-    psig = np.mean(np.abs(ref) ** 2, axis=0)
-    snr = 10 * np.log10(psig / np.maximum(perr, 1e-300))
-    return {"err_dbc": round(float(err_dbc), 2), "sfdr_db": round(float(sfdr), 2),
-            "snr_carrier_db": round(float(snr[carrier]), 2),
-            "snr_weak_tones_min_db": round(float(snr[weak].min()), 2),
-            "snr_weak_tones_median_db": round(float(np.median(snr[weak])), 2),
-            "max_rel_error": float(np.max(np.abs(got - ref)) / np.max(np.abs(ref)))}
+M, L, D = precision.M, precision.L, precision.D
+BLOCKS = precision.BLOCKS
+multitone, metrics = precision.multitone, precision.metrics
 
 
 def test_mfma_precision_against_fp64(dev):
