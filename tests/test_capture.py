@@ -36,8 +36,8 @@ def test_datatypes_map_to_the_convert_formats_and_the_rest_are_refused():
     assert [sigmf_datatype_to_format(d) for d in ("cu8", "ci8", "ci16_le", "ci24_le", "cf32_le",
                                                    "cu8_be", "ci8_le")] == \
         ["cu8", "cs8", "cs16", "cs24", "cf32", "cu8", "cs8"]
-    for bad, word in (("cf32_be", "big-endian"), ("rf32_le", "real"), ("ci32_le", "ci32_le"),
-                      ("x16", "does not begin")):
+    for bad, word in (("cf32_be", "big-endian"), ("rf32_le", "real"), ("ci32_le", "no convert"),
+                      ("x16", "does not parse"), ("ci16_xe", "does not parse")):
         with pytest.raises(CaptureError) as e:
             sigmf_datatype_to_format(bad)
         assert word in str(e.value) and bad in str(e.value)
@@ -156,7 +156,7 @@ def test_sidecar_checks(tmp_path):
     m = read_sigmf_metadata(meta)
     assert m.total_samples == 90 and m.segments[0].sample_start == 0
     _, meta = write_pair(tmp_path, "empty", ramp_cf32(100, 0), "cf32_le", 1_000_000, [])
-    assert "empty captures" in read_sigmf_metadata(meta).notes[0]
+    assert "captures is empty" in read_sigmf_metadata(meta).notes[0]
 
 
 # ---------------------------------------------------------------------------
