@@ -338,9 +338,26 @@ int32_t rvx_channel_centre(const rvx_grid* grid, int64_t rate, uint32_t channel,
 uint64_t rvx_channel_sample_to_input(const rvx_grid* grid, uint64_t group_delay_samples,
                                      uint64_t base, uint64_t m);
 /* core/engine/vrx_place.cpp:216 place: nearest channel round(center*M/rate),
- * exact rational residual, refusal when the channel cannot carry the band. */
+ * exact rational residual, the granted passband (clamp_to_channel; an empty
+ * grant is reported, plan_vrx refuses it); an NFM/WFM receiver granted less
+ * than its mode's default passband is refused (:398-440). */
 int32_t rvx_place(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* request,
                   rvx_vrx_placement* out);
+/* The planner's fitting queries (core/dsp/vrx_reference.h, vrx_reference.cpp):
+ * resolve_passband (the explicit pair, else the bandwidth shorthand: symmetric
+ * and one hertz narrow when odd, [0, B] for USB, [-B, 0] for LSB, else the
+ * mode's default), default_passband, max_channel_bandwidth (:1409: the
+ * channel rate less twice |residual|, rounded down), clamp_to_channel (:1359:
+ * each edge into [-max/2, +max/2]; empty when the channel carries nothing),
+ * the shorthand minimum_demod_rate (:1519) and demod_rate_for (:1670, the
+ * rate plan_vrx builds, refusing what it refuses). */
+int32_t rvx_resolve_passband(const rvx_vrx_request* request, int64_t* low, int64_t* high);
+int32_t rvx_default_passband(uint32_t mode, int64_t* low, int64_t* high);
+int64_t rvx_max_channel_bandwidth(const rvx_vrx_placement* placement);
+int32_t rvx_clamp_to_channel(const rvx_vrx_placement* placement, int64_t* low, int64_t* high);
+int64_t rvx_minimum_demod_rate(uint32_t mode, int64_t bandwidth, int64_t cw_pitch);
+int32_t rvx_demod_rate_for(const rvx_vrx_request* request, const rvx_vrx_placement* placement,
+                           int64_t audio_rate, int64_t* out);
 /* core/dsp/vrx_reference.cpp:2568 plan_vrx (AM, NFM, WFM mono, USB, LSB,
  * DSB, CW, SAM, raw; no stereo). The de-emphasis curve (resolve_deemphasis,
  * core/engine/vrx.h:302: 75 us on WFM programme audio by default, none

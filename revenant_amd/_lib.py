@@ -334,6 +334,13 @@ _SIGNATURES = {
                             vp, u64, vp, u64]),
     "rvx_place": (i32, [C.POINTER(Grid), i64, C.POINTER(VrxRequest), C.POINTER(VrxPlacement)]),
     "rvx_channel_count_for": (u32, [i64, i64]),
+    "rvx_resolve_passband": (i32, [C.POINTER(VrxRequest), C.POINTER(i64), C.POINTER(i64)]),
+    "rvx_default_passband": (i32, [u32, C.POINTER(i64), C.POINTER(i64)]),
+    "rvx_max_channel_bandwidth": (i64, [C.POINTER(VrxPlacement)]),
+    "rvx_clamp_to_channel": (i32, [C.POINTER(VrxPlacement), C.POINTER(i64), C.POINTER(i64)]),
+    "rvx_minimum_demod_rate": (i64, [u32, i64, i64]),
+    "rvx_demod_rate_for": (i32, [C.POINTER(VrxRequest), C.POINTER(VrxPlacement), i64,
+                                 C.POINTER(i64)]),
     "rvx_channel_centre": (i32, [C.POINTER(Grid), i64, u32, C.POINTER(i64), C.POINTER(i64)]),
     "rvx_channel_sample_to_input": (u64, [C.POINTER(Grid), u64, u64, u64]),
     "rvx_plan_vrx": (i32, [C.POINTER(Grid), i64, C.POINTER(VrxRequest), C.POINTER(VrxPlacement),

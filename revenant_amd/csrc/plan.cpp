@@ -164,6 +164,29 @@ Band clamp_band(const rvx_vrx_placement& p, Band b) {
     return {std::clamp(b.low, -lim, lim), std::clamp(b.high, -lim, lim)};
 }
 
+// Refuses a request one channel cannot deliver (vrx_reference.cpp:1440-1465):
+// the channel carries nothing at all (the receiver sits half a channel or
+// more from the channel's centre), or nothing of this band (it lies wholly
+// past the limit). The message gives the request's edges against the limit.
+int32_t channel_carries(const rvx_vrx_placement& p, Band req, const char* who) {
+    const int64_t w = widest(p);
+    if (w <= 0)
+        return rvx::fail(RVX_EINVAL,
+                         "%s: a %lld S/s channel with residual %lld/%lld Hz can carry nothing "
+                         "for this receiver (it is half a channel or more off the channel "
+                         "centre)",
+                         who, (long long)p.channel_rate, (long long)p.residual_num,
+                         (long long)p.residual_den);
+    if (clamp_band(p, req).width() > 0) return RVX_OK;
+    const int64_t lim = w / 2;
+    return rvx::fail(RVX_EINVAL,
+                     "%s: passband %lld..%lld Hz has nothing inside the %lld..%lld Hz one "
+                     "%lld S/s channel carries here (residual %lld/%lld Hz)",
+                     who, (long long)req.low, (long long)req.high, (long long)-lim,
+                     (long long)lim, (long long)p.channel_rate, (long long)p.residual_num,
+                     (long long)p.residual_den);
+}
+
 Band mix_frame(Band b, uint32_t mode, int64_t pitch) {
     if (mode != RVX_DEMOD_CW) return b;
     return {b.low + pitch, b.high + pitch};
@@ -175,6 +198,24 @@ int64_t minimum_rate(uint32_t mode, Band b) {
     const int64_t floor_rate = std::max((3 * width + 1) / 2, 2 * b.reach());
     if (mode == RVX_DEMOD_AM) return std::max(floor_rate, 2 * width);
     return floor_rate;
+}
+
+// The demodulation rate plan_vrx builds (vrx_reference.cpp:1670
+// demod_rate_for): the fitted band, moved by the CW pitch (a negative pitch
+// read as zero), needs minimum_rate; the rate is the next whole multiple of
+// the audio rate. Refuses what plan_vrx refuses for the band.
+int32_t demod_rate_of(const rvx_vrx_request& r, const rvx_vrx_placement& pl, int64_t audio_rate,
+                      Band& band, Band& mixed, int64_t& rate, const char* who) {
+    if (audio_rate <= 0)
+        return rvx::fail(RVX_EINVAL, "%s: audio rate %lld", who, (long long)audio_rate);
+    Band req;
+    if (int32_t rc = resolve_band(r, req, who)) return rc;
+    if (int32_t rc = channel_carries(pl, req, who)) return rc;
+    band = clamp_band(pl, req);
+    mixed = mix_frame(band, r.mode, std::max<int64_t>(0, r.cw_pitch));
+    const int64_t required = minimum_rate(r.mode, mixed);
+    rate = std::max<int64_t>(1, (required + audio_rate - 1) / audio_rate) * audio_rate;
+    return RVX_OK;
 }
 
 bool complex_output(uint32_t mode) { return mode == RVX_DEMOD_RAW; }
@@ -446,14 +487,82 @@ int32_t rvx_place(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* req
     while (2 * p.residual_num <= -span) p.residual_num += span;
     reduce(p.residual_num, p.residual_den);
     const Band granted = clamp_band(p, req);
-    if (granted.width() <= 0)
-        return rvx::fail(RVX_EINVAL,
-                         "%s: one %lld S/s channel cannot carry a %lld..%lld Hz passband here",
-                         who, (long long)p.channel_rate, (long long)req.low, (long long)req.high);
     p.granted_low = granted.low;
     p.granted_high = granted.high;
     p.bandwidth_clamped = (granted.low != req.low || granted.high != req.high) ? 1u : 0u;
+    // An empty grant is reported, not refused: plan_vrx refuses it. But a
+    // discriminator fed part of an FM channel recovers the wrong audio, not
+    // less of it, so an FM receiver granted less than its mode's own channel
+    // plan is refused here, naming the channel count that would carry it
+    // (vrx_place.cpp:398-440).
+    const Band plan_band = default_band(request->mode);
+    if (p.bandwidth_clamped && (request->mode == RVX_DEMOD_NFM || request->mode == RVX_DEMOD_WFM) &&
+        granted.width() < plan_band.width()) {
+        const uint32_t carry = rvx_channel_count_for(rate, plan_band.width());
+        const int64_t guaranteed = rate / int64_t(carry);
+        return rvx::fail(RVX_EINVAL,
+                         "%s: an FM receiver needs %lld Hz and one channel of this %u-channel "
+                         "grid carries %lld Hz of it (%lld..%lld); a cut FM channel demodulates "
+                         "to the wrong audio. %u channels guarantee %lld Hz%s",
+                         who, (long long)plan_band.width(), g.channels,
+                         (long long)granted.width(), (long long)granted.low,
+                         (long long)granted.high, carry, (long long)guaranteed,
+                         guaranteed >= plan_band.width() ? "" : " (the source rate is the limit)");
+    }
     *out = p;
+    return RVX_OK;
+}
+
+int32_t rvx_resolve_passband(const rvx_vrx_request* request, int64_t* low, int64_t* high) {
+    static const char* who = "rvx_resolve_passband";
+    if (!request || !low || !high) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    Band b;
+    if (int32_t rc = resolve_band(*request, b, who)) return rc;
+    *low = b.low;
+    *high = b.high;
+    return RVX_OK;
+}
+
+int32_t rvx_default_passband(uint32_t mode, int64_t* low, int64_t* high) {
+    if (!low || !high) return rvx::fail(RVX_EINVAL, "rvx_default_passband: null argument");
+    const Band b = default_band(mode);
+    *low = b.low;
+    *high = b.high;
+    return b.width() > 0 ? RVX_OK
+                         : rvx::fail(RVX_EINVAL, "rvx_default_passband: mode %u has none", mode);
+}
+
+int64_t rvx_max_channel_bandwidth(const rvx_vrx_placement* placement) {
+    return placement ? widest(*placement) : 0;
+}
+
+int32_t rvx_clamp_to_channel(const rvx_vrx_placement* placement, int64_t* low, int64_t* high) {
+    if (!placement || !low || !high)
+        return rvx::fail(RVX_EINVAL, "rvx_clamp_to_channel: null argument");
+    const Band b = clamp_band(*placement, Band{*low, *high});
+    *low = b.low;
+    *high = b.high;
+    return RVX_OK;
+}
+
+int64_t rvx_minimum_demod_rate(uint32_t mode, int64_t bandwidth, int64_t cw_pitch) {
+    if (bandwidth <= 0) return 0;
+    const int64_t half = bandwidth / 2;
+    Band b = mode == RVX_DEMOD_USB ? Band{0, bandwidth}
+             : mode == RVX_DEMOD_LSB ? Band{-bandwidth, 0}
+                                     : Band{-half, half};
+    return minimum_rate(mode, mix_frame(b, mode, std::max<int64_t>(0, cw_pitch)));
+}
+
+int32_t rvx_demod_rate_for(const rvx_vrx_request* request, const rvx_vrx_placement* placement,
+                           int64_t audio_rate, int64_t* out) {
+    static const char* who = "rvx_demod_rate_for";
+    if (!request || !placement || !out) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    Band band, mixed;
+    int64_t rate = 0;
+    if (int32_t rc = demod_rate_of(*request, *placement, audio_rate, band, mixed, rate, who))
+        return rc;
+    *out = rate;
     return RVX_OK;
 }
 
@@ -475,18 +584,13 @@ int32_t rvx_plan_vrx(const rvx_grid* grid, int64_t rate, const rvx_vrx_request* 
     plan.mode = r.mode;
     plan.channel_rate = pl.channel_rate;
     plan.audio_rate = r.audio_rate > 0 ? r.audio_rate : kDefaultAudioRate;
-    Band req;
-    if (int32_t rc = resolve_band(r, req, who)) return rc;
-    const Band band = clamp_band(pl, req);
-    if (band.width() <= 0) return rvx::fail(RVX_EINVAL, "%s: the channel carries nothing", who);
+    Band band, mixed;
+    if (int32_t rc = demod_rate_of(r, pl, plan.audio_rate, band, mixed, plan.demod_rate, who))
+        return rc;
     plan.passband_low = band.low;
     plan.passband_high = band.high;
     plan.bandwidth = band.width();
     const int64_t pitch = std::max<int64_t>(0, r.cw_pitch);
-    const Band mixed = mix_frame(band, r.mode, pitch);
-    const int64_t required = minimum_rate(r.mode, mixed);
-    const int64_t step = plan.audio_rate;
-    plan.demod_rate = std::max<int64_t>(1, (required + step - 1) / step) * step;
     const uint32_t decimation = uint32_t(plan.demod_rate / plan.audio_rate);
 
     const double stop_edge = 0.5 * double(std::min(plan.demod_rate, plan.channel_rate));

@@ -255,6 +255,47 @@ def channel_sample_to_input(grid: GridParams, prototype: PrototypeFilter, base: 
         C.byref(g), int(prototype.group_delay_samples), int(base) % 2**64, int(m) % 2**64))
 
 
+def resolve_passband(request: VrxRequest) -> tuple[int, int]:
+    """core/dsp/vrx_reference.h resolve_passband."""
+    lo, hi = C.c_int64(), C.c_int64()
+    r = request.c()
+    _lib.check(_lib.lib().rvx_resolve_passband(C.byref(r), C.byref(lo), C.byref(hi)))
+    return lo.value, hi.value
+
+
+def default_passband(mode: int) -> tuple[int, int]:
+    """core/dsp/vrx_reference.h default_passband."""
+    lo, hi = C.c_int64(), C.c_int64()
+    _lib.check(_lib.lib().rvx_default_passband(mode, C.byref(lo), C.byref(hi)))
+    return lo.value, hi.value
+
+
+def max_channel_bandwidth(placement: _lib.VrxPlacement) -> int:
+    """core/dsp/vrx_reference.cpp:1409."""
+    return int(_lib.lib().rvx_max_channel_bandwidth(C.byref(placement)))
+
+
+def clamp_to_channel(placement: _lib.VrxPlacement, band: tuple[int, int]) -> tuple[int, int]:
+    """core/dsp/vrx_reference.cpp:1359."""
+    lo, hi = C.c_int64(band[0]), C.c_int64(band[1])
+    _lib.check(_lib.lib().rvx_clamp_to_channel(C.byref(placement), C.byref(lo), C.byref(hi)))
+    return lo.value, hi.value
+
+
+def minimum_demod_rate(mode: int, bandwidth: int, cw_pitch: int = 0) -> int:
+    """core/dsp/vrx_reference.cpp:1519, the bandwidth shorthand overload."""
+    return int(_lib.lib().rvx_minimum_demod_rate(mode, bandwidth, cw_pitch))
+
+
+def demod_rate_for(request: VrxRequest, placement: _lib.VrxPlacement, audio_rate: int) -> int:
+    """core/dsp/vrx_reference.cpp:1670: the rate plan_vrx builds, cheaply."""
+    out = C.c_int64()
+    r = request.c()
+    _lib.check(_lib.lib().rvx_demod_rate_for(C.byref(r), C.byref(placement), audio_rate,
+                                             C.byref(out)))
+    return out.value
+
+
 def place(grid: GridParams, rate: int, request: VrxRequest) -> _lib.VrxPlacement:
     """core/engine/vrx_place.cpp:216."""
     out = _lib.VrxPlacement()
