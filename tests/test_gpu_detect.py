@@ -127,3 +127,34 @@ def test_detect_on_chain_spectrum(dev):
     assert peaks == opk and [(s, w) for s, w, _ in bands] == [(s, w) for s, w, _ in ob]
     assert len(bands) >= 5
     det.close()
+
+
+def _tie_scenes():
+    bins = 4096
+    plateau = np.ones(bins)
+    plateau[1000:1400] += 4.0
+    boxes = np.ones(bins)
+    for k in range(12):
+        boxes[500 + 100 * k:516 + 100 * k] += 4.0
+    return [("plateau", plateau, dict(max_sum_bins=64)),
+            ("equal-deflections", boxes, dict(max_sum_bins=64, max_peaks=5)),
+            ("equal-deflections-mirrored", boxes[::-1].copy(), dict(max_sum_bins=64,
+                                                                     max_peaks=5))]
+
+
+@pytest.mark.parametrize("name", ["plateau", "equal-deflections", "equal-deflections-mirrored"])
+def test_detect_tie_rules_match_oracle(dev, name):
+    """The closed-form scenes that pin the twin's plateau middle and its order
+    at equal deflection (tests/test_oracle_detect.py), on the device: same
+    peaks, same bands, same deflections."""
+    from revenant_amd.detect import Detector, config
+
+    (_, avg, kw), = [s for s in _tie_scenes() if s[0] == name]
+    det = Detector(dev, config(avg.size, BIN_WIDTH, 0.0, **kw))
+    bands, peaks = det.decide(to_dev(avg))
+    ob, _, _, opk = oracle.detect(oracle.detect_config(avg.size, BIN_WIDTH, 0.0, **kw), avg)
+    assert peaks == opk
+    assert [(s, w) for s, w, _ in bands] == [(s, w) for s, w, _ in ob]
+    assert [np.float64(d).tobytes() for _, _, d in bands] == \
+        [np.float64(d).tobytes() for _, _, d in ob]
+    det.close()

@@ -92,3 +92,42 @@ def test_boxes_on_noise_come_back_as_bands():
         b0, bw, _ = hit[0]
         overlap = min(b0 + bw, s + w) - max(b0, s)
         assert overlap >= 0.9 * w and bw <= 1.5 * w + 16, (s, w, hit)
+
+
+def test_a_plateau_seeds_at_its_middle():
+    """A flat box wider than the widest rung: every window of that rung wholly
+    inside it has the same deflection (exact sums: floor 1, excess 4), so the
+    maximum is a plateau and the seed is its middle (detector.cpp:1002-1012).
+    Growth then reaches one seed width each side, so the band is centred on
+    the box: [1104, 1296) for the box [1000, 1400) at rung 64. A seed taken at
+    the plateau's first or last position would give [1000, 1128) or
+    [1272, 1400)."""
+    bins = 4096
+    avg = np.ones(bins)
+    avg[1000:1400] += 4.0
+    cfg = oracle.detect_config(bins, BIN_WIDTH, threshold_db=0.0, max_sum_bins=64)
+    bands, floor, sigma, _ = oracle.detect(cfg, avg)
+    assert np.all(floor == 1.0) and np.all(sigma == 0.0)
+    assert [(s, w) for s, w, _ in bands] == [(1104, 192)]
+    assert bands[0][2] == float(np.float32(4.0 * 64 * 8 / 64))
+
+
+def test_equal_deflections_keep_the_lowest_starts_when_the_budget_binds():
+    """Twelve identical boxes give twelve peaks of exactly equal deflection
+    (16 bins of excess 4: 4 * 16 * 4 / 16 = 16); with a budget of five, the
+    order that breaks the tie (strongest, then lower start, then narrower;
+    detector.cpp:951-960) keeps the five lowest starts, whichever order the
+    scan met them in, and the accepted bands are those boxes."""
+    bins = 4096
+    avg = np.ones(bins)
+    starts = [500 + 100 * k for k in range(12)]
+    for s in starts:
+        avg[s:s + 16] += 4.0
+    cfg = oracle.detect_config(bins, BIN_WIDTH, threshold_db=0.0, max_sum_bins=64, max_peaks=5)
+    bands, _, _, peaks = oracle.detect(cfg, avg)
+    assert [(s, w) for s, w, _ in bands] == [(s, 16) for s in starts[:5]]
+    assert all(d == 16.0 for _, _, d in bands)
+    # mirrored, the same rule keeps the five lowest starts of the mirror image
+    bands_m, _, _, _ = oracle.detect(cfg, avg[::-1].copy())
+    mirrored = sorted(bins - s - 16 for s in starts)
+    assert [(s, w) for s, w, _ in bands_m] == [(s, 16) for s in mirrored[:5]]
