@@ -66,7 +66,9 @@ def _waveform(b: Burst, t: torch.Tensor, rate: float) -> torch.Tensor:
         tones = (2.0 * sym - (m - 1)) / (m - 1) * (b.bw_hz / 2.0 - baud / 2.0)
         idx = torch.clamp((tau * baud).long(), 0, nsym - 1)
         inst = b.freq_hz + tones[idx]
-        ph = two_pi * torch.cumsum(inst, 0) / rate  # continuous phase
+        # continuous phase; the running sum on the host, in order, because a
+        # device scan's addition order (and so its rounding) can vary run to run
+        ph = (two_pi * torch.cumsum(inst.cpu(), 0) / rate).to(t.device)
     elif b.kind == "chirp":
         k = b.bw_hz / b.dur
         ph = two_pi * ((b.freq_hz - b.bw_hz / 2.0) * tau + 0.5 * k * tau * tau)

@@ -81,3 +81,30 @@ def test_net_learns_boxes(dev):
     assert loss < first / 4, (first, loss)
     assert iou > 0.8, iou
     net.close()
+
+
+def test_training_is_deterministic(dev):
+    """One seed, one data set: two trainings end with bit-identical weights
+    (the kernels reduce in a fixed order and the batches come from a seeded
+    generator), and a different seed does not."""
+    from revenant_amd.learned import Net, to_bf16
+
+    g = rng(0x4C45524E44455431)
+    tiles = to_bf16(dev, to_dev(g.standard_normal((6, 32, 64)).astype(np.float32)))
+    labels = to_dev((g.random((6, 32, 64)) < 0.2).astype(np.float32))
+
+    def train(seed):
+        net = Net(dev, 32, 64, 4, seed=seed)
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(seed)
+        for step in range(60):
+            idx = torch.randint(0, 6, (4,), generator=gen, device="cuda")
+            net.train_step(tiles[idx].contiguous(), labels[idx].contiguous(), 2e-3, 3.0,
+                           want_loss=step % 20 == 0)
+        p = net.params()
+        net.close()
+        return p
+
+    a, b, c = train(5), train(5), train(6)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert not np.array_equal(a, c)

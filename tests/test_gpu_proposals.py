@@ -5,8 +5,9 @@ chain; labels from the plain detector on the noise-free copies; the network
 trained on the GPU by its own kernels; the proposal threshold picked on
 validation scenes; then, on held-out scenes, the propose-then-confirm
 decision against one plain decision per row: it keeps the plain method's
-recall (98% of it, with a margin for this small set) with better precision,
-in a fraction of its time.
+recall with better precision, in a fraction of its time. The run is the
+benchmark's own (96 training scenes, 6000 steps, 12 held-out scenes, seed 1),
+and training is deterministic, so the bound holds run to run.
 """
 import json
 
@@ -19,10 +20,10 @@ torch = pytest.importorskip("torch")
 def test_learned_detector_recall_and_speed(dev):
     from revenant_amd import proposals
 
-    res = proposals.run(dev, n_train=24, n_val=3, n_test=3, steps=1500, seed=3)
+    res = proposals.run(dev)
     print(json.dumps(res))
     assert res["loss_last"] < res["loss_first"]
-    assert res["events"] >= 10
-    assert res["recall_learned"] >= 0.9 * res["recall_plain"], res
+    assert res["events"] >= 100
+    assert res["recall_learned"] >= 0.98 * res["recall_plain"], res
     assert res["precision_learned"] > res["precision_plain"], res
     assert res["speedup"] > 5.0, res
