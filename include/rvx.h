@@ -1135,6 +1135,85 @@ int32_t rvx_detect(rvx_detector* detector, const double* average, const int32_t*
 const double* rvx_detector_floor(rvx_detector* detector);
 const double* rvx_detector_sigma(rvx_detector* detector);
 
+/* ---- Detection events -------------------------------------------------------
+ * core/detect/detector.cpp after the decision (host, as in the reference):
+ * accepted bands -> candidates (:1168-1334: occupied band, interior-gap
+ * split, the SNR test in 2500 Hz, centre and width in Hz from the bin edges),
+ * then per decision reject_residual (:1336) and update_tracks (:1585). A
+ * track's first_seen / last_detected / last_seen are the sample indices of
+ * the decisions that found it first and last: the event's boundaries. Not
+ * built: band shape, margin confidence, emitter memory, spur cancelling,
+ * channel association, level statistics. */
+typedef struct rvx_event_config {
+    double bin_width_hz;             /* the frame's bin spacing */
+    double bin_zero_hz;              /* bin 0's centre, relative to the source centre */
+    int64_t source_center_hz;        /* added to every candidate centre */
+    int64_t source_rate;             /* samples per second (the hold is in samples) */
+    double detection_threshold_db;   /* 6 (detector.h:600) */
+    double occupied_power_fraction;  /* 0.99 (:857) */
+    double split_gap_db;             /* 1 (:910) */
+    uint32_t split_gap_bins;         /* 8 (:909) */
+    uint32_t residual_decisions;     /* 4 (:1184); 0: never withheld */
+    double average_seconds;          /* 1 (:643) */
+    double residual_rate_tolerance;  /* 0.25 (:1183) */
+    double association_overlap;      /* 0.3 (:1188) */
+    double confidence_rise;          /* 0.35 (:1194) */
+    double confidence_half_life_seconds; /* 1 (:1195) */
+    double drop_confidence;          /* 0.05 (:1196) */
+    double centre_smoothing;         /* 0.4 (:1200) */
+    double bootstrap_hold_seconds;   /* 3 (:955) */
+    uint32_t birth_hits;             /* 3 (:918) */
+    uint32_t max_tracks;             /* 512 (:938) */
+} rvx_event_config;
+
+/* detector.h:299-360 Candidate (the fields built here) */
+typedef struct rvx_candidate {
+    uint32_t first_bin, last_bin;
+    int64_t center_hz, bandwidth_hz;
+    double snr_2500_db, noise_floor_dbfs, peak_dbfs;
+} rvx_candidate;
+
+#define RVX_TRACK_PENDING 0u
+#define RVX_TRACK_LIVE 1u
+#define RVX_TRACK_HELD 2u
+#define RVX_TRACK_MERGED 3u
+
+/* detector.h:362-575 Track (the fields built here) */
+typedef struct rvx_track {
+    uint64_t id;
+    uint32_t state;              /* RVX_TRACK_* */
+    uint32_t hits, misses, held_spells;
+    int64_t center_hz, bandwidth_hz;
+    double snr_2500_db, confidence;
+    uint64_t first_seen, last_seen, last_detected;  /* sample indices */
+    uint64_t merged_into;        /* parent id while merged */
+    uint32_t first_bin, last_bin;  /* the last candidate's bins */
+} rvx_track;
+
+typedef struct rvx_tracker_stats {
+    uint64_t decisions, candidates, candidates_residual, residual_parts_degenerate;
+    uint64_t tracks_born, tracks_dropped, births_refused, merges, splits;
+} rvx_tracker_stats;
+
+void rvx_event_config_defaults(rvx_event_config* config);
+/* average, floor: host, bins doubles (the decision's input and its floor);
+ * bands: the decision's accepted bands (rvx_detect), ascending and disjoint. */
+int32_t rvx_candidates(const rvx_event_config* config, const double* average, const double* floor,
+                       uint32_t bins, const rvx_detect_band* bands, uint32_t n_bands,
+                       rvx_candidate* out, uint32_t max_out, uint32_t* n_out);
+typedef struct rvx_tracker rvx_tracker;
+int32_t rvx_tracker_create(const rvx_event_config* config, uint32_t bins, rvx_tracker** out);
+void rvx_tracker_destroy(rvx_tracker* tracker);
+/* One decision at sample index `now`, elapsed_seconds after the previous one:
+ * residual rejection over the candidates, then the tracks' update.
+ * *n_kept (may be NULL): candidates left after residual rejection. */
+int32_t rvx_tracker_update(rvx_tracker* tracker, const double* average, const double* floor,
+                           const rvx_candidate* candidates, uint32_t n, uint64_t now,
+                           double elapsed_seconds, uint32_t* n_kept);
+int32_t rvx_tracker_tracks(const rvx_tracker* tracker, rvx_track* out, uint32_t max_out,
+                           uint32_t* n);
+int32_t rvx_tracker_stats_get(const rvx_tracker* tracker, rvx_tracker_stats* out);
+
 /* ---- Learned proposals ------------------------------------------------------
  * A five-layer 3x3 convolutional network (1 -> 32 -> 32 -> 32 -> 32 -> 1,
  * ReLU) over log-power spectrogram tiles, per-pixel logits of "a signal the

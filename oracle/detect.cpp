@@ -23,6 +23,8 @@
 #include <bit>
 #include <cmath>
 #include <cstdint>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 namespace {
@@ -282,6 +284,468 @@ int oracle_detect(const ODetectConfig* cfg, const double* average, double* floor
     if (accepted.size() > max_bands) return -1;
     for (size_t i = 0; i < accepted.size(); ++i)
         bands[i] = OBand{accepted[i].start, accepted[i].width, accepted[i].deflection};
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Detection events: core/detect/detector.cpp's find_candidates tail
+// (:1168-1275) and emit_candidate (:1277-1334), reject_residual (:1336-1583)
+// and update_tracks (:1585-1900), restated in the reference's statement order
+// without the band shape, margin confidence, emitter memory, spur cancelling,
+// channel association and level statistics.
+//
+// Pinning (tests/test_oracle_events.py): a box's candidate against its
+// occupied-band and edge closed forms; a band cut at a floor-level gap;
+// tracks of bursts switched on and off at known decisions against the birth,
+// hold and drop rules worked by hand (first_seen, last_detected, ids); two
+// tracks merging into one candidate and splitting again; an exponentially
+// emptying average withheld after residual_decisions decisions while a steady
+// band is not.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr double kDbPerNeperO = 4.342944819032518;
+constexpr std::size_t kResidualPartsO = 8;
+constexpr double kResidualPartFloorO = 0.5;
+
+double db_to_linear_o(double db) { return std::exp2(db * static_cast<double>(kDbToLog2)); }
+double linear_to_db_o(double linear) {
+    const double floored = linear > static_cast<double>(kSpectrumPowerFloor)
+                               ? linear
+                               : static_cast<double>(kSpectrumPowerFloor);
+    return 10.0 * std::log10(floored);
+}
+
+}  // namespace
+
+extern "C" {
+
+struct OEventConfig {
+    double bin_width_hz, bin_zero_hz;
+    int64_t source_center_hz, source_rate;
+    double detection_threshold_db, occupied_power_fraction, split_gap_db;
+    uint32_t split_gap_bins, residual_decisions;
+    double average_seconds, residual_rate_tolerance, association_overlap, confidence_rise,
+        confidence_half_life_seconds, drop_confidence, centre_smoothing, bootstrap_hold_seconds;
+    uint32_t birth_hits, max_tracks;
+};
+
+struct OCandidate {
+    uint32_t first_bin, last_bin;
+    int64_t center, bandwidth;
+    double snr_2500_db, noise_floor_dbfs, peak_dbfs;
+};
+
+struct OTrack {
+    uint64_t id;
+    uint32_t state, hits, misses, held_spells;
+    int64_t center, bandwidth;
+    double snr_2500_db, confidence;
+    uint64_t first_seen, last_seen, last_detected, merged_into;
+    uint32_t first_bin, last_bin;
+};
+
+}  // extern "C"
+
+namespace {
+
+enum : uint32_t { kPending = 0, kLive = 1, kHeld = 2, kMerged = 3 };
+
+struct EventDecision {
+    std::vector<double> excess_cumulative, floor_cumulative;
+    void build(const double* average, const double* floor, std::size_t bins) {
+        excess_cumulative.assign(bins + 1, 0.0);
+        floor_cumulative.assign(bins + 1, 0.0);
+        for (std::size_t i = 0; i < bins; ++i) {
+            const double excess = average[i] - floor[i];
+            excess_cumulative[i + 1] = excess_cumulative[i] + excess;
+            floor_cumulative[i + 1] = floor_cumulative[i] + floor[i];
+        }
+    }
+};
+
+bool emit_candidate_o(const OEventConfig& c, const EventDecision& d, const double* average,
+                      std::size_t bins, std::size_t first, std::size_t last,
+                      std::vector<OCandidate>& out) {
+    if (last < first || last >= bins) return false;
+    const auto count = static_cast<double>(last - first + 1);
+    const double excess = d.excess_cumulative[last + 1] - d.excess_cumulative[first];
+    const double noise = (d.floor_cumulative[last + 1] - d.floor_cumulative[first]) / count;
+    if (excess <= 0.0 || noise <= 0.0) return false;
+    const double snr = excess * c.bin_width_hz / (noise * kReferenceBandwidthHz);
+    if (!(snr > db_to_linear_o(c.detection_threshold_db))) return false;
+    double peak = 0.0;
+    for (std::size_t i = first; i <= last; ++i) peak = std::max(peak, average[i]);
+    const double low_edge = c.bin_zero_hz + (static_cast<double>(first) - 0.5) * c.bin_width_hz;
+    const double high_edge = c.bin_zero_hz + (static_cast<double>(last) + 0.5) * c.bin_width_hz;
+    OCandidate candidate{};
+    candidate.first_bin = static_cast<uint32_t>(first);
+    candidate.last_bin = static_cast<uint32_t>(last);
+    candidate.snr_2500_db = 10.0 * std::log10(snr);
+    candidate.noise_floor_dbfs = linear_to_db_o(noise);
+    candidate.peak_dbfs = linear_to_db_o(peak);
+    candidate.center = c.source_center_hz +
+                       static_cast<int64_t>(std::llround(0.5 * (low_edge + high_edge)));
+    candidate.bandwidth = static_cast<int64_t>(std::llround(high_edge - low_edge));
+    out.push_back(candidate);
+    return true;
+}
+
+struct OracleTracker {
+    OEventConfig config;
+    std::size_t bins = 0;
+    uint64_t next_id = 1;
+    EventDecision decision;
+    struct Decaying {
+        uint32_t first_bin = 0, last_bin = 0, run_first_bin = 0, run_last_bin = 0;
+        double level_db = 0.0, run_level_db = 0.0, run_seconds = 0.0;
+        uint32_t run = 0;
+        double run_part_db[kResidualPartsO] = {};
+    };
+    std::vector<Decaying> decaying;
+    std::vector<OTrack> all;
+
+    void reject_residual(std::vector<OCandidate>& candidates, double elapsed_seconds) {
+        const auto level_of = [this](uint32_t first, uint32_t last) {
+            const std::size_t low = std::min<std::size_t>(first, bins - 1);
+            const std::size_t high = std::min<std::size_t>(last, bins - 1);
+            const auto count = static_cast<double>(high - low + 1);
+            const double excess =
+                decision.excess_cumulative[high + 1] - decision.excess_cumulative[low];
+            return linear_to_db_o(excess / count);
+        };
+        const auto part_of = [&level_of](uint32_t first, uint32_t last, std::size_t p) {
+            const auto span = static_cast<uint64_t>(last - first + 1U);
+            const auto low_offset = static_cast<uint32_t>(span * p / kResidualPartsO);
+            const auto high_offset = static_cast<uint32_t>(span * (p + 1) / kResidualPartsO);
+            const uint32_t last_offset = high_offset > low_offset ? high_offset - 1U : low_offset;
+            return level_of(first + low_offset, first + last_offset);
+        };
+        std::vector<Decaying> next;
+        std::size_t kept = 0;
+        for (std::size_t c = 0; c < candidates.size(); ++c) {
+            const OCandidate candidate = candidates[c];
+            const Decaying* matched = nullptr;
+            uint32_t best_overlap = 0;
+            for (const Decaying& held : decaying) {
+                if (held.last_bin < candidate.first_bin || held.first_bin > candidate.last_bin)
+                    continue;
+                const uint32_t overlap = std::min(held.last_bin, candidate.last_bin) -
+                                         std::max(held.first_bin, candidate.first_bin) + 1U;
+                if (overlap > best_overlap) {
+                    best_overlap = overlap;
+                    matched = &held;
+                }
+            }
+            Decaying now;
+            now.first_bin = candidate.first_bin;
+            now.last_bin = candidate.last_bin;
+            now.run_first_bin = candidate.first_bin;
+            now.run_last_bin = candidate.last_bin;
+            bool falling = false;
+            if (matched != nullptr) {
+                const double carried = level_of(matched->run_first_bin, matched->run_last_bin);
+                if (carried < matched->level_db) {
+                    const double seconds = matched->run_seconds + elapsed_seconds;
+                    const double expected = kDbPerNeperO * seconds / config.average_seconds;
+                    const double fall = matched->run_level_db - carried;
+                    bool uniform = true;
+                    for (std::size_t p = 0; uniform && p < kResidualPartsO; ++p) {
+                        const double part_fall =
+                            matched->run_part_db[p] -
+                            part_of(matched->run_first_bin, matched->run_last_bin, p);
+                        uniform = part_fall >= kResidualPartFloorO * fall;
+                    }
+                    if (uniform && fall >= (1.0 - config.residual_rate_tolerance) * expected &&
+                        fall <= (1.0 + config.residual_rate_tolerance) * expected) {
+                        falling = true;
+                        now.run_first_bin = matched->run_first_bin;
+                        now.run_last_bin = matched->run_last_bin;
+                        now.level_db = carried;
+                        now.run_level_db = matched->run_level_db;
+                        now.run_seconds = seconds;
+                        now.run = matched->run + 1;
+                        for (std::size_t p = 0; p < kResidualPartsO; ++p)
+                            now.run_part_db[p] = matched->run_part_db[p];
+                    }
+                }
+            }
+            if (!falling) {
+                now.level_db = level_of(now.run_first_bin, now.run_last_bin);
+                now.run_level_db = now.level_db;
+                for (std::size_t p = 0; p < kResidualPartsO; ++p)
+                    now.run_part_db[p] = part_of(now.run_first_bin, now.run_last_bin, p);
+            }
+            next.push_back(now);
+            if (config.residual_decisions > 0 && now.run >= config.residual_decisions) continue;
+            candidates[kept] = candidate;
+            ++kept;
+        }
+        candidates.resize(kept);
+        decaying.swap(next);
+    }
+
+    void update_tracks(const std::vector<OCandidate>& candidates, uint64_t now,
+                       double elapsed_seconds) {
+        std::sort(all.begin(), all.end(), [](const OTrack& a, const OTrack& b) {
+            if (a.center != b.center) return a.center < b.center;
+            return a.id < b.id;
+        });
+        const std::size_t track_count = all.size(), candidate_count = candidates.size();
+        const auto band_of = [](int64_t center, int64_t bandwidth) {
+            const double half = 0.5 * static_cast<double>(std::max<int64_t>(1, bandwidth));
+            return std::pair<double, double>{static_cast<double>(center) - half,
+                                             static_cast<double>(center) + half};
+        };
+        struct P {
+            uint32_t track, candidate;
+            double score;
+            uint64_t first_seen, id;
+        };
+        std::vector<P> pairs;
+        for (std::size_t t = 0; t < track_count; ++t) {
+            const auto [track_low, track_high] = band_of(all[t].center, all[t].bandwidth);
+            const double track_width = track_high - track_low;
+            for (std::size_t c = 0; c < candidate_count; ++c) {
+                const auto [low, high] = band_of(candidates[c].center, candidates[c].bandwidth);
+                if (high < track_low) continue;
+                if (low > track_high) break;
+                const double width = high - low;
+                const double overlap = std::min(track_high, high) - std::max(track_low, low);
+                if (overlap <= 0.0) continue;
+                if (overlap / std::min(track_width, width) < config.association_overlap) continue;
+                pairs.push_back(P{static_cast<uint32_t>(t), static_cast<uint32_t>(c),
+                                  overlap / std::max(track_width, width), all[t].first_seen,
+                                  all[t].id});
+            }
+        }
+        std::sort(pairs.begin(), pairs.end(), [](const P& a, const P& b) {
+            if (a.score != b.score) return a.score > b.score;
+            if (a.first_seen != b.first_seen) return a.first_seen < b.first_seen;
+            if (a.id != b.id) return a.id < b.id;
+            return a.candidate < b.candidate;
+        });
+        constexpr uint32_t kNone = 0xFFFFFFFFu;
+        std::vector<uint32_t> candidate_of_track(track_count, kNone);
+        std::vector<uint32_t> track_of_candidate(candidate_count, kNone);
+        for (const P& pair : pairs) {
+            if (candidate_of_track[pair.track] != kNone) continue;
+            if (track_of_candidate[pair.candidate] != kNone) continue;
+            candidate_of_track[pair.track] = pair.candidate;
+            track_of_candidate[pair.candidate] = pair.track;
+        }
+        std::vector<uint32_t> gated;
+        for (std::size_t c = 0; c < candidate_count; ++c) {
+            const uint32_t owner = track_of_candidate[c];
+            if (owner == kNone) continue;
+            gated.clear();
+            if (all[owner].state != kPending) gated.push_back(owner);
+            for (const P& pair : pairs) {
+                if (pair.candidate != c || pair.track == owner) continue;
+                if (candidate_of_track[pair.track] != kNone) continue;
+                if (all[pair.track].state == kPending) continue;
+                gated.push_back(pair.track);
+            }
+            if (gated.empty() || (gated.size() == 1 && gated.front() == owner)) continue;
+            uint32_t eldest = gated.front();
+            for (const uint32_t t : gated) {
+                const OTrack& a = all[t];
+                const OTrack& b = all[eldest];
+                if (std::tie(a.first_seen, a.id) < std::tie(b.first_seen, b.id)) eldest = t;
+            }
+            if (eldest != owner) {
+                candidate_of_track[owner] = kNone;
+                candidate_of_track[eldest] = static_cast<uint32_t>(c);
+                track_of_candidate[c] = eldest;
+            }
+            const uint64_t parent = all[eldest].id;
+            for (const uint32_t t : gated) {
+                if (t == eldest) continue;
+                OTrack& child = all[t];
+                child.state = kMerged;
+                child.merged_into = parent;
+                child.last_seen = now;
+            }
+        }
+        const double decay = std::exp2(-elapsed_seconds / config.confidence_half_life_seconds);
+        const double hold_samples =
+            config.bootstrap_hold_seconds * static_cast<double>(config.source_rate);
+        std::vector<OTrack> survivors;
+        for (std::size_t t = 0; t < track_count; ++t) {
+            OTrack track = all[t];
+            const uint32_t assigned = candidate_of_track[t];
+            if (assigned != kNone) {
+                const OCandidate& candidate = candidates[assigned];
+                track.merged_into = 0;
+                const double smoothing = track.state == kPending ? 1.0 : config.centre_smoothing;
+                const double centre =
+                    static_cast<double>(track.center) +
+                    smoothing * static_cast<double>(candidate.center - track.center);
+                const double width =
+                    static_cast<double>(track.bandwidth) +
+                    smoothing * static_cast<double>(candidate.bandwidth - track.bandwidth);
+                track.center = static_cast<int64_t>(std::llround(centre));
+                track.bandwidth = std::max<int64_t>(1, static_cast<int64_t>(std::llround(width)));
+                track.snr_2500_db += smoothing * (candidate.snr_2500_db - track.snr_2500_db);
+                track.first_bin = candidate.first_bin;
+                track.last_bin = candidate.last_bin;
+                ++track.hits;
+                track.misses = 0;
+                track.last_detected = now;
+                track.last_seen = now;
+                track.confidence += (1.0 - track.confidence) * config.confidence_rise;
+                if (track.state == kPending) {
+                    if (track.hits >= config.birth_hits) track.state = kLive;
+                } else {
+                    track.state = kLive;
+                }
+                survivors.push_back(track);
+                continue;
+            }
+            if (track.state == kPending) continue;
+            const bool merged = track.state == kMerged;
+            if (!merged) {
+                if (track.state == kLive) ++track.held_spells;
+                ++track.misses;
+                track.state = kHeld;
+                track.confidence *= decay;
+            }
+            track.last_seen = now;
+            const auto silent = static_cast<double>(now - track.last_detected);
+            if (track.confidence < config.drop_confidence || silent > hold_samples) continue;
+            survivors.push_back(track);
+        }
+        for (std::size_t c = 0; c < candidate_count; ++c) {
+            if (track_of_candidate[c] != kNone) continue;
+            if (survivors.size() >= config.max_tracks) continue;
+            const OCandidate& candidate = candidates[c];
+            OTrack track{};
+            track.id = next_id++;
+            track.state = kPending;
+            track.center = candidate.center;
+            track.bandwidth = std::max<int64_t>(1, candidate.bandwidth);
+            track.snr_2500_db = candidate.snr_2500_db;
+            track.first_bin = candidate.first_bin;
+            track.last_bin = candidate.last_bin;
+            track.confidence = config.confidence_rise;
+            track.first_seen = now;
+            track.last_seen = now;
+            track.last_detected = now;
+            track.hits = 1;
+            if (track.hits >= config.birth_hits) track.state = kLive;
+            survivors.push_back(track);
+        }
+        all.swap(survivors);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int oracle_candidates(const OEventConfig* cfg, const double* average, const double* floor,
+                      uint32_t bins, const OBand* bands, uint32_t n_bands, OCandidate* out,
+                      uint32_t max_out, uint32_t* n_out) {
+    if (!cfg || !average || !floor || !n_out) return -1;
+    const OEventConfig& c = *cfg;
+    EventDecision d;
+    d.build(average, floor, bins);
+    const double gap_ratio = db_to_linear_o(c.split_gap_db);
+    std::vector<OCandidate> candidates;
+    std::vector<uint8_t> separator;
+    for (uint32_t index = 0; index < n_bands; ++index) {
+        const std::size_t begin = bands[index].start;
+        const std::size_t end = begin + bands[index].width;
+        double total = 0.0;
+        for (std::size_t i = begin; i < end; ++i) total += std::max(0.0, average[i] - floor[i]);
+        if (total <= 0.0) continue;
+        const double tail = total * (1.0 - c.occupied_power_fraction) * 0.5;
+        std::size_t low = begin;
+        double running = 0.0;
+        for (std::size_t i = begin; i < end; ++i) {
+            const double excess = std::max(0.0, average[i] - floor[i]);
+            if (running + excess > tail) {
+                low = i;
+                break;
+            }
+            running += excess;
+        }
+        std::size_t high = end - 1;
+        running = 0.0;
+        for (std::size_t i = end; i-- > begin;) {
+            const double excess = std::max(0.0, average[i] - floor[i]);
+            if (running + excess > tail) {
+                high = i;
+                break;
+            }
+            running += excess;
+        }
+        if (high < low) high = low;
+        const std::size_t count = high - low + 1;
+        separator.assign(count, 0);
+        std::size_t i = 0;
+        while (i < count) {
+            if (average[low + i] > floor[low + i] * gap_ratio) {
+                ++i;
+                continue;
+            }
+            std::size_t j = i;
+            while (j < count && average[low + j] <= floor[low + j] * gap_ratio) ++j;
+            if (j - i >= c.split_gap_bins)
+                std::fill(separator.begin() + static_cast<std::ptrdiff_t>(i),
+                          separator.begin() + static_cast<std::ptrdiff_t>(j), uint8_t{1});
+            i = j;
+        }
+        bool emitted = false;
+        i = 0;
+        while (i < count) {
+            if (separator[i] != 0) {
+                ++i;
+                continue;
+            }
+            std::size_t j = i;
+            while (j < count && separator[j] == 0) ++j;
+            emitted = emit_candidate_o(c, d, average, bins, low + i, low + j - 1, candidates) ||
+                      emitted;
+            i = j;
+        }
+        if (!emitted) static_cast<void>(emit_candidate_o(c, d, average, bins, low, high, candidates));
+    }
+    *n_out = static_cast<uint32_t>(candidates.size());
+    if (candidates.size() > max_out) return -1;
+    std::copy(candidates.begin(), candidates.end(), out);
+    return 0;
+}
+
+void* oracle_tracker_create(const OEventConfig* cfg, uint32_t bins) {
+    auto* t = new OracleTracker;
+    t->config = *cfg;
+    t->bins = bins;
+    return t;
+}
+
+void oracle_tracker_destroy(void* t) { delete static_cast<OracleTracker*>(t); }
+
+int oracle_tracker_update(void* handle, const double* average, const double* floor,
+                          const OCandidate* candidates, uint32_t n, uint64_t now,
+                          double elapsed_seconds, uint32_t* n_kept) {
+    auto* t = static_cast<OracleTracker*>(handle);
+    t->decision.build(average, floor, t->bins);
+    std::vector<OCandidate> c(candidates, candidates + n);
+    t->reject_residual(c, elapsed_seconds);
+    t->update_tracks(c, now, elapsed_seconds);
+    if (n_kept) *n_kept = static_cast<uint32_t>(c.size());
+    return 0;
+}
+
+int oracle_tracker_tracks(void* handle, OTrack* out, uint32_t max_out, uint32_t* n) {
+    auto* t = static_cast<OracleTracker*>(handle);
+    *n = static_cast<uint32_t>(t->all.size());
+    if (t->all.size() > max_out) return -1;
+    std::copy(t->all.begin(), t->all.end(), out);
     return 0;
 }
 

@@ -47,6 +47,92 @@ def detect_config(bins: int, bin_width_hz: float, threshold_db: float = 6.0, **k
     return c
 
 
+class OEventConfig(C.Structure):
+    _fields_ = [("bin_width_hz", C.c_double), ("bin_zero_hz", C.c_double),
+                ("source_center_hz", C.c_int64), ("source_rate", C.c_int64),
+                ("detection_threshold_db", C.c_double), ("occupied_power_fraction", C.c_double),
+                ("split_gap_db", C.c_double), ("split_gap_bins", u32),
+                ("residual_decisions", u32), ("average_seconds", C.c_double),
+                ("residual_rate_tolerance", C.c_double), ("association_overlap", C.c_double),
+                ("confidence_rise", C.c_double), ("confidence_half_life_seconds", C.c_double),
+                ("drop_confidence", C.c_double), ("centre_smoothing", C.c_double),
+                ("bootstrap_hold_seconds", C.c_double), ("birth_hits", u32),
+                ("max_tracks", u32)]
+
+
+class OCandidate(C.Structure):
+    _fields_ = [("first_bin", u32), ("last_bin", u32), ("center", C.c_int64),
+                ("bandwidth", C.c_int64), ("snr_2500_db", C.c_double),
+                ("noise_floor_dbfs", C.c_double), ("peak_dbfs", C.c_double)]
+
+
+class OTrack(C.Structure):
+    _fields_ = [("id", u64), ("state", u32), ("hits", u32), ("misses", u32),
+                ("held_spells", u32), ("center", C.c_int64), ("bandwidth", C.c_int64),
+                ("snr_2500_db", C.c_double), ("confidence", C.c_double), ("first_seen", u64),
+                ("last_seen", u64), ("last_detected", u64), ("merged_into", u64),
+                ("first_bin", u32), ("last_bin", u32)]
+
+
+def event_config(bin_width_hz: float, bin_zero_hz: float, source_rate: int,
+                 source_center_hz: int = 0, **kw) -> OEventConfig:
+    """DetectorConfig's defaults for the event stages (core/detect/detector.h:600-1200)."""
+    d = dict(detection_threshold_db=6.0, occupied_power_fraction=0.99, split_gap_db=1.0,
+             split_gap_bins=8, residual_decisions=4, average_seconds=1.0,
+             residual_rate_tolerance=0.25, association_overlap=0.3, confidence_rise=0.35,
+             confidence_half_life_seconds=1.0, drop_confidence=0.05, centre_smoothing=0.4,
+             bootstrap_hold_seconds=3.0, birth_hits=3, max_tracks=512)
+    d.update(kw)
+    c = OEventConfig()
+    c.bin_width_hz, c.bin_zero_hz = bin_width_hz, bin_zero_hz
+    c.source_center_hz, c.source_rate = source_center_hz, source_rate
+    for k, v in d.items():
+        setattr(c, k, v)
+    return c
+
+
+def candidates(cfg: OEventConfig, average: np.ndarray, floor: np.ndarray, bands) -> list:
+    avg = np.ascontiguousarray(average, dtype=np.float64)
+    flo = np.ascontiguousarray(floor, dtype=np.float64)
+    arr = (OBand * max(1, len(bands)))(*[OBand(int(s), int(w), float(d)) for s, w, d in bands])
+    cap = 4 * len(bands) + 4
+    out = (OCandidate * cap)()
+    n = u32()
+    _check(lib().oracle_candidates(C.byref(cfg), _a(avg), _a(flo), u32(avg.size), arr,
+                                   u32(len(bands)), out, u32(cap), C.byref(n)), "candidates")
+    return [out[i] for i in range(n.value)]
+
+
+class OracleTracker:
+    """reject_residual + update_tracks, decision by decision."""
+
+    def __init__(self, cfg: OEventConfig, bins: int):
+        lib().oracle_tracker_create.restype = C.c_void_p
+        self.h = C.c_void_p(lib().oracle_tracker_create(C.byref(cfg), u32(bins)))
+
+    def update(self, average, floor, cands, now: int, elapsed: float) -> int:
+        avg = np.ascontiguousarray(average, dtype=np.float64)
+        flo = np.ascontiguousarray(floor, dtype=np.float64)
+        arr = (OCandidate * max(1, len(cands)))(*cands)
+        kept = u32()
+        _check(lib().oracle_tracker_update(self.h, _a(avg), _a(flo), arr, u32(len(cands)),
+                                           u64(now), C.c_double(elapsed), C.byref(kept)),
+               "tracker_update")
+        return kept.value
+
+    def tracks(self) -> list:
+        n = u32()
+        cap = 4096
+        out = (OTrack * cap)()
+        _check(lib().oracle_tracker_tracks(self.h, out, u32(cap), C.byref(n)), "tracker_tracks")
+        return [out[i] for i in range(n.value)]
+
+    def close(self) -> None:
+        if self.h:
+            lib().oracle_tracker_destroy(self.h)
+            self.h = None
+
+
 _lib = None
 
 

@@ -288,6 +288,36 @@ class DetectBand(C.Structure):
     _fields_ = [("start", u32), ("width", u32), ("deflection", f64)]
 
 
+class EventConfig(C.Structure):
+    _fields_ = [("bin_width_hz", f64), ("bin_zero_hz", f64), ("source_center_hz", i64),
+                ("source_rate", i64), ("detection_threshold_db", f64),
+                ("occupied_power_fraction", f64), ("split_gap_db", f64), ("split_gap_bins", u32),
+                ("residual_decisions", u32), ("average_seconds", f64),
+                ("residual_rate_tolerance", f64), ("association_overlap", f64),
+                ("confidence_rise", f64), ("confidence_half_life_seconds", f64),
+                ("drop_confidence", f64), ("centre_smoothing", f64),
+                ("bootstrap_hold_seconds", f64), ("birth_hits", u32), ("max_tracks", u32)]
+
+
+class Candidate(C.Structure):
+    _fields_ = [("first_bin", u32), ("last_bin", u32), ("center_hz", i64), ("bandwidth_hz", i64),
+                ("snr_2500_db", f64), ("noise_floor_dbfs", f64), ("peak_dbfs", f64)]
+
+
+class Track(C.Structure):
+    _fields_ = [("id", u64), ("state", u32), ("hits", u32), ("misses", u32),
+                ("held_spells", u32), ("center_hz", i64), ("bandwidth_hz", i64),
+                ("snr_2500_db", f64), ("confidence", f64), ("first_seen", u64),
+                ("last_seen", u64), ("last_detected", u64), ("merged_into", u64),
+                ("first_bin", u32), ("last_bin", u32)]
+
+
+class TrackerStats(C.Structure):
+    _fields_ = [(n, u64) for n in ("decisions", "candidates", "candidates_residual",
+                                   "residual_parts_degenerate", "tracks_born", "tracks_dropped",
+                                   "births_refused", "merges", "splits")]
+
+
 VRX_TILE_SPAN = 2048  # RVX_VRX_TILE_SPAN
 PREC_BF16X3, PREC_FP16 = 1, 2  # RVX_PREC_*
 
@@ -454,6 +484,15 @@ _SIGNATURES = {
     "rvx_engine_front_end": (i32, [vp, C.POINTER(FrontEndEstimate), C.POINTER(IqCorrectParams)]),
     "rvx_detector_create": (i32, [vp, C.POINTER(DetectConfig), C.POINTER(vp)]),
     "rvx_detector_destroy": (None, [vp]),
+    "rvx_event_config_defaults": (None, [C.POINTER(EventConfig)]),
+    "rvx_candidates": (i32, [C.POINTER(EventConfig), vp, vp, u32, C.POINTER(DetectBand), u32,
+                             C.POINTER(Candidate), u32, C.POINTER(u32)]),
+    "rvx_tracker_create": (i32, [C.POINTER(EventConfig), u32, C.POINTER(vp)]),
+    "rvx_tracker_destroy": (None, [vp]),
+    "rvx_tracker_update": (i32, [vp, vp, vp, C.POINTER(Candidate), u32, u64, f64,
+                                 C.POINTER(u32)]),
+    "rvx_tracker_tracks": (i32, [vp, C.POINTER(Track), u32, C.POINTER(u32)]),
+    "rvx_tracker_stats_get": (i32, [vp, C.POINTER(TrackerStats)]),
     "rvx_detect": (i32, [vp, vp, vp, C.POINTER(DetectBand), u32, C.POINTER(u32),
                          C.POINTER(u32)]),
     "rvx_detector_floor": (vp, [vp]),

@@ -1,0 +1,536 @@
+// Detection events: from a plain decision's accepted bands to candidates, the
+// residual of a stopped signal withheld, and candidates followed from decision
+// to decision as tracks whose first and last sample indices are the event's
+// boundaries.
+//
+//   rvx_candidates        core/detect/detector.cpp:1168-1334 (find_candidates' tail,
+//                         emit_candidate): the occupied band, the interior-gap split,
+//                         the SNR test in the 2500 Hz reference bandwidth, centre and
+//                         width in hertz from the bin edges
+//   rvx_tracker_update    :1336-1583 reject_residual, then :1585-1900 update_tracks
+//
+// The reference runs these on the host after each decision, over tens of
+// candidates; so does this build. They work in double with the reference's
+// operation order (sequential prefix sums, the same comparisons), so the bins,
+// hertz and sample indices they produce are the reference's integers.
+// Not built: the band shape measurement, margin confidence, emitter memory,
+// spur cancelling, channel association and per-track level statistics.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <new>
+#include <vector>
+
+#include "common.h"
+
+namespace {
+
+constexpr double kReferenceBandwidthHz = 2500.0;       // detector.h:128
+constexpr float kDbToLog2 = 0.332192809488736234787F;   // detector.cpp:26
+constexpr double kPowerFloor = double(1.0e-20F);        // spectrum_reference.h:104
+constexpr double kDbPerNeper = 4.342944819032518;       // 10 / ln 10
+constexpr uint32_t kParts = 8;                          // detector.h:151
+constexpr double kPartFloor = 0.5;                      // detector.h:177
+constexpr uint32_t kUnassigned = 0xFFFFFFFFu;
+
+double from_db(double db) { return std::exp2(db * double(kDbToLog2)); }
+double to_db(double linear) { return 10.0 * std::log10(linear > kPowerFloor ? linear : kPowerFloor); }
+
+// Running sums over a decision's spectrum, in bin order (the reference's).
+struct Sums {
+    std::vector<double> excess, floor;  // bins + 1 each; excess is signed
+    void build(const double* avg, const double* flo, uint32_t bins) {
+        excess.assign(size_t(bins) + 1, 0.0);
+        floor.assign(size_t(bins) + 1, 0.0);
+        for (uint32_t i = 0; i < bins; ++i) {
+            excess[i + 1] = excess[i] + (avg[i] - flo[i]);
+            floor[i + 1] = floor[i] + flo[i];
+        }
+    }
+    double excess_over(size_t first, size_t last) const { return excess[last + 1] - excess[first]; }
+    double floor_over(size_t first, size_t last) const { return floor[last + 1] - floor[first]; }
+};
+
+int32_t check_config(const rvx_event_config& c, const char* who) {
+    if (!(c.bin_width_hz > 0.0) || !std::isfinite(c.bin_zero_hz))
+        return rvx::fail(RVX_EINVAL, "%s: bin width %g Hz", who, c.bin_width_hz);
+    if (!(c.occupied_power_fraction > 0.0 && c.occupied_power_fraction <= 1.0))
+        return rvx::fail(RVX_EINVAL, "%s: occupied power fraction %g", who,
+                         c.occupied_power_fraction);
+    if (c.birth_hits == 0 || c.max_tracks == 0)
+        return rvx::fail(RVX_EINVAL, "%s: birth_hits and max_tracks must be at least 1", who);
+    if (!(c.average_seconds > 0.0) || !(c.confidence_half_life_seconds > 0.0) ||
+        c.source_rate <= 0)
+        return rvx::fail(RVX_EINVAL, "%s: time constants and the source rate must be positive",
+                         who);
+    return RVX_OK;
+}
+
+// emit_candidate (:1277-1334) for bins [first, last]: false when the band
+// carries no excess or does not clear the threshold.
+bool emit(const rvx_event_config& c, const Sums& s, const double* avg, uint32_t bins,
+          size_t first, size_t last, std::vector<rvx_candidate>& out) {
+    if (last < first || last >= bins) return false;
+    const double count = double(last - first + 1);
+    const double excess = s.excess_over(first, last);
+    const double noise = s.floor_over(first, last) / count;
+    if (excess <= 0.0 || noise <= 0.0) return false;
+    const double snr = excess * c.bin_width_hz / (noise * kReferenceBandwidthHz);
+    if (!(snr > from_db(c.detection_threshold_db))) return false;
+    double peak = 0.0;
+    for (size_t i = first; i <= last; ++i) peak = std::max(peak, avg[i]);
+    const double low_edge = c.bin_zero_hz + (double(first) - 0.5) * c.bin_width_hz;
+    const double high_edge = c.bin_zero_hz + (double(last) + 0.5) * c.bin_width_hz;
+    rvx_candidate k{};
+    k.first_bin = uint32_t(first);
+    k.last_bin = uint32_t(last);
+    k.snr_2500_db = 10.0 * std::log10(snr);
+    k.noise_floor_dbfs = to_db(noise);
+    k.peak_dbfs = to_db(peak);
+    k.center_hz = c.source_center_hz + int64_t(std::llround(0.5 * (low_edge + high_edge)));
+    k.bandwidth_hz = int64_t(std::llround(high_edge - low_edge));
+    out.push_back(k);
+    return true;
+}
+
+// One accepted band to candidates (:1168-1275): trim to the span holding the
+// occupied fraction of its excess (the rest split between the tails), then cut
+// it at interior runs of at least split_gap_bins bins that sit within
+// split_gap_db of the floor.
+void band_candidates(const rvx_event_config& c, const Sums& s, const double* avg,
+                     const double* flo, uint32_t bins, size_t begin, size_t end,
+                     std::vector<uint8_t>& gap, std::vector<rvx_candidate>& out) {
+    double total = 0.0;
+    for (size_t i = begin; i < end; ++i) total += std::max(0.0, avg[i] - flo[i]);
+    if (total <= 0.0) return;
+    const double tail = total * (1.0 - c.occupied_power_fraction) * 0.5;
+    size_t low = begin, high = end - 1;
+    double acc = 0.0;
+    for (size_t i = begin; i < end; ++i) {
+        const double e = std::max(0.0, avg[i] - flo[i]);
+        if (acc + e > tail) {
+            low = i;
+            break;
+        }
+        acc += e;
+    }
+    acc = 0.0;
+    for (size_t i = end; i-- > begin;) {
+        const double e = std::max(0.0, avg[i] - flo[i]);
+        if (acc + e > tail) {
+            high = i;
+            break;
+        }
+        acc += e;
+    }
+    high = std::max(high, low);
+
+    const double ratio = from_db(c.split_gap_db);
+    const size_t n = high - low + 1;
+    gap.assign(n, 0);
+    for (size_t i = 0; i < n;) {
+        if (avg[low + i] > flo[low + i] * ratio) {
+            ++i;
+            continue;
+        }
+        size_t j = i;
+        while (j < n && avg[low + j] <= flo[low + j] * ratio) ++j;
+        if (j - i >= c.split_gap_bins) std::fill(gap.begin() + i, gap.begin() + j, uint8_t(1));
+        i = j;
+    }
+    bool any = false;
+    for (size_t i = 0; i < n;) {
+        if (gap[i]) {
+            ++i;
+            continue;
+        }
+        size_t j = i;
+        while (j < n && !gap[j]) ++j;
+        any = emit(c, s, avg, bins, low + i, low + j - 1, out) || any;
+        i = j;
+    }
+    if (!any) (void)emit(c, s, avg, bins, low, high, out);
+}
+
+struct Residual {
+    uint32_t first_bin = 0, last_bin = 0;      // the candidate's band this decision
+    uint32_t run_first = 0, run_last = 0;      // the band the run is measured over
+    double level_db = 0.0, run_level_db = 0.0, run_seconds = 0.0;
+    uint32_t run = 0;
+    double part_db[kParts] = {};
+};
+
+struct Pair {
+    uint32_t track, candidate;
+    double score;
+    uint64_t first_seen, id;
+};
+
+}  // namespace
+
+struct rvx_tracker {
+    rvx_event_config cfg{};
+    uint32_t bins = 0;
+    uint64_t next_id = 1;
+    Sums sums;
+    std::vector<Residual> residuals;
+    std::vector<rvx_track> tracks;
+    rvx_tracker_stats stats{};
+
+    double level_db(uint32_t first, uint32_t last) const {
+        const size_t lo = std::min<size_t>(first, bins - 1), hi = std::min<size_t>(last, bins - 1);
+        return to_db(sums.excess_over(lo, hi) / double(hi - lo + 1));
+    }
+    double part_db(uint32_t first, uint32_t last, uint32_t p) const {
+        const uint64_t span = uint64_t(last - first) + 1;
+        const uint32_t lo = uint32_t(span * p / kParts), hi = uint32_t(span * (p + 1) / kParts);
+        return level_db(first + lo, first + (hi > lo ? hi - 1 : lo));
+    }
+
+    // reject_residual (:1336-1583): a band whose mean excess has fallen, over
+    // every one of its parts, at the rate an emptying average falls at
+    // (4.34 dB per average time constant, within the tolerance) for
+    // residual_decisions decisions in a row is the trace of a signal that has
+    // stopped, and is withheld.
+    void reject_residual(std::vector<rvx_candidate>& cands, double elapsed) {
+        std::vector<Residual> next;
+        next.reserve(cands.size());
+        size_t kept = 0;
+        for (const rvx_candidate& k : cands) {
+            const Residual* prev = nullptr;
+            uint32_t best = 0;
+            for (const Residual& r : residuals) {
+                if (r.last_bin < k.first_bin || r.first_bin > k.last_bin) continue;
+                const uint32_t ov =
+                    std::min(r.last_bin, k.last_bin) - std::max(r.first_bin, k.first_bin) + 1u;
+                if (ov > best) {
+                    best = ov;
+                    prev = &r;
+                }
+            }
+            Residual now;
+            now.first_bin = now.run_first = k.first_bin;
+            now.last_bin = now.run_last = k.last_bin;
+            bool falling = false;
+            if (prev) {
+                const double carried = level_db(prev->run_first, prev->run_last);
+                if (carried < prev->level_db) {
+                    const double seconds = prev->run_seconds + elapsed;
+                    const double expected = kDbPerNeper * seconds / cfg.average_seconds;
+                    const double fall = prev->run_level_db - carried;
+                    if (uint64_t(prev->run_last - prev->run_first) + 1 < kParts)
+                        ++stats.residual_parts_degenerate;
+                    bool uniform = true;
+                    for (uint32_t p = 0; uniform && p < kParts; ++p)
+                        uniform = prev->part_db[p] - part_db(prev->run_first, prev->run_last, p) >=
+                                  kPartFloor * fall;
+                    const double tol = cfg.residual_rate_tolerance;
+                    if (uniform && fall >= (1.0 - tol) * expected && fall <= (1.0 + tol) * expected) {
+                        falling = true;
+                        now.run_first = prev->run_first;
+                        now.run_last = prev->run_last;
+                        now.level_db = carried;
+                        now.run_level_db = prev->run_level_db;
+                        now.run_seconds = seconds;
+                        now.run = prev->run + 1;
+                        std::copy(prev->part_db, prev->part_db + kParts, now.part_db);
+                    }
+                }
+            }
+            if (!falling) {
+                now.level_db = now.run_level_db = level_db(now.run_first, now.run_last);
+                for (uint32_t p = 0; p < kParts; ++p)
+                    now.part_db[p] = part_db(now.run_first, now.run_last, p);
+            }
+            next.push_back(now);
+            if (cfg.residual_decisions > 0 && now.run >= cfg.residual_decisions) {
+                ++stats.candidates_residual;
+                continue;
+            }
+            cands[kept++] = k;
+        }
+        cands.resize(kept);
+        residuals.swap(next);
+    }
+
+    static void band_of(int64_t centre, int64_t width, double& lo, double& hi) {
+        const double half = 0.5 * double(std::max<int64_t>(1, width));
+        lo = double(centre) - half;
+        hi = double(centre) + half;
+    }
+
+    // update_tracks (:1585-1900): gate track-candidate pairs on overlap over
+    // the narrower band, rank them on overlap over the wider, assign one to one
+    // best first; a candidate several born tracks were gated to goes to the
+    // eldest and the others become its merged children; assigned tracks are
+    // smoothed towards their candidate and their confidence rises; unassigned
+    // ones are held with decaying confidence until it falls below the drop
+    // level or the hold runs out; unassigned candidates start pending tracks,
+    // born after birth_hits consecutive hits.
+    void update(const std::vector<rvx_candidate>& cands, uint64_t now, double elapsed) {
+        std::sort(tracks.begin(), tracks.end(), [](const rvx_track& a, const rvx_track& b) {
+            return a.center_hz != b.center_hz ? a.center_hz < b.center_hz : a.id < b.id;
+        });
+        const size_t nt = tracks.size(), nc = cands.size();
+        std::vector<Pair> pairs;
+        for (size_t t = 0; t < nt; ++t) {
+            double tlo, thi;
+            band_of(tracks[t].center_hz, tracks[t].bandwidth_hz, tlo, thi);
+            for (size_t c = 0; c < nc; ++c) {
+                double lo, hi;
+                band_of(cands[c].center_hz, cands[c].bandwidth_hz, lo, hi);
+                if (hi < tlo) continue;
+                if (lo > thi) break;
+                const double ov = std::min(thi, hi) - std::max(tlo, lo);
+                if (ov <= 0.0) continue;
+                if (ov / std::min(thi - tlo, hi - lo) < cfg.association_overlap) continue;
+                pairs.push_back({uint32_t(t), uint32_t(c), ov / std::max(thi - tlo, hi - lo),
+                                 tracks[t].first_seen, tracks[t].id});
+            }
+        }
+        std::sort(pairs.begin(), pairs.end(), [](const Pair& a, const Pair& b) {
+            if (a.score != b.score) return a.score > b.score;
+            if (a.first_seen != b.first_seen) return a.first_seen < b.first_seen;
+            if (a.id != b.id) return a.id < b.id;
+            return a.candidate < b.candidate;
+        });
+        std::vector<uint32_t> cand_of(nt, kUnassigned), track_of(nc, kUnassigned);
+        for (const Pair& p : pairs) {
+            if (cand_of[p.track] != kUnassigned || track_of[p.candidate] != kUnassigned) continue;
+            cand_of[p.track] = p.candidate;
+            track_of[p.candidate] = p.track;
+        }
+        merge(pairs, cand_of, track_of, now);
+
+        const double decay = std::exp2(-elapsed / cfg.confidence_half_life_seconds);
+        const double hold = cfg.bootstrap_hold_seconds * double(cfg.source_rate);
+        std::vector<rvx_track> kept;
+        kept.reserve(nt + nc);
+        for (size_t t = 0; t < nt; ++t) {
+            rvx_track tr = tracks[t];
+            if (cand_of[t] != kUnassigned) {
+                follow(tr, cands[cand_of[t]], now);
+                kept.push_back(tr);
+                continue;
+            }
+            if (tr.state == RVX_TRACK_PENDING) continue;  // birth needs consecutive hits
+            if (tr.state != RVX_TRACK_MERGED) {
+                if (tr.state == RVX_TRACK_LIVE) ++tr.held_spells;
+                ++tr.misses;
+                tr.state = RVX_TRACK_HELD;
+                tr.confidence *= decay;
+            }
+            tr.last_seen = now;
+            if (tr.confidence < cfg.drop_confidence || double(now - tr.last_detected) > hold) {
+                ++stats.tracks_dropped;
+                continue;
+            }
+            kept.push_back(tr);
+        }
+        for (size_t c = 0; c < nc; ++c) {
+            if (track_of[c] != kUnassigned) continue;
+            if (kept.size() >= cfg.max_tracks) {
+                ++stats.births_refused;
+                continue;
+            }
+            const rvx_candidate& k = cands[c];
+            rvx_track tr{};
+            tr.id = next_id++;
+            tr.state = RVX_TRACK_PENDING;
+            tr.center_hz = k.center_hz;
+            tr.bandwidth_hz = std::max<int64_t>(1, k.bandwidth_hz);
+            tr.snr_2500_db = k.snr_2500_db;
+            tr.first_bin = k.first_bin;
+            tr.last_bin = k.last_bin;
+            tr.confidence = cfg.confidence_rise;
+            tr.first_seen = tr.last_seen = tr.last_detected = now;
+            tr.hits = 1;
+            if (tr.hits >= cfg.birth_hits) {
+                tr.state = RVX_TRACK_LIVE;
+                ++stats.tracks_born;
+            }
+            kept.push_back(tr);
+        }
+        tracks.swap(kept);
+    }
+
+    // A candidate several born tracks were gated to belongs to the eldest of
+    // them (first seen, then id); the rest become its children (:1688-1756).
+    void merge(const std::vector<Pair>& pairs, std::vector<uint32_t>& cand_of,
+               std::vector<uint32_t>& track_of, uint64_t now) {
+        std::vector<uint32_t> gated;
+        for (size_t c = 0; c < track_of.size(); ++c) {
+            const uint32_t owner = track_of[c];
+            if (owner == kUnassigned) continue;
+            gated.clear();
+            if (tracks[owner].state != RVX_TRACK_PENDING) gated.push_back(owner);
+            for (const Pair& p : pairs) {
+                if (p.candidate != c || p.track == owner || cand_of[p.track] != kUnassigned ||
+                    tracks[p.track].state == RVX_TRACK_PENDING)
+                    continue;
+                gated.push_back(p.track);
+            }
+            if (gated.empty() || (gated.size() == 1 && gated[0] == owner)) continue;
+            uint32_t eldest = gated[0];
+            for (uint32_t t : gated) {
+                const rvx_track &a = tracks[t], &b = tracks[eldest];
+                if (a.first_seen < b.first_seen || (a.first_seen == b.first_seen && a.id < b.id))
+                    eldest = t;
+            }
+            if (eldest != owner) {
+                cand_of[owner] = kUnassigned;
+                cand_of[eldest] = uint32_t(c);
+                track_of[c] = eldest;
+            }
+            const uint64_t parent = tracks[eldest].id;
+            for (uint32_t t : gated) {
+                if (t == eldest) continue;
+                rvx_track& child = tracks[t];
+                if (child.state != RVX_TRACK_MERGED || child.merged_into != parent) ++stats.merges;
+                child.state = RVX_TRACK_MERGED;
+                child.merged_into = parent;
+                child.last_seen = now;
+            }
+        }
+    }
+
+    void follow(rvx_track& tr, const rvx_candidate& k, uint64_t now) {
+        if (tr.state == RVX_TRACK_MERGED) ++stats.splits;
+        tr.merged_into = 0;
+        const double a = tr.state == RVX_TRACK_PENDING ? 1.0 : cfg.centre_smoothing;
+        const double centre = double(tr.center_hz) + a * double(k.center_hz - tr.center_hz);
+        const double width = double(tr.bandwidth_hz) + a * double(k.bandwidth_hz - tr.bandwidth_hz);
+        tr.center_hz = int64_t(std::llround(centre));
+        tr.bandwidth_hz = std::max<int64_t>(1, int64_t(std::llround(width)));
+        tr.snr_2500_db += a * (k.snr_2500_db - tr.snr_2500_db);
+        tr.first_bin = k.first_bin;
+        tr.last_bin = k.last_bin;
+        ++tr.hits;
+        tr.misses = 0;
+        tr.last_detected = tr.last_seen = now;
+        tr.confidence += (1.0 - tr.confidence) * cfg.confidence_rise;
+        if (tr.state == RVX_TRACK_PENDING) {
+            if (tr.hits >= cfg.birth_hits) {
+                tr.state = RVX_TRACK_LIVE;
+                ++stats.tracks_born;
+            }
+        } else {
+            tr.state = RVX_TRACK_LIVE;
+        }
+    }
+};
+
+extern "C" {
+
+void rvx_event_config_defaults(rvx_event_config* c) {
+    if (!c) return;
+    *c = rvx_event_config{};
+    c->detection_threshold_db = 6.0;       // detector.h:600
+    c->occupied_power_fraction = 0.99;     // :857
+    c->split_gap_bins = 8;                 // :909
+    c->split_gap_db = 1.0;                 // :910
+    c->birth_hits = 3;                     // :918
+    c->max_tracks = 512;                   // :938
+    c->bootstrap_hold_seconds = 3.0;       // :955
+    c->average_seconds = 1.0;              // :643
+    c->residual_rate_tolerance = 0.25;     // :1183
+    c->residual_decisions = 4;             // :1184
+    c->association_overlap = 0.3;          // :1188
+    c->confidence_rise = 0.35;             // :1194
+    c->confidence_half_life_seconds = 1.0; // :1195
+    c->drop_confidence = 0.05;             // :1196
+    c->centre_smoothing = 0.4;             // :1200
+}
+
+int32_t rvx_candidates(const rvx_event_config* config, const double* average, const double* floor,
+                       uint32_t bins, const rvx_detect_band* bands, uint32_t n_bands,
+                       rvx_candidate* out, uint32_t max_out, uint32_t* n_out) {
+    static const char* who = "rvx_candidates";
+    if (!config || !average || !floor || !n_out || (n_bands && !bands) || (max_out && !out))
+        return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    if (int32_t rc = check_config(*config, who)) return rc;
+    for (uint32_t i = 0; i < n_bands; ++i) {
+        if (bands[i].width == 0 || uint64_t(bands[i].start) + bands[i].width > bins)
+            return rvx::fail(RVX_EINVAL, "%s: band %u [%u, +%u) outside %u bins", who, i,
+                             bands[i].start, bands[i].width, bins);
+        if (i && bands[i].start < bands[i - 1].start + bands[i - 1].width)
+            return rvx::fail(RVX_EINVAL, "%s: bands must be disjoint and ascending", who);
+    }
+    try {
+        Sums s;
+        s.build(average, floor, bins);
+        std::vector<rvx_candidate> got;
+        std::vector<uint8_t> gap;
+        for (uint32_t i = 0; i < n_bands; ++i)
+            band_candidates(*config, s, average, floor, bins, bands[i].start,
+                            size_t(bands[i].start) + bands[i].width, gap, got);
+        *n_out = uint32_t(got.size());
+        if (got.size() > max_out)
+            return rvx::fail(RVX_EINVAL, "%s: %zu candidates, room for %u", who, got.size(),
+                             max_out);
+        std::copy(got.begin(), got.end(), out);
+    } catch (const std::bad_alloc&) {
+        return rvx::fail(RVX_ENOMEM, "%s: out of host memory", who);
+    }
+    return RVX_OK;
+}
+
+int32_t rvx_tracker_create(const rvx_event_config* config, uint32_t bins, rvx_tracker** out) {
+    static const char* who = "rvx_tracker_create";
+    if (!config || !out) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    *out = nullptr;
+    if (bins == 0) return rvx::fail(RVX_EINVAL, "%s: zero bins", who);
+    if (int32_t rc = check_config(*config, who)) return rc;
+    auto* t = new (std::nothrow) rvx_tracker;
+    if (!t) return rvx::fail(RVX_ENOMEM, "%s: out of host memory", who);
+    t->cfg = *config;
+    t->bins = bins;
+    *out = t;
+    return RVX_OK;
+}
+
+void rvx_tracker_destroy(rvx_tracker* tracker) { delete tracker; }
+
+int32_t rvx_tracker_update(rvx_tracker* t, const double* average, const double* floor,
+                           const rvx_candidate* candidates, uint32_t n, uint64_t now,
+                           double elapsed_seconds, uint32_t* n_kept) {
+    static const char* who = "rvx_tracker_update";
+    if (!t || !average || !floor || (n && !candidates))
+        return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    if (!(elapsed_seconds >= 0.0))
+        return rvx::fail(RVX_EINVAL, "%s: elapsed %g s", who, elapsed_seconds);
+    for (uint32_t i = 0; i < n; ++i)
+        if (candidates[i].last_bin >= t->bins || candidates[i].first_bin > candidates[i].last_bin)
+            return rvx::fail(RVX_EINVAL, "%s: candidate %u's bins are outside the spectrum", who,
+                             i);
+    try {
+        t->sums.build(average, floor, t->bins);
+        std::vector<rvx_candidate> cands(candidates, candidates + n);
+        t->reject_residual(cands, elapsed_seconds);
+        t->update(cands, now, elapsed_seconds);
+        if (n_kept) *n_kept = uint32_t(cands.size());
+        ++t->stats.decisions;
+        t->stats.candidates += cands.size();
+    } catch (const std::bad_alloc&) {
+        return rvx::fail(RVX_ENOMEM, "%s: out of host memory", who);
+    }
+    return RVX_OK;
+}
+
+int32_t rvx_tracker_tracks(const rvx_tracker* t, rvx_track* out, uint32_t max_out, uint32_t* n) {
+    if (!t || !n || (max_out && !out)) return rvx::fail(RVX_EINVAL, "rvx_tracker_tracks: null argument");
+    *n = uint32_t(t->tracks.size());
+    if (t->tracks.size() > max_out)
+        return rvx::fail(RVX_EINVAL, "rvx_tracker_tracks: %u tracks, room for %u", *n, max_out);
+    std::copy(t->tracks.begin(), t->tracks.end(), out);
+    return RVX_OK;
+}
+
+int32_t rvx_tracker_stats_get(const rvx_tracker* t, rvx_tracker_stats* out) {
+    if (!t || !out) return rvx::fail(RVX_EINVAL, "rvx_tracker_stats_get: null argument");
+    *out = t->stats;
+    return RVX_OK;
+}
+
+}  // extern "C"
