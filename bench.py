@@ -337,6 +337,10 @@ def main() -> int:
                 learned["speedup_vs_cpu_twin"] = round(
                     learned["ms_plain_cpu_twin_per_block"] / learned["ms_learned_per_block"], 1)
 
+    log("timing the plain detector's decision at 65,536 bins")
+    with torch.cuda.stream(side):
+        decision = plain_decision(dev, torch, cpu=not args.no_cpu)
+
     conditioning = None
     if not args.no_conditioning:
         log("timing the ring-conditioning stages and the carrier loops")
@@ -401,6 +405,7 @@ def main() -> int:
             "precision_vs_fp64": precision,
         },
         "learned_detector": learned,
+        "plain_decision": decision,
         "conditioning": conditioning,
         "cpu_baseline": cpu,
         "parity": parity,
@@ -815,6 +820,53 @@ def spot_check(torch, packed, g: Grid, partial, frames, alpha, batch, starts, co
         n_rx += 1
     return what + (f"; audio bit-exact vs oracle for {n_rx} receiver kinds" if rx_ok else
                    "; audio MISMATCH vs oracle")
+
+
+def plain_decision(dev, torch, cpu: bool, bins: int = 65536, reps: int = 40) -> dict:
+    """One plain-detector decision (noise floor, ladder, greedy growth) over
+    an averaged spectrum of the reference's shipped size, 65,536 bins: the
+    device decision (rvx_detect, median of `reps` calls, each waiting for its
+    result) beside the CPU twin (the oracle, one thread) on the same input,
+    and the bands of both compared."""
+    from revenant_amd.detect import Detector, config
+
+    g = np.random.default_rng(SEED)
+    avg = g.gamma(64.0, 1.0 / 64.0, bins) * 1e-6  # a 64-look average near 1e-6
+    for _ in range(bins // 4000):
+        w = int(g.integers(1, bins // 30))
+        s0 = int(g.integers(0, bins - w))
+        avg[s0:s0 + w] += avg[s0:s0 + w].mean() * float(g.uniform(0.05, 30.0))
+    bin_hz = RATE / M / 16
+    det = Detector(dev, config(bins, bin_hz))
+    x = torch.from_numpy(avg).cuda()
+    for _ in range(5):
+        det.decide(x)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        bands, peaks = det.decide(x)
+        times.append((time.perf_counter() - t0) * 1e3)
+    det.close()
+    out = {"bins": bins, "bin_hz": bin_hz, "peaks": peaks, "bands": len(bands),
+           "ms_gpu_decision": round(float(np.median(times)), 4),
+           "ms_reference_published": 7.4}
+    if cpu:
+        try:
+            from oracle import oracle
+        except Exception as exc:
+            out["cpu_twin"] = f"oracle unavailable: {exc}"
+            return out
+        cfg = oracle.detect_config(bins, bin_hz)
+        ct = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            ob, _, _, _ = oracle.detect(cfg, avg)
+            ct.append((time.perf_counter() - t0) * 1e3)
+        out["ms_cpu_twin"] = round(float(np.median(ct)), 3)
+        out["cpu_twin_cores"] = 1
+        out["bands_match_cpu_twin"] = [b[:2] for b in bands] == [b[:2] for b in ob]
+        out["speedup_vs_cpu_twin"] = round(out["ms_cpu_twin"] / out["ms_gpu_decision"], 1)
+    return out
 
 
 def plain_twin_per_block(dev, proposals) -> dict:

@@ -1124,10 +1124,13 @@ typedef struct rvx_detector rvx_detector;
 
 int32_t rvx_detector_create(rvx_ctx* ctx, const rvx_detect_config* config, rvx_detector** out);
 void rvx_detector_destroy(rvx_detector* detector);
-/* average: device, bins doubles. allow_prefix (device, bins + 1 int32,
- * optional): the running count of bins a proposal allows; a ladder window
- * is searched only where it holds an allowed bin (null: everywhere). Waits for
- * the result. *n_peaks: peaks the ladder found before the budget. */
+/* average: device, bins doubles, an averaged linear power spectrum (values
+ * >= 0: the percentile select orders the doubles' bit patterns). allow_prefix
+ * (device, bins + 1 int32, optional): the running count of bins a proposal
+ * allows; a ladder window is searched only where it holds an allowed bin
+ * (null: everywhere). Waits for the result; the three running sums and the
+ * greedy growth run on the calling thread. *n_peaks: peaks the ladder found
+ * before the budget. */
 int32_t rvx_detect(rvx_detector* detector, const double* average, const int32_t* allow_prefix,
                    rvx_detect_band* bands, uint32_t max_bands, uint32_t* n_bands,
                    uint32_t* n_peaks);

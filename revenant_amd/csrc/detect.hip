@@ -4,18 +4,27 @@
 //   knot_floor_kernel  one workgroup per knot: the two percentiles by radix
 //                      select on the doubles' bit patterns (non-negative
 //                      doubles order as unsigned integers, so rank r of the
-//                      window is exact, as nth_element's), then the ascending
-//                      excision sum by one wave, each value added in index
-//                      order as the reference's loop adds it;
-//   floor_interp_kernel per bin, the knots' linear interpolation;
-//   prefix_kernel      the three running sums (excess, floor, spread) in the
-//                      reference's sequential order, one wave per sum;
+//                      window is exact, as nth_element's), then the excision
+//                      sum: the workgroup compacts the kept values into LDS
+//                      in index order and one lane adds them one at a time,
+//                      as the reference's loop does;
+//   floor_interp_kernel per bin, the knots' linear interpolation, and the
+//                      excess avg - floor;
+//   (host)             the three running sums (excess, floor, spread) in the
+//                      reference's sequential order: each is a chain of
+//                      65,536 dependent double additions at the shipped
+//                      grid, about 15 ns apiece on one GPU lane against about
+//                      1 ns on a host core, so the host carries them (the
+//                      three chains interleaved) and sends two back;
 //   ladder_kernel      every (rung, position): the deflection, the threshold,
-//                      the local-maximum and plateau-middle rules; peaks are
-//                      appended to a list whose order does not matter, since
-//                      the host ranks them by the reference's total order.
-// The host keeps the strongest max_peaks (the best-N set the reference's heap
-// keeps) and grows them strongest first, exactly as find_candidates does.
+//                      the local-maximum and plateau-middle rules; each peak
+//                      is appended as one 64-bit key whose ascending order is
+//                      the reference's total order (stronger first, then the
+//                      lower start, then the narrower width);
+//   a radix sort of the keys (rocPRIM) puts the peaks in that order.
+// The host takes the first max_peaks (the best-N set the reference's heap
+// keeps) and grows them strongest first, exactly as find_candidates does,
+// with a bitmap of the accepted bins for the overlap test.
 //
 // Doubles use only +, -, *, / and comparisons on the device (IEEE, correctly
 // rounded); square roots, exp2 and the normal quantile are computed on the
@@ -27,6 +36,8 @@
 #include <bit>
 #include <cmath>
 #include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "common.h"
 
@@ -72,6 +83,13 @@ double normal_quantile(double p) {  // Acklam, detector.cpp:46-70
 
 __device__ __forceinline__ double dmax(double a, double b) { return a < b ? b : a; }
 
+// Number of set bits of `m` below this lane.
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
+constexpr uint32_t kKeepStage = 1024;  // window values per compaction stage (4 per thread)
+
 __global__ void __launch_bounds__(256)
 knot_floor_kernel(const double* __restrict__ avg, const KnotWindow* __restrict__ knots,
                   double quantile_span, double high_q, double excision_sigma,
@@ -79,7 +97,9 @@ knot_floor_kernel(const double* __restrict__ avg, const KnotWindow* __restrict__
     __shared__ uint32_t hist[2][256];
     __shared__ unsigned long long prefix[2];
     __shared__ uint32_t rank[2];
-    const uint32_t tid = threadIdx.x;
+    __shared__ uint32_t wave_kept[4][4];
+    __shared__ double kept_values[kKeepStage + 16];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const KnotWindow kw = knots[blockIdx.x];
     const uint32_t count = kw.high - kw.low;
     const double* w = avg + kw.low;
@@ -93,31 +113,49 @@ knot_floor_kernel(const double* __restrict__ avg, const KnotWindow* __restrict__
         for (uint32_t i = tid; i < 512; i += 256) hist[i >> 8][i & 255] = 0;
         __syncthreads();
         const unsigned long long p0 = prefix[0], p1 = prefix[1];
-        for (uint32_t i = tid; i < count; i += 256) {
-            const unsigned long long v = (unsigned long long)__double_as_longlong(w[i]);
-            const uint32_t bucket = uint32_t(v >> shift) & 255u;
-            if ((v & mask) == p0) atomicAdd(&hist[0][bucket], 1u);
-            if ((v & mask) == p1) atomicAdd(&hist[1][bucket], 1u);
+        for (uint32_t i0 = tid; i0 < count; i0 += 1024) {
+            unsigned long long v[4];
+            for (uint32_t q = 0; q < 4; ++q)  // four loads in flight, clamped in bounds
+                v[q] = (unsigned long long)__double_as_longlong(w[min(i0 + 256 * q, count - 1)]);
+            for (uint32_t q = 0; q < 4; ++q) {
+                if (i0 + 256 * q >= count) break;
+                const uint32_t bucket = uint32_t(v[q] >> shift) & 255u;
+                if ((v[q] & mask) == p0) atomicAdd(&hist[0][bucket], 1u);
+                if ((v[q] & mask) == p1) atomicAdd(&hist[1][bucket], 1u);
+            }
         }
         __syncthreads();
-        if (tid < 2) {
-            uint32_t below = 0, sel = 255;
-            const uint32_t r = rank[tid];
-            for (uint32_t bucket = 0; bucket < 256; ++bucket) {
-                const uint32_t h = hist[tid][bucket];
-                if (below + h > r) {
-                    sel = bucket;
-                    break;
-                }
-                below += h;
+        // Waves 0 and 1 pick the bucket holding rank r of their histogram:
+        // lane l owns buckets 4l..4l+3, a wave scan gives each lane the count
+        // up to its last bucket, and the first lane past r resolves the rest.
+        if (wave < 2) {
+            const uint32_t* h = hist[wave];
+            const uint32_t c[4] = {h[4 * lane], h[4 * lane + 1], h[4 * lane + 2], h[4 * lane + 3]};
+            const uint32_t own = c[0] + c[1] + c[2] + c[3];
+            uint32_t upto = own;
+            for (uint32_t d = 1; d < 64; d <<= 1) {
+                const uint32_t t = __shfl_up(upto, d);
+                if (lane >= d) upto += t;
             }
-            rank[tid] = r - below;
-            prefix[tid] |= (unsigned long long)sel << shift;
+            const uint32_t r = rank[wave];
+            const unsigned long long past = __ballot(upto > r);
+            const uint32_t pick = past ? uint32_t(__ffsll((long long)past) - 1) : 63u;
+            if (lane == pick) {
+                uint32_t below = upto - own, sel = 4 * lane + 3;
+                for (uint32_t k = 0; k < 4; ++k) {
+                    if (below + c[k] > r) {
+                        sel = 4 * lane + k;
+                        break;
+                    }
+                    below += c[k];
+                }
+                rank[wave] = r - below;
+                prefix[wave] |= (unsigned long long)sel << shift;
+            }
         }
         mask |= 255ull << shift;
         __syncthreads();
     }
-    if (tid >= 64) return;  // one wave sums
     const double lp = __longlong_as_double((long long)prefix[0]);
     const double hp = kw.high_rank > kw.low_rank ? __longlong_as_double((long long)prefix[1]) : lp;
     const double deviation = dmax(0.0, (hp - lp) / quantile_span);
@@ -125,18 +163,50 @@ knot_floor_kernel(const double* __restrict__ avg, const KnotWindow* __restrict__
     const double level = hp - spread;
     const double reach = excision_sigma * deviation;
     const double ceiling = level + reach;
+
+    // The excision mean: the values at or below the ceiling, added in index
+    // order. Each stage of 1024 values is compacted into LDS (element
+    // base + 256 g + tid lands after every kept element of lower index), then
+    // thread 0 adds them one at a time while the next stage loads.
     double sum = 0.0;
     uint32_t kept = 0;
-    for (uint32_t base = 0; base < count; base += 64) {
-        const uint32_t i = base + tid;
-        const double v = i < count ? w[i] : 0.0;
-        unsigned long long in = __ballot(i < count && v <= ceiling);
-        kept += __popcll(in);
-        while (in) {  // ascending index order, as the reference's loop
-            const int j = __ffsll((long long)in) - 1;
-            in &= in - 1;
-            sum = sum + __shfl(v, j);
+    double v[4];
+    for (uint32_t g = 0; g < 4; ++g) v[g] = w[min(g * 256 + tid, count - 1)];
+    for (uint32_t base = 0; base < count; base += kKeepStage) {
+        bool keep[4];
+        uint32_t below[4];
+        for (uint32_t g = 0; g < 4; ++g) {
+            keep[g] = base + g * 256 + tid < count && v[g] <= ceiling;
+            const unsigned long long m = __ballot(keep[g]);
+            below[g] = lanes_below(m);
+            if (lane == 0) wave_kept[g][wave] = uint32_t(__popcll(m));
         }
+        __syncthreads();
+        uint32_t run = 0, offset[4] = {0, 0, 0, 0};
+        for (uint32_t g = 0; g < 4; ++g)
+            for (uint32_t wv = 0; wv < 4; ++wv) {
+                if (wv == wave) offset[g] = run;
+                run += wave_kept[g][wv];
+            }
+        const uint32_t fill = run;
+        for (uint32_t g = 0; g < 4; ++g)
+            if (keep[g]) kept_values[offset[g] + below[g]] = v[g];
+        __syncthreads();
+        for (uint32_t g = 0; g < 4; ++g) v[g] = w[min(base + kKeepStage + g * 256 + tid, count - 1)];
+        if (tid == 0) {
+            // kept_values holds kKeepStage + 16 entries, so the read ahead stays in bounds
+            double cur[16], next[16];
+            for (uint32_t q = 0; q < 16; ++q) cur[q] = kept_values[q];
+            uint32_t k = 0;
+            for (; k + 16 <= fill; k += 16) {
+                for (uint32_t q = 0; q < 16; ++q) next[q] = kept_values[k + 16 + q];
+                for (uint32_t q = 0; q < 16; ++q) sum = sum + cur[q];
+                for (uint32_t q = 0; q < 16; ++q) cur[q] = next[q];
+            }
+            for (; k < fill; ++k) sum = sum + kept_values[k];
+        }
+        kept += fill;
+        __syncthreads();
     }
     if (tid == 0) {
         const double estimate = kept > 0 ? sum / double(kept) : level;
@@ -147,51 +217,32 @@ knot_floor_kernel(const double* __restrict__ avg, const KnotWindow* __restrict__
 
 __global__ void floor_interp_kernel(const double* __restrict__ pos, const double* __restrict__ kf,
                                     const double* __restrict__ ks, uint32_t knots, uint32_t bins,
-                                    double* __restrict__ floor_, double* __restrict__ sigma_) {
+                                    const double* __restrict__ avg, double* __restrict__ floor_,
+                                    double* __restrict__ sigma_, double* __restrict__ excess) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= bins) return;
+    double f, g;
     if (knots == 1) {
-        floor_[i] = kf[0];
-        sigma_[i] = ks[0];
-        return;
-    }
-    const double position = double(i);
-    uint32_t k = 0;
-    while (k + 2 < knots && pos[k + 1] < position) ++k;
-    const double span = pos[k + 1] - pos[k];
-    double t = 0.0;
-    if (span > 0.0) {
-        t = (position - pos[k]) / span;
-        t = t < 0.0 ? 0.0 : (1.0 < t ? 1.0 : t);
-    }
-    const double df = kf[k + 1] - kf[k], ds = ks[k + 1] - ks[k];
-    const double tf = t * df, ts = t * ds;
-    floor_[i] = kf[k] + tf;
-    sigma_[i] = ks[k] + ts;
-}
-
-// Wave a of three: running sum of excess (a = 0), floor (1) or spread (2),
-// out[0] = 0 and out[i + 1] = out[i] + x[i], one addition at a time.
-__global__ void __launch_bounds__(192)
-prefix_kernel(const double* __restrict__ avg, const double* __restrict__ floor_,
-              const double* __restrict__ sigma_, uint32_t bins, double* __restrict__ ec,
-              double* __restrict__ fc, double* __restrict__ sc) {
-    const uint32_t a = threadIdx.x / 64, lane = threadIdx.x % 64;
-    double* out = a == 0 ? ec : (a == 1 ? fc : sc);
-    if (lane == 0) out[0] = 0.0;
-    double acc = 0.0;
-    for (uint32_t base = 0; base < bins; base += 64) {
-        const uint32_t i = base + lane;
-        double x = 0.0;
-        if (i < bins) x = a == 0 ? avg[i] - floor_[i] : (a == 1 ? floor_[i] : sigma_[i]);
-        double mine = 0.0;
-        const uint32_t n = min(64u, bins - base);
-        for (uint32_t j = 0; j < n; ++j) {
-            acc = acc + __shfl(x, int(j));
-            if (lane == j) mine = acc;
+        f = kf[0];
+        g = ks[0];
+    } else {
+        const double position = double(i);
+        uint32_t k = 0;
+        while (k + 2 < knots && pos[k + 1] < position) ++k;
+        const double span = pos[k + 1] - pos[k];
+        double t = 0.0;
+        if (span > 0.0) {
+            t = (position - pos[k]) / span;
+            t = t < 0.0 ? 0.0 : (1.0 < t ? 1.0 : t);
         }
-        if (i < bins) out[i + 1] = mine;
+        const double df = kf[k + 1] - kf[k], ds = ks[k + 1] - ks[k];
+        const double tf = t * df, ts = t * ds;
+        f = kf[k] + tf;
+        g = ks[k] + ts;
     }
+    floor_[i] = f;
+    sigma_[i] = g;
+    excess[i] = avg[i] - f;
 }
 
 __device__ __forceinline__ float deflection(const double* ec, const double* fc, uint32_t s,
@@ -205,15 +256,29 @@ __device__ __forceinline__ float deflection(const double* ec, const double* fc, 
 
 struct Rung {
     uint32_t width;
-    uint32_t reserved;
+    uint32_t log2_width;
     double root;
     double threshold;
 };
 
+// A peak as one key: the deflection's bits inverted (a peak's deflection is
+// above a positive threshold, and positive floats order as their bits), then
+// the start, then log2 of the width, so ascending keys are stronger first,
+// lower start first, narrower first. bins <= 2^24 leaves the start 27 bits.
+__device__ __host__ inline uint64_t peak_key(float deflection, uint32_t start, uint32_t log2_width) {
+    return (uint64_t(~std::bit_cast<uint32_t>(deflection)) << 32) | (uint64_t(start) << 5) |
+           log2_width;
+}
+
+Peak peak_of(uint64_t key) {
+    const float d = std::bit_cast<float>(~uint32_t(key >> 32));
+    return Peak{uint32_t(key >> 5) & 0x7FFFFFFu, 1u << (key & 31u), double(d)};
+}
+
 __global__ void __launch_bounds__(256)
 ladder_kernel(const double* __restrict__ ec, const double* __restrict__ fc,
               const Rung* __restrict__ rungs, uint32_t bins, const int32_t* __restrict__ allow,
-              Peak* __restrict__ peaks, uint32_t capacity, uint32_t* __restrict__ count) {
+              uint64_t* __restrict__ keys, uint32_t capacity, uint32_t* __restrict__ count) {
     const Rung r = rungs[blockIdx.y];
     const uint32_t w = r.width;
     const uint32_t last = bins - w;
@@ -230,16 +295,10 @@ ladder_kernel(const double* __restrict__ ec, const double* __restrict__ fc,
     uint32_t end = s;
     while (end < last && deflection(ec, fc, end + 1, w, r.root) == here) ++end;
     const uint32_t idx = atomicAdd(count, 1u);
-    if (idx < capacity) peaks[idx] = Peak{s + (end - s) / 2, w, double(here)};
+    if (idx < capacity) keys[idx] = peak_key(here, s + (end - s) / 2, r.log2_width);
 }
 
 __global__ void zero_counter(uint32_t* c) { *c = 0u; }
-
-bool stronger(const Peak& a, const Peak& b) {
-    if (a.deflection != b.deflection) return a.deflection > b.deflection;
-    if (a.start != b.start) return a.start < b.start;
-    return a.width < b.width;
-}
 
 }  // namespace
 
@@ -254,19 +313,64 @@ struct rvx_detector {
     KnotWindow* windows = nullptr;
     double* pos = nullptr;
     double *kf = nullptr, *ks = nullptr, *floor_ = nullptr, *sigma_ = nullptr;
-    double *ec = nullptr, *fc = nullptr, *sc = nullptr;
+    double *ec = nullptr, *fc = nullptr, *excess = nullptr;
     Rung* rungs = nullptr;
-    Peak* peaks = nullptr;
+    uint64_t *keys = nullptr, *sorted = nullptr;
+    void* sort_temp = nullptr;
+    size_t sort_bytes = 0;
     uint32_t* counter = nullptr;
-    std::vector<double> avg_h, floor_h, fc_h, sc_h;
-    std::vector<Peak> peaks_h;
+    // pinned host mirrors
+    uint32_t* found_h = nullptr;
+    double *excess_h = nullptr, *floor_h = nullptr, *sigma_h = nullptr;
+    double *ec_h = nullptr, *fc_h = nullptr, *sc_h = nullptr;
+    uint64_t* keys_h = nullptr;
+    std::vector<uint64_t> taken;        // one bit per accepted bin
+    std::vector<uint64_t> taken_words;  // one bit per nonzero word of taken
+    std::vector<Peak> accepted;
 
     ~rvx_detector() {
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
         for (void* p : {(void*)windows, (void*)pos, (void*)kf, (void*)ks, (void*)floor_,
-                        (void*)sigma_, (void*)ec, (void*)fc, (void*)sc, (void*)rungs,
-                        (void*)peaks, (void*)counter})
+                        (void*)sigma_, (void*)ec, (void*)fc, (void*)excess,
+                        (void*)rungs, (void*)keys, (void*)sorted, sort_temp, (void*)counter})
             if (p) (void)hipFree(p);
+        for (void* p : {(void*)found_h, (void*)excess_h, (void*)floor_h, (void*)sigma_h,
+                        (void*)ec_h, (void*)fc_h, (void*)sc_h, (void*)keys_h})
+            if (p) (void)hipHostFree(p);
+    }
+
+    // Any set bit of `bits` in [lo, hi) (lo < hi). The words strictly
+    // between the two end words are looked up in `summary` (a bit per word of
+    // `bits`, set when that word is nonzero) when there is one, so a wide
+    // window costs a few words, and the narrow case has no data-dependent
+    // branch.
+    static bool any_set(const uint64_t* bits, const uint64_t* summary, size_t lo, size_t hi) {
+        const size_t a = lo >> 6, b = (hi - 1) >> 6;
+        const uint64_t first = ~0ull << (lo & 63), last = ~0ull >> (63 - ((hi - 1) & 63));
+        const uint64_t wa = bits[a] & first, wb = bits[b] & last;
+        uint64_t hit = a == b ? (wa & last) : (wa | wb);
+        if (b > a + 1) {
+            if (summary) return hit != 0 || any_set(summary, nullptr, a + 1, b);
+            for (size_t k = a + 1; k < b; ++k) hit |= bits[k];
+        }
+        return hit != 0;
+    }
+
+    bool overlaps_taken(size_t lo, size_t hi) const {  // any accepted bin in [lo, hi)
+        return any_set(taken.data(), taken_words.data(), lo, hi);
+    }
+
+    void take(size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi;) {
+            if ((i & 63) == 0 && i + 64 <= hi) {
+                taken[i >> 6] = ~0ull;
+                i += 64;
+            } else {
+                taken[i >> 6] |= 1ull << (i & 63);
+                ++i;
+            }
+        }
+        for (size_t w = lo >> 6; w <= (hi - 1) >> 6; ++w) taken_words[w >> 6] |= 1ull << (w & 63);
     }
 };
 
@@ -318,17 +422,21 @@ int32_t rvx_detector_create(rvx_ctx* ctx, const rvx_detect_config* config, rvx_d
     if (widest == 0) widest = uint32_t(std::bit_floor(std::max<size_t>(1, bins / 8)));
     widest = std::min<uint32_t>(widest, uint32_t(bins));
     uint64_t capacity = 0;
-    for (uint32_t w = 1; w <= widest; w *= 2) {
+    for (uint32_t w = 1, lg = 0; w <= widest; w *= 2, ++lg) {
         const double root = std::sqrt(double(w));
         const double threshold = threshold_linear * kReferenceBandwidthHz / (root * c.bin_width_hz);
-        d->rungs_h.push_back(Rung{w, 0, root, threshold});
+        d->rungs_h.push_back(Rung{w, lg, root, threshold});
         capacity += (bins - w + 2) / 2;  // strict local maxima: at most every other position
     }
     d->budget = c.max_peaks ? c.max_peaks : uint32_t(bins);
     d->capacity = uint32_t(std::min<uint64_t>(capacity, 0xFFFFFFFFull));
+    const size_t kept_keys = std::min<size_t>(d->capacity, d->budget);
     hipError_t err = hipSuccess;
     auto alloc = [&](auto** p, size_t bytes) {
         if (err == hipSuccess) err = hipMalloc((void**)p, bytes);
+    };
+    auto pinned = [&](auto** p, size_t bytes) {
+        if (err == hipSuccess) err = hipHostMalloc((void**)p, bytes, hipHostMallocDefault);
     };
     alloc(&d->windows, knots * sizeof(KnotWindow));
     alloc(&d->pos, knots * sizeof(double));
@@ -338,10 +446,23 @@ int32_t rvx_detector_create(rvx_ctx* ctx, const rvx_detect_config* config, rvx_d
     alloc(&d->sigma_, bins * sizeof(double));
     alloc(&d->ec, (bins + 1) * sizeof(double));
     alloc(&d->fc, (bins + 1) * sizeof(double));
-    alloc(&d->sc, (bins + 1) * sizeof(double));
+    alloc(&d->excess, bins * sizeof(double));
     alloc(&d->rungs, d->rungs_h.size() * sizeof(Rung));
-    alloc(&d->peaks, size_t(d->capacity) * sizeof(Peak));
+    alloc(&d->keys, size_t(d->capacity) * sizeof(uint64_t));
+    alloc(&d->sorted, size_t(d->capacity) * sizeof(uint64_t));
     alloc(&d->counter, sizeof(uint32_t));
+    if (err == hipSuccess)
+        err = rocprim::radix_sort_keys(nullptr, d->sort_bytes, d->keys, d->sorted, d->capacity, 0,
+                                       64, ctx->stream);
+    alloc(&d->sort_temp, std::max<size_t>(d->sort_bytes, 16));
+    pinned(&d->found_h, sizeof(uint32_t));
+    pinned(&d->excess_h, bins * sizeof(double));
+    pinned(&d->floor_h, bins * sizeof(double));
+    pinned(&d->sigma_h, bins * sizeof(double));
+    pinned(&d->ec_h, (bins + 1) * sizeof(double));
+    pinned(&d->fc_h, (bins + 1) * sizeof(double));
+    pinned(&d->sc_h, (bins + 1) * sizeof(double));
+    pinned(&d->keys_h, std::max<size_t>(kept_keys, 1) * sizeof(uint64_t));
     if (err == hipSuccess)
         err = hipMemcpy(d->windows, win.data(), knots * sizeof(KnotWindow), hipMemcpyHostToDevice);
     if (err == hipSuccess)
@@ -353,10 +474,8 @@ int32_t rvx_detector_create(rvx_ctx* ctx, const rvx_detect_config* config, rvx_d
         delete d;
         return rvx::fail(RVX_ENOMEM, "%s: %s", who, hipGetErrorString(err));
     }
-    d->avg_h.resize(bins);
-    d->floor_h.resize(bins);
-    d->fc_h.resize(bins + 1);
-    d->sc_h.resize(bins + 1);
+    d->taken.assign((bins + 63) / 64, 0);
+    d->taken_words.assign((d->taken.size() + 63) / 64, 0);
     *out = d;
     return RVX_OK;
 }
@@ -379,65 +498,88 @@ int32_t rvx_detect(rvx_detector* d, const double* average, const int32_t* allow_
     knot_floor_kernel<<<dim3(d->knots), dim3(256), 0, s>>>(
         average, d->windows, d->quantile_span, d->high_q, d->cfg.noise_excision_sigma, d->kf,
         d->ks);
-    floor_interp_kernel<<<dim3((bins + 255) / 256), dim3(256), 0, s>>>(d->pos, d->kf, d->ks,
-                                                                       d->knots, bins, d->floor_,
-                                                                       d->sigma_);
-    prefix_kernel<<<dim3(1), dim3(192), 0, s>>>(average, d->floor_, d->sigma_, bins, d->ec, d->fc,
-                                                d->sc);
-    zero_counter<<<dim3(1), dim3(1), 0, s>>>(d->counter);
-    ladder_kernel<<<dim3((bins + 255) / 256, uint32_t(d->rungs_h.size())), dim3(256), 0, s>>>(
-        d->ec, d->fc, d->rungs, bins, allow_prefix, d->peaks, d->capacity, d->counter);
+    floor_interp_kernel<<<dim3((bins + 255) / 256), dim3(256), 0, s>>>(
+        d->pos, d->kf, d->ks, d->knots, bins, average, d->floor_, d->sigma_, d->excess);
     if (int32_t rc = rvx::check_launch(who)) return rc;
-    uint32_t found = 0;
-    RVX_TRY_HIP(hipMemcpyAsync(&found, d->counter, sizeof(uint32_t), hipMemcpyDeviceToHost, s), who);
-    RVX_TRY_HIP(hipMemcpyAsync(d->avg_h.data(), average, bins * sizeof(double),
+    RVX_TRY_HIP(hipMemcpyAsync(d->excess_h, d->excess, bins * sizeof(double),
                                hipMemcpyDeviceToHost, s),
                 who);
-    RVX_TRY_HIP(hipMemcpyAsync(d->floor_h.data(), d->floor_, bins * sizeof(double),
+    RVX_TRY_HIP(hipMemcpyAsync(d->floor_h, d->floor_, bins * sizeof(double),
                                hipMemcpyDeviceToHost, s),
                 who);
-    RVX_TRY_HIP(hipMemcpyAsync(d->fc_h.data(), d->fc, (bins + 1) * sizeof(double),
-                               hipMemcpyDeviceToHost, s),
-                who);
-    RVX_TRY_HIP(hipMemcpyAsync(d->sc_h.data(), d->sc, (bins + 1) * sizeof(double),
+    RVX_TRY_HIP(hipMemcpyAsync(d->sigma_h, d->sigma_, bins * sizeof(double),
                                hipMemcpyDeviceToHost, s),
                 who);
     RVX_TRY_HIP(hipStreamSynchronize(s), who);
+    {  // the running sums, out[0] = 0 and out[i + 1] = out[i] + x[i]
+        const double *ex = d->excess_h, *fl = d->floor_h, *sg = d->sigma_h;
+        double *ec = d->ec_h, *fc = d->fc_h, *sc = d->sc_h;
+        double e = 0.0, f = 0.0, g = 0.0;
+        ec[0] = fc[0] = sc[0] = 0.0;
+        for (uint32_t i = 0; i < bins; ++i) {
+            e = e + ex[i];
+            f = f + fl[i];
+            g = g + sg[i];
+            ec[i + 1] = e;
+            fc[i + 1] = f;
+            sc[i + 1] = g;
+        }
+    }
+    RVX_TRY_HIP(hipMemcpyAsync(d->ec, d->ec_h, (bins + 1) * sizeof(double), hipMemcpyHostToDevice,
+                               s),
+                who);
+    RVX_TRY_HIP(hipMemcpyAsync(d->fc, d->fc_h, (bins + 1) * sizeof(double), hipMemcpyHostToDevice,
+                               s),
+                who);
+    zero_counter<<<dim3(1), dim3(1), 0, s>>>(d->counter);
+    ladder_kernel<<<dim3((bins + 255) / 256, uint32_t(d->rungs_h.size())), dim3(256), 0, s>>>(
+        d->ec, d->fc, d->rungs, bins, allow_prefix, d->keys, d->capacity, d->counter);
+    if (int32_t rc = rvx::check_launch(who)) return rc;
+    RVX_TRY_HIP(hipMemcpyAsync(d->found_h, d->counter, sizeof(uint32_t), hipMemcpyDeviceToHost, s),
+                who);
+    RVX_TRY_HIP(hipStreamSynchronize(s), who);
+    const uint32_t found = *d->found_h;
     if (found > d->capacity)
         return rvx::fail(RVX_EDEVICE, "%s: %u peaks overflowed a list of %u", who, found,
                          d->capacity);
-    d->peaks_h.resize(found);
-    if (found)
-        RVX_TRY_HIP(hipMemcpy(d->peaks_h.data(), d->peaks, found * sizeof(Peak),
-                              hipMemcpyDeviceToHost),
-                    who);
     if (n_peaks) *n_peaks = found;
-
-    // The best max_peaks by the reference's order, strongest first.
-    auto& peaks = d->peaks_h;
-    if (peaks.size() > d->budget) {
-        std::nth_element(peaks.begin(), peaks.begin() + d->budget, peaks.end(), stronger);
-        peaks.resize(d->budget);
+    // The best max_peaks by the reference's order, strongest first: the
+    // sorted keys' prefix.
+    const uint32_t n = std::min(found, d->budget);
+    if (n) {
+        const uint64_t* ordered = d->keys;
+        if (found > 1) {
+            size_t bytes = d->sort_bytes;
+            RVX_TRY_HIP(rocprim::radix_sort_keys(d->sort_temp, bytes, d->keys, d->sorted, found, 0,
+                                                 64, s),
+                        who);
+            ordered = d->sorted;
+        }
+        RVX_TRY_HIP(hipMemcpyAsync(d->keys_h, ordered, size_t(n) * sizeof(uint64_t),
+                                   hipMemcpyDeviceToHost, s),
+                    who);
+        RVX_TRY_HIP(hipStreamSynchronize(s), who);
     }
-    std::sort(peaks.begin(), peaks.end(), stronger);
 
     // Greedy growth (find_candidates :1060-1120): each winner grows onto its
     // shoulders within its seed width and the accepted neighbours, stepping
     // over gaps shorter than split_gap_bins; a peak overlapping an accepted
     // band is dropped.
-    const double* avg = d->avg_h.data();
-    const double* fl = d->floor_h.data();
-    const double* fcum = d->fc_h.data();
-    const double* scum = d->sc_h.data();
+    const double* ex = d->excess_h;
+    const double* fcum = d->fc_h;
+    const double* scum = d->sc_h;
     const size_t gap = d->cfg.split_gap_bins;
-    std::vector<Peak> accepted;
-    auto above = [&](size_t i, double level) { return avg[i] - fl[i] > level; };
-    for (const Peak& p : peaks) {
+    auto& accepted = d->accepted;
+    accepted.clear();
+    std::fill(d->taken.begin(), d->taken.end(), 0ull);
+    std::fill(d->taken_words.begin(), d->taken_words.end(), 0ull);
+    auto above = [&](size_t i, double level) { return ex[i] > level; };
+    for (uint32_t k = 0; k < n; ++k) {
+        const Peak p = peak_of(d->keys_h[k]);
         size_t lo = p.start, hi = size_t(p.start) + p.width;
+        if (d->overlaps_taken(lo, hi)) continue;
         auto at = std::lower_bound(accepted.begin(), accepted.end(), lo,
                                    [](const Peak& held, size_t v) { return held.start < v; });
-        if (at != accepted.end() && size_t(at->start) < hi) continue;
-        if (at != accepted.begin() && size_t((at - 1)->start) + (at - 1)->width > lo) continue;
         const double seed_floor = fcum[hi] - fcum[lo];
         const double seed_sigma = scum[hi] - scum[lo];
         const double level = std::max(d->cfg.edge_floor_fraction * seed_floor,
@@ -471,6 +613,7 @@ int32_t rvx_detect(rvx_detector* d, const double* average, const int32_t* allow_
             if (ahead >= limit) break;
             hi += ahead + 1;
         }
+        d->take(lo, hi);
         accepted.insert(at, Peak{uint32_t(lo), uint32_t(hi - lo), p.deflection});
     }
     *n_bands = uint32_t(accepted.size());
