@@ -1287,6 +1287,7 @@ typedef struct rvx_engine_stats {
                                 * receiver's audio ring before rvx_engine_read_audio
                                 * copied them (the reader fell more than the ring
                                 * behind) */
+    uint64_t grid_blocks;      /* blocks channelized on the added grids */
 } rvx_engine_stats;
 
 /* The front-end estimate and the correction the newest push's samples took
@@ -1304,6 +1305,42 @@ void rvx_engine_destroy(rvx_engine* engine);
 int32_t rvx_engine_add_vrx(rvx_engine* engine, const rvx_vrx_plan* plan, const float* fine_taps,
                            uint64_t n_fine_floats, const float* weights, uint64_t n_weights,
                            uint32_t audio_capacity, uint32_t tile, uint32_t* id);
+
+/* A second grid on the same input stream, for receivers wider than a channel
+ * of the engine's own. The reference runs one grid and coarsens it until the
+ * widest receiver fits (core/engine/engine.h:160-200, EngineConfig::channels
+ * and carry_receiver_hz, engine::channel_count_for); this engine keeps its
+ * fine grid for the spectrum and the narrow receivers and adds the coarse one
+ * beside it. The grid stores only the channels its receivers sit on: a
+ * channel takes one of max_rows ring rows when its first receiver arrives
+ * (the row is cleared and its stream starts at the grid's next block) and
+ * gives it back when its last receiver leaves. Each push channelizes the
+ * grid's complete blocks (the same chain, front-end correction included)
+ * after the engine's own. *grid: its id for rvx_engine_add_vrx_on (1, 2, ...;
+ * 0 names the engine's own grid). Its history must fit the engine's input
+ * window: M * taps_per_branch - 1 + decimation + max_push samples. */
+typedef struct rvx_engine_grid {
+    uint32_t channels;         /* M, a power of two <= 4096 */
+    uint32_t taps_per_branch;
+    uint32_t decimation;       /* D, dividing M */
+    uint32_t ring_blocks;      /* channel ring depth, a power of two */
+    uint32_t max_rows;         /* channels it stores at once, 1..M */
+} rvx_engine_grid;
+int32_t rvx_engine_add_grid(rvx_engine* engine, const rvx_engine_grid* grid,
+                            const float* prototype, uint64_t n_prototype, uint32_t* grid_id);
+/* rvx_engine_add_vrx on grid `grid` (the plan made with that grid's
+ * parameters: rvx_plan_vrx with its M, D and taps). */
+int32_t rvx_engine_add_vrx_on(rvx_engine* engine, uint32_t grid, const rvx_vrx_plan* plan,
+                              const float* fine_taps, uint64_t n_fine_floats,
+                              const float* weights, uint64_t n_weights, uint32_t audio_capacity,
+                              uint32_t tile, uint32_t* id);
+/* Grid `grid`'s channel ring (device, row-major, max_rows x ring_blocks cf32;
+ * grid 0: the engine's channel ring). */
+const void* rvx_engine_grid_ring(rvx_engine* engine, uint32_t grid);
+/* The ring row holding `channel` on grid `grid` and the first block written
+ * into it (grid 0: row = channel, the engine's first block). */
+int32_t rvx_engine_grid_row(rvx_engine* engine, uint32_t grid, uint32_t channel, uint32_t* row,
+                            uint64_t* first_block);
 int32_t rvx_engine_remove_vrx(rvx_engine* engine, uint32_t id);
 /* A mono receiver's audio noise stages (the reference's per-receiver stage,
  * noise_reference.h:190-520, recorded after its demodulator): the line

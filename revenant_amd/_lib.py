@@ -273,7 +273,12 @@ class EngineConfig(C.Structure):
 
 class EngineStats(C.Structure):
     _fields_ = [(n, u64) for n in ("samples_in", "next_block", "blocks", "spectrum_rows",
-                                   "audio_lost", "audio_overrun")]
+                                   "audio_lost", "audio_overrun", "grid_blocks")]
+
+
+class EngineGrid(C.Structure):
+    _fields_ = [(n, u32) for n in ("channels", "taps_per_branch", "decimation", "ring_blocks",
+                                   "max_rows")]
 
 
 class DetectConfig(C.Structure):
@@ -475,6 +480,11 @@ _SIGNATURES = {
     "rvx_engine_destroy": (None, [vp]),
     "rvx_engine_add_vrx": (i32, [vp, C.POINTER(VrxPlan), fptr, u64, fptr, u64, u32, u32,
                                  C.POINTER(u32)]),
+    "rvx_engine_add_vrx_on": (i32, [vp, u32, C.POINTER(VrxPlan), fptr, u64, fptr, u64, u32, u32,
+                                    C.POINTER(u32)]),
+    "rvx_engine_add_grid": (i32, [vp, C.POINTER(EngineGrid), fptr, u64, C.POINTER(u32)]),
+    "rvx_engine_grid_ring": (vp, [vp, u32]),
+    "rvx_engine_grid_row": (i32, [vp, u32, u32, C.POINTER(u32), C.POINTER(u64)]),
     "rvx_engine_remove_vrx": (i32, [vp, u32]),
     "rvx_engine_push": (i32, [vp, vp, u64, i32]),
     "rvx_engine_read_audio": (i32, [vp, u32, fptr, u64, C.POINTER(u64), C.POINTER(u64)]),

@@ -38,6 +38,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <tuple>
 #include <vector>
 
@@ -182,6 +183,8 @@ namespace {
 
 struct EngineRx {
     uint32_t id = 0;
+    uint32_t grid = 0;  // 0: the engine's own grid; k: the k-th added grid
+    uint32_t row = 0;   // ring row of the receiver's channel on its grid
     rvx_vrx_plan plan{};
     void* taps = nullptr;
     float* weights = nullptr;
@@ -263,6 +266,34 @@ int32_t device_alloc(T** p, uint64_t bytes, const char* who) {
     return RVX_OK;
 }
 
+// A grid added beside the engine's own (rvx_engine_add_grid). It reads the
+// same input window and stores only the channels its receivers sit on: each
+// gets a ring row when its first receiver arrives (the row is cleared and
+// its stream starts at the grid's next block) and gives it back when its
+// last receiver leaves.
+struct EngineGrid {
+    rvx_engine_grid cfg{};
+    uint64_t N = 0;  // prototype length M*L
+    float* taps = nullptr;
+    void* tw = nullptr;
+    bool taps_scaled = false;
+    void* ring = nullptr;          // max_rows x ring_blocks cf32, row-major
+    uint32_t* rows_d = nullptr;    // ring row of each channel (max_rows: not stored)
+    uint32_t* rows_h = nullptr;    // pinned staging of rows_d's uploads
+    std::vector<uint32_t> row_of;  // host copy of rows_d
+    std::vector<uint32_t> users;   // receivers on each row
+    std::vector<uint64_t> row_first;  // first block written into each row
+    bool rows_dirty = false;       // rows_d is behind row_of
+    uint32_t max_blocks = 0;
+    uint64_t next_block = 0;
+
+    ~EngineGrid() {
+        for (void* q : {(void*)taps, tw, ring, (void*)rows_d})
+            if (q) (void)hipFree(q);
+        if (rows_h) (void)hipHostFree(rows_h);
+    }
+};
+
 }  // namespace
 
 struct rvx_engine {
@@ -316,11 +347,21 @@ struct rvx_engine {
     rvx_iq_correct_params corr_before{0, 0, 0, 0.0f, 0.0f, 0.0f, 1.0f};
     rvx_iq_correct_params corr_now{0, 0, 0, 0.0f, 0.0f, 0.0f, 1.0f};
     uint64_t corr_from = 0;           // first sample that takes corr_now
+    std::vector<std::unique_ptr<EngineGrid>> grids;  // grid k is grids[k - 1]
     rvx_engine_stats stats{};
+
+    // The oldest input sample a block still to run on any grid reads.
+    uint64_t oldest_needed() const {
+        uint64_t need = next_block * cfg.decimation - (N - 1);
+        for (const auto& g : grids)
+            need = std::min(need, g->next_block * g->cfg.decimation - (g->N - 1));
+        return need;
+    }
 
     ~rvx_engine() {
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
         for (auto& r : rx) free_rx(r);
+        grids.clear();
         for (void* p : {(void*)taps, tw, nco, (void*)win, (void*)bounce, ring, (void*)partial,
                         (void*)rows, (void*)state, descs_d, work_d, (void*)counter, items_d})
             if (p) (void)hipFree(p);
@@ -344,6 +385,49 @@ struct rvx_engine {
 
 namespace {
 
+// Where a receiver's coarse channel lives: its grid's ring, the first slot of
+// its row, the ring depth, the first block of the row's stream, and the
+// channel samples the ring holds for it (none when newest < oldest).
+struct Source {
+    const void* ring;
+    uint64_t ring_n;  // cf32 entries
+    uint32_t base;
+    uint32_t blocks;
+    uint32_t max_blocks;  // blocks one push can add
+    uint64_t first;
+    uint64_t oldest, newest;
+    bool any;
+};
+
+Source source_of(const rvx_engine& e, const EngineRx& r) {
+    Source src{};
+    uint64_t next = 0;
+    if (r.grid == 0) {
+        src.ring = e.ring;
+        src.blocks = e.cfg.ring_blocks;
+        src.ring_n = uint64_t(e.cfg.channels) * src.blocks;
+        src.base = r.plan.placement.channel * src.blocks;
+        src.max_blocks = e.max_blocks;
+        src.first = e.first_block;
+        src.any = e.any;
+        next = e.next_block;
+    } else {
+        const EngineGrid& g = *e.grids[r.grid - 1];
+        src.ring = g.ring;
+        src.blocks = g.cfg.ring_blocks;
+        src.ring_n = uint64_t(g.cfg.max_rows) * src.blocks;
+        src.base = r.row * src.blocks;
+        src.max_blocks = g.max_blocks;
+        src.first = g.row_first[r.row];
+        src.any = g.next_block > src.first;
+        next = g.next_block;
+    }
+    src.newest = next - 1;
+    src.oldest = std::max(src.first, next >= src.blocks ? next - src.blocks : 0);
+    if (!src.any) src.oldest = src.newest + 1;
+    return src;
+}
+
 // One carrier-loop receiver's share of a push: audio a0 .. a0+count-1 from
 // fine samples first_fine .. the last, after the loop has rotated them.
 struct CarrierJob {
@@ -366,6 +450,7 @@ int32_t carrier_fine(rvx_engine& e, EngineRx& r, uint64_t a0, uint64_t count,
     }
     const uint64_t n_new = last + 1 - r.next_fine;
     const uint32_t mask = uint32_t(r.fine_cap - 1);
+    const Source src = source_of(e, r);
     const uint64_t limit = std::min<uint64_t>(max_fine_outputs(r.plan), 0x7FFFFFFFull);
     const uint64_t n_taps = uint64_t(r.plan.fine.phases) * r.plan.fine.taps + 1;
     for (uint64_t at = r.next_fine; at <= last;) {
@@ -373,12 +458,11 @@ int32_t carrier_fine(rvx_engine& e, EngineRx& r, uint64_t a0, uint64_t count,
         rvx_vrx_fine_params fp{};
         uint64_t first_input = 0;
         const bool b = r.blank_on;
-        if (int32_t rc = rvx_fine_block(&r.plan, b ? 0u : r.plan.placement.channel * e.cfg.ring_blocks,
-                                        e.cfg.ring_blocks - 1, mask, at, n, &fp, &first_input))
+        if (int32_t rc = rvx_fine_block(&r.plan, b ? 0u : src.base, src.blocks - 1, mask, at, n,
+                                        &fp, &first_input))
             return rc;
-        if (int32_t rc = rvx_vrx_fine(e.ctx, &r.plan.fine, &fp, b ? r.blanked : e.ring,
-                                      uint64_t(b ? 1u : e.cfg.channels) * e.cfg.ring_blocks,
-                                      r.taps, n_taps,
+        if (int32_t rc = rvx_vrx_fine(e.ctx, &r.plan.fine, &fp, b ? r.blanked : src.ring,
+                                      b ? uint64_t(src.blocks) : src.ring_n, r.taps, n_taps,
                                       e.nco, e.nco_entries, r.fine, r.fine_cap))
             return rc;
         at += n;
@@ -469,7 +553,7 @@ int32_t front_end_push(rvx_engine& e, uint64_t push_first, uint64_t samples, con
                                  (e.cfg.calibration & RVX_CALIBRATE_IQ) != 0, &want);
         const bool differs = std::memcmp(&want, &e.corr_now, sizeof(want)) != 0;
         // the oldest sample any block still to run reads
-        const uint64_t oldest = e.next_block * e.cfg.decimation - (e.N - 1);
+        const uint64_t oldest = e.oldest_needed();
         if (differs && (!e.corr_on || oldest >= e.corr_from)) {
             e.corr_before = e.corr_now;
             e.corr_now = want;
@@ -513,15 +597,16 @@ int32_t display_stream(rvx_engine& e, EngineRx& r, uint64_t lo, uint64_t hi, con
         r.display_next = r.display_from = (lo + hist + R - 1) / R;
     }
     const bool b = r.blank_on;
-    const void* src = b ? r.blanked : e.ring;
-    const uint32_t base = b ? 0u : r.plan.placement.channel * e.cfg.ring_blocks;
-    const uint64_t src_n = uint64_t(b ? 1u : e.cfg.channels) * e.cfg.ring_blocks;
+    const Source from = source_of(e, r);
+    const void* src = b ? r.blanked : from.ring;
+    const uint32_t base = b ? 0u : from.base;
+    const uint64_t src_n = b ? uint64_t(from.blocks) : from.ring_n;
     const uint64_t n_taps = RVX_DISPLAY_TAPS + 1;
     while (r.display_next <= j_hi) {
         const uint32_t n = uint32_t(std::min<uint64_t>(j_hi + 1 - r.display_next, r.display_cap));
         rvx_vrx_fine_params fp{};
         uint64_t first_input = 0;
-        if (int32_t rc = rvx_display_block(&r.display, base, e.cfg.ring_blocks - 1,
+        if (int32_t rc = rvx_display_block(&r.display, base, from.blocks - 1,
                                            uint32_t(r.display_cap - 1), r.display_next, n, &fp,
                                            &first_input))
             return rc;
@@ -545,28 +630,28 @@ int32_t display_stream(rvx_engine& e, EngineRx& r, uint64_t lo, uint64_t hi, con
 int32_t blank_channel(rvx_engine& e, EngineRx& r, uint64_t oldest, uint64_t newest,
                       const char* who) {
     const rvx_blanker_config& c = r.blank_cfg;
-    const uint32_t mask = e.cfg.ring_blocks - 1;
+    const Source src = source_of(e, r);
+    const uint32_t mask = src.blocks - 1;
     const uint64_t below = uint64_t(c.hang) + c.guard + c.window;
     if (!r.blank_started) {
         r.blank_started = true;
-        const uint64_t start =
-            oldest == e.first_block ? e.first_block : oldest + c.guard + c.window;
+        const uint64_t start = oldest == src.first ? src.first : oldest + c.guard + c.window;
         r.detect_next = start;
         r.apply_next = r.apply_first = start;
     }
     const uint64_t piece_max = uint64_t(mask) + 1 - below - c.lead - 1;
     rvx_blanker_params p{};
-    p.chan_base = r.plan.placement.channel * e.cfg.ring_blocks;
+    p.chan_base = src.base;
     p.chan_mask = mask;
     p.out_mask = mask;
     p.threshold = r.blank_threshold;
-    const uint64_t ring_n = uint64_t(e.cfg.channels) * e.cfg.ring_blocks;
+    const uint64_t ring_n = src.ring_n;
     while (r.detect_next <= newest) {
         const uint64_t n = std::min(newest + 1 - r.detect_next, piece_max);
         p.chan_first = uint32_t(r.detect_next & mask);
         p.out_first = p.chan_first;
         p.count = uint32_t(n);
-        if (int32_t rc = rvx_blank_detect(e.ctx, &c, &p, e.ring, ring_n, r.flags, mask + 1ull))
+        if (int32_t rc = rvx_blank_detect(e.ctx, &c, &p, src.ring, ring_n, r.flags, mask + 1ull))
             return rc;
         r.detect_next += n;
     }
@@ -576,7 +661,7 @@ int32_t blank_channel(rvx_engine& e, EngineRx& r, uint64_t oldest, uint64_t newe
         p.chan_first = uint32_t(r.apply_next & mask);
         p.out_first = p.chan_first;
         p.count = uint32_t(n);
-        if (int32_t rc = rvx_blank_apply(e.ctx, &c, &p, e.ring, ring_n, r.flags, mask + 1ull,
+        if (int32_t rc = rvx_blank_apply(e.ctx, &c, &p, src.ring, ring_n, r.flags, mask + 1ull,
                                          r.blanked, mask + 1ull))
             return rc;
         r.apply_next += n;
@@ -596,14 +681,14 @@ void reader_follows(rvx_engine& e, EngineRx& r) {
 }
 
 int32_t engine_receivers(rvx_engine& e, const char* who) {
-    const uint64_t newest = e.next_block - 1;
-    const uint64_t ring = e.cfg.ring_blocks;
-    const uint64_t oldest = std::max(e.first_block, newest + 1 >= ring ? newest + 1 - ring : 0);
     std::vector<rvx_vrx_span> spans;
     std::vector<CarrierJob> jobs;
     std::vector<rvx_vrx_carrier_item> items;
     for (auto& r : e.rx) r.pushed_n = 0;
     for (auto& r : e.rx) {
+        const Source src = source_of(e, r);
+        if (src.newest + 1 <= src.oldest) continue;  // nothing in its row yet
+        const uint64_t oldest = src.oldest, newest = src.newest;
         uint64_t a_lo = 0, n = 0;
         if (r.display_on) {
             if (r.blank_on) {
@@ -661,9 +746,9 @@ int32_t engine_receivers(rvx_engine& e, const char* who) {
             const uint64_t seg = std::min<uint64_t>({left, r.cap - pos, 0x7FFFFFFFull});
             rvx_vrx_span s{};
             s.plan = &r.plan;
-            s.channel_base = r.blank_on ? 0u : r.plan.placement.channel * e.cfg.ring_blocks;
-            s.channel_mask = e.cfg.ring_blocks - 1;
-            s.channel_ring = r.blank_on ? r.blanked : e.ring;
+            s.channel_base = r.blank_on ? 0u : src.base;
+            s.channel_mask = src.blocks - 1;
+            s.channel_ring = r.blank_on ? r.blanked : src.ring;
             s.taps = r.taps;
             s.weights = r.weights;
             s.audio = r.audio + pos * ch;
@@ -972,18 +1057,34 @@ int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const f
 
 void rvx_engine_destroy(rvx_engine* engine) { delete engine; }
 
-int32_t rvx_engine_add_vrx(rvx_engine* e, const rvx_vrx_plan* plan, const float* fine_taps,
-                           uint64_t n_fine_floats, const float* weights, uint64_t n_weights,
-                           uint32_t audio_capacity, uint32_t tile, uint32_t* id) {
+int32_t rvx_engine_add_vrx_on(rvx_engine* e, uint32_t grid, const rvx_vrx_plan* plan,
+                              const float* fine_taps, uint64_t n_fine_floats, const float* weights,
+                              uint64_t n_weights, uint32_t audio_capacity, uint32_t tile,
+                              uint32_t* id) {
     static const char* who = "rvx_engine_add_vrx";
     if (!e) return rvx::fail(RVX_EINVAL, "%s: null engine", who);
     RVX_ENTER(e->ctx, who);
     if (!plan || !fine_taps || !weights || !id)
         return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    if (grid > e->grids.size())
+        return rvx::fail(RVX_EINVAL, "%s: no grid %u (the engine has %zu)", who, grid,
+                         e->grids.size() + 1);
+    EngineGrid* g = grid ? e->grids[grid - 1].get() : nullptr;
+    const uint32_t M = g ? g->cfg.channels : e->cfg.channels;
     const rvx_vrx_plan& p = *plan;
-    if (p.placement.channel >= e->cfg.channels)
+    if (p.placement.channel >= M)
         return rvx::fail(RVX_EINVAL, "%s: channel %u on a %u-channel grid", who,
-                         p.placement.channel, e->cfg.channels);
+                         p.placement.channel, M);
+    uint32_t row = p.placement.channel;
+    if (g) {  // the channel's row, or a free one
+        row = g->row_of[p.placement.channel];
+        if (row == g->cfg.max_rows) {
+            row = uint32_t(std::find(g->users.begin(), g->users.end(), 0u) - g->users.begin());
+            if (row == g->cfg.max_rows)
+                return rvx::fail(RVX_EINVAL, "%s: grid %u stores %u channels and every one is "
+                                 "taken", who, grid, g->cfg.max_rows);
+        }
+    }
     if (n_fine_floats != 2 * (uint64_t(p.fine.phases) * p.fine.taps + 1))
         return rvx::fail(RVX_EINVAL, "%s: wants %llu fine-tap floats", who,
                          (unsigned long long)(2 * (uint64_t(p.fine.phases) * p.fine.taps + 1)));
@@ -1042,9 +1143,124 @@ int32_t rvx_engine_add_vrx(rvx_engine* e, const rvx_vrx_plan* plan, const float*
         free_rx(r);
         return rc;
     }
+    if (g && g->users[row] == 0) {
+        // A row's stream starts at the grid's next block, on cleared slots,
+        // so a window reaching below it reads zeros as on a new stream.
+        if ((rc = rvx_buf_clear(e->ctx, static_cast<char*>(g->ring) +
+                                            uint64_t(row) * g->cfg.ring_blocks * 8,
+                                uint64_t(g->cfg.ring_blocks) * 8))) {
+            free_rx(r);
+            return rc;
+        }
+        g->row_of[p.placement.channel] = row;
+        g->row_first[row] = g->next_block;
+        g->rows_dirty = true;
+    }
+    if (g) ++g->users[row];
+    r.grid = grid;
+    r.row = row;
     r.id = e->next_id++;
     *id = r.id;
     e->rx.push_back(r);
+    return RVX_OK;
+}
+
+int32_t rvx_engine_add_vrx(rvx_engine* e, const rvx_vrx_plan* plan, const float* fine_taps,
+                           uint64_t n_fine_floats, const float* weights, uint64_t n_weights,
+                           uint32_t audio_capacity, uint32_t tile, uint32_t* id) {
+    return rvx_engine_add_vrx_on(e, 0, plan, fine_taps, n_fine_floats, weights, n_weights,
+                                 audio_capacity, tile, id);
+}
+
+int32_t rvx_engine_add_grid(rvx_engine* e, const rvx_engine_grid* config, const float* prototype,
+                            uint64_t n_prototype, uint32_t* grid) {
+    static const char* who = "rvx_engine_add_grid";
+    if (!e) return rvx::fail(RVX_EINVAL, "%s: null engine", who);
+    RVX_ENTER(e->ctx, who);
+    if (!config || !prototype || !grid) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    const rvx_engine_grid c = *config;
+    const uint32_t M = c.channels, L = c.taps_per_branch, D = c.decimation;
+    if (M < 2 || !rvx::pow2(M) || M > 4096 || L == 0 || D == 0 || M % D != 0)
+        return rvx::fail(RVX_EINVAL, "%s: invalid grid M=%u L=%u D=%u", who, M, L, D);
+    if (n_prototype != uint64_t(M) * L)
+        return rvx::fail(RVX_EINVAL, "%s: wants %llu prototype taps", who,
+                         (unsigned long long)(uint64_t(M) * L));
+    if (!rvx::pow2(c.ring_blocks) || c.max_rows == 0 || c.max_rows > M)
+        return rvx::fail(RVX_EINVAL, "%s: ring_blocks must be a power of two and max_rows in "
+                         "1..%u", who, M);
+    auto g = std::make_unique<EngineGrid>();
+    g->cfg = c;
+    g->N = uint64_t(M) * L;
+    g->max_blocks = uint32_t(std::min<uint64_t>(uint64_t(e->cfg.max_push) / D + 2, 0xFFFFFFFFull));
+    if (g->max_blocks > c.ring_blocks)
+        return rvx::fail(RVX_EINVAL, "%s: a push can complete %u blocks; the ring holds %u", who,
+                         g->max_blocks, c.ring_blocks);
+    // After a push the grid keeps at most N - 1 + D samples of history; they
+    // and the next push must fit the engine's window and its bounce buffer.
+    const uint64_t kept = g->N - 1 + D;
+    const uint64_t bounce = e->N + uint64_t(e->max_blocks + 1) * e->cfg.decimation;
+    if (kept + e->cfg.max_push > e->win_cap || kept > bounce)
+        return rvx::fail(RVX_EINVAL, "%s: the grid keeps %llu samples of history; the engine's "
+                         "window holds %llu with a push of %u", who, (unsigned long long)kept,
+                         (unsigned long long)e->win_cap, e->cfg.max_push);
+    // the first block whose history the window still holds
+    g->next_block = (e->win_origin + g->N - 1 + D - 1) / D;
+    std::vector<float> tw(2 * size_t(M));
+    int32_t rc = rvx_build_twiddles(M, tw.data(), tw.size());
+    std::vector<float> scaled(n_prototype);
+    const float* upload = prototype;
+    if (!rc && e->taps_scaled && rvx_prescale_taps(prototype, n_prototype, scaled.data()) == RVX_OK) {
+        upload = scaled.data();
+        g->taps_scaled = true;
+    }
+    const uint64_t ring_bytes = uint64_t(c.max_rows) * c.ring_blocks * 8;
+    if (!rc) rc = device_alloc(&g->taps, n_prototype * sizeof(float), who);
+    if (!rc) rc = device_alloc(&g->tw, tw.size() * sizeof(float), who);
+    if (!rc) rc = device_alloc(&g->ring, ring_bytes, who);
+    if (!rc) rc = device_alloc(&g->rows_d, M * sizeof(uint32_t), who);
+    if (!rc && hipHostMalloc((void**)&g->rows_h, M * sizeof(uint32_t), hipHostMallocDefault) !=
+                   hipSuccess)
+        rc = rvx::fail(RVX_ENOMEM, "%s: pinned row map", who);
+    if (rc) return rc;
+    g->row_of.assign(M, c.max_rows);
+    g->users.assign(c.max_rows, 0);
+    g->row_first.assign(c.max_rows, 0);
+    hipError_t err = hipMemcpy(g->taps, upload, n_prototype * sizeof(float), hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = hipMemcpy(g->tw, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = hipMemcpy(g->rows_d, g->row_of.data(), M * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemset(g->ring, 0, ring_bytes);
+    if (err != hipSuccess)
+        return rvx::fail(RVX_EDEVICE, "%s: upload failed: %s", who, hipGetErrorString(err));
+    e->grids.push_back(std::move(g));
+    *grid = uint32_t(e->grids.size());
+    return RVX_OK;
+}
+
+const void* rvx_engine_grid_ring(rvx_engine* e, uint32_t grid) {
+    if (!e || grid == 0 || grid > e->grids.size()) return e && grid == 0 ? e->ring : nullptr;
+    return e->grids[grid - 1]->ring;
+}
+
+int32_t rvx_engine_grid_row(rvx_engine* e, uint32_t grid, uint32_t channel, uint32_t* row,
+                            uint64_t* first_block) {
+    static const char* who = "rvx_engine_grid_row";
+    if (!e || !row) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    if (grid == 0) {
+        if (channel >= e->cfg.channels)
+            return rvx::fail(RVX_EINVAL, "%s: channel %u on a %u-channel grid", who, channel,
+                             e->cfg.channels);
+        *row = channel;
+        if (first_block) *first_block = e->first_block;
+        return RVX_OK;
+    }
+    if (grid > e->grids.size()) return rvx::fail(RVX_EINVAL, "%s: no grid %u", who, grid);
+    const EngineGrid& g = *e->grids[grid - 1];
+    if (channel >= g.cfg.channels || g.row_of[channel] == g.cfg.max_rows)
+        return rvx::fail(RVX_EINVAL, "%s: channel %u is not stored on grid %u", who, channel, grid);
+    *row = g.row_of[channel];
+    if (first_block) *first_block = g.row_first[*row];
     return RVX_OK;
 }
 
@@ -1055,6 +1271,13 @@ int32_t rvx_engine_remove_vrx(rvx_engine* e, uint32_t id) {
     auto it = std::find_if(e->rx.begin(), e->rx.end(), [&](const EngineRx& r) { return r.id == id; });
     if (it == e->rx.end()) return rvx::fail(RVX_EINVAL, "%s: no receiver %u", who, id);
     RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
+    if (it->grid) {
+        EngineGrid& g = *e->grids[it->grid - 1];
+        if (--g.users[it->row] == 0) {
+            g.row_of[it->plan.placement.channel] = g.cfg.max_rows;
+            g.rows_dirty = true;
+        }
+    }
     free_rx(*it);
     e->rx.erase(it);
     return RVX_OK;
@@ -1163,9 +1386,10 @@ int32_t rvx_engine_set_blanker(rvx_engine* e, uint32_t id, const rvx_blanker_con
     auto it = std::find_if(e->rx.begin(), e->rx.end(), [&](const EngineRx& r) { return r.id == id; });
     if (it == e->rx.end()) return rvx::fail(RVX_EINVAL, "%s: no receiver %u", who, id);
     EngineRx& r = *it;
+    const Source src = source_of(*e, r);
     if (config) {
         rvx_blanker_params probe{};
-        probe.chan_mask = probe.out_mask = e->cfg.ring_blocks - 1;
+        probe.chan_mask = probe.out_mask = src.blocks - 1;
         probe.count = 1;
         probe.threshold = threshold;
         rvx_blanker_config c = *config;
@@ -1174,10 +1398,10 @@ int32_t rvx_engine_set_blanker(rvx_engine* e, uint32_t id, const rvx_blanker_con
         // a push's new blocks, the window below them and the lead above must
         // all still be in the ring when the blanker reads them
         const uint64_t reach = uint64_t(c.hang) + c.guard + c.window + c.lead;
-        if (uint64_t(e->max_blocks) + reach >= e->cfg.ring_blocks)
+        if (uint64_t(src.max_blocks) + reach >= src.blocks)
             return rvx::fail(RVX_EINVAL, "%s: a push of up to %u blocks plus the blanker's reach "
-                             "of %llu does not fit a %u-block ring", who, e->max_blocks,
-                             (unsigned long long)reach, e->cfg.ring_blocks);
+                             "of %llu does not fit a %u-block ring", who, src.max_blocks,
+                             (unsigned long long)reach, src.blocks);
     }
     RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
     for (void** q : {(void**)&r.flags, &r.blanked})
@@ -1188,7 +1412,7 @@ int32_t rvx_engine_set_blanker(rvx_engine* e, uint32_t id, const rvx_blanker_con
     r.blank_on = false;
     r.blank_started = false;
     if (config) {
-        const uint64_t n = e->cfg.ring_blocks;
+        const uint64_t n = src.blocks;
         if (int32_t rc = device_alloc(&r.flags, n * sizeof(float), who)) return rc;
         if (int32_t rc = device_alloc(&r.blanked, n * 8, who)) return rc;
         RVX_TRY_HIP(hipMemset(r.flags, 0, n * sizeof(float)), who);  // no flags yet
@@ -1257,6 +1481,54 @@ int32_t rvx_engine_set_noise(rvx_engine* e, uint32_t id, const rvx_line_config* 
     return RVX_OK;
 }
 
+static int32_t main_push(rvx_engine* e, uint64_t count, const char* who);
+
+// The complete blocks of an added grid (a push's share), stored into the
+// rows its receivers hold; with no row held the blocks are passed over.
+static int32_t grid_push(rvx_engine& e, EngineGrid& g, uint64_t stop, bool* ran,
+                         const char* who) {
+    const uint32_t D = g.cfg.decimation, M = g.cfg.channels;
+    if (stop == 0 || (stop - 1) / D < g.next_block) return RVX_OK;
+    const uint64_t count = std::min<uint64_t>((stop - 1) / D - g.next_block + 1, g.max_blocks);
+    const bool held = std::any_of(g.users.begin(), g.users.end(), [](uint32_t u) { return u; });
+    if (held) {
+        const hipStream_t s = e.ctx->stream;
+        if (g.rows_dirty) {
+            RVX_TRY_HIP(hipStreamSynchronize(s), who);  // the last upload has been read
+            std::memcpy(g.rows_h, g.row_of.data(), M * sizeof(uint32_t));
+            RVX_TRY_HIP(hipMemcpyAsync(g.rows_d, g.rows_h, M * sizeof(uint32_t),
+                                       hipMemcpyHostToDevice, s),
+                        who);
+            g.rows_dirty = false;
+        }
+        rvx_chain_desc d{};
+        d.format = e.cfg.format;
+        d.channels = M;
+        d.taps_per_branch = g.cfg.taps_per_branch;
+        d.decimation = D;
+        d.input_origin = e.win_origin;
+        d.input_samples = e.win_fill;
+        d.first_block = g.next_block;
+        d.block_count = uint32_t(count);
+        d.out_ring_blocks = g.cfg.ring_blocks;
+        d.ring_rows = g.cfg.max_rows;
+        d.channel_rows = g.rows_d;
+        d.taps_scaled = g.taps_scaled ? 1u : 0u;
+        if (e.corr_on) {
+            d.correction = &e.corr_now;
+            d.correction_before = &e.corr_before;
+            d.correction_from = e.corr_from;
+        }
+        if (int32_t rc = rvx_chain(e.ctx, &d, e.win, g.taps, g.N, g.tw, M, g.ring,
+                                   uint64_t(g.cfg.max_rows) * g.cfg.ring_blocks, nullptr, 0))
+            return rc;
+        e.stats.grid_blocks += count;
+        *ran = true;
+    }
+    g.next_block += count;
+    return RVX_OK;
+}
+
 int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int32_t on_device) {
     static const char* who = "rvx_engine_push";
     if (!e) return rvx::fail(RVX_EINVAL, "%s: null engine", who);
@@ -1272,7 +1544,7 @@ int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int3
     if (e->win_fill + samples > e->win_cap) {
         // Keep the history of the next block; move it to the window's front
         // through the bounce buffer (source and destination may overlap).
-        const uint64_t need_from = e->next_block * D - (e->N - 1);
+        const uint64_t need_from = e->oldest_needed();
         const uint64_t keep_from = std::min(std::max(need_from, e->win_origin), end);
         const uint64_t kept = end - keep_from;
         if (kept) {
@@ -1308,12 +1580,28 @@ int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int3
     }
 
     const uint64_t stop = e->win_origin + e->win_fill;  // one past the last sample
-    if (stop == 0 || (stop - 1) / D < e->next_block) return RVX_OK;
-    uint64_t count = (stop - 1) / D - e->next_block + 1;
-    const uint32_t unit = e->cfg.frame_blocks ? e->cfg.frame_blocks * e->cfg.row_frames : 1;
-    count -= count % unit;
-    count = std::min<uint64_t>(count, e->max_blocks);
-    if (count == 0) return RVX_OK;
+    bool ran = false;
+    uint64_t count = 0;
+    if (stop > 0 && (stop - 1) / D >= e->next_block) {
+        count = (stop - 1) / D - e->next_block + 1;
+        const uint32_t unit = e->cfg.frame_blocks ? e->cfg.frame_blocks * e->cfg.row_frames : 1;
+        count -= count % unit;
+        count = std::min<uint64_t>(count, e->max_blocks);
+    }
+    if (count) {
+        if (int32_t rc = main_push(e, count, who)) return rc;
+        ran = true;
+    }
+    for (auto& g : e->grids)
+        if (int32_t rc = grid_push(*e, *g, stop, &ran, who)) return rc;
+    if (!ran) return RVX_OK;
+    return engine_receivers(*e, who);
+}
+
+// The engine's own grid over `count` complete blocks: the chain, the
+// averaged spectrum's rows, and the block cursor.
+static int32_t main_push(rvx_engine* e, uint64_t count, const char* who) {
+    const uint32_t D = e->cfg.decimation, M = e->cfg.channels;
 
     rvx_chain_desc d{};
     d.format = e->cfg.format;
@@ -1363,7 +1651,7 @@ int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int3
     e->next_block += count;
     e->stats.blocks += count;
     e->stats.next_block = e->next_block;
-    return engine_receivers(*e, who);
+    return RVX_OK;
 }
 
 int32_t rvx_engine_read_audio(rvx_engine* e, uint32_t id, float* out, uint64_t max_values,

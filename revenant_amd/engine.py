@@ -40,16 +40,41 @@ class Engine:
         self.handle = h
         self.channels = {}
 
-    def add(self, plan: VrxPlan, audio_capacity: int, tile: int = 0) -> int:
+    def add(self, plan: VrxPlan, audio_capacity: int, tile: int = 0, grid: int = 0) -> int:
+        """A receiver on grid `grid` (0: the engine's own; else an id from
+        add_grid, with the plan made on that grid)."""
         taps = np.ascontiguousarray(plan.fine_taps.view(np.float32))
         w = np.ascontiguousarray(plan.weights, dtype=np.float32)
         rid = _lib.u32()
-        _lib.check(self.lib.rvx_engine_add_vrx(self.handle, C.byref(plan.plan),
-                                               taps.ctypes.data_as(_lib.fptr), taps.size,
-                                               w.ctypes.data_as(_lib.fptr), w.size,
-                                               audio_capacity, tile, C.byref(rid)))
+        _lib.check(self.lib.rvx_engine_add_vrx_on(self.handle, grid, C.byref(plan.plan),
+                                                  taps.ctypes.data_as(_lib.fptr), taps.size,
+                                                  w.ctypes.data_as(_lib.fptr), w.size,
+                                                  audio_capacity, tile, C.byref(rid)))
         self.channels[rid.value] = plan.demod.channels
         return rid.value
+
+    def add_grid(self, grid: GridParams, prototype: np.ndarray, ring_blocks: int,
+                 max_rows: int) -> int:
+        """A second grid on the same stream (rvx_engine_add_grid) storing up
+        to max_rows of its channels; returns its id."""
+        cfg = _lib.EngineGrid(grid.channels, grid.taps_per_branch, grid.decimation, ring_blocks,
+                              max_rows)
+        taps = np.ascontiguousarray(prototype, dtype=np.float32)
+        gid = _lib.u32()
+        _lib.check(self.lib.rvx_engine_add_grid(self.handle, C.byref(cfg),
+                                                taps.ctypes.data_as(_lib.fptr), taps.size,
+                                                C.byref(gid)))
+        return gid.value
+
+    def grid_ring(self, grid: int) -> int:
+        return self.lib.rvx_engine_grid_ring(self.handle, grid)
+
+    def grid_row(self, grid: int, channel: int) -> tuple[int, int]:
+        """(ring row holding `channel` on `grid`, first block written into it)."""
+        row, first = _lib.u32(), _lib.u64()
+        _lib.check(self.lib.rvx_engine_grid_row(self.handle, grid, channel, C.byref(row),
+                                                C.byref(first)))
+        return row.value, first.value
 
     def remove(self, rid: int) -> None:
         _lib.check(self.lib.rvx_engine_remove_vrx(self.handle, rid))
