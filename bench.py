@@ -186,6 +186,7 @@ def main() -> int:
         launches["ema_combine"] = lambda: dev.ema_combine(ema, partial, state, rows_out)
 
         batch = wide_grid = None
+        tiles = None
         audio_total = 0
         if vrx:
             plans, wide_set = build_receivers(main_grid)
@@ -294,6 +295,12 @@ def main() -> int:
         torch.cuda.synchronize()
         ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
         graph.close()
+
+        engine = None
+        if vrx:
+            log("the same step through the streaming engine (rvx_engine_*)")
+            engine = engine_leg(dev, torch, packed, plans, wide_set, tiles, starts, audio,
+                                args.steps, args.warmup, ms_per_step)
 
     in_samples = main_grid.blocks * D
     value = in_samples / (ms_per_step * 1e-3) / 1e6
@@ -404,6 +411,7 @@ def main() -> int:
                                **{k: round(v, 4) for k, v in mfma_ms.items()}},
             "precision_vs_fp64": precision,
         },
+        "engine_step": engine,
         "learned_detector": learned,
         "plain_decision": decision,
         "conditioning": conditioning,
@@ -820,6 +828,51 @@ def spot_check(torch, packed, g: Grid, partial, frames, alpha, batch, starts, co
         n_rx += 1
     return what + (f"; audio bit-exact vs oracle for {n_rx} receiver kinds" if rx_ok else
                    "; audio MISMATCH vs oracle")
+
+
+def engine_leg(dev, torch, packed, plans, wide_set, tiles, starts, audio, steps: int,
+               warmup: int, graph_ms: float) -> dict:
+    """The step through the streaming engine: one push of the 2^27 resident
+    codes per step into an engine with the 4096-channel grid (averaged
+    spectrum on) and the 512-channel grid added beside it for the 200 kHz
+    receivers, all 256 receivers placed on it (the tuned tiles). The first
+    push covers exactly the blocks and audio the graph step covers, so its
+    audio is compared with the graph step's bit for bit; the pushes after it
+    continue the stream (the same codes again) and are timed, the host's
+    planning and launches included."""
+    from revenant_amd.dsp import FORMAT_CS16, GridParams, design_prototype
+    from revenant_amd.engine import Engine
+
+    main = GridParams(M, TAPS, D)
+    wide = GridParams(WM, TAPS, WD)
+    eng = Engine(dev, main, design_prototype(main, beta=8.0).taps, FORMAT_CS16,
+                 max_push=SAMPLES, ring_blocks=1 << 17, frame_blocks=FRAME,
+                 row_frames=ROW_FRAMES, alpha=ALPHA)
+    gid = eng.add_grid(wide, design_prototype(wide, beta=8.0).taps, ring_blocks=1 << 20,
+                       max_rows=len(wide_set))
+    ids = [eng.add(p, audio_capacity=1 << 17, tile=(tiles[i] if tiles else 0),
+                   grid=gid if name == "wfm" else 0) for i, (name, p) in enumerate(plans)]
+    codes = packed.view(torch.int32)
+    eng.push(codes)
+    same = True
+    for rid, a0, want in zip(ids, starts, audio):
+        first, got = eng.read_audio(rid)
+        ref = want.cpu().numpy()
+        same &= first == a0 and np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+    for _ in range(warmup):
+        eng.push(codes)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.push(codes)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    st = eng.stats()
+    eng.close()
+    return {"ms_per_step": round(ms, 4), "MS_per_s": round(SAMPLES / (ms * 1e-3) / 1e6, 1),
+            "vs_graph_step": round(graph_ms / ms, 3), "pushes": 1 + warmup + steps,
+            "blocks": int(st.blocks), "grid_blocks": int(st.grid_blocks),
+            "first_push_audio_bit_exact_vs_graph_step": bool(same)}
 
 
 def plain_decision(dev, torch, cpu: bool, bins: int = 65536, reps: int = 40) -> dict:
