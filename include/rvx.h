@@ -205,6 +205,9 @@ typedef struct rvx_vrx_carrier_params {
 #define RVX_FORMAT_CS8 1
 #define RVX_FORMAT_CS16 2
 #define RVX_FORMAT_CS24 3
+/* Converted samples, cf32 pairs: the fused chain's input when conditioning
+ * (impulse excision) cleaned them after conversion (rvx_chain_desc.format). */
+#define RVX_FORMAT_CF32 4
 
 /* ------------------------------------------------------------------------ */
 /* Context, stream, buffers (core/gpu/context.h:64-143, buffer.h:33-73)      */
@@ -691,7 +694,7 @@ int32_t rvx_synth(rvx_ctx* ctx, const rvx_carrier* carriers, uint32_t ncarriers,
  * quadrant symmetry (entry k + M/4 is entry k turned by -i, bit for bit) and
  * reads only the entries it needs to form the rest. */
 typedef struct rvx_chain_desc {
-    int32_t format;           /* RVX_FORMAT_CU8 or RVX_FORMAT_CS16 */
+    int32_t format;           /* RVX_FORMAT_CU8, RVX_FORMAT_CS16 or RVX_FORMAT_CF32 */
     uint32_t channels;
     uint32_t taps_per_branch;
     uint32_t decimation;
@@ -1334,6 +1337,20 @@ int32_t rvx_engine_add_vrx_on(rvx_engine* engine, uint32_t grid, const rvx_vrx_p
                               const float* fine_taps, uint64_t n_fine_floats,
                               const float* weights, uint64_t n_weights, uint32_t audio_capacity,
                               uint32_t tile, uint32_t* id);
+/* Impulse excision before the channelizer, as the reference records it
+ * between the converted ring and pfb_branch (core/engine/graph.cpp:1490,
+ * record_impulse_excision): each push is converted (and corrected, with
+ * calibration on, under the correction it arrived with) into a cf32 sample
+ * ring, the cursor's four passes clean it in place, and the cleaned samples
+ * feed every grid's chain (RVX_FORMAT_CF32). Blocks run up to the excision's
+ * cleaned end, which trails the newest sample by max(max_width + lead, one
+ * segment). Set before the first push; config from rvx_design_excision,
+ * threshold from rvx_excision_threshold. */
+int32_t rvx_engine_set_excision(rvx_engine* engine, const rvx_excision_config* config,
+                                float threshold);
+/* The excision's RVX_EXCISE_COUNTERS words (samples excised, events, runs
+ * spared, unused); waits. */
+int32_t rvx_engine_excision_counters(rvx_engine* engine, uint32_t* counters);
 /* Grid `grid`'s channel ring (device, row-major, max_rows x ring_blocks cf32;
  * grid 0: the engine's channel ring). */
 const void* rvx_engine_grid_ring(rvx_engine* engine, uint32_t grid);

@@ -85,17 +85,36 @@ constexpr int kCs16Scaled = 0x102;
 // imbalance, exactly as rvx_iq_correct would correct the converted ring.
 constexpr int kCu8Corrected = 0x200;
 constexpr int kCs16Corrected = 0x202;
+// Samples already converted (cf32 pairs): the engine's conditioned window,
+// cleaned by impulse excision after conversion and correction.
+constexpr int kCf32 = RVX_FORMAT_CF32;
 __host__ __device__ constexpr bool is_cs16(int f) {
     return f == RVX_FORMAT_CS16 || f == kCs16Scaled || f == kCs16Corrected;
 }
+__host__ __device__ constexpr uint32_t bytes_per_sample(int f) {
+    return f == kCf32 ? 8u : (is_cs16(f) ? 4u : 2u);
+}
+// What one sample's load returns: a native code in 32 bits, or both floats.
+template <int FORMAT>
+struct CodeOf {
+    using type = uint32_t;
+};
+template <>
+struct CodeOf<kCf32> {
+    using type = uint64_t;
+};
+template <int FORMAT>
+using code_t = typename CodeOf<FORMAT>::type;
 __host__ __device__ constexpr bool is_corrected(int f) {
     return f == kCu8Corrected || f == kCs16Corrected;
 }
 
 // The native code of one sample (cs16: one word; cu8: two bytes).
 template <int FORMAT>
-__device__ __forceinline__ uint32_t load_code(const void* input, uint32_t index) {
-    if constexpr (is_cs16(FORMAT))
+__device__ __forceinline__ code_t<FORMAT> load_code(const void* input, uint32_t index) {
+    if constexpr (FORMAT == kCf32)
+        return static_cast<const uint64_t*>(input)[index];
+    else if constexpr (is_cs16(FORMAT))
         return static_cast<const uint32_t*>(input)[index];
     else
         return static_cast<const uint16_t*>(input)[index];
@@ -103,8 +122,10 @@ __device__ __forceinline__ uint32_t load_code(const void* input, uint32_t index)
 
 // convert_cs16_cf32.comp / convert_cu8_cf32.comp on one code.
 template <int FORMAT>
-__device__ __forceinline__ float2 convert_code(uint32_t w) {
-    if constexpr (FORMAT == kCs16Corrected) {
+__device__ __forceinline__ float2 convert_code(code_t<FORMAT> w) {
+    if constexpr (FORMAT == kCf32) {
+        return make_float2(__uint_as_float(uint32_t(w)), __uint_as_float(uint32_t(w >> 32)));
+    } else if constexpr (FORMAT == kCs16Corrected) {
         return convert_code<RVX_FORMAT_CS16>(w);
     } else if constexpr (FORMAT == kCu8Corrected) {
         return convert_code<RVX_FORMAT_CU8>(w);
@@ -137,9 +158,12 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, ui
 }
 
 template <int FORMAT>
-__device__ __forceinline__ uint32_t buffer_code(__amdgpu_buffer_rsrc_t r, uint32_t voff,
-                                                uint32_t soff) {
-    if constexpr (is_cs16(FORMAT))
+__device__ __forceinline__ code_t<FORMAT> buffer_code(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                                      uint32_t soff) {
+    if constexpr (FORMAT == kCf32) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        return uint64_t(v[0]) | (uint64_t(v[1]) << 32);
+    } else if constexpr (is_cs16(FORMAT))
         return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
     else
         return __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
@@ -148,6 +172,15 @@ __device__ __forceinline__ uint32_t buffer_code(__amdgpu_buffer_rsrc_t r, uint32
 template <int FORMAT>
 __device__ __forceinline__ float2 load_sample(const void* input, uint32_t index) {
     return convert_code<FORMAT>(load_code<FORMAT>(input, index));
+}
+
+// i' = i - dc_i, then q' = cross * i' + scale * (q - dc_q), in that order.
+__device__ __forceinline__ float2 apply_correction(float2 x, float dc_i, float dc_q, float cross,
+                                                   float scale) {
+    const float i = x.x - dc_i;
+    const float q_dc = x.y - dc_q;
+    const float q = cross * i + scale * q_dc;
+    return make_float2(i, q);
 }
 
 // iq_correct.comp on one converted sample at window offset `index`
@@ -159,14 +192,8 @@ __device__ __forceinline__ float2 correct_sample(float2 x, uint32_t index, const
         return x;
     } else {
         const bool before = index < a.corr_from;
-        const float dc_i = before ? a.corr[0] : a.corr[4];
-        const float dc_q = before ? a.corr[1] : a.corr[5];
-        const float cross = before ? a.corr[2] : a.corr[6];
-        const float scale = before ? a.corr[3] : a.corr[7];
-        const float i = x.x - dc_i;
-        const float q_dc = x.y - dc_q;
-        const float q = cross * i + scale * q_dc;
-        return make_float2(i, q);
+        return apply_correction(x, before ? a.corr[0] : a.corr[4], before ? a.corr[1] : a.corr[5],
+                                before ? a.corr[2] : a.corr[6], before ? a.corr[3] : a.corr[7]);
     }
 }
 
@@ -274,13 +301,13 @@ chain_kernel(ChainArgs a) {
     };
     // Codes of the first PF taps of the next block, loaded while the current
     // block's transform runs.
-    uint32_t raw[PF][V];
+    code_t<FORMAT> raw[PF][V];
     float ctap[REGTAPS ? 1 : PF][V];
     // The chunk's input window starts at the oldest sample its first block
     // reads; a sample's byte offset from there splits into a per-lane part
     // (block base minus branch, minus the oldest tap) and a per-tap uniform
     // part ((TAPS-1-q) rows of M samples).
-    constexpr uint32_t BPS = is_cs16(FORMAT) ? 4u : 2u;
+    constexpr uint32_t BPS = bytes_per_sample(FORMAT);
     constexpr uint32_t NTAPS = TAPS > 0 ? TAPS : 1;
     const uint32_t window0 =
         uint32_t((a.first_block + first) * a.decimation - a.input_origin) - (NTAPS * M - 1);
@@ -326,7 +353,7 @@ chain_kernel(ChainArgs a) {
                 // branches interleave, which no branch's sum can see.
 #pragma unroll
                 for (int q = 0; q < TAPS; ++q) {
-                    uint32_t cur[V];
+                    code_t<FORMAT> cur[V];
                     float tap[V];
 #pragma unroll
                     for (int i = 0; i < V; ++i) {
@@ -512,7 +539,7 @@ chain_pair_kernel(ChainArgs a) {
     constexpr int LOGT = F::LOGT;
     constexpr int LAST = F::PASSES - 1;
     constexpr int NS = C + 2 * TAPS - 1;  // samples per pair per sub-chunk
-    constexpr uint32_t BPS = is_cs16(FORMAT) ? 4u : 2u;
+    constexpr uint32_t BPS = bytes_per_sample(FORMAT);
 
     __shared__ float2 smem[S::BPW * M];
 
@@ -572,7 +599,7 @@ chain_pair_kernel(ChainArgs a) {
         // two segments of loads are in registers beside the sums.
         constexpr int SEG = 8;
         constexpr int NSEG = (NS + SEG - 1) / SEG;
-        uint32_t code[P][NS];
+        code_t<FORMAT> code[P][NS];
         float h_lo[P][TAPS], h_hi[P][TAPS];
         auto load_seg = [&](int i, int sg) {
 #pragma unroll
@@ -777,6 +804,31 @@ int32_t launch_chain(uint32_t logm, const ChainArgs& a, uint32_t taps, hipStream
     return RVX_OK;
 }
 
+// Conditioning: samples [first, first + count) of a code window converted
+// (and corrected, when `correct`) into a cf32 ring at slot (absolute & mask),
+// by the same operations the chain applies to each code it reads.
+template <int FORMAT>
+__global__ void __launch_bounds__(256)
+convert_ring_kernel(const void* __restrict__ codes, uint64_t first_off, uint64_t first_abs,
+                    uint32_t count, float2* __restrict__ ring, uint32_t mask, float4 corr,
+                    int correct) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float2 x = convert_code<FORMAT>(load_code<FORMAT>(codes, uint32_t(first_off + i)));
+    if (correct) x = apply_correction(x, corr.x, corr.y, corr.z, corr.w);
+    ring[uint32_t(first_abs + i) & mask] = x;
+}
+
+// Ring slots of samples [first_abs, first_abs + count) copied to a linear
+// window at window_off.
+__global__ void __launch_bounds__(256)
+ring_to_window_kernel(const float2* __restrict__ ring, uint32_t mask, uint64_t first_abs,
+                      uint32_t count, float2* __restrict__ window, uint64_t window_off) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    window[window_off + i] = ring[uint32_t(first_abs + i) & mask];
+}
+
 }  // namespace
 
 extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const void* input,
@@ -790,13 +842,16 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
     const uint32_t M = d.channels, T = d.taps_per_branch, D = d.decimation;
     if (d.taps_scaled > 1 || (d.taps_scaled && d.format != RVX_FORMAT_CS16))
         return rvx::fail(RVX_EINVAL, "%s: prescaled taps belong to cs16 input", who);
+    if (d.format == RVX_FORMAT_CF32 && d.correction)
+        return rvx::fail(RVX_EINVAL, "%s: cf32 input arrives corrected (the correction applies "
+                         "where samples are converted)", who);
     if (d.correction && d.taps_scaled)
         return rvx::fail(RVX_EINVAL,
                          "%s: a front-end correction needs the converted samples, not prescaled "
                          "taps", who);
-    if (d.format != RVX_FORMAT_CU8 && d.format != RVX_FORMAT_CS16)
-        return rvx::fail(RVX_EINVAL, "%s: format %d (the fused chain reads cu8 or cs16)", who,
-                         d.format);
+    if (d.format != RVX_FORMAT_CU8 && d.format != RVX_FORMAT_CS16 && d.format != RVX_FORMAT_CF32)
+        return rvx::fail(RVX_EINVAL, "%s: format %d (the fused chain reads cu8, cs16 or cf32)",
+                         who, d.format);
     if (M < 2 || !rvx::pow2(M) || M > 4096 || T == 0 || D == 0 || M % D != 0 ||
         uint64_t(M) * T > (1u << 20))
         return rvx::fail(RVX_EINVAL, "%s: invalid grid M=%u taps=%u D=%u (M <= 4096)", who, M, T,
@@ -817,9 +872,11 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
                          (unsigned long long)ring_samples,
                          (unsigned long long)(uint64_t(rows) * d.out_ring_blocks));
     const uint64_t N = uint64_t(M) * T;
-    if (d.input_samples >= (uint64_t(1) << 31))
-        return rvx::fail(RVX_EINVAL, "%s: %llu input samples exceed 2^31", who,
-                         (unsigned long long)d.input_samples);
+    // byte offsets into the input are 32-bit
+    const uint64_t max_in = d.format == RVX_FORMAT_CF32 ? (uint64_t(1) << 29) : (uint64_t(1) << 31);
+    if (d.input_samples >= max_in)
+        return rvx::fail(RVX_EINVAL, "%s: %llu input samples exceed %llu", who,
+                         (unsigned long long)d.input_samples, (unsigned long long)max_in);
     const uint64_t first_base = d.first_block * D;
     const uint64_t last_base = (d.first_block + d.block_count - 1) * D;
     if (first_base < d.input_origin + (N - 1))
@@ -884,6 +941,8 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
     if (d.correction)
         rc = d.format == RVX_FORMAT_CU8 ? launch_chain<kCu8Corrected>(logm, a, T, ctx->stream)
                                         : launch_chain<kCs16Corrected>(logm, a, T, ctx->stream);
+    else if (d.format == RVX_FORMAT_CF32)
+        rc = launch_chain<kCf32>(logm, a, T, ctx->stream);
     else
         rc = d.format == RVX_FORMAT_CU8 ? launch_chain<RVX_FORMAT_CU8>(logm, a, T, ctx->stream)
              : d.taps_scaled            ? launch_chain<kCs16Scaled>(logm, a, T, ctx->stream)
@@ -891,3 +950,45 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
     if (rc) return rc;
     return rvx::check_launch(who);
 }
+
+namespace rvx {
+
+int32_t convert_to_ring(rvx_ctx* ctx, int32_t format, const void* codes, uint64_t code_samples,
+                        uint64_t first_off, uint64_t first_abs, uint32_t count,
+                        const rvx_iq_correct_params* correction, void* ring, uint32_t ring_mask,
+                        const char* who) {
+    if (count == 0) return RVX_OK;
+    if (first_off + count > code_samples || first_off + count > (uint64_t(1) << 32))
+        return rvx::fail(RVX_EINVAL, "%s: conversion past the window", who);
+    if (count > uint64_t(ring_mask) + 1)
+        return rvx::fail(RVX_EINVAL, "%s: %u samples into a %llu-sample ring", who, count,
+                         (unsigned long long)ring_mask + 1);
+    const float4 c = correction ? make_float4(correction->dc_i, correction->dc_q,
+                                              correction->cross, correction->scale)
+                                : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    const dim3 grid((count + 255) / 256), block(256);
+    float2* r = static_cast<float2*>(ring);
+    if (format == RVX_FORMAT_CS16)
+        convert_ring_kernel<RVX_FORMAT_CS16><<<grid, block, 0, ctx->stream>>>(
+            codes, first_off, first_abs, count, r, ring_mask, c, correction ? 1 : 0);
+    else if (format == RVX_FORMAT_CU8)
+        convert_ring_kernel<RVX_FORMAT_CU8><<<grid, block, 0, ctx->stream>>>(
+            codes, first_off, first_abs, count, r, ring_mask, c, correction ? 1 : 0);
+    else
+        return rvx::fail(RVX_EINVAL, "%s: format %d", who, format);
+    return rvx::check_launch(who);
+}
+
+int32_t ring_to_window(rvx_ctx* ctx, const void* ring, uint32_t ring_mask, uint64_t first_abs,
+                       uint32_t count, void* window, uint64_t window_samples, uint64_t window_off,
+                       const char* who) {
+    if (count == 0) return RVX_OK;
+    if (window_off + count > window_samples || count > uint64_t(ring_mask) + 1)
+        return rvx::fail(RVX_EINVAL, "%s: copy past the window or the ring", who);
+    ring_to_window_kernel<<<dim3((count + 255) / 256), dim3(256), 0, ctx->stream>>>(
+        static_cast<const float2*>(ring), ring_mask, first_abs, count,
+        static_cast<float2*>(window), window_off);
+    return rvx::check_launch(who);
+}
+
+}  // namespace rvx

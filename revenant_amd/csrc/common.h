@@ -38,6 +38,18 @@ int32_t check_launch(const char* who);
 // Resolves `device`'s address of the batch launch's static scratch (the
 // device must be current); idempotent. Called by rvx_ctx_create.
 int32_t batch_scratch_setup(int device, const char* who);
+// Samples [first_off, first_off + count) of a code window converted (and
+// corrected with `correction` unless null) into a cf32 ring at slot
+// (first_abs + i) & ring_mask, as the fused chain converts and corrects them.
+int32_t convert_to_ring(rvx_ctx* ctx, int32_t format, const void* codes, uint64_t code_samples,
+                        uint64_t first_off, uint64_t first_abs, uint32_t count,
+                        const rvx_iq_correct_params* correction, void* ring, uint32_t ring_mask,
+                        const char* who);
+// Ring slots of samples [first_abs, first_abs + count) into a linear cf32
+// window from window_off.
+int32_t ring_to_window(rvx_ctx* ctx, const void* ring, uint32_t ring_mask, uint64_t first_abs,
+                       uint32_t count, void* window, uint64_t window_samples, uint64_t window_off,
+                       const char* who);
 
 }  // namespace rvx
 

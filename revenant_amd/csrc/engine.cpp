@@ -274,6 +274,7 @@ int32_t device_alloc(T** p, uint64_t bytes, const char* who) {
 struct EngineGrid {
     rvx_engine_grid cfg{};
     uint64_t N = 0;  // prototype length M*L
+    std::vector<float> proto;  // host copy (re-uploaded unscaled when excision turns on)
     float* taps = nullptr;
     void* tw = nullptr;
     bool taps_scaled = false;
@@ -348,7 +349,29 @@ struct rvx_engine {
     rvx_iq_correct_params corr_now{0, 0, 0, 0.0f, 0.0f, 0.0f, 1.0f};
     uint64_t corr_from = 0;           // first sample that takes corr_now
     std::vector<std::unique_ptr<EngineGrid>> grids;  // grid k is grids[k - 1]
+    std::vector<float> proto;  // host copy of the engine's own prototype
+    // Impulse excision before the channelizer (rvx_engine_set_excision): each
+    // push is converted (and corrected) into xring, the four passes clean it
+    // in place, the cleaned samples are copied into cwin (a cf32 window that
+    // mirrors win), and the chains read cwin up to cleaned_end.
+    bool excise = false;
+    rvx_excision_config xcfg{};
+    rvx_excision_rings xrings{};
+    rvx_excision_cursor* xcursor = nullptr;
+    uint32_t xmask = 0;
+    void* xring = nullptr;
+    float* xseg_power = nullptr;
+    float* xseg_level = nullptr;
+    uint32_t* xflags = nullptr;
+    void* xremoved = nullptr;
+    uint32_t* xcounters = nullptr;
+    void* cwin = nullptr;
+    void* cbounce = nullptr;
+    uint64_t cleaned_end = 0;
     rvx_engine_stats stats{};
+
+    // One past the last sample the chains may read.
+    uint64_t readable_end() const { return excise ? cleaned_end : win_origin + win_fill; }
 
     // The oldest input sample a block still to run on any grid reads.
     uint64_t oldest_needed() const {
@@ -362,6 +385,10 @@ struct rvx_engine {
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
         for (auto& r : rx) free_rx(r);
         grids.clear();
+        for (void* p : {xring, (void*)xseg_power, (void*)xseg_level, (void*)xflags, xremoved,
+                        (void*)xcounters, cwin, cbounce})
+            if (p) (void)hipFree(p);
+        if (xcursor) rvx_excision_cursor_destroy(xcursor);
         for (void* p : {(void*)taps, tw, nco, (void*)win, (void*)bounce, ring, (void*)partial,
                         (void*)rows, (void*)state, descs_d, work_d, (void*)counter, items_d})
             if (p) (void)hipFree(p);
@@ -979,6 +1006,8 @@ int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const f
     int32_t rc = rvx_build_twiddles(M, tw.data(), tw.size());
     if (!rc) rc = rvx_build_nco_table(16, nco.data(), nco.size());
     e->nco_entries = 1u << 16;
+    e->proto.assign(prototype, prototype + n_prototype);
+    e->cleaned_end = c.start_index;
     const uint64_t frames = c.frame_blocks ? e->max_blocks / c.frame_blocks : 0;
     if (!rc) rc = device_alloc(&e->taps, n_prototype * sizeof(float), who);
     if (!rc) rc = device_alloc(&e->tw, tw.size() * sizeof(float), who);
@@ -1191,6 +1220,7 @@ int32_t rvx_engine_add_grid(rvx_engine* e, const rvx_engine_grid* config, const 
     auto g = std::make_unique<EngineGrid>();
     g->cfg = c;
     g->N = uint64_t(M) * L;
+    g->proto.assign(prototype, prototype + n_prototype);
     g->max_blocks = uint32_t(std::min<uint64_t>(uint64_t(e->cfg.max_push) / D + 2, 0xFFFFFFFFull));
     if (g->max_blocks > c.ring_blocks)
         return rvx::fail(RVX_EINVAL, "%s: a push can complete %u blocks; the ring holds %u", who,
@@ -1235,6 +1265,91 @@ int32_t rvx_engine_add_grid(rvx_engine* e, const rvx_engine_grid* config, const 
         return rvx::fail(RVX_EDEVICE, "%s: upload failed: %s", who, hipGetErrorString(err));
     e->grids.push_back(std::move(g));
     *grid = uint32_t(e->grids.size());
+    return RVX_OK;
+}
+
+int32_t rvx_engine_set_excision(rvx_engine* e, const rvx_excision_config* config,
+                                float threshold) {
+    static const char* who = "rvx_engine_set_excision";
+    if (!e || !config) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    RVX_ENTER(e->ctx, who);
+    if (e->excise) return rvx::fail(RVX_EINVAL, "%s: excision is already on", who);
+    if (e->stats.samples_in)
+        return rvx::fail(RVX_EINVAL, "%s: set before the first push (the chains' input changes "
+                         "from codes to the cleaned samples)", who);
+    const rvx_excision_config c = *config;
+    // the sample ring: a push, the samples still waiting for their scan or
+    // their segment, and what the passes' rings need beside them
+    const uint64_t segment = uint64_t(1) << std::min(c.seg_log2, 31u);
+    const uint64_t lag = std::max<uint64_t>(uint64_t(c.max_width) + c.lead, segment);
+    if (lag > e->cfg.max_push)  // the blocks waiting behind the excision must fit the window
+        return rvx::fail(RVX_EINVAL, "%s: the excision trails the stream by %llu samples, more "
+                         "than a push of %u", who, (unsigned long long)lag, e->cfg.max_push);
+    uint64_t ring = 1;
+    while (ring < uint64_t(e->cfg.max_push) + lag + segment) ring <<= 1;
+    rvx_excision_rings rings{};
+    for (;; ring <<= 1) {
+        if (ring > (uint64_t(1) << 31))
+            return rvx::fail(RVX_EINVAL, "%s: no sample ring up to 2^31 fits a push of %u", who,
+                             e->cfg.max_push);
+        if (rvx_size_excision_rings(&c, e->cfg.max_push, 1, uint32_t(ring - 1), &rings) == RVX_OK)
+            break;
+    }
+    for (uint32_t pass : {0u, 2u}) {  // the config and threshold as the passes will check them
+        rvx_excision_config probe_c = c;
+        probe_c.pass = pass;
+        rvx_excision_params probe{};
+        probe.ring_mask = uint32_t(ring - 1);
+        probe.count = 1;
+        probe.seg_mask = rings.seg_mask;
+        probe.flag_mask = rings.flag_mask;
+        probe.removed_mask = rings.removed_mask;
+        probe.threshold = threshold;
+        if (int32_t rc = rvx_excision_check(&probe_c, &probe)) return rc;
+    }
+    const uint64_t bounce = e->N + uint64_t(e->max_blocks + 1) * e->cfg.decimation;
+    int32_t rc = device_alloc(&e->xring, ring * 8, who);
+    if (!rc) rc = device_alloc(&e->xseg_power, (uint64_t(rings.seg_mask) + 1) * 4, who);
+    if (!rc) rc = device_alloc(&e->xseg_level, (uint64_t(rings.seg_mask) + 1) * 4, who);
+    if (!rc) rc = device_alloc(&e->xflags, (uint64_t(rings.flag_mask) + 1) * 4, who);
+    if (!rc) rc = device_alloc(&e->xremoved, (uint64_t(rings.removed_mask) + 1) * 8, who);
+    if (!rc) rc = device_alloc(&e->xcounters, RVX_EXCISE_COUNTERS * 4, who);
+    if (!rc) rc = device_alloc(&e->cwin, e->win_cap * 8, who);
+    if (!rc) rc = device_alloc(&e->cbounce, bounce * 8, who);
+    if (!rc) rc = rvx_excision_cursor_create(&c, uint32_t(ring - 1), &rings, threshold,
+                                             &e->xcursor);
+    if (rc) return rc;  // what was allocated is freed with the engine
+    RVX_TRY_HIP(hipMemset(e->xcounters, 0, RVX_EXCISE_COUNTERS * 4), who);
+    // the chains now read converted samples: the taps go back to unscaled
+    if (e->taps_scaled) {
+        RVX_TRY_HIP(hipMemcpy(e->taps, e->proto.data(), e->proto.size() * 4,
+                              hipMemcpyHostToDevice),
+                    who);
+        e->taps_scaled = false;
+    }
+    for (auto& g : e->grids)
+        if (g->taps_scaled) {
+            RVX_TRY_HIP(hipMemcpy(g->taps, g->proto.data(), g->proto.size() * 4,
+                                  hipMemcpyHostToDevice),
+                        who);
+            g->taps_scaled = false;
+        }
+    e->xcfg = c;
+    e->xrings = rings;
+    e->xmask = uint32_t(ring - 1);
+    e->cleaned_end = e->win_origin;
+    e->excise = true;
+    return RVX_OK;
+}
+
+int32_t rvx_engine_excision_counters(rvx_engine* e, uint32_t* counters) {
+    static const char* who = "rvx_engine_excision_counters";
+    if (!e || !counters) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    RVX_ENTER(e->ctx, who);
+    if (!e->excise) return rvx::fail(RVX_EINVAL, "%s: excision is off", who);
+    RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
+    RVX_TRY_HIP(hipMemcpy(counters, e->xcounters, RVX_EXCISE_COUNTERS * 4, hipMemcpyDeviceToHost),
+                who);
     return RVX_OK;
 }
 
@@ -1483,6 +1598,51 @@ int32_t rvx_engine_set_noise(rvx_engine* e, uint32_t id, const rvx_line_config* 
 
 static int32_t main_push(rvx_engine* e, uint64_t count, const char* who);
 
+// Impulse excision over one push (graph.cpp:1490 record_impulse_excision): the
+// push's samples converted (with the correction they arrived under) into the
+// sample ring, the cursor's four passes cleaning it in place, and the samples
+// finished now copied into the cleaned window.
+static int32_t excise_push(rvx_engine& e, uint64_t push_first, uint64_t samples,
+                           const char* who) {
+    const uint64_t ring = uint64_t(e.xmask) + 1;
+    if (int32_t rc = rvx::convert_to_ring(e.ctx, e.cfg.format, e.win, e.win_fill,
+                                          push_first - e.win_origin, push_first,
+                                          uint32_t(samples), e.corr_on ? &e.corr_now : nullptr,
+                                          e.xring, e.xmask, who))
+        return rc;
+    rvx_excision_plan plan{};
+    if (int32_t rc = rvx_excision_cursor_plan(e.xcursor, push_first, uint32_t(samples), &plan))
+        return rc;
+    const uint64_t seg = ring >> e.xcfg.seg_log2;
+    const uint64_t flags = uint64_t(e.xrings.flag_mask) + 1;
+    const uint64_t removed = uint64_t(e.xrings.removed_mask) + 1;
+    if (plan.segment.count)
+        if (int32_t rc = rvx_excise_segment(e.ctx, &e.xcfg, &plan.segment, e.xring, ring,
+                                            e.xseg_power, seg))
+            return rc;
+    if (plan.reference.count)
+        if (int32_t rc = rvx_excise_reference(e.ctx, &e.xcfg, &plan.reference, e.xseg_power,
+                                              e.xseg_level, seg))
+            return rc;
+    if (plan.flag.count)
+        if (int32_t rc = rvx_excise_flag(e.ctx, &e.xcfg, &plan.flag, e.xring, ring, e.xseg_level,
+                                         seg, e.xflags, flags))
+            return rc;
+    if (plan.apply.count)
+        if (int32_t rc = rvx_excise_apply(e.ctx, &e.xcfg, &plan.apply, e.xring, ring, e.xflags,
+                                          flags, e.xring, ring, e.xremoved, removed,
+                                          e.xcounters))
+            return rc;
+    if (plan.cleaned_end > e.cleaned_end) {
+        const uint64_t n = plan.cleaned_end - e.cleaned_end;
+        if (int32_t rc = rvx::ring_to_window(e.ctx, e.xring, e.xmask, e.cleaned_end, uint32_t(n),
+                                             e.cwin, e.win_cap, e.cleaned_end - e.win_origin, who))
+            return rc;
+        e.cleaned_end = plan.cleaned_end;
+    }
+    return RVX_OK;
+}
+
 // The complete blocks of an added grid (a push's share), stored into the
 // rows its receivers hold; with no row held the blocks are passed over.
 static int32_t grid_push(rvx_engine& e, EngineGrid& g, uint64_t stop, bool* ran,
@@ -1502,24 +1662,25 @@ static int32_t grid_push(rvx_engine& e, EngineGrid& g, uint64_t stop, bool* ran,
             g.rows_dirty = false;
         }
         rvx_chain_desc d{};
-        d.format = e.cfg.format;
+        d.format = e.excise ? RVX_FORMAT_CF32 : e.cfg.format;
         d.channels = M;
         d.taps_per_branch = g.cfg.taps_per_branch;
         d.decimation = D;
         d.input_origin = e.win_origin;
-        d.input_samples = e.win_fill;
+        d.input_samples = e.readable_end() - e.win_origin;
         d.first_block = g.next_block;
         d.block_count = uint32_t(count);
         d.out_ring_blocks = g.cfg.ring_blocks;
         d.ring_rows = g.cfg.max_rows;
         d.channel_rows = g.rows_d;
         d.taps_scaled = g.taps_scaled ? 1u : 0u;
-        if (e.corr_on) {
+        if (e.corr_on && !e.excise) {
             d.correction = &e.corr_now;
             d.correction_before = &e.corr_before;
             d.correction_from = e.corr_from;
         }
-        if (int32_t rc = rvx_chain(e.ctx, &d, e.win, g.taps, g.N, g.tw, M, g.ring,
+        if (int32_t rc = rvx_chain(e.ctx, &d, e.excise ? e.cwin : e.win, g.taps, g.N, g.tw, M,
+                                   g.ring,
                                    uint64_t(g.cfg.max_rows) * g.cfg.ring_blocks, nullptr, 0))
             return rc;
         e.stats.grid_blocks += count;
@@ -1555,6 +1716,18 @@ int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int3
             RVX_TRY_HIP(hipMemcpyAsync(e->win, e->bounce, kept * e->bps, hipMemcpyDeviceToDevice, s),
                         who);
         }
+        if (e->excise) {  // the cleaned samples kept move with the codes
+            const uint64_t kept_c = e->cleaned_end > keep_from ? e->cleaned_end - keep_from : 0;
+            if (kept_c) {
+                const uint64_t off = (keep_from - e->win_origin) * 8;
+                RVX_TRY_HIP(hipMemcpyAsync(e->cbounce, static_cast<uint8_t*>(e->cwin) + off,
+                                           kept_c * 8, hipMemcpyDeviceToDevice, s),
+                            who);
+                RVX_TRY_HIP(hipMemcpyAsync(e->cwin, e->cbounce, kept_c * 8,
+                                           hipMemcpyDeviceToDevice, s),
+                            who);
+            }
+        }
         e->win_origin = keep_from;
         e->win_fill = kept;
     }
@@ -1578,8 +1751,11 @@ int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int3
     if (e->cfg.calibration) {
         if (int32_t rc = front_end_push(*e, push_first, samples, who)) return rc;
     }
+    if (e->excise) {
+        if (int32_t rc = excise_push(*e, push_first, samples, who)) return rc;
+    }
 
-    const uint64_t stop = e->win_origin + e->win_fill;  // one past the last sample
+    const uint64_t stop = e->readable_end();  // one past the last sample a chain may read
     bool ran = false;
     uint64_t count = 0;
     if (stop > 0 && (stop - 1) / D >= e->next_block) {
@@ -1604,25 +1780,26 @@ static int32_t main_push(rvx_engine* e, uint64_t count, const char* who) {
     const uint32_t D = e->cfg.decimation, M = e->cfg.channels;
 
     rvx_chain_desc d{};
-    d.format = e->cfg.format;
+    d.format = e->excise ? RVX_FORMAT_CF32 : e->cfg.format;
     d.channels = M;
     d.taps_per_branch = e->cfg.taps_per_branch;
     d.decimation = D;
     d.input_origin = e->win_origin;
-    d.input_samples = e->win_fill;
+    d.input_samples = e->readable_end() - e->win_origin;
     d.first_block = e->next_block;
     d.block_count = uint32_t(count);
     d.out_ring_blocks = e->cfg.ring_blocks;
     d.frame_blocks = e->cfg.frame_blocks;
     d.alpha = e->cfg.alpha;
     d.taps_scaled = e->taps_scaled ? 1u : 0u;
-    if (e->corr_on) {
+    if (e->corr_on && !e->excise) {
         d.correction = &e->corr_now;
         d.correction_before = &e->corr_before;
         d.correction_from = e->corr_from;
     }
     const uint64_t frames = e->cfg.frame_blocks ? count / e->cfg.frame_blocks : 0;
-    if (int32_t rc = rvx_chain(e->ctx, &d, e->win, e->taps, e->N, e->tw, M, e->ring,
+    if (int32_t rc = rvx_chain(e->ctx, &d, e->excise ? e->cwin : e->win, e->taps, e->N, e->tw, M,
+                               e->ring,
                                uint64_t(M) * e->cfg.ring_blocks, e->partial, frames * M))
         return rc;
     if (frames) {

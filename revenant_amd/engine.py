@@ -66,6 +66,17 @@ class Engine:
                                                 C.byref(gid)))
         return gid.value
 
+    def set_excision(self, config: _lib.ExcisionConfig, threshold: float) -> None:
+        """Impulse excision before the channelizer (rvx_engine_set_excision);
+        before the first push."""
+        _lib.check(self.lib.rvx_engine_set_excision(self.handle, C.byref(config), threshold))
+
+    def excision_counters(self) -> np.ndarray:
+        out = np.zeros(4, dtype=np.uint32)
+        _lib.check(self.lib.rvx_engine_excision_counters(
+            self.handle, out.ctypes.data_as(C.POINTER(_lib.u32))))
+        return out
+
     def grid_ring(self, grid: int) -> int:
         return self.lib.rvx_engine_grid_ring(self.handle, grid)
 

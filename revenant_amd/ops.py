@@ -266,7 +266,7 @@ class Device:
     def chain(self, desc: _lib.ChainDesc, packed: torch.Tensor, taps: torch.Tensor,
               twiddles: torch.Tensor, ring: torch.Tensor,
               partial: torch.Tensor | None = None) -> None:
-        bps = 2 if desc.format == 0 else 4
+        bps = {0: 2, 2: 4, 4: 8}.get(desc.format, 4)  # cu8, cs16, cf32
         if packed.numel() * packed.element_size() < desc.input_samples * bps:
             raise ValueError("input tensor smaller than input_samples")
         _lib.check(self.lib.rvx_chain(self.ctx, C.byref(desc), _p(packed), _p(taps), _n(taps),
