@@ -1351,6 +1351,22 @@ int32_t rvx_engine_set_excision(rvx_engine* engine, const rvx_excision_config* c
 /* The excision's RVX_EXCISE_COUNTERS words (samples excised, events, runs
  * spared, unused); waits. */
 int32_t rvx_engine_excision_counters(rvx_engine* engine, uint32_t* counters);
+/* Comb cancellation before the channelizer, in the reference's record order
+ * (graph.cpp:1715, record_comb_cancel: after conversion and correction,
+ * before excision). Each push projects the samples onto every line (the
+ * raw, pre-subtraction samples, one projection per line over the push) and,
+ * with subtract, removes the lines' a * tone from them. lines[k].phase0 is
+ * line k's 0.32 phase at absolute sample phase_index; each push advances it
+ * by step per sample, so results do not depend on how the stream is cut.
+ * The lines come from the caller's comb tracker (the reference's comb_bank);
+ * the first call, before the first push, opens the stage (the conditioned
+ * path, as with excision), later calls replace the lines. */
+int32_t rvx_engine_set_comb(rvx_engine* engine, const rvx_comb_line* lines, uint32_t n_lines,
+                            uint64_t phase_index, int32_t subtract);
+/* The last push's projections: *lines cf32 values (line order), over samples
+ * [*first_sample, *first_sample + *samples); waits. */
+int32_t rvx_engine_comb_projections(rvx_engine* engine, float* out, uint64_t n_floats,
+                                    uint64_t* first_sample, uint64_t* samples, uint32_t* lines);
 /* Grid `grid`'s channel ring (device, row-major, max_rows x ring_blocks cf32;
  * grid 0: the engine's channel ring). */
 const void* rvx_engine_grid_ring(rvx_engine* engine, uint32_t grid);

@@ -350,10 +350,13 @@ struct rvx_engine {
     uint64_t corr_from = 0;           // first sample that takes corr_now
     std::vector<std::unique_ptr<EngineGrid>> grids;  // grid k is grids[k - 1]
     std::vector<float> proto;  // host copy of the engine's own prototype
-    // Impulse excision before the channelizer (rvx_engine_set_excision): each
-    // push is converted (and corrected) into xring, the four passes clean it
-    // in place, the cleaned samples are copied into cwin (a cf32 window that
-    // mirrors win), and the chains read cwin up to cleaned_end.
+    // The conditioned path (set before the first push by rvx_engine_set_comb
+    // or rvx_engine_set_excision): each push is converted (and corrected) into
+    // xring, comb cancellation and impulse excision clean it in place, the
+    // cleaned samples are copied into cwin (a cf32 window that mirrors win),
+    // and the chains read cwin up to cleaned_end.
+    bool condition = false;
+    uint64_t xring_n = 0;
     bool excise = false;
     rvx_excision_config xcfg{};
     rvx_excision_rings xrings{};
@@ -368,10 +371,22 @@ struct rvx_engine {
     void* cwin = nullptr;
     void* cbounce = nullptr;
     uint64_t cleaned_end = 0;
+    // comb cancellation (rvx_engine_set_comb)
+    bool comb_on = false, comb_subtract = false;
+    std::vector<rvx_comb_line> comb_lines;  // phases at comb_index
+    uint64_t comb_index = 0;
+    rvx_comb_line* comb_h = nullptr;  // pinned: one push's lines
+    void* comb_d = nullptr;
+    void* comb_table = nullptr;       // the 4096-entry tone table
+    void* comb_removed = nullptr;     // xring_n cf32
+    float* comb_proj = nullptr;       // the last push's projections, a cf32 per line
+    hipEvent_t comb_staged = nullptr;
+    uint64_t proj_start = 0, proj_count = 0;
+    uint32_t proj_lines = 0;
     rvx_engine_stats stats{};
 
     // One past the last sample the chains may read.
-    uint64_t readable_end() const { return excise ? cleaned_end : win_origin + win_fill; }
+    uint64_t readable_end() const { return condition ? cleaned_end : win_origin + win_fill; }
 
     // The oldest input sample a block still to run on any grid reads.
     uint64_t oldest_needed() const {
@@ -386,8 +401,11 @@ struct rvx_engine {
         for (auto& r : rx) free_rx(r);
         grids.clear();
         for (void* p : {xring, (void*)xseg_power, (void*)xseg_level, (void*)xflags, xremoved,
-                        (void*)xcounters, cwin, cbounce})
+                        (void*)xcounters, cwin, cbounce, comb_d, comb_table, comb_removed,
+                        (void*)comb_proj})
             if (p) (void)hipFree(p);
+        if (comb_h) (void)hipHostFree(comb_h);
+        if (comb_staged) (void)hipEventDestroy(comb_staged);
         if (xcursor) rvx_excision_cursor_destroy(xcursor);
         for (void* p : {(void*)taps, tw, nco, (void*)win, (void*)bounce, ring, (void*)partial,
                         (void*)rows, (void*)state, descs_d, work_d, (void*)counter, items_d})
@@ -1268,6 +1286,48 @@ int32_t rvx_engine_add_grid(rvx_engine* e, const rvx_engine_grid* config, const 
     return RVX_OK;
 }
 
+// The conditioned path's buffers, before the first push: the sample ring (at
+// least `ring` samples, grown if a stage needs more), the cleaned window and
+// its bounce buffer; the chains' taps go back to unscaled, since they now
+// read converted samples.
+static int32_t condition_setup(rvx_engine& e, uint64_t ring, const char* who) {
+    if (e.stats.samples_in)
+        return rvx::fail(RVX_EINVAL, "%s: set before the first push (the chains' input changes "
+                         "from codes to the conditioned samples)", who);
+    const uint64_t bounce = e.N + uint64_t(e.max_blocks + 1) * e.cfg.decimation;
+    if (!e.cwin) {
+        if (int32_t rc = device_alloc(&e.cwin, e.win_cap * 8, who)) return rc;
+        if (int32_t rc = device_alloc(&e.cbounce, bounce * 8, who)) return rc;
+    }
+    if (e.xring_n < ring) {
+        if (e.xring) (void)hipFree(e.xring);
+        e.xring = nullptr;
+        if (int32_t rc = device_alloc(&e.xring, ring * 8, who)) return rc;
+        if (e.comb_removed) {
+            (void)hipFree(e.comb_removed);
+            e.comb_removed = nullptr;
+            if (int32_t rc = device_alloc(&e.comb_removed, ring * 8, who)) return rc;
+        }
+        e.xring_n = ring;
+        e.xmask = uint32_t(ring - 1);
+    }
+    if (e.taps_scaled) {
+        RVX_TRY_HIP(hipMemcpy(e.taps, e.proto.data(), e.proto.size() * 4, hipMemcpyHostToDevice),
+                    who);
+        e.taps_scaled = false;
+    }
+    for (auto& g : e.grids)
+        if (g->taps_scaled) {
+            RVX_TRY_HIP(hipMemcpy(g->taps, g->proto.data(), g->proto.size() * 4,
+                                  hipMemcpyHostToDevice),
+                        who);
+            g->taps_scaled = false;
+        }
+    e.cleaned_end = e.win_origin;
+    e.condition = true;
+    return RVX_OK;
+}
+
 int32_t rvx_engine_set_excision(rvx_engine* e, const rvx_excision_config* config,
                                 float threshold) {
     static const char* who = "rvx_engine_set_excision";
@@ -1287,6 +1347,7 @@ int32_t rvx_engine_set_excision(rvx_engine* e, const rvx_excision_config* config
                          "than a push of %u", who, (unsigned long long)lag, e->cfg.max_push);
     uint64_t ring = 1;
     while (ring < uint64_t(e->cfg.max_push) + lag + segment) ring <<= 1;
+    ring = std::max(ring, e->xring_n);
     rvx_excision_rings rings{};
     for (;; ring <<= 1) {
         if (ring > (uint64_t(1) << 31))
@@ -1307,38 +1368,74 @@ int32_t rvx_engine_set_excision(rvx_engine* e, const rvx_excision_config* config
         probe.threshold = threshold;
         if (int32_t rc = rvx_excision_check(&probe_c, &probe)) return rc;
     }
-    const uint64_t bounce = e->N + uint64_t(e->max_blocks + 1) * e->cfg.decimation;
-    int32_t rc = device_alloc(&e->xring, ring * 8, who);
-    if (!rc) rc = device_alloc(&e->xseg_power, (uint64_t(rings.seg_mask) + 1) * 4, who);
+    if (int32_t rc = condition_setup(*e, ring, who)) return rc;
+    int32_t rc = device_alloc(&e->xseg_power, (uint64_t(rings.seg_mask) + 1) * 4, who);
     if (!rc) rc = device_alloc(&e->xseg_level, (uint64_t(rings.seg_mask) + 1) * 4, who);
     if (!rc) rc = device_alloc(&e->xflags, (uint64_t(rings.flag_mask) + 1) * 4, who);
     if (!rc) rc = device_alloc(&e->xremoved, (uint64_t(rings.removed_mask) + 1) * 8, who);
     if (!rc) rc = device_alloc(&e->xcounters, RVX_EXCISE_COUNTERS * 4, who);
-    if (!rc) rc = device_alloc(&e->cwin, e->win_cap * 8, who);
-    if (!rc) rc = device_alloc(&e->cbounce, bounce * 8, who);
-    if (!rc) rc = rvx_excision_cursor_create(&c, uint32_t(ring - 1), &rings, threshold,
-                                             &e->xcursor);
+    if (!rc) rc = rvx_excision_cursor_create(&c, e->xmask, &rings, threshold, &e->xcursor);
     if (rc) return rc;  // what was allocated is freed with the engine
     RVX_TRY_HIP(hipMemset(e->xcounters, 0, RVX_EXCISE_COUNTERS * 4), who);
-    // the chains now read converted samples: the taps go back to unscaled
-    if (e->taps_scaled) {
-        RVX_TRY_HIP(hipMemcpy(e->taps, e->proto.data(), e->proto.size() * 4,
-                              hipMemcpyHostToDevice),
-                    who);
-        e->taps_scaled = false;
-    }
-    for (auto& g : e->grids)
-        if (g->taps_scaled) {
-            RVX_TRY_HIP(hipMemcpy(g->taps, g->proto.data(), g->proto.size() * 4,
-                                  hipMemcpyHostToDevice),
-                        who);
-            g->taps_scaled = false;
-        }
     e->xcfg = c;
     e->xrings = rings;
-    e->xmask = uint32_t(ring - 1);
-    e->cleaned_end = e->win_origin;
     e->excise = true;
+    return RVX_OK;
+}
+
+int32_t rvx_engine_set_comb(rvx_engine* e, const rvx_comb_line* lines, uint32_t n_lines,
+                            uint64_t phase_index, int32_t subtract) {
+    static const char* who = "rvx_engine_set_comb";
+    if (!e || (n_lines && !lines)) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    RVX_ENTER(e->ctx, who);
+    if (n_lines > RVX_COMB_MAX_LINES)
+        return rvx::fail(RVX_EINVAL, "%s: %u lines is more than the %u the stage cancels", who,
+                         n_lines, RVX_COMB_MAX_LINES);
+    for (uint32_t k = 0; k < n_lines; ++k)
+        if (!std::isfinite(lines[k].amplitude[0]) || !std::isfinite(lines[k].amplitude[1]))
+            return rvx::fail(RVX_EINVAL, "%s: line %u's amplitude is not finite", who, k);
+    if (!e->comb_on) {  // the first call opens the stage: before the first push
+        uint64_t ring = 1;
+        while (ring < uint64_t(e->cfg.max_push)) ring <<= 1;
+        if (int32_t rc = condition_setup(*e, std::max(ring, e->xring_n), who)) return rc;
+        const uint64_t table_n = uint64_t(1) << RVX_COMB_TABLE_LOG2;
+        std::vector<float> table(2 * table_n);
+        int32_t rc = rvx_comb_table(table.data(), table.size());
+        if (!rc) rc = device_alloc(&e->comb_d, RVX_COMB_MAX_LINES * sizeof(rvx_comb_line), who);
+        if (!rc) rc = device_alloc(&e->comb_table, table.size() * 4, who);
+        if (!rc) rc = device_alloc(&e->comb_removed, e->xring_n * 8, who);
+        if (!rc) rc = device_alloc(&e->comb_proj, RVX_COMB_MAX_LINES * 8, who);
+        if (rc) return rc;
+        RVX_TRY_HIP(hipHostMalloc((void**)&e->comb_h, RVX_COMB_MAX_LINES * sizeof(rvx_comb_line),
+                                  hipHostMallocDefault),
+                    who);
+        RVX_TRY_HIP(hipMemcpy(e->comb_table, table.data(), table.size() * 4,
+                              hipMemcpyHostToDevice),
+                    who);
+        RVX_TRY_HIP(hipEventCreateWithFlags(&e->comb_staged, hipEventDisableTiming), who);
+        RVX_TRY_HIP(hipEventRecord(e->comb_staged, e->ctx->stream), who);
+        e->comb_on = true;
+    }
+    e->comb_lines.assign(lines, lines + n_lines);
+    e->comb_index = phase_index;
+    e->comb_subtract = subtract != 0;
+    return RVX_OK;
+}
+
+int32_t rvx_engine_comb_projections(rvx_engine* e, float* out, uint64_t n_floats,
+                                    uint64_t* first_sample, uint64_t* samples, uint32_t* lines) {
+    static const char* who = "rvx_engine_comb_projections";
+    if (!e || !lines) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    RVX_ENTER(e->ctx, who);
+    if (!e->comb_on) return rvx::fail(RVX_EINVAL, "%s: the comb stage is off", who);
+    *lines = e->proj_lines;
+    if (first_sample) *first_sample = e->proj_start;
+    if (samples) *samples = e->proj_count;
+    if (e->proj_lines == 0) return RVX_OK;
+    if (!out || n_floats < 2ull * e->proj_lines)
+        return rvx::fail(RVX_EINVAL, "%s: wants %u floats", who, 2 * e->proj_lines);
+    RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
+    RVX_TRY_HIP(hipMemcpy(out, e->comb_proj, 2ull * e->proj_lines * 4, hipMemcpyDeviceToHost), who);
     return RVX_OK;
 }
 
@@ -1598,47 +1695,87 @@ int32_t rvx_engine_set_noise(rvx_engine* e, uint32_t id, const rvx_line_config* 
 
 static int32_t main_push(rvx_engine* e, uint64_t count, const char* who);
 
-// Impulse excision over one push (graph.cpp:1490 record_impulse_excision): the
-// push's samples converted (with the correction they arrived under) into the
-// sample ring, the cursor's four passes cleaning it in place, and the samples
+// The conditioned path over one push, in the reference's record order
+// (graph.cpp: the converted block, its correction, record_comb_cancel :1715,
+// record_impulse_excision :1490): the push's samples converted with the
+// correction they arrived under into the sample ring, the comb's lines
+// projected and subtracted, the excision cursor's four passes, and the samples
 // finished now copied into the cleaned window.
-static int32_t excise_push(rvx_engine& e, uint64_t push_first, uint64_t samples,
-                           const char* who) {
-    const uint64_t ring = uint64_t(e.xmask) + 1;
+static int32_t condition_push(rvx_engine& e, uint64_t push_first, uint64_t samples,
+                              const char* who) {
+    const uint64_t ring = e.xring_n;
     if (int32_t rc = rvx::convert_to_ring(e.ctx, e.cfg.format, e.win, e.win_fill,
                                           push_first - e.win_origin, push_first,
                                           uint32_t(samples), e.corr_on ? &e.corr_now : nullptr,
                                           e.xring, e.xmask, who))
         return rc;
-    rvx_excision_plan plan{};
-    if (int32_t rc = rvx_excision_cursor_plan(e.xcursor, push_first, uint32_t(samples), &plan))
-        return rc;
-    const uint64_t seg = ring >> e.xcfg.seg_log2;
-    const uint64_t flags = uint64_t(e.xrings.flag_mask) + 1;
-    const uint64_t removed = uint64_t(e.xrings.removed_mask) + 1;
-    if (plan.segment.count)
-        if (int32_t rc = rvx_excise_segment(e.ctx, &e.xcfg, &plan.segment, e.xring, ring,
-                                            e.xseg_power, seg))
+    if (e.comb_on && !e.comb_lines.empty()) {
+        const uint32_t n = uint32_t(e.comb_lines.size());
+        // each line's phase at the push's first sample: phase0 + step * (j0 - index), 0.32
+        const uint64_t since = push_first - e.comb_index;
+        RVX_TRY_HIP(hipEventSynchronize(e.comb_staged), who);  // the last upload has been read
+        for (uint32_t k = 0; k < n; ++k) {
+            rvx_comb_line l = e.comb_lines[k];
+            l.phase0 = uint32_t(l.phase0 + uint64_t(l.step) * (since & 0xFFFFFFFFull));
+            e.comb_h[k] = l;
+        }
+        RVX_TRY_HIP(hipMemcpyAsync(e.comb_d, e.comb_h, n * sizeof(rvx_comb_line),
+                                   hipMemcpyHostToDevice, e.ctx->stream),
+                    who);
+        RVX_TRY_HIP(hipEventRecord(e.comb_staged, e.ctx->stream), who);
+        rvx_comb_params p{};
+        p.ring_mask = e.xmask;
+        p.first = uint32_t(push_first & e.xmask);
+        p.count = uint32_t(samples);
+        p.lines = n;
+        p.sub_blocks = 1;
+        p.removed_mask = e.xmask;
+        const auto* lines = static_cast<const rvx_comb_line*>(e.comb_d);
+        if (int32_t rc = rvx_comb_project(e.ctx, &p, e.xring, ring, lines, n, e.comb_table,
+                                          uint64_t(1) << RVX_COMB_TABLE_LOG2, e.comb_proj, n))
             return rc;
-    if (plan.reference.count)
-        if (int32_t rc = rvx_excise_reference(e.ctx, &e.xcfg, &plan.reference, e.xseg_power,
-                                              e.xseg_level, seg))
+        e.proj_start = push_first;
+        e.proj_count = samples;
+        e.proj_lines = n;
+        if (e.comb_subtract)
+            if (int32_t rc = rvx_comb_subtract(e.ctx, &p, e.xring, ring, lines, n, e.comb_table,
+                                               uint64_t(1) << RVX_COMB_TABLE_LOG2, e.xring, ring,
+                                               e.comb_removed, ring))
+                return rc;
+    }
+    uint64_t finished = push_first + samples;
+    if (e.excise) {
+        rvx_excision_plan plan{};
+        if (int32_t rc = rvx_excision_cursor_plan(e.xcursor, push_first, uint32_t(samples), &plan))
             return rc;
-    if (plan.flag.count)
-        if (int32_t rc = rvx_excise_flag(e.ctx, &e.xcfg, &plan.flag, e.xring, ring, e.xseg_level,
-                                         seg, e.xflags, flags))
-            return rc;
-    if (plan.apply.count)
-        if (int32_t rc = rvx_excise_apply(e.ctx, &e.xcfg, &plan.apply, e.xring, ring, e.xflags,
-                                          flags, e.xring, ring, e.xremoved, removed,
-                                          e.xcounters))
-            return rc;
-    if (plan.cleaned_end > e.cleaned_end) {
-        const uint64_t n = plan.cleaned_end - e.cleaned_end;
+        const uint64_t seg = ring >> e.xcfg.seg_log2;
+        const uint64_t flags = uint64_t(e.xrings.flag_mask) + 1;
+        const uint64_t removed = uint64_t(e.xrings.removed_mask) + 1;
+        if (plan.segment.count)
+            if (int32_t rc = rvx_excise_segment(e.ctx, &e.xcfg, &plan.segment, e.xring, ring,
+                                                e.xseg_power, seg))
+                return rc;
+        if (plan.reference.count)
+            if (int32_t rc = rvx_excise_reference(e.ctx, &e.xcfg, &plan.reference, e.xseg_power,
+                                                  e.xseg_level, seg))
+                return rc;
+        if (plan.flag.count)
+            if (int32_t rc = rvx_excise_flag(e.ctx, &e.xcfg, &plan.flag, e.xring, ring,
+                                             e.xseg_level, seg, e.xflags, flags))
+                return rc;
+        if (plan.apply.count)
+            if (int32_t rc = rvx_excise_apply(e.ctx, &e.xcfg, &plan.apply, e.xring, ring,
+                                              e.xflags, flags, e.xring, ring, e.xremoved,
+                                              removed, e.xcounters))
+                return rc;
+        finished = plan.cleaned_end;
+    }
+    if (finished > e.cleaned_end) {
+        const uint64_t n = finished - e.cleaned_end;
         if (int32_t rc = rvx::ring_to_window(e.ctx, e.xring, e.xmask, e.cleaned_end, uint32_t(n),
                                              e.cwin, e.win_cap, e.cleaned_end - e.win_origin, who))
             return rc;
-        e.cleaned_end = plan.cleaned_end;
+        e.cleaned_end = finished;
     }
     return RVX_OK;
 }
@@ -1662,7 +1799,7 @@ static int32_t grid_push(rvx_engine& e, EngineGrid& g, uint64_t stop, bool* ran,
             g.rows_dirty = false;
         }
         rvx_chain_desc d{};
-        d.format = e.excise ? RVX_FORMAT_CF32 : e.cfg.format;
+        d.format = e.condition ? RVX_FORMAT_CF32 : e.cfg.format;
         d.channels = M;
         d.taps_per_branch = g.cfg.taps_per_branch;
         d.decimation = D;
@@ -1674,12 +1811,12 @@ static int32_t grid_push(rvx_engine& e, EngineGrid& g, uint64_t stop, bool* ran,
         d.ring_rows = g.cfg.max_rows;
         d.channel_rows = g.rows_d;
         d.taps_scaled = g.taps_scaled ? 1u : 0u;
-        if (e.corr_on && !e.excise) {
+        if (e.corr_on && !e.condition) {
             d.correction = &e.corr_now;
             d.correction_before = &e.corr_before;
             d.correction_from = e.corr_from;
         }
-        if (int32_t rc = rvx_chain(e.ctx, &d, e.excise ? e.cwin : e.win, g.taps, g.N, g.tw, M,
+        if (int32_t rc = rvx_chain(e.ctx, &d, e.condition ? e.cwin : e.win, g.taps, g.N, g.tw, M,
                                    g.ring,
                                    uint64_t(g.cfg.max_rows) * g.cfg.ring_blocks, nullptr, 0))
             return rc;
@@ -1716,7 +1853,7 @@ int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int3
             RVX_TRY_HIP(hipMemcpyAsync(e->win, e->bounce, kept * e->bps, hipMemcpyDeviceToDevice, s),
                         who);
         }
-        if (e->excise) {  // the cleaned samples kept move with the codes
+        if (e->condition) {  // the cleaned samples kept move with the codes
             const uint64_t kept_c = e->cleaned_end > keep_from ? e->cleaned_end - keep_from : 0;
             if (kept_c) {
                 const uint64_t off = (keep_from - e->win_origin) * 8;
@@ -1751,8 +1888,8 @@ int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int3
     if (e->cfg.calibration) {
         if (int32_t rc = front_end_push(*e, push_first, samples, who)) return rc;
     }
-    if (e->excise) {
-        if (int32_t rc = excise_push(*e, push_first, samples, who)) return rc;
+    if (e->condition) {
+        if (int32_t rc = condition_push(*e, push_first, samples, who)) return rc;
     }
 
     const uint64_t stop = e->readable_end();  // one past the last sample a chain may read
@@ -1780,7 +1917,7 @@ static int32_t main_push(rvx_engine* e, uint64_t count, const char* who) {
     const uint32_t D = e->cfg.decimation, M = e->cfg.channels;
 
     rvx_chain_desc d{};
-    d.format = e->excise ? RVX_FORMAT_CF32 : e->cfg.format;
+    d.format = e->condition ? RVX_FORMAT_CF32 : e->cfg.format;
     d.channels = M;
     d.taps_per_branch = e->cfg.taps_per_branch;
     d.decimation = D;
@@ -1792,13 +1929,13 @@ static int32_t main_push(rvx_engine* e, uint64_t count, const char* who) {
     d.frame_blocks = e->cfg.frame_blocks;
     d.alpha = e->cfg.alpha;
     d.taps_scaled = e->taps_scaled ? 1u : 0u;
-    if (e->corr_on && !e->excise) {
+    if (e->corr_on && !e->condition) {
         d.correction = &e->corr_now;
         d.correction_before = &e->corr_before;
         d.correction_from = e->corr_from;
     }
     const uint64_t frames = e->cfg.frame_blocks ? count / e->cfg.frame_blocks : 0;
-    if (int32_t rc = rvx_chain(e->ctx, &d, e->excise ? e->cwin : e->win, e->taps, e->N, e->tw, M,
+    if (int32_t rc = rvx_chain(e->ctx, &d, e->condition ? e->cwin : e->win, e->taps, e->N, e->tw, M,
                                e->ring,
                                uint64_t(M) * e->cfg.ring_blocks, e->partial, frames * M))
         return rc;

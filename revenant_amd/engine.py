@@ -71,6 +71,23 @@ class Engine:
         before the first push."""
         _lib.check(self.lib.rvx_engine_set_excision(self.handle, C.byref(config), threshold))
 
+    def set_comb(self, lines, phase_index: int = 0, subtract: bool = True) -> None:
+        """Comb cancellation (rvx_engine_set_comb): lines is a ctypes array
+        of CombLine (phases at absolute sample phase_index) or empty."""
+        n = len(lines) if lines is not None else 0
+        ptr = C.cast(lines, C.POINTER(_lib.CombLine)) if n else None
+        _lib.check(self.lib.rvx_engine_set_comb(self.handle, ptr, n, phase_index,
+                                                1 if subtract else 0))
+
+    def comb_projections(self):
+        """(projections complex64 per line, first sample, samples) of the last push."""
+        out = np.zeros(2 * 1024, dtype=np.float32)
+        first, count, lines = _lib.u64(), _lib.u64(), _lib.u32()
+        _lib.check(self.lib.rvx_engine_comb_projections(
+            self.handle, out.ctypes.data_as(_lib.fptr), out.size, C.byref(first), C.byref(count),
+            C.byref(lines)))
+        return out[:2 * lines.value].view(np.complex64).copy(), first.value, count.value
+
     def excision_counters(self) -> np.ndarray:
         out = np.zeros(4, dtype=np.uint32)
         _lib.check(self.lib.rvx_engine_excision_counters(
