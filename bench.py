@@ -646,6 +646,48 @@ def conditioning_leg(dev, torch, packed, g: Grid, partial, kernel_ms, steps,
             "sample": f"{n_cpu_rx} receivers x {count} samples"}
     del rings_c
     out["noise"] = noise_leg(dev, torch, gen, timed, cpu)
+    out["engine_conditioning"] = engine_conditioning_leg(dev, torch, packed)
+    return out
+
+
+def engine_conditioning_leg(dev, torch, packed, push: int = 1 << 24, pushes: int = 6) -> dict:
+    """The engine's conditioned path against its plain one on the 4096-channel
+    grid (averaged spectrum on, no receivers): pushes of 2^24 resident codes,
+    plain (the chain reads the codes), with impulse excision (each push
+    converted into the sample ring, the four passes, the cleaned samples into
+    the cf32 window, the chain's cf32 form), and with 64 comb lines projected
+    and subtracted ahead of the excision. ms per push, the first push untimed."""
+    from revenant_amd import _lib as L
+    from revenant_amd.dsp import FORMAT_CS16, GridParams, comb_step, design_prototype
+    from revenant_amd.engine import Engine
+    from revenant_amd.excision import design_excision, excision_threshold
+
+    grid = GridParams(M, TAPS, D)
+    proto = design_prototype(grid, beta=8.0).taps
+    codes = packed.view(torch.int32)
+    lines = (L.CombLine * 64)()
+    for k in range(64):
+        lines[k].phase0 = (0x9E3779B9 * (k + 1)) % (1 << 32)
+        lines[k].step = comb_step(1_000_000.0 * (k + 1) - 31_000_000.0, RATE)
+        lines[k].amplitude[0] = np.float32(1e-3)
+    out = {"push_samples": push}
+    for name in ("plain", "excision", "comb64_excision"):
+        eng = Engine(dev, grid, proto, FORMAT_CS16, max_push=push, ring_blocks=1 << 14,
+                     frame_blocks=FRAME, row_frames=ROW_FRAMES, alpha=ALPHA)
+        if name == "comb64_excision":
+            eng.set_comb(lines)
+        if name != "plain":
+            eng.set_excision(design_excision(RATE), excision_threshold(15.0))
+        eng.push(codes[:push])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(1, pushes + 1):
+            at = (k * push) % (SAMPLES - push)
+            eng.push(codes[at:at + push])
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / pushes
+        out[name] = {"ms_per_push": round(ms, 3), "MS_per_s": round(push / (ms * 1e-3) / 1e6, 1)}
+        eng.close()
     return out
 
 

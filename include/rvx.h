@@ -1363,10 +1363,13 @@ int32_t rvx_engine_excision_counters(rvx_engine* engine, uint32_t* counters);
  * path, as with excision), later calls replace the lines. */
 int32_t rvx_engine_set_comb(rvx_engine* engine, const rvx_comb_line* lines, uint32_t n_lines,
                             uint64_t phase_index, int32_t subtract);
-/* The last push's projections: *lines cf32 values (line order), over samples
- * [*first_sample, *first_sample + *samples); waits. */
+/* The last push's projections over samples [*first_sample, *first_sample +
+ * *samples): *lines x *sub_blocks cf32 values, line-major, sub-block j over
+ * the j-th equal part of the push (sub_blocks: the largest power of two up
+ * to RVX_COMB_MAX_SUB_BLOCKS dividing the push); waits. */
 int32_t rvx_engine_comb_projections(rvx_engine* engine, float* out, uint64_t n_floats,
-                                    uint64_t* first_sample, uint64_t* samples, uint32_t* lines);
+                                    uint64_t* first_sample, uint64_t* samples, uint32_t* lines,
+                                    uint32_t* sub_blocks);
 /* Grid `grid`'s channel ring (device, row-major, max_rows x ring_blocks cf32;
  * grid 0: the engine's channel ring). */
 const void* rvx_engine_grid_ring(rvx_engine* engine, uint32_t grid);

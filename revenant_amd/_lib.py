@@ -488,7 +488,7 @@ _SIGNATURES = {
     "rvx_engine_excision_counters": (i32, [vp, C.POINTER(u32)]),
     "rvx_engine_set_comb": (i32, [vp, C.POINTER(CombLine), u32, u64, i32]),
     "rvx_engine_comb_projections": (i32, [vp, fptr, u64, C.POINTER(u64), C.POINTER(u64),
-                                          C.POINTER(u32)]),
+                                          C.POINTER(u32), C.POINTER(u32)]),
     "rvx_engine_grid_row": (i32, [vp, u32, u32, C.POINTER(u32), C.POINTER(u64)]),
     "rvx_engine_remove_vrx": (i32, [vp, u32]),
     "rvx_engine_push": (i32, [vp, vp, u64, i32]),

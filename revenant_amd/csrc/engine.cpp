@@ -382,7 +382,7 @@ struct rvx_engine {
     float* comb_proj = nullptr;       // the last push's projections, a cf32 per line
     hipEvent_t comb_staged = nullptr;
     uint64_t proj_start = 0, proj_count = 0;
-    uint32_t proj_lines = 0;
+    uint32_t proj_lines = 0, proj_subs = 0;
     rvx_engine_stats stats{};
 
     // One past the last sample the chains may read.
@@ -1404,7 +1404,9 @@ int32_t rvx_engine_set_comb(rvx_engine* e, const rvx_comb_line* lines, uint32_t 
         if (!rc) rc = device_alloc(&e->comb_d, RVX_COMB_MAX_LINES * sizeof(rvx_comb_line), who);
         if (!rc) rc = device_alloc(&e->comb_table, table.size() * 4, who);
         if (!rc) rc = device_alloc(&e->comb_removed, e->xring_n * 8, who);
-        if (!rc) rc = device_alloc(&e->comb_proj, RVX_COMB_MAX_LINES * 8, who);
+        if (!rc)
+            rc = device_alloc(&e->comb_proj,
+                              uint64_t(RVX_COMB_MAX_LINES) * RVX_COMB_MAX_SUB_BLOCKS * 8, who);
         if (rc) return rc;
         RVX_TRY_HIP(hipHostMalloc((void**)&e->comb_h, RVX_COMB_MAX_LINES * sizeof(rvx_comb_line),
                                   hipHostMallocDefault),
@@ -1423,19 +1425,23 @@ int32_t rvx_engine_set_comb(rvx_engine* e, const rvx_comb_line* lines, uint32_t 
 }
 
 int32_t rvx_engine_comb_projections(rvx_engine* e, float* out, uint64_t n_floats,
-                                    uint64_t* first_sample, uint64_t* samples, uint32_t* lines) {
+                                    uint64_t* first_sample, uint64_t* samples, uint32_t* lines,
+                                    uint32_t* sub_blocks) {
     static const char* who = "rvx_engine_comb_projections";
-    if (!e || !lines) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    if (!e || !lines || !sub_blocks) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
     RVX_ENTER(e->ctx, who);
     if (!e->comb_on) return rvx::fail(RVX_EINVAL, "%s: the comb stage is off", who);
     *lines = e->proj_lines;
+    *sub_blocks = e->proj_subs;
     if (first_sample) *first_sample = e->proj_start;
     if (samples) *samples = e->proj_count;
-    if (e->proj_lines == 0) return RVX_OK;
-    if (!out || n_floats < 2ull * e->proj_lines)
-        return rvx::fail(RVX_EINVAL, "%s: wants %u floats", who, 2 * e->proj_lines);
+    const uint64_t values = uint64_t(e->proj_lines) * e->proj_subs;
+    if (values == 0) return RVX_OK;
+    if (!out || n_floats < 2 * values)
+        return rvx::fail(RVX_EINVAL, "%s: wants %llu floats", who,
+                         (unsigned long long)(2 * values));
     RVX_TRY_HIP(hipStreamSynchronize(e->ctx->stream), who);
-    RVX_TRY_HIP(hipMemcpy(out, e->comb_proj, 2ull * e->proj_lines * 4, hipMemcpyDeviceToHost), who);
+    RVX_TRY_HIP(hipMemcpy(out, e->comb_proj, 2 * values * 4, hipMemcpyDeviceToHost), who);
     return RVX_OK;
 }
 
@@ -1728,15 +1734,20 @@ static int32_t condition_push(rvx_engine& e, uint64_t push_first, uint64_t sampl
         p.first = uint32_t(push_first & e.xmask);
         p.count = uint32_t(samples);
         p.lines = n;
-        p.sub_blocks = 1;
+        // sub-blocks: the largest power of two up to 16 dividing the push (a
+        // projection's workgroup sums its sub-block in order)
+        p.sub_blocks = std::min<uint32_t>(uint32_t(samples & (~samples + 1)),
+                                          RVX_COMB_MAX_SUB_BLOCKS);
         p.removed_mask = e.xmask;
         const auto* lines = static_cast<const rvx_comb_line*>(e.comb_d);
         if (int32_t rc = rvx_comb_project(e.ctx, &p, e.xring, ring, lines, n, e.comb_table,
-                                          uint64_t(1) << RVX_COMB_TABLE_LOG2, e.comb_proj, n))
+                                          uint64_t(1) << RVX_COMB_TABLE_LOG2, e.comb_proj,
+                                          uint64_t(n) * p.sub_blocks))
             return rc;
         e.proj_start = push_first;
         e.proj_count = samples;
         e.proj_lines = n;
+        e.proj_subs = p.sub_blocks;
         if (e.comb_subtract)
             if (int32_t rc = rvx_comb_subtract(e.ctx, &p, e.xring, ring, lines, n, e.comb_table,
                                                uint64_t(1) << RVX_COMB_TABLE_LOG2, e.xring, ring,

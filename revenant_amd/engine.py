@@ -80,13 +80,16 @@ class Engine:
                                                 1 if subtract else 0))
 
     def comb_projections(self):
-        """(projections complex64 per line, first sample, samples) of the last push."""
-        out = np.zeros(2 * 1024, dtype=np.float32)
-        first, count, lines = _lib.u64(), _lib.u64(), _lib.u32()
+        """(projections complex64 [lines, sub_blocks], first sample, samples)
+        of the last push."""
+        out = np.zeros(2 * 1024 * 16, dtype=np.float32)
+        first, count, lines, subs = _lib.u64(), _lib.u64(), _lib.u32(), _lib.u32()
         _lib.check(self.lib.rvx_engine_comb_projections(
             self.handle, out.ctypes.data_as(_lib.fptr), out.size, C.byref(first), C.byref(count),
-            C.byref(lines)))
-        return out[:2 * lines.value].view(np.complex64).copy(), first.value, count.value
+            C.byref(lines), C.byref(subs)))
+        n = lines.value * subs.value
+        proj = out[:2 * n].view(np.complex64).reshape(lines.value, subs.value).copy()
+        return proj, first.value, count.value
 
     def excision_counters(self) -> np.ndarray:
         out = np.zeros(4, dtype=np.uint32)

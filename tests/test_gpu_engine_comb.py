@@ -159,18 +159,20 @@ def _check(stream, got, want, what):
 def test_engine_comb_matches_the_oracle(dev, stream):
     converted = stream[1]
     want = _composition(_oracle_subtract(converted), SAMPLES)
-    cuts = _cuts(0x43, SAMPLES, 2000, 65536)
+    cuts = _cuts(0x43, SAMPLES - 8192, 2000, 65536) + [8192]  # the last push in 16 sub-blocks
     got = _run(dev, stream, cuts, excise=False)
     _check(stream, got, want, "comb")
     # the last push's projections: the project twin over its raw samples
     proj, start, count = got[4]
     assert (start, count) == (SAMPLES - cuts[-1], cuts[-1])
+    subs = min(count & -count, 16)
+    assert proj.shape == (len(COMB), subs)
     cap = 1 << (SAMPLES - 1).bit_length()
-    p = _lib.CombParams(cap - 1, start & (cap - 1), count, len(COMB), 1, cap - 1)
+    p = _lib.CombParams(cap - 1, start & (cap - 1), count, len(COMB), subs, cap - 1)
     raw = np.zeros(cap, dtype=np.complex64)
     raw[:SAMPLES] = converted
     want_proj = oracle.comb_project(p, raw, _lines(start), comb_table())
-    assert_bit_exact(proj, want_proj, what="comb projections of the last push")
+    assert_bit_exact(proj.ravel(), want_proj, what="comb projections of the last push")
     # the spurs are gone: each comb channel's averaged power comes back to
     # within 0.5 dB of the same stream synthesised without the comb
     plain = _composition(converted, SAMPLES)[4]
