@@ -716,7 +716,20 @@ typedef struct rvx_chain_desc {
     const struct rvx_iq_correct_params* correction;
     const struct rvx_iq_correct_params* correction_before;
     uint64_t correction_from;
+    /* Samples from one channel's ring row to the next: 0 is out_ring_blocks
+     * (the reference's channel_ring_index, pfb_fft_reference.h:157-162);
+     * otherwise at least out_ring_blocks and a multiple of 4. Row r starts
+     * at r * ring_pitch, and ring_samples must cover
+     * (rows - 1) * ring_pitch + out_ring_blocks. A power-of-two row length
+     * puts every channel's write of one block on the same few HBM channels;
+     * padding each row by 32 blocks (256 bytes) spreads them
+     * (RVX_RING_PITCH_PAD). */
+    uint32_t ring_pitch;
+    uint32_t reserved;
 } rvx_chain_desc;
+
+/* The row padding (blocks) the benchmark and the engine's padded layout use. */
+#define RVX_RING_PITCH_PAD 32
 
 /* taps * 2^-15 exactly (host arrays), for a cs16 chain with taps_scaled = 1:
  * the conversion's scale moves into the taps and the products are unchanged
@@ -1270,6 +1283,10 @@ typedef struct rvx_engine_config {
     uint32_t calibration;      /* RVX_CALIBRATE_DC | RVX_CALIBRATE_IQ: front-end correction */
     uint64_t start_index;      /* absolute index of the first sample pushed */
     int64_t sample_rate;       /* Hz; the estimator's time constants (calibration only) */
+    uint32_t ring_pitch;       /* channel ring row stride in samples: 0 is ring_blocks,
+                                * else >= ring_blocks and a multiple of 4
+                                * (rvx_chain_desc.ring_pitch) */
+    uint32_t reserved;
 } rvx_engine_config;
 
 /* Engine::set_calibration's two halves (core/engine/engine.h, the front-end
@@ -1328,6 +1345,8 @@ typedef struct rvx_engine_grid {
     uint32_t decimation;       /* D, dividing M */
     uint32_t ring_blocks;      /* channel ring depth, a power of two */
     uint32_t max_rows;         /* channels it stores at once, 1..M */
+    uint32_t ring_pitch;       /* row stride in samples: 0 is ring_blocks (as
+                                * rvx_engine_config.ring_pitch) */
 } rvx_engine_grid;
 int32_t rvx_engine_add_grid(rvx_engine* engine, const rvx_engine_grid* grid,
                             const float* prototype, uint64_t n_prototype, uint32_t* grid_id);
@@ -1370,8 +1389,8 @@ int32_t rvx_engine_set_comb(rvx_engine* engine, const rvx_comb_line* lines, uint
 int32_t rvx_engine_comb_projections(rvx_engine* engine, float* out, uint64_t n_floats,
                                     uint64_t* first_sample, uint64_t* samples, uint32_t* lines,
                                     uint32_t* sub_blocks);
-/* Grid `grid`'s channel ring (device, row-major, max_rows x ring_blocks cf32;
- * grid 0: the engine's channel ring). */
+/* Grid `grid`'s channel ring (device, row-major: row r's ring_blocks cf32
+ * start at r * ring_pitch; max_rows rows; grid 0: the engine's channel ring). */
 const void* rvx_engine_grid_ring(rvx_engine* engine, uint32_t grid);
 /* The ring row holding `channel` on grid `grid` and the first block written
  * into it (grid 0: row = channel, the engine's first block). */
@@ -1421,7 +1440,8 @@ int32_t rvx_engine_read_audio(rvx_engine* engine, uint32_t id, float* out, uint6
                               uint64_t* first_audio, uint64_t* values);
 /* The averaged spectrum state (M floats, host) after the last row; waits. */
 int32_t rvx_engine_spectrum(rvx_engine* engine, float* out, uint64_t n);
-/* The channel ring (device, channel-major, M x ring_blocks cf32). */
+/* The channel ring (device, channel-major: channel k's ring_blocks cf32 start
+ * at k * ring_pitch). */
 const void* rvx_engine_channel_ring(rvx_engine* engine);
 int32_t rvx_engine_stats_get(rvx_engine* engine, rvx_engine_stats* out);
 

@@ -220,7 +220,10 @@ class ChainDesc(C.Structure):
                 ("first_block", u64), ("block_count", u32), ("out_ring_blocks", u32),
                 ("frame_blocks", u32), ("alpha", f32), ("ring_rows", u32), ("taps_scaled", u32),
                 ("channel_rows", vp), ("correction", vp), ("correction_before", vp),
-                ("correction_from", u64)]
+                ("correction_from", u64), ("ring_pitch", u32), ("reserved", u32)]
+
+
+RING_PITCH_PAD = 32  # RVX_RING_PITCH_PAD
 
 
 class VrxRequest(C.Structure):
@@ -268,7 +271,8 @@ class EngineConfig(C.Structure):
     _fields_ = [("format", i32), ("channels", u32), ("taps_per_branch", u32),
                 ("decimation", u32), ("max_push", u32), ("ring_blocks", u32),
                 ("frame_blocks", u32), ("row_frames", u32), ("alpha", f32), ("calibration", u32),
-                ("start_index", u64), ("sample_rate", i64)]
+                ("start_index", u64), ("sample_rate", i64), ("ring_pitch", u32),
+                ("reserved", u32)]
 
 
 class EngineStats(C.Structure):
@@ -278,7 +282,7 @@ class EngineStats(C.Structure):
 
 class EngineGrid(C.Structure):
     _fields_ = [(n, u32) for n in ("channels", "taps_per_branch", "decimation", "ring_blocks",
-                                   "max_rows")]
+                                   "max_rows", "ring_pitch")]
 
 
 class DetectConfig(C.Structure):

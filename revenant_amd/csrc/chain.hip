@@ -48,7 +48,7 @@ struct ChainArgs {
     uint32_t taps_per_branch;
     uint32_t block_count;
     uint32_t ring_mask;
-    uint32_t ring_blocks;
+    uint32_t ring_pitch;  // samples from one ring row to the next
     uint32_t chunk;   // blocks per workgroup (the averaging frame when averaging)
     uint32_t chunks;  // workgroups
     uint32_t average; // nonzero: write the frame partials
@@ -446,7 +446,7 @@ chain_kernel(ChainArgs a) {
                 stage[(b0 % STAGE) * M + k] = value;
             } else {
                 const uint32_t row = ring_row(a, k);
-                if (active && row < a.ring_rows) a.ring[uint64_t(row) * a.ring_blocks + slot] = value;
+                if (active && row < a.ring_rows) a.ring[uint64_t(row) * a.ring_pitch + slot] = value;
             }
             x[r] = value.x * value.x + value.y * value.y;
         }
@@ -464,7 +464,7 @@ chain_kernel(ChainArgs a) {
                     const uint32_t k = F::template pos<LAST>(t, r);
                     const uint32_t ring_r = ring_row(a, k);
                     if (ring_r >= a.ring_rows) continue;
-                    float2* row = a.ring + uint64_t(ring_r) * a.ring_blocks;
+                    float2* row = a.ring + uint64_t(ring_r) * a.ring_pitch;
                     if (whole) {
                         const float2 v0 = stage[0 * M + k], v1 = stage[1 * M + k];
                         const float2 v2 = stage[2 * M + k], v3 = stage[3 * M + k];
@@ -707,7 +707,7 @@ chain_pair_kernel(ChainArgs a) {
             }
             const uint32_t ring_r = ring_row(a, k);
             if (ring_r >= a.ring_rows || active == 0) continue;
-            float2* row = a.ring + uint64_t(ring_r) * a.ring_blocks;
+            float2* row = a.ring + uint64_t(ring_r) * a.ring_pitch;
             if (whole) {
 #pragma unroll
                 for (int b = 0; b < C; b += 2) {
@@ -867,10 +867,14 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
     const uint32_t rows = d.channel_rows ? d.ring_rows : M;
     if (d.channel_rows && (d.ring_rows == 0 || d.ring_rows > M))
         return rvx::fail(RVX_EINVAL, "%s: %u ring rows for %u channels", who, d.ring_rows, M);
-    if (ring_samples < uint64_t(rows) * d.out_ring_blocks)
+    const uint32_t pitch = d.ring_pitch ? d.ring_pitch : d.out_ring_blocks;
+    if (pitch < d.out_ring_blocks || (d.ring_pitch && d.ring_pitch % 4))
+        return rvx::fail(RVX_EINVAL, "%s: ring pitch %u (at least %u blocks, a multiple of 4)", who,
+                         d.ring_pitch, d.out_ring_blocks);
+    const uint64_t ring_need = uint64_t(rows - 1) * pitch + d.out_ring_blocks;
+    if (ring_samples < ring_need)
         return rvx::fail(RVX_EINVAL, "%s: channel ring holds %llu samples, needs %llu", who,
-                         (unsigned long long)ring_samples,
-                         (unsigned long long)(uint64_t(rows) * d.out_ring_blocks));
+                         (unsigned long long)ring_samples, (unsigned long long)ring_need);
     const uint64_t N = uint64_t(M) * T;
     // byte offsets into the input are 32-bit
     const uint64_t max_in = d.format == RVX_FORMAT_CF32 ? (uint64_t(1) << 29) : (uint64_t(1) << 31);
@@ -914,7 +918,7 @@ extern "C" int32_t rvx_chain(rvx_ctx* ctx, const rvx_chain_desc* desc, const voi
     a.taps_per_branch = T;
     a.block_count = d.block_count;
     a.ring_mask = d.out_ring_blocks - 1;
-    a.ring_blocks = d.out_ring_blocks;
+    a.ring_pitch = pitch;
     a.chunk = chunk;
     a.chunks = (d.block_count + chunk - 1) / chunk;
     a.average = d.frame_blocks != 0;

@@ -278,7 +278,7 @@ struct EngineGrid {
     float* taps = nullptr;
     void* tw = nullptr;
     bool taps_scaled = false;
-    void* ring = nullptr;          // max_rows x ring_blocks cf32, row-major
+    void* ring = nullptr;          // max_rows rows of ring_blocks cf32, ring_pitch apart
     uint32_t* rows_d = nullptr;    // ring row of each channel (max_rows: not stored)
     uint32_t* rows_h = nullptr;    // pinned staging of rows_d's uploads
     std::vector<uint32_t> row_of;  // host copy of rows_d
@@ -450,8 +450,8 @@ Source source_of(const rvx_engine& e, const EngineRx& r) {
     if (r.grid == 0) {
         src.ring = e.ring;
         src.blocks = e.cfg.ring_blocks;
-        src.ring_n = uint64_t(e.cfg.channels) * src.blocks;
-        src.base = r.plan.placement.channel * src.blocks;
+        src.ring_n = uint64_t(e.cfg.channels) * e.cfg.ring_pitch;
+        src.base = r.plan.placement.channel * e.cfg.ring_pitch;
         src.max_blocks = e.max_blocks;
         src.first = e.first_block;
         src.any = e.any;
@@ -460,8 +460,8 @@ Source source_of(const rvx_engine& e, const EngineRx& r) {
         const EngineGrid& g = *e.grids[r.grid - 1];
         src.ring = g.ring;
         src.blocks = g.cfg.ring_blocks;
-        src.ring_n = uint64_t(g.cfg.max_rows) * src.blocks;
-        src.base = r.row * src.blocks;
+        src.ring_n = uint64_t(g.cfg.max_rows) * g.cfg.ring_pitch;
+        src.base = r.row * g.cfg.ring_pitch;
         src.max_blocks = g.max_blocks;
         src.first = g.row_first[r.row];
         src.any = g.next_block > src.first;
@@ -990,6 +990,9 @@ int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const f
     if (c.max_push == 0 || !rvx::pow2(c.ring_blocks))
         return rvx::fail(RVX_EINVAL, "%s: max_push must be > 0 and ring_blocks a power of two",
                          who);
+    if (c.ring_pitch && (c.ring_pitch < c.ring_blocks || c.ring_pitch % 4))
+        return rvx::fail(RVX_EINVAL, "%s: ring pitch %u (at least %u, a multiple of 4)", who,
+                         c.ring_pitch, c.ring_blocks);
     const uint32_t unit = c.frame_blocks ? c.frame_blocks * std::max(1u, c.row_frames) : 1;
     if (c.frame_blocks && c.row_frames == 0)
         return rvx::fail(RVX_EINVAL, "%s: row_frames must be >= 1", who);
@@ -1001,6 +1004,7 @@ int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const f
     e->ctx = ctx;
     e->cfg = c;
     e->cfg.row_frames = std::max(1u, c.row_frames);
+    e->cfg.ring_pitch = c.ring_pitch ? c.ring_pitch : c.ring_blocks;
     e->N = uint64_t(M) * L;
     e->bps = c.format == RVX_FORMAT_CS16 ? 4 : 2;
     // Blocks one push can complete: its own, one straddling the previous end,
@@ -1032,9 +1036,9 @@ int32_t rvx_engine_create(rvx_ctx* ctx, const rvx_engine_config* config, const f
     if (!rc) rc = device_alloc(&e->nco, nco.size() * sizeof(float), who);
     if (!rc) rc = device_alloc(&e->win, e->win_cap * e->bps, who);
     if (!rc) rc = device_alloc(&e->bounce, (e->N + uint64_t(e->max_blocks + 1) * D) * e->bps, who);
-    if (!rc) rc = device_alloc(&e->ring, uint64_t(M) * c.ring_blocks * 8, who);
+    if (!rc) rc = device_alloc(&e->ring, uint64_t(M) * e->cfg.ring_pitch * 8, who);
     // cleared, so a window reaching below the stream's first block reads zeros
-    if (!rc && hipMemset(e->ring, 0, uint64_t(M) * c.ring_blocks * 8) != hipSuccess)
+    if (!rc && hipMemset(e->ring, 0, uint64_t(M) * e->cfg.ring_pitch * 8) != hipSuccess)
         rc = rvx::fail(RVX_EDEVICE, "%s: clearing the channel ring failed", who);
     if (!rc) rc = device_alloc(&e->counter, sizeof(uint32_t), who);
     if (!rc && c.calibration) {
@@ -1194,7 +1198,7 @@ int32_t rvx_engine_add_vrx_on(rvx_engine* e, uint32_t grid, const rvx_vrx_plan* 
         // A row's stream starts at the grid's next block, on cleared slots,
         // so a window reaching below it reads zeros as on a new stream.
         if ((rc = rvx_buf_clear(e->ctx, static_cast<char*>(g->ring) +
-                                            uint64_t(row) * g->cfg.ring_blocks * 8,
+                                            uint64_t(row) * g->cfg.ring_pitch * 8,
                                 uint64_t(g->cfg.ring_blocks) * 8))) {
             free_rx(r);
             return rc;
@@ -1235,8 +1239,12 @@ int32_t rvx_engine_add_grid(rvx_engine* e, const rvx_engine_grid* config, const 
     if (!rvx::pow2(c.ring_blocks) || c.max_rows == 0 || c.max_rows > M)
         return rvx::fail(RVX_EINVAL, "%s: ring_blocks must be a power of two and max_rows in "
                          "1..%u", who, M);
+    if (c.ring_pitch && (c.ring_pitch < c.ring_blocks || c.ring_pitch % 4))
+        return rvx::fail(RVX_EINVAL, "%s: ring pitch %u (at least %u, a multiple of 4)", who,
+                         c.ring_pitch, c.ring_blocks);
     auto g = std::make_unique<EngineGrid>();
     g->cfg = c;
+    g->cfg.ring_pitch = c.ring_pitch ? c.ring_pitch : c.ring_blocks;
     g->N = uint64_t(M) * L;
     g->proto.assign(prototype, prototype + n_prototype);
     g->max_blocks = uint32_t(std::min<uint64_t>(uint64_t(e->cfg.max_push) / D + 2, 0xFFFFFFFFull));
@@ -1261,7 +1269,7 @@ int32_t rvx_engine_add_grid(rvx_engine* e, const rvx_engine_grid* config, const 
         upload = scaled.data();
         g->taps_scaled = true;
     }
-    const uint64_t ring_bytes = uint64_t(c.max_rows) * c.ring_blocks * 8;
+    const uint64_t ring_bytes = uint64_t(c.max_rows) * g->cfg.ring_pitch * 8;
     if (!rc) rc = device_alloc(&g->taps, n_prototype * sizeof(float), who);
     if (!rc) rc = device_alloc(&g->tw, tw.size() * sizeof(float), who);
     if (!rc) rc = device_alloc(&g->ring, ring_bytes, who);
@@ -1819,6 +1827,7 @@ static int32_t grid_push(rvx_engine& e, EngineGrid& g, uint64_t stop, bool* ran,
         d.first_block = g.next_block;
         d.block_count = uint32_t(count);
         d.out_ring_blocks = g.cfg.ring_blocks;
+        d.ring_pitch = g.cfg.ring_pitch;
         d.ring_rows = g.cfg.max_rows;
         d.channel_rows = g.rows_d;
         d.taps_scaled = g.taps_scaled ? 1u : 0u;
@@ -1829,7 +1838,7 @@ static int32_t grid_push(rvx_engine& e, EngineGrid& g, uint64_t stop, bool* ran,
         }
         if (int32_t rc = rvx_chain(e.ctx, &d, e.condition ? e.cwin : e.win, g.taps, g.N, g.tw, M,
                                    g.ring,
-                                   uint64_t(g.cfg.max_rows) * g.cfg.ring_blocks, nullptr, 0))
+                                   uint64_t(g.cfg.max_rows) * g.cfg.ring_pitch, nullptr, 0))
             return rc;
         e.stats.grid_blocks += count;
         *ran = true;
@@ -1937,6 +1946,7 @@ static int32_t main_push(rvx_engine* e, uint64_t count, const char* who) {
     d.first_block = e->next_block;
     d.block_count = uint32_t(count);
     d.out_ring_blocks = e->cfg.ring_blocks;
+    d.ring_pitch = e->cfg.ring_pitch;
     d.frame_blocks = e->cfg.frame_blocks;
     d.alpha = e->cfg.alpha;
     d.taps_scaled = e->taps_scaled ? 1u : 0u;
@@ -1948,7 +1958,7 @@ static int32_t main_push(rvx_engine* e, uint64_t count, const char* who) {
     const uint64_t frames = e->cfg.frame_blocks ? count / e->cfg.frame_blocks : 0;
     if (int32_t rc = rvx_chain(e->ctx, &d, e->condition ? e->cwin : e->win, e->taps, e->N, e->tw, M,
                                e->ring,
-                               uint64_t(M) * e->cfg.ring_blocks, e->partial, frames * M))
+                               uint64_t(M) * e->cfg.ring_pitch, e->partial, frames * M))
         return rc;
     if (frames) {
         rvx_ema_params p{};

@@ -23,16 +23,21 @@ class Engine:
     def __init__(self, dev: Device, grid: GridParams, prototype: np.ndarray, fmt: int,
                  max_push: int, ring_blocks: int, frame_blocks: int = 0, row_frames: int = 1,
                  alpha: float = 1.0 / 16.0, start_index: int = 0, calibration: int = 0,
-                 sample_rate: int = 0):
+                 sample_rate: int = 0, ring_pitch: int = 0):
         """calibration: CALIBRATE_DC | CALIBRATE_IQ turns on the front-end
-        correction stage (sample_rate in Hz then sets its time constants)."""
+        correction stage (sample_rate in Hz then sets its time constants).
+        ring_pitch: the channel ring's row stride in samples (0: ring_blocks,
+        the reference's layout; ring_blocks + _lib.RING_PITCH_PAD spreads a
+        block's writes over the HBM channels)."""
         self.dev = dev
         self.lib = _lib.lib()
         self.grid = grid
         self.fmt = fmt
         cfg = _lib.EngineConfig(fmt, grid.channels, grid.taps_per_branch, grid.decimation,
                                 max_push, ring_blocks, frame_blocks, row_frames,
-                                np.float32(alpha), calibration, start_index, sample_rate)
+                                np.float32(alpha), calibration, start_index, sample_rate,
+                                ring_pitch)
+        self.ring_pitch = ring_pitch or ring_blocks
         taps = np.ascontiguousarray(prototype, dtype=np.float32)
         h = C.c_void_p()
         _lib.check(self.lib.rvx_engine_create(dev.ctx, C.byref(cfg), taps.ctypes.data_as(
@@ -54,11 +59,12 @@ class Engine:
         return rid.value
 
     def add_grid(self, grid: GridParams, prototype: np.ndarray, ring_blocks: int,
-                 max_rows: int) -> int:
+                 max_rows: int, ring_pitch: int = 0) -> int:
         """A second grid on the same stream (rvx_engine_add_grid) storing up
-        to max_rows of its channels; returns its id."""
+        to max_rows of its channels, rows ring_pitch samples apart (0:
+        ring_blocks); returns its id."""
         cfg = _lib.EngineGrid(grid.channels, grid.taps_per_branch, grid.decimation, ring_blocks,
-                              max_rows)
+                              max_rows, ring_pitch)
         taps = np.ascontiguousarray(prototype, dtype=np.float32)
         gid = _lib.u32()
         _lib.check(self.lib.rvx_engine_add_grid(self.handle, C.byref(cfg),

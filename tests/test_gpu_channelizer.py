@@ -290,6 +290,71 @@ def test_chain_stores_selected_channels(dev, grid, blocks):
 
 
 
+# (format, grid, blocks, frame, selected rows, padding): the pair-streaming
+# kernel with prescaled taps, the general kernel, and a selected-row ring.
+PITCH_CASES = [
+    (FORMAT_CS16, GridParams(4096, 16, 2048), 24, 4, None, 32),
+    (FORMAT_CU8, GridParams(64, 16, 64), 40, 4, None, 4),
+    (FORMAT_CS16, GridParams(512, 16, 256), 48, 0, [511, 3, 0, 257, 77], 32),
+]
+
+
+@pytest.mark.parametrize("fmt,grid,blocks,frame,chosen,pad", PITCH_CASES,
+                         ids=["M4096-pair-pad32", "M64-general-pad4", "M512-rows-pad32"])
+def test_chain_ring_pitch_keeps_rows_bit_exact(dev, fmt, grid, blocks, frame, chosen, pad):
+    """Rows ring_blocks + pad apart: every row holds the twins' channel bits,
+    and the padding between rows is never written."""
+    M, D, N = grid.channels, grid.decimation, grid.prototype_length
+    first_block = (N + D - 1) // D + 3
+    samples = (first_block + blocks) * D + 5
+    packed = _codes(fmt, samples, SEED_CHAIN + 11 * M)
+    taps = design_prototype(grid, beta=8.0).taps
+    tw = build_twiddles(M)
+    ring_blocks = 1 << (blocks - 1).bit_length()
+    pitch = ring_blocks + pad
+    alpha = np.float32(1.0 / 16.0)
+    od = oracle.OChain(fmt, M, grid.taps_per_branch, D, first_block, blocks, ring_blocks, frame,
+                       alpha, ema_frame_decay(alpha, frame) if frame else 0.0)
+    ring_cpu, partial_cpu = oracle.chain(od, packed, taps, tw, threads=4)
+    full = ring_cpu.reshape(M, ring_blocks, 2)
+    desc = _lib.ChainDesc(fmt, M, grid.taps_per_branch, D, 0, samples, first_block, blocks,
+                          ring_blocks, frame, alpha)
+    desc.ring_pitch = pitch
+    use = taps
+    if fmt == FORMAT_CS16 and chosen is None:
+        desc.taps_scaled = 1
+        use = prescale_taps(taps)
+    rows_dev = None
+    n_rows = M
+    if chosen is not None:
+        rows = np.full(M, 0xFFFFFFFF, dtype=np.uint32)
+        for r, k in enumerate(chosen):
+            rows[k] = r
+        rows_dev = to_dev(rows.view(np.int32))
+        desc.ring_rows = len(chosen)
+        desc.channel_rows = rows_dev.data_ptr()
+        n_rows = len(chosen)
+    ring = torch.full((2 * n_rows * pitch,), 3.0, dtype=torch.float32, device="cuda")
+    partial = (torch.zeros((blocks // frame) * M, dtype=torch.float32, device="cuda")
+               if frame else None)
+    print(f"ring pitch M={M} rows={n_rows} ring_blocks={ring_blocks} pitch={pitch}", flush=True)
+    dev.chain(desc, to_dev(packed), to_dev(use), to_dev(tw.view(np.float32)), ring, partial)
+    got = from_dev(ring).reshape(n_rows, pitch, 2)
+    slots = [(first_block + b) & (ring_blocks - 1) for b in range(blocks)]
+    channels = chosen if chosen is not None else list(range(M))
+    for r, k in enumerate(channels):
+        assert_bit_exact(got[r, slots], full[k, slots], SEED_CHAIN, f"row {r} = channel {k}")
+    assert np.all(got[:, ring_blocks:] == 3.0), "the padding between rows was written"
+    if frame:
+        assert_bit_exact(from_dev(partial), partial_cpu, SEED_CHAIN, "spectrum with a pitch")
+    # a pitch below the row length or off a multiple of 4 is refused
+    for bad in (ring_blocks - 4, ring_blocks + 2):
+        desc.ring_pitch = bad
+        with pytest.raises(_lib.RvxError, match="pitch"):
+            dev.chain(desc, to_dev(packed), to_dev(use), to_dev(tw.view(np.float32)), ring,
+                      partial)
+
+
 @pytest.mark.parametrize("grid,blocks", [(GridParams(4096, 16, 2048), 32),
                                          (GridParams(512, 16, 256), 64),
                                          (GridParams(1024, 17, 512), 24)],

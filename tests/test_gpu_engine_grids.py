@@ -60,13 +60,14 @@ def stream():
     return codes.view(np.uint32), main_plans, wide_plans
 
 
-def _engine(dev, max_rows=4, ring_blocks=1 << 15):
+def _engine(dev, max_rows=4, ring_blocks=1 << 15, pad=0):
     from revenant_amd.engine import Engine
 
     eng = Engine(dev, MAIN, design_prototype(MAIN, beta=8.0).taps, FORMAT_CS16, max_push=65536,
-                 ring_blocks=4096, frame_blocks=4, row_frames=2)
+                 ring_blocks=4096, frame_blocks=4, row_frames=2,
+                 ring_pitch=4096 + pad if pad else 0)
     gid = eng.add_grid(WIDE, design_prototype(WIDE, beta=8.0).taps, ring_blocks=ring_blocks,
-                       max_rows=max_rows)
+                       max_rows=max_rows, ring_pitch=ring_blocks + pad if pad else 0)
     return eng, gid
 
 
@@ -218,3 +219,35 @@ def test_add_grid_refuses_what_the_window_cannot_hold(dev):
     with pytest.raises(_lib.RvxError):  # max_rows beyond the channel count
         eng.add_grid(WIDE, design_prototype(WIDE, beta=8.0).taps, ring_blocks=1 << 15, max_rows=9)
     eng.close()
+
+
+def test_padded_ring_rows_change_no_output(dev, stream, wide_oracle):
+    """Both grids' rows ring_blocks + 32 apart: the added grid's receivers
+    still equal the oracle, and the engine's own receivers and channel ring
+    equal an unpadded engine's bit for bit."""
+    import ctypes as C
+
+    codes, main_plans, wide_plans = stream
+    first, blocks, cap, ring = wide_oracle
+    runs = []
+    for pad in (0, _lib.RING_PITCH_PAD):
+        eng, gid = _engine(dev, pad=pad)
+        assert eng.ring_pitch == 4096 + pad
+        ids = [eng.add(p, audio_capacity=1 << 16, grid=gid) for p in wide_plans]
+        main_ids = [eng.add(p, audio_capacity=1 << 16) for p in main_plans]
+        for at in range(0, SAMPLES, 50_000):
+            eng.push(codes[at:at + 50_000])
+        wide_audio = [eng.read_audio(i) for i in ids]
+        main_audio = [eng.read_audio(i) for i in main_ids]
+        pitch = eng.ring_pitch
+        host = np.zeros(2 * MAIN.channels * pitch, dtype=np.float32)
+        _lib.check(dev.lib.rvx_buf_read(dev.ctx, host.ctypes.data,
+                                        C.c_void_p(eng.channel_ring()), host.nbytes))
+        runs.append((wide_audio, main_audio, host.reshape(MAIN.channels, pitch, 2)[:, :4096]))
+        eng.close()
+    for v, (p, (a0, got)) in enumerate(zip(wide_plans, runs[1][0])):
+        _audio_vs_oracle(p, ring, cap, a0, got, f"padded wide receiver {v} vs oracle")
+    for (a0, x), (b0, y) in zip(runs[0][1], runs[1][1]):
+        assert a0 == b0
+        assert_bit_exact(y, x, what="main receiver, padded vs unpadded ring")
+    assert_bit_exact(runs[1][2], runs[0][2], what="channel ring rows, padded vs unpadded")
