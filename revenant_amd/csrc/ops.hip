@@ -336,8 +336,26 @@ __global__ void ema_partial_kernel(const float2* __restrict__ ring, rvx_ema_para
 
 // Row partials, in place into the rows buffer: S_r from zero over its frames.
 __global__ void ema_rows_kernel(const float* __restrict__ partial, rvx_ema_params p,
-                                float* __restrict__ rows, uint32_t nrows) {
+                                float* __restrict__ rows, uint32_t nrows, bool quad) {
     const uint64_t id = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (quad) {
+        // Four consecutive channels per lane, 16-byte loads and stores; each
+        // channel's sum is the same chain of operations.
+        const uint32_t quads = p.channels / 4;
+        if (id >= uint64_t(nrows) * quads) return;
+        const uint32_t r = uint32_t(id / quads), k = uint32_t(id % quads) * 4;
+        float4 S = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (uint32_t g = 0; g < p.row_frames; ++g) {
+            const float4 v = *reinterpret_cast<const float4*>(
+                partial + uint64_t(r * p.row_frames + g) * p.channels + k);
+            S.x = S.x * p.frame_decay + v.x;
+            S.y = S.y * p.frame_decay + v.y;
+            S.z = S.z * p.frame_decay + v.z;
+            S.w = S.w * p.frame_decay + v.w;
+        }
+        *reinterpret_cast<float4*>(rows + uint64_t(r) * p.channels + k) = S;
+        return;
+    }
     if (id >= uint64_t(nrows) * p.channels) return;
     const uint32_t r = uint32_t(id / p.channels), k = uint32_t(id % p.channels);
     float S = 0.0f;
@@ -347,25 +365,38 @@ __global__ void ema_rows_kernel(const float* __restrict__ partial, rvx_ema_param
 }
 
 // The serial part, one step per row: P = P * row_decay + S_r. Row values are
-// fetched kBatch at a time so the recurrence does not wait on each load.
+// fetched kBatch at a time, the next batch's loads in flight while this
+// batch's steps run, so the recurrence waits on neither.
 __global__ void ema_combine_kernel(rvx_ema_params p, float* __restrict__ state,
                                    float* __restrict__ rows, uint32_t nrows) {
     constexpr uint32_t kBatch = 16;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= p.channels) return;
     float P = state[k];
-    uint32_t r = 0;
-    for (; r + kBatch <= nrows; r += kBatch) {
-        float S[kBatch];
+    const uint32_t whole = nrows / kBatch * kBatch;
+    float S[2][kBatch];
+    if (whole) {
 #pragma unroll
-        for (uint32_t i = 0; i < kBatch; ++i) S[i] = rows[uint64_t(r + i) * p.channels + k];
+        for (uint32_t i = 0; i < kBatch; ++i) S[0][i] = rows[uint64_t(i) * p.channels + k];
+    }
+    for (uint32_t r = 0; r < whole; r += 2 * kBatch) {
 #pragma unroll
-        for (uint32_t i = 0; i < kBatch; ++i) {
-            P = P * p.row_decay + S[i];
-            rows[uint64_t(r + i) * p.channels + k] = P;
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t at = r + uint32_t(h) * kBatch;
+            if (at >= whole) break;
+            if (at + kBatch < whole) {
+#pragma unroll
+                for (uint32_t i = 0; i < kBatch; ++i)
+                    S[h ^ 1][i] = rows[uint64_t(at + kBatch + i) * p.channels + k];
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < kBatch; ++i) {
+                P = P * p.row_decay + S[h][i];
+                rows[uint64_t(at + i) * p.channels + k] = P;
+            }
         }
     }
-    for (; r < nrows; ++r) {
+    for (uint32_t r = whole; r < nrows; ++r) {
         P = P * p.row_decay + rows[uint64_t(r) * p.channels + k];
         rows[uint64_t(r) * p.channels + k] = P;
     }
@@ -630,8 +661,12 @@ int32_t rvx_ema_combine(rvx_ctx* ctx, const rvx_ema_params* params, const float*
     if (!partial || !state) return rvx::fail(RVX_EINVAL, "%s: null buffer", who);
     if (nrows == 0) return RVX_OK;
     const uint64_t n = uint64_t(nrows) * p.channels;
-    ema_rows_kernel<<<dim3(uint32_t((n + 255) / 256)), dim3(256), 0, ctx->stream>>>(partial, p,
-                                                                                    rows, nrows);
+    // four channels a lane when rows of channels split into 16-byte runs
+    const bool quad = p.channels % 4 == 0 && reinterpret_cast<uintptr_t>(partial) % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(rows) % 16 == 0;
+    const uint64_t lanes = quad ? n / 4 : n;
+    ema_rows_kernel<<<dim3(uint32_t((lanes + 255) / 256)), dim3(256), 0, ctx->stream>>>(
+        partial, p, rows, nrows, quad);
     ema_combine_kernel<<<dim3((p.channels + 63) / 64), dim3(64), 0, ctx->stream>>>(p, state, rows,
                                                                                   nrows);
     return rvx::check_launch(who);

@@ -42,7 +42,8 @@ def test_levels_match_twin(dev, bins):
 
 
 @pytest.mark.parametrize("M,frame_blocks,frames,row_frames",
-                         [(64, 16, 32, 1), (4096, 4, 64, 4), (5, 3, 7, 7), (33, 2, 60, 3)])
+                         [(64, 16, 32, 1), (4096, 4, 64, 4), (5, 3, 7, 7), (33, 2, 60, 3),
+                          (64, 2, 200, 2), (8, 1, 1000, 10)])
 def test_ema_matches_twin(dev, M, frame_blocks, frames, row_frames):
     blocks = 1 << (frame_blocks * frames - 1).bit_length()
     ring = adversarial(SEED_SPECTRUM + M, M * blocks)
