@@ -236,7 +236,8 @@ def test_recovery_finalises_a_segment_a_dead_writer_left_open(tmp_path):
 
 
 def test_retention_deletes_the_oldest_closed_segments_first(tmp_path):
-    w = CaptureWriter(str(tmp_path), max_segment_bytes=8192, retention_bytes=3 * 8192,
+    # room for three segments with their metas
+    w = CaptureWriter(str(tmp_path), max_segment_bytes=8192, retention_bytes=3 * (8192 + 2048),
                       stream_epoch=1)
     for k in range(6):
         w.write(cu8_block(k * 4096, 4096, 10_000_000, k), RATE, "cu8", ANCHOR)
