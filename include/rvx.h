@@ -956,6 +956,91 @@ int32_t rvx_comb_subtract(rvx_ctx* ctx, const rvx_comb_params* params, const voi
                           const void* table, uint64_t n_table, void* cleaned,
                           uint64_t cleaned_samples, void* removed, uint64_t removed_samples);
 
+/* ---- Comb cancellation (host half) -------------------------------------------
+ * core/dsp/comb_cancel.h: detect_comb/detect_combs (comb_detect.cpp), the
+ * CombTracker (comb_tracker.cpp) and the CombBank (comb_bank.cpp). The bank
+ * plans each block's lines for rvx_comb_project/subtract (or
+ * rvx_engine_set_comb) and reads back that block's projections. */
+#define RVX_COMB_MIN_LINES 8            /* comb_cancel.h:112 */
+#define RVX_COMB_MAX_COMBS 8            /* :126 */
+#define RVX_COMB_MAX_SPURS 16           /* :152 */
+#define RVX_COMB_SAME_LINE_HZ 45.0      /* :147 */
+#define RVX_COMB_OVERLAP_FRACTION 0.25  /* :141 */
+
+/* comb_cancel.h:310 CombLock */
+#define RVX_COMB_IDLE 0
+#define RVX_COMB_ACQUIRING 1
+#define RVX_COMB_LOCKED 2
+#define RVX_COMB_LOST 3
+#define RVX_COMB_RECOVERING 4
+
+/* comb_cancel.h:161-199 CombEstimate: line k at offset_hz + k * spacing_hz,
+ * k in [k_lo, k_hi]; a spur is one line at offset_hz (k_lo = k_hi = 0). */
+typedef struct rvx_comb_estimate {
+    double offset_hz;
+    double spacing_hz;
+    int32_t k_lo, k_hi;
+    uint32_t lines_found;
+    float median_height_db;
+    double ripple_hz;          /* sidebands at multiples of it either side of each line */
+    int32_t sideband_orders;
+    uint32_t spur;
+    double curvature_hz;       /* always 0: the straight model (see DESIGN) */
+} rvx_comb_estimate;
+
+/* comb_cancel.h:318-331 CombStatus, over the bank's combs (spurs counted apart). */
+typedef struct rvx_comb_status {
+    uint32_t lock;                /* RVX_COMB_*: locked if any comb is */
+    uint32_t lines_cancelled;
+    float suppression_db;         /* mean over cancelled lines, from the projections */
+    float spacing_drift_hz_per_s;
+    rvx_comb_estimate comb;       /* the comb cancelling the most lines */
+    uint32_t combs_active;
+    uint32_t spurs_cancelled;
+    float residual_rad;           /* its phase fit's weighted RMS residual */
+    uint32_t reserved;
+} rvx_comb_status;
+
+/* comb_cancel.h:225-231 detect_combs: every comb in a linear power spectrum of
+ * `bins` bins in natural order (DC at bins / 2, bin i centred on
+ * (i - bins / 2) * rate / bins), at most RVX_COMB_MAX_COMBS, strongest first.
+ * `halves` 1 holds each comb as its even and odd halves at twice the spacing
+ * (detect_combs); 0 reports a comb whole unless its halves disagree
+ * (detect_comb takes the first). `active`: combs already held, left out.
+ * Refuses a spectrum of under 64 bins, a rate <= 0 or a bin that is not a
+ * finite non-negative power. */
+int32_t rvx_comb_detect(const float* power, uint64_t bins, int64_t rate,
+                        const rvx_comb_estimate* active, uint32_t n_active, uint32_t halves,
+                        rvx_comb_estimate* out, uint32_t max_out, uint32_t* n_out);
+
+typedef struct rvx_comb_bank rvx_comb_bank;
+/* comb_cancel.h:601-678 CombBank. Single-threaded: one caller drives it. */
+int32_t rvx_comb_bank_create(int64_t rate, rvx_comb_bank** out);
+void rvx_comb_bank_destroy(rvx_comb_bank* bank);
+int32_t rvx_comb_bank_reset(rvx_comb_bank* bank);
+/* New detections: each not held by a running tracker starts in a free one
+ * with a share of the lines no tracker holds. */
+int32_t rvx_comb_bank_detected(rvx_comb_bank* bank, const rvx_comb_estimate* estimates,
+                               uint32_t n);
+/* Single carriers to cancel (CombBank::spurs), each in a one-line tracker. */
+int32_t rvx_comb_bank_spurs(rvx_comb_bank* bank, const rvx_comb_estimate* estimates, uint32_t n);
+int32_t rvx_comb_bank_drop_spurs(rvx_comb_bank* bank);
+/* The lines of block [start, start + count) with their phases at `start`:
+ * every tracker's in turn; *subtract when any is cancelling (the others'
+ * amplitudes are zero); *sub_blocks the count the trackers would like. */
+int32_t rvx_comb_bank_plan(rvx_comb_bank* bank, uint64_t start, uint32_t count,
+                           rvx_comb_line* lines, uint32_t max_lines, uint32_t* n_lines,
+                           uint32_t* sub_blocks, int32_t* subtract);
+/* A planned block's projections: lines x sub_blocks cf32, line-major, the
+ * lines in plan order (rvx_comb_project's or rvx_engine_comb_projections'
+ * layout); sub_blocks >= 2 for the frequency discriminator. */
+int32_t rvx_comb_bank_observe(rvx_comb_bank* bank, uint64_t start, uint32_t count,
+                              uint32_t sub_blocks, const float* projections, uint64_t n_floats);
+int32_t rvx_comb_bank_status(const rvx_comb_bank* bank, rvx_comb_status* out);
+/* Every running comb tracker's grid (CombBank::active_estimates). */
+int32_t rvx_comb_bank_active(const rvx_comb_bank* bank, rvx_comb_estimate* out, uint32_t max_out,
+                             uint32_t* n_out);
+
 /* ---- Per-receiver impulse blanker --------------------------------------------
  * core/dsp/noise_reference.h:59-190: impulses blanked on a receiver's coarse
  * channel at the channel rate, before its fine stage narrows them into

@@ -123,6 +123,22 @@ class CombLine(C.Structure):
                 ("amplitude", f32 * 2)]
 
 
+class CombEstimate(C.Structure):
+    _fields_ = [("offset_hz", f64), ("spacing_hz", f64), ("k_lo", i32), ("k_hi", i32),
+                ("lines_found", u32), ("median_height_db", f32), ("ripple_hz", f64),
+                ("sideband_orders", i32), ("spur", u32), ("curvature_hz", f64)]
+
+
+class CombStatus(C.Structure):
+    _fields_ = [("lock", u32), ("lines_cancelled", u32), ("suppression_db", f32),
+                ("spacing_drift_hz_per_s", f32), ("comb", CombEstimate), ("combs_active", u32),
+                ("spurs_cancelled", u32), ("residual_rad", f32), ("reserved", u32)]
+
+
+COMB_IDLE, COMB_ACQUIRING, COMB_LOCKED, COMB_LOST, COMB_RECOVERING = 0, 1, 2, 3, 4
+COMB_MIN_LINES, COMB_MAX_COMBS, COMB_MAX_LINES = 8, 8, 1024
+
+
 class CombParams(C.Structure):
     _fields_ = [(n, u32) for n in ("ring_mask", "first", "count", "lines", "sub_blocks",
                                    "removed_mask", "pad0", "pad1")]
@@ -455,6 +471,19 @@ _SIGNATURES = {
                                    C.POINTER(SpectralParams)]),
     "rvx_design_deemphasis": (i32, [u32, i64, vp, u32, C.POINTER(u32)]),
     "rvx_comb_step": (u32, [f64, i64]),
+    "rvx_comb_detect": (i32, [fptr, u64, i64, C.POINTER(CombEstimate), u32, u32,
+                              C.POINTER(CombEstimate), u32, C.POINTER(u32)]),
+    "rvx_comb_bank_create": (i32, [i64, C.POINTER(vp)]),
+    "rvx_comb_bank_destroy": (None, [vp]),
+    "rvx_comb_bank_reset": (i32, [vp]),
+    "rvx_comb_bank_detected": (i32, [vp, C.POINTER(CombEstimate), u32]),
+    "rvx_comb_bank_spurs": (i32, [vp, C.POINTER(CombEstimate), u32]),
+    "rvx_comb_bank_drop_spurs": (i32, [vp]),
+    "rvx_comb_bank_plan": (i32, [vp, u64, u32, C.POINTER(CombLine), u32, C.POINTER(u32),
+                                 C.POINTER(u32), C.POINTER(i32)]),
+    "rvx_comb_bank_observe": (i32, [vp, u64, u32, u32, fptr, u64]),
+    "rvx_comb_bank_status": (i32, [vp, C.POINTER(CombStatus)]),
+    "rvx_comb_bank_active": (i32, [vp, C.POINTER(CombEstimate), u32, C.POINTER(u32)]),
     "rvx_comb_table": (i32, [fptr, u64]),
     "rvx_comb_check": (i32, [C.POINTER(CombParams)]),
     "rvx_comb_project": (i32, [vp, C.POINTER(CombParams), vp, u64, vp, u64, vp, u64, vp, u64]),

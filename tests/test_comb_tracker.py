@@ -1,0 +1,120 @@
+"""The comb bank in a closed loop on the CPU (rvx_comb_bank_*; the tracker of
+core/dsp/comb_tracker.cpp and the bank of comb_bank.cpp): the drifting comb of
+tests/engine/test_engine_comb.cpp detected, locked and taken out of the
+spectrum, the kernels' projections restated in numpy. The same loop through
+the engine on the GPU is tests/test_gpu_comb_cancel.py."""
+import numpy as np
+import pytest
+
+import comb_loop as cl
+from revenant_amd import _lib
+from revenant_amd.comb import CombBank, detect_combs, spur
+from revenant_amd.dsp import comb_table
+
+SECONDS = 4.0
+MEASURE_FROM = 3.0
+
+
+@pytest.fixture(scope="module")
+def capture():
+    return cl.render(SECONDS)
+
+
+def test_a_drifting_comb_is_detected_locked_and_taken_out(capture):
+    table = comb_table()
+    bank = CombBank(cl.RATE)
+    found_any = []
+
+    def detect(last_second):
+        found = detect_combs(cl.spectrum(last_second, 16384), cl.RATE, bank.active())
+        found_any.append(len(found))
+        bank.detected(found)
+
+    cleaned = cl.run(capture, bank, table, detect=detect)
+    st = bank.status()
+    # up to the end of the last whole block the loop ran
+    end = (len(capture) // 65536) * 65536 / cl.RATE
+    off = cl.line_height_db(capture, MEASURE_FROM, t1=end)
+    on = cl.line_height_db(cleaned, MEASURE_FROM, t1=end)
+    print(f"comb off {off:.1f} dB, on {on:.1f} dB; lock {st.lock} lines {st.lines_cancelled} "
+          f"suppression {st.suppression_db:.1f} dB spacing {st.comb.spacing_hz:.3f} "
+          f"offset {st.comb.offset_hz:.3f} drift {st.spacing_drift_hz_per_s:.3f} Hz/s "
+          f"residual {st.residual_rad:.3f} rad; detections {found_any}")
+    assert off > 10.0
+    assert found_any and found_any[0] > 0
+    assert st.lock == _lib.COMB_LOCKED
+    assert st.lines_cancelled >= _lib.COMB_MIN_LINES
+    spacing = st.comb.spacing_hz
+    assert abs(spacing - cl.spacing_at(SECONDS)) < 5.0 or \
+        abs(spacing - 2 * cl.spacing_at(SECONDS)) < 10.0
+    assert on < off - 10.0 and on < 6.0
+    assert st.suppression_db > 6.0
+
+
+def test_a_named_spur_is_locked_and_cancelled():
+    f = 23_456.7
+    x = cl.tone_spur(3.0, f)
+    table = comb_table()
+    bank = CombBank(cl.RATE)
+    bank.spurs([spur(f + 20.0)])   # named from a coarse detection, 20 Hz off
+    cleaned = cl.run(x, bank, table)
+    st = bank.status()
+    assert st.spurs_cancelled == 1
+    nfft = 4096
+    b = int(round(f / (cl.RATE / nfft))) + nfft // 2
+    end = len(x) // 65536 * 65536  # the last whole block the loop ran
+    raw = cl.spectrum(x[2 * cl.RATE:end], nfft)
+    out = cl.spectrum(cleaned[2 * cl.RATE:end], nfft)
+    before = 10 * np.log10(raw[b] / np.median(raw))
+    after = 10 * np.log10(out[b] / np.median(out))
+    print(f"spur: {before:.1f} dB over the floor before, {after:.1f} dB after")
+    assert before > 10.0 and after < 3.0
+    bank.drop_spurs()
+    assert bank.status().spurs_cancelled == 0
+
+
+def test_noise_alone_starts_nothing_and_a_restart_forgets():
+    x = cl.render(1.5, lines=False)
+    table = comb_table()
+    bank = CombBank(cl.RATE)
+    found = detect_combs(cl.spectrum(x[: cl.RATE], 16384), cl.RATE)
+    assert found == []
+    bank.detected(found)
+    lines, n, subtract = bank.plan(0, 65536)
+    assert n == 0 and not subtract
+    assert bank.status().lock == _lib.COMB_IDLE
+    # a comb that is not there: acquisition never locks, and reset clears it
+    e = _lib.CombEstimate()
+    e.offset_hz, e.spacing_hz, e.k_lo, e.k_hi = 1000.0, 15_000.0, -10, 10
+    bank.detected([e])
+    assert bank.status().lock == _lib.COMB_ACQUIRING
+    cleaned = cl.run(x, bank, table)
+    assert bank.status().lock != _lib.COMB_LOCKED
+    assert np.array_equal(cleaned, x)  # nothing subtracted while not locked
+    bank.reset()
+    assert bank.status().lock == _lib.COMB_IDLE
+
+
+def test_observations_split_back_to_their_trackers():
+    """Two combs in two trackers: one plan holds both trackers' lines in turn,
+    and the observation of that block is split back by the counts."""
+    bank = CombBank(cl.RATE)
+    a, b = _lib.CombEstimate(), _lib.CombEstimate()
+    a.offset_hz, a.spacing_hz, a.k_lo, a.k_hi = 1_000.0, 15_000.0, -10, 10
+    b.offset_hz, b.spacing_hz, b.k_lo, b.k_hi = 3_333.0, 22_930.0, -5, 5
+    bank.detected([a, b])
+    assert bank.status().combs_active == 2
+    lines, n, _ = bank.plan(0, 65536)
+    assert n == 21 + 11
+    steps = [ln.step for ln in lines[:n]]
+    want = [_lib.lib().rvx_comb_step(a.offset_hz + k * a.spacing_hz, cl.RATE)
+            for k in range(-10, 11)] + \
+           [_lib.lib().rvx_comb_step(b.offset_hz + k * b.spacing_hz, cl.RATE)
+            for k in range(-5, 6)]
+    assert [abs(int(np.int32(np.uint32(s - w)))) <= 1 for s, w in zip(steps, want)] == [True] * n
+    # a second detection of a held comb is dropped
+    bank.detected([a])
+    assert bank.status().combs_active == 2
+    # projections of the wrong size are refused
+    with pytest.raises(_lib.RvxError):
+        bank.observe(0, 65536, np.zeros((n - 1, 16), np.complex64))
