@@ -8,9 +8,9 @@
 //   conv_kernel<CIN, COUTP, MODE>  forward (MODE 0: bias + ReLU -> bf16;
 //       MODE 1: bias -> fp32 logits) and the data gradient (MODE 2: the same
 //       product with the flipped, transposed weights, times the ReLU mask of
-//       the layer's input -> bf16). A workgroup owns 64 pixels of one image
-//       row; its input rows (with the 3x3 halo) and the packed weights sit in
-//       LDS; each wave owns 16 pixels and every output channel.
+//       the layer's input -> bf16). A workgroup owns 64 pixels of four image
+//       rows; its input rows (with the 3x3 halo) sit in LDS; each wave owns
+//       one row and every output channel, its weight fragments in registers.
 //   wgrad_kernel<CIN, COUTP>  the weight and bias gradients: per workgroup a
 //       band of image rows, dW += dY^T * im2col(X) on the matrix cores,
 //       partial sums written per workgroup and added in a fixed order.
@@ -30,13 +30,18 @@ namespace {
 
 typedef short short4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kLayers = 5;
 constexpr int kWidth = 32;
-constexpr int kTileW = 64;         // pixels per workgroup in the convolutions
+constexpr int kTileW = 64;         // pixels of a row in a convolution workgroup
+constexpr int kConvRows = 4;       // rows of a convolution workgroup (one per wave)
 constexpr int kWgradRows = 8;      // image rows per workgroup in the weight gradient
 
 __host__ __device__ constexpr int round16(int v) { return (v + 15) / 16 * 16; }
+// K of every product is padded to the 32 of v_mfma_f32_16x16x32_bf16.
+__host__ __device__ constexpr int round32(int v) { return (v + 31) / 32 * 32; }
 
 struct LayerShape {
     int cin, cout, coutp, k, kp;
@@ -44,7 +49,7 @@ struct LayerShape {
 __host__ __device__ constexpr LayerShape layer_shape(int l) {
     const int cin = l == 0 ? 1 : kWidth;
     const int cout = l == kLayers - 1 ? 1 : kWidth;
-    return LayerShape{cin, cout, round16(cout), 9 * cin, round16(9 * cin)};
+    return LayerShape{cin, cout, round16(cout), 9 * cin, round32(9 * cin)};
 }
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -54,79 +59,132 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
 
+__device__ __forceinline__ f32x4 mma32(uint4 a, uint4 b, f32x4 acc) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), acc, 0, 0, 0);
+}
+
 // MODE 0: Y = relu(conv(X, W) + b) as bf16, COUT channels per pixel.
 // MODE 1: Z = conv(X, W) + b as fp32, one channel per pixel.
 // MODE 2: dX = conv(dY, W') * (mask > 0) as bf16 (W' flipped/transposed).
+//
+// A workgroup owns kConvRows image rows of kTileW pixels; wave w owns row w
+// (four 16-pixel M tiles) and every output channel. On
+// v_mfma_f32_16x16x32_bf16 a lane's eight K values are k = 32 ks + 8 g + j;
+// with the K index tap * CIN + ci and CIN a multiple of 8 they are one tap's
+// eight consecutive channels, a 16-byte read of the input tile in LDS (its
+// pixel stride padded by 8 channels so the 16 pixels of a read hit distinct
+// banks). The wave's weight fragments stay in registers for the whole tile.
+// CIN = 1 (the first layer) gathers its nine taps one value at a time.
 template <int CIN, int COUTP, int MODE>
 __global__ void __launch_bounds__(256)
 conv_kernel(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
             const float* __restrict__ bias, const uint16_t* __restrict__ mask, int cout,
             int height, int width, void* __restrict__ out) {
-    constexpr int K = 9 * CIN, KP = round16(K), NT = COUTP / 16;
-    __shared__ uint16_t tile[3][kTileW + 2][CIN];
-    __shared__ uint16_t wl[COUTP][KP];
-    const int n = blockIdx.z, y = blockIdx.y, x0 = blockIdx.x * kTileW;
+    constexpr int K = 9 * CIN, KP = round32(K), NT = COUTP / 16, KS = KP / 32;
+    constexpr bool kVec = CIN % 8 == 0;
+    constexpr int PS = kVec ? CIN + 8 : CIN;  // pixel stride in the tile, bf16
+    constexpr int TW = kTileW + 2, TR = kConvRows + 2;
+    __shared__ __align__(16) uint16_t tile[TR * TW * PS];
+    const int n = blockIdx.z, y0 = blockIdx.y * kConvRows, x0 = blockIdx.x * kTileW;
     const int tid = threadIdx.x;
     const uint16_t* img = x + size_t(n) * height * width * CIN;
-    for (int e = tid; e < 3 * (kTileW + 2) * CIN; e += 256) {
-        const int ci = e % CIN, c = (e / CIN) % (kTileW + 2), r = e / (CIN * (kTileW + 2));
-        const int yy = y + r - 1, xx = x0 + c - 1;
-        tile[r][c][ci] = (yy >= 0 && yy < height && xx >= 0 && xx < width)
-                             ? img[(size_t(yy) * width + xx) * CIN + ci]
-                             : uint16_t(0);
+    if constexpr (kVec) {
+        constexpr int V = CIN / 8;  // 16-byte pieces per pixel
+        for (int e = tid; e < TR * TW * V; e += 256) {
+            const int v = e % V, c = (e / V) % TW, r = e / (V * TW);
+            const int yy = y0 + r - 1, xx = x0 + c - 1;
+            uint4 q = make_uint4(0, 0, 0, 0);
+            if (yy >= 0 && yy < height && xx >= 0 && xx < width)
+                q = *reinterpret_cast<const uint4*>(img + (size_t(yy) * width + xx) * CIN + 8 * v);
+            *reinterpret_cast<uint4*>(tile + (r * TW + c) * PS + 8 * v) = q;
+        }
+    } else {
+        for (int e = tid; e < TR * TW * CIN; e += 256) {
+            const int ci = e % CIN, c = (e / CIN) % TW, r = e / (CIN * TW);
+            const int yy = y0 + r - 1, xx = x0 + c - 1;
+            tile[(r * TW + c) * PS + ci] = (yy >= 0 && yy < height && xx >= 0 && xx < width)
+                                               ? img[(size_t(yy) * width + xx) * CIN + ci]
+                                               : uint16_t(0);
+        }
     }
-    for (int e = tid; e < COUTP * KP; e += 256) wl[e / KP][e % KP] = w[e];
-    __syncthreads();
-
     const int wave = tid / 64, lane = tid % 64;
     const int row = lane % 16, grp = lane / 16;
-    const int m = wave * 16 + row;  // this lane's pixel in the A operand
-    float4_t acc[NT];
+    // this wave's weight fragments: column co = 16 nt + row, k = 32 ks + 8 grp + j
+    uint4 wf[NT][KS];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = float4_t{0.f, 0.f, 0.f, 0.f};
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
-    for (int ks = 0; ks < KP / 16; ++ks) {
-        short4_t a;
+        for (int ks = 0; ks < KS; ++ks)
+            wf[t][ks] = *reinterpret_cast<const uint4*>(w + size_t(t * 16 + row) * KP + ks * 32 +
+                                                         grp * 8);
+    __syncthreads();
+
+    const int y = y0 + wave;
+    f32x4 acc[4][NT];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = ks * 16 + grp * 4 + j;
-            uint16_t v = 0;
-            if (k < K) {
-                const int ci = k % CIN, tap = k / CIN;
-                v = tile[tap / 3][m + tap % 3][ci];
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[mt][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            const int m = mt * 16 + row;  // pixel of this lane's A row
+            uint4 a;
+            if constexpr (kVec) {
+                const int k0 = ks * 32 + grp * 8;
+                const int tap = k0 / CIN, ci0 = k0 % CIN;
+                if (k0 < K)
+                    a = *reinterpret_cast<const uint4*>(
+                        tile + ((wave + tap / 3) * TW + m + tap % 3) * PS + ci0);
+                else
+                    a = make_uint4(0, 0, 0, 0);
+            } else {
+                uint16_t v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int k = ks * 32 + grp * 8 + j;
+                    v[j] = 0;
+                    if (k < K) {
+                        const int ci = k % CIN, tap = k / CIN;
+                        v[j] = tile[((wave + tap / 3) * TW + m + tap % 3) * PS + ci];
+                    }
+                }
+                a = make_uint4(uint32_t(v[0]) | (uint32_t(v[1]) << 16),
+                               uint32_t(v[2]) | (uint32_t(v[3]) << 16),
+                               uint32_t(v[4]) | (uint32_t(v[5]) << 16),
+                               uint32_t(v[6]) | (uint32_t(v[7]) << 16));
             }
-            a[j] = short(v);
-        }
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            short4_t b;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) b[j] = short(wl[t * 16 + row][ks * 16 + grp * 4 + j]);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            for (int t = 0; t < NT; ++t) acc[mt][t] = mma32(a, wf[t][ks], acc[mt][t]);
         }
     }
+    if (y >= height) return;
     // D layout: lane holds rows 4*grp + j (pixels) of column `row` (channel).
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + row;
-        if (co >= cout) continue;
+    for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int xx = x0 + wave * 16 + grp * 4 + j;
-            if (xx >= width) continue;
-            const size_t p = (size_t(n) * height + y) * width + xx;
-            float v = acc[t][j];
-            if constexpr (MODE == 0) {
-                v = v + bias[co];
-                static_cast<uint16_t*>(out)[p * cout + co] = f2bf(v > 0.f ? v : 0.f);
-            } else if constexpr (MODE == 1) {
-                static_cast<float*>(out)[p] = v + bias[co];
-            } else {
-                const bool on = bf2f(mask[p * cout + co]) > 0.f;
-                static_cast<uint16_t*>(out)[p * cout + co] = f2bf(on ? v : 0.f);
+        for (int t = 0; t < NT; ++t) {
+            const int co = t * 16 + row;
+            if (co >= cout) continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int xx = x0 + mt * 16 + grp * 4 + j;
+                if (xx >= width) continue;
+                const size_t p = (size_t(n) * height + y) * width + xx;
+                float v = acc[mt][t][j];
+                if constexpr (MODE == 0) {
+                    v = v + bias[co];
+                    static_cast<uint16_t*>(out)[p * cout + co] = f2bf(v > 0.f ? v : 0.f);
+                } else if constexpr (MODE == 1) {
+                    static_cast<float*>(out)[p] = v + bias[co];
+                } else {
+                    const bool on = bf2f(mask[p * cout + co]) > 0.f;
+                    static_cast<uint16_t*>(out)[p * cout + co] = f2bf(on ? v : 0.f);
+                }
             }
         }
-    }
 }
 
 // Partial weight/bias gradients of one band of kWgradRows image rows:
@@ -136,7 +194,7 @@ template <int CIN, int COUTP>
 __global__ void __launch_bounds__(256)
 wgrad_kernel(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy, int height,
              int width, float* __restrict__ part, float* __restrict__ bias_part) {
-    constexpr int K = 9 * CIN, KP = round16(K), MT = COUTP / 16, NTILES = KP / 16;
+    constexpr int K = 9 * CIN, KP = round32(K), MT = COUTP / 16, NTILES = KP / 16;
     constexpr int TILES = MT * NTILES;
     constexpr int PER_WAVE = (TILES + 3) / 4;
     __shared__ uint16_t xin[3][kTileW + 2][CIN];
@@ -176,21 +234,27 @@ wgrad_kernel(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy, in
                 if (tile_id >= TILES) break;
                 const int mt = tile_id / NTILES, nt = tile_id % NTILES;
 #pragma unroll
-                for (int ks = 0; ks < kTileW / 16; ++ks) {
-                    short4_t a, b;
+                for (int ks = 0; ks < kTileW / 32; ++ks) {
+                    uint32_t a[4], b[4];
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) {
-                        const int p = ks * 16 + grp * 4 + j;  // pixel (the GEMM's K)
-                        a[j] = short(dyl[p][mt * 16 + row]);
-                        const int k = nt * 16 + row;
-                        uint16_t v = 0;
-                        if (k < K) {
-                            const int ci = k % CIN, tap = k / CIN;
-                            v = xin[tap / 3][p + tap % 3][ci];
+                    for (int j = 0; j < 8; j += 2) {
+                        uint16_t av[2], bv[2];
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            const int p = ks * 32 + grp * 8 + j + h;  // pixel (the GEMM's K)
+                            av[h] = dyl[p][mt * 16 + row];
+                            const int k = nt * 16 + row;
+                            bv[h] = 0;
+                            if (k < K) {
+                                const int ci = k % CIN, tap = k / CIN;
+                                bv[h] = xin[tap / 3][p + tap % 3][ci];
+                            }
                         }
-                        b[j] = short(v);
+                        a[j / 2] = uint32_t(av[0]) | (uint32_t(av[1]) << 16);
+                        b[j / 2] = uint32_t(bv[0]) | (uint32_t(bv[1]) << 16);
                     }
-                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i], 0, 0, 0);
+                    acc[i] = mma32(make_uint4(a[0], a[1], a[2], a[3]),
+                                   make_uint4(b[0], b[1], b[2], b[3]), acc[i]);
                 }
             }
         }
@@ -303,7 +367,9 @@ __global__ void to_bf16_kernel(const float* __restrict__ in, int count, uint16_t
 template <int CIN, int COUTP, int MODE>
 void conv(const uint16_t* x, const uint16_t* w, const float* b, const uint16_t* mask, int cout,
           int batch, int h, int wd, void* out, hipStream_t s) {
-    conv_kernel<CIN, COUTP, MODE><<<dim3((wd + kTileW - 1) / kTileW, h, batch), dim3(256), 0, s>>>(
+    conv_kernel<CIN, COUTP, MODE><<<dim3((wd + kTileW - 1) / kTileW, (h + kConvRows - 1) / kConvRows,
+                                         batch),
+                                    dim3(256), 0, s>>>(
         x, w, b, mask, cout, h, wd, out);
 }
 
@@ -348,7 +414,7 @@ int32_t repack(rvx_net& n, hipStream_t s) {
         pack_kernel<<<dim3((sh.coutp * sh.kp + 255) / 256), dim3(256), 0, s>>>(
             n.params + n.woff[l], sh.cin, sh.cout, sh.coutp, sh.kp, n.packed[l]);
         if (l > 0) {
-            const int cinp = round16(sh.cin), kpp = round16(9 * sh.coutp);
+            const int cinp = round16(sh.cin), kpp = round32(9 * sh.coutp);
             flip_kernel<<<dim3((cinp * kpp + 255) / 256), dim3(256), 0, s>>>(
                 n.params + n.woff[l], sh.cin, sh.cout, sh.coutp, cinp, kpp, n.flipped[l]);
         }
@@ -419,7 +485,7 @@ int32_t rvx_net_create(rvx_ctx* ctx, uint32_t height, uint32_t width, uint32_t m
     for (int l = 0; l < kLayers; ++l) {
         const LayerShape sh = layer_shape(l);
         alloc(&n->packed[l], size_t(sh.coutp) * sh.kp * 2);
-        if (l > 0) alloc(&n->flipped[l], size_t(round16(sh.cin)) * round16(9 * sh.coutp) * 2);
+        if (l > 0) alloc(&n->flipped[l], size_t(round16(sh.cin)) * round32(9 * sh.coutp) * 2);
         alloc(&n->act[l], pix * size_t(l == 0 ? 1 : kWidth) * 2);
     }
     alloc(&n->logits, pix * sizeof(float));
