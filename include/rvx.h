@@ -608,7 +608,7 @@ int32_t rvx_vrx_desc_check(const rvx_vrx_desc* desc);
 
 /* Fine samples one batched work item holds on chip: a receiver's tile of n
  * audio outputs needs (n - 1) * R + hist + 1 <= RVX_VRX_TILE_SPAN. */
-#define RVX_VRX_TILE_SPAN 2048
+#define RVX_VRX_TILE_SPAN 3072
 
 /* One receiver's share of a batched launch: audio outputs first_audio ..
  * first_audio + count - 1, written to `audio` (count * channels floats). */

@@ -343,7 +343,7 @@ class TrackerStats(C.Structure):
                                    "births_refused", "merges", "splits")]
 
 
-VRX_TILE_SPAN = 2048  # RVX_VRX_TILE_SPAN
+VRX_TILE_SPAN = 3072  # RVX_VRX_TILE_SPAN
 PREC_BF16X3, PREC_FP16 = 1, 2  # RVX_PREC_*
 
 assert C.sizeof(VrxDesc) == 160 and C.sizeof(VrxWork) == 8 and C.sizeof(VrxSpan) == 64

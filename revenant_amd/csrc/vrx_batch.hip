@@ -31,13 +31,18 @@ static __device__ __align__(256) uint32_t scratch[64];
 namespace {
 
 constexpr uint32_t kBatchThreads = 256;
-constexpr uint32_t kMaxSpan = 2048;     // fine samples per tile: fine (8 B) + detector (4 B)
+// Fine samples per tile, 8 B each. The detector's values (and AM's
+// magnitudes) reuse the same bytes once the fine samples are read: the
+// magnitudes in the lower half as floats, the detector output in the upper.
+constexpr uint32_t kMaxSpan = RVX_VRX_TILE_SPAN;
 constexpr uint32_t kTapEntries = 2040;  // interpolated taps of every phase class (8 B each)
 constexpr uint32_t kPerThread = kMaxSpan / kBatchThreads;
+static_assert(kMaxSpan % kBatchThreads == 0, "whole detector rounds per thread");
 // Global memory seen through the constant address space: the caller's
 // promise that nothing writes it while the kernel runs.
 typedef const __attribute__((address_space(4))) float kconst_float;
 // 40,900 bytes of LDS: four workgroups fit a CU's 160 KiB.
+static_assert(kTapEntries * 8 + kMaxSpan * 8 + 4 == 40900, "LDS budget of four workgroups a CU");
 static_assert(sizeof(rvx_vrx_desc) == 160, "descriptor layout is part of the ABI");
 
 __host__ __device__ bool complex_tap(uint32_t mode) { return mode == 0 || (mode >= 8 && mode <= 11); }
@@ -56,9 +61,9 @@ vrx_batch_kernel(const rvx_vrx_desc* __restrict__ descs, uint32_t n_desc,
                  const float2* __restrict__ nco, uint32_t nco_log2_max) {
     __shared__ float2 tap_lds[kTapEntries];
     __shared__ float2 fine[kMaxSpan];
-    __shared__ float det[kMaxSpan];
     __shared__ uint32_t item_sh;
     float* mag = reinterpret_cast<float*>(fine);  // AM magnitudes replace the fine samples
+    float* det = mag + kMaxSpan;                  // and the detector output follows them
     const uint32_t tid = threadIdx.x;
 
     for (;;) {
@@ -168,16 +173,32 @@ vrx_batch_kernel(const rvx_vrx_desc* __restrict__ descs, uint32_t n_desc,
                 }
                 det[s] = mag[s + dh] - acc;
             }
-        } else if (dc.mode == RVX_DEMOD_NFM || dc.mode == RVX_DEMOD_WFM) {
-            for (uint32_t s = tid; s < det_span; s += kBatchThreads) {
-                const float2 z = fine[s + dh];
-                const float2 q = fine[s + dh - 1];
-                const float pr = z.x * q.x + z.y * q.y;
-                const float pi = z.y * q.x - z.x * q.y;
-                det[s] = rvx::det_atan2(pi, pr);
-            }
         } else {
-            for (uint32_t s = tid; s < det_span; s += kBatchThreads) det[s] = fine[s].x;
+            // every fine sample is read before the detector output overwrites it
+            const bool fm = dc.mode == RVX_DEMOD_NFM || dc.mode == RVX_DEMOD_WFM;
+            float v[kPerThread];
+#pragma unroll
+            for (uint32_t q = 0; q < kPerThread; ++q) {
+                const uint32_t s = tid + q * kBatchThreads;
+                v[q] = 0.0f;
+                if (s < det_span) {
+                    if (fm) {
+                        const float2 z = fine[s + dh];
+                        const float2 p = fine[s + dh - 1];
+                        const float pr = z.x * p.x + z.y * p.y;
+                        const float pi = z.y * p.x - z.x * p.y;
+                        v[q] = rvx::det_atan2(pi, pr);
+                    } else {
+                        v[q] = fine[s].x;
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t q = 0; q < kPerThread; ++q) {
+                const uint32_t s = tid + q * kBatchThreads;
+                if (s < det_span) det[s] = v[q];
+            }
         }
         __syncthreads();
 

@@ -147,32 +147,82 @@ __device__ __forceinline__ float2 fine_output_row(const rvx_vrx_fine_config& c,
     const uint32_t n_rel = i * p.step_whole + carry;
     const uint32_t base = p.in_offset + n_rel;
     // The ring through the global address space: its reads then wait on the
-    // vector-memory counter alone, not on the LDS reads of the taps as well,
-    // and eight are in flight per wait.
+    // vector-memory counter alone, not on the LDS reads of the taps as well.
     typedef float ring_f2 __attribute__((ext_vector_type(2)));
-    const __attribute__((address_space(1))) ring_f2* gch =
-        reinterpret_cast<const __attribute__((address_space(1))) ring_f2*>(uintptr_t(ch));
+    typedef const __attribute__((address_space(1))) ring_f2* ring_ptr;
+    const ring_ptr gch = reinterpret_cast<ring_ptr>(uintptr_t(ch));
     float ar = 0.0f, ai = 0.0f;
     uint32_t k = 0;
-    for (; k + 8 <= c.taps; k += 8) {
-        float2 s[8], t[8];
+    const uint32_t b0 = base & p.chan_mask;
+    if (b0 >= c.taps + 7u) {
+        // The window does not wrap: tap k reads q[-k], one pointer with
+        // immediate offsets, and each group of eight taps' reads is issued
+        // while the group before it is summed (the last reads run up to 15
+        // samples past the window, still inside the ring, and go unused). The
+        // sums run in the same order.
+        const ring_ptr q = gch + b0;
+        auto group = [&](const ring_f2 (&s)[8], uint32_t k0) {
+            float2 t[8];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const ring_f2 v = gch[(base - (k + u)) & p.chan_mask];
-            s[u] = make_float2(v.x, v.y);
-            t[u] = row[k + u];
-        }
+            for (int u = 0; u < 8; ++u) t[u] = row[k0 + u];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            ar = ar + (s[u].x * t[u].x - s[u].y * t[u].y);
-            ai = ai + (s[u].x * t[u].y + s[u].y * t[u].x);
+            for (int u = 0; u < 8; ++u) {
+                ar = ar + (s[u].x * t[u].x - s[u].y * t[u].y);
+                ai = ai + (s[u].x * t[u].y + s[u].y * t[u].x);
+            }
+        };
+        auto fetch = [&](ring_f2 (&s)[8], uint32_t k0) {
+            const ring_ptr qk = q - k0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s[u] = qk[-u];
+        };
+        // two buffers in turn: group k's samples are summed while group
+        // k + 8's are read
+        ring_f2 sa[8], sb[8];
+        fetch(sa, 0);
+        for (; k + 16 <= c.taps; k += 16) {
+            fetch(sb, k + 8);
+            group(sa, k);
+            fetch(sa, k + 16);
+            group(sb, k + 8);
         }
-    }
-    for (; k < c.taps; ++k) {
-        const float2 s = ch[(base - k) & p.chan_mask];
-        const float2 t = row[k];
-        ar = ar + (s.x * t.x - s.y * t.y);
-        ai = ai + (s.x * t.y + s.y * t.x);
+        if (k + 8 <= c.taps) {
+            fetch(sb, k + 8);
+            group(sa, k);
+            k += 8;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) sa[u] = sb[u];
+        }
+        // sa holds taps k .. k + 7; the tail uses those below c.taps
+#pragma unroll
+        for (int u = 0; u < 7; ++u) {
+            if (k + uint32_t(u) < c.taps) {
+                const float2 t = row[k + u];
+                ar = ar + (sa[u].x * t.x - sa[u].y * t.y);
+                ai = ai + (sa[u].x * t.y + sa[u].y * t.x);
+            }
+        }
+    } else {
+        for (; k + 8 <= c.taps; k += 8) {
+            float2 s[8], t[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const ring_f2 v = gch[(base - (k + u)) & p.chan_mask];
+                s[u] = make_float2(v.x, v.y);
+                t[u] = row[k + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                ar = ar + (s[u].x * t[u].x - s[u].y * t[u].y);
+                ai = ai + (s[u].x * t[u].y + s[u].y * t[u].x);
+            }
+        }
+        for (; k < c.taps; ++k) {
+            const float2 s = ch[(base - k) & p.chan_mask];
+            const float2 t = row[k];
+            ar = ar + (s.x * t.x - s.y * t.y);
+            ai = ai + (s.x * t.y + s.y * t.x);
+        }
     }
     const uint32_t prod_lo = i * p.nco_delta_low;
     const uint32_t prod_hi = __umulhi(i, p.nco_delta_low);
