@@ -349,6 +349,11 @@ struct rvx_engine {
     rvx_iq_correct_params corr_now{0, 0, 0, 0.0f, 0.0f, 0.0f, 1.0f};
     uint64_t corr_from = 0;           // first sample that takes corr_now
     std::vector<std::unique_ptr<EngineGrid>> grids;  // grid k is grids[k - 1]
+    // The added grids' chains run on a stream of their own, forked after the
+    // engine's chain and joined before the receivers, so they overlap the
+    // averaged spectrum's combine (a latency-bound recurrence over few waves).
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::vector<float> proto;  // host copy of the engine's own prototype
     // The conditioned path (set before the first push by rvx_engine_set_comb
     // or rvx_engine_set_excision): each push is converted (and corrected) into
@@ -397,6 +402,7 @@ struct rvx_engine {
     }
 
     ~rvx_engine() {
+        if (aux) (void)hipStreamSynchronize(aux);
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
         for (auto& r : rx) free_rx(r);
         grids.clear();
@@ -418,6 +424,9 @@ struct rvx_engine {
         if (noise_h) (void)hipHostFree(noise_h);
         if (noise_d) (void)hipFree(noise_d);
         if (noise_staged) (void)hipEventDestroy(noise_staged);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (aux) (void)hipStreamDestroy(aux);
         if (staged) (void)hipEventDestroy(staged);
         for (int k = 0; k < kInFlight; ++k) {
             if (sums_d[k]) (void)hipFree(sums_d[k]);
@@ -1707,7 +1716,7 @@ int32_t rvx_engine_set_noise(rvx_engine* e, uint32_t id, const rvx_line_config* 
     return RVX_OK;
 }
 
-static int32_t main_push(rvx_engine* e, uint64_t count, const char* who);
+static int32_t main_push(rvx_engine* e, uint64_t count, hipEvent_t fork, const char* who);
 
 // The conditioned path over one push, in the reference's record order
 // (graph.cpp: the converted block, its correction, record_comb_cancel :1715,
@@ -1921,19 +1930,40 @@ int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int3
         count -= count % unit;
         count = std::min<uint64_t>(count, e->max_blocks);
     }
+    const bool added = !e->grids.empty();
+    if (added && !e->aux) {
+        RVX_TRY_HIP(hipStreamCreateWithFlags(&e->aux, hipStreamNonBlocking), who);
+        RVX_TRY_HIP(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming), who);
+        RVX_TRY_HIP(hipEventCreateWithFlags(&e->join, hipEventDisableTiming), who);
+    }
     if (count) {
-        if (int32_t rc = main_push(e, count, who)) return rc;
+        if (int32_t rc = main_push(e, count, added ? e->fork : nullptr, who)) return rc;
         ran = true;
     }
-    for (auto& g : e->grids)
-        if (int32_t rc = grid_push(*e, *g, stop, &ran, who)) return rc;
+    if (added) {
+        if (!count) RVX_TRY_HIP(hipEventRecord(e->fork, s), who);
+        RVX_TRY_HIP(hipStreamWaitEvent(e->aux, e->fork, 0), who);
+        // The context's calls are serialised by its caller, so it can carry
+        // the second stream for the added grids' launches.
+        e->ctx->stream = e->aux;
+        int32_t rc = RVX_OK;
+        for (auto& g : e->grids)
+            if ((rc = grid_push(*e, *g, stop, &ran, who))) break;
+        e->ctx->stream = s;
+        // joined even after an error: later work on the engine's stream stays
+        // behind whatever the second stream was given
+        RVX_TRY_HIP(hipEventRecord(e->join, e->aux), who);
+        RVX_TRY_HIP(hipStreamWaitEvent(s, e->join, 0), who);
+        if (rc) return rc;
+    }
     if (!ran) return RVX_OK;
     return engine_receivers(*e, who);
 }
 
 // The engine's own grid over `count` complete blocks: the chain, the
-// averaged spectrum's rows, and the block cursor.
-static int32_t main_push(rvx_engine* e, uint64_t count, const char* who) {
+// averaged spectrum's rows, and the block cursor. `fork` (if any) is recorded
+// between the chain and the rows.
+static int32_t main_push(rvx_engine* e, uint64_t count, hipEvent_t fork, const char* who) {
     const uint32_t D = e->cfg.decimation, M = e->cfg.channels;
 
     rvx_chain_desc d{};
@@ -1960,6 +1990,7 @@ static int32_t main_push(rvx_engine* e, uint64_t count, const char* who) {
                                e->ring,
                                uint64_t(M) * e->cfg.ring_pitch, e->partial, frames * M))
         return rc;
+    if (fork) RVX_TRY_HIP(hipEventRecord(fork, e->ctx->stream), who);
     if (frames) {
         rvx_ema_params p{};
         p.channels = M;
