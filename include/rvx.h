@@ -1041,6 +1041,50 @@ int32_t rvx_comb_bank_status(const rvx_comb_bank* bank, rvx_comb_status* out);
 int32_t rvx_comb_bank_active(const rvx_comb_bank* bank, rvx_comb_estimate* out, uint32_t max_out,
                              uint32_t* n_out);
 
+/* ---- Adaptive noise profile (host) -------------------------------------------
+ * One receiver's learned noise spectrum taken out of its mono audio on the
+ * completion thread, after the audio is read back (replaces
+ * dsp::NoiseProfile, core/dsp/noise_profile.h:54-118, which
+ * core/engine/graph.cpp:2587-2592 runs per receiver slot). A periodic
+ * sqrt-Hann STFT at half overlap (1024 points at 32 kHz and up, 512 at 16 kHz,
+ * else 256; the window pair reconstructs exactly at unit gain), a Wiener gain
+ * per bin against the noise estimate floored at -strength dB, smoothed across
+ * frames and across neighbouring bins. The estimate, in order: a profile
+ * learned on request (one second of average power, kept until a reset), the
+ * average of frames taken while the squelch is shut (auto-learn), or minimum
+ * statistics over two seconds (auto-learn, no quiet frame yet). Pure host
+ * arithmetic; one object per receiver, single-threaded. */
+#define RVX_NOISE_PROFILE_MAX_STRENGTH_DB 20.0
+typedef struct rvx_noise_profile rvx_noise_profile;
+/* NoiseProfileSettings (noise_profile.h:44-50). A change of learn_count starts
+ * a one-second learn; a change of reset_count drops the learned profile and
+ * the automatic state; the first settings a profile sees are its baseline. */
+typedef struct rvx_noise_profile_settings {
+    int32_t enabled;
+    int32_t auto_learn;
+    double strength_db; /* clamped to 0 .. RVX_NOISE_PROFILE_MAX_STRENGTH_DB */
+    uint32_t learn_count;
+    uint32_t reset_count;
+} rvx_noise_profile_settings;
+int32_t rvx_noise_profile_create(rvx_noise_profile** out);
+void rvx_noise_profile_destroy(rvx_noise_profile* profile);
+/* NoiseProfile::process: `audio` (n floats, mono) in place, `latency` samples
+ * late while enabled. `quiet`: the squelch was shut for this block. A change
+ * of rate rebuilds the transform and drops every estimate; rate <= 0 leaves
+ * the audio and the state untouched, as the reference does. */
+int32_t rvx_noise_profile_process(rvx_noise_profile* profile, float* audio, uint64_t n,
+                                  double rate, int32_t quiet,
+                                  const rvx_noise_profile_settings* settings);
+/* NoiseProfile::retune: the automatic estimate is dropped, a learned one kept. */
+int32_t rvx_noise_profile_retune(rvx_noise_profile* profile);
+/* learning(), learned(), latency() and the number of bins of profile() (0
+ * while there is no estimate; size / 2 + 1 otherwise). */
+int32_t rvx_noise_profile_state(const rvx_noise_profile* profile, int32_t* learning,
+                                int32_t* learned, uint32_t* latency, uint32_t* bins);
+/* NoiseProfile::profile: the estimate in use, power per bin (up to max_out). */
+int32_t rvx_noise_profile_estimate(const rvx_noise_profile* profile, double* out,
+                                   uint32_t max_out, uint32_t* n_out);
+
 /* ---- Per-receiver impulse blanker --------------------------------------------
  * core/dsp/noise_reference.h:59-190: impulses blanked on a receiver's coarse
  * channel at the channel rate, before its fine stage narrows them into

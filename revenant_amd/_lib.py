@@ -135,6 +135,13 @@ class CombStatus(C.Structure):
                 ("spurs_cancelled", u32), ("residual_rad", f32), ("reserved", u32)]
 
 
+class NoiseProfileSettings(C.Structure):
+    _fields_ = [("enabled", i32), ("auto_learn", i32), ("strength_db", f64),
+                ("learn_count", u32), ("reset_count", u32)]
+
+
+NOISE_PROFILE_MAX_STRENGTH_DB = 20.0  # RVX_NOISE_PROFILE_MAX_STRENGTH_DB
+
 COMB_IDLE, COMB_ACQUIRING, COMB_LOCKED, COMB_LOST, COMB_RECOVERING = 0, 1, 2, 3, 4
 COMB_MIN_LINES, COMB_MAX_COMBS, COMB_MAX_LINES = 8, 8, 1024
 
@@ -473,6 +480,14 @@ _SIGNATURES = {
     "rvx_comb_step": (u32, [f64, i64]),
     "rvx_comb_detect": (i32, [fptr, u64, i64, C.POINTER(CombEstimate), u32, u32,
                               C.POINTER(CombEstimate), u32, C.POINTER(u32)]),
+    "rvx_noise_profile_create": (i32, [C.POINTER(vp)]),
+    "rvx_noise_profile_destroy": (None, [vp]),
+    "rvx_noise_profile_process": (i32, [vp, fptr, u64, f64, i32,
+                                        C.POINTER(NoiseProfileSettings)]),
+    "rvx_noise_profile_retune": (i32, [vp]),
+    "rvx_noise_profile_state": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(u32),
+                                      C.POINTER(u32)]),
+    "rvx_noise_profile_estimate": (i32, [vp, C.POINTER(f64), u32, C.POINTER(u32)]),
     "rvx_comb_bank_create": (i32, [i64, C.POINTER(vp)]),
     "rvx_comb_bank_destroy": (None, [vp]),
     "rvx_comb_bank_reset": (i32, [vp]),
