@@ -1022,9 +1022,29 @@ int32_t rvx_comb_bank_reset(rvx_comb_bank* bank);
  * with a share of the lines no tracker holds. */
 int32_t rvx_comb_bank_detected(rvx_comb_bank* bank, const rvx_comb_estimate* estimates,
                                uint32_t n);
-/* Single carriers to cancel (CombBank::spurs), each in a one-line tracker. */
+/* Single carriers to cancel (CombBank::spurs, comb_cancel.h:614-621), each in
+ * a one-line tracker. A spur missing from a later call is not let go (once
+ * cancelled, the wideband detector no longer sees it). A spur tracker ends when
+ * it loses its line, when it is withdrawn, or when it is dropped; a spur that
+ * went is not started again within RVX_COMB_SAME_LINE_HZ of its detected
+ * offset for RVX_COMB_SPUR_HOLD_OFF_S seconds of planned blocks. */
+#define RVX_COMB_SPUR_HOLD_OFF_S 3.0
 int32_t rvx_comb_bank_spurs(rvx_comb_bank* bank, const rvx_comb_estimate* estimates, uint32_t n);
+/* CombBank::drop_spurs (comb_cancel.h:635): every spur goes, each held off. */
 int32_t rvx_comb_bank_drop_spurs(rvx_comb_bank* bank);
+/* CombBank::withdraw_spurs (comb_cancel.h:621-624): every spur detected within
+ * RVX_COMB_SAME_LINE_HZ of one of these offsets goes (the detector stopped
+ * calling the carrier there a spur), held off. */
+int32_t rvx_comb_bank_withdraw_spurs(rvx_comb_bank* bank, const double* frequencies_hz,
+                                     uint32_t n);
+/* CombBank::SpurBand and drop_spurs_in (comb_cancel.h:626-634): a baseband
+ * range an open receiver listens to, edges included; every spur detected in
+ * one goes, held off. */
+typedef struct rvx_spur_band {
+    double low_hz;
+    double high_hz;
+} rvx_spur_band;
+int32_t rvx_comb_bank_drop_spurs_in(rvx_comb_bank* bank, const rvx_spur_band* bands, uint32_t n);
 /* The lines of block [start, start + count) with their phases at `start`:
  * every tracker's in turn; *subtract when any is cancelling (the others'
  * amplitudes are zero); *sub_blocks the count the trackers would like. */

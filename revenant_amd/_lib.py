@@ -135,6 +135,13 @@ class CombStatus(C.Structure):
                 ("spurs_cancelled", u32), ("residual_rad", f32), ("reserved", u32)]
 
 
+class SpurBand(C.Structure):
+    _fields_ = [("low_hz", f64), ("high_hz", f64)]
+
+
+COMB_SPUR_HOLD_OFF_S = 3.0  # RVX_COMB_SPUR_HOLD_OFF_S
+
+
 class NoiseProfileSettings(C.Structure):
     _fields_ = [("enabled", i32), ("auto_learn", i32), ("strength_db", f64),
                 ("learn_count", u32), ("reset_count", u32)]
@@ -494,6 +501,8 @@ _SIGNATURES = {
     "rvx_comb_bank_detected": (i32, [vp, C.POINTER(CombEstimate), u32]),
     "rvx_comb_bank_spurs": (i32, [vp, C.POINTER(CombEstimate), u32]),
     "rvx_comb_bank_drop_spurs": (i32, [vp]),
+    "rvx_comb_bank_withdraw_spurs": (i32, [vp, C.POINTER(f64), u32]),
+    "rvx_comb_bank_drop_spurs_in": (i32, [vp, C.POINTER(SpurBand), u32]),
     "rvx_comb_bank_plan": (i32, [vp, u64, u32, C.POINTER(CombLine), u32, C.POINTER(u32),
                                  C.POINTER(u32), C.POINTER(i32)]),
     "rvx_comb_bank_observe": (i32, [vp, u64, u32, u32, fptr, u64]),

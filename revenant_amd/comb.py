@@ -78,6 +78,16 @@ class CombBank:
     def drop_spurs(self) -> None:
         _lib.check(self.lib.rvx_comb_bank_drop_spurs(self.handle))
 
+    def withdraw_spurs(self, frequencies_hz) -> None:
+        """Every spur detected within 45 Hz of one of these offsets goes."""
+        f = (C.c_double * max(1, len(frequencies_hz)))(*[float(x) for x in frequencies_hz])
+        _lib.check(self.lib.rvx_comb_bank_withdraw_spurs(self.handle, f, len(frequencies_hz)))
+
+    def drop_spurs_in(self, bands) -> None:
+        """bands: (low_hz, high_hz) pairs, edges included."""
+        b = (_lib.SpurBand * max(1, len(bands)))(*[_lib.SpurBand(lo, hi) for lo, hi in bands])
+        _lib.check(self.lib.rvx_comb_bank_drop_spurs_in(self.handle, b, len(bands)))
+
     def reset(self) -> None:
         _lib.check(self.lib.rvx_comb_bank_reset(self.handle))
 

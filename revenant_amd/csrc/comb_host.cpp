@@ -627,6 +627,11 @@ public:
     uint64_t id() const { return id_; }
     uint32_t lock() const { return lock_; }
     bool spur() const { return spur_; }
+    // Where the detector put it (a spur's hold-off and withdrawal go by this).
+    double detected_offset_hz() const { return est_.offset_hz; }
+    // A spur tracker that has ended since its bank last looked.
+    bool ended_spur() const { return spur_ && started_ && !active() && !end_noted_; }
+    void note_end() { end_noted_ = true; }
     bool active() const {
         return lock_ == RVX_COMB_ACQUIRING || lock_ == RVX_COMB_LOCKED || lock_ == RVX_COMB_RECOVERING;
     }
@@ -635,6 +640,8 @@ public:
     void start(const rvx_comb_estimate& e, uint32_t max_lines) {
         est_ = e;
         spur_ = e.spur != 0;
+        started_ = true;
+        end_noted_ = false;
         min_lines_ = spur_ ? 1u : uint32_t(RVX_COMB_MIN_LINES);
         lock_ = RVX_COMB_ACQUIRING;
         stage_ = kFrequency;
@@ -928,6 +935,7 @@ public:
     }
 
 private:
+    bool started_ = false, end_noted_ = false;
     enum Stage { kFrequency, kPhase };
     int64_t rate_;
     uint64_t id_;
@@ -963,6 +971,35 @@ struct rvx_comb_bank {
     std::vector<Tracker> trackers;
     std::vector<Split> splits;
     uint64_t next_id = 1;
+    // A spur that went (lost its line, withdrawn or dropped) is not started
+    // again near the same offset until `until`: a spur that will not hold
+    // would otherwise restart on every detection.
+    struct HoldOff {
+        double offset_hz;
+        uint64_t until;
+    };
+    std::vector<HoldOff> hold_offs;
+    uint64_t now = 0;  // the latest block planned
+
+    void note_ended_spurs() {
+        const uint64_t span = uint64_t(RVX_COMB_SPUR_HOLD_OFF_S * double(rate));
+        for (Tracker& t : trackers)
+            if (t.ended_spur()) {
+                hold_offs.push_back({t.detected_offset_hz(), now + span});
+                t.note_end();
+            }
+    }
+    bool held_off(double f_hz) const {
+        for (const HoldOff& h : hold_offs)
+            if (std::abs(h.offset_hz - f_hz) < RVX_COMB_SAME_LINE_HZ) return true;
+        return false;
+    }
+    template <typename Pred>
+    void stop_spurs_if(Pred drop) {
+        for (Tracker& t : trackers)
+            if (t.spur() && t.active() && drop(t.detected_offset_hz())) t.stop(RVX_COMB_IDLE);
+        note_ended_spurs();
+    }
 
     uint32_t lines_held() const {
         uint32_t n = 0;
@@ -1001,6 +1038,7 @@ int32_t rvx_comb_bank_reset(rvx_comb_bank* bank) {
     if (!bank) return rvx::fail(RVX_EINVAL, "rvx_comb_bank_reset: null bank");
     bank->trackers.clear();
     bank->splits.clear();
+    bank->hold_offs.clear();
     return RVX_OK;
 }
 
@@ -1033,11 +1071,13 @@ int32_t rvx_comb_bank_detected(rvx_comb_bank* bank, const rvx_comb_estimate* est
 int32_t rvx_comb_bank_spurs(rvx_comb_bank* bank, const rvx_comb_estimate* estimates, uint32_t n) {
     static const char* who = "rvx_comb_bank_spurs";
     if (!bank || (n && !estimates)) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    bank->note_ended_spurs();
     for (uint32_t i = 0; i < n; ++i) {
         rvx_comb_estimate e = estimates[i];
         e.spur = 1;
         e.k_lo = e.k_hi = 0;
         e.spacing_hz = 0.0;
+        if (bank->held_off(e.offset_hz)) continue;  // went a moment ago
         bool held = false;
         for (const Tracker& t : bank->trackers)
             held = held || (t.active() && t.holds(e.offset_hz, RVX_COMB_SAME_LINE_HZ));
@@ -1051,8 +1091,30 @@ int32_t rvx_comb_bank_spurs(rvx_comb_bank* bank, const rvx_comb_estimate* estima
 
 int32_t rvx_comb_bank_drop_spurs(rvx_comb_bank* bank) {
     if (!bank) return rvx::fail(RVX_EINVAL, "rvx_comb_bank_drop_spurs: null bank");
-    for (Tracker& t : bank->trackers)
-        if (t.spur() && t.active()) t.stop(RVX_COMB_IDLE);
+    bank->stop_spurs_if([](double) { return true; });
+    return RVX_OK;
+}
+
+int32_t rvx_comb_bank_withdraw_spurs(rvx_comb_bank* bank, const double* frequencies_hz,
+                                     uint32_t n) {
+    if (!bank || (n && !frequencies_hz))
+        return rvx::fail(RVX_EINVAL, "rvx_comb_bank_withdraw_spurs: null argument");
+    bank->stop_spurs_if([&](double at) {
+        for (uint32_t i = 0; i < n; ++i)
+            if (std::abs(frequencies_hz[i] - at) < RVX_COMB_SAME_LINE_HZ) return true;
+        return false;
+    });
+    return RVX_OK;
+}
+
+int32_t rvx_comb_bank_drop_spurs_in(rvx_comb_bank* bank, const rvx_spur_band* bands, uint32_t n) {
+    if (!bank || (n && !bands))
+        return rvx::fail(RVX_EINVAL, "rvx_comb_bank_drop_spurs_in: null argument");
+    bank->stop_spurs_if([&](double at) {
+        for (uint32_t i = 0; i < n; ++i)
+            if (at >= bands[i].low_hz && at <= bands[i].high_hz) return true;
+        return false;
+    });
     return RVX_OK;
 }
 
@@ -1063,6 +1125,9 @@ int32_t rvx_comb_bank_plan(rvx_comb_bank* bank, uint64_t start, uint32_t count,
     if (!bank || !n_lines || !subtract || (max_lines && !lines))
         return rvx::fail(RVX_EINVAL, "%s: null argument", who);
     if (count < 2) return rvx::fail(RVX_EINVAL, "%s: a block of %u samples", who, count);
+    bank->now = start;
+    std::erase_if(bank->hold_offs, [&](const rvx_comb_bank::HoldOff& h) { return h.until <= start; });
+    bank->note_ended_spurs();  // a spur that lost its line since the last block
     std::vector<rvx_comb_line> all;
     bool sub = false;
     Split split;

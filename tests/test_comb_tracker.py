@@ -118,3 +118,57 @@ def test_observations_split_back_to_their_trackers():
     # projections of the wrong size are refused
     with pytest.raises(_lib.RvxError):
         bank.observe(0, 65536, np.zeros((n - 1, 16), np.complex64))
+
+
+def test_a_withdrawn_spur_goes_and_is_held_off():
+    """comb_cancel.h:614-624: a spur the detector withdraws goes at once, and a
+    spur that went is not started again near its offset for three seconds of
+    planned blocks (RVX_COMB_SPUR_HOLD_OFF_S); after that it may."""
+    f = 23_456.7
+    block = 65536
+    x = cl.tone_spur(3.0, f)
+    table = comb_table()
+    bank = CombBank(cl.RATE)
+    bank.spurs([spur(f)])
+    cl.run(x, bank, table)
+    assert bank.status().spurs_cancelled == 1
+    last = (len(x) - block) // block * block  # the last block the loop planned
+    bank.withdraw_spurs([f - 500.0])          # not this one
+    assert bank.status().spurs_cancelled == 1
+    bank.withdraw_spurs([f + 30.0])           # within 45 Hz of where it was detected
+    assert bank.status().spurs_cancelled == 0
+    t = last + block
+    assert bank.plan(t, block)[1] == 0
+    bank.spurs([spur(f + 10.0)])              # offered again at once: held off
+    assert bank.plan(t + block, block)[1] == 0
+    bank.spurs([spur(f + 500.0)])             # another carrier starts
+    assert bank.plan(t + 2 * block, block)[1] == 1
+    hold = int(_lib.COMB_SPUR_HOLD_OFF_S * cl.RATE)
+    bank.plan(last + hold + block, block)     # the hold-off has run out
+    bank.spurs([spur(f)])
+    assert bank.plan(last + hold + 2 * block, block)[1] == 2
+
+
+def test_spurs_dropped_in_bands_and_dropped_all_are_held_off():
+    """drop_spurs_in (comb_cancel.h:626-634) lets go of the spurs detected in a
+    band, edges included, and keeps the rest; drop_spurs lets go of all; each
+    one dropped is held off like a withdrawn one."""
+    block = 65536
+    bank = CombBank(cl.RATE)
+    bank.spurs([spur(10_000.0), spur(50_000.0)])
+    assert bank.plan(0, block)[1] == 2
+    bank.drop_spurs_in([(45_000.0, 55_000.0), (-80_000.0, -60_000.0)])
+    assert bank.plan(block, block)[1] == 1
+    bank.spurs([spur(50_010.0)])               # held off
+    assert bank.plan(2 * block, block)[1] == 1
+    bank.drop_spurs_in([(10_000.0, 10_000.0)])  # the band's edge holds it
+    assert bank.plan(3 * block, block)[1] == 0
+    bank.spurs([spur(80_000.0)])
+    assert bank.plan(4 * block, block)[1] == 1
+    bank.drop_spurs()
+    assert bank.plan(5 * block, block)[1] == 0
+    bank.spurs([spur(80_000.0), spur(10_000.0), spur(50_000.0)])  # all three held off
+    assert bank.plan(6 * block, block)[1] == 0
+    bank.reset()                               # a reset forgets the hold-offs too
+    bank.spurs([spur(80_000.0)])
+    assert bank.plan(7 * block, block)[1] == 1
