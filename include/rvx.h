@@ -1018,8 +1018,13 @@ typedef struct rvx_comb_bank rvx_comb_bank;
 int32_t rvx_comb_bank_create(int64_t rate, rvx_comb_bank** out);
 void rvx_comb_bank_destroy(rvx_comb_bank* bank);
 int32_t rvx_comb_bank_reset(rvx_comb_bank* bank);
-/* New detections: each not held by a running tracker starts in a free one
- * with a share of the lines no tracker holds. */
+/* New detections (CombBank::detected, comb_bank.cpp): each of at least
+ * RVX_COMB_MIN_LINES lines that no running comb tracker holds starts in a
+ * free one, strongest first; combs have first claim on the
+ * RVX_COMB_MAX_LINES slots (what spurs hold counts as free), each taking its
+ * lines if the rest still fit and an even share of what is free otherwise,
+ * none under RVX_COMB_MIN_LINES; the newest spurs then give back what a
+ * starting comb took, each held off. */
 int32_t rvx_comb_bank_detected(rvx_comb_bank* bank, const rvx_comb_estimate* estimates,
                                uint32_t n);
 /* Single carriers to cancel (CombBank::spurs, comb_cancel.h:614-621), each in

@@ -632,6 +632,7 @@ public:
     // A spur tracker that has ended since its bank last looked.
     bool ended_spur() const { return spur_ && started_ && !active() && !end_noted_; }
     void note_end() { end_noted_ = true; }
+    uint64_t order = 0;  // when its bank started it (newest spurs give way first)
     bool active() const {
         return lock_ == RVX_COMB_ACQUIRING || lock_ == RVX_COMB_LOCKED || lock_ == RVX_COMB_RECOVERING;
     }
@@ -979,7 +980,8 @@ struct rvx_comb_bank {
         uint64_t until;
     };
     std::vector<HoldOff> hold_offs;
-    uint64_t now = 0;  // the latest block planned
+    uint64_t now = 0;     // the latest block planned
+    uint64_t starts = 0;  // trackers started, for Tracker::order
 
     void note_ended_spurs() {
         const uint64_t span = uint64_t(RVX_COMB_SPUR_HOLD_OFF_S * double(rate));
@@ -1046,24 +1048,59 @@ int32_t rvx_comb_bank_detected(rvx_comb_bank* bank, const rvx_comb_estimate* est
                                uint32_t n) {
     static const char* who = "rvx_comb_bank_detected";
     if (!bank || (n && !estimates)) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    // Held by a running comb tracker (spurs are not counted: one line of a
+    // comb's is not the comb); asked again before each start, so a comb this
+    // call started counts too.
+    auto held = [&](const rvx_comb_estimate& e) {
+        for (const Tracker& t : bank->trackers)
+            if (!t.spur() && t.active() &&
+                (t.agrees(e) || t.overlap(e, RVX_COMB_SAME_LINE_HZ) >= RVX_COMB_OVERLAP_FRACTION))
+                return true;
+        return false;
+    };
+    auto lines_of = [](const rvx_comb_estimate& e) { return uint32_t(e.k_hi - e.k_lo + 1); };
     std::vector<const rvx_comb_estimate*> fresh;
     for (uint32_t i = 0; i < n; ++i) {
         const rvx_comb_estimate& e = estimates[i];
-        if (e.spur || e.k_hi < e.k_lo || !(e.spacing_hz > 0) || !std::isfinite(e.offset_hz))
+        if (e.spur || e.k_hi < e.k_lo || lines_of(e) < RVX_COMB_MIN_LINES ||
+            !(e.spacing_hz > 0) || !std::isfinite(e.offset_hz))
             continue;
-        bool held = false;
-        for (const Tracker& t : bank->trackers)
-            held = held || (t.active() && (t.agrees(e) || t.overlap(e, RVX_COMB_SAME_LINE_HZ) >=
-                                                              RVX_COMB_OVERLAP_FRACTION));
-        if (!held) fresh.push_back(&e);
+        if (!held(e)) fresh.push_back(&e);
     }
     for (size_t i = 0; i < fresh.size(); ++i) {
-        const uint32_t used = bank->lines_held();
-        if (used >= RVX_COMB_MAX_LINES) break;
-        const uint32_t share = (RVX_COMB_MAX_LINES - used) / uint32_t(fresh.size() - i);
+        // Combs have first claim: lines spurs hold count as free here, and the
+        // newest spurs give back what a comb that starts takes.
+        uint32_t used = 0, combs = 0;
+        for (const Tracker& t : bank->trackers)
+            if (!t.spur() && t.active()) {
+                used += t.lines();
+                ++combs;
+            }
+        if (combs >= RVX_COMB_MAX_COMBS || used >= RVX_COMB_MAX_LINES) break;
+        // Strongest first takes what it needs if the rest still fit, else an
+        // even share of what is free.
+        const uint32_t free = RVX_COMB_MAX_LINES - used;
+        uint32_t rest = 0;
+        for (size_t j = i + 1; j < fresh.size(); ++j) rest += lines_of(*fresh[j]);
+        const uint32_t even = free / uint32_t(fresh.size() - i);
+        const uint32_t room = rest < free ? free - rest : 0u;
+        const uint32_t share = std::min(lines_of(*fresh[i]), std::max(even, room));
+        if (share < RVX_COMB_MIN_LINES) break;
+        if (held(*fresh[i])) continue;
         Tracker* t = bank->free_slot(false);
-        if (!t || share == 0) break;
+        if (!t) break;
         t->start(*fresh[i], share);
+        t->order = ++bank->starts;
+        uint32_t total = bank->lines_held();
+        while (total > RVX_COMB_MAX_LINES) {
+            Tracker* newest = nullptr;
+            for (Tracker& s : bank->trackers)
+                if (s.spur() && s.active() && (!newest || s.order > newest->order)) newest = &s;
+            if (!newest) break;
+            total -= newest->lines();
+            newest->stop(RVX_COMB_IDLE);
+        }
+        bank->note_ended_spurs();
     }
     return RVX_OK;
 }
@@ -1085,6 +1122,7 @@ int32_t rvx_comb_bank_spurs(rvx_comb_bank* bank, const rvx_comb_estimate* estima
         Tracker* t = bank->free_slot(true);
         if (!t) break;
         t->start(e, 1);
+        t->order = ++bank->starts;
     }
     return RVX_OK;
 }

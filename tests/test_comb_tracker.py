@@ -172,3 +172,46 @@ def test_spurs_dropped_in_bands_and_dropped_all_are_held_off():
     bank.reset()                               # a reset forgets the hold-offs too
     bank.spurs([spur(80_000.0)])
     assert bank.plan(7 * block, block)[1] == 1
+
+
+def test_a_starting_comb_takes_its_lines_back_from_the_newest_spurs():
+    """comb_bank.cpp detected(): combs have first claim on the 1024 line slots,
+    so a comb needing 1022 lines starts beside four held spurs, and the two
+    newest spurs give their lines back (and are held off); a comb of fewer
+    than eight lines never starts, and the second of two combs whose needs do
+    not both fit takes an even share."""
+    block = 65536
+    bank = CombBank(cl.RATE)
+    bank.spurs([spur(f) for f in (100_000.0, 110_000.0, 120_000.0, 130_000.0)])
+    assert bank.plan(0, block)[1] == 4
+    e = _lib.CombEstimate()
+    e.offset_hz, e.spacing_hz, e.k_lo, e.k_hi = 500.0, 1_000.0, -511, 510  # 1022 lines
+    bank.detected([e])
+    lines, n, _ = bank.plan(block, block)
+    assert n == 1024
+    # the comb's lines sit at 500 + 1000 k Hz, never on a spur's 10 kHz multiple
+    freqs = [round(lines[i].step / 2**32 * cl.RATE) for i in range(n)]
+    kept = sorted(f for f in freqs if f in (100_000, 110_000, 120_000, 130_000))
+    assert kept == [100_000, 110_000]  # the two oldest spurs stayed
+    bank.spurs([spur(130_000.0)])        # went a moment ago: held off
+    assert bank.plan(2 * block, block)[1] == 1024
+
+    small = CombBank(cl.RATE)
+    e7 = _lib.CombEstimate()
+    e7.offset_hz, e7.spacing_hz, e7.k_lo, e7.k_hi = 500.0, 1_000.0, 0, 6  # seven lines
+    small.detected([e7])
+    assert small.plan(0, block)[1] == 0
+
+    two = CombBank(cl.RATE)
+    a, b = _lib.CombEstimate(), _lib.CombEstimate()
+    a.offset_hz, a.spacing_hz, a.k_lo, a.k_hi = 500.0, 1_000.0, -400, 399     # 800 lines
+    b.offset_hz, b.spacing_hz, b.k_lo, b.k_hi = 700.0, 3_333.0, -100, 99      # 200 lines
+    two.detected([a, b])
+    assert two.plan(0, block)[1] == 1000                                      # both fit
+    three = CombBank(cl.RATE)
+    c = _lib.CombEstimate()
+    c.offset_hz, c.spacing_hz, c.k_lo, c.k_hi = 900.0, 7_777.0, -300, 299     # 600 lines
+    three.detected([a, c])
+    # 800 + 600 do not fit: the first takes max(even 512, room 424) = 512, the
+    # second what is left, 512
+    assert three.plan(0, block)[1] == 1024
