@@ -620,6 +620,8 @@ void robust_fit(const std::vector<double>& k, const std::vector<double>& y,
     }
 }
 
+constexpr uint32_t kGrowCarriers = 16;  // lines a locked pass may grow by in one block
+
 class Tracker {
 public:
     Tracker(int64_t rate, uint64_t id) : rate_(rate), id_(id) {}
@@ -654,7 +656,8 @@ public:
         if (spur_) {
             lines_.push_back(Line{});
         } else {
-            // the central lines of the detected range, up to max_lines
+            // the central lines of the detected range, up to max_lines; the
+            // rest of the range is grown into once the pass is locked
             const int64_t n = int64_t(e.k_hi) - e.k_lo + 1;
             const int64_t take = std::min<int64_t>(n, max_lines);
             const int64_t lo = e.k_lo + (n - take) / 2;
@@ -663,7 +666,14 @@ public:
                 l.k = int32_t(k);
                 lines_.push_back(l);
             }
+            range_lo_ = e.k_lo;
+            range_hi_ = e.k_hi;
+            next_lo_ = int32_t(lo) - 1;
+            next_hi_ = int32_t(lo + take);
+            take_hi_ = true;
         }
+        good_ = false;
+        fitted_ = benched_ = 0;
         records_.clear();
         observed_ = settle_ = bad_ = stage_settle_ = 0;
         seed_pending_ = false;
@@ -676,6 +686,62 @@ public:
         lines_.clear();
         records_.clear();
         cancelled_ = 0;
+    }
+
+    // Growth (comb_tracker.cpp wants_lines / grow / shrink_to / extend_range):
+    // a healthy locked pass with more of its comb's range to reach asks for up
+    // to kGrowCarriers lines a block, taken alternately above and below; a
+    // pass past a fair share gives the excess back, keeping its central lines.
+    uint32_t wants_lines() const {
+        if (spur_ || lock_ != RVX_COMB_LOCKED || !good_ || 4 * benched_ > fitted_) return 0;
+        const int64_t left = std::max<int64_t>(0, int64_t(range_hi_) - next_hi_ + 1) +
+                             std::max<int64_t>(0, int64_t(next_lo_) - range_lo_ + 1);
+        return uint32_t(std::min<int64_t>(left, kGrowCarriers));
+    }
+    uint32_t grow(uint32_t slots) {
+        uint32_t taken = 0;
+        while (taken < slots) {
+            const bool hi_left = next_hi_ <= range_hi_, lo_left = next_lo_ >= range_lo_;
+            if (!hi_left && !lo_left) break;
+            const bool hi = hi_left && (take_hi_ || !lo_left);
+            Line l;  // unseeded: out of the phase fit until its first locked block
+            l.k = hi ? next_hi_++ : next_lo_--;
+            take_hi_ = !hi;
+            lines_.push_back(l);  // appended: plans already out still line up
+            ++taken;
+        }
+        return taken;
+    }
+    uint32_t shrink_to(uint32_t slots) {
+        const size_t floor = std::max<size_t>(slots, RVX_COMB_MIN_LINES);
+        if (spur_ || lines_.size() <= floor) return 0;
+        const size_t before = lines_.size();
+        std::vector<int32_t> ks;
+        for (const Line& l : lines_) ks.push_back(l.k);
+        std::sort(ks.begin(), ks.end());
+        const size_t first = (ks.size() - floor) / 2;
+        const int32_t lo = ks[first], hi = ks[first + floor - 1];
+        std::erase_if(lines_, [&](const Line& l) { return l.k < lo || l.k > hi; });
+        next_lo_ = lo - 1;
+        next_hi_ = hi + 1;
+        ++gen_;  // plans already out no longer name these lines
+        records_.clear();
+        return uint32_t(before - lines_.size());
+    }
+    // A detection of the comb this pass holds widens the range it grows into:
+    // the detection's end lines mapped onto this pass's own k.
+    void extend_range(const rvx_comb_estimate& e) {
+        if (spur_ || e.spur || !active() || e.k_hi < e.k_lo) return;
+        const double rate = double(rate_);
+        const double c = anchored_ ? signed_cycles(c_.freq) * rate : est_.offset_hz;
+        const double d = anchored_ ? signed_cycles(d_.freq) * rate : est_.spacing_hz;
+        if (!(d > 0.0)) return;
+        auto to_k = [&](int32_t k) {
+            return int32_t(std::lround((e.offset_hz + double(k) * e.spacing_hz - c) / d));
+        };
+        const int32_t a = to_k(e.k_lo), b = to_k(e.k_hi);
+        range_lo_ = std::min({range_lo_, a, b});
+        range_hi_ = std::max({range_hi_, a, b});
     }
 
     // Lines of the block [start, start + count), phases at its first sample.
@@ -727,8 +793,9 @@ public:
         if (at == records_.size()) return;
         Record r = std::move(records_[at]);
         records_.erase(records_.begin(), records_.begin() + ptrdiff_t(at) + 1);
+        // A record planned before the pass grew names its first lines only.
         if (r.gen != gen_ || !active() || n_lines != r.lines.size() || subs < 2 ||
-            r.lines.size() != lines_.size())
+            r.lines.size() > lines_.size())
             return;
         ++observed_;
         const uint64_t m = start + count / 2;
@@ -812,11 +879,18 @@ public:
             }
             seed_pending_ = false;
         } else {
+            // lines grown since the seeding have no amplitude yet: out of the fit
+            std::vector<double> wp(w);
+            uint32_t fitted = 0;
+            for (size_t i = 0; i < n; ++i) {
+                if (!lines_[i].seeded) wp[i] = 0.0;
+                fitted += wp[i] > 0 ? 1u : 0u;
+            }
             double pa, pb;
-            robust_fit(ks, ep, w, one, pa, pb);
+            robust_fit(ks, ep, wp, one, pa, pb);
             double res = 0.0, inno = 0.0;
             for (size_t i = 0; i < n; ++i) {
-                if (!lines_[i].strong) continue;
+                if (!lines_[i].strong || !lines_[i].seeded) continue;
                 const double fit = pa + pb * ks[i];
                 const double off = std::abs(wrap_half(ep[i] - fit)) * kTwoPi;
                 Line& l = lines_[i];
@@ -827,12 +901,14 @@ public:
                     l.outlier = 0;
                     if (++l.inlier >= kUnbenchBlocks) l.benched = false;
                 }
-                if (w[i] > 0) {
-                    res += w[i] * off * off;
+                if (wp[i] > 0) {
+                    res += wp[i] * off * off;
                     inno = std::max(inno, std::abs(fit));
                 }
             }
-            res = sw > 0 ? std::sqrt(res / sw) : 1e9;
+            double swp = 0.0;
+            for (size_t i = 0; i < n; ++i) swp += wp[i];
+            res = swp > 0 ? std::sqrt(res / swp) : 1e9;
             // third-order loop, all poles at exp(-T / 0.05 s)
             const double rho = std::exp(-T / (double(rate_) * kLoopSeconds));
             const double alpha = 1 - rho * rho * rho;
@@ -843,10 +919,18 @@ public:
             for (size_t i = 0; i < n; ++i) {
                 Line& l = lines_[i];
                 const Cd derot = vs_model[i] * std::polar(1.0, -kTwoPi * (pa + pb * ks[i]));
-                l.amp += kAmpWeight * (derot - l.amp);
+                if (l.seeded) {
+                    l.amp += kAmpWeight * (derot - l.amp);
+                } else {  // a grown line: seeded from its first locked block
+                    l.amp = derot;
+                    l.seeded = true;
+                }
             }
             const bool many = strong >= min_lines_ && 2 * benched <= n;
             const bool good = many && res < kLockResidual && inno < kLockInnovation;
+            good_ = good;
+            fitted_ = fitted;
+            benched_ = benched;
             const bool bad = !many || res > kBadResidual || f_centre > kBadFrequency ||
                              f_span > kBadFrequency;
             if (lock_ == RVX_COMB_LOCKED) {
@@ -902,9 +986,12 @@ public:
             return holds(e.offset_hz, tol_hz) ? 1.0 : 0.0;
         }
         rvx_comb_estimate held = mine;
-        if (!lines_.empty() && !spur_) {
-            held.k_lo = lines_.front().k;
-            held.k_hi = lines_.back().k;
+        if (!lines_.empty() && !spur_) {  // grown lines sit past the first ones, in any order
+            held.k_lo = held.k_hi = lines_.front().k;
+            for (const Line& l : lines_) {
+                held.k_lo = std::min(held.k_lo, l.k);
+                held.k_hi = std::max(held.k_hi, l.k);
+            }
         }
         return overlap_with(e, held, tol_hz);
     }
@@ -937,6 +1024,9 @@ public:
 
 private:
     bool started_ = false, end_noted_ = false;
+    int32_t range_lo_ = 0, range_hi_ = 0, next_lo_ = 0, next_hi_ = 0;
+    bool take_hi_ = true, good_ = false;
+    uint32_t fitted_ = 0, benched_ = 0;
     enum Stage { kFrequency, kPhase };
     int64_t rate_;
     uint64_t id_;
@@ -1059,6 +1149,19 @@ int32_t rvx_comb_bank_detected(rvx_comb_bank* bank, const rvx_comb_estimate* est
         return false;
     };
     auto lines_of = [](const rvx_comb_estimate& e) { return uint32_t(e.k_hi - e.k_lo + 1); };
+    // A detection of a held comb widens the range that comb's pass grows into,
+    // so a pass started on part of a comb reaches the rest of it.
+    for (uint32_t i = 0; i < n; ++i) {
+        const rvx_comb_estimate& e = estimates[i];
+        if (e.spur || e.k_hi < e.k_lo || !(e.spacing_hz > 0) || !std::isfinite(e.offset_hz))
+            continue;
+        for (Tracker& t : bank->trackers)
+            if (!t.spur() && t.active() &&
+                t.overlap(e, RVX_COMB_SAME_LINE_HZ) >= RVX_COMB_OVERLAP_FRACTION) {
+                t.extend_range(e);
+                break;
+            }
+    }
     std::vector<const rvx_comb_estimate*> fresh;
     for (uint32_t i = 0; i < n; ++i) {
         const rvx_comb_estimate& e = estimates[i];
@@ -1066,6 +1169,25 @@ int32_t rvx_comb_bank_detected(rvx_comb_bank* bank, const rvx_comb_estimate* est
             !(e.spacing_hz > 0) || !std::isfinite(e.offset_hz))
             continue;
         if (!held(e)) fresh.push_back(&e);
+    }
+    if (fresh.empty()) return RVX_OK;
+    {
+        // Passes that grew past a fair share give the excess back when new
+        // combs want slots that are not free: growth only borrows.
+        uint32_t held_slots = 0, running = 0;
+        for (const Tracker& t : bank->trackers)
+            if (!t.spur() && t.active()) {
+                held_slots += t.lines();
+                ++running;
+            }
+        const uint32_t incoming = std::min<uint32_t>(
+            uint32_t(fresh.size()), RVX_COMB_MAX_COMBS > running ? RVX_COMB_MAX_COMBS - running : 0u);
+        if (incoming > 0) {
+            const uint32_t fair = RVX_COMB_MAX_LINES / (running + incoming);
+            if (RVX_COMB_MAX_LINES - std::min(held_slots, uint32_t(RVX_COMB_MAX_LINES)) < fair * incoming)
+                for (Tracker& t : bank->trackers)
+                    if (!t.spur() && t.active() && t.lines() > fair) t.shrink_to(fair);
+        }
     }
     for (size_t i = 0; i < fresh.size(); ++i) {
         // Combs have first claim: lines spurs hold count as free here, and the
@@ -1166,6 +1288,16 @@ int32_t rvx_comb_bank_plan(rvx_comb_bank* bank, uint64_t start, uint32_t count,
     bank->now = start;
     std::erase_if(bank->hold_offs, [&](const rvx_comb_bank::HoldOff& h) { return h.until <= start; });
     bank->note_ended_spurs();  // a spur that lost its line since the last block
+    // Slots nobody holds go to healthy locked passes with more of their comb
+    // to reach, a few lines each per block; spurs keep theirs.
+    {
+        uint32_t held_slots = bank->lines_held();
+        for (Tracker& t : bank->trackers) {
+            if (held_slots >= RVX_COMB_MAX_LINES) break;
+            const uint32_t want = std::min(t.wants_lines(), RVX_COMB_MAX_LINES - held_slots);
+            if (want) held_slots += t.grow(want);
+        }
+    }
     std::vector<rvx_comb_line> all;
     bool sub = false;
     Split split;

@@ -96,16 +96,17 @@ def run(x: np.ndarray, bank, table, block: int = 65536, detect_every: float = 1.
     return out
 
 
-def line_height_db(x: np.ndarray, t0: float, nfft: int = 4096, t1: float | None = None) -> float:
-    """Mean over the comb's lines of its bin over the median bin, dB, over
-    the signal from t0 to t1 (the end when None; the lines placed where they
-    are at mid-span)."""
+def line_height_db(x: np.ndarray, t0: float, nfft: int = 4096, t1: float | None = None,
+                   ks=None) -> float:
+    """Mean over the comb's lines (those of `ks`, all when None) of its bin
+    over the median bin, dB, over the signal from t0 to t1 (the end when None;
+    the lines placed where they are at mid-span)."""
     seg = x[int(t0 * RATE):None if t1 is None else int(t1 * RATE)]
     p = spectrum(seg, nfft)
     med = np.median(p)
     mid = t0 + 0.5 * len(seg) / RATE
     heights = []
-    for k in range(K_LO, K_HI + 1):
+    for k in (range(K_LO, K_HI + 1) if ks is None else ks):
         f = offset_at(mid) + k * spacing_at(mid)
         b = int(round(f / (RATE / nfft))) + nfft // 2
         if 0 <= b < nfft:

@@ -215,3 +215,37 @@ def test_a_starting_comb_takes_its_lines_back_from_the_newest_spurs():
     # 800 + 600 do not fit: the first takes max(even 512, room 424) = 512, the
     # second what is left, 512
     assert three.plan(0, block)[1] == 1024
+
+
+def test_a_pass_started_on_part_of_a_comb_grows_to_cover_it(capture):
+    """comb_bank.cpp / comb_tracker.cpp growth: a pass started on the central
+    21 of the comb's 41 lines, handed later detections of the whole comb
+    (held, so they widen the range its pass may grow into), grows once locked
+    by up to 16 lines a block and takes the outer lines out as well."""
+    from revenant_amd.comb import detect_comb
+    import ctypes as C
+    table = comb_table()
+    bank = CombBank(cl.RATE)
+    whole = detect_comb(cl.spectrum(capture[: cl.RATE], 16384), cl.RATE)
+    assert whole is not None and whole.k_hi - whole.k_lo + 1 >= 35
+    part = _lib.CombEstimate()
+    C.memmove(C.byref(part), C.byref(whole), C.sizeof(part))
+    mid = (whole.k_lo + whole.k_hi) // 2
+    part.k_lo, part.k_hi = mid - 10, mid + 10
+    bank.detected([part])
+    assert bank.plan(0, 65536)[1] == 21
+
+    def detect(last_second):
+        bank.detected([whole])
+
+    cleaned = cl.run(capture, bank, table, detect=detect)
+    st = bank.status()
+    assert st.lock == _lib.COMB_LOCKED
+    assert st.lines_cancelled >= 35, st.lines_cancelled
+    end = (len(capture) // 65536) * 65536 / cl.RATE
+    outer = [k for k in range(cl.K_LO, cl.K_HI + 1) if abs(k) > 10]
+    off = cl.line_height_db(capture, MEASURE_FROM, t1=end, ks=outer)
+    on = cl.line_height_db(cleaned, MEASURE_FROM, t1=end, ks=outer)
+    print(f"outer lines: {off:.1f} dB over the floor before, {on:.1f} dB after; "
+          f"{st.lines_cancelled} lines cancelled")
+    assert off > 10.0 and on < off - 10.0 and on < 6.0
