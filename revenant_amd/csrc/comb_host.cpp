@@ -976,6 +976,14 @@ public:
         rvx_comb_estimate e = est_;
         e.offset_hz = signed_cycles(c_.freq) * double(rate_);
         e.spacing_hz = spur_ ? 0.0 : signed_cycles(d_.freq) * double(rate_);
+        if (!spur_ && !lines_.empty()) {  // the lines held now, grown ones included
+            e.k_lo = e.k_hi = lines_.front().k;
+            for (const Line& l : lines_) {
+                e.k_lo = std::min(e.k_lo, l.k);
+                e.k_hi = std::max(e.k_hi, l.k);
+            }
+            e.lines_found = uint32_t(lines_.size());
+        }
         return e;
     }
 
@@ -985,15 +993,7 @@ public:
         if (e.spur) {
             return holds(e.offset_hz, tol_hz) ? 1.0 : 0.0;
         }
-        rvx_comb_estimate held = mine;
-        if (!lines_.empty() && !spur_) {  // grown lines sit past the first ones, in any order
-            held.k_lo = held.k_hi = lines_.front().k;
-            for (const Line& l : lines_) {
-                held.k_lo = std::min(held.k_lo, l.k);
-                held.k_hi = std::max(held.k_hi, l.k);
-            }
-        }
-        return overlap_with(e, held, tol_hz);
+        return overlap_with(e, mine, tol_hz);  // over the k range held now
     }
 
     bool holds(double f_hz, double tol_hz) const {
