@@ -126,6 +126,35 @@ def test_batch_matches_per_receiver_and_oracle(dev, scene, tile, workgroups):
             assert_bit_exact(got, cpu, what=f"batch vs oracle, receiver {v}")
 
 
+SMALL = 1 << 10  # a channel ring the fine windows cross every pass
+
+
+@pytest.mark.parametrize("tile", [None, 64])
+def test_batch_windows_across_the_channel_ring_wrap(dev, scene, tile):
+    """On a 1024-sample channel ring the fine windows (up to 256 taps) cross
+    the ring's wrap many times per receiver, so the fine loop's wrapping form
+    runs beside its straight one (which needs a window clear of the wrap);
+    both against the oracle, bit for bit."""
+    from revenant_amd.receivers import Receiver, ReceiverBatch, max_tile
+
+    nco, ring, plans = scene
+    small = np.ascontiguousarray(ring[:, :SMALL])
+    ring_dev = to_dev(small.view(np.float32).ravel())
+    nco_dev = to_dev(nco.view(np.float32))
+    rxs = [Receiver(p, (v % CHANNELS) * SMALL, SMALL - 1, ring_dev) for v, p in enumerate(plans)]
+    batch = ReceiverBatch(dev, rxs, nco_dev)
+    starts = _starts(plans)
+    counts = [900 + 29 * v for v in range(len(rxs))]
+    audio = [torch.full((n * r.plan.demod.channels,), float("nan"), device="cuda")
+             for n, r in zip(counts, rxs)]
+    tiles = None if tile is None else [min(tile, max_tile(r.plan)) for r in rxs]
+    batch.prepare(starts, counts, audio, tiles)
+    batch.launch()
+    for v, rx in enumerate(rxs):
+        cpu = _oracle_audio(rx, small, nco, starts[v], counts[v])
+        assert_bit_exact(from_dev(audio[v]), cpu, what=f"batch vs oracle across the wrap, receiver {v}")
+
+
 def test_batch_relaunch_and_graph_replay(dev, scene):
     from revenant_amd.receivers import ReceiverBatch
 
