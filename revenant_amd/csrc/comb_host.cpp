@@ -531,6 +531,7 @@ constexpr double kBenchRad = 0.8;
 constexpr uint32_t kLockBlocks = 8, kBadBlocks = 6, kStageBlocks = 3;
 constexpr uint32_t kBenchBlocks = 4, kUnbenchBlocks = 8, kAcquireBlocks = 200;
 constexpr double kRecoverSeconds = 8.0;
+constexpr double kRecentreSeconds = 1.0;  // a recovery left to its loops this long before a detection recentres it
 constexpr double kAmpWeight = 1.0 / 16.0;
 constexpr uint32_t kPlanSubBlocks = 16;
 
@@ -963,6 +964,41 @@ public:
         if (nsum) suppression_ = float(sum / nsum);
     }
 
+    // A recovering pass pulls in from the model it held, which is right after
+    // a burst but too narrow once the comb has moved while the pass could not
+    // see it. A detection names where the comb is now: mapped onto this pass's
+    // own k it gives c and d, and the frequency stage pulls in from there.
+    // Once per recovery after the first kRecentreSeconds, unless the comb is
+    // known to have moved (`moved`: a detection on this spacing that overlaps
+    // none of the pass's lines).
+    bool same_spacing(const rvx_comb_estimate& e) const {
+        if (spur_ || e.spur || !anchored_ || e.k_hi < e.k_lo) return false;
+        const double d = signed_cycles(d_.freq) * double(rate_);
+        const double half = 0.5 * double(int64_t(e.k_hi) - e.k_lo + 1);
+        return d > 0.0 && std::abs(e.spacing_hz - d) * half < 0.25 * d;
+    }
+    bool recentre(const rvx_comb_estimate& e, bool moved) {
+        if (spur_ || e.spur || lock_ != RVX_COMB_RECOVERING || !anchored_ || e.k_hi < e.k_lo ||
+            !(e.spacing_hz > 0.0))
+            return false;
+        const double rate = double(rate_);
+        if (!moved && (recentred_ || double(int64_t(anchor_ - recovering_since_)) <
+                                         kRecentreSeconds * rate))
+            return false;
+        const double c = signed_cycles(c_.freq) * rate, d = signed_cycles(d_.freq) * rate;
+        if (!(d > 0.0)) return false;
+        const double j0 = std::round((e.offset_hz - c) / d);  // this pass's k at the estimate's 0
+        c_ = {c_.phase, fixed((e.offset_hz - j0 * e.spacing_hz) / rate), 0};
+        d_ = {d_.phase, fixed(e.spacing_hz / rate), 0};
+        recentred_ = true;
+        stage_ = kFrequency;
+        stage_settle_ = settle_ = bad_ = 0;
+        ++gen_;  // plans already out were made against the old model
+        records_.clear();
+        for (Line& l : lines_) l.seeded = false;
+        return true;
+    }
+
     void begin_recovery(uint64_t m) {
         lock_ = RVX_COMB_RECOVERING;
         stage_ = kFrequency;
@@ -970,6 +1006,7 @@ public:
         c_.rate = d_.rate = 0;  // frequencies held, rates dropped
         for (Line& l : lines_) l.seeded = false;
         recovering_since_ = m;
+        recentred_ = false;
     }
 
     rvx_comb_estimate estimate() const {
@@ -1044,6 +1081,7 @@ private:
     uint32_t observed_ = 0, settle_ = 0, bad_ = 0, stage_settle_ = 0;
     bool seed_pending_ = false;
     uint64_t recovering_since_ = 0;
+    bool recentred_ = false;
     float suppression_ = 0.0f;
     double residual_ = 0.0;
     uint32_t cancelled_ = 0;
@@ -1159,16 +1197,29 @@ int32_t rvx_comb_bank_detected(rvx_comb_bank* bank, const rvx_comb_estimate* est
             if (!t.spur() && t.active() &&
                 t.overlap(e, RVX_COMB_SAME_LINE_HZ) >= RVX_COMB_OVERLAP_FRACTION) {
                 t.extend_range(e);
+                t.recentre(e, false);
                 break;
             }
     }
+    // A comb on a recovering pass's spacing that overlaps none of its lines is
+    // that comb, moved while the pass could not see it: the pass takes it
+    // rather than a second pass starting beside a dying one.
+    auto moved = [&](const rvx_comb_estimate& e) {
+        for (Tracker& t : bank->trackers)
+            if (!t.spur() && t.lock() == RVX_COMB_RECOVERING && t.same_spacing(e) &&
+                t.recentre(e, true)) {
+                t.extend_range(e);
+                return true;
+            }
+        return false;
+    };
     std::vector<const rvx_comb_estimate*> fresh;
     for (uint32_t i = 0; i < n; ++i) {
         const rvx_comb_estimate& e = estimates[i];
         if (e.spur || e.k_hi < e.k_lo || lines_of(e) < RVX_COMB_MIN_LINES ||
             !(e.spacing_hz > 0) || !std::isfinite(e.offset_hz))
             continue;
-        if (!held(e)) fresh.push_back(&e);
+        if (!held(e) && !moved(e)) fresh.push_back(&e);
     }
     if (fresh.empty()) return RVX_OK;
     {

@@ -112,3 +112,18 @@ def line_height_db(x: np.ndarray, t0: float, nfft: int = 4096, t1: float | None 
         if 0 <= b < nfft:
             heights.append(10 * np.log10(p[b] / med))
     return float(np.mean(heights))
+
+
+def render_moved(seconds: float, jump_at: float, jump_hz: float) -> np.ndarray:
+    """The drifting comb over noise, every line shifted by jump_hz from
+    jump_at on (a comb that moves while a pass holds it, as a switching
+    supply's comb does when its load changes)."""
+    g = np.random.default_rng(SEED + 2)
+    n = int(seconds * RATE)
+    x = (g.normal(0.0, NOISE_RMS, n) + 1j * g.normal(0.0, NOISE_RMS, n)).astype(np.complex128)
+    t = np.arange(n) / RATE
+    shift = np.where(t >= jump_at, jump_hz * (t - jump_at), 0.0)
+    for k in range(K_LO, K_HI + 1):
+        cyc = cycles_at(k, t) + shift + g.uniform()
+        x += LINE_AMPLITUDE * np.exp(2j * np.pi * (cyc - np.floor(cyc)))
+    return x.astype(np.complex64)

@@ -249,3 +249,44 @@ def test_a_pass_started_on_part_of_a_comb_grows_to_cover_it(capture):
     print(f"outer lines: {off:.1f} dB over the floor before, {on:.1f} dB after; "
           f"{st.lines_cancelled} lines cancelled")
     assert off > 10.0 and on < off - 10.0 and on < 6.0
+
+
+def test_a_recovering_pass_takes_its_moved_comb():
+    """comb_bank.cpp detected() / comb_tracker.cpp recentre: the comb locked by
+    two passes (its halves) jumps 2 kHz at 2.3 s; the passes lose it and
+    recover; the next detection names the comb at its new place on the same
+    spacing, overlapping none of the passes' lines, and the recovering passes
+    take it (no pass starts beside them), relock and take its lines out."""
+    x = cl.render_moved(6.0, 2.3, 2000.0)
+    table = comb_table()
+    bank = CombBank(cl.RATE)
+    seen = []
+
+    def detect(last_second):
+        found = detect_combs(cl.spectrum(last_second, 16384), cl.RATE, bank.active())
+        before = bank.status().combs_active
+        bank.detected(found)
+        seen.append((len(found), before, bank.status().combs_active,
+                     [round(e.offset_hz) for e in found]))
+
+    cleaned = cl.run(x, bank, table, detect=detect)
+    print(seen)
+    # the detection after the jump: the moved comb found, no pass started
+    n_found, before, after, offsets = seen[2]
+    assert n_found >= 1 and any(abs(o - 3234) < 100 or abs(o + 11766) < 100 for o in offsets)
+    assert before == after == 2
+    st = bank.status()
+    assert st.lock == _lib.COMB_LOCKED
+    end = (len(x) // 65536) * 65536 / cl.RATE
+    seg = lambda s: s[int(5.0 * cl.RATE):int(end * cl.RATE)]
+    p_raw, p_out = cl.spectrum(seg(x), 4096), cl.spectrum(seg(cleaned), 4096)
+    heights = []
+    for name, p in (("raw", p_raw), ("cleaned", p_out)):
+        med = np.median(p)
+        mid = 0.5 * (5.0 + end)
+        h = [10 * np.log10(p[int(round((cl.offset_at(mid) + 2000.0 + k * cl.spacing_at(mid))
+                                        / (cl.RATE / 4096))) + 2048] / med)
+             for k in range(cl.K_LO, cl.K_HI + 1)]
+        heights.append(float(np.mean(h)))
+    print(f"moved comb over the floor, 5 s on: {heights[0]:.1f} dB raw, {heights[1]:.1f} dB cleaned")
+    assert heights[0] > 10.0 and heights[1] < 6.0
