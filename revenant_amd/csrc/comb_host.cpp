@@ -561,6 +561,7 @@ void correct(Axis& a, uint64_t anchor, uint64_t at, double dp, double df, double
 
 struct Line {
     int32_t k = 0;
+    int32_t m = 0;  // ripple sideband order (0: the carrier)
     Cd amp{};
     bool seeded = false;
     bool strong = false;
@@ -570,7 +571,7 @@ struct Line {
 };
 
 struct Planned {
-    int32_t k;
+    int32_t k, m;
     uint32_t phase0, step;
     Cd amp;
 };
@@ -654,19 +655,25 @@ public:
         c_ = {0, fixed(e.offset_hz / double(rate_)), 0};
         d_ = {0, spur_ ? 0 : fixed(e.spacing_hz / double(rate_)), 0};
         lines_.clear();
+        // Ripple sidebands only with a ripple to put them at, and only as many
+        // orders as leave room for RVX_COMB_MIN_LINES carriers.
+        orders_ = 0;
+        r_ = {};
+        if (!spur_ && e.ripple_hz > 0.0 && e.sideband_orders > 0) {
+            orders_ = std::min<int32_t>(e.sideband_orders,
+                                        int32_t((max_lines / RVX_COMB_MIN_LINES - 1) / 2));
+            if (orders_ > 0) r_ = {0, fixed(e.ripple_hz / double(rate_)), 0};
+        }
         if (spur_) {
             lines_.push_back(Line{});
         } else {
-            // the central lines of the detected range, up to max_lines; the
-            // rest of the range is grown into once the pass is locked
+            // the central carriers of the detected range, each followed by its
+            // sidebands (m = -orders .. orders), up to max_lines; the rest of
+            // the range is grown into once the pass is locked
             const int64_t n = int64_t(e.k_hi) - e.k_lo + 1;
-            const int64_t take = std::min<int64_t>(n, max_lines);
+            const int64_t take = std::min<int64_t>(n, max_lines / per_line());
             const int64_t lo = e.k_lo + (n - take) / 2;
-            for (int64_t k = lo; k < lo + take; ++k) {
-                Line l;
-                l.k = int32_t(k);
-                lines_.push_back(l);
-            }
+            for (int64_t k = lo; k < lo + take; ++k) add_group(int32_t(k));
             range_lo_ = e.k_lo;
             range_hi_ = e.k_hi;
             next_lo_ = int32_t(lo) - 1;
@@ -693,28 +700,37 @@ public:
     // a healthy locked pass with more of its comb's range to reach asks for up
     // to kGrowCarriers lines a block, taken alternately above and below; a
     // pass past a fair share gives the excess back, keeping its central lines.
+    uint32_t per_line() const { return uint32_t(2 * orders_ + 1); }
+    // A carrier and its sidebands, the carrier in the middle (slot of order m
+    // minus m is its carrier's).
+    void add_group(int32_t k) {
+        for (int32_t m = -orders_; m <= orders_; ++m) {
+            Line l;  // unseeded: out of the phase fit until its first locked block
+            l.k = k;
+            l.m = m;
+            lines_.push_back(l);
+        }
+    }
     uint32_t wants_lines() const {
         if (spur_ || lock_ != RVX_COMB_LOCKED || !good_ || 4 * benched_ > fitted_) return 0;
         const int64_t left = std::max<int64_t>(0, int64_t(range_hi_) - next_hi_ + 1) +
                              std::max<int64_t>(0, int64_t(next_lo_) - range_lo_ + 1);
-        return uint32_t(std::min<int64_t>(left, kGrowCarriers));
+        return uint32_t(std::min<int64_t>(left, kGrowCarriers)) * per_line();
     }
     uint32_t grow(uint32_t slots) {
         uint32_t taken = 0;
-        while (taken < slots) {
+        while (taken + per_line() <= slots) {
             const bool hi_left = next_hi_ <= range_hi_, lo_left = next_lo_ >= range_lo_;
             if (!hi_left && !lo_left) break;
             const bool hi = hi_left && (take_hi_ || !lo_left);
-            Line l;  // unseeded: out of the phase fit until its first locked block
-            l.k = hi ? next_hi_++ : next_lo_--;
+            add_group(hi ? next_hi_++ : next_lo_--);  // appended: plans already out still line up
             take_hi_ = !hi;
-            lines_.push_back(l);  // appended: plans already out still line up
-            ++taken;
+            taken += per_line();
         }
         return taken;
     }
     uint32_t shrink_to(uint32_t slots) {
-        const size_t floor = std::max<size_t>(slots, RVX_COMB_MIN_LINES);
+        const size_t floor = std::max<size_t>(slots, size_t(RVX_COMB_MIN_LINES) * per_line());
         if (spur_ || lines_.size() <= floor) return 0;
         const size_t before = lines_.size();
         std::vector<int32_t> ks;
@@ -756,6 +772,7 @@ public:
         // re-anchor at the newest middle, so advances stay short
         c_ = advanced(c_, int64_t(m - anchor_));
         d_ = advanced(d_, int64_t(m - anchor_));
+        r_ = advanced(r_, int64_t(m - anchor_));
         anchor_ = m;
         Record r;
         r.start = start;
@@ -763,16 +780,21 @@ public:
         r.gen = gen_;
         r.subtract = lock_ == RVX_COMB_LOCKED;
         cancelled_ = 0;
-        for (const Line& l : lines_) {
-            const uint64_t uk = uint64_t(int64_t(l.k));
-            const uint64_t ph = c_.phase + uk * d_.phase, fr = c_.freq + uk * d_.freq;
+        for (size_t slot = 0; slot < lines_.size(); ++slot) {
+            const Line& l = lines_[slot];
+            const uint64_t uk = uint64_t(int64_t(l.k)), um = uint64_t(int64_t(l.m));
+            const uint64_t ph = c_.phase + uk * d_.phase + um * r_.phase;
+            const uint64_t fr = c_.freq + uk * d_.freq + um * r_.freq;
             const uint32_t step = uint32_t((fr + (uint64_t(1) << 31)) >> 32);
             const uint32_t mid = uint32_t((ph + (uint64_t(1) << 31)) >> 32);
             const uint32_t phase0 = mid - (count / 2) * step;
-            const bool cut = r.subtract && l.seeded && l.strong && !l.benched;
+            // a sideband follows its carrier onto the bench, and is cut only
+            // while its carrier stands out
+            const Line& carrier = lines_[slot - size_t(ptrdiff_t(l.m))];
+            const bool cut = r.subtract && l.seeded && carrier.strong && !carrier.benched;
             const Cd amp = cut ? l.amp : Cd(0.0, 0.0);
             cancelled_ += cut ? 1u : 0u;
-            r.lines.push_back({l.k, phase0, step, amp});
+            r.lines.push_back({l.k, l.m, phase0, step, amp});
             rvx_comb_line cl{};
             cl.phase0 = phase0;
             cl.step = step;
@@ -801,20 +823,40 @@ public:
         ++observed_;
         const uint64_t m = start + count / 2;
         const Axis mc = advanced(c_, int64_t(m - anchor_)), md = advanced(d_, int64_t(m - anchor_));
+        const Axis mr = advanced(r_, int64_t(m - anchor_));
         const double L = double(count) / double(subs);
         const size_t n = r.lines.size();
-        std::vector<double> ks(n), ef(n), ep(n), w(n);
+        std::vector<double> ks(n), ef(n), ep(n), w(n), snr_of(n);
         std::vector<Cd> vs_model(n);
         uint32_t strong = 0, benched = 0;
+        // Block sums; with sidebands, each line's sum loses what the other
+        // lines of its carrier put into it: line j's amplitude (its sum over
+        // count) times the closed-form sum of tone_j conj(tone_i) over the block.
+        std::vector<Cd> block(n, Cd(0, 0));
+        for (size_t i = 0; i < n; ++i)
+            for (uint32_t s = 0; s < subs; ++s) block[i] += Cd(proj[i * subs + s]);
+        std::vector<Cd> clean(block);
+        if (orders_ > 0) {
+            const double N = double(count);
+            for (size_t i = 0; i < n; ++i)
+                for (size_t j = 0; j < n; ++j) {
+                    if (j == i || r.lines[j].k != r.lines[i].k) continue;
+                    const double dphi = double(int32_t(r.lines[j].phase0 - r.lines[i].phase0)) / kTwo32;
+                    const double dw = double(int32_t(r.lines[j].step - r.lines[i].step)) / kTwo32;
+                    Cd gain = std::abs(dw) < 1e-12
+                                  ? Cd(N, 0.0)
+                                  : (1.0 - std::polar(1.0, kTwoPi * dw * N)) /
+                                        (1.0 - std::polar(1.0, kTwoPi * dw));
+                    gain *= std::polar(1.0, kTwoPi * dphi);
+                    clean[i] -= block[j] / N * gain;
+                }
+        }
         for (size_t i = 0; i < n; ++i) {
             const Planned& pl = r.lines[i];
             Line& l = lines_[i];
-            Cd P(0, 0), D(0, 0);
-            for (uint32_t s = 0; s < subs; ++s) {
-                const Cd p(proj[i * subs + s]);
-                P += p;
-                if (s + 1 < subs) D += Cd(proj[i * subs + s + 1]) * std::conj(p);
-            }
+            Cd P = clean[i], D(0, 0);
+            for (uint32_t s = 0; s + 1 < subs; ++s)
+                D += Cd(proj[i * subs + s + 1]) * std::conj(Cd(proj[i * subs + s]));
             const Cd z = std::abs(D) > 0 ? D / std::abs(D) : Cd(1, 0);
             Cd cc(0, 0), zs(1, 0);
             for (uint32_t s = 0; s < subs; ++s, zs *= std::conj(z)) cc += Cd(proj[i * subs + s]) * zs;
@@ -825,7 +867,8 @@ public:
             const double np = noise / double(subs - 1) / L;  // per sample
             const double tp = std::norm(cc) / (L * L);
             const double snr = np > 0 ? tp * double(count) / np : 1e12;
-            l.strong = snr > kStandOut;
+            snr_of[i] = snr;
+            l.strong = pl.m == 0 && snr > kStandOut;  // a sideband rides on its carrier's
             l.noise += kAmpWeight * (np - l.noise);
             const uint64_t uk = uint64_t(int64_t(pl.k));
             const double f_plan = double(int32_t(pl.step)) / kTwo32;
@@ -833,7 +876,8 @@ public:
             const double f_model = signed_cycles(mc.freq + uk * md.freq);
             ef[i] = wrap_half(f_meas - f_model);
             const double ph_plan = double(uint32_t(pl.phase0 + (count / 2) * pl.step)) / kTwo32;
-            const double ph_model = double(mc.phase + uk * md.phase) / kTwo64;
+            const uint64_t um = uint64_t(int64_t(pl.m));
+            const double ph_model = double(mc.phase + uk * md.phase + um * mr.phase) / kTwo64;
             const Cd a = P / double(count);
             vs_model[i] = a * std::polar(1.0, kTwoPi * (ph_plan - ph_model));
             ep[i] = l.seeded ? std::arg(vs_model[i] / l.amp) / kTwoPi : 0.0;
@@ -842,9 +886,11 @@ public:
                 l.after += kAmpWeight * (std::norm(a - pl.amp) - l.after);
             }
             ks[i] = double(pl.k);
-            w[i] = l.strong && !l.benched ? snr : 0.0;
+            // the carriers alone steer c and d and decide the lock
+            w[i] = pl.m == 0 && l.strong && !l.benched ? snr : 0.0;
+            if (pl.m != 0) ef[i] = 0.0;
             strong += l.strong ? 1u : 0u;
-            benched += l.benched ? 1u : 0u;
+            benched += pl.m == 0 && l.benched ? 1u : 0u;
         }
         const double T = double(count);
         const bool one = spur_;
@@ -917,9 +963,34 @@ public:
             const double gamma = (1 - rho) * (1 - rho) * (1 - rho);
             correct(c_, anchor_, m, alpha * pa, beta * pa / T, gamma * pa / (T * T));
             if (!one) correct(d_, anchor_, m, alpha * pb, beta * pb / T, gamma * pb / (T * T));
+            // The ripple: a sideband's phase error past what the carriers' fit
+            // gives its k is m times the ripple's; the weighted least-squares
+            // slope through the origin in m, weighted by the carrier's SNR.
+            double ripple = 0.0;
+            if (orders_ > 0 && fitted >= min_lines_) {
+                double swy = 0.0, sww = 0.0;
+                uint32_t used = 0;
+                for (size_t i = 0; i < n; ++i) {
+                    const Line& l = lines_[i];
+                    if (r.lines[i].m == 0 || !l.seeded || std::abs(l.amp) <= 0.0) continue;
+                    const size_t ci = size_t(ptrdiff_t(i) - r.lines[i].m);
+                    if (ci >= n || snr_of[ci] <= kStandOut) continue;
+                    const double mm = double(r.lines[i].m);
+                    const double y = wrap_half(ep[i] - (pa + pb * ks[i]));
+                    swy += snr_of[ci] * y * mm;
+                    sww += snr_of[ci] * mm * mm;
+                    ++used;
+                }
+                if (used >= min_lines_ && sww > 0.0) {
+                    ripple = swy / sww;
+                    correct(r_, anchor_, m, alpha * ripple, beta * ripple / T,
+                            gamma * ripple / (T * T));
+                }
+            }
             for (size_t i = 0; i < n; ++i) {
                 Line& l = lines_[i];
-                const Cd derot = vs_model[i] * std::polar(1.0, -kTwoPi * (pa + pb * ks[i]));
+                const Cd derot = vs_model[i] * std::polar(
+                    1.0, -kTwoPi * (pa + pb * ks[i] + double(r.lines[i].m) * ripple));
                 if (l.seeded) {
                     l.amp += kAmpWeight * (derot - l.amp);
                 } else {  // a grown line: seeded from its first locked block
@@ -1003,7 +1074,7 @@ public:
         lock_ = RVX_COMB_RECOVERING;
         stage_ = kFrequency;
         stage_settle_ = settle_ = bad_ = 0;
-        c_.rate = d_.rate = 0;  // frequencies held, rates dropped
+        c_.rate = d_.rate = r_.rate = 0;  // frequencies held, rates dropped
         for (Line& l : lines_) l.seeded = false;
         recovering_since_ = m;
         recentred_ = false;
@@ -1019,7 +1090,9 @@ public:
                 e.k_lo = std::min(e.k_lo, l.k);
                 e.k_hi = std::max(e.k_hi, l.k);
             }
-            e.lines_found = uint32_t(lines_.size());
+            e.lines_found = uint32_t(lines_.size()) / per_line();
+            e.ripple_hz = orders_ > 0 ? signed_cycles(r_.freq) * double(rate_) : 0.0;
+            e.sideband_orders = orders_;
         }
         return e;
     }
@@ -1036,7 +1109,9 @@ public:
     bool holds(double f_hz, double tol_hz) const {
         const rvx_comb_estimate mine = estimate();
         for (const Line& l : lines_)
-            if (std::abs(mine.offset_hz + double(l.k) * mine.spacing_hz - f_hz) <= tol_hz) return true;
+            if (std::abs(mine.offset_hz + double(l.k) * mine.spacing_hz +
+                         double(l.m) * mine.ripple_hz - f_hz) <= tol_hz)
+                return true;
         return false;
     }
 
@@ -1075,7 +1150,8 @@ private:
     uint64_t gen_ = 0;
     bool anchored_ = false;
     uint64_t anchor_ = 0;
-    Axis c_{}, d_{};
+    Axis c_{}, d_{}, r_{};  // offset, spacing, ripple (r_ zero without sidebands)
+    int32_t orders_ = 0;
     std::vector<Line> lines_;
     std::vector<Record> records_;
     uint32_t observed_ = 0, settle_ = 0, bad_ = 0, stage_settle_ = 0;
@@ -1187,6 +1263,11 @@ int32_t rvx_comb_bank_detected(rvx_comb_bank* bank, const rvx_comb_estimate* est
         return false;
     };
     auto lines_of = [](const rvx_comb_estimate& e) { return uint32_t(e.k_hi - e.k_lo + 1); };
+    // What a comb asks for: its lines, each with its sidebands.
+    auto need = [&](const rvx_comb_estimate& e) {
+        const uint32_t orders = e.ripple_hz > 0.0 ? uint32_t(std::max(e.sideband_orders, 0)) : 0u;
+        return lines_of(e) * (2 * orders + 1);
+    };
     // A detection of a held comb widens the range that comb's pass grows into,
     // so a pass started on part of a comb reaches the rest of it.
     for (uint32_t i = 0; i < n; ++i) {
@@ -1254,10 +1335,10 @@ int32_t rvx_comb_bank_detected(rvx_comb_bank* bank, const rvx_comb_estimate* est
         // even share of what is free.
         const uint32_t free = RVX_COMB_MAX_LINES - used;
         uint32_t rest = 0;
-        for (size_t j = i + 1; j < fresh.size(); ++j) rest += lines_of(*fresh[j]);
+        for (size_t j = i + 1; j < fresh.size(); ++j) rest += need(*fresh[j]);
         const uint32_t even = free / uint32_t(fresh.size() - i);
         const uint32_t room = rest < free ? free - rest : 0u;
-        const uint32_t share = std::min(lines_of(*fresh[i]), std::max(even, room));
+        const uint32_t share = std::min(need(*fresh[i]), std::max(even, room));
         if (share < RVX_COMB_MIN_LINES) break;
         if (held(*fresh[i])) continue;
         Tracker* t = bank->free_slot(false);

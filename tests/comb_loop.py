@@ -127,3 +127,18 @@ def render_moved(seconds: float, jump_at: float, jump_hz: float) -> np.ndarray:
         cyc = cycles_at(k, t) + shift + g.uniform()
         x += LINE_AMPLITUDE * np.exp(2j * np.pi * (cyc - np.floor(cyc)))
     return x.astype(np.complex64)
+
+
+def render_rippled(seconds: float, ripple_hz: float, depth: float, seed: int = SEED + 3) -> np.ndarray:
+    """The drifting comb over noise with every line amplitude-modulated by a
+    ripple (1 + depth cos(2 pi ripple t + theta_k)): sidebands depth / 2 of
+    the carrier at +- ripple_hz, as a rectifier's ripple puts them."""
+    g = np.random.default_rng(seed)
+    n = int(seconds * RATE)
+    x = (g.normal(0.0, NOISE_RMS, n) + 1j * g.normal(0.0, NOISE_RMS, n)).astype(np.complex128)
+    t = np.arange(n) / RATE
+    for k in range(K_LO, K_HI + 1):
+        cyc = cycles_at(k, t) + g.uniform()
+        env = 1.0 + depth * np.cos(2 * np.pi * ripple_hz * t + 2 * np.pi * g.uniform())
+        x += LINE_AMPLITUDE * env * np.exp(2j * np.pi * (cyc - np.floor(cyc)))
+    return x.astype(np.complex64)

@@ -290,3 +290,48 @@ def test_a_recovering_pass_takes_its_moved_comb():
         heights.append(float(np.mean(h)))
     print(f"moved comb over the floor, 5 s on: {heights[0]:.1f} dB raw, {heights[1]:.1f} dB cleaned")
     assert heights[0] > 10.0 and heights[1] < 6.0
+
+
+def test_ripple_sidebands_are_tracked_and_taken_out():
+    """comb_tracker.cpp sidebands: every line of the drifting comb amplitude-
+    modulated at 500 Hz (depth 0.6: sidebands 10.5 dB under their carriers);
+    the detection reports the ripple and one order, each pass holds its
+    carriers with their sidebands (the other lines' leakage taken out of each
+    line's block sum in closed form), steers the ripple by the sidebands' phase
+    errors past the carriers' fit, and cancels carriers and sidebands alike."""
+    ripple = 500.0
+    x = cl.render_rippled(4.0, ripple, 0.6)
+    table = comb_table()
+    bank = CombBank(cl.RATE)
+    found_first = []
+
+    def detect(last_second):
+        found = detect_combs(cl.spectrum(last_second, 16384), cl.RATE, bank.active())
+        found_first.append(found)
+        bank.detected(found)
+
+    cleaned = cl.run(x, bank, table, detect=detect)
+    first = found_first[0]
+    assert first and all(abs(e.ripple_hz - ripple) < 5.0 and e.sideband_orders == 1 for e in first)
+    st = bank.status()
+    assert st.lock == _lib.COMB_LOCKED
+    assert st.lines_cancelled >= 3 * 35, st.lines_cancelled
+    assert abs(st.comb.ripple_hz - ripple) < 1.0 and st.comb.sideband_orders == 1
+    end = (len(x) // 65536) * 65536 / cl.RATE
+
+    def height(sig, sides):
+        seg = sig[int(MEASURE_FROM * cl.RATE):int(end * cl.RATE)]
+        p = cl.spectrum(seg, 16384)
+        med, mid = np.median(p), 0.5 * (MEASURE_FROM + end)
+        h = [10 * np.log10(p[int(round((cl.offset_at(mid) + k * cl.spacing_at(mid) + sgn * ripple)
+                                        / (cl.RATE / 16384))) + 8192] / med)
+             for k in range(cl.K_LO, cl.K_HI + 1) for sgn in sides]
+        return float(np.mean(h))
+
+    carriers = (height(x, (0,)), height(cleaned, (0,)))
+    sidebands = (height(x, (-1, 1)), height(cleaned, (-1, 1)))
+    print(f"carriers {carriers[0]:.1f} -> {carriers[1]:.1f} dB, sidebands {sidebands[0]:.1f} -> "
+          f"{sidebands[1]:.1f} dB over the floor; {st.lines_cancelled} lines cancelled, ripple "
+          f"{st.comb.ripple_hz:.3f} Hz")
+    assert carriers[0] > 20.0 and carriers[1] < 6.0
+    assert sidebands[0] > 12.0 and sidebands[1] < 6.0
