@@ -398,14 +398,19 @@ double overlap_with(const rvx_comb_estimate& e, const rvx_comb_estimate& held, d
     if (e.k_hi < e.k_lo) return 0.0;
     uint32_t on = 0;
     for (int32_t k = e.k_lo; k <= e.k_hi; ++k) {
-        const double f = e.offset_hz + double(k) * e.spacing_hz;
+        const double f = e.offset_hz + double(k) * e.spacing_hz + double(k) * k * e.curvature_hz;
         if (held.spacing_hz <= 0) {
             if (std::abs(f - held.offset_hz) <= tol_hz) ++on;
             continue;
         }
-        const double kk = std::round((f - held.offset_hz) / held.spacing_hz);
+        // the held grid's line nearest f: round against the straight grid, then
+        // correct once for the curvature at that k
+        const double k0 = std::round((f - held.offset_hz) / held.spacing_hz);
+        const double kk =
+            std::round((f - held.offset_hz - held.curvature_hz * k0 * k0) / held.spacing_hz);
         if (kk < held.k_lo || kk > held.k_hi) continue;
-        if (std::abs(f - (held.offset_hz + kk * held.spacing_hz)) <= tol_hz) ++on;
+        const double g = held.offset_hz + kk * held.spacing_hz + kk * kk * held.curvature_hz;
+        if (std::abs(f - g) <= tol_hz) ++on;
     }
     return double(on) / double(e.k_hi - e.k_lo + 1);
 }
@@ -622,7 +627,85 @@ void robust_fit(const std::vector<double>& k, const std::vector<double>& y,
     }
 }
 
+// Weighted least squares y = a + b k + e k^2 (e = 0 unless `curved`), with the
+// same Huber reweighting as robust_fit; rms is the weighted RMS residual of
+// the last pass.
+void robust_fit3(const std::vector<double>& k, const std::vector<double>& y,
+                 std::vector<double> w, bool curved, double& a, double& b, double& e,
+                 double& rms) {
+    a = b = e = rms = 0.0;
+    if (!curved) {
+        robust_fit(k, y, w, false, a, b);
+    } else {
+        for (int pass = 0; pass < 3; ++pass) {
+            double S[5] = {0, 0, 0, 0, 0}, T[3] = {0, 0, 0};
+            for (size_t i = 0; i < k.size(); ++i) {
+                double kp = 1.0;
+                for (int j = 0; j < 5; ++j, kp *= k[i]) {
+                    S[j] += w[i] * kp;
+                    if (j < 3) T[j] += w[i] * kp * y[i];
+                }
+            }
+            if (S[0] <= 0) return;
+            double A[3][4] = {{S[0], S[1], S[2], T[0]}, {S[1], S[2], S[3], T[1]},
+                              {S[2], S[3], S[4], T[2]}};
+            bool ok = true;
+            for (int c = 0; c < 3 && ok; ++c) {  // partial pivoting
+                int piv = c;
+                for (int r = c + 1; r < 3; ++r)
+                    if (std::abs(A[r][c]) > std::abs(A[piv][c])) piv = r;
+                if (std::abs(A[piv][c]) <= 1e-12 * S[0]) ok = false;
+                if (!ok) break;
+                for (int j = 0; j < 4; ++j) std::swap(A[c][j], A[piv][j]);
+                for (int r = 0; r < 3; ++r)
+                    if (r != c) {
+                        const double f = A[r][c] / A[c][c];
+                        for (int j = 0; j < 4; ++j) A[r][j] -= f * A[c][j];
+                    }
+            }
+            if (!ok) {  // too few distinct k for a curve
+                robust_fit(k, y, w, false, a, b);
+                e = 0.0;
+                break;
+            }
+            a = A[0][3] / A[0][0];
+            b = A[1][3] / A[1][1];
+            e = A[2][3] / A[2][2];
+            if (pass == 2) break;
+            std::vector<double> r;
+            for (size_t i = 0; i < k.size(); ++i)
+                if (w[i] > 0) r.push_back(std::abs(y[i] - a - b * k[i] - e * k[i] * k[i]));
+            if (r.size() < 3) break;
+            const double med = quantile(r, 0.5);
+            if (med <= 0) break;
+            for (size_t i = 0; i < k.size(); ++i) {
+                const double ri = std::abs(y[i] - a - b * k[i] - e * k[i] * k[i]);
+                if (ri > 2.5 * med) w[i] *= 2.5 * med / ri;
+            }
+        }
+    }
+    double sw = 0.0, s2 = 0.0;
+    for (size_t i = 0; i < k.size(); ++i) {
+        const double ri = y[i] - a - b * k[i] - e * k[i] * k[i];
+        sw += w[i];
+        s2 += w[i] * ri * ri;
+    }
+    rms = sw > 0 ? std::sqrt(s2 / sw) : 0.0;
+}
+
 constexpr uint32_t kGrowCarriers = 16;  // lines a locked pass may grow by in one block
+// The curved model c + k d + k^2 e (comb_tracker.cpp choose_model): taken after
+// 4 blocks in a row in which the quadratic fit of the carriers' frequency
+// errors has under half the straight fit's mean square and bends the outermost
+// line by over 0.05 cycles a block (with at least 24 lines over 16 k); let go
+// after 64 blocks in which the curvature held bends it by under 0.02.
+constexpr uint32_t kCurveMinLines = 24;
+constexpr double kCurveMinSpan = 16.0;
+constexpr double kCurveGain = 0.5;
+constexpr double kCurveOnCycles = 0.05;
+constexpr double kCurveOffCycles = 0.02;
+constexpr uint32_t kCurveVotes = 4;
+constexpr uint32_t kStraightVotes = 64;
 
 class Tracker {
 public:
@@ -659,6 +742,9 @@ public:
         // orders as leave room for RVX_COMB_MIN_LINES carriers.
         orders_ = 0;
         r_ = {};
+        e_ = {};
+        curved_ = false;
+        curve_votes_ = straight_votes_ = 0;
         if (!spur_ && e.ripple_hz > 0.0 && e.sideband_orders > 0) {
             orders_ = std::min<int32_t>(e.sideband_orders,
                                         int32_t((max_lines / RVX_COMB_MIN_LINES - 1) / 2));
@@ -773,6 +859,7 @@ public:
         c_ = advanced(c_, int64_t(m - anchor_));
         d_ = advanced(d_, int64_t(m - anchor_));
         r_ = advanced(r_, int64_t(m - anchor_));
+        e_ = advanced(e_, int64_t(m - anchor_));
         anchor_ = m;
         Record r;
         r.start = start;
@@ -783,8 +870,9 @@ public:
         for (size_t slot = 0; slot < lines_.size(); ++slot) {
             const Line& l = lines_[slot];
             const uint64_t uk = uint64_t(int64_t(l.k)), um = uint64_t(int64_t(l.m));
-            const uint64_t ph = c_.phase + uk * d_.phase + um * r_.phase;
-            const uint64_t fr = c_.freq + uk * d_.freq + um * r_.freq;
+            const uint64_t uk2 = uint64_t(int64_t(l.k) * int64_t(l.k));
+            const uint64_t ph = c_.phase + uk * d_.phase + uk2 * e_.phase + um * r_.phase;
+            const uint64_t fr = c_.freq + uk * d_.freq + uk2 * e_.freq + um * r_.freq;
             const uint32_t step = uint32_t((fr + (uint64_t(1) << 31)) >> 32);
             const uint32_t mid = uint32_t((ph + (uint64_t(1) << 31)) >> 32);
             const uint32_t phase0 = mid - (count / 2) * step;
@@ -824,6 +912,7 @@ public:
         const uint64_t m = start + count / 2;
         const Axis mc = advanced(c_, int64_t(m - anchor_)), md = advanced(d_, int64_t(m - anchor_));
         const Axis mr = advanced(r_, int64_t(m - anchor_));
+        const Axis me = advanced(e_, int64_t(m - anchor_));
         const double L = double(count) / double(subs);
         const size_t n = r.lines.size();
         std::vector<double> ks(n), ef(n), ep(n), w(n), snr_of(n);
@@ -873,11 +962,13 @@ public:
             const uint64_t uk = uint64_t(int64_t(pl.k));
             const double f_plan = double(int32_t(pl.step)) / kTwo32;
             const double f_meas = f_plan + std::arg(D) / (kTwoPi * L);
-            const double f_model = signed_cycles(mc.freq + uk * md.freq);
+            const uint64_t uk2 = uint64_t(int64_t(pl.k) * int64_t(pl.k));
+            const double f_model = signed_cycles(mc.freq + uk * md.freq + uk2 * me.freq);
             ef[i] = wrap_half(f_meas - f_model);
             const double ph_plan = double(uint32_t(pl.phase0 + (count / 2) * pl.step)) / kTwo32;
             const uint64_t um = uint64_t(int64_t(pl.m));
-            const double ph_model = double(mc.phase + uk * md.phase + um * mr.phase) / kTwo64;
+            const double ph_model =
+                double(mc.phase + uk * md.phase + uk2 * me.phase + um * mr.phase) / kTwo64;
             const Cd a = P / double(count);
             vs_model[i] = a * std::polar(1.0, kTwoPi * (ph_plan - ph_model));
             ep[i] = l.seeded ? std::arg(vs_model[i] / l.amp) / kTwoPi : 0.0;
@@ -904,13 +995,19 @@ public:
             }
         kbar = sw > 0 ? kbar / sw : 0.0;
         const double span = sw > 0 ? kmax - kmin : 0.0;
-        double fa, fb;
-        robust_fit(ks, ef, w, one, fa, fb);
-        const double f_centre = std::abs(fa + fb * kbar) * T, f_span = std::abs(fb) * span * T;
+        if (!one) choose_model(ks, ef, w, kmin, kmax, T);
+        double fa, fb, fe = 0.0, frms = 0.0;
+        if (one)
+            robust_fit(ks, ef, w, true, fa, fb);
+        else
+            robust_fit3(ks, ef, w, curved_, fa, fb, fe, frms);
+        const double f_centre = std::abs(fa + fb * kbar + fe * kbar * kbar) * T;
+        const double f_span = (std::abs(fb) * span + std::abs(fe) * 0.25 * span * span) * T;
         if (stage_ == kFrequency) {
             // frequency-locked: freq += 0.5 e, rate += e / 16 per block
             correct(c_, anchor_, m, 0.0, 0.5 * fa, fa / (16.0 * T));
             if (!one) correct(d_, anchor_, m, 0.0, 0.5 * fb, fb / (16.0 * T));
+            if (curved_) correct(e_, anchor_, m, 0.0, 0.5 * fe, fe / (16.0 * T));
             stage_settle_ = (sw > 0 && f_centre < kStageExit && f_span < kStageExit &&
                              strong >= min_lines_)
                                 ? stage_settle_ + 1
@@ -933,12 +1030,15 @@ public:
                 if (!lines_[i].seeded) wp[i] = 0.0;
                 fitted += wp[i] > 0 ? 1u : 0u;
             }
-            double pa, pb;
-            robust_fit(ks, ep, wp, one, pa, pb);
+            double pa, pb, pe = 0.0, prms = 0.0;
+            if (one)
+                robust_fit(ks, ep, wp, true, pa, pb);
+            else
+                robust_fit3(ks, ep, wp, curved_, pa, pb, pe, prms);
             double res = 0.0, inno = 0.0;
             for (size_t i = 0; i < n; ++i) {
                 if (!lines_[i].strong || !lines_[i].seeded) continue;
-                const double fit = pa + pb * ks[i];
+                const double fit = pa + pb * ks[i] + pe * ks[i] * ks[i];
                 const double off = std::abs(wrap_half(ep[i] - fit)) * kTwoPi;
                 Line& l = lines_[i];
                 if (off > kBenchRad) {
@@ -963,6 +1063,7 @@ public:
             const double gamma = (1 - rho) * (1 - rho) * (1 - rho);
             correct(c_, anchor_, m, alpha * pa, beta * pa / T, gamma * pa / (T * T));
             if (!one) correct(d_, anchor_, m, alpha * pb, beta * pb / T, gamma * pb / (T * T));
+            if (curved_) correct(e_, anchor_, m, alpha * pe, beta * pe / T, gamma * pe / (T * T));
             // The ripple: a sideband's phase error past what the carriers' fit
             // gives its k is m times the ripple's; the weighted least-squares
             // slope through the origin in m, weighted by the carrier's SNR.
@@ -976,7 +1077,7 @@ public:
                     const size_t ci = size_t(ptrdiff_t(i) - r.lines[i].m);
                     if (ci >= n || snr_of[ci] <= kStandOut) continue;
                     const double mm = double(r.lines[i].m);
-                    const double y = wrap_half(ep[i] - (pa + pb * ks[i]));
+                    const double y = wrap_half(ep[i] - (pa + pb * ks[i] + pe * ks[i] * ks[i]));
                     swy += snr_of[ci] * y * mm;
                     sww += snr_of[ci] * mm * mm;
                     ++used;
@@ -990,7 +1091,8 @@ public:
             for (size_t i = 0; i < n; ++i) {
                 Line& l = lines_[i];
                 const Cd derot = vs_model[i] * std::polar(
-                    1.0, -kTwoPi * (pa + pb * ks[i] + double(r.lines[i].m) * ripple));
+                    1.0, -kTwoPi * (pa + pb * ks[i] + pe * ks[i] * ks[i] +
+                                    double(r.lines[i].m) * ripple));
                 if (l.seeded) {
                     l.amp += kAmpWeight * (derot - l.amp);
                 } else {  // a grown line: seeded from its first locked block
@@ -1070,11 +1172,47 @@ public:
         return true;
     }
 
+    // Straight or curved (see kCurveVotes), judged on the carriers' frequency
+    // errors each block: the curve is taken when its fit wins decisively and
+    // bends the outer lines, and dropped, set to exactly zero, once what it
+    // holds bends them negligibly (judged on the model, not the innovations,
+    // which go quiet once the curvature is tracked).
+    void choose_model(const std::vector<double>& ks, const std::vector<double>& ys,
+                      const std::vector<double>& w, double kmin, double kmax, double T) {
+        uint32_t lines = 0;
+        for (double x : w) lines += x > 0 ? 1u : 0u;
+        if (spur_ || lines < kCurveMinLines || !(kmax - kmin >= kCurveMinSpan)) {
+            curve_votes_ = 0;
+            return;
+        }
+        const double half = 0.5 * (kmax - kmin);
+        if (!curved_) {
+            double a, b, e, rms_s, rms_c;
+            robust_fit3(ks, ys, w, false, a, b, e, rms_s);
+            robust_fit3(ks, ys, w, true, a, b, e, rms_c);
+            const double bend = std::abs(e) * half * half * T;
+            const bool better = rms_c * rms_c < kCurveGain * rms_s * rms_s && bend > kCurveOnCycles;
+            curve_votes_ = better ? curve_votes_ + 1 : 0;
+            if (curve_votes_ >= kCurveVotes) {
+                curved_ = true;
+                curve_votes_ = straight_votes_ = 0;
+            }
+            return;
+        }
+        const double bend = std::abs(signed_cycles(e_.freq)) * half * half * T;
+        straight_votes_ = bend < kCurveOffCycles ? straight_votes_ + 1 : 0;
+        if (straight_votes_ >= kStraightVotes) {
+            e_ = {};
+            curved_ = false;
+            straight_votes_ = 0;
+        }
+    }
+
     void begin_recovery(uint64_t m) {
         lock_ = RVX_COMB_RECOVERING;
         stage_ = kFrequency;
         stage_settle_ = settle_ = bad_ = 0;
-        c_.rate = d_.rate = r_.rate = 0;  // frequencies held, rates dropped
+        c_.rate = d_.rate = r_.rate = e_.rate = 0;  // frequencies held, rates dropped
         for (Line& l : lines_) l.seeded = false;
         recovering_since_ = m;
         recentred_ = false;
@@ -1092,6 +1230,7 @@ public:
             }
             e.lines_found = uint32_t(lines_.size()) / per_line();
             e.ripple_hz = orders_ > 0 ? signed_cycles(r_.freq) * double(rate_) : 0.0;
+            e.curvature_hz = curved_ ? signed_cycles(e_.freq) * double(rate_) : 0.0;
             e.sideband_orders = orders_;
         }
         return e;
@@ -1110,7 +1249,8 @@ public:
         const rvx_comb_estimate mine = estimate();
         for (const Line& l : lines_)
             if (std::abs(mine.offset_hz + double(l.k) * mine.spacing_hz +
-                         double(l.m) * mine.ripple_hz - f_hz) <= tol_hz)
+                         double(l.k) * l.k * mine.curvature_hz + double(l.m) * mine.ripple_hz -
+                         f_hz) <= tol_hz)
                 return true;
         return false;
     }
@@ -1151,6 +1291,9 @@ private:
     bool anchored_ = false;
     uint64_t anchor_ = 0;
     Axis c_{}, d_{}, r_{};  // offset, spacing, ripple (r_ zero without sidebands)
+    Axis e_{};              // curvature: exactly zero while the straight model is in use
+    bool curved_ = false;
+    uint32_t curve_votes_ = 0, straight_votes_ = 0;
     int32_t orders_ = 0;
     std::vector<Line> lines_;
     std::vector<Record> records_;

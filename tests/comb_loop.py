@@ -142,3 +142,16 @@ def render_rippled(seconds: float, ripple_hz: float, depth: float, seed: int = S
         env = 1.0 + depth * np.cos(2 * np.pi * ripple_hz * t + 2 * np.pi * g.uniform())
         x += LINE_AMPLITUDE * env * np.exp(2j * np.pi * (cyc - np.floor(cyc)))
     return x.astype(np.complex64)
+
+
+def render_curved(seconds: float, curvature_hz: float, seed: int = SEED + 4) -> np.ndarray:
+    """The drifting comb over noise with line k moved by k^2 curvature_hz: a
+    comb whose spacing changes across it (c + k d + k^2 e)."""
+    g = np.random.default_rng(seed)
+    n = int(seconds * RATE)
+    x = (g.normal(0.0, NOISE_RMS, n) + 1j * g.normal(0.0, NOISE_RMS, n)).astype(np.complex128)
+    t = np.arange(n) / RATE
+    for k in range(K_LO, K_HI + 1):
+        cyc = cycles_at(k, t) + k * k * curvature_hz * t + g.uniform()
+        x += LINE_AMPLITUDE * np.exp(2j * np.pi * (cyc - np.floor(cyc)))
+    return x.astype(np.complex64)

@@ -335,3 +335,40 @@ def test_ripple_sidebands_are_tracked_and_taken_out():
           f"{st.comb.ripple_hz:.3f} Hz")
     assert carriers[0] > 20.0 and carriers[1] < 6.0
     assert sidebands[0] > 12.0 and sidebands[1] < 6.0
+
+
+def test_a_curved_comb_is_tracked_with_its_curvature():
+    """comb_tracker.cpp choose_model: line k of the comb at c + k d + k^2 e with
+    e = 0.05 Hz (20 Hz off a straight grid at the outer lines, 0.7 cycles a
+    block: a straight pass never locks on it, residual 1.7 rad). The pass votes
+    the curved model in on the carriers' frequency errors, tracks e, locks and
+    takes every line out."""
+    from revenant_amd.comb import detect_comb
+    e = 0.05
+    x = cl.render_curved(4.0, e)
+    table = comb_table()
+    bank = CombBank(cl.RATE)
+
+    def detect(last_second):
+        whole = detect_comb(cl.spectrum(last_second, 16384), cl.RATE)
+        bank.detected([whole] if whole is not None else [])
+
+    cleaned = cl.run(x, bank, table, detect=detect)
+    st = bank.status()
+    assert st.lock == _lib.COMB_LOCKED
+    assert abs(st.comb.curvature_hz - e) < 0.002, st.comb.curvature_hz
+    assert st.lines_cancelled >= 35
+    end = (len(x) // 65536) * 65536 / cl.RATE
+
+    def height(sig):
+        seg = sig[int(MEASURE_FROM * cl.RATE):int(end * cl.RATE)]
+        p = cl.spectrum(seg, 4096)
+        med, mid = np.median(p), 0.5 * (MEASURE_FROM + end)
+        return float(np.mean([10 * np.log10(p[int(round(
+            (cl.offset_at(mid) + k * cl.spacing_at(mid) + k * k * e) / (cl.RATE / 4096))) + 2048]
+            / med) for k in range(cl.K_LO, cl.K_HI + 1)]))
+
+    before, after = height(x), height(cleaned)
+    print(f"curved comb: {before:.1f} -> {after:.1f} dB over the floor, curvature "
+          f"{st.comb.curvature_hz:.4f} Hz, {st.lines_cancelled} lines")
+    assert before > 20.0 and after < 6.0
