@@ -1337,6 +1337,9 @@ typedef struct rvx_event_config {
     double bootstrap_hold_seconds;   /* 3 (:955) */
     uint32_t birth_hits;             /* 3 (:918) */
     uint32_t max_tracks;             /* 512 (:938) */
+    uint32_t grid_channels;          /* the channelizer's M for a track's channel; 0: none (:591) */
+    uint32_t reserved;
+    double channel_hysteresis;       /* 0.15 of a channel past the midpoint (:1205) */
 } rvx_event_config;
 
 /* detector.h:299-360 Candidate (the fields built here) */
@@ -1344,6 +1347,9 @@ typedef struct rvx_candidate {
     uint32_t first_bin, last_bin;
     int64_t center_hz, bandwidth_hz;
     double snr_2500_db, noise_floor_dbfs, peak_dbfs;
+    /* characterise::margin_confidence(snr_2500_db, detection_threshold_db):
+     * 0 under the threshold, else 1 - exp(-(snr - threshold) / 6 dB) / 2 */
+    double margin_confidence;
 } rvx_candidate;
 
 #define RVX_TRACK_PENDING 0u
@@ -1361,6 +1367,19 @@ typedef struct rvx_track {
     uint64_t first_seen, last_seen, last_detected;  /* sample indices */
     uint64_t merged_into;        /* parent id while merged */
     uint32_t first_bin, last_bin;  /* the last candidate's bins */
+    double margin_confidence;    /* of the smoothed snr_2500_db, as the candidate's */
+    /* The band's level in each spectrum frame (rvx_tracker_measure_levels): a
+     * running mean and sum of squared deviations (Welford) of 10 log10 of the
+     * summed linear power over level_first_bin .. level_last_bin, restarted
+     * when the track's bins change; sigma = sqrt(level_m2 / (frames - 1)). */
+    double level_mean_db, level_m2;
+    uint32_t level_first_bin, level_last_bin, level_frames, level_bins_known;
+    /* The grid channel nearest the centre (with grid_channels set): moved
+     * only when the centre is more than 0.5 + channel_hysteresis channels
+     * from the one held, so a track on a boundary does not flip. */
+    int64_t channel_index;       /* unwrapped: round((centre - source centre) M / rate) */
+    uint32_t channel;            /* channel_index reduced into [0, M) */
+    uint32_t channel_valid;
 } rvx_track;
 
 typedef struct rvx_tracker_stats {
@@ -1380,6 +1399,9 @@ void rvx_tracker_destroy(rvx_tracker* tracker);
 /* One decision at sample index `now`, elapsed_seconds after the previous one:
  * residual rejection over the candidates, then the tracks' update.
  * *n_kept (may be NULL): candidates left after residual rejection. */
+/* Detector::measure_levels (detector.cpp:629-646): one spectrum frame (dB,
+ * `bins` floats) into every live track's level statistics. */
+int32_t rvx_tracker_measure_levels(rvx_tracker* tracker, const float* power_db, uint32_t bins);
 int32_t rvx_tracker_update(rvx_tracker* tracker, const double* average, const double* floor,
                            const rvx_candidate* candidates, uint32_t n, uint64_t now,
                            double elapsed_seconds, uint32_t* n_kept);

@@ -335,12 +335,14 @@ class EventConfig(C.Structure):
                 ("residual_rate_tolerance", f64), ("association_overlap", f64),
                 ("confidence_rise", f64), ("confidence_half_life_seconds", f64),
                 ("drop_confidence", f64), ("centre_smoothing", f64),
-                ("bootstrap_hold_seconds", f64), ("birth_hits", u32), ("max_tracks", u32)]
+                ("bootstrap_hold_seconds", f64), ("birth_hits", u32), ("max_tracks", u32),
+                ("grid_channels", u32), ("reserved", u32), ("channel_hysteresis", f64)]
 
 
 class Candidate(C.Structure):
     _fields_ = [("first_bin", u32), ("last_bin", u32), ("center_hz", i64), ("bandwidth_hz", i64),
-                ("snr_2500_db", f64), ("noise_floor_dbfs", f64), ("peak_dbfs", f64)]
+                ("snr_2500_db", f64), ("noise_floor_dbfs", f64), ("peak_dbfs", f64),
+                ("margin_confidence", f64)]
 
 
 class Track(C.Structure):
@@ -348,7 +350,10 @@ class Track(C.Structure):
                 ("held_spells", u32), ("center_hz", i64), ("bandwidth_hz", i64),
                 ("snr_2500_db", f64), ("confidence", f64), ("first_seen", u64),
                 ("last_seen", u64), ("last_detected", u64), ("merged_into", u64),
-                ("first_bin", u32), ("last_bin", u32)]
+                ("first_bin", u32), ("last_bin", u32), ("margin_confidence", f64),
+                ("level_mean_db", f64), ("level_m2", f64), ("level_first_bin", u32),
+                ("level_last_bin", u32), ("level_frames", u32), ("level_bins_known", u32),
+                ("channel_index", i64), ("channel", u32), ("channel_valid", u32)]
 
 
 class TrackerStats(C.Structure):
@@ -561,6 +566,7 @@ _SIGNATURES = {
                              C.POINTER(Candidate), u32, C.POINTER(u32)]),
     "rvx_tracker_create": (i32, [C.POINTER(EventConfig), u32, C.POINTER(vp)]),
     "rvx_tracker_destroy": (None, [vp]),
+    "rvx_tracker_measure_levels": (i32, [vp, fptr, u32]),
     "rvx_tracker_update": (i32, [vp, vp, vp, C.POINTER(Candidate), u32, u64, f64,
                                  C.POINTER(u32)]),
     "rvx_tracker_tracks": (i32, [vp, C.POINTER(Track), u32, C.POINTER(u32)]),

@@ -52,6 +52,20 @@ struct Sums {
     double floor_over(size_t first, size_t last) const { return floor[last + 1] - floor[first]; }
 };
 
+// characterise::margin_confidence (transform.cpp:272-279): 0 under the
+// threshold, then rising from one half toward one with a 6 dB scale.
+double margin_confidence(double margin_db, double threshold_db) {
+    if (margin_db < threshold_db) return 0.0;
+    return 1.0 - 0.5 * std::exp(-(margin_db - threshold_db) / 6.0);
+}
+
+// round(n / d) with halves away from zero, in integers (detector.cpp:82-88).
+int64_t divide_nearest(int64_t n, int64_t d) {
+    if (d == 0) return 0;
+    const int64_t half = d / 2;
+    return n >= 0 ? (n + half) / d : (n - half) / d;
+}
+
 int32_t check_config(const rvx_event_config& c, const char* who) {
     if (!(c.bin_width_hz > 0.0) || !std::isfinite(c.bin_zero_hz))
         return rvx::fail(RVX_EINVAL, "%s: bin width %g Hz", who, c.bin_width_hz);
@@ -64,6 +78,8 @@ int32_t check_config(const rvx_event_config& c, const char* who) {
         c.source_rate <= 0)
         return rvx::fail(RVX_EINVAL, "%s: time constants and the source rate must be positive",
                          who);
+    if (!(c.channel_hysteresis >= 0.0) || !std::isfinite(c.channel_hysteresis))
+        return rvx::fail(RVX_EINVAL, "%s: channel hysteresis %g", who, c.channel_hysteresis);
     return RVX_OK;
 }
 
@@ -86,6 +102,7 @@ bool emit(const rvx_event_config& c, const Sums& s, const double* avg, uint32_t 
     k.first_bin = uint32_t(first);
     k.last_bin = uint32_t(last);
     k.snr_2500_db = 10.0 * std::log10(snr);
+    k.margin_confidence = margin_confidence(k.snr_2500_db, c.detection_threshold_db);
     k.noise_floor_dbfs = to_db(noise);
     k.peak_dbfs = to_db(peak);
     k.center_hz = c.source_center_hz + int64_t(std::llround(0.5 * (low_edge + high_edge)));
@@ -341,8 +358,12 @@ struct rvx_tracker {
             tr.center_hz = k.center_hz;
             tr.bandwidth_hz = std::max<int64_t>(1, k.bandwidth_hz);
             tr.snr_2500_db = k.snr_2500_db;
+            tr.margin_confidence = k.margin_confidence;
             tr.first_bin = k.first_bin;
             tr.last_bin = k.last_bin;
+            tr.level_first_bin = k.first_bin;
+            tr.level_last_bin = k.last_bin;
+            tr.level_bins_known = 1;
             tr.confidence = cfg.confidence_rise;
             tr.first_seen = tr.last_seen = tr.last_detected = now;
             tr.hits = 1;
@@ -350,6 +371,7 @@ struct rvx_tracker {
                 tr.state = RVX_TRACK_LIVE;
                 ++stats.tracks_born;
             }
+            update_channel(tr);
             kept.push_back(tr);
         }
         tracks.swap(kept);
@@ -404,8 +426,18 @@ struct rvx_tracker {
         tr.center_hz = int64_t(std::llround(centre));
         tr.bandwidth_hz = std::max<int64_t>(1, int64_t(std::llround(width)));
         tr.snr_2500_db += a * (k.snr_2500_db - tr.snr_2500_db);
+        tr.margin_confidence = margin_confidence(tr.snr_2500_db, cfg.detection_threshold_db);
         tr.first_bin = k.first_bin;
         tr.last_bin = k.last_bin;
+        // a level over other bins is another quantity: it starts again
+        if (tr.level_bins_known &&
+            (tr.level_first_bin != k.first_bin || tr.level_last_bin != k.last_bin)) {
+            tr.level_frames = 0;
+            tr.level_mean_db = tr.level_m2 = 0.0;
+        }
+        tr.level_first_bin = k.first_bin;
+        tr.level_last_bin = k.last_bin;
+        tr.level_bins_known = 1;
         ++tr.hits;
         tr.misses = 0;
         tr.last_detected = tr.last_seen = now;
@@ -417,6 +449,50 @@ struct rvx_tracker {
             }
         } else {
             tr.state = RVX_TRACK_LIVE;
+        }
+        update_channel(tr);
+    }
+
+    // update_channel (detector.cpp:1978-2007): the nearest grid channel, moved
+    // only past 0.5 + channel_hysteresis channels from the one held (place()
+    // rounds to the nearest, so a track parked on a boundary would flip with
+    // every hertz its measurement moves, each flip a new channel and table).
+    void update_channel(rvx_track& tr) const {
+        if (cfg.grid_channels == 0) {
+            tr.channel_valid = 0;
+            tr.channel = 0;
+            tr.channel_index = 0;
+            return;
+        }
+        const int64_t channels = int64_t(cfg.grid_channels);
+        const int64_t baseband = tr.center_hz - cfg.source_center_hz;
+        const double exact = double(baseband) * double(channels) / double(cfg.source_rate);
+        const double distance = std::abs(exact - double(tr.channel_index));
+        if (!tr.channel_valid || distance > 0.5 + cfg.channel_hysteresis) {
+            tr.channel_index = divide_nearest(baseband * channels, cfg.source_rate);
+            tr.channel_valid = 1;
+        }
+        int64_t wrapped = tr.channel_index % channels;
+        if (wrapped < 0) wrapped += channels;
+        tr.channel = uint32_t(wrapped);
+    }
+
+    // measure_levels (detector.cpp:629-646): every live track's band level in
+    // this frame, summed in linear power (each bin's dB times log2(10) / 10,
+    // in float, through exp2), into its running mean and squared deviations.
+    void measure_levels(const float* power, uint32_t n) {
+        constexpr float kDbToLog2 = 0.332192809488736234787f;
+        for (rvx_track& tr : tracks) {
+            if (tr.state != RVX_TRACK_LIVE || !tr.level_bins_known || tr.level_last_bin >= n)
+                continue;
+            double sum = 0.0;
+            for (uint32_t i = tr.level_first_bin; i <= tr.level_last_bin; ++i)
+                sum += double(std::exp2(power[i] * kDbToLog2));
+            const double level = 10.0 * std::log10(std::max(sum, 1e-30));
+            ++tr.level_frames;
+            const double delta = level - tr.level_mean_db;
+            tr.level_mean_db += delta / double(tr.level_frames);
+            tr.level_m2 += delta * (level - tr.level_mean_db);
         }
     }
 };
@@ -441,6 +517,8 @@ void rvx_event_config_defaults(rvx_event_config* c) {
     c->confidence_half_life_seconds = 1.0; // :1195
     c->drop_confidence = 0.05;             // :1196
     c->centre_smoothing = 0.4;             // :1200
+    c->grid_channels = 0;                  // :591
+    c->channel_hysteresis = 0.15;          // :1205
 }
 
 int32_t rvx_candidates(const rvx_event_config* config, const double* average, const double* floor,
@@ -491,6 +569,13 @@ int32_t rvx_tracker_create(const rvx_event_config* config, uint32_t bins, rvx_tr
 }
 
 void rvx_tracker_destroy(rvx_tracker* tracker) { delete tracker; }
+
+int32_t rvx_tracker_measure_levels(rvx_tracker* t, const float* power_db, uint32_t bins) {
+    if (!t || (bins && !power_db))
+        return rvx::fail(RVX_EINVAL, "rvx_tracker_measure_levels: null argument");
+    t->measure_levels(power_db, bins);
+    return RVX_OK;
+}
 
 int32_t rvx_tracker_update(rvx_tracker* t, const double* average, const double* floor,
                            const rvx_candidate* candidates, uint32_t n, uint64_t now,

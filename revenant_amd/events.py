@@ -57,6 +57,13 @@ class Tracker:
                                                C.byref(kept)))
         return kept.value
 
+    def measure_levels(self, power_db) -> None:
+        """One spectrum frame (dB per bin) into every live track's level
+        statistics (Detector::measure_levels)."""
+        p = np.ascontiguousarray(power_db, dtype=np.float32)
+        _lib.check(self.lib.rvx_tracker_measure_levels(self.handle, p.ctypes.data_as(_lib.fptr),
+                                                       p.size))
+
     def tracks(self) -> list[_lib.Track]:
         n = C.c_uint32()
         cap = 1024
