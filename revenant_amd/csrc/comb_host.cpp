@@ -839,8 +839,11 @@ public:
         const double c = anchored_ ? signed_cycles(c_.freq) * rate : est_.offset_hz;
         const double d = anchored_ ? signed_cycles(d_.freq) * rate : est_.spacing_hz;
         if (!(d > 0.0)) return;
+        const double cv = curved_ ? signed_cycles(e_.freq) * rate : 0.0;
         auto to_k = [&](int32_t k) {
-            return int32_t(std::lround((e.offset_hz + double(k) * e.spacing_hz - c) / d));
+            const double f = e.offset_hz + double(k) * e.spacing_hz + double(k) * k * e.curvature_hz;
+            const double k0 = std::round((f - c) / d);
+            return int32_t(std::lround((f - c - cv * k0 * k0) / d));
         };
         const int32_t a = to_k(e.k_lo), b = to_k(e.k_hi);
         range_lo_ = std::min({range_lo_, a, b});
@@ -1160,7 +1163,11 @@ public:
             return false;
         const double c = signed_cycles(c_.freq) * rate, d = signed_cycles(d_.freq) * rate;
         if (!(d > 0.0)) return false;
-        const double j0 = std::round((e.offset_hz - c) / d);  // this pass's k at the estimate's 0
+        // this pass's k at the estimate's 0: rounded against the straight grid,
+        // then corrected once for the curvature the pass holds at that k
+        const double cv = curved_ ? signed_cycles(e_.freq) * rate : 0.0;
+        const double k0 = std::round((e.offset_hz - c) / d);
+        const double j0 = std::round((e.offset_hz - c - cv * k0 * k0) / d);
         c_ = {c_.phase, fixed((e.offset_hz - j0 * e.spacing_hz) / rate), 0};
         d_ = {d_.phase, fixed(e.spacing_hz / rate), 0};
         recentred_ = true;
