@@ -538,7 +538,9 @@ constexpr uint32_t kBenchBlocks = 4, kUnbenchBlocks = 8, kAcquireBlocks = 200;
 constexpr double kRecoverSeconds = 8.0;
 constexpr double kRecentreSeconds = 1.0;  // a recovery left to its loops this long before a detection recentres it
 constexpr double kAmpWeight = 1.0 / 16.0;
-constexpr uint32_t kPlanSubBlocks = 16;
+constexpr uint32_t kPlanSubBlocks = 16;     // the most sub-blocks a plan asks for
+constexpr double kMinSubBlockSeconds = 0.005;  // a narrowed pass's shortest sub-block
+constexpr double kNarrowCycles = 1.0;          // frequency error a block under which a pass narrows
 
 // One parameter (c or d) as an exact polynomial in the sample index: phase
 // (cycles), frequency (cycles per sample) and its rate of change, x 2^64.
@@ -584,6 +586,7 @@ struct Planned {
 struct Record {
     uint64_t start = 0;
     uint32_t count = 0;
+    uint32_t subs = 0;  // the sub-blocks this tracker asked for
     uint64_t gen = 0;
     bool subtract = false;
     std::vector<Planned> lines;
@@ -768,11 +771,24 @@ public:
         }
         good_ = false;
         fitted_ = benched_ = 0;
+        wide_ = true;
         records_.clear();
         observed_ = settle_ = bad_ = stage_settle_ = 0;
         seed_pending_ = false;
         suppression_ = 0.0f;
         cancelled_ = 0;
+    }
+
+    // Sub-blocks of a block's projections: the most (16, at least 16 samples
+    // each) while the pass pulls in wide, a frequency step of up to half a
+    // cycle per sub-block; once its error is under a cycle a block, the most
+    // that keep each at least 5 ms long, for less noise per sub-block.
+    uint32_t pick_sub_blocks(uint32_t count) const {
+        const uint32_t shortest =
+            wide_ ? 16u : std::max<uint32_t>(16u, uint32_t(kMinSubBlockSeconds * double(rate_)));
+        for (uint32_t s = kPlanSubBlocks; s > 1; s /= 2)
+            if (count % s == 0 && count / s >= shortest) return s;
+        return 1;
     }
 
     void stop(uint32_t lock) {
@@ -851,7 +867,8 @@ public:
     }
 
     // Lines of the block [start, start + count), phases at its first sample.
-    void plan(uint64_t start, uint32_t count, std::vector<rvx_comb_line>& out, bool& subtract) {
+    void plan(uint64_t start, uint32_t count, std::vector<rvx_comb_line>& out, bool& subtract,
+              uint32_t& subs) {
         if (!active() || lines_.empty()) return;
         const uint64_t m = start + count / 2;
         if (!anchored_) {
@@ -867,6 +884,8 @@ public:
         Record r;
         r.start = start;
         r.count = count;
+        r.subs = pick_sub_blocks(count);
+        subs = std::max(subs, r.subs);
         r.gen = gen_;
         r.subtract = lock_ == RVX_COMB_LOCKED;
         cancelled_ = 0;
@@ -911,6 +930,18 @@ public:
         if (r.gen != gen_ || !active() || n_lines != r.lines.size() || subs < 2 ||
             r.lines.size() > lines_.size())
             return;
+        // Projections taken over more sub-blocks than this pass asked for (the
+        // bank plans with the most any pass wants): adjacent ones summed back.
+        std::vector<std::complex<float>> merged;
+        if (r.subs >= 2 && subs > r.subs && subs % r.subs == 0) {
+            const uint32_t f = subs / r.subs;
+            merged.assign(size_t(n_lines) * r.subs, std::complex<float>(0, 0));
+            for (uint32_t i = 0; i < n_lines; ++i)
+                for (uint32_t s = 0; s < subs; ++s)
+                    merged[size_t(i) * r.subs + s / f] += proj[size_t(i) * subs + s];
+            proj = merged.data();
+            subs = r.subs;
+        }
         ++observed_;
         const uint64_t m = start + count / 2;
         const Axis mc = advanced(c_, int64_t(m - anchor_)), md = advanced(d_, int64_t(m - anchor_));
@@ -1011,6 +1042,7 @@ public:
             correct(c_, anchor_, m, 0.0, 0.5 * fa, fa / (16.0 * T));
             if (!one) correct(d_, anchor_, m, 0.0, 0.5 * fb, fb / (16.0 * T));
             if (curved_) correct(e_, anchor_, m, 0.0, 0.5 * fe, fe / (16.0 * T));
+            if (wide_ && sw > 0 && f_centre + f_span < kNarrowCycles) wide_ = false;
             stage_settle_ = (sw > 0 && f_centre < kStageExit && f_span < kStageExit &&
                              strong >= min_lines_)
                                 ? stage_settle_ + 1
@@ -1172,6 +1204,7 @@ public:
         d_ = {d_.phase, fixed(e.spacing_hz / rate), 0};
         recentred_ = true;
         stage_ = kFrequency;
+        wide_ = true;  // the detection is coarser than the model was
         stage_settle_ = settle_ = bad_ = 0;
         ++gen_;  // plans already out were made against the old model
         records_.clear();
@@ -1223,6 +1256,7 @@ public:
         for (Line& l : lines_) l.seeded = false;
         recovering_since_ = m;
         recentred_ = false;
+        wide_ = false;  // the model it held is close: long sub-blocks
     }
 
     rvx_comb_estimate estimate() const {
@@ -1308,6 +1342,7 @@ private:
     bool seed_pending_ = false;
     uint64_t recovering_since_ = 0;
     bool recentred_ = false;
+    bool wide_ = true;
     float suppression_ = 0.0f;
     double residual_ = 0.0;
     uint32_t cancelled_ = 0;
@@ -1585,9 +1620,10 @@ int32_t rvx_comb_bank_plan(rvx_comb_bank* bank, uint64_t start, uint32_t count,
     Split split;
     split.start = start;
     split.count = count;
+    uint32_t subs = 1;
     for (Tracker& t : bank->trackers) {
         const size_t before = all.size();
-        t.plan(start, count, all, sub);
+        t.plan(start, count, all, sub, subs);
         if (all.size() > before) split.rows.push_back({t.id(), uint32_t(all.size() - before)});
     }
     if (all.size() > max_lines)
@@ -1596,7 +1632,7 @@ int32_t rvx_comb_bank_plan(rvx_comb_bank* bank, uint64_t start, uint32_t count,
     std::copy(all.begin(), all.end(), lines);
     *n_lines = uint32_t(all.size());
     *subtract = sub ? 1 : 0;
-    if (sub_blocks) *sub_blocks = kPlanSubBlocks;
+    if (sub_blocks) *sub_blocks = all.empty() ? kPlanSubBlocks : subs;
     bank->splits.push_back(std::move(split));
     while (bank->splits.size() > 64) bank->splits.erase(bank->splits.begin());
     return RVX_OK;
