@@ -985,7 +985,9 @@ typedef struct rvx_comb_estimate {
     double ripple_hz;          /* sidebands at multiples of it either side of each line */
     int32_t sideband_orders;
     uint32_t spur;
-    double curvature_hz;       /* always 0: the straight model (see DESIGN) */
+    double curvature_hz;       /* line k at offset + k spacing + k^2 curvature: 0 from a
+                                  detection (straight grids); a pass's own once it has
+                                  voted the curved model in */
 } rvx_comb_estimate;
 
 /* comb_cancel.h:318-331 CombStatus, over the bank's combs (spurs counted apart). */
