@@ -1352,6 +1352,15 @@ typedef struct rvx_candidate {
     /* characterise::margin_confidence(snr_2500_db, detection_threshold_db):
      * 0 under the threshold, else 1 - exp(-(snr - threshold) / 6 dB) / 2 */
     double margin_confidence;
+    /* BandShape (core/detect/shape.h:53-272): measurements of the band's
+     * excess (each bin's average over floor, negatives taken as 0), no
+     * verdicts: peak over mean; the strongest three adjacent bins' share
+     * (1 for bands of three bins or fewer); the lower half's share of the two
+     * halves; skirt bins (outside the band, within twice its width and its own
+     * region between the neighbouring decision bands, over a tenth of its mean)
+     * per band bin, and how many bins the walk had room for. */
+    double shape_peak_to_mean, shape_concentration, shape_lower_fraction, shape_skirt_fraction;
+    uint32_t shape_skirt_bins_available, shape_measured;
 } rvx_candidate;
 
 #define RVX_TRACK_PENDING 0u
@@ -1382,6 +1391,10 @@ typedef struct rvx_track {
     int64_t channel_index;       /* unwrapped: round((centre - source centre) M / rate) */
     uint32_t channel;            /* channel_index reduced into [0, M) */
     uint32_t channel_valid;
+    /* the band shape of the last candidate taken (not smoothed: it stays what
+     * the last detection measured while the track holds) */
+    double shape_peak_to_mean, shape_concentration, shape_lower_fraction, shape_skirt_fraction;
+    uint32_t shape_skirt_bins_available, shape_measured;
 } rvx_track;
 
 typedef struct rvx_tracker_stats {

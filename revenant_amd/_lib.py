@@ -342,7 +342,10 @@ class EventConfig(C.Structure):
 class Candidate(C.Structure):
     _fields_ = [("first_bin", u32), ("last_bin", u32), ("center_hz", i64), ("bandwidth_hz", i64),
                 ("snr_2500_db", f64), ("noise_floor_dbfs", f64), ("peak_dbfs", f64),
-                ("margin_confidence", f64)]
+                ("margin_confidence", f64), ("shape_peak_to_mean", f64),
+                ("shape_concentration", f64), ("shape_lower_fraction", f64),
+                ("shape_skirt_fraction", f64), ("shape_skirt_bins_available", u32),
+                ("shape_measured", u32)]
 
 
 class Track(C.Structure):
@@ -353,7 +356,10 @@ class Track(C.Structure):
                 ("first_bin", u32), ("last_bin", u32), ("margin_confidence", f64),
                 ("level_mean_db", f64), ("level_m2", f64), ("level_first_bin", u32),
                 ("level_last_bin", u32), ("level_frames", u32), ("level_bins_known", u32),
-                ("channel_index", i64), ("channel", u32), ("channel_valid", u32)]
+                ("channel_index", i64), ("channel", u32), ("channel_valid", u32),
+                ("shape_peak_to_mean", f64), ("shape_concentration", f64),
+                ("shape_lower_fraction", f64), ("shape_skirt_fraction", f64),
+                ("shape_skirt_bins_available", u32), ("shape_measured", u32)]
 
 
 class TrackerStats(C.Structure):

@@ -40,12 +40,18 @@ double to_db(double linear) { return 10.0 * std::log10(linear > kPowerFloor ? li
 // Running sums over a decision's spectrum, in bin order (the reference's).
 struct Sums {
     std::vector<double> excess, floor;  // bins + 1 each; excess is signed
+    // Per bin, the excess a band's shape is read from: a bin under its floor
+    // is noise there, not a dip in the signal, so it counts as 0 (the sums
+    // keep it signed: they answer how much power a band holds).
+    std::vector<double> clamped;
     void build(const double* avg, const double* flo, uint32_t bins) {
         excess.assign(size_t(bins) + 1, 0.0);
         floor.assign(size_t(bins) + 1, 0.0);
+        clamped.assign(bins, 0.0);
         for (uint32_t i = 0; i < bins; ++i) {
             excess[i + 1] = excess[i] + (avg[i] - flo[i]);
             floor[i + 1] = floor[i] + flo[i];
+            clamped[i] = std::max(0.0, avg[i] - flo[i]);
         }
     }
     double excess_over(size_t first, size_t last) const { return excess[last + 1] - excess[first]; }
@@ -83,10 +89,57 @@ int32_t check_config(const rvx_event_config& c, const char* who) {
     return RVX_OK;
 }
 
+// measure_band (shape.h:275-372) over the clamped excess of [first, last],
+// looking for skirts no further than [low, high) and twice the band's width.
+void measure_shape(const std::vector<double>& x, size_t first, size_t last, size_t low,
+                   size_t high, rvx_candidate& k) {
+    if (x.empty() || last < first || last >= x.size() || low > first || high < last + 1 ||
+        high > x.size())
+        return;
+    const size_t width = last - first + 1;
+    double total = 0.0, peak = 0.0;
+    for (size_t i = first; i <= last; ++i) {
+        total += x[i];
+        peak = std::max(peak, x[i]);
+    }
+    if (!(total > 0.0)) return;
+    const double mean = total / double(width);
+    k.shape_peak_to_mean = peak / mean;
+    if (width <= 3) {
+        k.shape_concentration = 1.0;
+    } else {
+        double run = x[first] + x[first + 1] + x[first + 2], best = run;
+        for (size_t i = first + 3; i <= last; ++i) {
+            run += x[i] - x[i - 3];
+            best = std::max(best, run);
+        }
+        k.shape_concentration = std::min(1.0, best / total);
+    }
+    const size_t half = width / 2;
+    double lower = 0.0, upper = 0.0;
+    for (size_t i = 0; i < half; ++i) {
+        lower += x[first + i];
+        upper += x[last - i];
+    }
+    k.shape_lower_fraction = lower + upper > 0.0 ? lower / (lower + upper) : 0.5;
+    const double end_level = mean * 0.1;               // kSkirtEndFraction
+    const size_t reach = size_t(double(width) * 2.0);  // kSkirtReachWidths
+    const size_t low_stop = std::min(first - low, reach);
+    const size_t high_stop = std::min(high - (last + 1), reach);
+    size_t below = 0, above = 0;
+    while (below < low_stop && x[first - below - 1] > end_level) ++below;
+    while (above < high_stop && x[last + above + 1] > end_level) ++above;
+    k.shape_skirt_bins_available = uint32_t(low_stop + high_stop);
+    k.shape_skirt_fraction = double(below + above) / double(width);
+    k.shape_measured = 1;
+}
+
 // emit_candidate (:1277-1334) for bins [first, last]: false when the band
-// carries no excess or does not clear the threshold.
+// carries no excess or does not clear the threshold. [own_low, own_high) is
+// the region between the neighbouring decision bands, where its skirts may be.
 bool emit(const rvx_event_config& c, const Sums& s, const double* avg, uint32_t bins,
-          size_t first, size_t last, std::vector<rvx_candidate>& out) {
+          size_t first, size_t last, size_t own_low, size_t own_high,
+          std::vector<rvx_candidate>& out) {
     if (last < first || last >= bins) return false;
     const double count = double(last - first + 1);
     const double excess = s.excess_over(first, last);
@@ -107,6 +160,7 @@ bool emit(const rvx_event_config& c, const Sums& s, const double* avg, uint32_t 
     k.peak_dbfs = to_db(peak);
     k.center_hz = c.source_center_hz + int64_t(std::llround(0.5 * (low_edge + high_edge)));
     k.bandwidth_hz = int64_t(std::llround(high_edge - low_edge));
+    measure_shape(s.clamped, first, last, std::min(own_low, first), std::max(own_high, last + 1), k);
     out.push_back(k);
     return true;
 }
@@ -116,8 +170,8 @@ bool emit(const rvx_event_config& c, const Sums& s, const double* avg, uint32_t 
 // it at interior runs of at least split_gap_bins bins that sit within
 // split_gap_db of the floor.
 void band_candidates(const rvx_event_config& c, const Sums& s, const double* avg,
-                     const double* flo, uint32_t bins, size_t begin, size_t end,
-                     std::vector<uint8_t>& gap, std::vector<rvx_candidate>& out) {
+                     const double* flo, uint32_t bins, size_t begin, size_t end, size_t own_low,
+                     size_t own_high, std::vector<uint8_t>& gap, std::vector<rvx_candidate>& out) {
     double total = 0.0;
     for (size_t i = begin; i < end; ++i) total += std::max(0.0, avg[i] - flo[i]);
     if (total <= 0.0) return;
@@ -164,10 +218,10 @@ void band_candidates(const rvx_event_config& c, const Sums& s, const double* avg
         }
         size_t j = i;
         while (j < n && !gap[j]) ++j;
-        any = emit(c, s, avg, bins, low + i, low + j - 1, out) || any;
+        any = emit(c, s, avg, bins, low + i, low + j - 1, own_low, own_high, out) || any;
         i = j;
     }
-    if (!any) (void)emit(c, s, avg, bins, low, high, out);
+    if (!any) (void)emit(c, s, avg, bins, low, high, own_low, own_high, out);
 }
 
 struct Residual {
@@ -359,6 +413,7 @@ struct rvx_tracker {
             tr.bandwidth_hz = std::max<int64_t>(1, k.bandwidth_hz);
             tr.snr_2500_db = k.snr_2500_db;
             tr.margin_confidence = k.margin_confidence;
+            take_shape(tr, k);
             tr.first_bin = k.first_bin;
             tr.last_bin = k.last_bin;
             tr.level_first_bin = k.first_bin;
@@ -427,6 +482,7 @@ struct rvx_tracker {
         tr.bandwidth_hz = std::max<int64_t>(1, int64_t(std::llround(width)));
         tr.snr_2500_db += a * (k.snr_2500_db - tr.snr_2500_db);
         tr.margin_confidence = margin_confidence(tr.snr_2500_db, cfg.detection_threshold_db);
+        take_shape(tr, k);  // taken, not blended
         tr.first_bin = k.first_bin;
         tr.last_bin = k.last_bin;
         // a level over other bins is another quantity: it starts again
@@ -451,6 +507,15 @@ struct rvx_tracker {
             tr.state = RVX_TRACK_LIVE;
         }
         update_channel(tr);
+    }
+
+    static void take_shape(rvx_track& tr, const rvx_candidate& k) {
+        tr.shape_peak_to_mean = k.shape_peak_to_mean;
+        tr.shape_concentration = k.shape_concentration;
+        tr.shape_lower_fraction = k.shape_lower_fraction;
+        tr.shape_skirt_fraction = k.shape_skirt_fraction;
+        tr.shape_skirt_bins_available = k.shape_skirt_bins_available;
+        tr.shape_measured = k.shape_measured;
     }
 
     // update_channel (detector.cpp:1978-2007): the nearest grid channel, moved
@@ -540,9 +605,13 @@ int32_t rvx_candidates(const rvx_event_config* config, const double* average, co
         s.build(average, floor, bins);
         std::vector<rvx_candidate> got;
         std::vector<uint8_t> gap;
-        for (uint32_t i = 0; i < n_bands; ++i)
+        for (uint32_t i = 0; i < n_bands; ++i) {
+            // the run between the neighbouring bands is this one's to look in
+            const size_t own_low = i == 0 ? 0 : size_t(bands[i - 1].start) + bands[i - 1].width;
+            const size_t own_high = i + 1 == n_bands ? size_t(bins) : size_t(bands[i + 1].start);
             band_candidates(*config, s, average, floor, bins, bands[i].start,
-                            size_t(bands[i].start) + bands[i].width, gap, got);
+                            size_t(bands[i].start) + bands[i].width, own_low, own_high, gap, got);
+        }
         *n_out = uint32_t(got.size());
         if (got.size() > max_out)
             return rvx::fail(RVX_EINVAL, "%s: %zu candidates, room for %u", who, got.size(),

@@ -117,3 +117,60 @@ def test_grid_channel_moves_only_past_the_hysteresis():
     t3, _ = _tracker()
     tr = _step(t3, [_candidate(100, 102, CENTER + 7 * width)], 0)[0]
     assert (tr.channel_valid, tr.channel, tr.channel_index) == (0, 0, 0)
+
+
+def _shapes(excess, bands):
+    """Candidates of a frame whose floor is 1 and whose average is 1 + excess."""
+    cfg = events.event_config(BW, ZERO, RATE, CENTER)
+    floor = np.ones(BINS)
+    return events.candidates(cfg, floor + excess, floor, [(s, w, 0.0) for s, w in bands])
+
+
+def test_band_shapes_by_their_definitions():
+    """measure_band (core/detect/shape.h:275-372): peak over mean, the best
+    three adjacent bins' share, the lower half's share of the halves, and the
+    skirt bins over a tenth of the mean within twice the width and the band's
+    own region, on bands whose answers are fixed by construction."""
+    x = np.zeros(BINS)
+    x[1000:1040] = 2.0                          # flat, 40 bins
+    x[2000:2003] = [1.0, 10.0, 1.0]             # a carrier over three bins
+    x[3000:3020] = np.arange(1.0, 21.0)         # a ramp ...
+    x[3020:3025] = 2.0                          # ... with a 5-bin skirt past its band
+    flat, carrier, ramp = _shapes(x, [(1000, 40), (1996, 10), (3000, 20)])
+    # flat: the occupied band is all 40 bins
+    assert (flat.first_bin, flat.last_bin, flat.shape_measured) == (1000, 1039, 1)
+    assert flat.shape_peak_to_mean == pytest.approx(1.0, abs=1e-12)
+    assert flat.shape_concentration == pytest.approx(6.0 / 80.0, abs=1e-12)
+    assert flat.shape_lower_fraction == pytest.approx(0.5, abs=1e-12)
+    assert flat.shape_skirt_fraction == 0.0
+    assert flat.shape_skirt_bins_available == 80 + 80   # twice the width each side, room enough
+    # carrier: three bins or fewer are wholly concentrated
+    assert (carrier.first_bin, carrier.last_bin) == (2000, 2002)
+    assert carrier.shape_peak_to_mean == pytest.approx(10.0 / 4.0, abs=1e-12)
+    assert carrier.shape_concentration == 1.0
+    assert carrier.shape_lower_fraction == pytest.approx(0.5, abs=1e-12)
+    # ramp: the occupied band drops its first bin (1 of 210 is under the
+    # half-percent tail); 19 bins of 2..20, mean 11, halves of 9
+    assert (ramp.first_bin, ramp.last_bin) == (3001, 3019)
+    assert ramp.shape_peak_to_mean == pytest.approx(20.0 / 11.0, abs=1e-12)
+    assert ramp.shape_concentration == pytest.approx((18 + 19 + 20) / 209.0, abs=1e-12)
+    assert ramp.shape_lower_fraction == pytest.approx(54.0 / (54.0 + 144.0), abs=1e-12)
+    # below: bin 3000 holds 1, under 1.1; above: the five skirt bins hold 2
+    assert ramp.shape_skirt_fraction == pytest.approx(5.0 / 19.0, abs=1e-12)
+
+
+def test_a_skirt_walk_stops_at_the_neighbouring_band():
+    x = np.zeros(BINS)
+    x[3400:3410] = 4.0
+    x[3410:3420] = 4.0 * 0.5     # a skirt ten bins long ...
+    x[3412:3416] = 8.0           # ... with another band starting two bins past the first
+    a, b = _shapes(x, [(3400, 10), (3412, 4)])[:2]
+    assert (a.first_bin, a.last_bin) == (3400, 3409)
+    # its own region ends where the next band starts: two bins of room above
+    assert a.shape_skirt_bins_available == 20 + 2
+    assert a.shape_skirt_fraction == pytest.approx(2.0 / 10.0, abs=1e-12)
+    # a track takes its candidate's shape as measured
+    t, _ = _tracker()
+    tr = _step(t, [a], 0)[0]
+    assert (tr.shape_peak_to_mean, tr.shape_skirt_fraction, tr.shape_measured) == \
+        (a.shape_peak_to_mean, a.shape_skirt_fraction, 1)
