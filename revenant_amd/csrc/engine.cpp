@@ -1931,10 +1931,21 @@ int32_t rvx_engine_push(rvx_engine* e, const void* codes, uint64_t samples, int3
         count = std::min<uint64_t>(count, e->max_blocks);
     }
     const bool added = !e->grids.empty();
-    if (added && !e->aux) {
-        RVX_TRY_HIP(hipStreamCreateWithFlags(&e->aux, hipStreamNonBlocking), who);
-        RVX_TRY_HIP(hipEventCreateWithFlags(&e->fork, hipEventDisableTiming), who);
-        RVX_TRY_HIP(hipEventCreateWithFlags(&e->join, hipEventDisableTiming), who);
+    if (added && !e->aux) {  // all three or none, so a failed push leaves nothing half made
+        hipStream_t aux = nullptr;
+        hipEvent_t fork = nullptr, join = nullptr;
+        hipError_t err = hipStreamCreateWithFlags(&aux, hipStreamNonBlocking);
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+        if (err == hipSuccess) err = hipEventCreateWithFlags(&join, hipEventDisableTiming);
+        if (err != hipSuccess) {
+            if (join) (void)hipEventDestroy(join);
+            if (fork) (void)hipEventDestroy(fork);
+            if (aux) (void)hipStreamDestroy(aux);
+            return rvx::fail(RVX_EDEVICE, "%s: second stream: %s", who, hipGetErrorString(err));
+        }
+        e->aux = aux;
+        e->fork = fork;
+        e->join = join;
     }
     if (count) {
         if (int32_t rc = main_push(e, count, added ? e->fork : nullptr, who)) return rc;
