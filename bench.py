@@ -375,6 +375,11 @@ def main() -> int:
         conditioning = conditioning_leg(dev, torch, packed, main_grid, partial, kernel_ms,
                                         args.steps, cpu=not args.no_cpu)
 
+    host = None
+    if not args.no_conditioning:
+        log("timing the host stages (noise profile, comb detection and bank)")
+        host = host_leg()
+
     workload = ("PFB + 256 VRX: 2^27 cs16 @ 122.88 MS/s (400 carriers + AWGN) -> 4096-ch PFB "
                 "(D=2048, 16 taps/branch, Kaiser beta 8) -> channel ring + |X|^2 EMA "
                 "alpha=1/16; 512-ch PFB (D=256) for the WFM receivers; 256 VRX "
@@ -436,6 +441,7 @@ def main() -> int:
         "learned_detector": learned,
         "plain_decision": decision,
         "conditioning": conditioning,
+        "host_stages": host,
         "cpu_baseline": cpu,
         "parity": parity,
     }
@@ -448,6 +454,62 @@ def main() -> int:
         torch.cuda.synchronize()
     dev.bind_stream()
     return 0
+
+
+def host_leg() -> dict:
+    """The stages that run on the host as in the reference, each timed on one
+    core: the adaptive noise profile over one receiver's 48 kHz audio (per
+    second of audio), comb detection on a 16,384-bin averaged spectrum, and
+    the comb bank's plan and observe per block for a 41-line comb."""
+    from revenant_amd.comb import CombBank, detect_combs
+    from revenant_amd.noise_profile import NoiseProfile, NoiseProfileSettings
+
+    out = {}
+    g = np.random.default_rng(0x484F5354)
+    rate, seconds = 48_000, 10
+    t = np.arange(rate * seconds) / rate
+    audio = (0.05 * np.sin(2 * np.pi * 120 * t) + 0.1 * np.sin(2 * np.pi * 1500 * t)
+             + g.normal(0.0, 0.01, t.size)).astype(np.float32)
+    prof = NoiseProfile()
+    cfg = NoiseProfileSettings(enabled=True, strength_db=12.0)
+    t0 = time.perf_counter()
+    for i in range(0, audio.size, 960):  # 20 ms blocks, quiet for the first second
+        prof.process(audio[i:i + 960], rate, i < rate, cfg)
+    ms = (time.perf_counter() - t0) * 1e3 / seconds
+    out["noise_profile"] = {"ms_per_audio_second": round(ms, 3),
+                            "realtime_factor": round(1e3 / ms, 1), "cores": 1,
+                            "what": "one receiver, 48 kHz mono, 1024-point STFT pair per 512 samples"}
+
+    crate, n = 1_152_000, 1 << 20
+    tt = np.arange(n) / crate
+    x = g.normal(0.0, 0.01, n) + 1j * g.normal(0.0, 0.01, n)
+    for k in range(-20, 21):
+        x = x + 0.004 * np.exp(2j * np.pi * ((1234.0 + 15_000.0 * k) * tt + g.uniform()))
+    frames = x[: (n // 16384) * 16384].reshape(-1, 16384) * np.hanning(16384)
+    power = np.fft.fftshift(np.mean(np.abs(np.fft.fft(frames, axis=1)) ** 2, axis=0))
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        found = detect_combs(power, crate)
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    out["comb_detect"] = {"ms_per_spectrum": round(ms, 3), "bins": 16384, "combs": len(found),
+                          "cores": 1}
+
+    bank = CombBank(crate)
+    bank.detected(found)
+    block, blocks, lines = 65536, 200, 0
+    t0 = time.perf_counter()
+    for b in range(blocks):
+        ln, nl, _ = bank.plan(b * block, block)
+        lines = nl
+        if nl:
+            proj = (g.normal(size=(nl, 16)) + 1j * g.normal(size=(nl, 16))).astype(np.complex64)
+            bank.observe(b * block, block, proj)
+    ms = (time.perf_counter() - t0) * 1e3 / blocks
+    out["comb_bank"] = {"ms_per_block": round(ms, 4), "lines": lines, "block_samples": block,
+                        "realtime_factor_1152ksps": round(block / crate * 1e3 / ms, 1),
+                        "cores": 1}
+    return out
 
 
 def conditioning_leg(dev, torch, packed, g: Grid, partial, kernel_ms, steps,
