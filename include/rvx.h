@@ -1316,9 +1316,11 @@ const double* rvx_detector_sigma(rvx_detector* detector);
  * split, the SNR test in 2500 Hz, centre and width in Hz from the bin edges),
  * then per decision reject_residual (:1336) and update_tracks (:1585). A
  * track's first_seen / last_detected / last_seen are the sample indices of
- * the decisions that found it first and last: the event's boundaries. Not
- * built: band shape, margin confidence, emitter memory, spur cancelling,
- * channel association, level statistics. */
+ * the decisions that found it first and last: the event's boundaries. Each
+ * decision ends with the spur rule (mark_spurs, :687-780). Emitter memory
+ * and the probes stay outside: their findings that the rule reads, a probe's
+ * family and an emitter's key-up count, are the caller's to record
+ * (rvx_tracker_record_probe, rvx_tracker_set_keyups). */
 typedef struct rvx_event_config {
     double bin_width_hz;             /* the frame's bin spacing */
     double bin_zero_hz;              /* bin 0's centre, relative to the source centre */
@@ -1395,7 +1397,27 @@ typedef struct rvx_track {
      * the last detection measured while the track holds) */
     double shape_peak_to_mean, shape_concentration, shape_lower_fraction, shape_skirt_fraction;
     uint32_t shape_skirt_bins_available, shape_measured;
+    /* What a probe named the track (RVX_FAMILY_*, with double_sideband for a
+     * carrier) and its emitter's key-ups, as the caller recorded them. */
+    uint32_t classification, double_sideband, keyups;
+    /* An unkeyed carrier the engine may cancel (detector.h:521-535): spur by
+     * the rule or held while being cancelled; lo_spur when it stands at the
+     * baseband offset of a spur seen under another tuning. */
+    uint32_t spur, lo_spur;
+    uint32_t reserved_spur;
 } rvx_track;
+
+#define RVX_FAMILY_UNKNOWN 0u      /* no answer yet */
+#define RVX_FAMILY_UNMODULATED 1u  /* a bare carrier */
+#define RVX_FAMILY_OTHER 2u        /* any other family a probe names */
+
+/* SpurBook's entry (detector.h:1262-1292): a carrier that passed the spur
+ * rule, by its baseband offset and the source centre it was seen under. */
+typedef struct rvx_spur_sighting {
+    int64_t baseband_hz, center_hz;
+    uint64_t last_seen;          /* sample index */
+} rvx_spur_sighting;
+#define RVX_SPUR_BOOK_CAPACITY 256u
 
 typedef struct rvx_tracker_stats {
     uint64_t decisions, candidates, candidates_residual, residual_parts_degenerate;
@@ -1423,6 +1445,25 @@ int32_t rvx_tracker_update(rvx_tracker* tracker, const double* average, const do
 int32_t rvx_tracker_tracks(const rvx_tracker* tracker, rvx_track* out, uint32_t max_out,
                            uint32_t* n);
 int32_t rvx_tracker_stats_get(const rvx_tracker* tracker, rvx_tracker_stats* out);
+/* Detector::record_probe (detector.cpp:465-510), the part the spur rule
+ * reads: a probe's family for the track `id`, one the caller lets drive
+ * detection. RVX_EINVAL when no track has that id. */
+int32_t rvx_tracker_record_probe(rvx_tracker* tracker, uint64_t id, uint32_t family,
+                                 uint32_t double_sideband);
+/* refine::EmitterMemory::keyups of the track's emitter (counted by the
+ * caller's emitter memory, detector.cpp:1910-1927). */
+int32_t rvx_tracker_set_keyups(rvx_tracker* tracker, uint64_t id, uint32_t keyups);
+/* Detector::withdrawn_spurs (detector.h:1400-1403): absolute centres (Hz) of
+ * the spurs the latest decision took back, for the engine to stop cancelling. */
+int32_t rvx_tracker_withdrawn_spurs(const rvx_tracker* tracker, int64_t* out, uint32_t max_out,
+                                    uint32_t* n);
+/* Detector::spur_book / adopt_spur_book (detector.h:1397-1398): the sightings,
+ * carried by the caller from the tracker before a retune to the one after.
+ * Sightings unseen for 300 s of source time are forgotten at each decision. */
+int32_t rvx_tracker_spur_book(const rvx_tracker* tracker, rvx_spur_sighting* out,
+                              uint32_t max_out, uint32_t* n);
+int32_t rvx_tracker_adopt_spur_book(rvx_tracker* tracker, const rvx_spur_sighting* book,
+                                    uint32_t n);
 
 /* ---- Learned proposals ------------------------------------------------------
  * A five-layer 3x3 convolutional network (1 -> 32 -> 32 -> 32 -> 32 -> 1,

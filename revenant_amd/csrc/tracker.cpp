@@ -33,6 +33,19 @@ constexpr double kDbPerNeper = 4.342944819032518;       // 10 / ln 10
 constexpr uint32_t kParts = 8;                          // detector.h:151
 constexpr double kPartFloor = 0.5;                      // detector.h:177
 constexpr uint32_t kUnassigned = 0xFFFFFFFFu;
+// The spur rule's numbers (detector.h:1302-1314, refine.h:179): a line is at
+// most 500 Hz or 3 bins wide, its frame-to-frame level sigma under 1.5 dB over
+// 50 frames, 500 Hz clear of DC, continuous for 5 s; sightings match within
+// 30 Hz of offset, tunings differ by 2 kHz, and both are kept 300 s.
+constexpr double kSpurMaxBandwidthHz = 500.0;
+constexpr double kSpurMaxBins = 3.0;
+constexpr double kSpurLevelSigmaDb = 1.5;
+constexpr uint32_t kSpurMinFrames = 50;
+constexpr double kSpurDcGuardHz = 500.0;
+constexpr double kContinuousSeconds = 5.0;
+constexpr double kLoSpurToleranceHz = 30.0;
+constexpr double kLoSpurMinRetuneHz = 2000.0;
+constexpr double kSpurMemorySeconds = 300.0;
 
 double from_db(double db) { return std::exp2(db * double(kDbToLog2)); }
 double to_db(double linear) { return 10.0 * std::log10(linear > kPowerFloor ? linear : kPowerFloor); }
@@ -248,6 +261,17 @@ struct rvx_tracker {
     std::vector<Residual> residuals;
     std::vector<rvx_track> tracks;
     rvx_tracker_stats stats{};
+    // Narrow carriers that passed the spur rule, by baseband offset and the
+    // tuning they were seen at (kept across a retune by the caller).
+    std::vector<rvx_spur_sighting> book;
+    // Spurs named for cancelling, by absolute centre, and when last named.
+    struct Named {
+        int64_t center_hz;
+        uint64_t last;
+    };
+    std::vector<Named> cancelling;
+    // Spurs the latest decision took back, absolute centres.
+    std::vector<int64_t> withdrawn;
 
     double level_db(uint32_t first, uint32_t last) const {
         const size_t lo = std::min<size_t>(first, bins - 1), hi = std::min<size_t>(last, bins - 1);
@@ -542,6 +566,129 @@ struct rvx_tracker {
         tr.channel = uint32_t(wrapped);
     }
 
+    // SpurBook::note (detector.cpp:648-663): one sighting per tuning and
+    // offset (refreshed in place); with the book full, the longest unseen goes.
+    void note_sighting(int64_t baseband, int64_t centre, uint64_t now) {
+        for (rvx_spur_sighting& e : book) {
+            if (e.center_hz == centre && std::abs(double(e.baseband_hz - baseband)) <= kLoSpurToleranceHz) {
+                e.baseband_hz = baseband;
+                e.last_seen = now;
+                return;
+            }
+        }
+        if (book.size() >= RVX_SPUR_BOOK_CAPACITY)
+            book.erase(std::min_element(book.begin(), book.end(),
+                                        [](const rvx_spur_sighting& a, const rvx_spur_sighting& b) {
+                                            return a.last_seen < b.last_seen;
+                                        }));
+        book.push_back(rvx_spur_sighting{baseband, centre, now});
+    }
+
+    // Whether a live track stands at the absolute frequency `at`.
+    bool live_at(int64_t at) const {
+        for (const rvx_track& o : tracks)
+            if (o.state == RVX_TRACK_LIVE &&
+                std::abs(double(o.center_hz - at)) <= kLoSpurToleranceHz + 0.5 * double(o.bandwidth_hz))
+                return true;
+        return false;
+    }
+
+    // SpurBook::stayed_at (:665-672): a sighting at this offset under a tuning
+    // at least 2 kHz away, whose old absolute frequency nothing occupies now.
+    bool stayed_at(int64_t baseband, int64_t centre) const {
+        for (const rvx_spur_sighting& e : book)
+            if (std::abs(double(e.center_hz - centre)) >= kLoSpurMinRetuneHz &&
+                std::abs(double(e.baseband_hz - baseband)) <= kLoSpurToleranceHz &&
+                !live_at(e.baseband_hz + e.center_hz))
+                return true;
+        return false;
+    }
+
+    double near_hz() const { return std::max(kLoSpurToleranceHz, cfg.bin_width_hz); }
+
+    bool is_cancelling(int64_t center) const {
+        for (const Named& n : cancelling)
+            if (std::abs(double(n.center_hz - center)) <= near_hz()) return true;
+        return false;
+    }
+
+    void withdraw(const rvx_track& tr) {
+        withdrawn.push_back(tr.center_hz);
+        const double near = near_hz();
+        cancelling.erase(std::remove_if(cancelling.begin(), cancelling.end(),
+                                        [&](const Named& n) {
+                                            return std::abs(double(n.center_hz - tr.center_hz)) <= near;
+                                        }),
+                         cancelling.end());
+    }
+
+    // mark_spurs (detector.cpp:687-780), at every decision. A spur is a live,
+    // narrow track off DC that a probe called a bare carrier (not double
+    // sideband; no answer yet leaves it open), continuous for 5 s with no held
+    // spell and at most one key-up on its emitter, its level steady. Or, once
+    // steady, a lo_spur: it sits at the baseband offset of a carrier that
+    // passed the rule under another tuning, whose old frequency stands empty.
+    // A named spur stays named while it is being cancelled (what is left of it
+    // proves nothing about keying or level); a probe naming another family,
+    // or the track widening, withdraws it. Only by-rule carriers are noted.
+    void mark_spurs(uint64_t now) {
+        const double rate = double(std::max<int64_t>(1, cfg.source_rate));
+        const double widest = std::max(kSpurMaxBandwidthHz, kSpurMaxBins * cfg.bin_width_hz);
+        withdrawn.clear();
+        for (rvx_track& tr : tracks) {
+            const bool was = tr.spur != 0;
+            const bool carrier = tr.classification == RVX_FAMILY_UNMODULATED && !tr.double_sideband;
+            const bool unnamed = tr.classification == RVX_FAMILY_UNKNOWN;
+            const int64_t baseband = tr.center_hz - cfg.source_center_hz;
+            const bool narrow = double(tr.bandwidth_hz) <= widest &&
+                                std::abs(double(baseband)) > kSpurDcGuardHz;
+            if (!(carrier || unnamed) || !narrow) {
+                tr.spur = tr.lo_spur = 0;
+                if (was && tr.state != RVX_TRACK_PENDING) withdraw(tr);
+                continue;
+            }
+            if (tr.state != RVX_TRACK_LIVE) {
+                tr.spur = 0;  // held or merged: quiet because cancelled, not withdrawn
+                continue;
+            }
+            const double age = double(now - tr.first_seen) / rate;
+            const bool unkeyed = tr.held_spells == 0 && tr.keyups <= 1 && age >= kContinuousSeconds;
+            const double sigma =
+                tr.level_frames < 2 ? 0.0 : std::sqrt(tr.level_m2 / double(tr.level_frames - 1));
+            const bool steady = tr.level_frames >= kSpurMinFrames && sigma <= kSpurLevelSigmaDb;
+            const bool by_rule = carrier && unkeyed && steady;
+            if (!tr.lo_spur && steady && stayed_at(baseband, cfg.source_center_hz)) tr.lo_spur = 1;
+            if (by_rule) note_sighting(baseband, cfg.source_center_hz, now);
+            const bool kept = was && is_cancelling(tr.center_hz);
+            tr.spur = (kept || by_rule || (tr.lo_spur && steady)) ? 1u : 0u;
+            if (was && !tr.spur) withdraw(tr);
+            if (tr.spur) {
+                auto named = std::find_if(cancelling.begin(), cancelling.end(), [&](const Named& n) {
+                    return std::abs(double(n.center_hz - tr.center_hz)) <= near_hz();
+                });
+                if (named == cancelling.end())
+                    cancelling.push_back(Named{tr.center_hz, now});
+                else
+                    named->last = now;
+            }
+        }
+        const auto forget = uint64_t(kSpurMemorySeconds * rate);
+        book.erase(std::remove_if(book.begin(), book.end(),
+                                  [&](const rvx_spur_sighting& e) {
+                                      return now > e.last_seen && now - e.last_seen > forget;
+                                  }),
+                   book.end());
+        cancelling.erase(std::remove_if(cancelling.begin(), cancelling.end(),
+                                        [&](const Named& n) { return now > n.last && now - n.last > forget; }),
+                         cancelling.end());
+    }
+
+    rvx_track* find(uint64_t id) {
+        for (rvx_track& tr : tracks)
+            if (tr.id == id) return &tr;
+        return nullptr;
+    }
+
     // measure_levels (detector.cpp:629-646): every live track's band level in
     // this frame, summed in linear power (each bin's dB times log2(10) / 10,
     // in float, through exp2), into its running mean and squared deviations.
@@ -663,6 +810,7 @@ int32_t rvx_tracker_update(rvx_tracker* t, const double* average, const double* 
         std::vector<rvx_candidate> cands(candidates, candidates + n);
         t->reject_residual(cands, elapsed_seconds);
         t->update(cands, now, elapsed_seconds);
+        t->mark_spurs(now);
         if (n_kept) *n_kept = uint32_t(cands.size());
         ++t->stats.decisions;
         t->stats.candidates += cands.size();
@@ -678,6 +826,64 @@ int32_t rvx_tracker_tracks(const rvx_tracker* t, rvx_track* out, uint32_t max_ou
     if (t->tracks.size() > max_out)
         return rvx::fail(RVX_EINVAL, "rvx_tracker_tracks: %u tracks, room for %u", *n, max_out);
     std::copy(t->tracks.begin(), t->tracks.end(), out);
+    return RVX_OK;
+}
+
+int32_t rvx_tracker_record_probe(rvx_tracker* t, uint64_t id, uint32_t family,
+                                 uint32_t double_sideband) {
+    static const char* who = "rvx_tracker_record_probe";
+    if (!t) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    if (family > RVX_FAMILY_OTHER) return rvx::fail(RVX_EINVAL, "%s: family %u", who, family);
+    rvx_track* tr = t->find(id);
+    if (!tr) return rvx::fail(RVX_EINVAL, "%s: no track %llu", who, (unsigned long long)id);
+    tr->classification = family;
+    tr->double_sideband = double_sideband ? 1u : 0u;
+    return RVX_OK;
+}
+
+int32_t rvx_tracker_set_keyups(rvx_tracker* t, uint64_t id, uint32_t keyups) {
+    if (!t) return rvx::fail(RVX_EINVAL, "rvx_tracker_set_keyups: null argument");
+    rvx_track* tr = t->find(id);
+    if (!tr)
+        return rvx::fail(RVX_EINVAL, "rvx_tracker_set_keyups: no track %llu", (unsigned long long)id);
+    tr->keyups = keyups;
+    return RVX_OK;
+}
+
+int32_t rvx_tracker_withdrawn_spurs(const rvx_tracker* t, int64_t* out, uint32_t max_out,
+                                    uint32_t* n) {
+    if (!t || !n || (max_out && !out))
+        return rvx::fail(RVX_EINVAL, "rvx_tracker_withdrawn_spurs: null argument");
+    *n = uint32_t(t->withdrawn.size());
+    if (t->withdrawn.size() > max_out)
+        return rvx::fail(RVX_EINVAL, "rvx_tracker_withdrawn_spurs: %u withdrawn, room for %u", *n,
+                         max_out);
+    std::copy(t->withdrawn.begin(), t->withdrawn.end(), out);
+    return RVX_OK;
+}
+
+int32_t rvx_tracker_spur_book(const rvx_tracker* t, rvx_spur_sighting* out, uint32_t max_out,
+                              uint32_t* n) {
+    if (!t || !n || (max_out && !out))
+        return rvx::fail(RVX_EINVAL, "rvx_tracker_spur_book: null argument");
+    *n = uint32_t(t->book.size());
+    if (t->book.size() > max_out)
+        return rvx::fail(RVX_EINVAL, "rvx_tracker_spur_book: %u sightings, room for %u", *n, max_out);
+    std::copy(t->book.begin(), t->book.end(), out);
+    return RVX_OK;
+}
+
+int32_t rvx_tracker_adopt_spur_book(rvx_tracker* t, const rvx_spur_sighting* book, uint32_t n) {
+    static const char* who = "rvx_tracker_adopt_spur_book";
+    if (!t || (n && !book)) return rvx::fail(RVX_EINVAL, "%s: null argument", who);
+    if (n > RVX_SPUR_BOOK_CAPACITY)
+        return rvx::fail(RVX_EINVAL, "%s: %u sightings, the book holds %u", who, n,
+                         RVX_SPUR_BOOK_CAPACITY);
+    try {
+        t->book.assign(book, book + n);
+    } catch (const std::bad_alloc&) {
+        return rvx::fail(RVX_ENOMEM, "%s: out of host memory", who);
+    }
     return RVX_OK;
 }
 

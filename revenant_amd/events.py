@@ -76,6 +76,42 @@ class Tracker:
                 _lib.check(rc)
             cap = n.value
 
+    def record_probe(self, track_id: int, family: int, double_sideband: bool = False) -> None:
+        """A probe's family for a track (Detector::record_probe): FAMILY_UNKNOWN,
+        FAMILY_UNMODULATED (with double_sideband for a DSB carrier) or FAMILY_OTHER."""
+        _lib.check(self.lib.rvx_tracker_record_probe(self.handle, int(track_id), int(family),
+                                                     int(bool(double_sideband))))
+
+    def set_keyups(self, track_id: int, keyups: int) -> None:
+        """The key-ups the caller's emitter memory has counted for the track's emitter."""
+        _lib.check(self.lib.rvx_tracker_set_keyups(self.handle, int(track_id), int(keyups)))
+
+    def withdrawn_spurs(self) -> list[int]:
+        """Absolute centres (Hz) of the spurs the latest decision took back."""
+        n = C.c_uint32()
+        cap = 64
+        while True:
+            out = (C.c_int64 * cap)()
+            rc = self.lib.rvx_tracker_withdrawn_spurs(self.handle, out, cap, C.byref(n))
+            if rc == 0:
+                return [int(out[i]) for i in range(n.value)]
+            if n.value <= cap:
+                _lib.check(rc)
+            cap = n.value
+
+    def spur_book(self) -> list[_lib.SpurSighting]:
+        """The sightings, to carry across a retune (Detector::spur_book)."""
+        n = C.c_uint32()
+        out = (_lib.SpurSighting * _lib.SPUR_BOOK_CAPACITY)()
+        _lib.check(self.lib.rvx_tracker_spur_book(self.handle, out, _lib.SPUR_BOOK_CAPACITY,
+                                                  C.byref(n)))
+        return [out[i] for i in range(n.value)]
+
+    def adopt_spur_book(self, book: list) -> None:
+        """Takes a book from the tracker before a retune (Detector::adopt_spur_book)."""
+        arr = (_lib.SpurSighting * max(1, len(book)))(*book)
+        _lib.check(self.lib.rvx_tracker_adopt_spur_book(self.handle, arr, len(book)))
+
     def stats(self) -> _lib.TrackerStats:
         s = _lib.TrackerStats()
         _lib.check(self.lib.rvx_tracker_stats_get(self.handle, C.byref(s)))
