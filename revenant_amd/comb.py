@@ -56,6 +56,29 @@ def spur(offset_hz: float) -> _lib.CombEstimate:
     return e
 
 
+def take_spurs(bank, tracks, withdrawn_hz, source_center_hz: int, listened=()) -> None:
+    """The detector's spur rule handed to the bank, once per decision
+    (core/rpc/server.cpp:5661-5681, core/engine/graph.cpp:1615-1655): spurs
+    inside a band an open receiver listens to go first, then the ones the
+    decision withdrew, then every track named a spur is offered as a one-line
+    estimate at its baseband offset unless a listened band holds it.
+    tracks: the tracker's tracks; withdrawn_hz: its withdrawn centres
+    (absolute Hz); listened: (low_hz, high_hz) baseband pairs, edges included."""
+    bank.drop_spurs_in(listened)
+    bank.withdraw_spurs([float(c - source_center_hz) for c in withdrawn_hz])
+    offered = []
+    for tr in tracks:
+        if not tr.spur:
+            continue
+        f = float(tr.center_hz - source_center_hz)
+        if any(lo <= f <= hi for lo, hi in listened):
+            continue
+        e = spur(f)
+        e.lines_found = 1
+        offered.append(e)
+    bank.spurs(offered)
+
+
 class CombBank:
     """Up to eight comb trackers (and sixteen spur trackers) behind one plan."""
 

@@ -8,7 +8,8 @@ retune is a spur at once, one at the same absolute frequency or with its old
 frequency still occupied is not; a probe renaming a spur withdraws it) and on
 the rule's edges: the 50-frame and 1.5 dB level test, the 500 Hz / 3-bin
 width, the DC guard, held spells, a spur held while cancelled, the book's
-256-entry eviction and 300 s memory. The probe's family and the emitter's
+256-entry eviction and 300 s memory; and the named spurs handed to the comb
+bank (graph.cpp:1615-1655). The probe's family and the emitter's
 key-ups are recorded by hand, as the reference's tests record a probe, so the
 rule is what is under test. Host only."""
 import numpy as np
@@ -249,3 +250,24 @@ def test_book_capacity_eviction_and_memory():
         t2.record_probe(12345, _lib.FAMILY_UNMODULATED)
     with pytest.raises(RuntimeError):
         t2.record_probe(12345, 7)
+
+
+def test_named_spurs_reach_the_comb_bank_and_withdrawals_take_them_out():
+    """The handoff (server.cpp:5661-5681, graph.cpp:1615-1655): a spur the
+    rule names becomes a one-line spur tracker in the bank at its baseband
+    offset; one inside a listened band does not; a withdrawal removes it."""
+    from revenant_amd.comb import CombBank, take_spurs
+    block = 65_536
+    s, tr = _probed_scene()
+    assert tr.spur
+    offset = tr.center_hz - CENTER
+    bank = CombBank(RATE)
+    take_spurs(bank, s.t.tracks(), s.t.withdrawn_spurs(), CENTER,
+               listened=[(offset - 1000.0, offset + 1000.0)])
+    assert bank.plan(0, block)[1] == 0           # a receiver listens there
+    take_spurs(bank, s.t.tracks(), s.t.withdrawn_spurs(), CENTER)
+    assert bank.plan(block, block)[1] == 1       # one spur tracker, one line
+    s.t.record_probe(tr.id, _lib.FAMILY_OTHER)
+    s.step([SPUR_BIN])
+    take_spurs(bank, s.t.tracks(), s.t.withdrawn_spurs(), CENTER)
+    assert bank.plan(2 * block, block)[1] == 0
