@@ -1317,10 +1317,10 @@ const double* rvx_detector_sigma(rvx_detector* detector);
  * then per decision reject_residual (:1336) and update_tracks (:1585). A
  * track's first_seen / last_detected / last_seen are the sample indices of
  * the decisions that found it first and last: the event's boundaries. Each
- * decision ends with the spur rule (mark_spurs, :687-780). Emitter memory
- * and the probes stay outside: their findings that the rule reads, a probe's
- * family and an emitter's key-up count, are the caller's to record
- * (rvx_tracker_record_probe, rvx_tracker_set_keyups). */
+ * decision ends with the emitters' key-up count and the spur rule
+ * (:1885-1932, mark_spurs :687-780). The probes stay outside: the finding the
+ * rule reads, a probe's family, is the caller's to record
+ * (rvx_tracker_record_probe), and remembers the track's emitter. */
 typedef struct rvx_event_config {
     double bin_width_hz;             /* the frame's bin spacing */
     double bin_zero_hz;              /* bin 0's centre, relative to the source centre */
@@ -1398,13 +1398,20 @@ typedef struct rvx_track {
     double shape_peak_to_mean, shape_concentration, shape_lower_fraction, shape_skirt_fraction;
     uint32_t shape_skirt_bins_available, shape_measured;
     /* What a probe named the track (RVX_FAMILY_*, with double_sideband for a
-     * carrier) and its emitter's key-ups, as the caller recorded them. */
+     * carrier), as the caller recorded it, and its emitter's key-ups. */
     uint32_t classification, double_sideband, keyups;
     /* An unkeyed carrier the engine may cancel (detector.h:521-535): spur by
      * the rule or held while being cancelled; lo_spur when it stands at the
      * baseband offset of a spur seen under another tuning. */
     uint32_t spur, lo_spur;
     uint32_t reserved_spur;
+    /* The emitter it belongs to (refine::EmitterMemory, refine.h:261-295):
+     * remembered at the track's first probe, joined by a track born within
+     * max(1 kHz, half the narrower width) of it; 0 for none. Coming live after
+     * more than a second of the emitter's silence is one of its key-ups (not
+     * while the carrier there is being cancelled); an emitter quiet for
+     * 1800 s is forgotten, and at most 512 are kept, the longest quiet going. */
+    uint64_t emitter;
 } rvx_track;
 
 #define RVX_FAMILY_UNKNOWN 0u      /* no answer yet */
@@ -1447,12 +1454,10 @@ int32_t rvx_tracker_tracks(const rvx_tracker* tracker, rvx_track* out, uint32_t 
 int32_t rvx_tracker_stats_get(const rvx_tracker* tracker, rvx_tracker_stats* out);
 /* Detector::record_probe (detector.cpp:465-510), the part the spur rule
  * reads: a probe's family for the track `id`, one the caller lets drive
- * detection. RVX_EINVAL when no track has that id. */
+ * detection; the track's first probe remembers its emitter (at the last
+ * decision's sample index). RVX_EINVAL when no track has that id. */
 int32_t rvx_tracker_record_probe(rvx_tracker* tracker, uint64_t id, uint32_t family,
                                  uint32_t double_sideband);
-/* refine::EmitterMemory::keyups of the track's emitter (counted by the
- * caller's emitter memory, detector.cpp:1910-1927). */
-int32_t rvx_tracker_set_keyups(rvx_tracker* tracker, uint64_t id, uint32_t keyups);
 /* Detector::withdrawn_spurs (detector.h:1400-1403): absolute centres (Hz) of
  * the spurs the latest decision took back, for the engine to stop cancelling. */
 int32_t rvx_tracker_withdrawn_spurs(const rvx_tracker* tracker, int64_t* out, uint32_t max_out,

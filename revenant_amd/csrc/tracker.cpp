@@ -46,6 +46,12 @@ constexpr double kContinuousSeconds = 5.0;
 constexpr double kLoSpurToleranceHz = 30.0;
 constexpr double kLoSpurMinRetuneHz = 2000.0;
 constexpr double kSpurMemorySeconds = 300.0;
+// The emitter memory's (refine.h:184-189, 277): coming live after a second of
+// an emitter's silence is a key-up; an emitter is kept 1800 s, at most 512.
+constexpr double kKeyupGapSeconds = 1.0;
+constexpr double kEmitterMemorySeconds = 1800.0;
+constexpr size_t kEmitterCapacity = 512;
+constexpr int64_t kEmitterMatchMinHz = 1000;
 
 double from_db(double db) { return std::exp2(db * double(kDbToLog2)); }
 double to_db(double linear) { return 10.0 * std::log10(linear > kPowerFloor ? linear : kPowerFloor); }
@@ -272,6 +278,76 @@ struct rvx_tracker {
     std::vector<Named> cancelling;
     // Spurs the latest decision took back, absolute centres.
     std::vector<int64_t> withdrawn;
+    // Emitters (refine::EmitterMemory): remembered at a track's first probe,
+    // joined by tracks born near them, their key-ups counted per decision.
+    struct Emitter {
+        uint64_t id;
+        int64_t center_hz, bandwidth_hz;
+        uint64_t last_live;
+        uint32_t keyups;
+    };
+    std::vector<Emitter> emitters;  // in the order remembered
+    uint64_t next_emitter = 1;
+    uint64_t last_decision = 0;
+
+    // EmitterBook::match (refine.cpp:329-342): the nearest emitter within half
+    // the narrower width, never under 1 kHz (ties: the earliest remembered).
+    Emitter* match_emitter(int64_t center, int64_t bandwidth) {
+        Emitter* best = nullptr;
+        int64_t best_distance = INT64_MAX;
+        for (Emitter& e : emitters) {
+            const int64_t tolerance =
+                std::max<int64_t>(kEmitterMatchMinHz, std::min(bandwidth, e.bandwidth_hz) / 2);
+            const int64_t distance = std::abs(center - e.center_hz);
+            if (distance <= tolerance && distance < best_distance) {
+                best = &e;
+                best_distance = distance;
+            }
+        }
+        return best;
+    }
+
+    Emitter* find_emitter(uint64_t id) {
+        if (id == 0) return nullptr;
+        for (Emitter& e : emitters)
+            if (e.id == id) return &e;
+        return nullptr;
+    }
+
+    // EmitterBook::remember (:349-367): full, the longest quiet goes (ties:
+    // the earliest remembered).
+    Emitter& remember_emitter(int64_t center, int64_t bandwidth, uint64_t now) {
+        if (emitters.size() >= kEmitterCapacity)
+            emitters.erase(std::min_element(emitters.begin(), emitters.end(),
+                                            [](const Emitter& a, const Emitter& b) {
+                                                return a.last_live < b.last_live;
+                                            }));
+        emitters.push_back(Emitter{next_emitter++, center, bandwidth, now, 0});
+        return emitters.back();
+    }
+
+    // detector.cpp:1910-1930: a live track keeps its emitter remembered, and
+    // coming live after the emitter was quiet for over a second is a key-up
+    // (unless the carrier there is being cancelled, which silences it); then
+    // emitters quiet for 1800 s are forgotten.
+    void count_keyups(uint64_t now) {
+        const double rate = double(std::max<int64_t>(1, cfg.source_rate));
+        const auto gap = uint64_t(kKeyupGapSeconds * rate);
+        for (const rvx_track& tr : tracks) {
+            if (tr.state != RVX_TRACK_LIVE || tr.emitter == 0) continue;
+            if (Emitter* e = find_emitter(tr.emitter)) {
+                if (now > e->last_live && now - e->last_live > gap && !is_cancelling(tr.center_hz))
+                    ++e->keyups;
+                e->last_live = now;
+            }
+        }
+        const auto forget = uint64_t(kEmitterMemorySeconds * rate);
+        emitters.erase(std::remove_if(emitters.begin(), emitters.end(),
+                                      [&](const Emitter& e) {
+                                          return now > e.last_live && now - e.last_live > forget;
+                                      }),
+                       emitters.end());
+    }
 
     double level_db(uint32_t first, uint32_t last) const {
         const size_t lo = std::min<size_t>(first, bins - 1), hi = std::min<size_t>(last, bins - 1);
@@ -446,6 +522,9 @@ struct rvx_tracker {
             tr.confidence = cfg.confidence_rise;
             tr.first_seen = tr.last_seen = tr.last_detected = now;
             tr.hits = 1;
+            // born where an emitter was: it joins it (key-ups are counted
+            // once it is live, so a pending track that dies is not one)
+            if (const Emitter* e = match_emitter(tr.center_hz, tr.bandwidth_hz)) tr.emitter = e->id;
             if (tr.hits >= cfg.birth_hits) {
                 tr.state = RVX_TRACK_LIVE;
                 ++stats.tracks_born;
@@ -635,7 +714,16 @@ struct rvx_tracker {
         const double rate = double(std::max<int64_t>(1, cfg.source_rate));
         const double widest = std::max(kSpurMaxBandwidthHz, kSpurMaxBins * cfg.bin_width_hz);
         withdrawn.clear();
-        for (rvx_track& tr : tracks) {
+        std::vector<size_t> order(tracks.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+        std::sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+            const rvx_track &x = tracks[a], &y = tracks[b];
+            return x.center_hz != y.center_hz ? x.center_hz < y.center_hz : x.id < y.id;
+        });
+        for (const size_t i : order) {
+            rvx_track& tr = tracks[i];
+            const Emitter* emitter = find_emitter(tr.emitter);
+            tr.keyups = emitter ? emitter->keyups : 0;
             const bool was = tr.spur != 0;
             const bool carrier = tr.classification == RVX_FAMILY_UNMODULATED && !tr.double_sideband;
             const bool unnamed = tr.classification == RVX_FAMILY_UNKNOWN;
@@ -810,7 +898,9 @@ int32_t rvx_tracker_update(rvx_tracker* t, const double* average, const double* 
         std::vector<rvx_candidate> cands(candidates, candidates + n);
         t->reject_residual(cands, elapsed_seconds);
         t->update(cands, now, elapsed_seconds);
+        t->count_keyups(now);
         t->mark_spurs(now);
+        t->last_decision = now;
         if (n_kept) *n_kept = uint32_t(cands.size());
         ++t->stats.decisions;
         t->stats.candidates += cands.size();
@@ -836,17 +926,15 @@ int32_t rvx_tracker_record_probe(rvx_tracker* t, uint64_t id, uint32_t family,
     if (family > RVX_FAMILY_OTHER) return rvx::fail(RVX_EINVAL, "%s: family %u", who, family);
     rvx_track* tr = t->find(id);
     if (!tr) return rvx::fail(RVX_EINVAL, "%s: no track %llu", who, (unsigned long long)id);
+    try {
+        // the track's first probe remembers its emitter (refine_track, :405-411)
+        if (!t->find_emitter(tr->emitter))
+            tr->emitter = t->remember_emitter(tr->center_hz, tr->bandwidth_hz, t->last_decision).id;
+    } catch (const std::bad_alloc&) {
+        return rvx::fail(RVX_ENOMEM, "%s: out of host memory", who);
+    }
     tr->classification = family;
     tr->double_sideband = double_sideband ? 1u : 0u;
-    return RVX_OK;
-}
-
-int32_t rvx_tracker_set_keyups(rvx_tracker* t, uint64_t id, uint32_t keyups) {
-    if (!t) return rvx::fail(RVX_EINVAL, "rvx_tracker_set_keyups: null argument");
-    rvx_track* tr = t->find(id);
-    if (!tr)
-        return rvx::fail(RVX_EINVAL, "rvx_tracker_set_keyups: no track %llu", (unsigned long long)id);
-    tr->keyups = keyups;
     return RVX_OK;
 }
 

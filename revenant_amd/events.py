@@ -82,10 +82,6 @@ class Tracker:
         _lib.check(self.lib.rvx_tracker_record_probe(self.handle, int(track_id), int(family),
                                                      int(bool(double_sideband))))
 
-    def set_keyups(self, track_id: int, keyups: int) -> None:
-        """The key-ups the caller's emitter memory has counted for the track's emitter."""
-        _lib.check(self.lib.rvx_tracker_set_keyups(self.handle, int(track_id), int(keyups)))
-
     def withdrawn_spurs(self) -> list[int]:
         """Absolute centres (Hz) of the spurs the latest decision took back."""
         n = C.c_uint32()

@@ -1,17 +1,18 @@
 """The spur rule at the end of every decision (Detector::mark_spurs,
-core/detect/detector.cpp:687-780) and the spur book carried across a retune
-(SpurBook, :648-685), on the cases of the reference's
-tests/detect/test_detector.cpp:2679-2920 (a constant carrier probed as one is
-a spur after five continuous seconds; keyed, fading, double-sideband and
-unprobed carriers are not; a carrier at the same baseband offset after a
-retune is a spur at once, one at the same absolute frequency or with its old
-frequency still occupied is not; a probe renaming a spur withdraws it) and on
-the rule's edges: the 50-frame and 1.5 dB level test, the 500 Hz / 3-bin
-width, the DC guard, held spells, a spur held while cancelled, the book's
-256-entry eviction and 300 s memory; and the named spurs handed to the comb
-bank (graph.cpp:1615-1655). The probe's family and the emitter's
-key-ups are recorded by hand, as the reference's tests record a probe, so the
-rule is what is under test. Host only."""
+core/detect/detector.cpp:687-780), the emitter memory whose key-ups it reads
+(refine.h:261-295, detector.cpp:1885-1930) and the spur book carried across a
+retune (SpurBook, :648-685), on the cases of the reference's
+tests/detect/test_detector.cpp:2679-2920: a constant carrier probed as one is
+a spur after five continuous seconds; keyed (two key-ups on its emitter),
+fading, double-sideband and unprobed carriers are not; a carrier at the same
+baseband offset after a retune is a spur at once, one at the same absolute
+frequency or with its old frequency still occupied is not; a probe renaming a
+spur withdraws it. Also the rule's edges (the 50-frame and 1.5 dB level test,
+the 500 Hz / 3-bin width, the DC guard, held spells, a spur kept while
+cancelled, the book's 256-entry eviction and 300 s memory, the emitter's 1 kHz
+match) and the named spurs handed to the comb bank (graph.cpp:1615-1655). The
+probe's family is recorded by hand, as the reference's tests record a probe,
+so the rule is what is under test. Host only."""
 import numpy as np
 import pytest
 
@@ -106,17 +107,51 @@ def test_a_constant_carrier_is_a_spur_after_five_continuous_seconds():
     assert book[0].baseband_hz == _at(SPUR_BIN) - CENTER and book[0].center_hz == CENTER
 
 
-@pytest.mark.parametrize("case", ["keyups", "fading", "dsb", "other", "unprobed", "held"])
-def test_what_is_not_a_spur(case):
-    if case == "keyups":
+@pytest.mark.parametrize("keyups", [1, 2])
+def test_keyups_are_counted_on_the_emitter_and_two_rule_a_carrier_out(keyups):
+    """The emitter memory (refine.h:261-295, detector.cpp:1885-1930): the
+    first probe remembers the track's emitter; a carrier that goes away until
+    its track is dropped and comes back is a new track on the same emitter,
+    and its coming live after more than a second of silence is a key-up. At
+    most one key-up still lets a probed carrier be a spur; two do not."""
+    s = Scene()
+    s.run(0.5, [SPUR_BIN])
+    tr = s.track_at(SPUR_BIN)
+    assert tr.emitter == 0                      # no probe yet, no emitter
+    s.t.record_probe(tr.id, _lib.FAMILY_UNMODULATED)
+    emitter = s.track_at(SPUR_BIN).emitter
+    assert emitter != 0
+    for n in range(1, keyups + 1):
+        s.run(6.0, [])                          # off: held, then dropped
+        assert s.track_at(SPUR_BIN) is None
+        s.run(0.5, [SPUR_BIN])                  # back: a new track joins the emitter
+        tr = s.track_at(SPUR_BIN)
+        assert tr.state == 1 and tr.emitter == emitter and tr.keyups == n
+        assert tr.held_spells == 0
+    s.t.record_probe(tr.id, _lib.FAMILY_UNMODULATED)
+    s.run(7.0, [SPUR_BIN])
+    tr = s.track_at(SPUR_BIN)
+    assert tr.keyups == keyups
+    assert tr.spur == (keyups <= 1)
+
+
+def test_a_track_joins_an_emitter_within_its_tolerance():
+    """EmitterBook::match: within max(1 kHz, half the narrower width)."""
+    for offset_bins, joins in ((1, True), (2, False)):
         s = Scene()
         s.run(0.5, [SPUR_BIN])
-        tr = s.track_at(SPUR_BIN)
-        s.t.record_probe(tr.id, _lib.FAMILY_UNMODULATED)
-        s.t.set_keyups(tr.id, 2)               # keyed CW: its emitter keyed up twice
-        tr = s.run(7.0, [SPUR_BIN])
-        tr = s.track_at(SPUR_BIN)
-    elif case == "fading":                     # 6 dB over a 4 s period: sigma > 1.5 dB
+        s.t.record_probe(s.track_at(SPUR_BIN).id, _lib.FAMILY_UNMODULATED)
+        emitter = s.track_at(SPUR_BIN).emitter
+        s.run(6.0, [])
+        s.run(0.5, [SPUR_BIN + offset_bins])    # 1 kHz wide, 1 or 2 kHz away
+        tr = s.track_at(SPUR_BIN + offset_bins)
+        assert (tr.emitter == emitter) == joins
+        assert tr.keyups == (1 if joins else 0)
+
+
+@pytest.mark.parametrize("case", ["fading", "dsb", "other", "unprobed", "held"])
+def test_what_is_not_a_spur(case):
+    if case == "fading":                     # 6 dB over a 4 s period: sigma > 1.5 dB
         _, tr = _probed_scene(level_db=lambda k: -70.0 + 6.0 * np.sin(2 * np.pi * k / 40.0))
     elif case == "dsb":                        # an AM carrier, as steady as a spur
         _, tr = _probed_scene(dsb=True)
