@@ -1317,8 +1317,9 @@ const double* rvx_detector_sigma(rvx_detector* detector);
  * then per decision reject_residual (:1336) and update_tracks (:1585). A
  * track's first_seen / last_detected / last_seen are the sample indices of
  * the decisions that found it first and last: the event's boundaries. Each
- * decision ends with the emitters' key-up count and the spur rule
- * (:1885-1932, mark_spurs :687-780). The probes stay outside: the finding the
+ * decision ends with the emitters' key-up count, the spur rule and the
+ * hybrid sideband mark (:1885-1933, mark_spurs :687-780, mark_hd_hybrids
+ * :1943-1976). The probes stay outside: the finding the
  * rule reads, a probe's family, is the caller's to record
  * (rvx_tracker_record_probe), and remembers the track's emitter. */
 typedef struct rvx_event_config {
@@ -1412,7 +1413,26 @@ typedef struct rvx_track {
      * while the carrier there is being cancelled); an emitter quiet for
      * 1800 s is forgotten, and at most 512 are kept, the longest quiet going. */
     uint64_t emitter;
+    /* Hybrid digital sidebands beside wideband FM (detector.h:537-560,
+     * mark_hd_hybrids): measured at each decision that detects a live track
+     * at least 50 kHz wide; hd_hybrid after 2 passing readings in a row,
+     * cleared after 3 failing; the sidebands' level against the analogue
+     * signal (dBc), the first passing reading taken whole, then smoothed 0.3. */
+    double hd_lower_dbc, hd_upper_dbc;
+    uint32_t hd_passes, hd_fails, hd_hybrid, reserved_hd;
 } rvx_track;
+
+/* HdSidebandReading (core/detect/hd_hybrid.h): with `measured`, the levels of
+ * the digital sidebands (129-198 kHz from the analogue signal's centroid, found
+ * within 100 kHz of the given centre) against the analogue excess (dBc), their
+ * floor and the 105-120 kHz gap (dB), and each side's inner-over-outer tilt;
+ * `hybrid` when each side is 3 dB over its floor and 1.5 dB over its gap,
+ * tilted by at most 3 dB, and the sides within 6 dB of each other. */
+typedef struct rvx_hd_reading {
+    uint32_t measured, hybrid;
+    double centre_hz, lower_dbc, upper_dbc, lower_over_floor_db, upper_over_floor_db;
+    double lower_over_gap_db, upper_over_gap_db, lower_tilt_db, upper_tilt_db;
+} rvx_hd_reading;
 
 #define RVX_FAMILY_UNKNOWN 0u      /* no answer yet */
 #define RVX_FAMILY_UNMODULATED 1u  /* a bare carrier */
@@ -1452,6 +1472,12 @@ int32_t rvx_tracker_update(rvx_tracker* tracker, const double* average, const do
 int32_t rvx_tracker_tracks(const rvx_tracker* tracker, rvx_track* out, uint32_t max_out,
                            uint32_t* n);
 int32_t rvx_tracker_stats_get(const rvx_tracker* tracker, rvx_tracker_stats* out);
+/* measure_hd_sidebands (hd_hybrid.cpp): power and floor are a decision's
+ * average and floor (linear, `bins` doubles), bin_zero_hz bin 0's centre and
+ * centre_hz the track's, both relative to the source centre. */
+int32_t rvx_measure_hd_sidebands(const double* power, const double* floor, uint32_t bins,
+                                 double bin_zero_hz, double bin_width_hz, double centre_hz,
+                                 rvx_hd_reading* out);
 /* Detector::record_probe (detector.cpp:465-510), the part the spur rule
  * reads: a probe's family for the track `id`, one the caller lets drive
  * detection; the track's first probe remembers its emitter (at the last

@@ -361,7 +361,16 @@ class Track(C.Structure):
                 ("shape_lower_fraction", f64), ("shape_skirt_fraction", f64),
                 ("shape_skirt_bins_available", u32), ("shape_measured", u32),
                 ("classification", u32), ("double_sideband", u32), ("keyups", u32),
-                ("spur", u32), ("lo_spur", u32), ("reserved_spur", u32), ("emitter", u64)]
+                ("spur", u32), ("lo_spur", u32), ("reserved_spur", u32), ("emitter", u64),
+                ("hd_lower_dbc", f64), ("hd_upper_dbc", f64), ("hd_passes", u32),
+                ("hd_fails", u32), ("hd_hybrid", u32), ("reserved_hd", u32)]
+
+
+class HdReading(C.Structure):
+    _fields_ = [("measured", u32), ("hybrid", u32), ("centre_hz", f64), ("lower_dbc", f64),
+                ("upper_dbc", f64), ("lower_over_floor_db", f64), ("upper_over_floor_db", f64),
+                ("lower_over_gap_db", f64), ("upper_over_gap_db", f64), ("lower_tilt_db", f64),
+                ("upper_tilt_db", f64)]
 
 
 FAMILY_UNKNOWN, FAMILY_UNMODULATED, FAMILY_OTHER = 0, 1, 2
@@ -588,6 +597,7 @@ _SIGNATURES = {
     "rvx_tracker_tracks": (i32, [vp, C.POINTER(Track), u32, C.POINTER(u32)]),
     "rvx_tracker_stats_get": (i32, [vp, C.POINTER(TrackerStats)]),
     "rvx_tracker_record_probe": (i32, [vp, u64, u32, u32]),
+    "rvx_measure_hd_sidebands": (i32, [vp, vp, u32, f64, f64, f64, C.POINTER(HdReading)]),
     "rvx_tracker_withdrawn_spurs": (i32, [vp, C.POINTER(i64), u32, C.POINTER(u32)]),
     "rvx_tracker_spur_book": (i32, [vp, C.POINTER(SpurSighting), u32, C.POINTER(u32)]),
     "rvx_tracker_adopt_spur_book": (i32, [vp, C.POINTER(SpurSighting), u32]),

@@ -52,6 +52,104 @@ constexpr double kKeyupGapSeconds = 1.0;
 constexpr double kEmitterMemorySeconds = 1800.0;
 constexpr size_t kEmitterCapacity = 512;
 constexpr int64_t kEmitterMatchMinHz = 1000;
+// Hybrid digital sidebands beside wideband FM (hd_hybrid.h): the analogue
+// signal within 100 kHz of its centroid, the digital carriers 129-198 kHz out
+// on each side, the guard gap 105-120 kHz out; each side 3 dB over the floor,
+// 1.5 dB over the gap, tilted by at most 3 dB, within 6 dB of the other;
+// tracks of at least 50 kHz, confirmed after 2 decisions, released after 3.
+constexpr double kHdInnerHz = 129000.0, kHdOuterHz = 198000.0, kHdAnalogueHalfHz = 100000.0;
+constexpr double kHdGapInnerHz = 105000.0, kHdGapOuterHz = 120000.0;
+constexpr double kHdOverFloorDb = 3.0, kHdOverGapDb = 1.5, kHdMaxTiltDb = 3.0,
+                 kHdMaxAsymmetryDb = 6.0;
+constexpr int64_t kHdMinTrackHz = 50000;
+constexpr uint32_t kHdConfirm = 2, kHdRelease = 3;
+constexpr double kHdLevelSmoothing = 0.3;  // detector.cpp:1947
+
+// Power and floor summed over the bins whose centres lie in [low, high].
+struct HdSums {
+    double power = 0.0, floor = 0.0;
+    size_t bins = 0;
+    double excess() const { return std::max(0.0, power - floor); }
+    double density() const { return bins ? power / double(bins) : 0.0; }
+};
+
+struct HdSpectrum {
+    const double *power, *floor;
+    size_t n;
+    double zero, width;
+
+    HdSums sum(double low, double high) const {
+        HdSums out;
+        const double first = std::ceil((low - zero) / width), last = std::floor((high - zero) / width);
+        if (!(first >= 0.0) || !(last < double(n)) || last < first) return out;
+        for (size_t i = size_t(first); i <= size_t(last); ++i) {
+            out.power += power[i];
+            out.floor += floor[i];
+            ++out.bins;
+        }
+        return out;
+    }
+    // Excess-weighted mean frequency within half_hz of centre.
+    double centroid(double centre, double half_hz) const {
+        const double first = std::max(0.0, std::ceil((centre - half_hz - zero) / width));
+        const double last = std::min(double(n) - 1.0, std::floor((centre + half_hz - zero) / width));
+        double weight = 0.0, moment = 0.0;
+        for (double b = first; b <= last; b += 1.0) {
+            const size_t i = size_t(b);
+            const double e = std::max(0.0, power[i] - floor[i]);
+            weight += e;
+            moment += e * (zero + b * width);
+        }
+        return weight > 0.0 ? moment / weight : centre;
+    }
+};
+
+double hd_ratio_db(double a, double b) {
+    if (!(a > 0.0) || !(b > 0.0)) return a > 0.0 ? 99.0 : -99.0;
+    return 10.0 * std::log10(a / b);
+}
+
+// measure_hd_sidebands (hd_hybrid.cpp): the analogue centre by two centroid
+// passes (clamped within 100 kHz), then the sidebands' level against the
+// analogue excess, the floor and the gap, and each side's inner-over-outer tilt.
+rvx_hd_reading measure_hd(const double* power, const double* floor, size_t n, double zero,
+                          double width, double centre_hz) {
+    rvx_hd_reading out{};
+    if (n == 0 || !(width > 0.0) || !std::isfinite(centre_hz) || !std::isfinite(zero)) return out;
+    const HdSpectrum sp{power, floor, n, zero, width};
+    double centre = centre_hz;
+    for (int pass = 0; pass < 2; ++pass) centre = sp.centroid(centre, kHdAnalogueHalfHz);
+    centre = std::clamp(centre, centre_hz - kHdAnalogueHalfHz, centre_hz + kHdAnalogueHalfHz);
+    out.centre_hz = centre;
+    const HdSums analogue = sp.sum(centre - kHdAnalogueHalfHz, centre + kHdAnalogueHalfHz);
+    const HdSums lower = sp.sum(centre - kHdOuterHz, centre - kHdInnerHz);
+    const HdSums upper = sp.sum(centre + kHdInnerHz, centre + kHdOuterHz);
+    const HdSums lower_gap = sp.sum(centre - kHdGapOuterHz, centre - kHdGapInnerHz);
+    const HdSums upper_gap = sp.sum(centre + kHdGapInnerHz, centre + kHdGapOuterHz);
+    const double middle = 0.5 * (kHdInnerHz + kHdOuterHz);
+    const HdSums lower_inner = sp.sum(centre - middle, centre - kHdInnerHz);
+    const HdSums lower_outer = sp.sum(centre - kHdOuterHz, centre - middle);
+    const HdSums upper_inner = sp.sum(centre + kHdInnerHz, centre + middle);
+    const HdSums upper_outer = sp.sum(centre + middle, centre + kHdOuterHz);
+    if (!analogue.bins || !lower.bins || !upper.bins || !lower_gap.bins || !upper_gap.bins ||
+        !(analogue.excess() > 0.0))
+        return out;
+    out.measured = 1;
+    out.lower_dbc = hd_ratio_db(lower.excess(), analogue.excess());
+    out.upper_dbc = hd_ratio_db(upper.excess(), analogue.excess());
+    out.lower_over_floor_db = hd_ratio_db(lower.power, lower.floor);
+    out.upper_over_floor_db = hd_ratio_db(upper.power, upper.floor);
+    out.lower_over_gap_db = hd_ratio_db(lower.density(), lower_gap.density());
+    out.upper_over_gap_db = hd_ratio_db(upper.density(), upper_gap.density());
+    out.lower_tilt_db = hd_ratio_db(lower_inner.density(), lower_outer.density());
+    out.upper_tilt_db = hd_ratio_db(upper_inner.density(), upper_outer.density());
+    out.hybrid = out.lower_over_floor_db >= kHdOverFloorDb && out.upper_over_floor_db >= kHdOverFloorDb &&
+                 out.lower_over_gap_db >= kHdOverGapDb && out.upper_over_gap_db >= kHdOverGapDb &&
+                 std::abs(out.lower_tilt_db) <= kHdMaxTiltDb &&
+                 std::abs(out.upper_tilt_db) <= kHdMaxTiltDb &&
+                 std::abs(out.lower_dbc - out.upper_dbc) <= kHdMaxAsymmetryDb;
+    return out;
+}
 
 double from_db(double db) { return std::exp2(db * double(kDbToLog2)); }
 double to_db(double linear) { return 10.0 * std::log10(linear > kPowerFloor ? linear : kPowerFloor); }
@@ -771,6 +869,30 @@ struct rvx_tracker {
                          cancelling.end());
     }
 
+    // mark_hd_hybrids (detector.cpp:1943-1976): every live track at least
+    // 50 kHz wide detected at this decision is measured; passing readings
+    // smooth its two levels (the first taken whole) and confirm it after two
+    // in a row, failing ones release it after three.
+    void mark_hd_hybrids(const double* average, const double* floor, uint64_t now) {
+        for (rvx_track& tr : tracks) {
+            if (tr.state != RVX_TRACK_LIVE || tr.last_detected != now || tr.bandwidth_hz < kHdMinTrackHz)
+                continue;
+            const rvx_hd_reading r = measure_hd(average, floor, bins, cfg.bin_zero_hz, cfg.bin_width_hz,
+                                                double(tr.center_hz - cfg.source_center_hz));
+            if (!r.measured) continue;
+            if (r.hybrid) {
+                tr.hd_fails = 0;
+                const double w = tr.hd_passes == 0 && !tr.hd_hybrid ? 1.0 : kHdLevelSmoothing;
+                tr.hd_lower_dbc += w * (r.lower_dbc - tr.hd_lower_dbc);
+                tr.hd_upper_dbc += w * (r.upper_dbc - tr.hd_upper_dbc);
+                if (++tr.hd_passes >= kHdConfirm) tr.hd_hybrid = 1;
+            } else {
+                tr.hd_passes = 0;
+                if (++tr.hd_fails >= kHdRelease) tr.hd_hybrid = 0;
+            }
+        }
+    }
+
     rvx_track* find(uint64_t id) {
         for (rvx_track& tr : tracks)
             if (tr.id == id) return &tr;
@@ -900,6 +1022,7 @@ int32_t rvx_tracker_update(rvx_tracker* t, const double* average, const double* 
         t->update(cands, now, elapsed_seconds);
         t->count_keyups(now);
         t->mark_spurs(now);
+        t->mark_hd_hybrids(average, floor, now);
         t->last_decision = now;
         if (n_kept) *n_kept = uint32_t(cands.size());
         ++t->stats.decisions;
@@ -972,6 +1095,15 @@ int32_t rvx_tracker_adopt_spur_book(rvx_tracker* t, const rvx_spur_sighting* boo
     } catch (const std::bad_alloc&) {
         return rvx::fail(RVX_ENOMEM, "%s: out of host memory", who);
     }
+    return RVX_OK;
+}
+
+int32_t rvx_measure_hd_sidebands(const double* power, const double* floor, uint32_t bins,
+                                 double bin_zero_hz, double bin_width_hz, double centre_hz,
+                                 rvx_hd_reading* out) {
+    if (!out || (bins && (!power || !floor)))
+        return rvx::fail(RVX_EINVAL, "rvx_measure_hd_sidebands: null argument");
+    *out = measure_hd(power, floor, bins, bin_zero_hz, bin_width_hz, centre_hz);
     return RVX_OK;
 }
 

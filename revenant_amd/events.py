@@ -40,6 +40,20 @@ def candidates(cfg: _lib.EventConfig, average, floor, bands) -> list[_lib.Candid
     return [out[i] for i in range(n.value)]
 
 
+def measure_hd_sidebands(power, floor, bin_zero_hz: float, bin_width_hz: float,
+                         centre_hz: float) -> _lib.HdReading:
+    """The hybrid digital sideband reading around a wideband FM signal at
+    centre_hz (core/detect/hd_hybrid.h), over a decision's average and floor."""
+    p, f = _f64(power), _f64(floor)
+    if p.size != f.size:
+        raise ValueError("power and floor differ in length")
+    out = _lib.HdReading()
+    _lib.check(_lib.lib().rvx_measure_hd_sidebands(p.ctypes.data, f.ctypes.data, p.size,
+                                                   float(bin_zero_hz), float(bin_width_hz),
+                                                   float(centre_hz), C.byref(out)))
+    return out
+
+
 class Tracker:
     def __init__(self, cfg: _lib.EventConfig, bins: int):
         self.lib = _lib.lib()
