@@ -306,3 +306,18 @@ def test_named_spurs_reach_the_comb_bank_and_withdrawals_take_them_out():
     s.step([SPUR_BIN])
     take_spurs(bank, s.t.tracks(), s.t.withdrawn_spurs(), CENTER)
     assert bank.plan(2 * block, block)[1] == 0
+
+
+def test_a_cancelled_carrier_coming_back_is_not_a_keyup():
+    """detector.cpp:1921: while the carrier at a centre is being cancelled,
+    its going quiet is the cancellation's doing, so its return on the same
+    emitter counts no key-up (the same return after no spur was named counts
+    one, test_keyups_are_counted_on_the_emitter_and_two_rule_a_carrier_out)."""
+    s, tr = _probed_scene()
+    assert tr.spur
+    emitter = tr.emitter
+    s.run(6.0, [])                              # quiet (cancelled): dropped
+    assert s.track_at(SPUR_BIN) is None
+    s.run(0.5, [SPUR_BIN])
+    tr = s.track_at(SPUR_BIN)
+    assert tr.state == 1 and tr.emitter == emitter and tr.keyups == 0
